@@ -1,0 +1,272 @@
+/*
+ * swr.h -- C ABI of the MI355X fixed-function triangle draw path.
+ *
+ * One call renders one triangle-list draw into caller-owned device memory,
+ * with the semantics of SwiftShader's sw::Renderer::draw for the FILL
+ * polygon mode: post-vertex-stage clip-space vertices in, colour / depth /
+ * stencil attachments and an occlusion count out.
+ *
+ * Reference interface replaced (SwiftShader, legacy 28.4 precision build):
+ *   sw::Renderer::draw(const vk::GraphicsPipeline*, const vk::DynamicState&,
+ *       unsigned count, int baseVertex, CountedEvent*, int instanceID,
+ *       int layer, void *indexBuffer, const VkRect2D &renderArea,
+ *       const vk::Pipeline::PushConstantStorage&, bool update)
+ *                                             src/Device/Renderer.hpp:211-213
+ *   completion: sw::Renderer::synchronize()    src/Device/Renderer.hpp:218
+ *   MSAA resolve: sw::Blitter::resolve()       src/Device/Blitter.hpp:147
+ *
+ * The descriptor carries what DrawCall / DrawData carry
+ * (src/Device/Renderer.hpp:58-113): vertex outputs, viewport and scissor,
+ * the SetupProcessor / PixelProcessor state subsets, blend constants,
+ * stencil reference and masks, the sw::Sampler subset and per-level mip
+ * descriptors, and the attachment pointers, pitches and sample slices.
+ *
+ * Every pointer inside a descriptor is a DEVICE pointer (HBM), owned by the
+ * caller.  The library owns only its scratch (setup records, tile bins),
+ * grown on demand.  All enum values are the Vulkan ones, so a Vulkan front
+ * end can copy them through unchanged.
+ *
+ * Error behaviour: functions return SWR_OK (0) or a negative SWR_ERROR_*
+ * code; they never abort.  swr_last_error() returns a message for the last
+ * failure on the calling thread.  A primitive that the reference would cull
+ * (trivially rejected, clipped away, back-facing, zero coverage) is simply
+ * not drawn -- that is not an error, as in SetupRoutine.
+ *
+ * Threading: calls are thread-compatible; one stream per caller thread.
+ * Work is enqueued on the given HIP stream (NULL = default stream) and is
+ * complete when the stream is synchronised.
+ */
+#ifndef SWR_H_
+#define SWR_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SWR_ABI_VERSION 1
+
+/* ---- status codes -------------------------------------------------------- */
+#define SWR_OK 0
+#define SWR_ERROR_INVALID_ARGUMENT (-1)
+#define SWR_ERROR_UNSUPPORTED (-2)
+#define SWR_ERROR_OUT_OF_MEMORY (-3)
+#define SWR_ERROR_DEVICE (-4)
+
+/* ---- Vulkan enum values used by the descriptor -------------------------- */
+/* VkFormat */
+#define SWR_FORMAT_UNDEFINED 0
+#define SWR_FORMAT_R8G8B8A8_UNORM 37
+#define SWR_FORMAT_R32G32B32A32_SFLOAT 109
+#define SWR_FORMAT_D32_SFLOAT 126
+#define SWR_FORMAT_S8_UINT 127
+/* VkCompareOp */
+#define SWR_COMPARE_NEVER 0
+#define SWR_COMPARE_LESS 1
+#define SWR_COMPARE_EQUAL 2
+#define SWR_COMPARE_LESS_OR_EQUAL 3
+#define SWR_COMPARE_GREATER 4
+#define SWR_COMPARE_NOT_EQUAL 5
+#define SWR_COMPARE_GREATER_OR_EQUAL 6
+#define SWR_COMPARE_ALWAYS 7
+/* VkStencilOp */
+#define SWR_STENCIL_OP_KEEP 0
+#define SWR_STENCIL_OP_ZERO 1
+#define SWR_STENCIL_OP_REPLACE 2
+#define SWR_STENCIL_OP_INCREMENT_AND_CLAMP 3
+#define SWR_STENCIL_OP_DECREMENT_AND_CLAMP 4
+#define SWR_STENCIL_OP_INVERT 5
+#define SWR_STENCIL_OP_INCREMENT_AND_WRAP 6
+#define SWR_STENCIL_OP_DECREMENT_AND_WRAP 7
+/* VkBlendFactor */
+#define SWR_BLEND_FACTOR_ZERO 0
+#define SWR_BLEND_FACTOR_ONE 1
+#define SWR_BLEND_FACTOR_SRC_COLOR 2
+#define SWR_BLEND_FACTOR_ONE_MINUS_SRC_COLOR 3
+#define SWR_BLEND_FACTOR_DST_COLOR 4
+#define SWR_BLEND_FACTOR_ONE_MINUS_DST_COLOR 5
+#define SWR_BLEND_FACTOR_SRC_ALPHA 6
+#define SWR_BLEND_FACTOR_ONE_MINUS_SRC_ALPHA 7
+#define SWR_BLEND_FACTOR_DST_ALPHA 8
+#define SWR_BLEND_FACTOR_ONE_MINUS_DST_ALPHA 9
+#define SWR_BLEND_FACTOR_CONSTANT_COLOR 10
+#define SWR_BLEND_FACTOR_ONE_MINUS_CONSTANT_COLOR 11
+#define SWR_BLEND_FACTOR_CONSTANT_ALPHA 12
+#define SWR_BLEND_FACTOR_ONE_MINUS_CONSTANT_ALPHA 13
+#define SWR_BLEND_FACTOR_SRC_ALPHA_SATURATE 14
+/* VkBlendOp */
+#define SWR_BLEND_OP_ADD 0
+#define SWR_BLEND_OP_SUBTRACT 1
+#define SWR_BLEND_OP_REVERSE_SUBTRACT 2
+#define SWR_BLEND_OP_MIN 3
+#define SWR_BLEND_OP_MAX 4
+/* VkCullModeFlagBits / VkFrontFace */
+#define SWR_CULL_NONE 0
+#define SWR_CULL_FRONT 1
+#define SWR_CULL_BACK 2
+#define SWR_CULL_FRONT_AND_BACK 3
+#define SWR_FRONT_FACE_COUNTER_CLOCKWISE 0
+#define SWR_FRONT_FACE_CLOCKWISE 1
+/* VkFilter / VkSamplerMipmapMode / VkSamplerAddressMode */
+#define SWR_FILTER_NEAREST 0
+#define SWR_FILTER_LINEAR 1
+#define SWR_MIPMAP_MODE_NEAREST 0
+#define SWR_MIPMAP_MODE_LINEAR 1
+#define SWR_ADDRESS_REPEAT 0
+#define SWR_ADDRESS_CLAMP_TO_EDGE 2
+
+/* ---- fixed fragment-shader family ---------------------------------------- */
+/* Inputs: location 0 = vec4 colour, location 1 = vec2 texcoord, both
+ * perspective-correct.  Output: location 0 vec4. */
+#define SWR_SHADER_INTERPOLATE 0 /* out = colour                           */
+#define SWR_SHADER_SAMPLE 1      /* out = texture(s, uv)                    */
+#define SWR_SHADER_MODULATE 2    /* out = texture(s, uv) * colour           */
+
+#define SWR_MAX_MIP_LEVELS 15 /* sw::MIPMAP_LEVELS, src/Device/Config.hpp:21 */
+
+typedef struct swr_viewport { /* VkViewport */
+  float x, y, width, height, min_depth, max_depth;
+} swr_viewport;
+
+typedef struct swr_rect { /* VkRect2D */
+  int32_t x, y;
+  uint32_t width, height;
+} swr_rect;
+
+typedef struct swr_stencil_op_state { /* VkStencilOpState */
+  uint32_t fail_op, pass_op, depth_fail_op, compare_op;
+  uint32_t compare_mask, write_mask, reference;
+} swr_stencil_op_state;
+
+typedef struct swr_blend_state { /* VkPipelineColorBlendAttachmentState */
+  uint32_t blend_enable;
+  uint32_t src_color_blend_factor, dst_color_blend_factor, color_blend_op;
+  uint32_t src_alpha_blend_factor, dst_alpha_blend_factor, alpha_blend_op;
+  uint32_t color_write_mask; /* bit0 R .. bit3 A */
+} swr_blend_state;
+
+typedef struct swr_sampler_state { /* VkSamplerCreateInfo subset */
+  uint32_t mag_filter, min_filter, mipmap_mode;
+  uint32_t address_mode_u, address_mode_v;
+  float mip_lod_bias;
+  uint32_t anisotropy_enable;
+  float max_anisotropy;
+  float min_lod, max_lod; /* clamped to [0, 13] as vk::SamplerState does */
+} swr_sampler_state;
+
+typedef struct swr_image_level { /* one mip level of a 2D image view */
+  const void *data;         /* device pointer to texel (0,0)            */
+  uint32_t width, height;   /* texels                                   */
+  uint32_t row_pitch_bytes; /* multiple of the texel size               */
+  uint32_t reserved;
+} swr_image_level;
+
+typedef struct swr_texture { /* combined image sampler, 2D view */
+  uint32_t format;          /* R8G8B8A8_UNORM or R32G32B32A32_SFLOAT    */
+  uint32_t level_count;     /* levels in the view, 1..15                */
+  swr_image_level levels[SWR_MAX_MIP_LEVELS];
+} swr_texture;
+
+typedef struct swr_attachment {
+  void *data;                  /* device pointer to sample 0, pixel (0,0) */
+  uint32_t format;             /* colour: RGBA8 / RGBA32F; depth: D32F; stencil: S8 */
+  uint32_t row_pitch_bytes;
+  uint64_t sample_pitch_bytes; /* distance between sample planes (4x MSAA) */
+} swr_attachment;
+
+typedef struct swr_draw_desc {
+  /* Vertex-stage outputs, one entry per vertex (structure of arrays). */
+  const float *positions;   /* float4 clip-space x, y, z, w (required) */
+  const float *colors;      /* float4 (shader input location 0) or NULL */
+  const float *texcoords;   /* float2 (shader input location 1) or NULL */
+  uint32_t vertex_count;
+  uint32_t triangle_count;  /* TRIANGLE_LIST primitives */
+  const uint32_t *indices;  /* 3 per triangle, or NULL for 0,1,2,3,...  */
+
+  /* Pre-rasterisation state. */
+  swr_viewport viewport;
+  swr_rect scissor;
+  swr_rect render_area;
+  uint32_t cull_mode;
+  uint32_t front_face;
+
+  /* Multisample state. */
+  uint32_t samples;      /* 1 or 4 */
+  uint32_t sample_mask;  /* VkPipelineMultisampleStateCreateInfo.pSampleMask[0] */
+
+  /* Depth / stencil state. */
+  uint32_t depth_test_enable, depth_write_enable, depth_compare_op;
+  uint32_t stencil_test_enable;
+  swr_stencil_op_state front, back;
+
+  /* Colour blend state (one attachment). */
+  swr_blend_state blend;
+  float blend_constants[4];
+
+  /* Fragment shader (fixed family) and its combined image sampler. */
+  uint32_t shader;
+  swr_sampler_state sampler;
+  swr_texture texture;
+
+  /* Attachments; a NULL data pointer means "no such attachment". */
+  uint32_t framebuffer_width, framebuffer_height;
+  swr_attachment color, depth, stencil;
+
+  /* Optional device pointer: += samples passing depth/stencil
+   * (the VK_QUERY_TYPE_OCCLUSION result of the reference). */
+  uint64_t *occlusion_count;
+} swr_draw_desc;
+
+typedef struct swr_resolve_desc { /* 4x -> 1x colour resolve */
+  uint32_t width, height;
+  uint32_t format;               /* RGBA8 or RGBA32F */
+  const void *src;               /* multisampled source, sample 0 plane */
+  uint32_t src_row_pitch_bytes;
+  uint64_t src_sample_pitch_bytes;
+  uint32_t src_samples;          /* 4 */
+  void *dst;
+  uint32_t dst_row_pitch_bytes;
+} swr_resolve_desc;
+
+/* Tuning knobs of the tiled rasteriser (0 = library default). */
+typedef struct swr_tuning {
+  uint32_t tile_width, tile_height; /* pixels per screen tile */
+  uint32_t reserved[6];
+} swr_tuning;
+
+/* Returns SWR_ABI_VERSION. */
+int swr_abi_version(void);
+
+/* Message for the last failing call on this thread ("" if none). */
+const char *swr_last_error(void);
+
+/* Selects the HIP device used by later calls on this thread and prepares
+ * per-device state.  Optional: the first call on a device does it. */
+int swr_init(int device);
+
+/* Releases library scratch on the current device. */
+int swr_release(void);
+
+/* Renders one draw (sw::Renderer::draw).  Reads and writes only the memory
+ * the descriptor names plus library scratch.  Asynchronous on `stream`. */
+int swr_draw(const swr_draw_desc *desc, void *stream);
+
+/* Renders `count` independent draws in one submission, in array order
+ * (draws may target different attachments).  Equivalent to `count` calls
+ * of swr_draw on the same stream. */
+int swr_draw_batch(const swr_draw_desc *descs, uint32_t count, void *stream);
+
+/* MSAA colour resolve (sw::Blitter::resolve, src/Device/Blitter.cpp:2035). */
+int swr_resolve(const swr_resolve_desc *desc, void *stream);
+
+/* Sets / reads the rasteriser tuning used by later draws on this device. */
+int swr_set_tuning(const swr_tuning *tuning);
+int swr_get_tuning(swr_tuning *tuning);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SWR_H_ */

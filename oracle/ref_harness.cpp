@@ -1,0 +1,65 @@
+// ref_harness.cpp -- C entry points into reference sources compiled as-is.
+//
+// TEST INFRASTRUCTURE ONLY (see oracle/Makefile, target `ref`).  This file
+// is this project's own code; it is linked with three unmodified reference
+// translation units that compile standalone with g++ and the reference's
+// own include directories:
+//   src/Device/Clipper.cpp    (Sutherland-Hodgman clipping)
+//   src/Pipeline/Constants.cpp (sample offsets and anisotropy tables)
+//   src/System/Half.cpp       (needed by Constants.cpp)
+// The result, oracle/_ref/libswr_ref.so, lets the tests check the oracle's
+// clipper bit-exactly and its constant tables value by value.  The rest of
+// the draw path is JIT-generated through Reactor/Subzero and needs the
+// reference's CMake build, so it is not built here.
+#include <cstdint>
+#include <cstring>
+#include <new>
+#include <xmmintrin.h>
+
+#include "Device/Clipper.hpp"
+#include "Device/Polygon.hpp"
+#include "Device/Renderer.hpp"
+#include "Pipeline/Constants.hpp"
+
+extern "C" {
+
+// Clips triangle (v0, v1, v2) with sw::Clipper::Clip; writes the resulting
+// polygon (n float4s) to `out`.  depthClipNegativeOneToOne is false, which
+// is the only DrawCall field Clip() reads; the rest of the object is unused.
+__attribute__((visibility("default"))) int ref_clip_triangle(const float *v0, const float *v1, const float *v2,
+                                                              int flagsOr, float *out, int *outN) {
+  unsigned csr = _mm_getcsr();
+  _mm_setcsr(csr | 0x8040);  // FTZ | DAZ as on the reference's worker threads
+  alignas(64) static unsigned char drawStorage[sizeof(sw::DrawCall)];
+  memset(drawStorage, 0, sizeof(drawStorage));
+  const sw::DrawCall &draw = *reinterpret_cast<const sw::DrawCall *>(drawStorage);
+  sw::float4 a, b, c;
+  memcpy(&a, v0, 16);
+  memcpy(&b, v1, 16);
+  memcpy(&c, v2, 16);
+  sw::Polygon polygon(&a, &b, &c);
+  bool visible = sw::Clipper::Clip(polygon, flagsOr, draw);
+  *outN = polygon.n;
+  for (int i = 0; i < polygon.n && i < 16; i++) memcpy(out + 4 * i, polygon.P[polygon.i][i], 16);
+  _mm_setcsr(csr);
+  return visible ? 1 : 0;
+}
+
+// Copies the on-path tables of sw::Constants.
+__attribute__((visibility("default"))) void ref_constants(int *xf, int *yf, int *msOffsets, uint16_t *cweight,
+                                                          float *uvweight, float *uvstart) {
+  static sw::Constants *constants = new sw::Constants();
+  for (int q = 0; q < 4; q++) {
+    xf[q] = constants->Xf[q];
+    yf[q] = constants->Yf[q];
+  }
+  msOffsets[0] = sw::yMinMultiSampleOffset;
+  msOffsets[1] = sw::yMaxMultiSampleOffset;
+  for (int n = 0; n < 17; n++) {
+    cweight[n] = constants->cWeight[n][0];
+    uvweight[n] = constants->uvWeight[n][0];
+    uvstart[n] = constants->uvStart[n][0];
+  }
+}
+
+}  // extern "C"

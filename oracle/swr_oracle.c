@@ -1,0 +1,1307 @@
+/*
+ * swr_oracle.c -- CPU oracle for the fixed-function triangle draw path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker the parity tests
+ * compare the HIP kernels against; it is never part of the product path
+ * and nothing in the swiftshader_amd package may call it.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+ *
+ * It restates, scalar and in the reference's own operation order, what
+ * SwiftShader's legacy-precision (28.4 sub-pixel) Subzero build computes:
+ *
+ *   vertex tail      src/Pipeline/VertexRoutine.cpp:116-154, 601-611
+ *   viewport/scissor src/Device/Renderer.cpp:300-345
+ *   triangle loop    src/Device/Renderer.cpp:733-776 (setupSolidTriangles)
+ *   clipping         src/Device/Clipper.cpp:22-299
+ *   setup            src/Pipeline/SetupRoutine.cpp:36-621
+ *   rasteriser       src/Device/QuadRasterizer.cpp:36-250
+ *   pixel pipeline   src/Pipeline/PixelRoutine.cpp:97-358, 360-902,
+ *                    1203-1393, 1617-1961, 1963-2936;
+ *                    src/Pipeline/PixelProgram.cpp:138-364
+ *   sampler          src/Pipeline/SamplerCore.cpp:59-253, 278-909,
+ *                    1061-1248, 1333-1422, 1593-1800, 2357-2638
+ *   sampler state    src/Pipeline/SpirvShaderSampling.cpp:33-135,275-431,
+ *                    src/Vulkan/VkSampler.hpp:33-36,
+ *                    src/Vulkan/VkDescriptorSetLayout.cpp:298-334,468-503
+ *   blend state      src/Device/Context.cpp:1090-1311
+ *   resolve          src/Device/Blitter.cpp:1524-1545, 2076-2200
+ *   numerics         src/Reactor/SubzeroReactor.cpp (RoundInt, Int(Float),
+ *                    Min/Max, Frac, Floor, MulHigh, Short4(Int4)),
+ *                    src/Pipeline/ShaderCore.cpp:691-722
+ *
+ * The worker threads of the reference run with FTZ and DAZ set
+ * (src/System/SwiftConfig.cpp:136-139); the entry points here set the same
+ * MXCSR bits for the duration of the call.  Build with -ffp-contract=off so
+ * that no multiply-add is fused (the reference's MulAdd is unfused,
+ * src/Reactor/SubzeroReactor.cpp:3723-3727).
+ *
+ * Ordering: the reference processes primitives of a draw in API order per
+ * pixel (cluster tickets, src/Device/Renderer.cpp:551-662); this oracle runs
+ * every primitive to completion before the next, which yields the same
+ * per-pixel order.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <xmmintrin.h>
+
+#include "../include/swr.h"
+
+#define EXPORT __attribute__((visibility("default")))
+
+/* ===================================================================== */
+/* Reactor / x86 numeric primitives                                       */
+/* ===================================================================== */
+
+static inline uint32_t f2u(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static inline float u2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+
+/* Min/Max: ordered compare + select, NaN -> second operand
+ * (SubzeroReactor.cpp:3745-3771; Reactor.cpp:3980-3988 for scalars). */
+static inline float rr_min(float x, float y) { return x < y ? x : y; }
+static inline float rr_max(float x, float y) { return x > y ? x : y; }
+static inline int imin(int a, int b) { return a < b ? a : b; }
+static inline int imax(int a, int b) { return a > b ? a : b; }
+
+/* cvtps2dq: round to nearest even; out of range / NaN -> 0x80000000. */
+static inline int32_t rr_round_int(float x) {
+  if (!(x >= -2147483648.0f && x < 2147483648.0f)) return INT32_MIN;
+  return (int32_t)nearbyintf(x);
+}
+/* RoundIntClamped: Min(x, 2147483520.f) then cvtps2dq (SubzeroReactor.cpp:3417-3440). */
+static inline int32_t rr_round_int_clamped(float x) {
+  return rr_round_int(rr_min(x, 2147483520.0f));
+}
+/* Int(Float) / Int4(Float4): cvttps2dq truncation, out of range -> 0x80000000. */
+static inline int32_t rr_trunc_int(float x) {
+  if (!(x >= -2147483648.0f && x < 2147483648.0f)) return INT32_MIN;
+  return (int32_t)x;
+}
+/* Floor via roundps(1); Frac = x - floor(x), clamped below 1.0 (SubzeroReactor.cpp:3974-4014). */
+static inline float rr_floor(float x) { return floorf(x); }
+static inline float rr_frac(float x) { return rr_min(x - floorf(x), u2f(0x3F7FFFFFu)); }
+/* MulHigh on 16-bit lanes (pmulhuw / pmulhw). */
+static inline uint16_t mulhi_u16(uint16_t a, uint16_t b) { return (uint16_t)(((uint32_t)a * (uint32_t)b) >> 16); }
+static inline int16_t mulhi_s16(int16_t a, int16_t b) { return (int16_t)(((int32_t)a * (int32_t)b) >> 16); }
+/* Saturating unsigned 16-bit add/sub (paddusw / psubusw). */
+static inline uint16_t addsat_u16(uint16_t a, uint16_t b) { uint32_t s = (uint32_t)a + b; return s > 0xFFFF ? 0xFFFF : (uint16_t)s; }
+static inline uint16_t subsat_u16(uint16_t a, uint16_t b) { return a > b ? (uint16_t)(a - b) : 0; }
+/* Wrapping 32-bit signed arithmetic (Reactor Int). */
+static inline int32_t wadd(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
+static inline int32_t wsub(int32_t a, int32_t b) { return (int32_t)((uint32_t)a - (uint32_t)b); }
+static inline int32_t wmul(int32_t a, int32_t b) { return (int32_t)((uint32_t)a * (uint32_t)b); }
+
+/* log2(sqrt(x)) by exponent-bit arithmetic (SamplerCore.cpp:1333-1341). */
+static inline float log2sqrt(float lod) {
+  lod = lod * lod;
+  lod = (float)(int32_t)f2u(lod) - (float)0x3F800000;
+  lod = lod * u2f(0x33000000u);
+  return lod;
+}
+
+/* Exported for the known-answer tests of the primitives. */
+EXPORT int32_t swo_round_int(float x) { return rr_round_int(x); }
+EXPORT int32_t swo_round_int_clamped(float x) { return rr_round_int_clamped(x); }
+EXPORT int32_t swo_trunc_int(float x) { return rr_trunc_int(x); }
+EXPORT uint32_t swo_mulhigh_u16x2(uint32_t a, uint32_t b) {
+  return (uint32_t)mulhi_u16((uint16_t)a, (uint16_t)b) | ((uint32_t)mulhi_u16((uint16_t)(a >> 16), (uint16_t)(b >> 16)) << 16);
+}
+EXPORT uint32_t swo_mulhigh_s16x2(uint32_t a, uint32_t b) {
+  return (uint32_t)(uint16_t)mulhi_s16((int16_t)a, (int16_t)b) | ((uint32_t)(uint16_t)mulhi_s16((int16_t)(a >> 16), (int16_t)(b >> 16)) << 16);
+}
+EXPORT float swo_log2sqrt(float x) { return log2sqrt(x); }
+EXPORT float swo_frac(float x) { return rr_frac(x); }
+
+/* ===================================================================== */
+/* Constants (src/Pipeline/Constants.hpp:26-52, Constants.cpp:87-145,291-297) */
+/* ===================================================================== */
+
+static const float kSampleLocationsX[4] = {0.375f - 0.5f, 0.875f - 0.5f, 0.125f - 0.5f, 0.625f - 0.5f};
+static const float kSampleLocationsY[4] = {0.125f - 0.5f, 0.375f - 0.5f, 0.625f - 0.5f, 0.875f - 0.5f};
+static int kXf[4], kYf[4];           /* toFixedPoint(location, 4) */
+static int kYMinMSOffset, kYMaxMSOffset;
+static uint16_t kCWeight[17];
+static float kUVWeight[17], kUVStart[17];
+static int kConstantsReady = 0;
+
+static int to_fixed_point(float v, int p) { return (int)(v * (float)(1 << p)); }
+
+static void init_constants(void) {
+  if (kConstantsReady) return;
+  for (int q = 0; q < 4; q++) {
+    kXf[q] = to_fixed_point(kSampleLocationsX[q], 4);
+    kYf[q] = to_fixed_point(kSampleLocationsY[q], 4);
+  }
+  float maxY = kSampleLocationsY[0];
+  for (int q = 1; q < 4; q++) maxY = maxY > kSampleLocationsY[q] ? maxY : kSampleLocationsY[q];
+  kYMinMSOffset = to_fixed_point(1.0f, 4) - to_fixed_point(maxY, 4) - 1;
+  kYMaxMSOffset = to_fixed_point(1.0f, 4) + to_fixed_point(maxY, 4) - 1;
+  /* cWeight[N] = 0xFFFF / N rounded as the table lists (N=0,1 -> 0xFFFF, 2 -> 0x8000). */
+  kCWeight[0] = 0xFFFF;
+  kCWeight[1] = 0xFFFF;
+  kCWeight[2] = 0x8000;
+  for (int n = 3; n <= 16; n++) kCWeight[n] = (uint16_t)(0xFFFF / n);
+  kCWeight[4] = 0x4000;
+  kCWeight[8] = 0x2000;
+  kCWeight[16] = 0x1000;
+  kUVWeight[0] = 1.0f;
+  kUVStart[0] = -0.0f / 2.0f;
+  for (int n = 1; n <= 16; n++) {
+    kUVWeight[n] = 1.0f / (float)n;
+    kUVStart[n] = -(float)(n - 1) / (float)(2 * n);
+  }
+  kConstantsReady = 1;
+}
+
+EXPORT void swo_constants(int *xf, int *yf, int *ms_offsets, uint16_t *cweight, float *uvweight, float *uvstart) {
+  init_constants();
+  for (int q = 0; q < 4; q++) { xf[q] = kXf[q]; yf[q] = kYf[q]; }
+  ms_offsets[0] = kYMinMSOffset;
+  ms_offsets[1] = kYMaxMSOffset;
+  for (int n = 0; n <= 16; n++) { cweight[n] = kCWeight[n]; uvweight[n] = kUVWeight[n]; uvstart[n] = kUVStart[n]; }
+}
+
+/* ===================================================================== */
+/* Draw state                                                             */
+/* ===================================================================== */
+
+enum { CLIP_RIGHT = 1, CLIP_TOP = 2, CLIP_FAR = 4, CLIP_LEFT = 8, CLIP_BOTTOM = 16, CLIP_NEAR = 32,
+       CLIP_FRUSTUM = 63, CLIP_FINITE = 128 };
+enum { FILTER_POINT, FILTER_MIN_POINT_MAG_LINEAR, FILTER_MIN_LINEAR_MAG_POINT, FILTER_LINEAR, FILTER_ANISOTROPIC };
+enum { MIPMAP_NONE, MIPMAP_POINT, MIPMAP_LINEAR };
+enum { ADDR_WRAP, ADDR_CLAMP };
+enum { BLEND_OP_SRC_EXT = 1000148000, BLEND_OP_DST_EXT = 1000148001, BLEND_OP_ZERO_EXT = 1000148002 };
+
+#define NATTR 6 /* colour rgba + uv */
+
+typedef struct {
+  float pos[4];
+  int clipFlags;
+  int X, Y;
+  float pz, pw;
+  float v[NATTR];
+} OVertex;
+
+typedef struct { float A, B, C; } Plane;
+
+typedef struct {
+  int yMin, yMax;
+  float x0, y0;
+  Plane z, w, V[NATTR];
+  int frontFacing;
+} OPrim;
+
+typedef struct {
+  const uint8_t *data[SWR_MAX_MIP_LEVELS];
+  int width[SWR_MAX_MIP_LEVELS], height[SWR_MAX_MIP_LEVELS];
+  int pitchP[SWR_MAX_MIP_LEVELS];
+  uint16_t uHalf[SWR_MAX_MIP_LEVELS], vHalf[SWR_MAX_MIP_LEVELS];
+  float wwhh[4];
+  int isFloat;
+  /* sw::Sampler subset */
+  int filter, mipmap, addrU, addrV;
+  float mipLodBias, maxAnisotropy, minLod, maxLod;
+} OTexture;
+
+typedef struct {
+  const swr_draw_desc *d;
+  /* DrawData */
+  float WxF, HxF, X0xF, Y0xF, depthRange, depthNear;
+  int scissorX0, scissorX1, scissorY0, scissorY1;
+  /* PixelProcessor::State subset */
+  int ms;             /* multiSampleCount */
+  int enableMS;
+  unsigned msMask;
+  int depthTestActive, depthWriteEnable, depthCompare;
+  int stencilActive;
+  int colorWriteActive; /* 4-bit mask, 0 if no colour attachment */
+  int colorIsFloat;
+  int blendEnable;
+  int srcF, dstF, colorOp, srcFA, dstFA, alphaOp;
+  float blendConstU[4], invBlendConstU[4], blendConstF[4], invBlendConstF[4];
+  /* interpolants */
+  int useColor, useUV;
+  OTexture tex;
+  /* outline storage: rows [-4, H+4) */
+  int outlineRows;
+  uint16_t *left[4], *right[4];
+  uint64_t occlusion;
+} Ctx;
+
+#define OUTLINE_OFFSET 4
+static inline uint16_t *LEFT(Ctx *c, int q, int y) { return &c->left[q][y + OUTLINE_OFFSET]; }
+static inline uint16_t *RIGHT(Ctx *c, int q, int y) { return &c->right[q][y + OUTLINE_OFFSET]; }
+
+/* ===================================================================== */
+/* Vertex tail: clip flags + 28.4 projection (VertexRoutine.cpp:116-154,601-611) */
+/* ===================================================================== */
+
+static void vertex_routine(const Ctx *c, OVertex *v) {
+  float x = v->pos[0], y = v->pos[1], z = v->pos[2], w = v->pos[3];
+  int flags = 0;
+  if (w < x) flags |= CLIP_RIGHT;
+  if (w < y) flags |= CLIP_TOP;
+  if (!(-w <= x)) flags |= CLIP_LEFT;
+  if (!(-w <= y)) flags |= CLIP_BOTTOM;
+  /* depthClipEnable defaults to !depthClampEnable = true */
+  if (w < z) flags |= CLIP_FAR;
+  if (!(0.0f <= z)) flags |= CLIP_NEAR;
+  const float maxPos = u2f(0x7F7FFFFFu);
+  if (fabsf(x) <= maxPos && fabsf(y) <= maxPos && fabsf(z) <= maxPos) flags |= CLIP_FINITE;
+  v->clipFlags = flags;
+
+  /* w == 0 -> OR in the bits of 1.0f (so -0.0 becomes -1.0) */
+  float ww = u2f(f2u(w) | ((w == 0.0f) ? 0x3F800000u : 0u));
+  float rhw = 1.0f / ww;
+  v->X = rr_round_int_clamped(c->X0xF + (x * rhw) * c->WxF);
+  v->Y = rr_round_int_clamped(c->Y0xF + (y * rhw) * c->HxF);
+  v->pz = z * rhw;
+  v->pw = rhw;
+}
+
+/* ===================================================================== */
+/* Clipper (Clipper.cpp:22-299)                                          */
+/* ===================================================================== */
+
+typedef struct {
+  float B[16][4];
+  const float *P[16][16];
+  int n, i, b;
+} Polygon;
+
+static void clip_edge(float *Vo, const float *Vi, const float *Vj, float di, float dj) {
+  float D = 1.0f / (dj - di);
+  for (int k = 0; k < 4; k++) Vo[k] = (dj * Vi[k] - di * Vj[k]) * D;
+}
+
+/* plane: 0 near (z), 1 far (w-z), 2 left (w+x), 3 right (w-x), 4 top (w-y), 5 bottom (w+y) */
+static float clip_distance(const float *v, int plane) {
+  switch (plane) {
+    case 0: return v[2];
+    case 1: return v[3] - v[2];
+    case 2: return v[3] + v[0];
+    case 3: return v[3] - v[0];
+    case 4: return v[3] - v[1];
+    default: return v[3] + v[1];
+  }
+}
+
+static void clip_plane(Polygon *p, int plane) {
+  const float **V = p->P[p->i];
+  const float **T = p->P[p->i + 1];
+  int t = 0;
+  for (int i = 0; i < p->n; i++) {
+    int j = (i == p->n - 1) ? 0 : i + 1;
+    float di = clip_distance(V[i], plane);
+    float dj = clip_distance(V[j], plane);
+    if (di >= 0) {
+      T[t++] = V[i];
+      if (dj < 0) {
+        clip_edge(p->B[p->b], V[i], V[j], di, dj);
+        T[t++] = p->B[p->b++];
+      }
+    } else {
+      if (dj > 0) {
+        clip_edge(p->B[p->b], V[j], V[i], dj, di);
+        T[t++] = p->B[p->b++];
+      }
+    }
+  }
+  p->n = t;
+  p->i += 1;
+}
+
+static int clipper_clip(Polygon *p, int flagsOr) {
+  static const int order[6] = {CLIP_NEAR, CLIP_FAR, CLIP_LEFT, CLIP_RIGHT, CLIP_TOP, CLIP_BOTTOM};
+  if (flagsOr & CLIP_FRUSTUM) {
+    for (int k = 0; k < 6; k++) {
+      if (k > 0 && p->n < 3) break;
+      if (flagsOr & order[k]) clip_plane(p, k);
+    }
+  }
+  return p->n >= 3;
+}
+
+/* Exported for the clipper cross-check against the compiled reference. */
+EXPORT int swo_clip_triangle(const float *v0, const float *v1, const float *v2, int flagsOr, float *out, int *outN) {
+  unsigned csr = _mm_getcsr();
+  _mm_setcsr(csr | 0x8040);
+  Polygon p;
+  p.P[0][0] = v0; p.P[0][1] = v1; p.P[0][2] = v2;
+  p.n = 3; p.i = 0; p.b = 0;
+  int r = clipper_clip(&p, flagsOr);
+  *outN = p.n;
+  for (int k = 0; k < p.n && k < 16; k++) memcpy(out + 4 * k, p.P[p.i][k], 16);
+  _mm_setcsr(csr);
+  return r;
+}
+
+/* ===================================================================== */
+/* Setup routine (SetupRoutine.cpp:36-621)                                */
+/* ===================================================================== */
+
+static void setup_edge(Ctx *c, int q, int Xa, int Ya, int Xb, int Yb) {
+  if (Ya == Yb) return;
+  int swap = Yb < Ya;
+  int X1 = swap ? Xb : Xa, X2 = swap ? Xa : Xb;
+  int Y1 = swap ? Yb : Ya, Y2 = swap ? Ya : Yb;
+  int y1 = wadd(Y1, 15) >> 4;
+  int y2 = wadd(Y2, 15) >> 4;
+  int yMin = imax(y1, c->scissorY0);
+  int yMax = imin(y2, c->scissorY1);
+  if (!(yMin < yMax)) return;
+  int xMin = c->scissorX0, xMax = c->scissorX1;
+  uint16_t *edge = swap ? c->right[q] : c->left[q];
+
+  int DX12 = wsub(X2, X1);
+  int DY12 = wsub(Y2, Y1);
+  int FDX12 = (int32_t)((uint32_t)DX12 << 4);
+  int FDY12 = (int32_t)((uint32_t)DY12 << 4);
+  if (FDY12 <= 0) return; /* unreachable for coordinates inside the viewport */
+  int X = wadd(wmul(DX12, wsub((int32_t)((uint32_t)y1 << 4), Y1)), wmul(X1 & 15, DY12));
+  int x = wadd(X1 >> 4, X / FDY12);
+  int d = X % FDY12;
+  int ceil_ = wsub(0, d) >> 31;
+  x = wsub(x, ceil_);
+  d = wsub(d, ceil_ & FDY12);
+  int Q = FDX12 / FDY12;
+  int R = FDX12 % FDY12;
+  int floor_ = R >> 31;
+  Q = wadd(Q, floor_);
+  R = wadd(R, floor_ & FDY12);
+  int D = FDY12;
+  int y = y1;
+  do {
+    if (y >= yMin) edge[y + OUTLINE_OFFSET] = (uint16_t)(int16_t)imin(imax(x, xMin), xMax);
+    x = wadd(x, Q);
+    d = wadd(d, R);
+    int overflow = wsub(0, d) >> 31;
+    d = wsub(d, D & overflow);
+    x = wsub(x, overflow);
+    y++;
+  } while (y < yMax);
+}
+
+/* Returns 1 if the primitive is visible. */
+static int setup_routine(Ctx *c, OPrim *prim, const OVertex *tri[3], const Polygon *poly) {
+  int X[17], Y[17];
+  X[0] = tri[0]->X; X[1] = tri[1]->X; X[2] = tri[2]->X;
+  Y[0] = tri[0]->Y; Y[1] = tri[1]->Y; Y[2] = tri[2]->Y;
+  int d = 1;
+  {
+    float x0 = (float)X[0], x1 = (float)X[1], x2 = (float)X[2];
+    float y0 = (float)Y[0], y1 = (float)Y[1], y2 = (float)Y[2];
+    float A = (y0 - y2) * x1 + (y2 - y1) * x0 + (y1 - y0) * x2;
+    int32_t w0w1w2 = (int32_t)(f2u(tri[0]->pos[3]) ^ f2u(tri[1]->pos[3]) ^ f2u(tri[2]->pos[3]));
+    if (w0w1w2 < 0) A = -A;
+    int frontFacing = (c->d->front_face == SWR_FRONT_FACE_COUNTER_CLOCKWISE) ? (A >= 0.0f) : (A <= 0.0f);
+    if ((c->d->cull_mode & SWR_CULL_FRONT) && frontFacing) return 0;
+    if ((c->d->cull_mode & SWR_CULL_BACK) && !frontFacing) return 0;
+    d = (A > 0.0f) ? d : 0;
+    prim->frontFacing = frontFacing;
+  }
+  int n = poly->n;
+  int m = poly->i;
+  if (m != 0) {
+    for (int i = 0; i < n; i++) {
+      const float *v = poly->P[m][i];
+      float w = v[3];
+      float rhw = (w != 0.0f) ? 1.0f / w : 1.0f;
+      X[i] = rr_round_int(c->X0xF + (v[0] * rhw) * c->WxF);
+      Y[i] = rr_round_int(c->Y0xF + (v[1] * rhw) * c->HxF);
+    }
+  }
+  int yMin = Y[0], yMax = Y[0];
+  for (int i = 1; i < n; i++) { yMin = imin(Y[i], yMin); yMax = imax(Y[i], yMax); }
+  if (c->enableMS) {
+    yMin = wadd(yMin, kYMinMSOffset) >> 4;
+    yMax = wadd(yMax, kYMaxMSOffset) >> 4;
+  } else {
+    yMin = wadd(yMin, 15) >> 4;
+    yMax = wadd(yMax, 15) >> 4;
+  }
+  yMin = imax(yMin, c->scissorY0);
+  yMax = imin(yMax, c->scissorY1);
+  if (yMin >= yMax) return 0;
+
+  for (int q = 0; q < c->ms; q++) {
+    int Xq[17], Yq[17];
+    for (int i = 0; i < n; i++) {
+      Xq[i] = X[i];
+      Yq[i] = Y[i];
+      if (c->enableMS) { Xq[i] = wsub(Xq[i], kXf[q]); Yq[i] = wsub(Yq[i], kYf[q]); }
+    }
+    if (c->enableMS) {
+      int x = imin(imax(wadd(X[0], 15) >> 4, c->scissorX0), c->scissorX1);
+      for (int y = yMin - 1; y < yMax + 1; y++) { *LEFT(c, q, y) = (uint16_t)x; *RIGHT(c, q, y) = (uint16_t)x; }
+    }
+    Xq[n] = Xq[0];
+    Yq[n] = Yq[0];
+    for (int i = 0; i < n; i++) setup_edge(c, q, Xq[i + 1 - d], Yq[i + 1 - d], Xq[i + d], Yq[i + d]);
+    if (!c->enableMS) {
+      while (yMin < yMax && *LEFT(c, q, yMin) == *RIGHT(c, q, yMin)) yMin++;
+      while (yMax > yMin && *LEFT(c, q, yMax - 1) == *RIGHT(c, q, yMax - 1)) yMax--;
+      if (yMin == yMax) return 0;
+      *LEFT(c, q, yMin - 1) = *LEFT(c, q, yMin);
+      *RIGHT(c, q, yMin - 1) = *LEFT(c, q, yMin);
+      *LEFT(c, q, yMax) = *LEFT(c, q, yMax - 1);
+      *RIGHT(c, q, yMax) = *LEFT(c, q, yMax - 1);
+    }
+  }
+  prim->yMin = yMin;
+  prim->yMax = yMax;
+
+  /* Sort by minimum y (clip-space y), then by maximum w. */
+  const OVertex *v0 = tri[0], *v1 = tri[1], *v2 = tri[2];
+  {
+    float y0 = v0->pos[1], y1 = v1->pos[1], y2 = v2->pos[1];
+    float ym = rr_min(rr_min(y0, y1), y2);
+    const OVertex *vX;
+    if (ym == y1) { vX = v0; v0 = v1; v1 = v2; v2 = vX; }
+    if (ym == y2) { vX = v2; v2 = v1; v1 = v0; v0 = vX; }
+  }
+  {
+    float w0 = v0->pos[3], w1 = v1->pos[3], w2 = v2->pos[3];
+    float wm = rr_max(rr_max(w0, w1), w2);
+    const OVertex *vX;
+    if (wm == w1) { vX = v0; v0 = v1; v1 = v2; v2 = vX; }
+    if (wm == w2) { vX = v2; v2 = v1; v1 = v0; v0 = vX; }
+  }
+  float w1 = v1->pos[3], w2 = v2->pos[3];
+  float rhw0 = v0->pw;
+  int X0 = v0->X, X1 = v1->X, X2 = v2->X;
+  int Y0 = v0->Y, Y1 = v1->Y, Y2 = v2->Y;
+  float fx0 = (float)X0 * (1.0f / 16.0f);
+  float fy0 = (float)Y0 * (1.0f / 16.0f);
+  prim->x0 = fx0;
+  prim->y0 = fy0;
+  X1 = wsub(X1, X0); Y1 = wsub(Y1, Y0);
+  X2 = wsub(X2, X0); Y2 = wsub(Y2, Y0);
+  float x1 = w1 * (1.0f / 16.0f) * (float)X1;
+  float y1 = w1 * (1.0f / 16.0f) * (float)Y1;
+  float x2 = w2 * (1.0f / 16.0f) * (float)X2;
+  float y2 = w2 * (1.0f / 16.0f) * (float)Y2;
+  float a = x1 * y2 - x2 * y1;
+  float M[3][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
+  M[0][2] = rhw0;
+  if (a != 0.0f) {
+    float A = 1.0f / a;
+    float D = A * rhw0;
+    M[0][0] = (y1 * w2 - y2 * w1) * D;
+    M[0][1] = (x2 * w1 - x1 * w2) * D;
+    M[1][0] = y2 * A;
+    M[1][1] = -x2 * A;
+    M[2][0] = -y1 * A;
+    M[2][1] = x1 * A;
+  }
+  /* interpolateW: a fragment shader is always present */
+  prim->w.A = (M[0][0] + M[1][0]) + M[2][0];
+  prim->w.B = (M[0][1] + M[1][1]) + M[2][1];
+  prim->w.C = (M[0][2] + M[1][2]) + M[2][2];
+  if (c->depthTestActive) {
+    float z0 = v0->pz, z1 = v1->pz, z2 = v2->pz;
+    z1 -= z0;
+    z2 -= z0;
+    float fx1 = (float)X1 * (1.0f / 16.0f);
+    float fy1 = (float)Y1 * (1.0f / 16.0f);
+    float fx2 = (float)X2 * (1.0f / 16.0f);
+    float fy2 = (float)Y2 * (1.0f / 16.0f);
+    float D = c->depthRange / (fx1 * fy2 - fx2 * fy1);
+    prim->z.A = (fy2 * z1 - fy1 * z2) * D;
+    prim->z.B = (fx1 * z2 - fx2 * z1) * D;
+    prim->z.C = z0 * c->depthRange + c->depthNear;
+  }
+  /* Perspective-correct gradients (setupGradient, SetupRoutine.cpp:514-548). */
+  for (int k = 0; k < NATTR; k++) {
+    float i0 = v0->v[k], i1 = v1->v[k], i2 = v2->v[k];
+    prim->V[k].A = (i0 * M[0][0] + i1 * M[1][0]) + i2 * M[2][0];
+    prim->V[k].B = (i0 * M[0][1] + i1 * M[1][1]) + i2 * M[2][1];
+    prim->V[k].C = (i0 * M[0][2] + i1 * M[1][2]) + i2 * M[2][2];
+  }
+  return 1;
+}
+
+/* ===================================================================== */
+/* Sampler (SamplerCore.cpp)                                              */
+/* ===================================================================== */
+
+/* 16-bit texel fetch, RGBA8 -> (c << 8) per channel (sampleTexel, :1678-1800). */
+static void texel16(const OTexture *t, int level, uint16_t u, uint16_t v, uint16_t out[4]) {
+  uint16_t uu = mulhi_u16(u, (uint16_t)t->width[level]);
+  uint16_t vv = mulhi_u16(v, (uint16_t)t->height[level]);
+  /* pmaddwd with (1, pitchP): signed 16-bit products summed in 32 bits */
+  int32_t index = (int32_t)(int16_t)uu * 1 + (int32_t)(int16_t)vv * (int32_t)(int16_t)t->pitchP[level];
+  const uint8_t *p = t->data[level] + (size_t)(uint32_t)index * 4;
+  for (int k = 0; k < 4; k++) out[k] = (uint16_t)(p[k] << 8);
+}
+
+static uint16_t address16(float uw, int mode) {
+  if (mode == ADDR_CLAMP) {
+    float cl = rr_min(rr_max(uw, 0.0f), 65535.0f / 65536.0f);
+    return (uint16_t)rr_trunc_int(cl * 65536.0f);
+  }
+  return (uint16_t)rr_trunc_int(uw * 65536.0f);
+}
+
+static uint16_t offset_sample16(uint16_t uvw, uint16_t half, int filter, float lod, int wrap, int count) {
+  uint16_t off = half;
+  if (filter == FILTER_MIN_LINEAR_MAG_POINT) off &= !(lod <= 0.0f) ? 0xFFFF : 0;         /* CmpNLE(lod,0) */
+  else if (filter == FILTER_MIN_POINT_MAG_LINEAR) off &= (lod <= 0.0f) ? 0xFFFF : 0;      /* CmpLE(lod,0) */
+  if (wrap) return (uint16_t)(count < 0 ? uvw - off : uvw + off);
+  return count < 0 ? subsat_u16(uvw, off) : addsat_u16(uvw, off);
+}
+
+static int select_mip(const OTexture *t, float lod, int second) {
+  int ilod;
+  if (t->mipmap == MIPMAP_NONE) return 0;
+  if (t->mipmap == MIPMAP_POINT) ilod = rr_round_int(lod);
+  else ilod = rr_trunc_int(lod);
+  return ilod + second;
+}
+
+/* sampleQuad2D, 16-bit path, one lane (:668-909 with bilinearInterpolate :516-666). */
+static void sample_quad16(const OTexture *t, float u, float v, float lod, int second, uint16_t out[4]) {
+  int lvl = select_mip(t, lod, second);
+  uint16_t uu = address16(u, t->addrU);
+  uint16_t vv = address16(v, t->addrV);
+  if (t->filter == FILTER_POINT) { texel16(t, lvl, uu, vv, out); return; }
+  uint16_t u0 = offset_sample16(uu, t->uHalf[lvl], t->filter, lod, t->addrU == ADDR_WRAP, -1);
+  uint16_t v0 = offset_sample16(vv, t->vHalf[lvl], t->filter, lod, t->addrV == ADDR_WRAP, -1);
+  uint16_t u1 = offset_sample16(uu, t->uHalf[lvl], t->filter, lod, t->addrU == ADDR_WRAP, +1);
+  uint16_t v1 = offset_sample16(vv, t->vHalf[lvl], t->filter, lod, t->addrV == ADDR_WRAP, +1);
+  uint16_t c00[4], c10[4], c01[4], c11[4];
+  texel16(t, lvl, u0, v0, c00);
+  texel16(t, lvl, u1, v0, c10);
+  texel16(t, lvl, u0, v1, c01);
+  texel16(t, lvl, u1, v1, c11);
+  uint16_t f0u = (uint16_t)((uint32_t)u0 * (uint32_t)(uint16_t)t->width[lvl]);
+  uint16_t f0v = (uint16_t)((uint32_t)v0 * (uint32_t)(uint16_t)t->height[lvl]);
+  uint16_t f1u = (uint16_t)~f0u, f1v = (uint16_t)~f0v;
+  uint16_t f0u0v = mulhi_u16(f0u, f0v), f1u0v = mulhi_u16(f1u, f0v);
+  uint16_t f0u1v = mulhi_u16(f0u, f1v), f1u1v = mulhi_u16(f1u, f1v);
+  for (int k = 0; k < 4; k++) {
+    uint16_t a = mulhi_u16(c00[k], f1u1v), b = mulhi_u16(c10[k], f0u1v);
+    uint16_t cc = mulhi_u16(c01[k], f1u0v), dd = mulhi_u16(c11[k], f0u0v);
+    out[k] = (uint16_t)((uint16_t)(a + b) + (uint16_t)(cc + dd));
+  }
+}
+
+/* sampleAniso, 16-bit (:384-466). */
+static void sample_aniso16(const OTexture *t, float u, float v, float lod, float aniso, float uDelta, float vDelta,
+                           int second, uint16_t out[4]) {
+  if (t->filter != FILTER_ANISOTROPIC) { sample_quad16(t, u, v, lod, second, out); return; }
+  int N = rr_round_int(aniso);
+  float A = kUVWeight[N], B = kUVStart[N];
+  uint16_t cw = kCWeight[N];
+  uint16_t sum[4] = {0, 0, 0, 0};
+  float du = uDelta, dv = vDelta;
+  float u0 = u + B * du;
+  float v0 = v + B * dv;
+  du = du * A;
+  dv = dv * A;
+  int i = 0;
+  do {
+    uint16_t cq[4];
+    sample_quad16(t, u0, v0, lod, second, cq);
+    u0 = u0 + du;
+    v0 = v0 + dv;
+    for (int k = 0; k < 4; k++) sum[k] = (uint16_t)(sum[k] + mulhi_u16(cq[k], cw));
+    i++;
+  } while (i < N);
+  memcpy(out, sum, sizeof(sum));
+}
+
+/* 32-bit float path: address (:2463-2638), sampleFloat2D (:1155-1248). */
+static void address_float(const OTexture *t, float uvw, int dim, int mode, float lod, int *x0, int *x1, float *f) {
+  int filterMask;
+  if (t->filter == FILTER_POINT) filterMask = 0;
+  else if (t->filter == FILTER_MIN_LINEAR_MAG_POINT) filterMask = !(lod <= 0.0f) ? -1 : 0;
+  else if (t->filter == FILTER_MIN_POINT_MAG_LINEAR) filterMask = (lod <= 0.0f) ? -1 : 0;
+  else filterMask = -1;
+  float coord;
+  if (mode == ADDR_CLAMP) coord = rr_min(rr_max(uvw, 0.0f), u2f(0x3F7FFFFFu));
+  else coord = rr_frac(uvw);
+  coord = coord * (float)dim;
+  if (t->filter == FILTER_POINT) {
+    *x0 = rr_trunc_int(coord);
+    *f = 0.0f;
+  } else {
+    if (t->filter == FILTER_MIN_POINT_MAG_LINEAR || t->filter == FILTER_MIN_LINEAR_MAG_POINT)
+      coord = coord - u2f(f2u(0.5f) & (uint32_t)filterMask);
+    else
+      coord = coord - 0.5f;
+    float fl = rr_floor(coord);
+    *x0 = rr_trunc_int(fl);
+    *f = coord - fl;
+  }
+  *x1 = *x0 - filterMask;
+  if (t->filter != FILTER_POINT) {
+    if (mode == ADDR_CLAMP) {
+      *x0 = imax(*x0, 0);
+      *x1 = imin(*x1, dim - 1);
+    } else {
+      if (*x0 < 0) *x0 = dim - 1;
+      if (!(*x1 < dim)) *x1 = 0;
+    }
+  }
+}
+
+static void texelf(const OTexture *t, int lvl, int x, int yoff, float out[4]) {
+  const float *p = (const float *)t->data[lvl] + (size_t)(uint32_t)(x + yoff) * 4;
+  memcpy(out, p, 16);
+}
+
+static void sample_float2d(const OTexture *t, float u, float v, float lod, int second, float out[4]) {
+  int lvl = select_mip(t, lod, second);
+  int x0, x1, y0, y1;
+  float fu, fv;
+  address_float(t, u, t->width[lvl], t->addrU, lod, &x0, &x1, &fu);
+  address_float(t, v, t->height[lvl], t->addrV, lod, &y0, &y1, &fv);
+  int pitch = t->pitchP[lvl];
+  y0 = wmul(y0, pitch);
+  if (t->filter == FILTER_POINT) { texelf(t, lvl, x0, y0, out); return; }
+  y1 = wmul(y1, pitch);
+  float c00[4], c10[4], c01[4], c11[4];
+  texelf(t, lvl, x0, y0, c00);
+  texelf(t, lvl, x1, y0, c10);
+  texelf(t, lvl, x0, y1, c01);
+  texelf(t, lvl, x1, y1, c11);
+  for (int k = 0; k < 4; k++) {
+    c00[k] = c00[k] + fu * (c10[k] - c00[k]);
+    c01[k] = c01[k] + fu * (c11[k] - c01[k]);
+    out[k] = c00[k] + fv * (c01[k] - c00[k]);
+  }
+}
+
+static void sample_float_aniso(const OTexture *t, float u, float v, float lod, float aniso, float uDelta, float vDelta,
+                               int second, float out[4]) {
+  if (t->filter != FILTER_ANISOTROPIC) { sample_float2d(t, u, v, lod, second, out); return; }
+  int N = rr_round_int(aniso);
+  float A = kUVWeight[N], B = kUVStart[N];
+  float sum[4] = {0, 0, 0, 0};
+  float du = uDelta, dv = vDelta;
+  float u0 = u + B * du;
+  float v0 = v + B * dv;
+  du = du * A;
+  dv = dv * A;
+  int i = 0;
+  do {
+    float cq[4];
+    sample_float2d(t, u0, v0, lod, second, cq);
+    u0 = u0 + du;
+    v0 = v0 + dv;
+    for (int k = 0; k < 4; k++) sum[k] = sum[k] + cq[k] * A;
+    i++;
+  } while (i < N);
+  memcpy(out, sum, sizeof(sum));
+}
+
+/* sampleTexture128 for an implicit-LOD 2D sample of one quad (4 lanes). */
+static void sample_texture(const OTexture *t, const float u[4], const float v[4], float out[4][4]) {
+  float lod, aniso = 0.0f, uDelta = 0.0f, vDelta = 0.0f;
+  int singleMip = (t->minLod == t->maxLod);
+  int skipLod = singleMip && t->filter != FILTER_ANISOTROPIC;
+  if (skipLod) {
+    lod = t->minLod;
+  } else {
+    /* computeLod2D (:1376-1422) */
+    float duvdxy[4] = {u[1] - u[0], u[2] - u[0], v[1] - v[0], v[2] - v[0]};
+    float dUV[4] = {duvdxy[0] * t->wwhh[0], duvdxy[1] * t->wwhh[1], duvdxy[2] * t->wwhh[2], duvdxy[3] * t->wwhh[3]};
+    float sq[4] = {dUV[0] * dUV[0], dUV[1] * dUV[1], dUV[2] * dUV[2], dUV[3] * dUV[3]};
+    float dUV2x = sq[0] + sq[2], dUV2y = sq[1] + sq[3];
+    lod = rr_max(dUV2x, dUV2y);
+    if (t->filter == FILTER_ANISOTROPIC) {
+      float det = dUV[0] * dUV[3] - dUV[1] * dUV[2];
+      det = det > 0.0f ? det : -det;
+      int mask = !(dUV2x < dUV2y);
+      uDelta = mask ? duvdxy[0] : duvdxy[1];
+      vDelta = mask ? duvdxy[2] : duvdxy[3];
+      aniso = lod * (1.0f / det);
+      aniso = rr_min(aniso, t->maxAnisotropy);
+      lod = lod * (1.0f / (aniso * aniso));
+    }
+    lod = log2sqrt(lod);
+    lod = lod + t->mipLodBias;
+  }
+  if (!skipLod) {
+    lod = rr_max(lod, t->minLod);
+    lod = rr_min(lod, t->maxLod);
+  }
+  for (int lane = 0; lane < 4; lane++) {
+    if (t->isFloat) {
+      float c[4];
+      sample_float_aniso(t, u[lane], v[lane], lod, aniso, uDelta, vDelta, 0, c);
+      if (t->mipmap == MIPMAP_LINEAR) {
+        float cc[4];
+        sample_float_aniso(t, u[lane], v[lane], lod, aniso, uDelta, vDelta, 1, cc);
+        float l4 = rr_frac(lod);
+        for (int k = 0; k < 4; k++) c[k] = (cc[k] - c[k]) * l4 + c[k];
+      }
+      memcpy(out[lane], c, 16);
+    } else {
+      uint16_t c[4];
+      sample_aniso16(t, u[lane], v[lane], lod, aniso, uDelta, vDelta, 0, c);
+      if (t->mipmap == MIPMAP_LINEAR) {
+        uint16_t cc[4];
+        sample_aniso16(t, u[lane], v[lane], lod, aniso, uDelta, vDelta, 1, cc);
+        float l = lod * 65536.0f;
+        uint16_t utri = (uint16_t)rr_trunc_int(l);
+        uint16_t inv = (uint16_t)~utri;
+        for (int k = 0; k < 4; k++) c[k] = (uint16_t)(mulhi_u16(c[k], inv) + mulhi_u16(cc[k], utri));
+      }
+      const float scale = 1.0f / 65280.0f;
+      for (int k = 0; k < 4; k++) out[lane][k] = (float)c[k] * scale;
+    }
+  }
+}
+
+/* ===================================================================== */
+/* Pixel routine (PixelRoutine.cpp / PixelProgram.cpp)                    */
+/* ===================================================================== */
+
+static uint8_t stencil_op(uint8_t v, int op, uint8_t ref) {
+  switch (op) {
+    case SWR_STENCIL_OP_KEEP: return v;
+    case SWR_STENCIL_OP_ZERO: return 0;
+    case SWR_STENCIL_OP_REPLACE: return ref;
+    case SWR_STENCIL_OP_INCREMENT_AND_CLAMP: return v == 255 ? 255 : (uint8_t)(v + 1);
+    case SWR_STENCIL_OP_DECREMENT_AND_CLAMP: return v == 0 ? 0 : (uint8_t)(v - 1);
+    case SWR_STENCIL_OP_INVERT: return (uint8_t)~v;
+    case SWR_STENCIL_OP_INCREMENT_AND_WRAP: return (uint8_t)(v + 1);
+    default: return (uint8_t)(v - 1);
+  }
+}
+
+/* (ref & mask) OP (value & mask), via the biased signed compares of :405-449. */
+static int stencil_compare(uint8_t value, const swr_stencil_op_state *s) {
+  int useCompareMask = s->compare_mask != 0xFF;
+  uint8_t val = useCompareMask ? (uint8_t)(value & s->compare_mask) : value;
+  uint8_t refMasked = (uint8_t)(s->reference & s->compare_mask & 0xFF);
+  int8_t refSigned = (int8_t)(uint8_t)(((s->reference & s->compare_mask) + 0x80) & 0xFF);
+  int8_t valSigned = (int8_t)(uint8_t)(val + 0x80);
+  switch (s->compare_op) {
+    case SWR_COMPARE_ALWAYS: return 1;
+    case SWR_COMPARE_NEVER: return 0;
+    case SWR_COMPARE_LESS: return valSigned > refSigned;
+    case SWR_COMPARE_EQUAL: return val == refMasked;
+    case SWR_COMPARE_NOT_EQUAL: return val != refMasked;
+    case SWR_COMPARE_LESS_OR_EQUAL: return (valSigned > refSigned) || (val == refMasked);
+    case SWR_COMPARE_GREATER: return refSigned > valSigned;
+    default: return !(valSigned > refSigned); /* GREATER_OR_EQUAL */
+  }
+}
+
+static uint8_t *stencil_at(Ctx *c, int q, int x, int y) {
+  return (uint8_t *)c->d->stencil.data + (size_t)q * c->d->stencil.sample_pitch_bytes +
+         (size_t)y * c->d->stencil.row_pitch_bytes + (size_t)x;
+}
+static float *depth_at(Ctx *c, int q, int x, int y) {
+  return (float *)((uint8_t *)c->d->depth.data + (size_t)q * c->d->depth.sample_pitch_bytes +
+                   (size_t)y * c->d->depth.row_pitch_bytes + (size_t)x * 4);
+}
+static uint8_t *color_at(Ctx *c, int q, int x, int y) {
+  return (uint8_t *)c->d->color.data + (size_t)q * c->d->color.sample_pitch_bytes +
+         (size_t)y * c->d->color.row_pitch_bytes + (size_t)x * (c->colorIsFloat ? 16 : 4);
+}
+
+static float blend_factor_rgb(const Ctx *c, int f, int comp, const float *src, const float *dst) {
+  switch (f) {
+    case SWR_BLEND_FACTOR_ZERO: return 0.0f;
+    case SWR_BLEND_FACTOR_ONE: return 1.0f;
+    case SWR_BLEND_FACTOR_SRC_COLOR: return src[comp];
+    case SWR_BLEND_FACTOR_ONE_MINUS_SRC_COLOR: return 1.0f - src[comp];
+    case SWR_BLEND_FACTOR_DST_COLOR: return dst[comp];
+    case SWR_BLEND_FACTOR_ONE_MINUS_DST_COLOR: return 1.0f - dst[comp];
+    case SWR_BLEND_FACTOR_SRC_ALPHA: return src[3];
+    case SWR_BLEND_FACTOR_ONE_MINUS_SRC_ALPHA: return 1.0f - src[3];
+    case SWR_BLEND_FACTOR_DST_ALPHA: return dst[3];
+    case SWR_BLEND_FACTOR_ONE_MINUS_DST_ALPHA: return 1.0f - dst[3];
+    case SWR_BLEND_FACTOR_SRC_ALPHA_SATURATE: return rr_min(1.0f - dst[3], src[3]);
+    case SWR_BLEND_FACTOR_CONSTANT_COLOR: return c->colorIsFloat ? c->blendConstF[comp] : c->blendConstU[comp];
+    case SWR_BLEND_FACTOR_CONSTANT_ALPHA: return c->colorIsFloat ? c->blendConstF[3] : c->blendConstU[3];
+    case SWR_BLEND_FACTOR_ONE_MINUS_CONSTANT_COLOR: return c->colorIsFloat ? c->invBlendConstF[comp] : c->invBlendConstU[comp];
+    default: return c->colorIsFloat ? c->invBlendConstF[3] : c->invBlendConstU[3];
+  }
+}
+
+static float blend_factor_alpha(const Ctx *c, int f, const float *src, const float *dst) {
+  switch (f) {
+    case SWR_BLEND_FACTOR_ZERO: return 0.0f;
+    case SWR_BLEND_FACTOR_ONE: return 1.0f;
+    case SWR_BLEND_FACTOR_SRC_COLOR:
+    case SWR_BLEND_FACTOR_SRC_ALPHA: return src[3];
+    case SWR_BLEND_FACTOR_ONE_MINUS_SRC_COLOR:
+    case SWR_BLEND_FACTOR_ONE_MINUS_SRC_ALPHA: return 1.0f - src[3];
+    case SWR_BLEND_FACTOR_DST_COLOR:
+    case SWR_BLEND_FACTOR_DST_ALPHA: return dst[3];
+    case SWR_BLEND_FACTOR_ONE_MINUS_DST_COLOR:
+    case SWR_BLEND_FACTOR_ONE_MINUS_DST_ALPHA: return 1.0f - dst[3];
+    case SWR_BLEND_FACTOR_SRC_ALPHA_SATURATE: return 1.0f;
+    case SWR_BLEND_FACTOR_CONSTANT_COLOR:
+    case SWR_BLEND_FACTOR_CONSTANT_ALPHA: return c->colorIsFloat ? c->blendConstF[3] : c->blendConstU[3];
+    default: return c->colorIsFloat ? c->invBlendConstF[3] : c->invBlendConstU[3];
+  }
+}
+
+static float blend_apply(int op, float s, float sf, float d, float df) {
+  switch (op) {
+    case SWR_BLEND_OP_ADD: return s * sf + d * df;
+    case SWR_BLEND_OP_SUBTRACT: return s * sf - d * df;
+    case SWR_BLEND_OP_REVERSE_SUBTRACT: return d * df - s * sf;
+    case SWR_BLEND_OP_MIN: return rr_min(s, d);
+    case SWR_BLEND_OP_MAX: return rr_max(s, d);
+    case BLEND_OP_SRC_EXT: return s;
+    case BLEND_OP_DST_EXT: return d;
+    default: return 0.0f; /* ZERO_EXT */
+  }
+}
+
+/* alphaBlend (PixelRoutine.cpp:1617-1961) for one pixel. */
+static void alpha_blend(Ctx *c, const uint8_t *buf, const float src[4], float out[4]) {
+  if (!c->blendEnable) { memcpy(out, src, 16); return; }
+  float dst[4];
+  if (c->colorIsFloat) {
+    memcpy(dst, buf, 16);
+  } else {
+    for (int k = 0; k < 4; k++) dst[k] = (float)(buf[k] * 257) * (1.0f / 65535.0f);
+  }
+  float sf[4], df[4];
+  for (int k = 0; k < 3; k++) {
+    sf[k] = blend_factor_rgb(c, c->srcF, k, src, dst);
+    df[k] = blend_factor_rgb(c, c->dstF, k, src, dst);
+  }
+  sf[3] = blend_factor_alpha(c, c->srcFA, src, dst);
+  df[3] = blend_factor_alpha(c, c->dstFA, src, dst);
+  for (int k = 0; k < 3; k++) out[k] = blend_apply(c->colorOp, src[k], sf[k], dst[k], df[k]);
+  out[3] = blend_apply(c->alphaOp, src[3], sf[3], dst[3], df[3]);
+}
+
+static void write_color(Ctx *c, uint8_t *buf, const float col[4]) {
+  int wm = c->colorWriteActive;
+  if (c->colorIsFloat) {
+    float *p = (float *)buf;
+    for (int k = 0; k < 4; k++)
+      if (wm & (1 << k)) p[k] = col[k];
+  } else {
+    for (int k = 0; k < 4; k++) {
+      float v = rr_min(rr_max(col[k], 0.0f), 1.0f);
+      int32_t iv = rr_round_int(v * 255.0f);
+      uint8_t b = (uint8_t)(iv < 0 ? 0 : (iv > 255 ? 255 : iv)); /* PackUnsigned saturation */
+      if (wm & (1 << k)) buf[k] = b;
+    }
+  }
+}
+
+static void quad(Ctx *c, const OPrim *prim, int x, int y, const int cMask[4], const float Dz[4][4], const float Dw[4],
+                 const float Dv[NATTR][4]) {
+  static const float xm[4] = {0, 1, 0, 1};
+  int zMask[4] = {0, 0, 0, 0}, sMask[4] = {0, 0, 0, 0};
+  int samples[4], ns = 0;
+  for (int q = 0; q < c->ms; q++)
+    if (c->msMask & (1u << q)) samples[ns++] = q;
+  if (ns == 0) return;
+  for (int s = 0; s < ns; s++) { int q = samples[s]; zMask[q] = cMask[q]; sMask[q] = cMask[q]; }
+
+  /* stencilTest */
+  if (c->stencilActive) {
+    for (int s = 0; s < ns; s++) {
+      int q = samples[s];
+      int m = 0;
+      for (int i = 0; i < 4; i++) {
+        if (!(cMask[q] & (1 << i))) continue; /* uncovered lanes are masked out */
+        uint8_t val = *stencil_at(c, q, x + (i & 1), y + (i >> 1));
+        int pass = prim->frontFacing ? stencil_compare(val, &c->d->front) : stencil_compare(val, &c->d->back);
+        if (pass) m |= 1 << i;
+      }
+      sMask[q] &= m;
+    }
+  }
+
+  float xFragment[4];
+  for (int i = 0; i < 4; i++) xFragment[i] = ((float)x + xm[i]) - prim->x0;
+  float z[4][4];
+  if (c->depthTestActive) {
+    for (int s = 0; s < ns; s++) {
+      int q = samples[s];
+      for (int i = 0; i < 4; i++) {
+        float xx = xFragment[i];
+        if (c->enableMS) xx = xx - kSampleLocationsX[q];
+        z[q][i] = xx * prim->z.A + Dz[q][i];
+      }
+    }
+  }
+  /* Late fragment tests: the shader runs first. */
+  float rhw[4];
+  for (int i = 0; i < 4; i++) {
+    float w = xFragment[i] * prim->w.A + Dw[i];
+    rhw[i] = 1.0f / w;
+  }
+  float in[NATTR][4];
+  for (int k = 0; k < NATTR; k++) {
+    if ((k < 4 && !c->useColor) || (k >= 4 && !c->useUV)) continue;
+    for (int i = 0; i < 4; i++) in[k][i] = (xFragment[i] * prim->V[k].A + Dv[k][i]) * rhw[i];
+  }
+  float col[4][4];
+  int shader = c->d->shader;
+  if (shader == SWR_SHADER_INTERPOLATE) {
+    for (int i = 0; i < 4; i++)
+      for (int k = 0; k < 4; k++) col[i][k] = in[k][i];
+  } else {
+    sample_texture(&c->tex, in[4], in[5], col);
+    if (shader == SWR_SHADER_MODULATE)
+      for (int i = 0; i < 4; i++)
+        for (int k = 0; k < 4; k++) col[i][k] = col[i][k] * in[k][i];
+  }
+  /* clampColor for UNORM targets */
+  if (!c->colorIsFloat && c->colorWriteActive)
+    for (int i = 0; i < 4; i++)
+      for (int k = 0; k < 4; k++) col[i][k] = rr_min(rr_max(col[i][k], 0.0f), 1.0f);
+
+  int depthPass = 0;
+  for (int s = 0; s < ns; s++) {
+    int q = samples[s];
+    if (!c->depthTestActive) { depthPass = 1; continue; }
+    int m = 0;
+    for (int i = 0; i < 4; i++) {
+      z[q][i] = rr_min(rr_max(z[q][i], 0.0f), 1.0f); /* clampDepth: D32F without unrestricted range */
+      float Z = z[q][i];
+      if (!(cMask[q] & (1 << i))) continue; /* uncovered lanes are masked out */
+      float zb = *depth_at(c, q, x + (i & 1), y + (i >> 1));
+      int t;
+      switch (c->depthCompare) {
+        case SWR_COMPARE_ALWAYS: t = 1; break;
+        case SWR_COMPARE_NEVER: t = 0; break;
+        case SWR_COMPARE_EQUAL: t = (zb == Z); break;
+        case SWR_COMPARE_NOT_EQUAL: t = !(zb == Z); break;
+        case SWR_COMPARE_LESS: t = !(zb <= Z); break;
+        case SWR_COMPARE_GREATER_OR_EQUAL: t = (zb <= Z); break;
+        case SWR_COMPARE_LESS_OR_EQUAL: t = !(zb < Z); break;
+        default: t = (zb < Z); break; /* GREATER */
+      }
+      if (t) m |= 1 << i;
+    }
+    zMask[q] = m & cMask[q];
+    if (c->depthCompare == SWR_COMPARE_ALWAYS) zMask[q] = cMask[q];
+    if (c->depthCompare == SWR_COMPARE_NEVER) zMask[q] = 0;
+    if (c->stencilActive) zMask[q] &= sMask[q];
+    if (zMask[q] != 0) depthPass = 1;
+  }
+
+  if (depthPass) {
+    if (c->depthWriteEnable) {
+      for (int s = 0; s < ns; s++) {
+        int q = samples[s];
+        for (int i = 0; i < 4; i++)
+          if (zMask[q] & (1 << i)) *depth_at(c, q, x + (i & 1), y + (i >> 1)) = z[q][i];
+      }
+    }
+    for (int s = 0; s < ns; s++) {
+      int q = samples[s];
+      c->occlusion += (uint64_t)__builtin_popcount((unsigned)(zMask[q] & sMask[q]));
+    }
+    if (c->colorWriteActive) {
+      for (int s = 0; s < ns; s++) {
+        int q = samples[s];
+        int xMask = c->depthTestActive ? zMask[q] : cMask[q];
+        if (c->stencilActive) xMask &= sMask[q];
+        for (int i = 0; i < 4; i++) {
+          if (!(xMask & (1 << i))) continue;
+          uint8_t *buf = color_at(c, q, x + (i & 1), y + (i >> 1));
+          float bl[4];
+          alpha_blend(c, buf, col[i], bl);
+          write_color(c, buf, bl);
+        }
+      }
+    }
+  }
+
+  /* writeStencil */
+  if (c->stencilActive) {
+    const swr_stencil_op_state *F = &c->d->front, *Bk = &c->d->back;
+    int allKeep = F->pass_op == 0 && F->depth_fail_op == 0 && F->fail_op == 0 && Bk->pass_op == 0 &&
+                  Bk->depth_fail_op == 0 && Bk->fail_op == 0;
+    if (!allKeep && (F->write_mask != 0 || Bk->write_mask != 0)) {
+      const swr_stencil_op_state *S = prim->frontFacing ? F : Bk;
+      for (int s = 0; s < ns; s++) {
+        int q = samples[s];
+        for (int i = 0; i < 4; i++) {
+          if (!(cMask[q] & (1 << i))) continue;
+          uint8_t *p = stencil_at(c, q, x + (i & 1), y + (i >> 1));
+          uint8_t bv = *p;
+          int op;
+          if (!(sMask[q] & (1 << i))) op = S->fail_op;
+          else if (c->depthTestActive && !(zMask[q] & (1 << i))) op = S->depth_fail_op;
+          else op = S->pass_op;
+          uint8_t nv = stencil_op(bv, op, (uint8_t)S->reference);
+          if ((S->write_mask & 0xFF) != 0xFF) nv = (uint8_t)((nv & S->write_mask) | (bv & ~S->write_mask));
+          *p = nv;
+        }
+      }
+    }
+  }
+}
+
+/* QuadRasterizer::rasterize (QuadRasterizer.cpp:71-233) for all row pairs. */
+static void rasterize(Ctx *c, const OPrim *prim) {
+  for (int y = prim->yMin & ~1; y < prim->yMax; y += 2) {
+    int x0 = imin(*LEFT(c, 0, y), *LEFT(c, 0, y + 1));
+    for (int q = 1; q < c->ms; q++) x0 = imin(x0, imin(*LEFT(c, q, y), *LEFT(c, q, y + 1)));
+    x0 &= ~1;
+    int x1 = imax(*RIGHT(c, 0, y), *RIGHT(c, 0, y + 1));
+    for (int q = 1; q < c->ms; q++) x1 = imax(x1, imax(*RIGHT(c, q, y), *RIGHT(c, q, y + 1)));
+    static const float ym[4] = {0, 0, 1, 1};
+    float yFragment[4];
+    for (int i = 0; i < 4; i++) yFragment[i] = ((float)y + ym[i]) - prim->y0;
+    float Dz[4][4];
+    if (c->depthTestActive) {
+      for (int q = 0; q < c->ms; q++)
+        for (int i = 0; i < 4; i++) {
+          float yy = yFragment[i];
+          if (c->enableMS) yy = yy + kSampleLocationsY[q];
+          Dz[q][i] = prim->z.C + yy * prim->z.B;
+        }
+    }
+    if (!(x0 < x1)) continue;
+    float Dw[4], Dv[NATTR][4];
+    for (int i = 0; i < 4; i++) Dw[i] = prim->w.C + yFragment[i] * prim->w.B;
+    for (int k = 0; k < NATTR; k++)
+      for (int i = 0; i < 4; i++) Dv[k][i] = prim->V[k].C + yFragment[i] * prim->V[k].B;
+    int16_t xl[4][4], xr[4][4];
+    for (int q = 0; q < c->ms; q++) {
+      int16_t L0 = (int16_t)*LEFT(c, q, y), L1 = (int16_t)*LEFT(c, q, y + 1);
+      int16_t R0 = (int16_t)*RIGHT(c, q, y), R1 = (int16_t)*RIGHT(c, q, y + 1);
+      xl[q][0] = (int16_t)(L0 - 1); xl[q][1] = (int16_t)(L0 - 2); xl[q][2] = (int16_t)(L1 - 1); xl[q][3] = (int16_t)(L1 - 2);
+      xr[q][0] = R0; xr[q][1] = (int16_t)(R0 - 1); xr[q][2] = R1; xr[q][3] = (int16_t)(R1 - 1);
+    }
+    for (int x = x0; x < x1; x += 2) {
+      int cMask[4] = {0, 0, 0, 0};
+      int16_t xx = (int16_t)x;
+      for (int q = 0; q < c->ms; q++) {
+        if (!(c->msMask & (1u << q))) continue;
+        int i = c->enableMS ? q : 0;
+        int m = 0;
+        for (int l = 0; l < 4; l++)
+          if (xx > xl[i][l] && xr[i][l] > xx) m |= 1 << l;
+        cMask[q] = m;
+      }
+      quad(c, prim, x, y, cMask, Dz, Dw, Dv);
+    }
+  }
+}
+
+/* ===================================================================== */
+/* State derivation                                                       */
+/* ===================================================================== */
+
+static int blend_operation(int op, int sf, int df, int isFloat) {
+  switch (op) {
+    case SWR_BLEND_OP_ADD:
+      if (sf == SWR_BLEND_FACTOR_ZERO) {
+        if (df == SWR_BLEND_FACTOR_ZERO) return BLEND_OP_ZERO_EXT;
+        if (df == SWR_BLEND_FACTOR_ONE) return BLEND_OP_DST_EXT;
+      } else if (sf == SWR_BLEND_FACTOR_ONE) {
+        if (df == SWR_BLEND_FACTOR_ZERO) return BLEND_OP_SRC_EXT;
+      }
+      break;
+    case SWR_BLEND_OP_SUBTRACT:
+      if (sf == SWR_BLEND_FACTOR_ZERO) {
+        if (df == SWR_BLEND_FACTOR_ZERO) return BLEND_OP_ZERO_EXT;
+        if (!isFloat) return BLEND_OP_ZERO_EXT;
+      } else if (sf == SWR_BLEND_FACTOR_ONE) {
+        if (df == SWR_BLEND_FACTOR_ZERO) return BLEND_OP_SRC_EXT;
+      }
+      break;
+    case SWR_BLEND_OP_REVERSE_SUBTRACT:
+      if (sf == SWR_BLEND_FACTOR_ZERO) {
+        if (df == SWR_BLEND_FACTOR_ZERO) return BLEND_OP_ZERO_EXT;
+        if (df == SWR_BLEND_FACTOR_ONE) return BLEND_OP_DST_EXT;
+      } else {
+        if (df == SWR_BLEND_FACTOR_ZERO && !isFloat) return BLEND_OP_ZERO_EXT;
+      }
+      break;
+    default: break;
+  }
+  return op;
+}
+
+static int blend_factor(int op, int f) {
+  return (op == SWR_BLEND_OP_MIN || op == SWR_BLEND_OP_MAX) ? SWR_BLEND_FACTOR_ONE : f;
+}
+
+static float clampf(float x, float lo, float hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+static int init_ctx(Ctx *c, const swr_draw_desc *d) {
+  memset(c, 0, sizeof(*c));
+  c->d = d;
+  init_constants();
+  if (d->samples != 1 && d->samples != 4) return SWR_ERROR_UNSUPPORTED;
+  const swr_viewport *vp = &d->viewport;
+  float W = 0.5f * vp->width, H = 0.5f * vp->height;
+  float X0 = vp->x + W, Y0 = vp->y + H;
+  c->WxF = W * 16.0f;
+  c->HxF = H * 16.0f;
+  c->X0xF = X0 * 16.0f - 16.0f / 2;
+  c->Y0xF = Y0 * 16.0f - 16.0f / 2;
+  c->depthRange = vp->max_depth - vp->min_depth;
+  c->depthNear = vp->min_depth;
+  int ax0 = d->render_area.x, ay0 = d->render_area.y;
+  int ax1 = ax0 + (int)d->render_area.width, ay1 = ay0 + (int)d->render_area.height;
+  c->scissorX0 = imin(imax(d->scissor.x, ax0), ax1);
+  c->scissorX1 = imin(imax(d->scissor.x + (int)d->scissor.width, ax0), ax1);
+  c->scissorY0 = imin(imax(d->scissor.y, ay0), ay1);
+  c->scissorY1 = imin(imax(d->scissor.y + (int)d->scissor.height, ay0), ay1);
+  c->ms = (int)d->samples;
+  c->enableMS = c->ms > 1;
+  c->msMask = d->sample_mask & (0xFFFFFFFFu >> (32 - c->ms));
+
+  c->depthTestActive = d->depth.data && d->depth_test_enable;
+  c->depthWriteEnable = c->depthTestActive && d->depth_write_enable;
+  c->depthCompare = (int)d->depth_compare_op;
+  c->stencilActive = d->stencil.data && d->stencil_test_enable;
+
+  c->colorIsFloat = d->color.format == SWR_FORMAT_R32G32B32A32_SFLOAT;
+  const swr_blend_state *b = &d->blend;
+  int cop = blend_operation((int)b->color_blend_op, (int)b->src_color_blend_factor, (int)b->dst_color_blend_factor, c->colorIsFloat);
+  int aop = blend_operation((int)b->alpha_blend_op, (int)b->src_alpha_blend_factor, (int)b->dst_alpha_blend_factor, c->colorIsFloat);
+  if (d->color.data && d->color.format != SWR_FORMAT_UNDEFINED)
+    c->colorWriteActive = (cop == BLEND_OP_DST_EXT && aop == BLEND_OP_DST_EXT) ? 0 : (int)(b->color_write_mask & 0xF);
+  c->blendEnable = d->color.data && b->blend_enable && c->colorWriteActive && (cop != BLEND_OP_SRC_EXT || aop != BLEND_OP_SRC_EXT);
+  if (c->blendEnable) {
+    c->srcF = blend_factor((int)b->color_blend_op, (int)b->src_color_blend_factor);
+    c->dstF = blend_factor((int)b->color_blend_op, (int)b->dst_color_blend_factor);
+    c->colorOp = cop;
+    c->srcFA = blend_factor((int)b->alpha_blend_op, (int)b->src_alpha_blend_factor);
+    c->dstFA = blend_factor((int)b->alpha_blend_op, (int)b->dst_alpha_blend_factor);
+    c->alphaOp = aop;
+  }
+  for (int k = 0; k < 4; k++) {
+    float bc = d->blend_constants[k];
+    c->blendConstF[k] = bc;
+    c->invBlendConstF[k] = 1.0f - bc;
+    c->blendConstU[k] = clampf(bc, 0.0f, 1.0f);
+    c->invBlendConstU[k] = 1.0f - clampf(bc, 0.0f, 1.0f);
+  }
+  c->useColor = d->shader != SWR_SHADER_SAMPLE;
+  c->useUV = d->shader != SWR_SHADER_INTERPOLATE;
+
+  if (c->useUV) {
+    OTexture *t = &c->tex;
+    const swr_texture *tx = &d->texture;
+    const swr_sampler_state *s = &d->sampler;
+    if (tx->level_count < 1 || tx->level_count > SWR_MAX_MIP_LEVELS) return SWR_ERROR_INVALID_ARGUMENT;
+    t->isFloat = tx->format == SWR_FORMAT_R32G32B32A32_SFLOAT;
+    int texelBytes = t->isFloat ? 16 : 4;
+    for (int l = 0; l < SWR_MAX_MIP_LEVELS; l++) {
+      int vl = imin(l, (int)tx->level_count - 1);
+      t->data[l] = (const uint8_t *)tx->levels[vl].data;
+      t->width[l] = (int)tx->levels[vl].width;
+      t->height[l] = (int)tx->levels[vl].height;
+      t->pitchP[l] = (int)(tx->levels[vl].row_pitch_bytes / (uint32_t)texelBytes);
+      t->uHalf[l] = (uint16_t)(0x8000 / (uint32_t)t->width[l]);
+      t->vHalf[l] = (uint16_t)(0x8000 / (uint32_t)t->height[l]);
+    }
+    t->wwhh[0] = t->wwhh[1] = (float)tx->levels[0].width;
+    t->wwhh[2] = t->wwhh[3] = (float)tx->levels[0].height;
+    if (s->anisotropy_enable) t->filter = FILTER_ANISOTROPIC;
+    else if (s->mag_filter == SWR_FILTER_NEAREST) t->filter = s->min_filter == SWR_FILTER_LINEAR ? FILTER_MIN_LINEAR_MAG_POINT : FILTER_POINT;
+    else t->filter = s->min_filter == SWR_FILTER_NEAREST ? FILTER_MIN_POINT_MAG_LINEAR : FILTER_LINEAR;
+    t->mipmap = s->mipmap_mode == SWR_MIPMAP_MODE_LINEAR ? MIPMAP_LINEAR : MIPMAP_POINT;
+    t->addrU = s->address_mode_u == SWR_ADDRESS_CLAMP_TO_EDGE ? ADDR_CLAMP : ADDR_WRAP;
+    t->addrV = s->address_mode_v == SWR_ADDRESS_CLAMP_TO_EDGE ? ADDR_CLAMP : ADDR_WRAP;
+    t->mipLodBias = s->mip_lod_bias;
+    t->maxAnisotropy = s->max_anisotropy;
+    t->minLod = clampf(s->min_lod, 0.0f, 13.0f);
+    t->maxLod = clampf(s->max_lod, 0.0f, 13.0f);
+    if (tx->level_count == 1 && t->filter != FILTER_MIN_POINT_MAG_LINEAR && t->filter != FILTER_MIN_LINEAR_MAG_POINT) {
+      t->minLod = 0.0f;
+      t->maxLod = 0.0f;
+    }
+  }
+  c->outlineRows = (int)d->framebuffer_height + 2 * OUTLINE_OFFSET + 2;
+  for (int q = 0; q < c->ms; q++) {
+    c->left[q] = (uint16_t *)calloc((size_t)c->outlineRows, 2);
+    c->right[q] = (uint16_t *)calloc((size_t)c->outlineRows, 2);
+    if (!c->left[q] || !c->right[q]) return SWR_ERROR_OUT_OF_MEMORY;
+  }
+  return SWR_OK;
+}
+
+static void free_ctx(Ctx *c) {
+  for (int q = 0; q < 4; q++) { free(c->left[q]); free(c->right[q]); }
+}
+
+static void fetch_vertex(const swr_draw_desc *d, uint32_t idx, OVertex *v) {
+  memcpy(v->pos, d->positions + 4 * (size_t)idx, 16);
+  for (int k = 0; k < NATTR; k++) v->v[k] = 0.0f;
+  if (d->colors) memcpy(v->v, d->colors + 4 * (size_t)idx, 16);
+  if (d->texcoords) memcpy(v->v + 4, d->texcoords + 2 * (size_t)idx, 8);
+}
+
+/* Renders one draw with host pointers in the descriptor. */
+EXPORT int swo_draw(const swr_draw_desc *d) {
+  if (!d || !d->positions) return SWR_ERROR_INVALID_ARGUMENT;
+  unsigned csr = _mm_getcsr();
+  _mm_setcsr(csr | 0x8040); /* FTZ | DAZ, as the reference's worker threads */
+  Ctx c;
+  int r = init_ctx(&c, d);
+  if (r == SWR_OK) {
+    for (uint32_t t = 0; t < d->triangle_count; t++) {
+      OVertex V[3];
+      for (int k = 0; k < 3; k++) {
+        uint32_t idx = d->indices ? d->indices[3 * (size_t)t + k] : 3 * t + (uint32_t)k;
+        fetch_vertex(d, idx, &V[k]);
+        vertex_routine(&c, &V[k]);
+      }
+      /* setupSolidTriangles (Renderer.cpp:733-776); cullMask is always 15 here. */
+      if ((V[0].clipFlags & V[1].clipFlags & V[2].clipFlags) != CLIP_FINITE) continue;
+      Polygon poly;
+      poly.P[0][0] = V[0].pos; poly.P[0][1] = V[1].pos; poly.P[0][2] = V[2].pos;
+      poly.n = 3; poly.i = 0; poly.b = 0;
+      int flagsOr = V[0].clipFlags | V[1].clipFlags | V[2].clipFlags;
+      if (flagsOr != CLIP_FINITE && !clipper_clip(&poly, flagsOr)) continue;
+      const OVertex *tri[3] = {&V[0], &V[1], &V[2]};
+      OPrim prim;
+      memset(&prim, 0, sizeof(prim));
+      if (!setup_routine(&c, &prim, tri, &poly)) continue;
+      rasterize(&c, &prim);
+    }
+    if (d->occlusion_count) *d->occlusion_count += c.occlusion;
+  }
+  free_ctx(&c);
+  _mm_setcsr(csr);
+  return r;
+}
+
+/* ===================================================================== */
+/* Resolve (Blitter.cpp:1524-1545 float; :2076-2200 RGBA8 4x fast path)  */
+/* ===================================================================== */
+
+EXPORT int swo_resolve(const swr_resolve_desc *r) {
+  if (!r || r->src_samples != 4) return SWR_ERROR_UNSUPPORTED;
+  unsigned csr = _mm_getcsr();
+  _mm_setcsr(csr | 0x8040);
+  for (uint32_t y = 0; y < r->height; y++) {
+    for (uint32_t x = 0; x < r->width; x++) {
+      if (r->format == SWR_FORMAT_R8G8B8A8_UNORM) {
+        const uint8_t *s[4];
+        for (int q = 0; q < 4; q++)
+          s[q] = (const uint8_t *)r->src + q * r->src_sample_pitch_bytes + (size_t)y * r->src_row_pitch_bytes + x * 4;
+        uint8_t *o = (uint8_t *)r->dst + (size_t)y * r->dst_row_pitch_bytes + x * 4;
+        for (int k = 0; k < 4; k++) {
+          unsigned a01 = (s[0][k] + s[1][k] + 1u) >> 1;
+          unsigned a23 = (s[2][k] + s[3][k] + 1u) >> 1;
+          o[k] = (uint8_t)((a01 + a23 + 1u) >> 1);
+        }
+      } else {
+        const float *s[4];
+        for (int q = 0; q < 4; q++)
+          s[q] = (const float *)((const uint8_t *)r->src + q * r->src_sample_pitch_bytes + (size_t)y * r->src_row_pitch_bytes + x * 16);
+        float *o = (float *)((uint8_t *)r->dst + (size_t)y * r->dst_row_pitch_bytes + x * 16);
+        for (int k = 0; k < 4; k++) o[k] = (((s[0][k] + s[1][k]) + s[2][k]) + s[3][k]) * 0.25f;
+      }
+    }
+  }
+  _mm_setcsr(csr);
+  return SWR_OK;
+}

@@ -1,0 +1,748 @@
+// swr_kernels.hip -- HIP kernels of the fixed-function draw path (gfx950).
+//
+// Pipeline of one draw:
+//   setup_kernel     one thread per triangle: vertex tail (clip flags,
+//                    28.4 projection), trivial reject, clipping, culling,
+//                    plane equations, pixel/tile bounding box.
+//   bin_emit_kernel  one thread per visible triangle: (tile, triangle)
+//                    pairs, triangle-major, at offsets from a prefix sum.
+//   (stable radix sort of the pairs by tile keeps API order inside a tile)
+//   tile_range_kernel  per-tile [start, end) into the sorted pair list.
+//   raster_kernel    one wavefront per 16x16 screen tile, one 2x2 quad per
+//                    lane; the tile's depth / stencil / colour live in
+//                    registers while the wave walks its triangle list in
+//                    order: coverage, stencil, depth, shading, blend.
+//   resolve_kernel   4x -> 1x colour resolve.
+//
+// Reference semantics: src/Pipeline/VertexRoutine.cpp:116-154,601-611,
+// src/Device/Clipper.cpp, src/Pipeline/SetupRoutine.cpp,
+// src/Device/QuadRasterizer.cpp, src/Pipeline/PixelRoutine.cpp,
+// src/Pipeline/PixelProgram.cpp, src/Pipeline/SamplerCore.cpp,
+// src/Device/Blitter.cpp:1524-1545,2076-2200.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "swr_device.hpp"
+#include "swr_params.hpp"
+
+namespace swr {
+
+enum : int { CLIP_RIGHT = 1, CLIP_TOP = 2, CLIP_FAR = 4, CLIP_LEFT = 8, CLIP_BOTTOM = 16, CLIP_NEAR = 32,
+             CLIP_FRUSTUM = 63, CLIP_FINITE = 128 };
+
+// Standard 4x sample locations relative to the pixel centre
+// (Constants.hpp:26-47) and their 28.4 offsets Xf/Yf (Constants.cpp:291-297).
+__device__ __constant__ const float kSampleX[4] = {-0.125f, 0.375f, -0.375f, 0.125f};
+__device__ __constant__ const float kSampleY[4] = {-0.375f, -0.125f, 0.125f, 0.375f};
+__device__ __constant__ const int kXf[4] = {-2, 6, -6, 2};
+__device__ __constant__ const int kYf[4] = {-6, -2, 2, 6};
+
+// ---------------------------------------------------------------------------
+// Setup
+// ---------------------------------------------------------------------------
+struct ClipVtx { float v[4]; };
+
+__device__ __forceinline__ float clip_dist(const ClipVtx &p, int plane) {
+  switch (plane) {
+    case 0: return p.v[2];
+    case 1: return p.v[3] - p.v[2];
+    case 2: return p.v[3] + p.v[0];
+    case 3: return p.v[3] - p.v[0];
+    case 4: return p.v[3] - p.v[1];
+    default: return p.v[3] + p.v[1];
+  }
+}
+
+__device__ __forceinline__ ClipVtx clip_edge(const ClipVtx &Vi, const ClipVtx &Vj, float di, float dj) {
+  float D = 1.0f / (dj - di);
+  ClipVtx o;
+#pragma unroll
+  for (int k = 0; k < 4; k++) o.v[k] = (dj * Vi.v[k] - di * Vj.v[k]) * D;
+  return o;
+}
+
+// Sutherland-Hodgman against near, far, left, right, top, bottom in that
+// order, stopping below 3 vertices (Clipper.cpp:271-299).  Returns the
+// number of clip stages applied (the reference's Polygon::i).
+__device__ int clip_polygon(ClipVtx (&poly)[kMaxPolygon], int &n, int flagsOr) {
+  const int bits[6] = {CLIP_NEAR, CLIP_FAR, CLIP_LEFT, CLIP_RIGHT, CLIP_TOP, CLIP_BOTTOM};
+  int stages = 0;
+  ClipVtx tmp[kMaxPolygon];
+  for (int k = 0; k < 6; k++) {
+    if (k > 0 && n < 3) break;
+    if (!(flagsOr & bits[k])) continue;
+    int t = 0;
+    for (int i = 0; i < n; i++) {
+      int j = (i == n - 1) ? 0 : i + 1;
+      float di = clip_dist(poly[i], k), dj = clip_dist(poly[j], k);
+      if (di >= 0) {
+        if (t < kMaxPolygon) tmp[t] = poly[i];
+        t++;
+        if (dj < 0) {
+          if (t < kMaxPolygon) tmp[t] = clip_edge(poly[i], poly[j], di, dj);
+          t++;
+        }
+      } else if (dj > 0) {
+        if (t < kMaxPolygon) tmp[t] = clip_edge(poly[j], poly[i], dj, di);
+        t++;
+      }
+    }
+    n = t < kMaxPolygon ? t : kMaxPolygon;
+    for (int i = 0; i < n; i++) poly[i] = tmp[i];
+    stages++;
+  }
+  return stages;
+}
+
+__device__ __forceinline__ int wadd(int a, int b) { return (int)((unsigned)a + (unsigned)b); }
+__device__ __forceinline__ int wsub(int a, int b) { return (int)((unsigned)a - (unsigned)b); }
+
+__global__ __launch_bounds__(256) void setup_kernel(DrawParams P, TriRecord *__restrict__ rec,
+                                                    int *__restrict__ poly, uint32_t *__restrict__ tileCount) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= P.triCount) return;
+
+  uint32_t vi[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    vi[k] = P.idx ? P.idx[3 * t + k] : (uint32_t)(3 * t + k);
+    // Out-of-range indices read the last vertex instead of faulting.
+    vi[k] = vi[k] < (uint32_t)P.vertexCount ? vi[k] : (uint32_t)P.vertexCount - 1;
+  }
+
+  float pos[3][4], attr[3][6];
+  int flags[3], PX[3], PY[3];
+  float pz[3], pw[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    float4 p = reinterpret_cast<const float4 *>(P.pos)[vi[k]];
+    pos[k][0] = p.x; pos[k][1] = p.y; pos[k][2] = p.z; pos[k][3] = p.w;
+    if (P.col) {
+      float4 c = reinterpret_cast<const float4 *>(P.col)[vi[k]];
+      attr[k][0] = c.x; attr[k][1] = c.y; attr[k][2] = c.z; attr[k][3] = c.w;
+    } else {
+      attr[k][0] = attr[k][1] = attr[k][2] = attr[k][3] = 0.0f;
+    }
+    if (P.uv) {
+      float2 u = reinterpret_cast<const float2 *>(P.uv)[vi[k]];
+      attr[k][4] = u.x; attr[k][5] = u.y;
+    } else {
+      attr[k][4] = attr[k][5] = 0.0f;
+    }
+    // Clip flags (VertexRoutine.cpp:116-154).
+    float x = p.x, y = p.y, z = p.z, w = p.w;
+    int f = 0;
+    if (w < x) f |= CLIP_RIGHT;
+    if (w < y) f |= CLIP_TOP;
+    if (!(-w <= x)) f |= CLIP_LEFT;
+    if (!(-w <= y)) f |= CLIP_BOTTOM;
+    if (w < z) f |= CLIP_FAR;
+    if (!(0.0f <= z)) f |= CLIP_NEAR;
+    const float maxPos = __uint_as_float(0x7F7FFFFFu);
+    if (__builtin_fabsf(x) <= maxPos && __builtin_fabsf(y) <= maxPos && __builtin_fabsf(z) <= maxPos) f |= CLIP_FINITE;
+    flags[k] = f;
+    // Projection (VertexRoutine.cpp:601-611).
+    float ww = __uint_as_float(__float_as_uint(w) | ((w == 0.0f) ? 0x3F800000u : 0u));
+    float rhw = 1.0f / ww;
+    PX[k] = rr_round_int_clamped(P.X0xF + (x * rhw) * P.WxF);
+    PY[k] = rr_round_int_clamped(P.Y0xF + (y * rhw) * P.HxF);
+    pz[k] = z * rhw;
+    pw[k] = rhw;
+  }
+
+  TriRecord &R = rec[t];
+  tileCount[t] = 0;
+  R.n = 0;
+  // setupSolidTriangles (Renderer.cpp:733-776).
+  if ((flags[0] & flags[1] & flags[2]) != CLIP_FINITE) return;
+  int flagsOr = flags[0] | flags[1] | flags[2];
+  ClipVtx cp[kMaxPolygon];
+  int n = 3;
+  int stages = 0;
+  if (flagsOr != CLIP_FINITE) {
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+      for (int c = 0; c < 4; c++) cp[k].v[c] = pos[k][c];
+    stages = clip_polygon(cp, n, flagsOr);
+    if (n < 3) return;
+  }
+
+  // Culling and facing (SetupRoutine.cpp:71-120).
+  int d = 1;
+  int frontFacing;
+  {
+    float x0 = (float)PX[0], x1 = (float)PX[1], x2 = (float)PX[2];
+    float y0 = (float)PY[0], y1 = (float)PY[1], y2 = (float)PY[2];
+    float A = (y0 - y2) * x1 + (y2 - y1) * x0 + (y1 - y0) * x2;
+    int w012 = (int)(__float_as_uint(pos[0][3]) ^ __float_as_uint(pos[1][3]) ^ __float_as_uint(pos[2][3]));
+    if (w012 < 0) A = -A;
+    frontFacing = (P.frontFace == 0) ? (A >= 0.0f) : (A <= 0.0f);
+    if ((P.cullMode & 1) && frontFacing) return;
+    if ((P.cullMode & 2) && !frontFacing) return;
+    d = (A > 0.0f) ? 1 : 0;
+  }
+
+  // Polygon in 28.4 (reprojected when clipped, SetupRoutine.cpp:122-143).
+  int X[kMaxPolygon], Y[kMaxPolygon];
+  if (stages != 0) {
+    for (int i = 0; i < n; i++) {
+      float w = cp[i].v[3];
+      float rhw = (w != 0.0f) ? 1.0f / w : 1.0f;
+      X[i] = rr_round_int(P.X0xF + (cp[i].v[0] * rhw) * P.WxF);
+      Y[i] = rr_round_int(P.Y0xF + (cp[i].v[1] * rhw) * P.HxF);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 3; i++) { X[i] = PX[i]; Y[i] = PY[i]; }
+  }
+  int yMin = Y[0], yMax = Y[0], xLo = X[0], xHi = X[0];
+  for (int i = 1; i < n; i++) {
+    yMin = min(yMin, Y[i]); yMax = max(yMax, Y[i]);
+    xLo = min(xLo, X[i]); xHi = max(xHi, X[i]);
+  }
+  if (P.ms > 1) {
+    yMin = wadd(yMin, 9) >> 4;   // yMinMultiSampleOffset (Constants.hpp:51)
+    yMax = wadd(yMax, 21) >> 4;  // yMaxMultiSampleOffset (Constants.hpp:52)
+  } else {
+    yMin = wadd(yMin, 15) >> 4;
+    yMax = wadd(yMax, 15) >> 4;
+  }
+  yMin = max(yMin, P.sy0);
+  yMax = min(yMax, P.sy1);
+  if (yMin >= yMax) return;
+  // Columns that can hold coverage: left >= ceil(min X / 16), right <= ceil(max X / 16).
+  const int sxo = (P.ms > 1) ? 6 : 0;
+  int xMin = max(wadd(wsub(xLo, sxo), 15) >> 4, P.sx0);
+  int xMax = min(wadd(wadd(xHi, sxo), 15) >> 4, P.sx1);
+  if (xMin >= xMax) return;
+
+  // Sort by minimum y, then by maximum w (SetupRoutine.cpp:270-294).
+  int o0 = 0, o1 = 1, o2 = 2;
+  {
+    float y0 = pos[0][1], y1 = pos[1][1], y2 = pos[2][1];
+    float ym = rr_min(rr_min(y0, y1), y2);
+    if (ym == y1) { int tt = o0; o0 = o1; o1 = o2; o2 = tt; }
+    if (ym == y2) { int tt = o2; o2 = o1; o1 = o0; o0 = tt; }
+  }
+  {
+    float w0 = pos[o0][3], w1 = pos[o1][3], w2 = pos[o2][3];
+    float wm = rr_max(rr_max(w0, w1), w2);
+    if (wm == w1) { int tt = o0; o0 = o1; o1 = o2; o2 = tt; }
+    if (wm == w2) { int tt = o2; o2 = o1; o1 = o0; o0 = tt; }
+  }
+  // Plane equations (SetupRoutine.cpp:296-548).
+  float w1 = pos[o1][3], w2 = pos[o2][3];
+  float rhw0 = pw[o0];
+  int X0 = PX[o0], X1 = PX[o1], X2 = PX[o2];
+  int Y0 = PY[o0], Y1 = PY[o1], Y2 = PY[o2];
+  R.x0 = (float)X0 * (1.0f / 16.0f);
+  R.y0 = (float)Y0 * (1.0f / 16.0f);
+  X1 = wsub(X1, X0); Y1 = wsub(Y1, Y0);
+  X2 = wsub(X2, X0); Y2 = wsub(Y2, Y0);
+  float x1 = w1 * (1.0f / 16.0f) * (float)X1;
+  float y1 = w1 * (1.0f / 16.0f) * (float)Y1;
+  float x2 = w2 * (1.0f / 16.0f) * (float)X2;
+  float y2 = w2 * (1.0f / 16.0f) * (float)Y2;
+  float a = x1 * y2 - x2 * y1;
+  float M00 = 0.0f, M01 = 0.0f, M02 = rhw0, M10 = 0.0f, M11 = 0.0f, M20 = 0.0f, M21 = 0.0f;
+  if (a != 0.0f) {
+    float A = 1.0f / a;
+    float D = A * rhw0;
+    M00 = (y1 * w2 - y2 * w1) * D;
+    M01 = (x2 * w1 - x1 * w2) * D;
+    M10 = y2 * A;
+    M11 = -x2 * A;
+    M20 = -y1 * A;
+    M21 = x1 * A;
+  }
+  R.wA = (M00 + M10) + M20;
+  R.wB = (M01 + M11) + M21;
+  R.wC = (M02 + 0.0f) + 0.0f;
+  if (P.depthTest) {
+    float z0 = pz[o0], z1 = pz[o1] - z0, z2 = pz[o2] - z0;
+    float fx1 = (float)X1 * (1.0f / 16.0f), fy1 = (float)Y1 * (1.0f / 16.0f);
+    float fx2 = (float)X2 * (1.0f / 16.0f), fy2 = (float)Y2 * (1.0f / 16.0f);
+    float D = P.depthRange / (fx1 * fy2 - fx2 * fy1);
+    R.zA = (fy2 * z1 - fy1 * z2) * D;
+    R.zB = (fx1 * z2 - fx2 * z1) * D;
+    R.zC = z0 * P.depthRange + P.depthNear;
+  } else {
+    R.zA = R.zB = R.zC = 0.0f;
+  }
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    float i0 = attr[o0][k], i1 = attr[o1][k], i2 = attr[o2][k];
+    R.V[k][0] = (i0 * M00 + i1 * M10) + i2 * M20;
+    R.V[k][1] = (i0 * M01 + i1 * M11) + i2 * M21;
+    R.V[k][2] = (i0 * M02 + i1 * 0.0f) + i2 * 0.0f;
+  }
+
+  R.flags = d | (frontFacing ? 2 : 0);
+  R.yMin = yMin;
+  R.yMax = yMax;
+  R.xMin = xMin;
+  R.xMax = xMax;
+  if (n == 3) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) { R.X[i] = X[i]; R.Y[i] = Y[i]; }
+  } else {
+    int *pp = poly + (size_t)t * (2 * kMaxPolygon);
+    for (int i = 0; i < n; i++) { pp[i] = X[i]; pp[kMaxPolygon + i] = Y[i]; }
+  }
+  R.n = n;
+  int tx0 = xMin / P.tileW, tx1 = (xMax - 1) / P.tileW;
+  int ty0 = yMin / P.tileH, ty1 = (yMax - 1) / P.tileH;
+  tileCount[t] = (uint32_t)((tx1 - tx0 + 1) * (ty1 - ty0 + 1));
+}
+
+// ---------------------------------------------------------------------------
+// Binning
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void bin_emit_kernel(DrawParams P, const TriRecord *__restrict__ rec,
+                                                       const uint32_t *__restrict__ tileOffset,
+                                                       uint32_t *__restrict__ keys, uint32_t *__restrict__ vals) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= P.triCount) return;
+  uint32_t off = tileOffset[t];
+  uint32_t cnt = tileOffset[t + 1] - off;
+  if (cnt == 0) return;
+  const TriRecord &R = rec[t];
+  int tx0 = R.xMin / P.tileW, tx1 = (R.xMax - 1) / P.tileW;
+  int ty0 = R.yMin / P.tileH, ty1 = (R.yMax - 1) / P.tileH;
+  uint32_t k = off;
+  for (int ty = ty0; ty <= ty1; ty++)
+    for (int tx = tx0; tx <= tx1; tx++) {
+      keys[k] = (uint32_t)(ty * P.tilesX + tx);
+      vals[k] = (uint32_t)t;
+      k++;
+    }
+}
+
+__global__ void fill_u32_kernel(uint32_t *p, uint32_t v, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void tile_range_kernel(const uint32_t *__restrict__ keys, uint32_t n, uint32_t *__restrict__ start,
+                                  uint32_t *__restrict__ end) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t k = keys[i];
+  if (i == 0 || keys[i - 1] != k) start[k] = i;
+  if (i == n - 1 || keys[i + 1] != k) end[k] = i + 1;
+}
+
+// ---------------------------------------------------------------------------
+// Tile rasteriser
+// ---------------------------------------------------------------------------
+// An edge of the sample-shifted polygon, in the form the per-pixel test
+// needs.  The reference walks each edge with an exact integer DDA that
+// stores x = ceil(Xe(y) / 16) for rows [ceil(Y1/16), ceil(Y2/16)) into the
+// left or right span (SetupRoutine.cpp:550-621), and a lane is covered iff
+// left <= x < right (QuadRasterizer.cpp:181-206).  With
+//   E(x, y) = DY * (16 x - X1) - DX * (16 y - Y1),   DY > 0,
+// "ceil(Xe/16) <= x" is exactly E >= 0 and "x < ceil(Xe/16)" is exactly
+// E < 0, so the span test becomes two integer sign tests per pixel.
+struct Edge {
+  int e0;     // E at the tile origin, clamped to +-2^30 (sign exact over the tile)
+  int ex;     // dE per pixel step in x (16 DY)
+  int ey;     // dE per pixel step in y (-16 DX)
+  int y1, y2; // rows covered
+  int right;  // writes the right span (edge runs upwards)
+};
+
+__device__ __forceinline__ Edge make_edge(int Xa, int Ya, int Xb, int Yb, int ox, int oy) {
+  Edge e;
+  bool swap = Yb < Ya;
+  int X1 = swap ? Xb : Xa, X2 = swap ? Xa : Xb;
+  int Y1 = swap ? Yb : Ya, Y2 = swap ? Ya : Yb;
+  int DX = X2 - X1, DY = Y2 - Y1;
+  e.y1 = (Y1 + 15) >> 4;
+  e.y2 = (Y2 + 15) >> 4;
+  long long E = (long long)DY * (long long)(16 * ox - X1) - (long long)DX * (long long)(16 * oy - Y1);
+  const long long lim = 1ll << 30;
+  e.e0 = (int)(E > lim ? lim : (E < -lim ? -lim : E));
+  e.ex = 16 * DY;
+  e.ey = -16 * DX;
+  e.right = swap ? 1 : 0;
+  if (Ya == Yb) { e.y1 = 0; e.y2 = 0; }  // horizontal edges write nothing
+  return e;
+}
+
+template <int MS, bool FCOL, bool FTEX>
+struct TileState {
+  float zb[MS][4];
+  uint32_t sb[MS][4];
+  uint32_t cb[MS][4];          // RGBA8 target
+  float4 cf[FCOL ? MS : 1][4]; // RGBA32F target
+};
+
+template <int MS, bool FCOL, bool FTEX>
+__global__ __launch_bounds__(256) void raster_kernel(DrawParams P, const TriRecord *__restrict__ rec,
+                                                     const int *__restrict__ poly,
+                                                     const uint32_t *__restrict__ binTri,
+                                                     const uint32_t *__restrict__ tileStart,
+                                                     const uint32_t *__restrict__ tileEnd) {
+  const int lane = threadIdx.x & 63;
+  const int tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (tile >= P.tilesX * P.tilesY) return;
+  const int tx = tile % P.tilesX, ty = tile / P.tilesX;
+  const int ox = tx * 16, oy = ty * 16;
+  const int qx = lane & 7, qy = lane >> 3;
+  const int px = ox + 2 * qx, py = oy + 2 * qy;  // top-left pixel of this lane's quad
+
+  int inMask = 0;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    int x = px + (i & 1), y = py + (i >> 1);
+    if (x >= P.sx0 && x < P.sx1 && y >= P.sy0 && y < P.sy1) inMask |= 1 << i;
+  }
+
+  const uint32_t start = tileStart[tile], end = tileEnd[tile];
+  if (start >= end) return;  // nothing drawn in this tile: memory stays as is
+
+  TileState<MS, FCOL, FTEX> S;
+  // Load the tile's framebuffer (only pixels inside the scissor).
+#pragma unroll
+  for (int q = 0; q < MS; q++)
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      S.zb[q][i] = 0.0f;
+      S.sb[q][i] = 0;
+      S.cb[q][i] = 0;
+      if (FCOL) S.cf[FCOL ? q : 0][i] = make_float4(0, 0, 0, 0);
+      if (!(inMask & (1 << i))) continue;
+      int x = px + (i & 1), y = py + (i >> 1);
+      if (P.depthTest)
+        S.zb[q][i] = *reinterpret_cast<const float *>(P.depth + q * P.depthSlice + (long long)y * P.depthPitch + 4ll * x);
+      if (P.stencilActive) S.sb[q][i] = (uint8_t)P.stencil[q * P.stencilSlice + (long long)y * P.stencilPitch + x];
+      if (P.colorMask) {
+        if (FCOL)
+          S.cf[FCOL ? q : 0][i] =
+              *reinterpret_cast<const float4 *>(P.color + q * P.colorSlice + (long long)y * P.colorPitch + 16ll * x);
+        else
+          S.cb[q][i] = *reinterpret_cast<const uint32_t *>(P.color + q * P.colorSlice + (long long)y * P.colorPitch + 4ll * x);
+      }
+    }
+
+  unsigned long long occl = 0;
+  const TexParams &T = P.tex;
+
+  for (uint32_t e = start; e < end; e++) {
+    const uint32_t t = __builtin_amdgcn_readfirstlane(binTri[e]);
+    const TriRecord &R = rec[t];
+    const int n = R.n;
+    const int d = R.flags & 1;
+    const bool front = (R.flags & 2) != 0;
+
+    // ---- coverage (per sample) ----
+    int cMask[MS];
+    int anyCov = 0;
+#pragma unroll
+    for (int q = 0; q < MS; q++) {
+      const int xo = (MS > 1) ? kXf[q] : 0, yo = (MS > 1) ? kYf[q] : 0;
+      int L[2] = {-1, -1}, LX[2] = {0, 0}, Rr[2] = {0, 0}, RX[2] = {0, 0};
+      for (int i = 0; i < n; i++) {
+        int ia = i + 1 - d, ib = i + d;
+        ia = ia >= n ? ia - n : ia;
+        ib = ib >= n ? ib - n : ib;
+        int Xa, Ya, Xb, Yb;
+        if (n == 3) {
+          Xa = R.X[ia]; Ya = R.Y[ia]; Xb = R.X[ib]; Yb = R.Y[ib];
+        } else {
+          const int *pp = poly + (size_t)t * (2 * kMaxPolygon);
+          Xa = pp[ia]; Ya = pp[kMaxPolygon + ia]; Xb = pp[ib]; Yb = pp[kMaxPolygon + ib];
+        }
+        Edge E = make_edge(Xa - xo, Ya - yo, Xb - xo, Yb - yo, ox, oy);
+#pragma unroll
+        for (int r = 0; r < 2; r++) {
+          int y = py + r;
+          if (y >= E.y1 && y < E.y2) {
+            int v = E.e0 + E.ex * (px - ox) + E.ey * (y - oy);
+            if (E.right) { Rr[r] = v; RX[r] = E.ex; }
+            else { L[r] = v; LX[r] = E.ex; }
+          }
+        }
+      }
+      int m = 0;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        int r = i >> 1, dx = i & 1;
+        int lv = L[r] + (dx ? LX[r] : 0), rv = Rr[r] + (dx ? RX[r] : 0);
+        if (lv >= 0 && rv < 0) m |= 1 << i;
+      }
+      m &= inMask;
+      if (!((P.msMask >> q) & 1)) m = 0;
+      cMask[q] = m;
+      anyCov |= m;
+    }
+    if (!__any(anyCov)) continue;
+    if (!anyCov) continue;  // lanes without coverage have nothing to do for this primitive
+
+    const StencilParams &SP = front ? P.front : P.back;
+    int sMask[MS], zMask[MS];
+#pragma unroll
+    for (int q = 0; q < MS; q++) { sMask[q] = cMask[q]; zMask[q] = cMask[q]; }
+    if (P.stencilActive) {
+#pragma unroll
+      for (int q = 0; q < MS; q++) {
+        int m = 0;
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+          if ((cMask[q] >> i) & 1)
+            if (stencil_compare(SP, S.sb[q][i])) m |= 1 << i;
+        sMask[q] &= m;
+      }
+    }
+
+    // xFragment / yFragment of the four lanes (QuadRasterizer.cpp:126, PixelRoutine.cpp:125).
+    float xf[4], yf[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      xf[i] = ((float)px + (float)(i & 1)) - R.x0;
+      yf[i] = ((float)py + (float)(i >> 1)) - R.y0;
+    }
+
+    float zq[MS][4];
+    bool depthPass = false;
+    if (P.depthTest) {
+#pragma unroll
+      for (int q = 0; q < MS; q++) {
+        int m = 0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          float yy = (MS > 1) ? yf[i] + kSampleY[q] : yf[i];
+          float xx = (MS > 1) ? xf[i] - kSampleX[q] : xf[i];
+          float Dz = R.zC + yy * R.zB;
+          float z = xx * R.zA + Dz;
+          z = rr_min(rr_max(z, 0.0f), 1.0f);
+          zq[q][i] = z;
+          float zb = S.zb[q][i];
+          bool pass;
+          switch (P.depthCompare) {
+            case 0: pass = false; break;
+            case 1: pass = !(zb <= z); break;
+            case 2: pass = (zb == z); break;
+            case 3: pass = !(zb < z); break;
+            case 4: pass = (zb < z); break;
+            case 5: pass = !(zb == z); break;
+            case 6: pass = (zb <= z); break;
+            default: pass = true; break;
+          }
+          if (pass) m |= 1 << i;
+        }
+        zMask[q] = m & cMask[q];
+        if (P.stencilActive) zMask[q] &= sMask[q];
+        if (zMask[q]) depthPass = true;
+      }
+    } else {
+      depthPass = true;
+    }
+
+    // ---- stencil write (PixelRoutine.cpp:754-817) ----
+    if (P.stencilWrite) {
+#pragma unroll
+      for (int q = 0; q < MS; q++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          if (!((cMask[q] >> i) & 1)) continue;
+          int op;
+          if (!((sMask[q] >> i) & 1)) op = SP.failOp;
+          else if (P.depthTest && !((zMask[q] >> i) & 1)) op = SP.depthFailOp;
+          else op = SP.passOp;
+          uint32_t bv = S.sb[q][i];
+          uint32_t nv = stencil_apply_op(bv, op, SP.reference);
+          if (SP.useWriteMask) nv = (nv & SP.writeMask) | (bv & ~SP.writeMask);
+          S.sb[q][i] = nv & 0xFFu;
+        }
+    }
+
+    if (!depthPass) continue;
+
+    // ---- depth write + occlusion ----
+#pragma unroll
+    for (int q = 0; q < MS; q++) {
+      if (P.depthWrite) {
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+          if ((zMask[q] >> i) & 1) S.zb[q][i] = zq[q][i];
+      }
+      occl += __builtin_popcount((unsigned)(zMask[q] & sMask[q]));
+    }
+
+    if (!P.colorMask) continue;
+    int xMask[MS];
+    int shadeMask = 0;
+#pragma unroll
+    for (int q = 0; q < MS; q++) {
+      xMask[q] = P.depthTest ? zMask[q] : cMask[q];
+      if (P.stencilActive) xMask[q] &= sMask[q];
+      shadeMask |= xMask[q];
+    }
+    if (!shadeMask) continue;
+
+    // ---- fragment shader (late tests: results only matter where written) ----
+    float rhw[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      float Dw = R.wC + yf[i] * R.wB;
+      float w = xf[i] * R.wA + Dw;
+      rhw[i] = 1.0f / w;
+    }
+    float4 col[4];
+    float u[4], v[4];
+    if (P.useUV) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        u[i] = (xf[i] * R.V[4][0] + (R.V[4][2] + yf[i] * R.V[4][1])) * rhw[i];
+        v[i] = (xf[i] * R.V[5][0] + (R.V[5][2] + yf[i] * R.V[5][1])) * rhw[i];
+      }
+    }
+    if (P.shader == SHADER_INTERPOLATE) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        if (!((shadeMask >> i) & 1)) continue;
+        col[i].x = (xf[i] * R.V[0][0] + (R.V[0][2] + yf[i] * R.V[0][1])) * rhw[i];
+        col[i].y = (xf[i] * R.V[1][0] + (R.V[1][2] + yf[i] * R.V[1][1])) * rhw[i];
+        col[i].z = (xf[i] * R.V[2][0] + (R.V[2][2] + yf[i] * R.V[2][1])) * rhw[i];
+        col[i].w = (xf[i] * R.V[3][0] + (R.V[3][2] + yf[i] * R.V[3][1])) * rhw[i];
+      }
+    } else {
+      QuadLod Lq = quad_lod(T, u, v);
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        if (!((shadeMask >> i) & 1)) continue;
+        float4 c = sample_lane(T, u[i], v[i], Lq);
+        if (P.shader == SHADER_MODULATE) {
+          c.x = c.x * ((xf[i] * R.V[0][0] + (R.V[0][2] + yf[i] * R.V[0][1])) * rhw[i]);
+          c.y = c.y * ((xf[i] * R.V[1][0] + (R.V[1][2] + yf[i] * R.V[1][1])) * rhw[i]);
+          c.z = c.z * ((xf[i] * R.V[2][0] + (R.V[2][2] + yf[i] * R.V[2][1])) * rhw[i]);
+          c.w = c.w * ((xf[i] * R.V[3][0] + (R.V[3][2] + yf[i] * R.V[3][1])) * rhw[i]);
+        }
+        col[i] = c;
+      }
+    }
+
+    // ---- clamp, blend, write (PixelProgram.cpp:261-364, PixelRoutine.cpp:1617-2936) ----
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      if (!((shadeMask >> i) & 1)) continue;
+      float4 s = col[i];
+      if (!FCOL) {
+        s.x = rr_min(rr_max(s.x, 0.0f), 1.0f);
+        s.y = rr_min(rr_max(s.y, 0.0f), 1.0f);
+        s.z = rr_min(rr_max(s.z, 0.0f), 1.0f);
+        s.w = rr_min(rr_max(s.w, 0.0f), 1.0f);
+      }
+#pragma unroll
+      for (int q = 0; q < MS; q++) {
+        if (!((xMask[q] >> i) & 1)) continue;
+        float4 dst;
+        if (FCOL) {
+          dst = S.cf[FCOL ? q : 0][i];
+        } else {
+          uint32_t p = S.cb[q][i];
+          dst = make_float4(unorm8_to_float(p & 0xFFu), unorm8_to_float((p >> 8) & 0xFFu),
+                            unorm8_to_float((p >> 16) & 0xFFu), unorm8_to_float(p >> 24));
+        }
+        float4 o = P.blendEnable ? alpha_blend(P, s, dst) : s;
+        if (FCOL) {
+          float4 &c = S.cf[FCOL ? q : 0][i];
+          if (P.colorMask & 1) c.x = o.x;
+          if (P.colorMask & 2) c.y = o.y;
+          if (P.colorMask & 4) c.z = o.z;
+          if (P.colorMask & 8) c.w = o.w;
+        } else {
+          uint32_t p = S.cb[q][i];
+          uint32_t r = float_to_unorm8(o.x), g = float_to_unorm8(o.y), b = float_to_unorm8(o.z), a = float_to_unorm8(o.w);
+          if (P.colorMask & 1) p = (p & 0xFFFFFF00u) | r;
+          if (P.colorMask & 2) p = (p & 0xFFFF00FFu) | (g << 8);
+          if (P.colorMask & 4) p = (p & 0xFF00FFFFu) | (b << 16);
+          if (P.colorMask & 8) p = (p & 0x00FFFFFFu) | (a << 24);
+          S.cb[q][i] = p;
+        }
+      }
+    }
+  }
+
+  // Write the tile back.
+#pragma unroll
+  for (int q = 0; q < MS; q++)
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      if (!(inMask & (1 << i))) continue;
+      int x = px + (i & 1), y = py + (i >> 1);
+      if (P.depthWrite)
+        *reinterpret_cast<float *>(P.depth + q * P.depthSlice + (long long)y * P.depthPitch + 4ll * x) = S.zb[q][i];
+      if (P.stencilWrite) P.stencil[q * P.stencilSlice + (long long)y * P.stencilPitch + x] = (char)S.sb[q][i];
+      if (P.colorMask) {
+        if (FCOL)
+          *reinterpret_cast<float4 *>(P.color + q * P.colorSlice + (long long)y * P.colorPitch + 16ll * x) =
+              S.cf[FCOL ? q : 0][i];
+        else
+          *reinterpret_cast<uint32_t *>(P.color + q * P.colorSlice + (long long)y * P.colorPitch + 4ll * x) = S.cb[q][i];
+      }
+    }
+
+  if (P.occlusion) {
+    // Wave reduction of the per-lane sample counts, then one atomic.
+    unsigned long long v = occl;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0 && v) atomicAdd(P.occlusion, v);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// MSAA resolve (Blitter.cpp:2076-2200 for RGBA8, :1524-1545 for float)
+// ---------------------------------------------------------------------------
+__global__ void resolve_rgba8_kernel(const char *src, int srcPitch, long long slice, char *dst, int dstPitch, int w,
+                                     int h) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= w || y >= h) return;
+  uint32_t s[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) s[q] = *reinterpret_cast<const uint32_t *>(src + q * slice + (long long)y * srcPitch + 4ll * x);
+  uint32_t o = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    uint32_t a = (s[0] >> (8 * k)) & 0xFF, b = (s[1] >> (8 * k)) & 0xFF, c = (s[2] >> (8 * k)) & 0xFF,
+             dd = (s[3] >> (8 * k)) & 0xFF;
+    uint32_t ab = (a + b + 1) >> 1, cd = (c + dd + 1) >> 1;
+    o |= ((ab + cd + 1) >> 1) << (8 * k);
+  }
+  *reinterpret_cast<uint32_t *>(dst + (long long)y * dstPitch + 4ll * x) = o;
+}
+
+__global__ void resolve_rgba32f_kernel(const char *src, int srcPitch, long long slice, char *dst, int dstPitch, int w,
+                                       int h) {
+  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= w || y >= h) return;
+  float4 s[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) s[q] = *reinterpret_cast<const float4 *>(src + q * slice + (long long)y * srcPitch + 16ll * x);
+  float4 o;
+  o.x = (((s[0].x + s[1].x) + s[2].x) + s[3].x) * 0.25f;
+  o.y = (((s[0].y + s[1].y) + s[2].y) + s[3].y) * 0.25f;
+  o.z = (((s[0].z + s[1].z) + s[2].z) + s[3].z) * 0.25f;
+  o.w = (((s[0].w + s[1].w) + s[2].w) + s[3].w) * 0.25f;
+  *reinterpret_cast<float4 *>(dst + (long long)y * dstPitch + 16ll * x) = o;
+}
+
+template __global__ void raster_kernel<1, false, false>(DrawParams, const TriRecord *, const int *, const uint32_t *,
+                                                        const uint32_t *, const uint32_t *);
+template __global__ void raster_kernel<1, false, true>(DrawParams, const TriRecord *, const int *, const uint32_t *,
+                                                       const uint32_t *, const uint32_t *);
+template __global__ void raster_kernel<1, true, false>(DrawParams, const TriRecord *, const int *, const uint32_t *,
+                                                       const uint32_t *, const uint32_t *);
+template __global__ void raster_kernel<1, true, true>(DrawParams, const TriRecord *, const int *, const uint32_t *,
+                                                      const uint32_t *, const uint32_t *);
+template __global__ void raster_kernel<4, false, false>(DrawParams, const TriRecord *, const int *, const uint32_t *,
+                                                        const uint32_t *, const uint32_t *);
+template __global__ void raster_kernel<4, false, true>(DrawParams, const TriRecord *, const int *, const uint32_t *,
+                                                       const uint32_t *, const uint32_t *);
+template __global__ void raster_kernel<4, true, false>(DrawParams, const TriRecord *, const int *, const uint32_t *,
+                                                       const uint32_t *, const uint32_t *);
+template __global__ void raster_kernel<4, true, true>(DrawParams, const TriRecord *, const int *, const uint32_t *,
+                                                      const uint32_t *, const uint32_t *);
+
+}  // namespace swr

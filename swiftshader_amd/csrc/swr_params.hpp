@@ -1,0 +1,104 @@
+// swr_params.hpp -- POD state shared by the host launcher and the kernels.
+//
+// DrawParams is the GPU form of what the reference keeps in DrawData and
+// the SetupProcessor / PixelProcessor states (src/Device/Renderer.hpp:58-113,
+// src/Device/SetupProcessor.hpp:38-63, src/Device/PixelProcessor.hpp:36-108),
+// derived once per draw on the host.  TriRecord is the per-primitive setup
+// output the tile rasteriser consumes (the reference's sw::Primitive,
+// src/Device/Primitive.hpp:30-70, without its O(height) span outline: spans
+// are re-derived per pixel from the edges).
+#pragma once
+
+#include <stdint.h>
+
+namespace swr {
+
+enum : int { FILT_POINT = 0, FILT_MIN_POINT_MAG_LINEAR, FILT_MIN_LINEAR_MAG_POINT, FILT_LINEAR, FILT_ANISOTROPIC };
+enum : int { MIP_NONE = 0, MIP_POINT, MIP_LINEAR };
+// Canonical blend operations beyond VkBlendOp (Context.cpp:1181-1271).
+enum : int { BLEND_OP_SRC = 1000148000, BLEND_OP_DST = 1000148001, BLEND_OP_ZERO = 1000148002 };
+enum : int { SHADER_INTERPOLATE = 0, SHADER_SAMPLE = 1, SHADER_MODULATE = 2 };
+
+constexpr int kMipLevels = 15;
+constexpr int kMaxPolygon = 16;
+
+struct TexParams {
+  const void *data[kMipLevels];
+  int width[kMipLevels];
+  int height[kMipLevels];
+  int pitchP[kMipLevels];
+  uint32_t uHalf[kMipLevels];
+  uint32_t vHalf[kMipLevels];
+  float wwhh[4];
+  int isFloat;
+  int filter, mipmap, addrU, addrV;  // addr: 0 wrap, 1 clamp
+  int skipLod;
+  float mipLodBias, maxAnisotropy, minLod, maxLod;
+};
+
+struct StencilParams {
+  int compareOp, failOp, passOp, depthFailOp;
+  int useCompareMask, useWriteMask;
+  uint32_t compareMask, writeMask, reference, refMasked;
+  int32_t refSigned;
+};
+
+struct DrawParams {
+  // Inputs.
+  const float *pos;
+  const float *col;
+  const float *uv;
+  const uint32_t *idx;
+  int triCount;
+  int vertexCount;
+  // Viewport / scissor (Renderer.cpp:300-345).
+  float WxF, HxF, X0xF, Y0xF, depthRange, depthNear;
+  int sx0, sx1, sy0, sy1;
+  int width, height;
+  // Rasterisation.
+  int cullMode, frontFace;
+  int ms, msMask;
+  // Depth / stencil.
+  int depthTest, depthWrite, depthCompare;
+  int stencilActive, stencilWrite;
+  StencilParams front, back;
+  // Colour.
+  int colorMask;   // canonical 4-bit write mask; 0 = no colour output
+  int colorFloat;  // RGBA32F target
+  int blendEnable;
+  int srcF, dstF, colorOp, srcFA, dstFA, alphaOp;
+  float blendConst[4], invBlendConst[4];
+  int shader, useColor, useUV;
+  TexParams tex;
+  // Attachments.
+  char *color;
+  int colorPitch;
+  long long colorSlice;
+  char *depth;
+  int depthPitch;
+  long long depthSlice;
+  char *stencil;
+  int stencilPitch;
+  long long stencilSlice;
+  unsigned long long *occlusion;
+  // Tiling.
+  int tileW, tileH, tilesX, tilesY;
+};
+
+// Per-primitive setup output.  n == 0 marks a culled primitive.
+struct TriRecord {
+  int n;            // polygon vertex count (3 unless clipped)
+  int flags;        // bit0: d (winding), bit1: front facing
+  int yMin, yMax;   // rows with possible coverage (scissor-clamped)
+  int X[3], Y[3];   // 28.4 polygon vertices when n == 3
+  float x0, y0;
+  float zA, zB, zC;
+  float wA, wB, wC;
+  float V[6][3];    // colour rgba, uv
+  int xMin, xMax;   // pixel columns with possible coverage
+  int pad[2];
+};
+
+static_assert(sizeof(TriRecord) == 160, "TriRecord layout");
+
+}  // namespace swr

@@ -1,0 +1,133 @@
+"""Parity scenes shared by the CPU and GPU tests (seeded, small sizes)."""
+import numpy as np
+
+from swiftshader_amd import abi, scenes
+from swiftshader_amd.scenes import DrawData, TextureData
+
+
+def _clip_heavy(seed=11, n=400, size=96):
+    """Triangles with vertices well outside the frustum (near/far/sides)."""
+    rng = np.random.default_rng(seed)
+    xy = rng.uniform(-2.5, 2.5, (n * 3, 2))
+    w = rng.uniform(-0.5, 3.0, (n * 3, 1))
+    z = rng.uniform(-0.8, 1.6, (n * 3, 1))
+    pos = np.concatenate([xy * np.abs(w), z * np.abs(w), w], axis=1).astype(np.float32)
+    col = rng.uniform(0, 1, (n * 3, 4)).astype(np.float32)
+    uv = rng.uniform(-1, 3, (n * 3, 2)).astype(np.float32)
+    tex = TextureData(format=abi.FORMAT_R8G8B8A8_UNORM,
+                      levels=scenes.box_mip_chain(scenes.noise_texture_rgba8(rng, 64)))
+    return DrawData(width=size, height=size - 17, positions=pos, colors=col, texcoords=uv,
+                    shader=abi.SHADER_MODULATE, texture=tex)
+
+
+def _stencil_scene(seed=12, size=80, samples=1):
+    rng = np.random.default_rng(seed)
+    pos, uv, col = scenes.screen_triangles(rng, 600, size, size, area_hi=800)
+    d = DrawData(width=size, height=size, positions=pos, colors=col, texcoords=uv, samples=samples,
+                 shader=abi.SHADER_INTERPOLATE, has_stencil=True, stencil_test=1, clear_stencil=3,
+                 depth_compare=abi.COMPARE_LESS_OR_EQUAL,
+                 front=dict(fail_op=abi.STENCIL_OP_INCREMENT_AND_WRAP, pass_op=abi.STENCIL_OP_INCREMENT_AND_CLAMP,
+                            depth_fail_op=abi.STENCIL_OP_INVERT, compare_op=abi.COMPARE_LESS_OR_EQUAL,
+                            compare_mask=0xFE, write_mask=0xF7, reference=2),
+                 back=dict(fail_op=abi.STENCIL_OP_DECREMENT_AND_WRAP, pass_op=abi.STENCIL_OP_REPLACE,
+                           depth_fail_op=abi.STENCIL_OP_ZERO, compare_op=abi.COMPARE_NOT_EQUAL,
+                           compare_mask=0xFF, write_mask=0xFF, reference=5))
+    return d
+
+
+def _blend_variety(seed=13, size=72):
+    rng = np.random.default_rng(seed)
+    pos, uv, col = scenes.screen_triangles(rng, 500, size, size + 9, area_hi=600)
+    col = (col * 1.4 - 0.2).astype(np.float32)  # out-of-range colours exercise the clamps
+    return DrawData(width=size, height=size + 9, positions=pos, colors=col, texcoords=uv,
+                    shader=abi.SHADER_INTERPOLATE, blend_enable=1,
+                    blend=(abi.BLEND_FACTOR_ONE_MINUS_DST_COLOR, abi.BLEND_FACTOR_SRC_ALPHA_SATURATE,
+                           abi.BLEND_OP_REVERSE_SUBTRACT, abi.BLEND_FACTOR_CONSTANT_ALPHA,
+                           abi.BLEND_FACTOR_ONE_MINUS_CONSTANT_COLOR, abi.BLEND_OP_ADD),
+                    blend_constants=(0.25, 1.5, -0.5, 0.75), color_write_mask=0xB,
+                    clear_color=(0.2, 0.4, 0.6, 0.8), depth_compare=abi.COMPARE_GREATER_OR_EQUAL,
+                    clear_depth=0.0)
+
+
+def _float_target(seed=14, size=96, tris=1500, samples=1, aniso=False, address=abi.ADDRESS_REPEAT):
+    rng = np.random.default_rng(seed)
+    tex = TextureData(format=abi.FORMAT_R32G32B32A32_SFLOAT, levels=scenes.float_texture(rng, 128),
+                      anisotropy_enable=1 if aniso else 0, max_anisotropy=8.0 if aniso else 1.0,
+                      address_u=address, address_v=address)
+    pos, uv, col = scenes.screen_triangles(rng, tris, size, size - 8, area_hi=2000)
+    return DrawData(width=size, height=size - 8, positions=pos, colors=col, texcoords=uv, samples=samples,
+                    shader=abi.SHADER_MODULATE, texture=tex, color_format=abi.FORMAT_R32G32B32A32_SFLOAT,
+                    blend_enable=1, blend=(abi.BLEND_FACTOR_SRC_ALPHA, abi.BLEND_FACTOR_ONE_MINUS_SRC_ALPHA,
+                                           abi.BLEND_OP_ADD, abi.BLEND_FACTOR_ONE, abi.BLEND_FACTOR_ONE,
+                                           abi.BLEND_OP_MAX))
+
+
+def _rgba8_texture_case(seed=15, size=(160, 90), tris=3000, tex=128, address=abi.ADDRESS_REPEAT, samples=1,
+                        aniso=False, mag=abi.FILTER_LINEAR, minf=abi.FILTER_LINEAR, mip=abi.MIPMAP_MODE_LINEAR,
+                        npot=None, shader=abi.SHADER_MODULATE, bias=0.0):
+    rng = np.random.default_rng(seed)
+    w, h = size
+    tw = npot[0] if npot else tex
+    th = npot[1] if npot else tex
+    t = TextureData(format=abi.FORMAT_R8G8B8A8_UNORM, levels=scenes.box_mip_chain(scenes.noise_texture_rgba8(rng, tw, th)),
+                    address_u=address, address_v=address, anisotropy_enable=1 if aniso else 0,
+                    max_anisotropy=8.0 if aniso else 1.0, mag_filter=mag, min_filter=minf, mipmap_mode=mip,
+                    mip_lod_bias=bias)
+    pos, uv, col = scenes.screen_triangles(rng, tris, w, h, area_hi=3000)
+    return DrawData(width=w, height=h, positions=pos, colors=col, texcoords=uv, samples=samples, shader=shader,
+                    texture=t, blend_enable=1,
+                    blend=(abi.BLEND_FACTOR_SRC_ALPHA, abi.BLEND_FACTOR_ONE_MINUS_SRC_ALPHA, abi.BLEND_OP_ADD,
+                           abi.BLEND_FACTOR_SRC_ALPHA, abi.BLEND_FACTOR_ONE_MINUS_SRC_ALPHA, abi.BLEND_OP_ADD))
+
+
+def _scissor_cull(seed=16):
+    rng = np.random.default_rng(seed)
+    pos, uv, col = scenes.screen_triangles(rng, 800, 101, 77, area_hi=1500)
+    return DrawData(width=101, height=77, positions=pos, colors=col, shader=abi.SHADER_INTERPOLATE,
+                    cull_mode=abi.CULL_BACK, front_face=abi.FRONT_FACE_CLOCKWISE, scissor=(7, 5, 83, 61),
+                    viewport=(-3.0, 2.5, 107.0, 70.0, 0.1, 0.9))
+
+
+def _indexed(seed=17, size=64):
+    """A tessellated grid (shared edges) drawn with an index buffer."""
+    rng = np.random.default_rng(seed)
+    g = 9
+    ys, xs = np.meshgrid(np.linspace(-1.1, 1.05, g), np.linspace(-1.02, 1.1, g), indexing="ij")
+    jitter = rng.uniform(-0.04, 0.04, (g, g, 2))
+    x = xs + jitter[..., 0]
+    y = ys + jitter[..., 1]
+    z = rng.uniform(0.1, 0.9, (g, g))
+    w = rng.uniform(0.7, 1.6, (g, g))
+    pos = np.stack([x * w, y * w, z * w, w], -1).reshape(-1, 4).astype(np.float32)
+    col = rng.uniform(0, 1, (g * g, 4)).astype(np.float32)
+    idx = []
+    for i in range(g - 1):
+        for j in range(g - 1):
+            a, b, c, d = i * g + j, i * g + j + 1, (i + 1) * g + j, (i + 1) * g + j + 1
+            idx += [a, b, c, b, d, c]
+    return DrawData(width=size, height=size, positions=pos, colors=col, indices=np.asarray(idx, np.uint32),
+                    shader=abi.SHADER_INTERPOLATE, depth_compare=abi.COMPARE_ALWAYS)
+
+
+def parity_cases():
+    """(name, DrawData) pairs; every one must match the oracle bit for bit."""
+    return [
+        ("plain_gouraud_256", scenes.plain_gouraud()),
+        ("textured_small", scenes.textured_1080p(triangles=3000, width=192, height=108, tex_size=128)),
+        ("rgba8_clamp_npot", _rgba8_texture_case(address=abi.ADDRESS_CLAMP_TO_EDGE, npot=(100, 37), seed=21)),
+        ("rgba8_nearest_mips", _rgba8_texture_case(mag=abi.FILTER_NEAREST, minf=abi.FILTER_NEAREST,
+                                                  mip=abi.MIPMAP_MODE_NEAREST, seed=22, shader=abi.SHADER_SAMPLE)),
+        ("rgba8_mixed_filters", _rgba8_texture_case(mag=abi.FILTER_NEAREST, minf=abi.FILTER_LINEAR, seed=23,
+                                                   bias=0.7)),
+        ("rgba8_aniso", _rgba8_texture_case(aniso=True, seed=24)),
+        ("rgba8_msaa4_aniso", _rgba8_texture_case(aniso=True, samples=4, seed=25, size=(70, 50), tris=900)),
+        ("float_target", _float_target()),
+        ("float_msaa4_aniso", _float_target(samples=4, aniso=True, seed=26, size=64, tris=700)),
+        ("float_clamp_aniso", _float_target(aniso=True, seed=27, address=abi.ADDRESS_CLAMP_TO_EDGE)),
+        ("clip_heavy", _clip_heavy()),
+        ("stencil_ops", _stencil_scene()),
+        ("stencil_msaa4", _stencil_scene(seed=28, samples=4)),
+        ("blend_variety", _blend_variety()),
+        ("scissor_cull_viewport", _scissor_cull()),
+        ("indexed_grid", _indexed()),
+    ]

@@ -1,0 +1,133 @@
+"""Runs the CPU oracle (oracle/libswr_oracle.so) on a DrawData.
+
+Test infrastructure: the oracle is the checker, never the product.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from swiftshader_amd import abi
+from swiftshader_amd.desc import Buffers, build_desc, color_bytes_per_pixel
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "libswr_oracle.so")
+REF_SO = os.path.join(ORACLE_DIR, "_ref", "libswr_ref.so")
+
+_oracle = None
+_ref = None
+
+
+def build_oracle():
+    src = os.path.join(ORACLE_DIR, "swr_oracle.c")
+    if not os.path.exists(ORACLE_SO) or os.path.getmtime(ORACLE_SO) < os.path.getmtime(src):
+        subprocess.run(["make", "-s", "-C", ORACLE_DIR, "libswr_oracle.so"], check=True)
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        build_oracle()
+        L = ctypes.CDLL(ORACLE_SO)
+        L.swo_draw.argtypes = [ctypes.POINTER(abi.DrawDesc)]
+        L.swo_draw.restype = ctypes.c_int
+        L.swo_resolve.argtypes = [ctypes.POINTER(abi.ResolveDesc)]
+        L.swo_resolve.restype = ctypes.c_int
+        for f in ("swo_round_int", "swo_round_int_clamped", "swo_trunc_int"):
+            getattr(L, f).argtypes = [ctypes.c_float]
+            getattr(L, f).restype = ctypes.c_int32
+        for f in ("swo_mulhigh_u16x2", "swo_mulhigh_s16x2"):
+            getattr(L, f).argtypes = [ctypes.c_uint32, ctypes.c_uint32]
+            getattr(L, f).restype = ctypes.c_uint32
+        for f in ("swo_log2sqrt", "swo_frac"):
+            getattr(L, f).argtypes = [ctypes.c_float]
+            getattr(L, f).restype = ctypes.c_float
+        fp = ctypes.POINTER(ctypes.c_float)
+        L.swo_clip_triangle.argtypes = [fp, fp, fp, ctypes.c_int, fp, ctypes.POINTER(ctypes.c_int)]
+        L.swo_clip_triangle.restype = ctypes.c_int
+        _oracle = L
+    return _oracle
+
+
+def reference_lib():
+    """oracle/_ref/libswr_ref.so (reference sources compiled as-is), or None."""
+    global _ref
+    if _ref is None and os.path.exists(REF_SO):
+        L = ctypes.CDLL(REF_SO)
+        fp = ctypes.POINTER(ctypes.c_float)
+        L.ref_clip_triangle.argtypes = [fp, fp, fp, ctypes.c_int, fp, ctypes.POINTER(ctypes.c_int)]
+        L.ref_clip_triangle.restype = ctypes.c_int
+        _ref = L
+    return _ref
+
+
+def host_targets(draw):
+    S, H, W = draw.samples, draw.height, draw.width
+    if draw.color_format == abi.FORMAT_R32G32B32A32_SFLOAT:
+        color = np.empty((S, H, W, 4), np.float32)
+        color[:] = np.asarray(draw.clear_color, np.float32)
+    else:
+        color = np.empty((S, H, W, 4), np.uint8)
+        cc = [int(np.clip(np.rint(np.float32(v) * np.float32(255.0)), 0, 255)) for v in draw.clear_color]
+        color[:] = np.asarray(cc, np.uint8)
+    depth = np.full((S, H, W), np.float32(draw.clear_depth), np.float32) if draw.has_depth else None
+    stencil = np.full((S, H, W), draw.clear_stencil, np.uint8) if draw.has_stencil else None
+    return color, depth, stencil
+
+
+def oracle_render(draw, targets=None):
+    """Renders `draw` with the oracle; returns (color, depth, stencil, occlusion)."""
+    L = oracle()
+    color, depth, stencil = targets if targets is not None else host_targets(draw)
+    keep = []
+
+    def arr(a, dt):
+        a = np.ascontiguousarray(a, dtype=dt)
+        keep.append(a)
+        return a.ctypes.data
+
+    buf = Buffers(positions=arr(draw.positions, np.float32))
+    if draw.colors is not None:
+        buf.colors = arr(draw.colors, np.float32)
+    if draw.texcoords is not None:
+        buf.texcoords = arr(draw.texcoords, np.float32)
+    if draw.indices is not None:
+        buf.indices = arr(draw.indices, np.uint32)
+    if draw.texture is not None:
+        levels = []
+        for l in draw.texture.levels:
+            a = np.ascontiguousarray(l)
+            keep.append(a)
+            levels.append((a.ctypes.data, a.shape[1], a.shape[0], a.shape[1] * a.shape[2] * a.itemsize))
+        buf.texture_levels = levels
+    S, H, W = draw.samples, draw.height, draw.width
+    bpp = color_bytes_per_pixel(draw.color_format)
+    buf.color, buf.color_pitch, buf.color_sample_pitch = color.ctypes.data, W * bpp, H * W * bpp
+    if depth is not None:
+        buf.depth, buf.depth_pitch, buf.depth_sample_pitch = depth.ctypes.data, W * 4, H * W * 4
+    if stencil is not None:
+        buf.stencil, buf.stencil_pitch, buf.stencil_sample_pitch = stencil.ctypes.data, W, H * W
+    occ = np.zeros(1, np.uint64)
+    buf.occlusion = occ.ctypes.data
+    desc = build_desc(draw, buf)
+    rc = L.swo_draw(ctypes.byref(desc))
+    if rc != 0:
+        raise RuntimeError(f"oracle swo_draw failed: {rc}")
+    return color, depth, stencil, int(occ[0])
+
+
+def oracle_resolve(src):
+    """4x -> 1x resolve of a (4, H, W, 4) array with the oracle."""
+    L = oracle()
+    S, H, W, _ = src.shape
+    src = np.ascontiguousarray(src)
+    bpp = 16 if src.dtype == np.float32 else 4
+    fmt = abi.FORMAT_R32G32B32A32_SFLOAT if src.dtype == np.float32 else abi.FORMAT_R8G8B8A8_UNORM
+    dst = np.zeros((1, H, W, 4), src.dtype)
+    r = abi.ResolveDesc(W, H, fmt, src.ctypes.data, W * bpp, H * W * bpp, S, dst.ctypes.data, W * bpp)
+    rc = L.swo_resolve(ctypes.byref(r))
+    if rc != 0:
+        raise RuntimeError(f"oracle swo_resolve failed: {rc}")
+    return dst

@@ -1,0 +1,75 @@
+"""GPU parity: the HIP draw path against the CPU oracle, bit for bit.
+
+Every case renders the same seeded scene through libswr.so (C ABI) on the
+GPU and through the oracle on the CPU and requires identical colour,
+depth, stencil bytes and occlusion count (the tolerances north of exact
+that the brief allows -- depth 2^-22 relative, UNORM8 1 LSB -- are not
+needed: the kernels reproduce the reference's arithmetic exactly).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from cases import parity_cases
+from oracle_runner import oracle_render, oracle_resolve
+
+pytestmark = pytest.mark.gpu
+
+CASES = parity_cases()
+
+
+def gpu_render(draw):
+    import torch
+    from swiftshader_amd import runtime
+    dd = runtime.prepare(draw, "cuda:0")
+    runtime.draw(dd)
+    torch.cuda.synchronize()
+    t = dd.targets
+    color = t.color.cpu().numpy()
+    depth = t.depth.cpu().numpy() if t.depth is not None else None
+    stencil = t.stencil.cpu().numpy() if t.stencil is not None else None
+    return color, depth, stencil, int(t.occlusion.cpu().item())
+
+
+def _compare(name, ref, got):
+    rc, rz, rs, ro = ref
+    gc, gz, gs, go = got
+    if rc.dtype == np.float32:
+        bad = (rc.view(np.uint32) != gc.view(np.uint32))
+    else:
+        bad = rc != gc
+    assert not bad.any(), f"{name}: {int(bad.sum())} colour components differ; first at {np.argwhere(bad)[0]}"
+    if rz is not None:
+        badz = rz.view(np.uint32) != gz.view(np.uint32)
+        assert not badz.any(), f"{name}: {int(badz.sum())} depth values differ; first at {np.argwhere(badz)[0]}"
+    if rs is not None:
+        assert np.array_equal(rs, gs), f"{name}: stencil differs"
+    assert ro == go, f"{name}: occlusion {go} != oracle {ro}"
+
+
+@pytest.mark.parametrize("name,draw", CASES, ids=[c[0] for c in CASES])
+def test_draw_matches_oracle(name, draw):
+    print(f"[gpu parity] {name} {draw.width}x{draw.height} samples={draw.samples} tris={draw.triangle_count}",
+          flush=True)
+    sys.stdout.flush()
+    ref = oracle_render(draw)
+    got = gpu_render(draw)
+    _compare(name, ref, got)
+
+
+def test_resolve_matches_oracle():
+    import torch
+    from swiftshader_amd import runtime
+    rng = np.random.default_rng(5)
+    for dt in (np.uint8, np.float32):
+        src = (rng.integers(0, 256, (4, 33, 47, 4)).astype(np.uint8) if dt == np.uint8
+               else rng.normal(0.5, 0.3, (4, 33, 47, 4)).astype(np.float32))
+        print(f"[gpu parity] resolve {src.dtype}", flush=True)
+        ref = oracle_resolve(src)
+        s = torch.from_numpy(src).cuda()
+        d = torch.zeros((1, 33, 47, 4), dtype=s.dtype, device="cuda")
+        runtime.resolve(s, d)
+        torch.cuda.synchronize()
+        got = d.cpu().numpy()
+        assert np.array_equal(ref.view(np.uint8), got.view(np.uint8))
