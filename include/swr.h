@@ -261,6 +261,25 @@ int swr_draw_batch(const swr_draw_desc *descs, uint32_t count, void *stream);
 /* MSAA colour resolve (sw::Blitter::resolve, src/Device/Blitter.cpp:2035). */
 int swr_resolve(const swr_resolve_desc *desc, void *stream);
 
+/* Per-stage GPU time of the draws enqueued while profiling was enabled,
+ * measured with HIP events recorded on the caller's stream (the analogue
+ * of the reference's timestamp queries, src/Vulkan/VkQueryPool.cpp:217). */
+typedef struct swr_profile {
+  double setup_ms;   /* vertex tail, clipping, setup, tile counts      */
+  double bin_ms;     /* prefix sum, pair emission, sort, tile ranges   */
+  double raster_ms;  /* tile rasteriser (coverage .. colour write)     */
+  double total_ms;   /* first to last event of each draw, summed       */
+  uint64_t draws;    /* draws measured                                 */
+  uint64_t pairs;    /* (tile, triangle) pairs binned                  */
+} swr_profile;
+
+/* Enables (1) or disables (0) event recording on the current device. */
+int swr_profile_enable(int enable);
+
+/* Waits for the recorded events, returns the sums since the last read
+ * and resets them. */
+int swr_profile_read(swr_profile *out);
+
 /* Sets / reads the rasteriser tuning used by later draws on this device. */
 int swr_set_tuning(const swr_tuning *tuning);
 int swr_get_tuning(swr_tuning *tuning);

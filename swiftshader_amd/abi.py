@@ -118,10 +118,16 @@ class Tuning(C.Structure):
     _fields_ = [("tile_width", C.c_uint32), ("tile_height", C.c_uint32), ("reserved", C.c_uint32 * 6)]
 
 
+class Profile(C.Structure):
+    _fields_ = [("setup_ms", C.c_double), ("bin_ms", C.c_double), ("raster_ms", C.c_double),
+                ("total_ms", C.c_double), ("draws", C.c_uint64), ("pairs", C.c_uint64)]
+
+
 # Symbols declared by include/swr.h, in declaration order.
 EXPORTED_SYMBOLS = (
     "swr_abi_version", "swr_last_error", "swr_init", "swr_release", "swr_draw",
-    "swr_draw_batch", "swr_resolve", "swr_set_tuning", "swr_get_tuning",
+    "swr_draw_batch", "swr_resolve", "swr_profile_enable", "swr_profile_read",
+    "swr_set_tuning", "swr_get_tuning",
 )
 
 
@@ -138,6 +144,10 @@ def bind(lib):
     lib.swr_draw_batch.restype = C.c_int
     lib.swr_resolve.argtypes = [C.POINTER(ResolveDesc), C.c_void_p]
     lib.swr_resolve.restype = C.c_int
+    lib.swr_profile_enable.argtypes = [C.c_int]
+    lib.swr_profile_enable.restype = C.c_int
+    lib.swr_profile_read.argtypes = [C.POINTER(Profile)]
+    lib.swr_profile_read.restype = C.c_int
     lib.swr_set_tuning.argtypes = [C.POINTER(Tuning)]
     lib.swr_set_tuning.restype = C.c_int
     lib.swr_get_tuning.argtypes = [C.POINTER(Tuning)]

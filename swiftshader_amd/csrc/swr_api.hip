@@ -16,6 +16,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/swr.h"
 #include "swr_kernels.hip"
@@ -68,12 +69,35 @@ struct Scratch {
   template <typename T> T *as() const { return reinterpret_cast<T *>(ptr); }
 };
 
+// Events of one profiled draw: setup | bin | raster boundaries.
+enum { EV_START, EV_SETUP, EV_BIN, EV_RASTER, EV_COUNT };
+struct EventSet {
+  hipEvent_t ev[EV_COUNT];
+  uint64_t pairs;
+};
+
 struct DeviceCtx {
   int device = -1;
   Scratch rec, poly, cnt, off, keysA, keysB, valsA, valsB, scanTmp, sortTmp, tStart, tEnd;
   uint32_t *hostTotal = nullptr;  // pinned
   swr_tuning tuning{};
+  bool profiling = false;
+  std::vector<EventSet> pending, pool;
+  swr_profile acc{};
 };
+
+hipError_t take_events(DeviceCtx *c, EventSet &es) {
+  if (!c->pool.empty()) {
+    es = c->pool.back();
+    c->pool.pop_back();
+    return hipSuccess;
+  }
+  for (int i = 0; i < EV_COUNT; i++) {
+    hipError_t e = hipEventCreate(&es.ev[i]);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
 
 std::mutex g_mutex;
 DeviceCtx *g_ctx[64] = {};
@@ -380,16 +404,26 @@ int draw_one(DeviceCtx *c, const swr_draw_desc *desc, hipStream_t s) {
   SWR_HIP_CHECK(c->scanTmp.ensure(scanBytes));
 
   const int threads = 256;
+  EventSet es{};
+  const bool prof = c->profiling;
+  if (prof) {
+    SWR_HIP_CHECK(take_events(c, es));
+    SWR_HIP_CHECK(hipEventRecord(es.ev[EV_START], s));
+  }
   hipLaunchKernelGGL(setup_kernel, dim3((T + threads - 1) / threads), dim3(threads), 0, s, P, c->rec.as<TriRecord>(),
                      c->poly.as<int>(), c->cnt.as<uint32_t>());
   hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(1), 0, s, c->cnt.as<uint32_t>() + T, 0u, 1u);
   SWR_HIP_CHECK(hipGetLastError());
+  if (prof) SWR_HIP_CHECK(hipEventRecord(es.ev[EV_SETUP], s));
   SWR_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(c->scanTmp.ptr, scanBytes, c->cnt.as<uint32_t>(),
                                                  c->off.as<uint32_t>(), (int)(T + 1), s));
   SWR_HIP_CHECK(hipMemcpyAsync(c->hostTotal, c->off.as<uint32_t>() + T, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   SWR_HIP_CHECK(hipStreamSynchronize(s));
   const uint32_t pairs = *c->hostTotal;
-  if (pairs == 0) return SWR_OK;
+  if (pairs == 0) {
+    if (prof) c->pool.push_back(es);
+    return SWR_OK;
+  }
 
   SWR_HIP_CHECK(c->keysA.ensure(sizeof(uint32_t) * (size_t)pairs));
   SWR_HIP_CHECK(c->keysB.ensure(sizeof(uint32_t) * (size_t)pairs));
@@ -416,6 +450,7 @@ int draw_one(DeviceCtx *c, const swr_draw_desc *desc, hipStream_t s) {
   hipLaunchKernelGGL(tile_range_kernel, dim3((pairs + 255) / 256), dim3(256), 0, s, c->keysB.as<uint32_t>(), pairs,
                      c->tStart.as<uint32_t>(), c->tEnd.as<uint32_t>());
   SWR_HIP_CHECK(hipGetLastError());
+  if (prof) SWR_HIP_CHECK(hipEventRecord(es.ev[EV_BIN], s));
 
   dim3 grid((tiles + 3) / 4), block(256);
   const TriRecord *rec = c->rec.as<TriRecord>();
@@ -436,6 +471,11 @@ int draw_one(DeviceCtx *c, const swr_draw_desc *desc, hipStream_t s) {
     else launch_raster<4, true, true>(P, grid, block, s, rec, poly, bin, ts, te);
   }
   SWR_HIP_CHECK(hipGetLastError());
+  if (prof) {
+    SWR_HIP_CHECK(hipEventRecord(es.ev[EV_RASTER], s));
+    es.pairs = pairs;
+    c->pending.push_back(es);
+  }
   return SWR_OK;
 }
 
@@ -507,6 +547,40 @@ int swr_resolve(const swr_resolve_desc *d, void *stream) {
                        (long long)d->src_sample_pitch_bytes, (char *)d->dst, (int)d->dst_row_pitch_bytes,
                        (int)d->width, (int)d->height);
   SWR_HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}
+
+int swr_profile_enable(int enable) {
+  DeviceCtx *c = nullptr;
+  int r = current_ctx(&c);
+  if (r != SWR_OK) return r;
+  c->profiling = enable != 0;
+  return SWR_OK;
+}
+
+int swr_profile_read(swr_profile *out) {
+  if (!out) return fail(SWR_ERROR_INVALID_ARGUMENT, "profile is NULL");
+  DeviceCtx *c = nullptr;
+  int r = current_ctx(&c);
+  if (r != SWR_OK) return r;
+  for (EventSet &es : c->pending) {
+    SWR_HIP_CHECK(hipEventSynchronize(es.ev[EV_RASTER]));
+    float a = 0, b = 0, d = 0, t = 0;
+    SWR_HIP_CHECK(hipEventElapsedTime(&a, es.ev[EV_START], es.ev[EV_SETUP]));
+    SWR_HIP_CHECK(hipEventElapsedTime(&b, es.ev[EV_SETUP], es.ev[EV_BIN]));
+    SWR_HIP_CHECK(hipEventElapsedTime(&d, es.ev[EV_BIN], es.ev[EV_RASTER]));
+    SWR_HIP_CHECK(hipEventElapsedTime(&t, es.ev[EV_START], es.ev[EV_RASTER]));
+    c->acc.setup_ms += a;
+    c->acc.bin_ms += b;
+    c->acc.raster_ms += d;
+    c->acc.total_ms += t;
+    c->acc.draws += 1;
+    c->acc.pairs += es.pairs;
+    c->pool.push_back(es);
+  }
+  c->pending.clear();
+  *out = c->acc;
+  c->acc = swr_profile{};
   return SWR_OK;
 }
 

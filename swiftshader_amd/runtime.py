@@ -152,6 +152,18 @@ def resolve(src: torch.Tensor, dst: torch.Tensor, stream=None):
     check(lib().swr_resolve(ctypes.byref(r), _stream_handle(stream)), "swr_resolve")
 
 
+def profile_enable(enable: bool = True):
+    check(lib().swr_profile_enable(1 if enable else 0), "swr_profile_enable")
+
+
+def profile_read() -> dict:
+    """Per-stage GPU milliseconds summed over the draws since the last read."""
+    p = abi.Profile()
+    check(lib().swr_profile_read(ctypes.byref(p)), "swr_profile_read")
+    return {"setup_ms": p.setup_ms, "bin_ms": p.bin_ms, "raster_ms": p.raster_ms, "total_ms": p.total_ms,
+            "draws": int(p.draws), "pairs": int(p.pairs)}
+
+
 def render(draw_data: DrawData, device="cuda:0", stream=None):
     """Convenience: upload, clear, draw, return the targets (device tensors)."""
     dd = prepare(draw_data, device)
