@@ -44,6 +44,7 @@ def oracle():
         for f in ("swo_log2sqrt", "swo_frac"):
             getattr(L, f).argtypes = [ctypes.c_float]
             getattr(L, f).restype = ctypes.c_float
+        L.swo_constants.restype = None
         fp = ctypes.POINTER(ctypes.c_float)
         L.swo_clip_triangle.argtypes = [fp, fp, fp, ctypes.c_int, fp, ctypes.POINTER(ctypes.c_int)]
         L.swo_clip_triangle.restype = ctypes.c_int
@@ -59,6 +60,7 @@ def reference_lib():
         fp = ctypes.POINTER(ctypes.c_float)
         L.ref_clip_triangle.argtypes = [fp, fp, fp, ctypes.c_int, fp, ctypes.POINTER(ctypes.c_int)]
         L.ref_clip_triangle.restype = ctypes.c_int
+        L.ref_constants.restype = None
         _ref = L
     return _ref
 
