@@ -452,7 +452,7 @@ int draw_one(DeviceCtx *c, const swr_draw_desc *desc, hipStream_t s) {
   SWR_HIP_CHECK(hipGetLastError());
   if (prof) SWR_HIP_CHECK(hipEventRecord(es.ev[EV_BIN], s));
 
-  dim3 grid((tiles + 3) / 4), block(256);
+  dim3 grid(tiles), block(256);  // one workgroup per bin, one wave per 8x8 block
   const TriRecord *rec = c->rec.as<TriRecord>();
   const int *poly = c->poly.as<int>();
   const uint32_t *bin = c->valsB.as<uint32_t>();

@@ -279,9 +279,11 @@ __device__ __forceinline__ QuadLod quad_lod(const TexParams &T, const float u[4]
   return L;
 }
 
-// One lane of sampleTexture128 after the LOD is known.
+// One lane of sampleTexture128 after the LOD is known (FTEX: RGBA32F
+// texture, 32-bit float path; otherwise RGBA8, 16-bit fixed-point path).
+template <bool FTEX>
 __device__ __forceinline__ float4 sample_lane(const TexParams &T, float u, float v, const QuadLod &L) {
-  if (T.isFloat) {
+  if (FTEX) {
     float4 c = sample_float_aniso(T, u, v, L.lod, L.aniso, L.uDelta, L.vDelta, 0);
     if (T.mipmap == MIP_LINEAR) {
       float4 cc = sample_float_aniso(T, u, v, L.lod, L.aniso, L.uDelta, L.vDelta, 1);

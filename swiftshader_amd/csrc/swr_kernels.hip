@@ -335,47 +335,102 @@ __global__ void tile_range_kernel(const uint32_t *__restrict__ keys, uint32_t n,
 // ---------------------------------------------------------------------------
 // Tile rasteriser
 // ---------------------------------------------------------------------------
-// An edge of the sample-shifted polygon, in the form the per-pixel test
-// needs.  The reference walks each edge with an exact integer DDA that
-// stores x = ceil(Xe(y) / 16) for rows [ceil(Y1/16), ceil(Y2/16)) into the
-// left or right span (SetupRoutine.cpp:550-621), and a lane is covered iff
-// left <= x < right (QuadRasterizer.cpp:181-206).  With
-//   E(x, y) = DY * (16 x - X1) - DX * (16 y - Y1),   DY > 0,
+// Work decomposition: one workgroup per 16x16 bin, one wavefront per 8x8
+// block of the bin, one pixel per lane (lanes 4k..4k+3 form a 2x2 quad).
+// Each wave walks the bin's triangle list in API order, 64 entries at a
+// time: every lane computes the exact 8x8 coverage mask of one triangle
+// (triangle-parallel), a ballot compacts the triangles that cover the
+// block, and the wave then applies them one by one (pixel-parallel) to the
+// block's depth / stencil / colour, which stay in registers throughout.
+//
+// Coverage is the reference's span rule evaluated per sample point.  The
+// reference walks each edge with an exact integer DDA that stores
+// x = ceil(Xe(y) / 16) for rows [ceil(Y1/16), ceil(Y2/16)) into the left or
+// right span, later edges overwriting earlier ones (SetupRoutine.cpp:
+// 550-621), and a sample is covered iff left <= x < right
+// (QuadRasterizer.cpp:181-206).  With
+//   E(x, y) = DY * (16 x - X1) - DX * (16 y - Y1),   DY = Y2 - Y1 > 0,
 // "ceil(Xe/16) <= x" is exactly E >= 0 and "x < ceil(Xe/16)" is exactly
-// E < 0, so the span test becomes two integer sign tests per pixel.
-struct Edge {
-  int e0;     // E at the tile origin, clamped to +-2^30 (sign exact over the tile)
-  int ex;     // dE per pixel step in x (16 DY)
-  int ey;     // dE per pixel step in y (-16 DX)
-  int y1, y2; // rows covered
-  int right;  // writes the right span (edge runs upwards)
+// E < 0, so the span test becomes two integer sign tests per sample.  E at
+// the block origin is formed in 64 bits and clamped to +-2^30: beyond that
+// its sign cannot change inside the block, and the 32-bit steps are exact.
+
+struct LaneEdge {
+  int e;      // E at the block origin (clamped)
+  int ex;     // dE / dx (16 DY > 0)
+  int ey;     // dE / dy (-16 DX)
+  int y1, y2; // rows the edge spans
+  int right;  // writes the right span
 };
 
-__device__ __forceinline__ Edge make_edge(int Xa, int Ya, int Xb, int Yb, int ox, int oy) {
-  Edge e;
-  bool swap = Yb < Ya;
-  int X1 = swap ? Xb : Xa, X2 = swap ? Xa : Xb;
-  int Y1 = swap ? Yb : Ya, Y2 = swap ? Ya : Yb;
-  int DX = X2 - X1, DY = Y2 - Y1;
-  e.y1 = (Y1 + 15) >> 4;
-  e.y2 = (Y2 + 15) >> 4;
-  long long E = (long long)DY * (long long)(16 * ox - X1) - (long long)DX * (long long)(16 * oy - Y1);
+__device__ __forceinline__ LaneEdge lane_edge(int Xa, int Ya, int Xb, int Yb, int ox, int oy) {
+  LaneEdge E;
+  const bool swap = Yb < Ya;
+  const int X1 = swap ? Xb : Xa, X2 = swap ? Xa : Xb;
+  const int Y1 = swap ? Yb : Ya, Y2 = swap ? Ya : Yb;
+  const int DX = X2 - X1, DY = Y2 - Y1;
+  E.y1 = (Y1 + 15) >> 4;  // a horizontal edge gets y1 == y2 and spans no row
+  E.y2 = (Y2 + 15) >> 4;
+  long long v = (long long)DY * (long long)(16 * ox - X1) - (long long)DX * (long long)(16 * oy - Y1);
   const long long lim = 1ll << 30;
-  e.e0 = (int)(E > lim ? lim : (E < -lim ? -lim : E));
-  e.ex = 16 * DY;
-  e.ey = -16 * DX;
-  e.right = swap ? 1 : 0;
-  if (Ya == Yb) { e.y1 = 0; e.y2 = 0; }  // horizontal edges write nothing
-  return e;
+  E.e = (int)(v > lim ? lim : (v < -lim ? -lim : v));
+  E.ex = 16 * DY;
+  E.ey = -16 * DX;
+  E.right = swap ? 1 : 0;
+  return E;
 }
 
-template <int MS, bool FCOL, bool FTEX>
-struct TileState {
-  float zb[MS][4];
-  uint32_t sb[MS][4];
-  uint32_t cb[MS][4];          // RGBA8 target
-  float4 cf[FCOL ? MS : 1][4]; // RGBA32F target
+// Coverage mask (bit r*8+c = pixel (ox+c, oy+r)) of polygon (X, Y, n) for
+// the sample at 28.4 offset (xo, yo), before the scissor mask is applied.
+template <typename VFetch>
+__device__ __forceinline__ uint64_t block_coverage(const VFetch &vtx, int n, int d, int xo, int yo, int ox, int oy) {
+  int Lv[8], Lx[8], Rv[8], Rx[8];
+#pragma unroll
+  for (int r = 0; r < 8; r++) { Lv[r] = -1; Lx[r] = 0; Rv[r] = 0; Rx[r] = 0; }
+  for (int i = 0; i < n; i++) {
+    int ia = i + 1 - d, ib = i + d;
+    ia = ia >= n ? ia - n : ia;
+    ib = ib >= n ? ib - n : ib;
+    int Xa, Ya, Xb, Yb;
+    vtx(ia, Xa, Ya);
+    vtx(ib, Xb, Yb);
+    const LaneEdge E = lane_edge(Xa - xo, Ya - yo, Xb - xo, Yb - yo, ox, oy);
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+      const int y = oy + r;
+      if (y >= E.y1 && y < E.y2) {
+        const int v = E.e + E.ey * r;
+        if (E.right) { Rv[r] = v; Rx[r] = E.ex; }
+        else { Lv[r] = v; Lx[r] = E.ex; }
+      }
+    }
+  }
+  uint64_t mask = 0;
+#pragma unroll
+  for (int r = 0; r < 8; r++) {
+    // Covered iff lv >= 0 && rv < 0, i.e. the sign bit of (rv & ~lv).
+    int lv = Lv[r] + 7 * Lx[r], rv = Rv[r] + 7 * Rx[r];
+    uint32_t bits = 0;
+#pragma unroll
+    for (int c = 7; c >= 0; c--) {
+      bits = (bits << 1) | ((uint32_t)(rv & ~lv) >> 31);
+      lv -= Lx[r];
+      rv -= Rx[r];
+    }
+    mask |= (uint64_t)bits << (8 * r);
+  }
+  return mask;
+}
+
+template <int MS, bool FCOL>
+struct PixelState {
+  float zb[MS];
+  uint32_t sb[MS];
+  uint32_t cb[FCOL ? 1 : MS];
+  float4 cf[FCOL ? MS : 1];
 };
+
+__device__ __forceinline__ float quad_bcast(float v, int src) { return __shfl(v, src, 64); }
 
 template <int MS, bool FCOL, bool FTEX>
 __global__ __launch_bounds__(256) void raster_kernel(DrawParams P, const TriRecord *__restrict__ rec,
@@ -384,140 +439,134 @@ __global__ __launch_bounds__(256) void raster_kernel(DrawParams P, const TriReco
                                                      const uint32_t *__restrict__ tileStart,
                                                      const uint32_t *__restrict__ tileEnd) {
   const int lane = threadIdx.x & 63;
-  const int tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (tile >= P.tilesX * P.tilesY) return;
-  const int tx = tile % P.tilesX, ty = tile / P.tilesX;
-  const int ox = tx * 16, oy = ty * 16;
-  const int qx = lane & 7, qy = lane >> 3;
-  const int px = ox + 2 * qx, py = oy + 2 * qy;  // top-left pixel of this lane's quad
+  const int wave = threadIdx.x >> 6;
+  const int bin = blockIdx.x;
+  const int bx = bin % P.tilesX, by = bin / P.tilesX;
+  const int ox = bx * 16 + 8 * (wave & 1), oy = by * 16 + 8 * (wave >> 1);
+  const int quad = lane >> 2, sub = lane & 3;
+  const int cx = 2 * (quad & 3) + (sub & 1), cy = 2 * (quad >> 2) + (sub >> 1);
+  const int px = ox + cx, py = oy + cy;
+  const int myBit = cy * 8 + cx;
+  const int quadBase = lane & ~3;
 
-  int inMask = 0;
+  const uint32_t start = tileStart[bin], end = tileEnd[bin];
+  if (start >= end) return;
+  // Blocks entirely outside the scissor have nothing to do.
+  if (ox >= P.sx1 || oy >= P.sy1 || ox + 8 <= P.sx0 || oy + 8 <= P.sy0) return;
+
+  // Scissor mask of the block (rows and columns inside [s*0, s*1)).
+  uint64_t scissorMask = 0;
+  {
+    uint32_t cols = 0;
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    int x = px + (i & 1), y = py + (i >> 1);
-    if (x >= P.sx0 && x < P.sx1 && y >= P.sy0 && y < P.sy1) inMask |= 1 << i;
+    for (int c = 0; c < 8; c++)
+      if (ox + c >= P.sx0 && ox + c < P.sx1) cols |= 1u << c;
+#pragma unroll
+    for (int r = 0; r < 8; r++)
+      if (oy + r >= P.sy0 && oy + r < P.sy1) scissorMask |= (uint64_t)cols << (8 * r);
   }
+  const bool inside = (scissorMask >> myBit) & 1;
 
-  const uint32_t start = tileStart[tile], end = tileEnd[tile];
-  if (start >= end) return;  // nothing drawn in this tile: memory stays as is
-
-  TileState<MS, FCOL, FTEX> S;
-  // Load the tile's framebuffer (only pixels inside the scissor).
+  PixelState<MS, FCOL> S;
 #pragma unroll
-  for (int q = 0; q < MS; q++)
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      S.zb[q][i] = 0.0f;
-      S.sb[q][i] = 0;
-      S.cb[q][i] = 0;
-      if (FCOL) S.cf[FCOL ? q : 0][i] = make_float4(0, 0, 0, 0);
-      if (!(inMask & (1 << i))) continue;
-      int x = px + (i & 1), y = py + (i >> 1);
-      if (P.depthTest)
-        S.zb[q][i] = *reinterpret_cast<const float *>(P.depth + q * P.depthSlice + (long long)y * P.depthPitch + 4ll * x);
-      if (P.stencilActive) S.sb[q][i] = (uint8_t)P.stencil[q * P.stencilSlice + (long long)y * P.stencilPitch + x];
-      if (P.colorMask) {
-        if (FCOL)
-          S.cf[FCOL ? q : 0][i] =
-              *reinterpret_cast<const float4 *>(P.color + q * P.colorSlice + (long long)y * P.colorPitch + 16ll * x);
-        else
-          S.cb[q][i] = *reinterpret_cast<const uint32_t *>(P.color + q * P.colorSlice + (long long)y * P.colorPitch + 4ll * x);
-      }
+  for (int q = 0; q < MS; q++) {
+    S.zb[q] = 0.0f;
+    S.sb[q] = 0;
+    if (!FCOL) S.cb[FCOL ? 0 : q] = 0;
+    if (FCOL) S.cf[FCOL ? q : 0] = make_float4(0, 0, 0, 0);
+    if (!inside) continue;
+    if (P.depthTest)
+      S.zb[q] = *reinterpret_cast<const float *>(P.depth + q * P.depthSlice + (long long)py * P.depthPitch + 4ll * px);
+    if (P.stencilActive) S.sb[q] = (uint8_t)P.stencil[q * P.stencilSlice + (long long)py * P.stencilPitch + px];
+    if (P.colorMask) {
+      if (FCOL)
+        S.cf[FCOL ? q : 0] =
+            *reinterpret_cast<const float4 *>(P.color + q * P.colorSlice + (long long)py * P.colorPitch + 16ll * px);
+      else
+        S.cb[FCOL ? 0 : q] =
+            *reinterpret_cast<const uint32_t *>(P.color + q * P.colorSlice + (long long)py * P.colorPitch + 4ll * px);
     }
+  }
 
   unsigned long long occl = 0;
   const TexParams &T = P.tex;
+  const int bx0 = ox, by0 = oy;
 
-  for (uint32_t e = start; e < end; e++) {
-    const uint32_t t = __builtin_amdgcn_readfirstlane(binTri[e]);
-    const TriRecord &R = rec[t];
-    const int n = R.n;
-    const int d = R.flags & 1;
-    const bool front = (R.flags & 2) != 0;
-
-    // ---- coverage (per sample) ----
-    int cMask[MS];
-    int anyCov = 0;
+  for (uint32_t base = start; base < end; base += 64) {
+    // ---- triangle-parallel coverage of this block ----
+    const uint32_t e = base + lane;
+    uint32_t tid = 0;
+    uint32_t mlo[MS], mhi[MS];
 #pragma unroll
-    for (int q = 0; q < MS; q++) {
-      const int xo = (MS > 1) ? kXf[q] : 0, yo = (MS > 1) ? kYf[q] : 0;
-      int L[2] = {-1, -1}, LX[2] = {0, 0}, Rr[2] = {0, 0}, RX[2] = {0, 0};
-      for (int i = 0; i < n; i++) {
-        int ia = i + 1 - d, ib = i + d;
-        ia = ia >= n ? ia - n : ia;
-        ib = ib >= n ? ib - n : ib;
-        int Xa, Ya, Xb, Yb;
-        if (n == 3) {
-          Xa = R.X[ia]; Ya = R.Y[ia]; Xb = R.X[ib]; Yb = R.Y[ib];
-        } else {
-          const int *pp = poly + (size_t)t * (2 * kMaxPolygon);
-          Xa = pp[ia]; Ya = pp[kMaxPolygon + ia]; Xb = pp[ib]; Yb = pp[kMaxPolygon + ib];
-        }
-        Edge E = make_edge(Xa - xo, Ya - yo, Xb - xo, Yb - yo, ox, oy);
-#pragma unroll
-        for (int r = 0; r < 2; r++) {
-          int y = py + r;
-          if (y >= E.y1 && y < E.y2) {
-            int v = E.e0 + E.ex * (px - ox) + E.ey * (y - oy);
-            if (E.right) { Rr[r] = v; RX[r] = E.ex; }
-            else { L[r] = v; LX[r] = E.ex; }
+    for (int q = 0; q < MS; q++) { mlo[q] = 0; mhi[q] = 0; }
+    if (e < end) {
+      tid = binTri[e];
+      const int4 h0 = reinterpret_cast<const int4 *>(rec + tid)[0];  // n, flags, yMin, yMax
+      const int4 h1 = reinterpret_cast<const int4 *>(rec + tid)[1];  // xMin, xMax, X0, X1
+      const int4 h2 = reinterpret_cast<const int4 *>(rec + tid)[2];  // X2, Y0, Y1, Y2
+      const int n = h0.x, d = h0.y & 1;
+      if (h0.z < by0 + 8 && h0.w > by0 && h1.x < bx0 + 8 && h1.y > bx0) {
+        const int *pp = poly + (size_t)tid * (2 * kMaxPolygon);
+        auto vtx = [&](int i, int &Xo, int &Yo) {
+          if (n == 3) {
+            Xo = i == 0 ? h1.z : (i == 1 ? h1.w : h2.x);
+            Yo = i == 0 ? h2.y : (i == 1 ? h2.z : h2.w);
+          } else {
+            Xo = pp[i];
+            Yo = pp[kMaxPolygon + i];
           }
+        };
+#pragma unroll
+        for (int q = 0; q < MS; q++) {
+          if (!((P.msMask >> q) & 1)) continue;
+          const int xo = (MS > 1) ? kXf[q] : 0, yo = (MS > 1) ? kYf[q] : 0;
+          uint64_t m = block_coverage(vtx, n, d, xo, yo, bx0, by0) & scissorMask;
+          mlo[q] = (uint32_t)m;
+          mhi[q] = (uint32_t)(m >> 32);
         }
       }
-      int m = 0;
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-        int r = i >> 1, dx = i & 1;
-        int lv = L[r] + (dx ? LX[r] : 0), rv = Rr[r] + (dx ? RX[r] : 0);
-        if (lv >= 0 && rv < 0) m |= 1 << i;
-      }
-      m &= inMask;
-      if (!((P.msMask >> q) & 1)) m = 0;
-      cMask[q] = m;
-      anyCov |= m;
     }
-    if (!__any(anyCov)) continue;
-    if (!anyCov) continue;  // lanes without coverage have nothing to do for this primitive
-
-    const StencilParams &SP = front ? P.front : P.back;
-    int sMask[MS], zMask[MS];
+    uint32_t anyBits = 0;
 #pragma unroll
-    for (int q = 0; q < MS; q++) { sMask[q] = cMask[q]; zMask[q] = cMask[q]; }
-    if (P.stencilActive) {
+    for (int q = 0; q < MS; q++) anyBits |= mlo[q] | mhi[q];
+    uint64_t todo = __ballot(anyBits != 0);
+
+    // ---- pixel-parallel application, in list order ----
+    while (todo) {
+      const int j = __builtin_ctzll(todo);
+      todo &= todo - 1;
+      const uint32_t t = __builtin_amdgcn_readlane(tid, j);
+      int cMask = 0;  // bit q = sample q of this lane's pixel is covered
 #pragma unroll
       for (int q = 0; q < MS; q++) {
-        int m = 0;
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-          if ((cMask[q] >> i) & 1)
-            if (stencil_compare(SP, S.sb[q][i])) m |= 1 << i;
-        sMask[q] &= m;
+        const uint32_t lo = __builtin_amdgcn_readlane(mlo[q], j), hi = __builtin_amdgcn_readlane(mhi[q], j);
+        const uint32_t word = myBit < 32 ? lo : hi;
+        cMask |= (int)((word >> (myBit & 31)) & 1u) << q;
       }
-    }
+      const TriRecord &R = rec[t];
+      const bool front = (R.flags & 2) != 0;
+      const StencilParams &SP = front ? P.front : P.back;
 
-    // xFragment / yFragment of the four lanes (QuadRasterizer.cpp:126, PixelRoutine.cpp:125).
-    float xf[4], yf[4];
+      int sMask = cMask, zMask = cMask;
+      if (cMask && P.stencilActive) {
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      xf[i] = ((float)px + (float)(i & 1)) - R.x0;
-      yf[i] = ((float)py + (float)(i >> 1)) - R.y0;
-    }
-
-    float zq[MS][4];
-    bool depthPass = false;
-    if (P.depthTest) {
+        for (int q = 0; q < MS; q++)
+          if (((cMask >> q) & 1) && !stencil_compare(SP, S.sb[q])) sMask &= ~(1 << q);
+      }
+      const float xf = (float)px - R.x0;  // xFragment (PixelRoutine.cpp:125)
+      const float yf = (float)py - R.y0;  // yFragment (QuadRasterizer.cpp:126)
+      float zq[MS];
+      if (P.depthTest) {
+        zMask = 0;
 #pragma unroll
-      for (int q = 0; q < MS; q++) {
-        int m = 0;
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-          float yy = (MS > 1) ? yf[i] + kSampleY[q] : yf[i];
-          float xx = (MS > 1) ? xf[i] - kSampleX[q] : xf[i];
-          float Dz = R.zC + yy * R.zB;
+        for (int q = 0; q < MS; q++) {
+          const float yy = (MS > 1) ? yf + kSampleY[q] : yf;
+          const float xx = (MS > 1) ? xf - kSampleX[q] : xf;
+          const float Dz = R.zC + yy * R.zB;
           float z = xx * R.zA + Dz;
           z = rr_min(rr_max(z, 0.0f), 1.0f);
-          zq[q][i] = z;
-          float zb = S.zb[q][i];
+          zq[q] = z;
+          const float zb = S.zb[q];
           bool pass;
           switch (P.depthCompare) {
             case 0: pass = false; break;
@@ -529,105 +578,77 @@ __global__ __launch_bounds__(256) void raster_kernel(DrawParams P, const TriReco
             case 6: pass = (zb <= z); break;
             default: pass = true; break;
           }
-          if (pass) m |= 1 << i;
+          if (pass) zMask |= 1 << q;
         }
-        zMask[q] = m & cMask[q];
-        if (P.stencilActive) zMask[q] &= sMask[q];
-        if (zMask[q]) depthPass = true;
+        zMask &= cMask;
+        if (P.stencilActive) zMask &= sMask;
       }
-    } else {
-      depthPass = true;
-    }
-
-    // ---- stencil write (PixelRoutine.cpp:754-817) ----
-    if (P.stencilWrite) {
+      if (P.stencilWrite && cMask) {
 #pragma unroll
-      for (int q = 0; q < MS; q++)
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-          if (!((cMask[q] >> i) & 1)) continue;
+        for (int q = 0; q < MS; q++) {
+          if (!((cMask >> q) & 1)) continue;
           int op;
-          if (!((sMask[q] >> i) & 1)) op = SP.failOp;
-          else if (P.depthTest && !((zMask[q] >> i) & 1)) op = SP.depthFailOp;
+          if (!((sMask >> q) & 1)) op = SP.failOp;
+          else if (P.depthTest && !((zMask >> q) & 1)) op = SP.depthFailOp;
           else op = SP.passOp;
-          uint32_t bv = S.sb[q][i];
+          const uint32_t bv = S.sb[q];
           uint32_t nv = stencil_apply_op(bv, op, SP.reference);
           if (SP.useWriteMask) nv = (nv & SP.writeMask) | (bv & ~SP.writeMask);
-          S.sb[q][i] = nv & 0xFFu;
+          S.sb[q] = nv & 0xFFu;
         }
-    }
-
-    if (!depthPass) continue;
-
-    // ---- depth write + occlusion ----
-#pragma unroll
-    for (int q = 0; q < MS; q++) {
+      }
       if (P.depthWrite) {
 #pragma unroll
-        for (int i = 0; i < 4; i++)
-          if ((zMask[q] >> i) & 1) S.zb[q][i] = zq[q][i];
+        for (int q = 0; q < MS; q++)
+          if ((zMask >> q) & 1) S.zb[q] = zq[q];
       }
-      occl += __builtin_popcount((unsigned)(zMask[q] & sMask[q]));
-    }
+      occl += __builtin_popcount((unsigned)(zMask & sMask));
+      if (!P.colorMask) continue;
+      int xMask = P.depthTest ? zMask : cMask;
+      if (P.stencilActive) xMask &= sMask;
+      const uint64_t need = __ballot(xMask != 0);
+      if (!need) continue;
+      const bool quadNeed = ((need >> quadBase) & 0xFull) != 0;
+      if (!quadNeed) continue;
 
-    if (!P.colorMask) continue;
-    int xMask[MS];
-    int shadeMask = 0;
-#pragma unroll
-    for (int q = 0; q < MS; q++) {
-      xMask[q] = P.depthTest ? zMask[q] : cMask[q];
-      if (P.stencilActive) xMask[q] &= sMask[q];
-      shadeMask |= xMask[q];
-    }
-    if (!shadeMask) continue;
-
-    // ---- fragment shader (late tests: results only matter where written) ----
-    float rhw[4];
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      float Dw = R.wC + yf[i] * R.wB;
-      float w = xf[i] * R.wA + Dw;
-      rhw[i] = 1.0f / w;
-    }
-    float4 col[4];
-    float u[4], v[4];
-    if (P.useUV) {
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-        u[i] = (xf[i] * R.V[4][0] + (R.V[4][2] + yf[i] * R.V[4][1])) * rhw[i];
-        v[i] = (xf[i] * R.V[5][0] + (R.V[5][2] + yf[i] * R.V[5][1])) * rhw[i];
+      // ---- fragment shader for the quads that write (helper lanes included) ----
+      const float Dw = R.wC + yf * R.wB;
+      const float w = xf * R.wA + Dw;
+      const float rhw = 1.0f / w;
+      float4 col = make_float4(0, 0, 0, 0);
+      if (P.shader != SHADER_SAMPLE) {
+        col.x = (xf * R.V[0][0] + (R.V[0][2] + yf * R.V[0][1])) * rhw;
+        col.y = (xf * R.V[1][0] + (R.V[1][2] + yf * R.V[1][1])) * rhw;
+        col.z = (xf * R.V[2][0] + (R.V[2][2] + yf * R.V[2][1])) * rhw;
+        col.w = (xf * R.V[3][0] + (R.V[3][2] + yf * R.V[3][1])) * rhw;
       }
-    }
-    if (P.shader == SHADER_INTERPOLATE) {
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-        if (!((shadeMask >> i) & 1)) continue;
-        col[i].x = (xf[i] * R.V[0][0] + (R.V[0][2] + yf[i] * R.V[0][1])) * rhw[i];
-        col[i].y = (xf[i] * R.V[1][0] + (R.V[1][2] + yf[i] * R.V[1][1])) * rhw[i];
-        col[i].z = (xf[i] * R.V[2][0] + (R.V[2][2] + yf[i] * R.V[2][1])) * rhw[i];
-        col[i].w = (xf[i] * R.V[3][0] + (R.V[3][2] + yf[i] * R.V[3][1])) * rhw[i];
-      }
-    } else {
-      QuadLod Lq = quad_lod(T, u, v);
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-        if (!((shadeMask >> i) & 1)) continue;
-        float4 c = sample_lane(T, u[i], v[i], Lq);
+      if (P.shader != SHADER_INTERPOLATE) {
+        const float u = (xf * R.V[4][0] + (R.V[4][2] + yf * R.V[4][1])) * rhw;
+        const float v = (xf * R.V[5][0] + (R.V[5][2] + yf * R.V[5][1])) * rhw;
+        float uq[4], vq[4];
+        uq[0] = quad_bcast(u, quadBase + 0);
+        uq[1] = quad_bcast(u, quadBase + 1);
+        uq[2] = quad_bcast(u, quadBase + 2);
+        vq[0] = quad_bcast(v, quadBase + 0);
+        vq[1] = quad_bcast(v, quadBase + 1);
+        vq[2] = quad_bcast(v, quadBase + 2);
+        uq[3] = u;
+        vq[3] = v;
+        if (!xMask) continue;
+        const QuadLod Lq = quad_lod(T, uq, vq);
+        float4 c = sample_lane<FTEX>(T, u, v, Lq);
         if (P.shader == SHADER_MODULATE) {
-          c.x = c.x * ((xf[i] * R.V[0][0] + (R.V[0][2] + yf[i] * R.V[0][1])) * rhw[i]);
-          c.y = c.y * ((xf[i] * R.V[1][0] + (R.V[1][2] + yf[i] * R.V[1][1])) * rhw[i]);
-          c.z = c.z * ((xf[i] * R.V[2][0] + (R.V[2][2] + yf[i] * R.V[2][1])) * rhw[i]);
-          c.w = c.w * ((xf[i] * R.V[3][0] + (R.V[3][2] + yf[i] * R.V[3][1])) * rhw[i]);
+          c.x = c.x * col.x;
+          c.y = c.y * col.y;
+          c.z = c.z * col.z;
+          c.w = c.w * col.w;
         }
-        col[i] = c;
+        col = c;
       }
-    }
+      if (!xMask) continue;
 
-    // ---- clamp, blend, write (PixelProgram.cpp:261-364, PixelRoutine.cpp:1617-2936) ----
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      if (!((shadeMask >> i) & 1)) continue;
-      float4 s = col[i];
+      // ---- clamp, blend, write (PixelProgram.cpp:261-364, PixelRoutine.cpp:1617-2936) ----
+      float4 s = col;
       if (!FCOL) {
         s.x = rr_min(rr_max(s.x, 0.0f), 1.0f);
         s.y = rr_min(rr_max(s.y, 0.0f), 1.0f);
@@ -636,56 +657,53 @@ __global__ __launch_bounds__(256) void raster_kernel(DrawParams P, const TriReco
       }
 #pragma unroll
       for (int q = 0; q < MS; q++) {
-        if (!((xMask[q] >> i) & 1)) continue;
+        if (!((xMask >> q) & 1)) continue;
         float4 dst;
         if (FCOL) {
-          dst = S.cf[FCOL ? q : 0][i];
+          dst = S.cf[FCOL ? q : 0];
         } else {
-          uint32_t p = S.cb[q][i];
+          const uint32_t p = S.cb[FCOL ? 0 : q];
           dst = make_float4(unorm8_to_float(p & 0xFFu), unorm8_to_float((p >> 8) & 0xFFu),
                             unorm8_to_float((p >> 16) & 0xFFu), unorm8_to_float(p >> 24));
         }
-        float4 o = P.blendEnable ? alpha_blend(P, s, dst) : s;
+        const float4 o = P.blendEnable ? alpha_blend(P, s, dst) : s;
         if (FCOL) {
-          float4 &c = S.cf[FCOL ? q : 0][i];
+          float4 &c = S.cf[FCOL ? q : 0];
           if (P.colorMask & 1) c.x = o.x;
           if (P.colorMask & 2) c.y = o.y;
           if (P.colorMask & 4) c.z = o.z;
           if (P.colorMask & 8) c.w = o.w;
         } else {
-          uint32_t p = S.cb[q][i];
-          uint32_t r = float_to_unorm8(o.x), g = float_to_unorm8(o.y), b = float_to_unorm8(o.z), a = float_to_unorm8(o.w);
-          if (P.colorMask & 1) p = (p & 0xFFFFFF00u) | r;
-          if (P.colorMask & 2) p = (p & 0xFFFF00FFu) | (g << 8);
-          if (P.colorMask & 4) p = (p & 0xFF00FFFFu) | (b << 16);
-          if (P.colorMask & 8) p = (p & 0x00FFFFFFu) | (a << 24);
-          S.cb[q][i] = p;
+          uint32_t p = S.cb[FCOL ? 0 : q];
+          if (P.colorMask & 1) p = (p & 0xFFFFFF00u) | float_to_unorm8(o.x);
+          if (P.colorMask & 2) p = (p & 0xFFFF00FFu) | (float_to_unorm8(o.y) << 8);
+          if (P.colorMask & 4) p = (p & 0xFF00FFFFu) | (float_to_unorm8(o.z) << 16);
+          if (P.colorMask & 8) p = (p & 0x00FFFFFFu) | (float_to_unorm8(o.w) << 24);
+          S.cb[FCOL ? 0 : q] = p;
         }
       }
     }
   }
 
-  // Write the tile back.
+  // Write the block back.
+  if (inside) {
 #pragma unroll
-  for (int q = 0; q < MS; q++)
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      if (!(inMask & (1 << i))) continue;
-      int x = px + (i & 1), y = py + (i >> 1);
+    for (int q = 0; q < MS; q++) {
       if (P.depthWrite)
-        *reinterpret_cast<float *>(P.depth + q * P.depthSlice + (long long)y * P.depthPitch + 4ll * x) = S.zb[q][i];
-      if (P.stencilWrite) P.stencil[q * P.stencilSlice + (long long)y * P.stencilPitch + x] = (char)S.sb[q][i];
+        *reinterpret_cast<float *>(P.depth + q * P.depthSlice + (long long)py * P.depthPitch + 4ll * px) = S.zb[q];
+      if (P.stencilWrite) P.stencil[q * P.stencilSlice + (long long)py * P.stencilPitch + px] = (char)S.sb[q];
       if (P.colorMask) {
         if (FCOL)
-          *reinterpret_cast<float4 *>(P.color + q * P.colorSlice + (long long)y * P.colorPitch + 16ll * x) =
-              S.cf[FCOL ? q : 0][i];
+          *reinterpret_cast<float4 *>(P.color + q * P.colorSlice + (long long)py * P.colorPitch + 16ll * px) =
+              S.cf[FCOL ? q : 0];
         else
-          *reinterpret_cast<uint32_t *>(P.color + q * P.colorSlice + (long long)y * P.colorPitch + 4ll * x) = S.cb[q][i];
+          *reinterpret_cast<uint32_t *>(P.color + q * P.colorSlice + (long long)py * P.colorPitch + 4ll * px) =
+              S.cb[FCOL ? 0 : q];
       }
     }
+  }
 
   if (P.occlusion) {
-    // Wave reduction of the per-lane sample counts, then one atomic.
     unsigned long long v = occl;
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     if (lane == 0 && v) atomicAdd(P.occlusion, v);
