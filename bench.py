@@ -47,7 +47,21 @@ def parse():
     ap.add_argument("--cpu-triangles", type=int, default=40000,
                     help="triangles in the CPU-baseline sample (0 disables it)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 256 MiB device copy (calibrates the HBM byte counters)")
     return ap.parse_args()
+
+
+def copy_probe():
+    """A plain device-to-device copy of known size, for scaling FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    n = 256 << 20
+    src = torch.empty(n, dtype=torch.uint8, device="cuda:0").fill_(1)
+    dst = torch.empty_like(src)
+    for _ in range(4):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n, "copy_probe_bytes_written": n, "launches": 4}), flush=True)
 
 
 def traffic_from_profiles():
@@ -75,6 +89,9 @@ def cpu_baseline(draw, n_tri, gpu_count_fn):
 
 def main():
     args = parse()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
               "(shard whole draws or screen tiles across GPUs: see DESIGN.md)", file=sys.stderr)
@@ -153,7 +170,10 @@ def main():
                    "fragments_per_step": F, "depth_passing_per_step": P, "triangles": T,
                    "stage_ms_per_step": {k: round(prof[k] / max(prof["draws"], 1), 4)
                                          for k in ("setup_ms", "bin_ms", "raster_ms", "total_ms")},
-                   "pairs_per_step": prof["pairs"] // max(prof["draws"], 1)},
+                   "pairs_per_step": prof["pairs"] // max(prof["draws"], 1),
+                   "block_tests_per_step": prof["block_tests"] // max(prof["draws"], 1),
+                   "block_covers_per_step": prof["block_covers"] // max(prof["draws"], 1),
+                   "shaded_per_step": prof["shaded"] // max(prof["draws"], 1)},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

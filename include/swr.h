@@ -271,6 +271,9 @@ typedef struct swr_profile {
   double total_ms;   /* first to last event of each draw, summed       */
   uint64_t draws;    /* draws measured                                 */
   uint64_t pairs;    /* (tile, triangle) pairs binned                  */
+  uint64_t block_tests;    /* (8x8 block, primitive) coverage evaluations */
+  uint64_t block_covers;   /* ... of which covered at least one sample    */
+  uint64_t shaded;         /* fragments shaded (passed all tests)         */
 } swr_profile;
 
 /* Enables (1) or disables (0) event recording on the current device. */

@@ -120,7 +120,8 @@ class Tuning(C.Structure):
 
 class Profile(C.Structure):
     _fields_ = [("setup_ms", C.c_double), ("bin_ms", C.c_double), ("raster_ms", C.c_double),
-                ("total_ms", C.c_double), ("draws", C.c_uint64), ("pairs", C.c_uint64)]
+                ("total_ms", C.c_double), ("draws", C.c_uint64), ("pairs", C.c_uint64),
+                ("block_tests", C.c_uint64), ("block_covers", C.c_uint64), ("shaded", C.c_uint64)]
 
 
 # Symbols declared by include/swr.h, in declaration order.

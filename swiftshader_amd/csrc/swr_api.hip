@@ -78,7 +78,7 @@ struct EventSet {
 
 struct DeviceCtx {
   int device = -1;
-  Scratch rec, poly, cnt, off, keysA, keysB, valsA, valsB, scanTmp, sortTmp, tStart, tEnd;
+  Scratch rec, poly, cnt, off, keysA, keysB, valsA, valsB, scanTmp, sortTmp, tStart, tEnd, stats;
   uint32_t *hostTotal = nullptr;  // pinned
   swr_tuning tuning{};
   bool profiling = false;
@@ -407,6 +407,8 @@ int draw_one(DeviceCtx *c, const swr_draw_desc *desc, hipStream_t s) {
   EventSet es{};
   const bool prof = c->profiling;
   if (prof) {
+    SWR_HIP_CHECK(c->stats.ensure(4 * sizeof(unsigned long long)));
+    P.stats = c->stats.as<unsigned long long>();
     SWR_HIP_CHECK(take_events(c, es));
     SWR_HIP_CHECK(hipEventRecord(es.ev[EV_START], s));
   }
@@ -439,8 +441,9 @@ int draw_one(DeviceCtx *c, const swr_draw_desc *desc, hipStream_t s) {
                                                    bits, s));
   SWR_HIP_CHECK(c->sortTmp.ensure(sortBytes));
 
-  hipLaunchKernelGGL(bin_emit_kernel, dim3((T + threads - 1) / threads), dim3(threads), 0, s, P,
-                     c->rec.as<TriRecord>(), c->off.as<uint32_t>(), c->keysA.as<uint32_t>(), c->valsA.as<uint32_t>());
+  hipLaunchKernelGGL(bin_emit_kernel, dim3((pairs + threads - 1) / threads), dim3(threads), 0, s, P,
+                     c->rec.as<TriRecord>(), c->off.as<uint32_t>(), pairs, c->keysA.as<uint32_t>(),
+                     c->valsA.as<uint32_t>());
   SWR_HIP_CHECK(hipGetLastError());
   SWR_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(c->sortTmp.ptr, sortBytes, c->keysA.as<uint32_t>(),
                                                    c->keysB.as<uint32_t>(), c->valsA.as<uint32_t>(),
@@ -554,6 +557,10 @@ int swr_profile_enable(int enable) {
   DeviceCtx *c = nullptr;
   int r = current_ctx(&c);
   if (r != SWR_OK) return r;
+  if (enable && !c->profiling) {
+    SWR_HIP_CHECK(c->stats.ensure(4 * sizeof(unsigned long long)));
+    SWR_HIP_CHECK(hipMemset(c->stats.ptr, 0, 4 * sizeof(unsigned long long)));
+  }
   c->profiling = enable != 0;
   return SWR_OK;
 }
@@ -579,6 +586,15 @@ int swr_profile_read(swr_profile *out) {
     c->pool.push_back(es);
   }
   c->pending.clear();
+  if (c->stats.ptr) {
+    unsigned long long st[4] = {0, 0, 0, 0};
+    SWR_HIP_CHECK(hipDeviceSynchronize());
+    SWR_HIP_CHECK(hipMemcpy(st, c->stats.ptr, sizeof(st), hipMemcpyDeviceToHost));
+    SWR_HIP_CHECK(hipMemset(c->stats.ptr, 0, sizeof(st)));
+    c->acc.block_tests = st[0];
+    c->acc.block_covers = st[1];
+    c->acc.shaded = st[2];
+  }
   *out = c->acc;
   c->acc = swr_profile{};
   return SWR_OK;

@@ -277,6 +277,28 @@ __global__ __launch_bounds__(256) void setup_kernel(DrawParams P, TriRecord *__r
     R.V[k][2] = (i0 * M02 + i1 * 0.0f) + i2 * 0.0f;
   }
 
+  // Lower bound of z as the rasteriser evaluates it, x*A + (C + y*B) in
+  // float, over the bounding box widened by half a pixel (sample offsets):
+  // the affine minimum at a corner, less a margin of 8 ulp of the largest
+  // term.  Lets a tile skip primitives that cannot pass its depth test.
+  float zlo = -__builtin_inff();
+  if (P.depthTest && __builtin_isfinite(R.zA) && __builtin_isfinite(R.zB) && __builtin_isfinite(R.zC)) {
+    const double xa = (double)xMin - 0.5 - (double)R.x0, xb = (double)(xMax - 1) + 0.5 - (double)R.x0;
+    const double ya = (double)yMin - 0.5 - (double)R.y0, yb = (double)(yMax - 1) + 0.5 - (double)R.y0;
+    const double A = R.zA, B = R.zB, C = R.zC;
+    const double m = fmin(fmin(A * xa + B * ya, A * xb + B * ya), fmin(A * xa + B * yb, A * xb + B * yb)) + C;
+    const double mag = fabs(A) * fmax(fabs(xa), fabs(xb)) + fabs(C) + 2.0 * fabs(B) * fmax(fabs(ya), fabs(yb));
+    const double lo = m - mag * 0x1p-21 - 1e-30;
+    if (__builtin_isfinite(lo)) {
+      float f = (float)lo;
+      if ((double)f > lo) {  // step one float towards -inf
+        const uint32_t u = __float_as_uint(f);
+        f = (f > 0.0f) ? __uint_as_float(u - 1) : (f < 0.0f ? __uint_as_float(u + 1) : -1.17549435e-38f);
+      }
+      zlo = f;
+    }
+  }
+  R.zlo = zlo;
   R.flags = d | (frontFacing ? 2 : 0);
   R.yMin = yMin;
   R.yMax = yMax;
@@ -298,24 +320,27 @@ __global__ __launch_bounds__(256) void setup_kernel(DrawParams P, TriRecord *__r
 // ---------------------------------------------------------------------------
 // Binning
 // ---------------------------------------------------------------------------
+// One thread per (tile, primitive) pair: the pair's primitive is found by
+// binary search over the prefix sum of per-primitive tile counts, so the
+// pairs of large primitives are written in parallel and coalesced.
 __global__ __launch_bounds__(256) void bin_emit_kernel(DrawParams P, const TriRecord *__restrict__ rec,
-                                                       const uint32_t *__restrict__ tileOffset,
+                                                       const uint32_t *__restrict__ tileOffset, uint32_t pairs,
                                                        uint32_t *__restrict__ keys, uint32_t *__restrict__ vals) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= P.triCount) return;
-  uint32_t off = tileOffset[t];
-  uint32_t cnt = tileOffset[t + 1] - off;
-  if (cnt == 0) return;
-  const TriRecord &R = rec[t];
-  int tx0 = R.xMin / P.tileW, tx1 = (R.xMax - 1) / P.tileW;
-  int ty0 = R.yMin / P.tileH, ty1 = (R.yMax - 1) / P.tileH;
-  uint32_t k = off;
-  for (int ty = ty0; ty <= ty1; ty++)
-    for (int tx = tx0; tx <= tx1; tx++) {
-      keys[k] = (uint32_t)(ty * P.tilesX + tx);
-      vals[k] = (uint32_t)t;
-      k++;
-    }
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= pairs) return;
+  int lo = 0, hi = P.triCount;  // tileOffset[lo] <= k < tileOffset[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (tileOffset[mid] <= k) lo = mid;
+    else hi = mid;
+  }
+  const TriRecord &R = rec[lo];
+  const uint32_t i = k - tileOffset[lo];
+  const int tx0 = R.xMin / P.tileW, tx1 = (R.xMax - 1) / P.tileW;
+  const int ty0 = R.yMin / P.tileH;
+  const uint32_t w = (uint32_t)(tx1 - tx0 + 1);
+  keys[k] = (uint32_t)((ty0 + (int)(i / w)) * P.tilesX + tx0 + (int)(i % w));
+  vals[k] = (uint32_t)lo;
 }
 
 __global__ void fill_u32_kernel(uint32_t *p, uint32_t v, uint32_t n) {
@@ -430,7 +455,107 @@ struct PixelState {
   float4 cf[FCOL ? MS : 1];
 };
 
-__device__ __forceinline__ float quad_bcast(float v, int src) { return __shfl(v, src, 64); }
+// Pixel of lane `l` inside its 8x8 block: lanes 4k..4k+3 form the 2x2 quad
+// k, quads in row-major order over the block.
+__device__ __forceinline__ int lane_cx(int l) { return 2 * ((l >> 2) & 3) + (l & 1); }
+__device__ __forceinline__ int lane_cy(int l) { return 2 * (l >> 4) + ((l >> 1) & 1); }
+
+__device__ __forceinline__ float rdl(float v, int j) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+
+// Interpolated attribute k at (xf, yf): (x*A + (C + y*B)) * rhw, the
+// reference's operation order (QuadRasterizer.cpp:235-250).
+__device__ __forceinline__ float interp(const TriRecord &R, int k, float xf, float yf, float rhw) {
+  return (xf * R.V[k][0] + (R.V[k][2] + yf * R.V[k][1])) * rhw;
+}
+
+__device__ __forceinline__ float quad_rhw(const TriRecord &R, float xf, float yf) {
+  const float Dw = R.wC + yf * R.wB;
+  return 1.0f / (xf * R.wA + Dw);
+}
+
+// Shades one queued fragment: the fixed shader family at pixel (px, py)
+// of primitive R, with the texture LOD of the 2x2 quad the pixel belongs
+// to (helper lanes evaluated from the plane equations).  Output clamped
+// for UNORM targets (PixelProgram::clampColor).
+template <bool FCOL, bool FTEX>
+__device__ __forceinline__ float4 shade(const DrawParams &P, const TriRecord &R, int px, int py) {
+  const float xf = (float)px - R.x0, yf = (float)py - R.y0;
+  const float rhw = quad_rhw(R, xf, yf);
+  float4 col = make_float4(0, 0, 0, 0);
+  if (P.shader != SHADER_SAMPLE) {
+    col.x = interp(R, 0, xf, yf, rhw);
+    col.y = interp(R, 1, xf, yf, rhw);
+    col.z = interp(R, 2, xf, yf, rhw);
+    col.w = interp(R, 3, xf, yf, rhw);
+  }
+  if (P.shader != SHADER_INTERPOLATE) {
+    const int qx = px & ~1, qy = py & ~1;
+    float uq[4], vq[4];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      const float xi = (float)(qx + (i & 1)) - R.x0, yi = (float)(qy + (i >> 1)) - R.y0;
+      const float ri = quad_rhw(R, xi, yi);
+      uq[i] = interp(R, 4, xi, yi, ri);
+      vq[i] = interp(R, 5, xi, yi, ri);
+    }
+    const float u = interp(R, 4, xf, yf, rhw), v = interp(R, 5, xf, yf, rhw);
+    uq[3] = u;
+    vq[3] = v;
+    const QuadLod Lq = quad_lod(P.tex, uq, vq);
+    float4 c = sample_lane<FTEX>(P.tex, u, v, Lq);
+    if (P.shader == SHADER_MODULATE) {
+      c.x = c.x * col.x;
+      c.y = c.y * col.y;
+      c.z = c.z * col.z;
+      c.w = c.w * col.w;
+    }
+    col = c;
+  }
+  if (!FCOL) {
+    col.x = rr_min(rr_max(col.x, 0.0f), 1.0f);
+    col.y = rr_min(rr_max(col.y, 0.0f), 1.0f);
+    col.z = rr_min(rr_max(col.z, 0.0f), 1.0f);
+    col.w = rr_min(rr_max(col.w, 0.0f), 1.0f);
+  }
+  return col;
+}
+
+// Blend a shaded colour into the samples of this lane's pixel selected by
+// xMask, then write (PixelRoutine.cpp:1617-2936).
+template <int MS, bool FCOL>
+__device__ __forceinline__ void blend_write(const DrawParams &P, PixelState<MS, FCOL> &S, float4 s, int xMask) {
+#pragma unroll
+  for (int q = 0; q < MS; q++) {
+    if (!((xMask >> q) & 1)) continue;
+    float4 dst;
+    if (FCOL) {
+      dst = S.cf[FCOL ? q : 0];
+    } else {
+      const uint32_t p = S.cb[FCOL ? 0 : q];
+      dst = make_float4(unorm8_to_float(p & 0xFFu), unorm8_to_float((p >> 8) & 0xFFu),
+                        unorm8_to_float((p >> 16) & 0xFFu), unorm8_to_float(p >> 24));
+    }
+    const float4 o = P.blendEnable ? alpha_blend(P, s, dst) : s;
+    if (FCOL) {
+      float4 &c = S.cf[FCOL ? q : 0];
+      if (P.colorMask & 1) c.x = o.x;
+      if (P.colorMask & 2) c.y = o.y;
+      if (P.colorMask & 4) c.z = o.z;
+      if (P.colorMask & 8) c.w = o.w;
+    } else {
+      uint32_t p = S.cb[FCOL ? 0 : q];
+      if (P.colorMask & 1) p = (p & 0xFFFFFF00u) | float_to_unorm8(o.x);
+      if (P.colorMask & 2) p = (p & 0xFFFF00FFu) | (float_to_unorm8(o.y) << 8);
+      if (P.colorMask & 4) p = (p & 0xFF00FFFFu) | (float_to_unorm8(o.z) << 16);
+      if (P.colorMask & 8) p = (p & 0x00FFFFFFu) | (float_to_unorm8(o.w) << 24);
+      S.cb[FCOL ? 0 : q] = p;
+    }
+  }
+}
+
+constexpr int kQueue = 64;  // deferred-shading queue entries per wave
 
 template <int MS, bool FCOL, bool FTEX>
 __global__ __launch_bounds__(256) void raster_kernel(DrawParams P, const TriRecord *__restrict__ rec,
@@ -438,23 +563,25 @@ __global__ __launch_bounds__(256) void raster_kernel(DrawParams P, const TriReco
                                                      const uint32_t *__restrict__ binTri,
                                                      const uint32_t *__restrict__ tileStart,
                                                      const uint32_t *__restrict__ tileEnd) {
+  // Per-wave queue of fragments that passed the tests and await shading,
+  // in API order: primitive, owner lane | sample mask << 8, shaded colour.
+  __shared__ uint32_t qTri[4][kQueue];
+  __shared__ uint32_t qPix[4][kQueue];
+  __shared__ float4 qCol[4][kQueue];
+
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int bin = blockIdx.x;
   const int bx = bin % P.tilesX, by = bin / P.tilesX;
   const int ox = bx * 16 + 8 * (wave & 1), oy = by * 16 + 8 * (wave >> 1);
-  const int quad = lane >> 2, sub = lane & 3;
-  const int cx = 2 * (quad & 3) + (sub & 1), cy = 2 * (quad >> 2) + (sub >> 1);
+  const int cx = lane_cx(lane), cy = lane_cy(lane);
   const int px = ox + cx, py = oy + cy;
   const int myBit = cy * 8 + cx;
-  const int quadBase = lane & ~3;
 
   const uint32_t start = tileStart[bin], end = tileEnd[bin];
   if (start >= end) return;
-  // Blocks entirely outside the scissor have nothing to do.
   if (ox >= P.sx1 || oy >= P.sy1 || ox + 8 <= P.sx0 || oy + 8 <= P.sy0) return;
 
-  // Scissor mask of the block (rows and columns inside [s*0, s*1)).
   uint64_t scissorMask = 0;
   {
     uint32_t cols = 0;
@@ -488,82 +615,117 @@ __global__ __launch_bounds__(256) void raster_kernel(DrawParams P, const TriReco
     }
   }
 
-  unsigned long long occl = 0;
-  const TexParams &T = P.tex;
-  const int bx0 = ox, by0 = oy;
+  // Hierarchical depth: a primitive whose evaluated depth is bounded below
+  // by zlo cannot pass LESS (LESS_OR_EQUAL) anywhere in a block whose
+  // largest stored depth is <= zlo (< zlo).  Only when a failing fragment
+  // has no side effect (no stencil writes).
+  const bool hiz = P.depthTest && !P.stencilWrite && (P.depthCompare == 1 || P.depthCompare == 3);
 
-  for (uint32_t base = start; base < end; base += 64) {
+  unsigned long long occl = 0;
+  uint32_t nTests = 0, nCovers = 0, nShaded = 0;  // statistics (profiling only)
+  int qCount = 0;           // wave-uniform queue length
+  uint64_t myEntries = 0;   // queue slots holding fragments of this lane's pixel
+
+  auto flush = [&]() __attribute__((always_inline)) {
+    // Shade the queued fragments, one per lane...
+    if (lane < qCount) {
+      const uint32_t t = qTri[wave][lane];
+      const uint32_t pix = qPix[wave][lane];
+      const int l = (int)(pix & 63u);
+      qCol[wave][lane] = shade<FCOL, FTEX>(P, rec[t], ox + lane_cx(l), oy + lane_cy(l));
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    // ...then blend them into their pixels in queue (API) order.
+    uint64_t m = myEntries;
+    while (m) {
+      const int k = __builtin_ctzll(m);
+      m &= m - 1;
+      blend_write<MS, FCOL>(P, S, qCol[wave][k], (int)(qPix[wave][k] >> 8));
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    myEntries = 0;
+    qCount = 0;
+  };
+
+  // One chunk of up to 64 list entries: lanes holding a candidate (box
+  // meets the block, hierarchical depth test passed) compute its exact
+  // coverage; then the covering primitives are applied in list order.
+  auto process_chunk = [&](bool act, uint32_t tid) __attribute__((always_inline)) {
     // ---- triangle-parallel coverage of this block ----
-    const uint32_t e = base + lane;
-    uint32_t tid = 0;
     uint32_t mlo[MS], mhi[MS];
+    int flags = 0;
+    float x0 = 0, y0 = 0, zA = 0, zB = 0, zC = 0;
 #pragma unroll
     for (int q = 0; q < MS; q++) { mlo[q] = 0; mhi[q] = 0; }
-    if (e < end) {
-      tid = binTri[e];
-      const int4 h0 = reinterpret_cast<const int4 *>(rec + tid)[0];  // n, flags, yMin, yMax
-      const int4 h1 = reinterpret_cast<const int4 *>(rec + tid)[1];  // xMin, xMax, X0, X1
-      const int4 h2 = reinterpret_cast<const int4 *>(rec + tid)[2];  // X2, Y0, Y1, Y2
+    if (act) {
+      const int4 *rp = reinterpret_cast<const int4 *>(rec + tid);
+      const int4 h0 = rp[0];  // n, flags, yMin, yMax
+      const int4 h1 = rp[1];  // xMin, xMax, X0, X1
+      const int4 h2 = rp[2];  // X2, Y0, Y1, Y2
+      const float4 h3 = reinterpret_cast<const float4 *>(rp)[3];  // zlo, x0, y0, zA
+      const float4 h4 = reinterpret_cast<const float4 *>(rp)[4];  // zB, zC, wA, wB
       const int n = h0.x, d = h0.y & 1;
-      if (h0.z < by0 + 8 && h0.w > by0 && h1.x < bx0 + 8 && h1.y > bx0) {
-        const int *pp = poly + (size_t)tid * (2 * kMaxPolygon);
-        auto vtx = [&](int i, int &Xo, int &Yo) {
-          if (n == 3) {
-            Xo = i == 0 ? h1.z : (i == 1 ? h1.w : h2.x);
-            Yo = i == 0 ? h2.y : (i == 1 ? h2.z : h2.w);
-          } else {
-            Xo = pp[i];
-            Yo = pp[kMaxPolygon + i];
-          }
-        };
-#pragma unroll
-        for (int q = 0; q < MS; q++) {
-          if (!((P.msMask >> q) & 1)) continue;
-          const int xo = (MS > 1) ? kXf[q] : 0, yo = (MS > 1) ? kYf[q] : 0;
-          uint64_t m = block_coverage(vtx, n, d, xo, yo, bx0, by0) & scissorMask;
-          mlo[q] = (uint32_t)m;
-          mhi[q] = (uint32_t)(m >> 32);
+      flags = h0.y;
+      x0 = h3.y; y0 = h3.z; zA = h3.w; zB = h4.x; zC = h4.y;
+      const int *pp = poly + (size_t)tid * (2 * kMaxPolygon);
+      auto vtx = [&](int i, int &Xo, int &Yo) {
+        if (n == 3) {
+          Xo = i == 0 ? h1.z : (i == 1 ? h1.w : h2.x);
+          Yo = i == 0 ? h2.y : (i == 1 ? h2.z : h2.w);
+        } else {
+          Xo = pp[i];
+          Yo = pp[kMaxPolygon + i];
         }
+      };
+#pragma unroll
+      for (int q = 0; q < MS; q++) {
+        if (!((P.msMask >> q) & 1)) continue;
+        const int xo = (MS > 1) ? kXf[q] : 0, yo = (MS > 1) ? kYf[q] : 0;
+        const uint64_t m = block_coverage(vtx, n, d, xo, yo, ox, oy) & scissorMask;
+        mlo[q] = (uint32_t)m;
+        mhi[q] = (uint32_t)(m >> 32);
       }
     }
     uint32_t anyBits = 0;
 #pragma unroll
     for (int q = 0; q < MS; q++) anyBits |= mlo[q] | mhi[q];
     uint64_t todo = __ballot(anyBits != 0);
+    nCovers += anyBits != 0 ? 1u : 0u;
 
-    // ---- pixel-parallel application, in list order ----
+    // ---- pixel-parallel tests, in list order ----
     while (todo) {
       const int j = __builtin_ctzll(todo);
       todo &= todo - 1;
-      const uint32_t t = __builtin_amdgcn_readlane(tid, j);
-      int cMask = 0;  // bit q = sample q of this lane's pixel is covered
+      int cMask = 0;  // bit q: sample q of this lane's pixel is covered
 #pragma unroll
       for (int q = 0; q < MS; q++) {
         const uint32_t lo = __builtin_amdgcn_readlane(mlo[q], j), hi = __builtin_amdgcn_readlane(mhi[q], j);
         const uint32_t word = myBit < 32 ? lo : hi;
         cMask |= (int)((word >> (myBit & 31)) & 1u) << q;
       }
-      const TriRecord &R = rec[t];
-      const bool front = (R.flags & 2) != 0;
-      const StencilParams &SP = front ? P.front : P.back;
+      const int fl = __builtin_amdgcn_readlane(flags, j);
+      const StencilParams &SP = (fl & 2) ? P.front : P.back;
 
       int sMask = cMask, zMask = cMask;
-      if (cMask && P.stencilActive) {
+      if (P.stencilActive && cMask) {
 #pragma unroll
         for (int q = 0; q < MS; q++)
           if (((cMask >> q) & 1) && !stencil_compare(SP, S.sb[q])) sMask &= ~(1 << q);
       }
-      const float xf = (float)px - R.x0;  // xFragment (PixelRoutine.cpp:125)
-      const float yf = (float)py - R.y0;  // yFragment (QuadRasterizer.cpp:126)
       float zq[MS];
       if (P.depthTest) {
+        const float xf = (float)px - rdl(x0, j);  // xFragment (PixelRoutine.cpp:125)
+        const float yf = (float)py - rdl(y0, j);  // yFragment (QuadRasterizer.cpp:126)
+        const float a = rdl(zA, j), b = rdl(zB, j), c = rdl(zC, j);
         zMask = 0;
 #pragma unroll
         for (int q = 0; q < MS; q++) {
           const float yy = (MS > 1) ? yf + kSampleY[q] : yf;
           const float xx = (MS > 1) ? xf - kSampleX[q] : xf;
-          const float Dz = R.zC + yy * R.zB;
-          float z = xx * R.zA + Dz;
+          const float Dz = c + yy * b;
+          float z = xx * a + Dz;
           z = rr_min(rr_max(z, 0.0f), 1.0f);
           zq[q] = z;
           const float zb = S.zb[q];
@@ -608,82 +770,68 @@ __global__ __launch_bounds__(256) void raster_kernel(DrawParams P, const TriReco
       if (P.stencilActive) xMask &= sMask;
       const uint64_t need = __ballot(xMask != 0);
       if (!need) continue;
-      const bool quadNeed = ((need >> quadBase) & 0xFull) != 0;
-      if (!quadNeed) continue;
+      // Queue the fragments that write colour; shading is deferred so that
+      // it runs with every lane busy.
+      const int cnt = __builtin_popcountll(need);
+      if (qCount + cnt > kQueue) flush();
+      if (xMask) {
+        const int pos = qCount + __builtin_amdgcn_mbcnt_hi((uint32_t)(need >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0));
+        qTri[wave][pos] = __builtin_amdgcn_readlane(tid, j);
+        qPix[wave][pos] = (uint32_t)lane | ((uint32_t)xMask << 8);
+        myEntries |= 1ull << pos;
+        nShaded++;
+      }
+      qCount += cnt;
+    }
+  };
 
-      // ---- fragment shader for the quads that write (helper lanes included) ----
-      const float Dw = R.wC + yf * R.wB;
-      const float w = xf * R.wA + Dw;
-      const float rhw = 1.0f / w;
-      float4 col = make_float4(0, 0, 0, 0);
-      if (P.shader != SHADER_SAMPLE) {
-        col.x = (xf * R.V[0][0] + (R.V[0][2] + yf * R.V[0][1])) * rhw;
-        col.y = (xf * R.V[1][0] + (R.V[1][2] + yf * R.V[1][1])) * rhw;
-        col.z = (xf * R.V[2][0] + (R.V[2][2] + yf * R.V[2][1])) * rhw;
-        col.w = (xf * R.V[3][0] + (R.V[3][2] + yf * R.V[3][1])) * rhw;
-      }
-      if (P.shader != SHADER_INTERPOLATE) {
-        const float u = (xf * R.V[4][0] + (R.V[4][2] + yf * R.V[4][1])) * rhw;
-        const float v = (xf * R.V[5][0] + (R.V[5][2] + yf * R.V[5][1])) * rhw;
-        float uq[4], vq[4];
-        uq[0] = quad_bcast(u, quadBase + 0);
-        uq[1] = quad_bcast(u, quadBase + 1);
-        uq[2] = quad_bcast(u, quadBase + 2);
-        vq[0] = quad_bcast(v, quadBase + 0);
-        vq[1] = quad_bcast(v, quadBase + 1);
-        vq[2] = quad_bcast(v, quadBase + 2);
-        uq[3] = u;
-        vq[3] = v;
-        if (!xMask) continue;
-        const QuadLod Lq = quad_lod(T, uq, vq);
-        float4 c = sample_lane<FTEX>(T, u, v, Lq);
-        if (P.shader == SHADER_MODULATE) {
-          c.x = c.x * col.x;
-          c.y = c.y * col.y;
-          c.z = c.z * col.z;
-          c.w = c.w * col.w;
-        }
-        col = c;
-      }
-      if (!xMask) continue;
-
-      // ---- clamp, blend, write (PixelProgram.cpp:261-364, PixelRoutine.cpp:1617-2936) ----
-      float4 s = col;
-      if (!FCOL) {
-        s.x = rr_min(rr_max(s.x, 0.0f), 1.0f);
-        s.y = rr_min(rr_max(s.y, 0.0f), 1.0f);
-        s.z = rr_min(rr_max(s.z, 0.0f), 1.0f);
-        s.w = rr_min(rr_max(s.w, 0.0f), 1.0f);
-      }
+  for (uint32_t base = start; base < end; base += 64) {
+    // ---- block depth bound for the hierarchical test ----
+    float zmax = __builtin_inff();
+    if (hiz) {
+      float v = -__builtin_inff();
+      bool nan = false;
+      if (inside) {
 #pragma unroll
-      for (int q = 0; q < MS; q++) {
-        if (!((xMask >> q) & 1)) continue;
-        float4 dst;
-        if (FCOL) {
-          dst = S.cf[FCOL ? q : 0];
-        } else {
-          const uint32_t p = S.cb[FCOL ? 0 : q];
-          dst = make_float4(unorm8_to_float(p & 0xFFu), unorm8_to_float((p >> 8) & 0xFFu),
-                            unorm8_to_float((p >> 16) & 0xFFu), unorm8_to_float(p >> 24));
+        for (int q = 0; q < MS; q++) {
+          v = fmaxf(v, S.zb[q]);
+          nan |= S.zb[q] != S.zb[q];
         }
-        const float4 o = P.blendEnable ? alpha_blend(P, s, dst) : s;
-        if (FCOL) {
-          float4 &c = S.cf[FCOL ? q : 0];
-          if (P.colorMask & 1) c.x = o.x;
-          if (P.colorMask & 2) c.y = o.y;
-          if (P.colorMask & 4) c.z = o.z;
-          if (P.colorMask & 8) c.w = o.w;
-        } else {
-          uint32_t p = S.cb[FCOL ? 0 : q];
-          if (P.colorMask & 1) p = (p & 0xFFFFFF00u) | float_to_unorm8(o.x);
-          if (P.colorMask & 2) p = (p & 0xFFFF00FFu) | (float_to_unorm8(o.y) << 8);
-          if (P.colorMask & 4) p = (p & 0xFF00FFFFu) | (float_to_unorm8(o.z) << 16);
-          if (P.colorMask & 8) p = (p & 0x00FFFFFFu) | (float_to_unorm8(o.w) << 24);
-          S.cb[FCOL ? 0 : q] = p;
-        }
+      }
+      for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+      zmax = __ballot(nan) ? __builtin_inff() : v;
+    }
+    // ---- candidate test: box overlap, then the hierarchical depth bound ----
+    const uint32_t e = base + lane;
+    bool test = false;
+    uint32_t tid = 0;
+    if (e < end) {
+      tid = binTri[e];
+      const int4 *rp = reinterpret_cast<const int4 *>(rec + tid);
+      const int4 h0 = rp[0];  // n, flags, yMin, yMax
+      const int4 h1 = rp[1];  // xMin, xMax, ...
+      test = h0.z < oy + 8 && h0.w > oy && h1.x < ox + 8 && h1.y > ox;
+      if (hiz && test) {
+        const float4 h3 = reinterpret_cast<const float4 *>(rp)[3];  // zlo, x0, y0, zA
+        const float4 h4 = reinterpret_cast<const float4 *>(rp)[4];  // zB, zC, ...
+        const float x0 = h3.y, y0 = h3.z, zA = h3.w, zB = h4.x, zC = h4.y;
+        // Lower bound of the evaluated plane over this block (sample offsets
+        // included), tightened with the primitive's bound over its box.
+        const float xa = (float)ox - 0.5f - x0, xb = (float)(ox + 7) + 0.5f - x0;
+        const float ya = (float)oy - 0.5f - y0, yb = (float)(oy + 7) + 0.5f - y0;
+        const float m = fminf(fminf(zA * xa + zB * ya, zA * xb + zB * ya), fminf(zA * xa + zB * yb, zA * xb + zB * yb)) + zC;
+        const float mag = fabsf(zA) * fmaxf(fabsf(xa), fabsf(xb)) + fabsf(zC) + 2.0f * fabsf(zB) * fmaxf(fabsf(ya), fabsf(yb));
+        float zl = fmaxf(h3.x, m - mag * 0x1p-19f);
+        if (!(zl == zl)) zl = -__builtin_inff();
+        zl = fminf(zl, 1.0f);
+        test = !(P.depthCompare == 1 ? zl >= zmax : zl > zmax);
       }
     }
+    nTests += test ? 1u : 0u;
+    if (__ballot(test)) process_chunk(test, tid);
   }
+  if (qCount) flush();
 
   // Write the block back.
   if (inside) {
@@ -707,6 +855,19 @@ __global__ __launch_bounds__(256) void raster_kernel(DrawParams P, const TriReco
     unsigned long long v = occl;
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     if (lane == 0 && v) atomicAdd(P.occlusion, v);
+  }
+  if (P.stats) {
+    unsigned long long a = nTests, b = nCovers, c = nShaded;
+    for (int off = 32; off > 0; off >>= 1) {
+      a += __shfl_xor(a, off, 64);
+      b += __shfl_xor(b, off, 64);
+      c += __shfl_xor(c, off, 64);
+    }
+    if (lane == 0) {
+      atomicAdd(P.stats + 0, a);
+      atomicAdd(P.stats + 1, b);
+      atomicAdd(P.stats + 2, c);
+    }
   }
 }
 

@@ -81,24 +81,26 @@ struct DrawParams {
   int stencilPitch;
   long long stencilSlice;
   unsigned long long *occlusion;
+  unsigned long long *stats;  // [block tests, block covers, shaded] or NULL
   // Tiling.
   int tileW, tileH, tilesX, tilesY;
 };
 
 // Per-primitive setup output.  n == 0 marks a culled primitive.  The first
-// 48 bytes are what the coverage pass reads per lane; the rest is read with
-// scalar loads by the wave that shades the primitive.
+// 80 bytes are what the coverage and depth passes read per lane; the rest
+// (w plane and attribute planes) is read when fragments are shaded.
 struct TriRecord {
   int n;            // polygon vertex count (3 unless clipped)
   int flags;        // bit0: d (winding), bit1: front facing
   int yMin, yMax;   // rows with possible coverage (scissor-clamped)
   int xMin, xMax;   // pixel columns with possible coverage
   int X[3], Y[3];   // 28.4 polygon vertices when n == 3
+  float zlo;        // lower bound of the evaluated depth over the bounding box
   float x0, y0;
   float zA, zB, zC;
   float wA, wB, wC;
   float V[6][3];    // colour rgba, uv
-  int pad[2];
+  int pad;
 };
 
 static_assert(sizeof(TriRecord) == 160, "TriRecord layout");

@@ -161,7 +161,8 @@ def profile_read() -> dict:
     p = abi.Profile()
     check(lib().swr_profile_read(ctypes.byref(p)), "swr_profile_read")
     return {"setup_ms": p.setup_ms, "bin_ms": p.bin_ms, "raster_ms": p.raster_ms, "total_ms": p.total_ms,
-            "draws": int(p.draws), "pairs": int(p.pairs)}
+            "draws": int(p.draws), "pairs": int(p.pairs), "block_tests": int(p.block_tests),
+            "block_covers": int(p.block_covers), "shaded": int(p.shaded)}
 
 
 def render(draw_data: DrawData, device="cuda:0", stream=None):
