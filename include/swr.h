@@ -46,7 +46,7 @@
 extern "C" {
 #endif
 
-#define SWR_ABI_VERSION 1
+#define SWR_ABI_VERSION 2
 
 /* ---- status codes -------------------------------------------------------- */
 #define SWR_OK 0
@@ -59,6 +59,7 @@ extern "C" {
 /* VkFormat */
 #define SWR_FORMAT_UNDEFINED 0
 #define SWR_FORMAT_R8G8B8A8_UNORM 37
+#define SWR_FORMAT_R32_UINT 98
 #define SWR_FORMAT_R32G32B32A32_SFLOAT 109
 #define SWR_FORMAT_D32_SFLOAT 126
 #define SWR_FORMAT_S8_UINT 127
@@ -217,6 +218,13 @@ typedef struct swr_draw_desc {
   /* Optional device pointer: += samples passing depth/stencil
    * (the VK_QUERY_TYPE_OCCLUSION result of the reference). */
   uint64_t *occlusion_count;
+
+  /* Optional R32_UINT attachment, one plane per sample (same pitches as the
+   * other attachments): each sample whose colour a fragment writes receives
+   * 1 + that fragment's primitive index in draw order (0 = never written).
+   * This is the visibility record swr_draw_backward reads.  It has no
+   * counterpart in the reference; it requires an RGBA32F colour attachment. */
+  swr_attachment primitive_id;
 } swr_draw_desc;
 
 typedef struct swr_resolve_desc { /* 4x -> 1x colour resolve */
@@ -229,6 +237,35 @@ typedef struct swr_resolve_desc { /* 4x -> 1x colour resolve */
   void *dst;
   uint32_t dst_row_pitch_bytes;
 } swr_resolve_desc;
+
+/* Gradients of a draw (differentiable rendering; no counterpart in the
+ * reference).  The draw is the one a swr_draw_desc describes, rendered with
+ * its primitive_id attachment; blending must be off, the colour attachment
+ * RGBA32F and a sampled texture RGBA32F.
+ *
+ * What is differentiated: coverage and visibility are discrete and held
+ * fixed (the primitive_id record); so are the texture LOD, the anisotropic
+ * probe count and axis, and the mip levels, which take the forward pass's
+ * values.  Per covered sample, the colour is the fixed shader evaluated at
+ * the pixel centre with perspective-correct barycentrics of the unsnapped
+ * projected vertices, and the texture filter is the forward's
+ * (bilinear / trilinear / anisotropic, REPEAT or CLAMP_TO_EDGE).  z does not
+ * reach the colour, so d_positions[.z] stays 0.  All outputs are
+ * ACCUMULATED (+=) with float atomics; NULL outputs are skipped. */
+typedef struct swr_grad_desc {
+  const float *d_color;                /* dL/d(colour attachment): float4 per sample */
+  uint32_t d_color_row_pitch_bytes;
+  uint64_t d_color_sample_pitch_bytes;
+  float *d_positions;                  /* float4 per vertex (x, y, z, w)        */
+  float *d_colors;                     /* float4 per vertex                      */
+  float *d_texcoords;                  /* float2 per vertex                      */
+  void *d_texture[SWR_MAX_MIP_LEVELS]; /* float4 per texel, per level, with the
+                                          texture level's row pitch              */
+  float *sample_state;                 /* optional float4 per sample: the frozen
+                                          (lod, anisotropy, uDelta, vDelta) used */
+  uint32_t sample_state_row_pitch_bytes;
+  uint64_t sample_state_sample_pitch_bytes;
+} swr_grad_desc;
 
 /* Tuning knobs of the tiled rasteriser (0 = library default). */
 typedef struct swr_tuning {
@@ -260,6 +297,22 @@ int swr_draw_batch(const swr_draw_desc *descs, uint32_t count, void *stream);
 
 /* MSAA colour resolve (sw::Blitter::resolve, src/Device/Blitter.cpp:2035). */
 int swr_resolve(const swr_resolve_desc *desc, void *stream);
+
+/* Gradients of the draw `desc` (see swr_grad_desc).  Asynchronous on
+ * `stream`; the forward draw must have completed on the same stream. */
+int swr_draw_backward(const swr_draw_desc *desc, const swr_grad_desc *grad, void *stream);
+
+/* Fills levels 1..level_count-1 of `tex` from level 0 by 2x2 box filtering
+ * (level k+1 = 0.25 * ((a + b) + (c + d)) for RGBA32F; (a+b+c+d+2)>>2 per
+ * channel for RGBA8; odd sizes halve with floor and clamp the second
+ * texel).  The mip chain a differentiable texture is sampled through. */
+int swr_generate_mips(const swr_texture *tex, void *stream);
+
+/* Adjoint of swr_generate_mips for RGBA32F: adds each level's gradient,
+ * from the coarsest up, into the level below (every texel receives 0.25 of
+ * the gradient of each box it belongs to).  Level 0 then holds the gradient
+ * with respect to the base texture. */
+int swr_mip_backward(const swr_texture *grad, void *stream);
 
 /* Per-stage GPU time of the draws enqueued while profiling was enabled,
  * measured with HIP events recorded on the caller's stream (the analogue

@@ -227,6 +227,7 @@ typedef struct {
   int outlineRows;
   uint16_t *left[4], *right[4];
   uint64_t occlusion;
+  uint32_t curTri; /* primitive index in draw order (for the primitive_id record) */
 } Ctx;
 
 #define OUTLINE_OFFSET 4
@@ -805,6 +806,11 @@ static uint8_t *color_at(Ctx *c, int q, int x, int y) {
          (size_t)y * c->d->color.row_pitch_bytes + (size_t)x * (c->colorIsFloat ? 16 : 4);
 }
 
+static uint32_t *id_at(Ctx *c, int q, int x, int y) {
+  const swr_attachment *a = &c->d->primitive_id;
+  return (uint32_t *)((uint8_t *)a->data + q * a->sample_pitch_bytes + (size_t)y * a->row_pitch_bytes + (size_t)x * 4);
+}
+
 static float blend_factor_rgb(const Ctx *c, int f, int comp, const float *src, const float *dst) {
   switch (f) {
     case SWR_BLEND_FACTOR_ZERO: return 0.0f;
@@ -1011,6 +1017,7 @@ static void quad(Ctx *c, const OPrim *prim, int x, int y, const int cMask[4], co
           float bl[4];
           alpha_blend(c, buf, col[i], bl);
           write_color(c, buf, bl);
+          if (c->d->primitive_id.data) *id_at(c, q, x + (i & 1), y + (i >> 1)) = c->curTri + 1u;
         }
       }
     }
@@ -1161,6 +1168,8 @@ static int init_ctx(Ctx *c, const swr_draw_desc *d) {
   c->stencilActive = d->stencil.data && d->stencil_test_enable;
 
   c->colorIsFloat = d->color.format == SWR_FORMAT_R32G32B32A32_SFLOAT;
+  if (d->primitive_id.data && (!c->colorIsFloat || d->primitive_id.format != SWR_FORMAT_R32_UINT))
+    return SWR_ERROR_UNSUPPORTED;
   const swr_blend_state *b = &d->blend;
   int cop = blend_operation((int)b->color_blend_op, (int)b->src_color_blend_factor, (int)b->dst_color_blend_factor, c->colorIsFloat);
   int aop = blend_operation((int)b->alpha_blend_op, (int)b->src_alpha_blend_factor, (int)b->dst_alpha_blend_factor, c->colorIsFloat);
@@ -1264,6 +1273,7 @@ EXPORT int swo_draw(const swr_draw_desc *d) {
       OPrim prim;
       memset(&prim, 0, sizeof(prim));
       if (!setup_routine(&c, &prim, tri, &poly)) continue;
+      c.curTri = t;
       rasterize(&c, &prim);
     }
     if (d->occlusion_count) *d->occlusion_count += c.occlusion;

@@ -7,7 +7,7 @@ DrawData in the reference (src/Device/Renderer.hpp:58-113, 211-213).
 """
 import ctypes as C
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 OK = 0
 ERROR_INVALID_ARGUMENT = -1
@@ -17,6 +17,7 @@ ERROR_DEVICE = -4
 
 FORMAT_UNDEFINED = 0
 FORMAT_R8G8B8A8_UNORM = 37
+FORMAT_R32_UINT = 98
 FORMAT_R32G32B32A32_SFLOAT = 109
 FORMAT_D32_SFLOAT = 126
 FORMAT_S8_UINT = 127
@@ -105,6 +106,7 @@ class DrawDesc(C.Structure):
         ("framebuffer_width", C.c_uint32), ("framebuffer_height", C.c_uint32),
         ("color", Attachment), ("depth", Attachment), ("stencil", Attachment),
         ("occlusion_count", C.c_void_p),
+        ("primitive_id", Attachment),
     ]
 
 
@@ -112,6 +114,15 @@ class ResolveDesc(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("format", C.c_uint32),
                 ("src", C.c_void_p), ("src_row_pitch_bytes", C.c_uint32), ("src_sample_pitch_bytes", C.c_uint64),
                 ("src_samples", C.c_uint32), ("dst", C.c_void_p), ("dst_row_pitch_bytes", C.c_uint32)]
+
+
+class GradDesc(C.Structure):
+    _fields_ = [("d_color", C.c_void_p), ("d_color_row_pitch_bytes", C.c_uint32),
+                ("d_color_sample_pitch_bytes", C.c_uint64),
+                ("d_positions", C.c_void_p), ("d_colors", C.c_void_p), ("d_texcoords", C.c_void_p),
+                ("d_texture", C.c_void_p * MAX_MIP_LEVELS),
+                ("sample_state", C.c_void_p), ("sample_state_row_pitch_bytes", C.c_uint32),
+                ("sample_state_sample_pitch_bytes", C.c_uint64)]
 
 
 class Tuning(C.Structure):
@@ -127,8 +138,8 @@ class Profile(C.Structure):
 # Symbols declared by include/swr.h, in declaration order.
 EXPORTED_SYMBOLS = (
     "swr_abi_version", "swr_last_error", "swr_init", "swr_release", "swr_draw",
-    "swr_draw_batch", "swr_resolve", "swr_profile_enable", "swr_profile_read",
-    "swr_set_tuning", "swr_get_tuning",
+    "swr_draw_batch", "swr_resolve", "swr_draw_backward", "swr_generate_mips", "swr_mip_backward",
+    "swr_profile_enable", "swr_profile_read", "swr_set_tuning", "swr_get_tuning",
 )
 
 
@@ -145,6 +156,12 @@ def bind(lib):
     lib.swr_draw_batch.restype = C.c_int
     lib.swr_resolve.argtypes = [C.POINTER(ResolveDesc), C.c_void_p]
     lib.swr_resolve.restype = C.c_int
+    lib.swr_draw_backward.argtypes = [C.POINTER(DrawDesc), C.POINTER(GradDesc), C.c_void_p]
+    lib.swr_draw_backward.restype = C.c_int
+    lib.swr_generate_mips.argtypes = [C.POINTER(Texture), C.c_void_p]
+    lib.swr_generate_mips.restype = C.c_int
+    lib.swr_mip_backward.argtypes = [C.POINTER(Texture), C.c_void_p]
+    lib.swr_mip_backward.restype = C.c_int
     lib.swr_profile_enable.argtypes = [C.c_int]
     lib.swr_profile_enable.restype = C.c_int
     lib.swr_profile_read.argtypes = [C.POINTER(Profile)]

@@ -20,6 +20,7 @@
 
 #include "../../include/swr.h"
 #include "swr_kernels.hip"
+#include "swr_grad.hip"
 
 using namespace swr;
 
@@ -372,6 +373,18 @@ int build_params(const swr_draw_desc *d, const swr_tuning &tuning, DrawParams &P
   P.stencilPitch = (int)d->stencil.row_pitch_bytes;
   P.stencilSlice = (long long)d->stencil.sample_pitch_bytes;
   P.occlusion = reinterpret_cast<unsigned long long *>(d->occlusion_count);
+  if (d->primitive_id.data) {
+    if (!hasColor || !P.colorFloat || d->primitive_id.format != SWR_FORMAT_R32_UINT)
+      return fail(SWR_ERROR_UNSUPPORTED, "primitive_id needs an R32_UINT attachment and an RGBA32F colour attachment");
+    if (d->primitive_id.row_pitch_bytes < d->framebuffer_width * 4)
+      return fail(SWR_ERROR_INVALID_ARGUMENT, "primitive_id row pitch too small");
+    if (d->samples > 1 &&
+        d->primitive_id.sample_pitch_bytes < (uint64_t)d->primitive_id.row_pitch_bytes * d->framebuffer_height)
+      return fail(SWR_ERROR_INVALID_ARGUMENT, "primitive_id sample pitch too small");
+    P.primId = (char *)d->primitive_id.data;
+    P.primIdPitch = (int)d->primitive_id.row_pitch_bytes;
+    P.primIdSlice = (long long)d->primitive_id.sample_pitch_bytes;
+  }
 
   P.tileW = (int)tuning.tile_width;
   P.tileH = (int)tuning.tile_height;
@@ -482,9 +495,125 @@ int draw_one(DeviceCtx *c, const swr_draw_desc *desc, hipStream_t s) {
   return SWR_OK;
 }
 
+int draw_backward(DeviceCtx *c, const swr_draw_desc *desc, const swr_grad_desc *g, hipStream_t s) {
+  if (!g) return fail(SWR_ERROR_INVALID_ARGUMENT, "grad descriptor is NULL");
+  DrawParams P;
+  int r = build_params(desc, c->tuning, P);
+  if (r != SWR_OK) return r;
+  if (!P.primId) return fail(SWR_ERROR_INVALID_ARGUMENT, "backward needs the draw's primitive_id attachment");
+  if (P.blendEnable) return fail(SWR_ERROR_UNSUPPORTED, "backward of a blended draw");
+  if (!g->d_color) return fail(SWR_ERROR_INVALID_ARGUMENT, "d_color is NULL");
+  if (g->d_color_row_pitch_bytes < desc->framebuffer_width * 16 ||
+      (P.ms > 1 && g->d_color_sample_pitch_bytes < (uint64_t)g->d_color_row_pitch_bytes * desc->framebuffer_height))
+    return fail(SWR_ERROR_INVALID_ARGUMENT, "d_color pitches too small");
+  if (g->sample_state &&
+      (g->sample_state_row_pitch_bytes < desc->framebuffer_width * 16 ||
+       (P.ms > 1 &&
+        g->sample_state_sample_pitch_bytes < (uint64_t)g->sample_state_row_pitch_bytes * desc->framebuffer_height)))
+    return fail(SWR_ERROR_INVALID_ARGUMENT, "sample_state pitches too small");
+  GradParams G{};
+  G.dColor = (const char *)g->d_color;
+  G.dColorPitch = (int)g->d_color_row_pitch_bytes;
+  G.dColorSlice = (long long)g->d_color_sample_pitch_bytes;
+  G.dPos = g->d_positions;
+  G.dCol = P.useColor ? g->d_colors : nullptr;
+  G.dUV = P.useUV ? g->d_texcoords : nullptr;
+  G.state = (char *)g->sample_state;
+  G.statePitch = (int)g->sample_state_row_pitch_bytes;
+  G.stateSlice = (long long)g->sample_state_sample_pitch_bytes;
+  if (P.useUV) {
+    if (!P.tex.isFloat) return fail(SWR_ERROR_UNSUPPORTED, "backward samples RGBA32F textures only");
+    if (P.tex.filter == FILT_MIN_POINT_MAG_LINEAR || P.tex.filter == FILT_MIN_LINEAR_MAG_POINT)
+      return fail(SWR_ERROR_UNSUPPORTED, "backward of mixed min/mag filters");
+    const uint32_t n = desc->texture.level_count;
+    for (int l = 0; l < kMipLevels; l++) G.dTex[l] = (float *)g->d_texture[std::min<uint32_t>(l, n - 1)];
+  }
+  const uint32_t T = desc->triangle_count;
+  if (T == 0) return SWR_OK;
+  SWR_HIP_CHECK(c->rec.ensure(sizeof(TriRecord) * (size_t)T));
+  SWR_HIP_CHECK(c->poly.ensure(sizeof(int) * 2 * kMaxPolygon * (size_t)T));
+  SWR_HIP_CHECK(c->cnt.ensure(sizeof(uint32_t) * ((size_t)T + 1)));
+  hipLaunchKernelGGL(setup_kernel, dim3((T + 255) / 256), dim3(256), 0, s, P, c->rec.as<TriRecord>(),
+                     c->poly.as<int>(), c->cnt.as<uint32_t>());
+  dim3 grid((P.width + 15) / 16, (P.height + 15) / 16), block(256);
+  if (P.ms == 1)
+    hipLaunchKernelGGL(backward_kernel<1>, grid, block, 0, s, P, G, (const TriRecord *)c->rec.as<TriRecord>());
+  else
+    hipLaunchKernelGGL(backward_kernel<4>, grid, block, 0, s, P, G, (const TriRecord *)c->rec.as<TriRecord>());
+  SWR_HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}
+
+// Validates a mip chain: level k+1 is max(level k / 2, 1) on each axis.
+int check_chain(const swr_texture *t, uint32_t texelBytes) {
+  if (t->level_count < 1 || t->level_count > SWR_MAX_MIP_LEVELS)
+    return fail(SWR_ERROR_INVALID_ARGUMENT, "level_count %u outside 1..15", t->level_count);
+  for (uint32_t l = 0; l < t->level_count; l++) {
+    const swr_image_level &v = t->levels[l];
+    if (!v.data || v.width == 0 || v.height == 0 || v.row_pitch_bytes % texelBytes ||
+        v.row_pitch_bytes / texelBytes < v.width)
+      return fail(SWR_ERROR_INVALID_ARGUMENT, "level %u invalid", l);
+    if (l > 0) {
+      const swr_image_level &u = t->levels[l - 1];
+      if (v.width != std::max(u.width / 2, 1u) || v.height != std::max(u.height / 2, 1u))
+        return fail(SWR_ERROR_INVALID_ARGUMENT, "level %u is not half of level %u", l, l - 1);
+    }
+  }
+  return SWR_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int swr_draw_backward(const swr_draw_desc *desc, const swr_grad_desc *grad, void *stream) {
+  DeviceCtx *c = nullptr;
+  int r = current_ctx(&c);
+  if (r != SWR_OK) return r;
+  if (!desc) return fail(SWR_ERROR_INVALID_ARGUMENT, "desc is NULL");
+  return draw_backward(c, desc, grad, reinterpret_cast<hipStream_t>(stream));
+}
+
+int swr_generate_mips(const swr_texture *tex, void *stream) {
+  if (!tex) return fail(SWR_ERROR_INVALID_ARGUMENT, "texture is NULL");
+  const bool f = tex->format == SWR_FORMAT_R32G32B32A32_SFLOAT;
+  if (!f && tex->format != SWR_FORMAT_R8G8B8A8_UNORM) return fail(SWR_ERROR_UNSUPPORTED, "format %u", tex->format);
+  const uint32_t tb = f ? 16 : 4;
+  int r = check_chain(tex, tb);
+  if (r != SWR_OK) return r;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  for (uint32_t l = 1; l < tex->level_count; l++) {
+    const swr_image_level &a = tex->levels[l - 1], &b = tex->levels[l];
+    dim3 grid((b.width + 255) / 256, b.height), block(256);
+    if (f)
+      hipLaunchKernelGGL(mip_down_rgba32f_kernel, grid, block, 0, s, (const float4 *)a.data, (int)a.width,
+                         (int)a.height, (int)(a.row_pitch_bytes / tb), (float4 *)b.data, (int)b.width, (int)b.height,
+                         (int)(b.row_pitch_bytes / tb));
+    else
+      hipLaunchKernelGGL(mip_down_rgba8_kernel, grid, block, 0, s, (const uint32_t *)a.data, (int)a.width,
+                         (int)a.height, (int)(a.row_pitch_bytes / tb), (uint32_t *)b.data, (int)b.width, (int)b.height,
+                         (int)(b.row_pitch_bytes / tb));
+  }
+  SWR_HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}
+
+int swr_mip_backward(const swr_texture *grad, void *stream) {
+  if (!grad) return fail(SWR_ERROR_INVALID_ARGUMENT, "texture is NULL");
+  if (grad->format != SWR_FORMAT_R32G32B32A32_SFLOAT) return fail(SWR_ERROR_UNSUPPORTED, "format %u", grad->format);
+  int r = check_chain(grad, 16);
+  if (r != SWR_OK) return r;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  for (uint32_t l = grad->level_count - 1; l >= 1; l--) {
+    const swr_image_level &fine = grad->levels[l - 1], &coarse = grad->levels[l];
+    dim3 grid((fine.width + 255) / 256, fine.height), block(256);
+    hipLaunchKernelGGL(mip_up_rgba32f_kernel, grid, block, 0, s, (float4 *)fine.data, (int)fine.width,
+                       (int)fine.height, (int)(fine.row_pitch_bytes / 16), (const float4 *)coarse.data,
+                       (int)coarse.width, (int)coarse.height, (int)(coarse.row_pitch_bytes / 16));
+  }
+  SWR_HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}
 
 int swr_abi_version(void) { return SWR_ABI_VERSION; }
 

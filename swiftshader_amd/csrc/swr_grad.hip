@@ -1,0 +1,380 @@
+// swr_grad.hip -- gradients of a draw, and the mip chain with its adjoint.
+//
+// Differentiable rendering has no counterpart in the reference; the forward
+// function being differentiated is the reference's fixed-function draw
+// (PixelProgram.cpp:138-364 shading, SamplerCore.cpp float path) with:
+//   * coverage and visibility held fixed: each covered sample's colour comes
+//     from the primitive recorded in the primitive_id attachment;
+//   * the texture's sampling state held fixed at the forward pass's values:
+//     LOD (hence mip levels and trilinear weight), anisotropic probe count
+//     and axis -- recomputed here bit-for-bit from the same plane equations;
+//   * attributes interpolated at the pixel centre with perspective-correct
+//     barycentrics of the UNSNAPPED projected vertices, so that positions
+//     have a gradient (the forward's 28.4 snapping has zero derivative).
+// The per-fragment chain runs in FP64 (screen positions near 2^10 pixels
+// and texel coordinates near 2^12 lose too much in FP32 for 1e-4 gradient
+// agreement); the results are accumulated into FP32 buffers with float
+// atomics.
+//
+//   backward_kernel    one thread per pixel; per distinct owning primitive
+//                      of the pixel's samples: re-derive the fragment,
+//                      scatter dL/dtexel over the filter footprint, and
+//                      dL/d(colour, uv, x, y, w) of the three vertices.
+//   mip_down_kernel    2x2 box filter, level k -> k+1.
+//   mip_up_kernel      its adjoint, level k+1 -> k (accumulating).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "swr_device.hpp"
+#include "swr_params.hpp"
+
+namespace swr {
+
+struct GradParams {
+  const char *dColor;
+  int dColorPitch;
+  long long dColorSlice;
+  float *dPos;
+  float *dCol;
+  float *dUV;
+  float *dTex[kMipLevels];  // per level (levels past the view repeat the last)
+  char *state;
+  int statePitch;
+  long long stateSlice;
+};
+
+__device__ __forceinline__ void atomic_add4(float *p, double a, double b, double c, double d) {
+  unsafeAtomicAdd(p + 0, (float)a);
+  unsafeAtomicAdd(p + 1, (float)b);
+  unsafeAtomicAdd(p + 2, (float)c);
+  unsafeAtomicAdd(p + 3, (float)d);
+}
+
+// One axis of the float-path address computation (SamplerCore.cpp:2463-2638)
+// in FP64: tap indices x0/x1, the filter fraction f and df/d(uvw).
+__device__ __forceinline__ void axis64(int filter, double uvw, int dim, int clampMode, int &x0, int &x1, double &f,
+                                       double &dfdu) {
+  double c, dc;
+  if (clampMode) {
+    const double hi = 1.0 - 0x1p-24;  // 0x3F7FFFFF
+    if (!(uvw > 0.0)) { c = 0.0; dc = 0.0; }
+    else if (uvw > hi) { c = hi; dc = 0.0; }
+    else { c = uvw; dc = 1.0; }
+  } else {
+    c = uvw - floor(uvw);
+    dc = 1.0;
+    if (!(c >= 0.0 && c <= 1.0)) {  // NaN / infinite coordinate: texel 0, no gradient
+      c = 0.0;
+      dc = 0.0;
+    }
+  }
+  double coord = c * (double)dim;
+  if (filter == FILT_POINT) {
+    x0 = (int)coord;
+    x0 = x0 < dim - 1 ? x0 : dim - 1;
+    x1 = x0;
+    f = 0.0;
+    dfdu = 0.0;
+    return;
+  }
+  coord -= 0.5;
+  const double fl = floor(coord);
+  x0 = (int)fl;
+  f = coord - fl;
+  dfdu = dc * (double)dim;
+  x1 = x0 + 1;
+  if (clampMode) {
+    x0 = x0 > 0 ? x0 : 0;
+    x1 = x1 < dim - 1 ? x1 : dim - 1;
+  } else {
+    if (x0 < 0) x0 = dim - 1;
+    if (x1 >= dim) x1 = 0;
+  }
+}
+
+struct Acc4 { double x, y, z, w; };
+
+// One bilinear (or point) footprint at level `lvl` with weight `wgt`:
+// adds wgt * filter(u, v) to `val`, wgt * d(filter . gTex)/du, /dv to dU, dV,
+// and scatters gTex * wgt * tap weight into the level's texel gradient.
+__device__ __forceinline__ void tap_backward(const TexParams &T, const GradParams &G, int lvl, double u, double v,
+                                             double wgt, const Acc4 &gTex, Acc4 &val, double &dU, double &dV) {
+  int x0, x1, y0, y1;
+  double fu, fv, dfu, dfv;
+  axis64(T.filter, u, T.width[lvl], T.addrU, x0, x1, fu, dfu);
+  axis64(T.filter, v, T.height[lvl], T.addrV, y0, y1, fv, dfv);
+  const int pitch = T.pitchP[lvl];
+  const float4 *tex = reinterpret_cast<const float4 *>(T.data[lvl]);
+  const int i00 = x0 + y0 * pitch, i10 = x1 + y0 * pitch, i01 = x0 + y1 * pitch, i11 = x1 + y1 * pitch;
+  const float4 c00 = tex[i00], c10 = tex[i10], c01 = tex[i01], c11 = tex[i11];
+  const double w00 = (1.0 - fu) * (1.0 - fv), w10 = fu * (1.0 - fv), w01 = (1.0 - fu) * fv, w11 = fu * fv;
+  val.x += wgt * (w00 * c00.x + w10 * c10.x + w01 * c01.x + w11 * c11.x);
+  val.y += wgt * (w00 * c00.y + w10 * c10.y + w01 * c01.y + w11 * c11.y);
+  val.z += wgt * (w00 * c00.z + w10 * c10.z + w01 * c01.z + w11 * c11.z);
+  val.w += wgt * (w00 * c00.w + w10 * c10.w + w01 * c01.w + w11 * c11.w);
+  const double d00 = gTex.x * c00.x + gTex.y * c00.y + gTex.z * c00.z + gTex.w * c00.w;
+  const double d10 = gTex.x * c10.x + gTex.y * c10.y + gTex.z * c10.z + gTex.w * c10.w;
+  const double d01 = gTex.x * c01.x + gTex.y * c01.y + gTex.z * c01.z + gTex.w * c01.w;
+  const double d11 = gTex.x * c11.x + gTex.y * c11.y + gTex.z * c11.z + gTex.w * c11.w;
+  dU += wgt * dfu * ((1.0 - fv) * (d10 - d00) + fv * (d11 - d01));
+  dV += wgt * dfv * ((1.0 - fu) * (d01 - d00) + fu * (d11 - d10));
+  float *g = G.dTex[lvl];
+  if (g) {
+    const double a = wgt * w00, b = wgt * w10, c = wgt * w01, d = wgt * w11;
+    if (a != 0.0) atomic_add4(g + 4 * (size_t)(uint32_t)i00, gTex.x * a, gTex.y * a, gTex.z * a, gTex.w * a);
+    if (b != 0.0) atomic_add4(g + 4 * (size_t)(uint32_t)i10, gTex.x * b, gTex.y * b, gTex.z * b, gTex.w * b);
+    if (c != 0.0) atomic_add4(g + 4 * (size_t)(uint32_t)i01, gTex.x * c, gTex.y * c, gTex.z * c, gTex.w * c);
+    if (d != 0.0) atomic_add4(g + 4 * (size_t)(uint32_t)i11, gTex.x * d, gTex.y * d, gTex.z * d, gTex.w * d);
+  }
+}
+
+// The forward filter at (u, v) with the frozen state L: anisotropic probes
+// along (uDelta, vDelta) (sampleFloatAniso, SamplerCore.cpp:1085-1141),
+// each trilinear between two levels (sampleTexture128, :200-253).
+__device__ __forceinline__ void texture_backward(const TexParams &T, const GradParams &G, const QuadLod &L, double u,
+                                                 double v, const Acc4 &gTex, Acc4 &val, double &dU, double &dV) {
+  int N = 1;
+  double A = 1.0, B = 0.0;
+  if (T.filter == FILT_ANISOTROPIC) {
+    N = rr_round_int(L.aniso);
+    A = (double)uv_weight(N);
+    B = (double)uv_start(N);
+  }
+  const int lvl0 = select_mip(T, L.lod, 0);
+  double wl0 = 1.0, wl1 = 0.0;
+  if (T.mipmap == MIP_LINEAR) {
+    wl1 = (double)rr_frac(L.lod);
+    wl0 = 1.0 - wl1;
+  }
+  int i = 0;
+  do {
+    const double t = B + (double)i * A;
+    const double ui = u + t * (double)L.uDelta, vi = v + t * (double)L.vDelta;
+    tap_backward(T, G, lvl0, ui, vi, A * wl0, gTex, val, dU, dV);
+    if (wl1 != 0.0) tap_backward(T, G, lvl0 + 1, ui, vi, A * wl1, gTex, val, dU, dV);
+    i++;
+  } while (i < N);
+}
+
+__device__ __forceinline__ uint32_t vertex_index(const DrawParams &P, uint32_t t, int k) {
+  uint32_t vi = P.idx ? P.idx[3 * (size_t)t + k] : 3u * t + (uint32_t)k;
+  return vi < (uint32_t)P.vertexCount ? vi : (uint32_t)P.vertexCount - 1u;
+}
+
+// Gradient of one fragment: primitive t at pixel (px, py), upstream g.
+__device__ void fragment_backward(const DrawParams &P, const GradParams &G, const TriRecord &R, uint32_t t, int px,
+                                  int py, const Acc4 &g, const QuadLod &L) {
+  uint32_t vi[3];
+  double X[3], Y[3], rhw[3], x[3], y[3], w[3];
+  const double X0 = (double)P.X0xF / 16.0, Y0 = (double)P.Y0xF / 16.0;
+  const double Wh = (double)P.WxF / 16.0, Hh = (double)P.HxF / 16.0;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    vi[k] = vertex_index(P, t, k);
+    const float4 p = reinterpret_cast<const float4 *>(P.pos)[vi[k]];
+    x[k] = p.x;
+    y[k] = p.y;
+    w[k] = p.w;
+    rhw[k] = p.w != 0.0f ? 1.0 / (double)p.w : 1.0;
+    X[k] = X0 + x[k] * rhw[k] * Wh;
+    Y[k] = Y0 + y[k] * rhw[k] * Hh;
+  }
+  const double D = (Y[1] - Y[2]) * (X[0] - X[2]) + (X[2] - X[1]) * (Y[0] - Y[2]);
+  if (!(D != 0.0)) return;
+  const double iD = 1.0 / D;
+  const double gx[3] = {(Y[1] - Y[2]) * iD, (Y[2] - Y[0]) * iD, (Y[0] - Y[1]) * iD};
+  const double gy[3] = {(X[2] - X[1]) * iD, (X[0] - X[2]) * iD, (X[1] - X[0]) * iD};
+  const double ex = (double)px - X[2], ey = (double)py - Y[2];
+  double lam[3];
+  lam[0] = gx[0] * ex + gy[0] * ey;
+  lam[1] = gx[1] * ex + gy[1] * ey;
+  lam[2] = 1.0 - lam[0] - lam[1];
+  double r[3], S = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    r[k] = lam[k] * rhw[k];
+    S += r[k];
+  }
+  if (!(S != 0.0)) return;
+  const double iS = 1.0 / S;
+  double b[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) b[k] = r[k] * iS;
+
+  // Attributes at the fragment.
+  float4 vc[3];
+  float2 vt[3];
+  Acc4 col = {1.0, 1.0, 1.0, 1.0};
+  double u = 0.0, v = 0.0;
+  if (P.useColor) {
+    col = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      vc[k] = reinterpret_cast<const float4 *>(P.col)[vi[k]];
+      col.x += b[k] * vc[k].x;
+      col.y += b[k] * vc[k].y;
+      col.z += b[k] * vc[k].z;
+      col.w += b[k] * vc[k].w;
+    }
+  }
+  if (P.useUV) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      vt[k] = reinterpret_cast<const float2 *>(P.uv)[vi[k]];
+      u += b[k] * vt[k].x;
+      v += b[k] * vt[k].y;
+    }
+  }
+
+  // Shader: out = colour | texture | texture * colour.
+  Acc4 dcol = {0.0, 0.0, 0.0, 0.0};
+  double dU = 0.0, dV = 0.0;
+  if (P.useUV) {
+    const Acc4 gTex = P.useColor ? Acc4{g.x * col.x, g.y * col.y, g.z * col.z, g.w * col.w} : g;
+    Acc4 tex = {0.0, 0.0, 0.0, 0.0};
+    texture_backward(P.tex, G, L, u, v, gTex, tex, dU, dV);
+    if (P.useColor) dcol = {g.x * tex.x, g.y * tex.y, g.z * tex.z, g.w * tex.w};
+  } else {
+    dcol = g;
+  }
+
+  // Attribute gradients and dL/db.
+  double Gk[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    Gk[k] = 0.0;
+    if (P.useColor) {
+      if (G.dCol) atomic_add4(G.dCol + 4 * (size_t)vi[k], b[k] * dcol.x, b[k] * dcol.y, b[k] * dcol.z, b[k] * dcol.w);
+      Gk[k] += dcol.x * vc[k].x + dcol.y * vc[k].y + dcol.z * vc[k].z + dcol.w * vc[k].w;
+    }
+    if (P.useUV) {
+      if (G.dUV) {
+        unsafeAtomicAdd(G.dUV + 2 * (size_t)vi[k] + 0, (float)(b[k] * dU));
+        unsafeAtomicAdd(G.dUV + 2 * (size_t)vi[k] + 1, (float)(b[k] * dV));
+      }
+      Gk[k] += dU * vt[k].x + dV * vt[k].y;
+    }
+  }
+  if (!G.dPos) return;
+  // b = r / sum(r), r_k = lambda_k * rhw_k.
+  const double Abar = b[0] * Gk[0] + b[1] * Gk[1] + b[2] * Gk[2];
+  double dlam[3], drhw[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const double dr = (Gk[k] - Abar) * iS;
+    dlam[k] = dr * rhw[k];
+    drhw[k] = dr * lam[k];
+  }
+  // d lambda_i / d X_j = -lambda_j * d lambda_i / d px (likewise for Y).
+  const double Gx = dlam[0] * gx[0] + dlam[1] * gx[1] + dlam[2] * gx[2];
+  const double Gy = dlam[0] * gy[0] + dlam[1] * gy[1] + dlam[2] * gy[2];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const double dX = -lam[k] * Gx, dY = -lam[k] * Gy;
+    const double dx = dX * rhw[k] * Wh, dy = dY * rhw[k] * Hh;
+    const double dr = drhw[k] + dX * x[k] * Wh + dY * y[k] * Hh;
+    const double dw = w[k] != 0.0 ? -dr * rhw[k] * rhw[k] : 0.0;
+    float *o = G.dPos + 4 * (size_t)vi[k];
+    unsafeAtomicAdd(o + 0, (float)dx);
+    unsafeAtomicAdd(o + 1, (float)dy);
+    unsafeAtomicAdd(o + 3, (float)dw);
+  }
+}
+
+template <int MS>
+__global__ __launch_bounds__(256) void backward_kernel(DrawParams P, GradParams G, const TriRecord *__restrict__ rec) {
+  const int px = blockIdx.x * 16 + (threadIdx.x & 15);
+  const int py = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (px >= P.width || py >= P.height) return;
+  uint32_t id[MS];
+#pragma unroll
+  for (int q = 0; q < MS; q++)
+    id[q] = *reinterpret_cast<const uint32_t *>(P.primId + q * P.primIdSlice + (long long)py * P.primIdPitch + 4ll * px);
+#pragma unroll
+  for (int q = 0; q < MS; q++) {
+    const uint32_t t1 = id[q];
+    if (!t1 || t1 > (uint32_t)P.triCount) continue;
+    bool seen = false;
+#pragma unroll
+    for (int q2 = 0; q2 < q; q2++) seen |= id[q2] == t1;
+    if (seen) continue;
+    // Samples of this pixel owned by the same primitive share its fragment
+    // (shaded once at the pixel centre): their upstream gradients add.
+    Acc4 g = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q2 = q; q2 < MS; q2++) {
+      if (id[q2] != t1) continue;
+      const float4 d = *reinterpret_cast<const float4 *>(G.dColor + q2 * G.dColorSlice + (long long)py * G.dColorPitch +
+                                                         16ll * px);
+      g.x += d.x;
+      g.y += d.y;
+      g.z += d.z;
+      g.w += d.w;
+    }
+    const uint32_t t = t1 - 1u;
+    const TriRecord &R = rec[t];
+    QuadLod L = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (P.useUV) L = quad_state(P, R, px, py);
+    if (G.state) {
+#pragma unroll
+      for (int q2 = q; q2 < MS; q2++)
+        if (id[q2] == t1)
+          *reinterpret_cast<float4 *>(G.state + q2 * G.stateSlice + (long long)py * G.statePitch + 16ll * px) =
+              make_float4(L.lod, L.aniso, L.uDelta, L.vDelta);
+    }
+    fragment_backward(P, G, R, t, px, py, g, L);
+  }
+}
+
+// 2x2 box filter (odd sizes: floor, the second texel clamped).
+__global__ void mip_down_rgba32f_kernel(const float4 *src, int sw, int sh, int sp, float4 *dst, int dw, int dh, int dp) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= dw || y >= dh) return;
+  const int x0 = 2 * x < sw - 1 ? 2 * x : sw - 1, x1 = 2 * x + 1 < sw - 1 ? 2 * x + 1 : sw - 1;
+  const int y0 = 2 * y < sh - 1 ? 2 * y : sh - 1, y1 = 2 * y + 1 < sh - 1 ? 2 * y + 1 : sh - 1;
+  const float4 a = src[x0 + (size_t)y0 * sp], b = src[x1 + (size_t)y0 * sp];
+  const float4 c = src[x0 + (size_t)y1 * sp], d = src[x1 + (size_t)y1 * sp];
+  dst[x + (size_t)y * dp] = make_float4(((a.x + b.x) + (c.x + d.x)) * 0.25f, ((a.y + b.y) + (c.y + d.y)) * 0.25f,
+                                        ((a.z + b.z) + (c.z + d.z)) * 0.25f, ((a.w + b.w) + (c.w + d.w)) * 0.25f);
+}
+
+__global__ void mip_down_rgba8_kernel(const uint32_t *src, int sw, int sh, int sp, uint32_t *dst, int dw, int dh,
+                                      int dp) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= dw || y >= dh) return;
+  const int x0 = 2 * x < sw - 1 ? 2 * x : sw - 1, x1 = 2 * x + 1 < sw - 1 ? 2 * x + 1 : sw - 1;
+  const int y0 = 2 * y < sh - 1 ? 2 * y : sh - 1, y1 = 2 * y + 1 < sh - 1 ? 2 * y + 1 : sh - 1;
+  const uint32_t a = src[x0 + (size_t)y0 * sp], b = src[x1 + (size_t)y0 * sp];
+  const uint32_t c = src[x0 + (size_t)y1 * sp], d = src[x1 + (size_t)y1 * sp];
+  uint32_t o = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const uint32_t s = ((a >> (8 * k)) & 0xFFu) + ((b >> (8 * k)) & 0xFFu) + ((c >> (8 * k)) & 0xFFu) +
+                       ((d >> (8 * k)) & 0xFFu) + 2u;
+    o |= (s >> 2) << (8 * k);
+  }
+  dst[x + (size_t)y * dp] = o;
+}
+
+// Adjoint of mip_down_rgba32f_kernel: texel (x, y) of the finer level
+// receives 0.25 * (its multiplicity in the box) of its parent's gradient.
+__global__ void mip_up_rgba32f_kernel(float4 *fine, int fw, int fh, int fp, const float4 *coarse, int cw, int ch,
+                                      int cp) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= fw || y >= fh) return;
+  const int mx = fw == 1 ? 2 : (x < 2 * cw ? 1 : 0);
+  const int my = fh == 1 ? 2 : (y < 2 * ch ? 1 : 0);
+  if (!mx || !my) return;
+  const float4 g = coarse[(x >> 1) + (size_t)(y >> 1) * cp];
+  const float s = 0.25f * (float)(mx * my);
+  float4 &f = fine[x + (size_t)y * fp];
+  f.x += g.x * s;
+  f.y += g.y * s;
+  f.z += g.z * s;
+  f.w += g.w * s;
+}
+
+template __global__ void backward_kernel<1>(DrawParams, GradParams, const TriRecord *);
+template __global__ void backward_kernel<4>(DrawParams, GradParams, const TriRecord *);
+
+}  // namespace swr

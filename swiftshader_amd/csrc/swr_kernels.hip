@@ -453,6 +453,7 @@ struct PixelState {
   uint32_t sb[MS];
   uint32_t cb[FCOL ? 1 : MS];
   float4 cf[FCOL ? MS : 1];
+  uint32_t id[FCOL ? MS : 1];  // primitive_id record (float targets only)
 };
 
 // Pixel of lane `l` inside its 8x8 block: lanes 4k..4k+3 form the 2x2 quad
@@ -475,6 +476,25 @@ __device__ __forceinline__ float quad_rhw(const TriRecord &R, float xf, float yf
   return 1.0f / (xf * R.wA + Dw);
 }
 
+// Texture LOD state of the 2x2 quad holding pixel (px, py): the lane
+// differences of u, v over the quad's pixels (0,0), (1,0), (0,1), helper
+// lanes included, evaluated from the plane equations (computeLod2D,
+// SamplerCore.cpp:1376-1422).
+__device__ __forceinline__ QuadLod quad_state(const DrawParams &P, const TriRecord &R, int px, int py) {
+  const int qx = px & ~1, qy = py & ~1;
+  float uq[4], vq[4];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    const float xi = (float)(qx + (i & 1)) - R.x0, yi = (float)(qy + (i >> 1)) - R.y0;
+    const float ri = quad_rhw(R, xi, yi);
+    uq[i] = interp(R, 4, xi, yi, ri);
+    vq[i] = interp(R, 5, xi, yi, ri);
+  }
+  uq[3] = 0.0f;
+  vq[3] = 0.0f;
+  return quad_lod(P.tex, uq, vq);
+}
+
 // Shades one queued fragment: the fixed shader family at pixel (px, py)
 // of primitive R, with the texture LOD of the 2x2 quad the pixel belongs
 // to (helper lanes evaluated from the plane equations).  Output clamped
@@ -491,19 +511,8 @@ __device__ __forceinline__ float4 shade(const DrawParams &P, const TriRecord &R,
     col.w = interp(R, 3, xf, yf, rhw);
   }
   if (P.shader != SHADER_INTERPOLATE) {
-    const int qx = px & ~1, qy = py & ~1;
-    float uq[4], vq[4];
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-      const float xi = (float)(qx + (i & 1)) - R.x0, yi = (float)(qy + (i >> 1)) - R.y0;
-      const float ri = quad_rhw(R, xi, yi);
-      uq[i] = interp(R, 4, xi, yi, ri);
-      vq[i] = interp(R, 5, xi, yi, ri);
-    }
     const float u = interp(R, 4, xf, yf, rhw), v = interp(R, 5, xf, yf, rhw);
-    uq[3] = u;
-    vq[3] = v;
-    const QuadLod Lq = quad_lod(P.tex, uq, vq);
+    const QuadLod Lq = quad_state(P, R, px, py);
     float4 c = sample_lane<FTEX>(P.tex, u, v, Lq);
     if (P.shader == SHADER_MODULATE) {
       c.x = c.x * col.x;
@@ -525,7 +534,8 @@ __device__ __forceinline__ float4 shade(const DrawParams &P, const TriRecord &R,
 // Blend a shaded colour into the samples of this lane's pixel selected by
 // xMask, then write (PixelRoutine.cpp:1617-2936).
 template <int MS, bool FCOL>
-__device__ __forceinline__ void blend_write(const DrawParams &P, PixelState<MS, FCOL> &S, float4 s, int xMask) {
+__device__ __forceinline__ void blend_write(const DrawParams &P, PixelState<MS, FCOL> &S, float4 s, int xMask,
+                                            uint32_t tri) {
 #pragma unroll
   for (int q = 0; q < MS; q++) {
     if (!((xMask >> q) & 1)) continue;
@@ -544,6 +554,7 @@ __device__ __forceinline__ void blend_write(const DrawParams &P, PixelState<MS, 
       if (P.colorMask & 2) c.y = o.y;
       if (P.colorMask & 4) c.z = o.z;
       if (P.colorMask & 8) c.w = o.w;
+      S.id[FCOL ? q : 0] = tri + 1u;
     } else {
       uint32_t p = S.cb[FCOL ? 0 : q];
       if (P.colorMask & 1) p = (p & 0xFFFFFF00u) | float_to_unorm8(o.x);
@@ -578,9 +589,12 @@ __global__ __launch_bounds__(256) void raster_kernel(DrawParams P, const TriReco
   const int px = ox + cx, py = oy + cy;
   const int myBit = cy * 8 + cx;
 
-  const uint32_t start = tileStart[bin], end = tileEnd[bin];
-  if (start >= end) return;
-  if (ox >= P.sx1 || oy >= P.sy1 || ox + 8 <= P.sx0 || oy + 8 <= P.sy0) return;
+  const uint32_t start = tileStart[bin];
+  if (start >= tileEnd[bin]) return;  // uniform over the workgroup
+  // A block outside the scissor walks no list (no early return: the
+  // workgroup reduces its counters together at the end).
+  const bool waveActive = !(ox >= P.sx1 || oy >= P.sy1 || ox + 8 <= P.sx0 || oy + 8 <= P.sy0);
+  const uint32_t end = waveActive ? tileEnd[bin] : start;
 
   uint64_t scissorMask = 0;
   {
@@ -601,7 +615,11 @@ __global__ __launch_bounds__(256) void raster_kernel(DrawParams P, const TriReco
     S.sb[q] = 0;
     if (!FCOL) S.cb[FCOL ? 0 : q] = 0;
     if (FCOL) S.cf[FCOL ? q : 0] = make_float4(0, 0, 0, 0);
+    if (FCOL) S.id[FCOL ? q : 0] = 0;
     if (!inside) continue;
+    if (FCOL && P.primId)
+      S.id[FCOL ? q : 0] =
+          *reinterpret_cast<const uint32_t *>(P.primId + q * P.primIdSlice + (long long)py * P.primIdPitch + 4ll * px);
     if (P.depthTest)
       S.zb[q] = *reinterpret_cast<const float *>(P.depth + q * P.depthSlice + (long long)py * P.depthPitch + 4ll * px);
     if (P.stencilActive) S.sb[q] = (uint8_t)P.stencil[q * P.stencilSlice + (long long)py * P.stencilPitch + px];
@@ -641,7 +659,7 @@ __global__ __launch_bounds__(256) void raster_kernel(DrawParams P, const TriReco
     while (m) {
       const int k = __builtin_ctzll(m);
       m &= m - 1;
-      blend_write<MS, FCOL>(P, S, qCol[wave][k], (int)(qPix[wave][k] >> 8));
+      blend_write<MS, FCOL>(P, S, qCol[wave][k], (int)(qPix[wave][k] >> 8), qTri[wave][k]);
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
@@ -840,6 +858,9 @@ __global__ __launch_bounds__(256) void raster_kernel(DrawParams P, const TriReco
       if (P.depthWrite)
         *reinterpret_cast<float *>(P.depth + q * P.depthSlice + (long long)py * P.depthPitch + 4ll * px) = S.zb[q];
       if (P.stencilWrite) P.stencil[q * P.stencilSlice + (long long)py * P.stencilPitch + px] = (char)S.sb[q];
+      if (FCOL && P.primId)
+        *reinterpret_cast<uint32_t *>(P.primId + q * P.primIdSlice + (long long)py * P.primIdPitch + 4ll * px) =
+            S.id[FCOL ? q : 0];
       if (P.colorMask) {
         if (FCOL)
           *reinterpret_cast<float4 *>(P.color + q * P.colorSlice + (long long)py * P.colorPitch + 16ll * px) =
@@ -851,22 +872,27 @@ __global__ __launch_bounds__(256) void raster_kernel(DrawParams P, const TriReco
     }
   }
 
-  if (P.occlusion) {
-    unsigned long long v = occl;
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if (lane == 0 && v) atomicAdd(P.occlusion, v);
-  }
-  if (P.stats) {
-    unsigned long long a = nTests, b = nCovers, c = nShaded;
-    for (int off = 32; off > 0; off >>= 1) {
-      a += __shfl_xor(a, off, 64);
-      b += __shfl_xor(b, off, 64);
-      c += __shfl_xor(c, off, 64);
-    }
+  // Counters: reduce over the wave, then over the workgroup, so that one
+  // atomic per counter and workgroup reaches the (single) global address.
+  __shared__ unsigned long long red[4][4];
+  if (P.occlusion || P.stats) {
+    unsigned long long v[4] = {occl, nTests, nCovers, nShaded};
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
     if (lane == 0) {
-      atomicAdd(P.stats + 0, a);
-      atomicAdd(P.stats + 1, b);
-      atomicAdd(P.stats + 2, c);
+#pragma unroll
+      for (int k = 0; k < 4; k++) red[wave][k] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+      const int k = threadIdx.x;
+      const unsigned long long t = red[0][k] + red[1][k] + red[2][k] + red[3][k];
+      if (k == 0) {
+        if (P.occlusion && t) atomicAdd(P.occlusion, t);
+      } else if (P.stats) {
+        atomicAdd(P.stats + (k - 1), t);
+      }
     }
   }
 }

@@ -80,6 +80,9 @@ struct DrawParams {
   char *stencil;
   int stencilPitch;
   long long stencilSlice;
+  char *primId;  // R32_UINT visibility record (RGBA32F targets only) or NULL
+  int primIdPitch;
+  long long primIdSlice;
   unsigned long long *occlusion;
   unsigned long long *stats;  // [block tests, block covers, shaded] or NULL
   // Tiling.

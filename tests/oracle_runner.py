@@ -79,7 +79,15 @@ def host_targets(draw):
     return color, depth, stencil
 
 
-def oracle_render(draw, targets=None):
+def oracle_render_ids(draw):
+    """Renders `draw` (RGBA32F target) with the primitive_id record; returns
+    (color, depth, ids (S, H, W) uint32, occlusion)."""
+    ids = np.zeros((draw.samples, draw.height, draw.width), np.uint32)
+    color, depth, _, occ = oracle_render(draw, ids=ids)
+    return color, depth, ids, occ
+
+
+def oracle_render(draw, targets=None, ids=None):
     """Renders `draw` with the oracle; returns (color, depth, stencil, occlusion)."""
     L = oracle()
     color, depth, stencil = targets if targets is not None else host_targets(draw)
@@ -114,6 +122,8 @@ def oracle_render(draw, targets=None):
     occ = np.zeros(1, np.uint64)
     buf.occlusion = occ.ctypes.data
     desc = build_desc(draw, buf)
+    if ids is not None:
+        desc.primitive_id = abi.Attachment(ids.ctypes.data, abi.FORMAT_R32_UINT, W * 4, H * W * 4)
     rc = L.swo_draw(ctypes.byref(desc))
     if rc != 0:
         raise RuntimeError(f"oracle swo_draw failed: {rc}")

@@ -42,7 +42,7 @@ def test_struct_sizes_match_the_c_layout():
     assert ctypes.sizeof(abi.ImageLevel) == 24
     assert ctypes.sizeof(abi.Texture) == 8 + 15 * 24
     assert ctypes.sizeof(abi.Attachment) == 24
-    assert abi.DrawDesc.occlusion_count.offset + 8 == ctypes.sizeof(abi.DrawDesc)
+    assert abi.DrawDesc.primitive_id.offset + ctypes.sizeof(abi.Attachment) == ctypes.sizeof(abi.DrawDesc)
 
 
 def test_struct_layout_against_the_header(tmp_path):
@@ -52,17 +52,23 @@ def test_struct_layout_against_the_header(tmp_path):
     cc = shutil.which("gcc")
     if cc is None:
         pytest.skip("gcc not available")
-    fields = [f[0] for f in abi.DrawDesc._fields_]
+    structs = [("swr_draw_desc", abi.DrawDesc), ("swr_grad_desc", abi.GradDesc), ("swr_texture", abi.Texture),
+               ("swr_resolve_desc", abi.ResolveDesc), ("swr_profile", abi.Profile)]
     src = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
-    src += [f'printf("%zu\\n", offsetof(swr_draw_desc, {f}));' for f in fields]
-    src += ['printf("%zu\\n", sizeof(swr_draw_desc)); return 0; }']
+    want = []
+    for cname, py in structs:
+        for f in py._fields_:
+            src.append(f'printf("%zu\\n", offsetof({cname}, {f[0]}));')
+            want.append(getattr(py, f[0]).offset)
+        src.append(f'printf("%zu\\n", sizeof({cname}));')
+        want.append(ctypes.sizeof(py))
+    src.append("return 0; }")
     c = tmp_path / "probe.c"
     c.write_text("\n".join(src))
     exe = tmp_path / "probe"
     subprocess.run([cc, str(c), "-o", str(exe)], check=True)
     out = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
-    assert out[:-1] == [getattr(abi.DrawDesc, f).offset for f in fields]
-    assert out[-1] == ctypes.sizeof(abi.DrawDesc)
+    assert out == want
 
 
 def test_version_and_error_paths_without_gpu():
@@ -75,3 +81,5 @@ def test_version_and_error_paths_without_gpu():
     assert isinstance(lib.swr_last_error(), bytes)
     assert lib.swr_set_tuning(None) < 0
     assert lib.swr_resolve(None, None) < 0
+    assert lib.swr_generate_mips(None, None) < 0
+    assert lib.swr_mip_backward(None, None) < 0
