@@ -269,9 +269,24 @@ typedef struct swr_grad_desc {
 
 /* Tuning knobs of the tiled rasteriser (0 = library default). */
 typedef struct swr_tuning {
-  uint32_t tile_width, tile_height; /* pixels per screen tile */
-  uint32_t reserved[6];
+  uint32_t tile_width, tile_height;    /* pixels per screen tile: 8 or 16 each */
+  uint32_t raster_workgroups_per_cu;   /* persistent raster workgroups per CU;
+                                          ~0u = one workgroup per tile instead */
+  uint32_t reserved[5];
 } swr_tuning;
+
+/* A batch of draws prepared once and enqueued many times (no counterpart
+ * in the reference, whose draws are enqueued one by one): the descriptors
+ * are validated and their derived state uploaded, the tiled pipeline's
+ * scratch is sized from a first setup pass (the (tile, primitive) pair
+ * count, plus the given headroom), and every later run enqueues a fixed
+ * sequence of kernels that allocates nothing and never waits on the host --
+ * so a run can be captured into a HIP graph and replayed.  The memory the
+ * descriptors point to is read at run time: vertex and texel contents and
+ * attachments may change between runs, triangle counts and sizes may not.
+ * A run whose geometry needs more pairs than planned renders incompletely
+ * and is reported by swr_batch_plan_check. */
+typedef struct swr_batch_plan swr_batch_plan;
 
 /* Returns SWR_ABI_VERSION. */
 int swr_abi_version(void);
@@ -297,6 +312,28 @@ int swr_draw_batch(const swr_draw_desc *descs, uint32_t count, void *stream);
 
 /* MSAA colour resolve (sw::Blitter::resolve, src/Device/Blitter.cpp:2035). */
 int swr_resolve(const swr_resolve_desc *desc, void *stream);
+
+/* Prepares `count` draws (as swr_draw_batch would render them) for
+ * repeated runs; pair_headroom_percent (e.g. 10) over-provisions the pair
+ * buffers beyond the planned geometry.  Synchronises `stream`. */
+int swr_batch_plan_create(const swr_draw_desc *descs, uint32_t count, uint32_t pair_headroom_percent,
+                          swr_batch_plan **plan, void *stream);
+
+/* Enqueues the planned draws on `stream` (no allocation, no host wait). */
+int swr_batch_plan_run(swr_batch_plan *plan, void *stream);
+
+/* Waits for the device and reports the largest pair count any run needed
+ * beyond the plan's capacity (0: every run fit). */
+int swr_batch_plan_check(swr_batch_plan *plan, uint32_t *pairs_needed);
+
+/* Times the plan under candidate tunings (tile size, raster workgroups per
+ * CU), keeps the fastest and returns it in *best (may be NULL).  Renders
+ * the batch several times: the attachments hold its last rendering.
+ * Synchronises `stream`. */
+int swr_batch_plan_autotune(swr_batch_plan *plan, swr_tuning *best, void *stream);
+
+/* Frees the plan (waits for the device first). */
+int swr_batch_plan_destroy(swr_batch_plan *plan);
 
 /* Gradients of the draw `desc` (see swr_grad_desc).  Asynchronous on
  * `stream`; the forward draw must have completed on the same stream. */

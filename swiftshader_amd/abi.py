@@ -126,7 +126,8 @@ class GradDesc(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("tile_width", C.c_uint32), ("tile_height", C.c_uint32), ("reserved", C.c_uint32 * 6)]
+    _fields_ = [("tile_width", C.c_uint32), ("tile_height", C.c_uint32), ("raster_workgroups_per_cu", C.c_uint32),
+                ("reserved", C.c_uint32 * 5)]
 
 
 class Profile(C.Structure):
@@ -138,7 +139,8 @@ class Profile(C.Structure):
 # Symbols declared by include/swr.h, in declaration order.
 EXPORTED_SYMBOLS = (
     "swr_abi_version", "swr_last_error", "swr_init", "swr_release", "swr_draw",
-    "swr_draw_batch", "swr_resolve", "swr_draw_backward", "swr_generate_mips", "swr_mip_backward",
+    "swr_draw_batch", "swr_resolve", "swr_batch_plan_create", "swr_batch_plan_run", "swr_batch_plan_check",
+    "swr_batch_plan_autotune", "swr_batch_plan_destroy", "swr_draw_backward", "swr_generate_mips", "swr_mip_backward",
     "swr_profile_enable", "swr_profile_read", "swr_set_tuning", "swr_get_tuning",
 )
 
@@ -156,6 +158,17 @@ def bind(lib):
     lib.swr_draw_batch.restype = C.c_int
     lib.swr_resolve.argtypes = [C.POINTER(ResolveDesc), C.c_void_p]
     lib.swr_resolve.restype = C.c_int
+    lib.swr_batch_plan_create.argtypes = [C.POINTER(DrawDesc), C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p),
+                                          C.c_void_p]
+    lib.swr_batch_plan_create.restype = C.c_int
+    lib.swr_batch_plan_run.argtypes = [C.c_void_p, C.c_void_p]
+    lib.swr_batch_plan_run.restype = C.c_int
+    lib.swr_batch_plan_check.argtypes = [C.c_void_p, C.POINTER(C.c_uint32)]
+    lib.swr_batch_plan_check.restype = C.c_int
+    lib.swr_batch_plan_autotune.argtypes = [C.c_void_p, C.POINTER(Tuning), C.c_void_p]
+    lib.swr_batch_plan_autotune.restype = C.c_int
+    lib.swr_batch_plan_destroy.argtypes = [C.c_void_p]
+    lib.swr_batch_plan_destroy.restype = C.c_int
     lib.swr_draw_backward.argtypes = [C.POINTER(DrawDesc), C.POINTER(GradDesc), C.c_void_p]
     lib.swr_draw_backward.restype = C.c_int
     lib.swr_generate_mips.argtypes = [C.POINTER(Texture), C.c_void_p]

@@ -20,6 +20,7 @@
 
 #include "../../include/swr.h"
 #include "swr_kernels.hip"
+#include "swr_batch.hip"
 #include "swr_grad.hip"
 
 using namespace swr;
@@ -75,12 +76,32 @@ enum { EV_START, EV_SETUP, EV_BIN, EV_RASTER, EV_COUNT };
 struct EventSet {
   hipEvent_t ev[EV_COUNT];
   uint64_t pairs;
+  uint64_t draws;
+};
+
+// Device scratch of the tiled pipeline for one segment at a time.
+struct Pipeline {
+  Scratch params, aux;                      // DrawParams[n]; blockBase[n], tileBase[n]
+  Scratch rec, poly, cnt, off, sums;        // per primitive slot
+  Scratch keysA, keysB, valsA, valsB, hist; // per pair
+  Scratch tStart, tEnd;                     // per tile
+  Scratch counters;                         // work counters (per variant), overflow flags
+  void release() {
+    for (Scratch *s : {&params, &aux, &rec, &poly, &cnt, &off, &sums, &keysA, &keysB, &valsA, &valsB, &hist, &tStart,
+                       &tEnd, &counters})
+      s->release();
+  }
 };
 
 struct DeviceCtx {
   int device = -1;
-  Scratch rec, poly, cnt, off, keysA, keysB, valsA, valsB, scanTmp, sortTmp, tStart, tEnd, stats;
+  int cus = 256;
+  Pipeline pipe;
+  Scratch stats;
   uint32_t *hostTotal = nullptr;  // pinned
+  std::vector<char> staging;      // host image of the params upload
+  void *pinned = nullptr;         // pinned staging buffer for the params upload
+  size_t pinnedBytes = 0;
   swr_tuning tuning{};
   bool profiling = false;
   std::vector<EventSet> pending, pool;
@@ -113,6 +134,10 @@ int current_ctx(DeviceCtx **out) {
     c->device = dev;
     c->tuning.tile_width = 16;
     c->tuning.tile_height = 16;
+    c->tuning.raster_workgroups_per_cu = 4;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
+      c->cus = prop.multiProcessorCount;
     hipError_t e = hipHostMalloc((void **)&c->hostTotal, sizeof(uint32_t) * 4, hipHostMallocDefault);
     if (e != hipSuccess) {
       delete c;
@@ -393,104 +418,286 @@ int build_params(const swr_draw_desc *d, const swr_tuning &tuning, DrawParams &P
   return SWR_OK;
 }
 
-template <int MS, bool FCOL, bool FTEX>
-void launch_raster(const DrawParams &P, dim3 grid, dim3 block, hipStream_t s, const TriRecord *rec, const int *poly,
-                   const uint32_t *bin, const uint32_t *ts, const uint32_t *te) {
-  hipLaunchKernelGGL((raster_kernel<MS, FCOL, FTEX>), grid, block, 0, s, P, rec, poly, bin, ts, te);
+// ---------------------------------------------------------------------------
+// Batched pipeline
+// ---------------------------------------------------------------------------
+// Raster variant of a draw: samples, float colour target, float texture.
+int variant_of(const DrawParams &P) {
+  const bool ftex = P.useUV && P.tex.isFloat;
+  return (P.ms > 1 ? 4 : 0) | (P.colorFloat ? 2 : 0) | (ftex ? 1 : 0);
 }
 
-int draw_one(DeviceCtx *c, const swr_draw_desc *desc, hipStream_t s) {
-  DrawParams P;
-  int r = build_params(desc, c->tuning, P);
+// Host plan of a batch: per-draw state in enqueue order, split into
+// segments of draws that share no attachment (so that the draws of a
+// segment can be rendered concurrently), each segment's draws grouped by
+// raster variant.
+struct BatchLayout {
+  struct Seg {
+    int first = 0, count = 0;  // draws [first, first + count) of P
+    uint32_t T = 0, blocks = 0, tiles = 0;
+    std::vector<std::pair<int, int>> variants;  // [lo, hi) draw ranges (segment-relative)
+    uint32_t pairCap = 0;                       // planned pair capacity (plans only)
+  };
+  std::vector<DrawParams> P;
+  std::vector<uint32_t> blockBase, tileBase;  // per draw, segment-relative
+  std::vector<Seg> segs;
+};
+
+bool shares_attachment(const DrawParams &a, const DrawParams &b) {
+  auto same = [](const void *x, const void *y) { return x && x == y; };
+  return same(a.color, b.color) || same(a.depth, b.depth) || same(a.stencil, b.stencil) || same(a.primId, b.primId);
+}
+
+int layout_batch(const swr_draw_desc *descs, uint32_t n, const swr_tuning &tuning, BatchLayout &L) {
+  L = BatchLayout{};
+  std::vector<DrawParams> all(n);
+  for (uint32_t i = 0; i < n; i++) {
+    int r = build_params(&descs[i], tuning, all[i]);
+    if (r != SWR_OK) return fail(r, "draw %u: %s", i, g_error.c_str());
+  }
+  uint32_t i = 0;
+  while (i < n) {
+    // Extend the segment while the next draw shares no attachment with it.
+    uint32_t j = i + 1;
+    while (j < n) {
+      bool clash = false;
+      for (uint32_t k = i; k < j && !clash; k++) clash = shares_attachment(all[k], all[j]);
+      if (clash) break;
+      j++;
+    }
+    std::vector<int> order;
+    for (uint32_t k = i; k < j; k++)
+      if (all[k].triCount > 0) order.push_back((int)k);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return variant_of(all[a]) < variant_of(all[b]); });
+    BatchLayout::Seg sg;
+    sg.first = (int)L.P.size();
+    for (size_t k = 0; k < order.size(); k++) {
+      DrawParams P = all[order[k]];
+      P.triBase = (int)sg.T;
+      L.blockBase.push_back(sg.blocks);
+      L.tileBase.push_back(sg.tiles);
+      sg.T += (uint32_t)P.triCount;
+      sg.blocks += (uint32_t)(P.triCount + 255) / 256;
+      sg.tiles += (uint32_t)(P.tilesX * P.tilesY);
+      if (k == 0 || variant_of(P) != variant_of(L.P.back()))
+        sg.variants.push_back({(int)k, (int)k + 1});
+      else
+        sg.variants.back().second = (int)k + 1;
+      L.P.push_back(P);
+    }
+    sg.count = (int)order.size();
+    if (sg.count) L.segs.push_back(sg);
+    i = j;
+  }
+  if (L.P.size() > (1u << 20)) return fail(SWR_ERROR_UNSUPPORTED, "too many draws in one batch");
+  for (const auto &sg : L.segs)
+    if ((uint64_t)sg.T > (1u << 28) || (uint64_t)sg.tiles > (1u << 30))
+      return fail(SWR_ERROR_UNSUPPORTED, "batch segment too large");
+  return SWR_OK;
+}
+
+// Uploads the per-draw params and the block / tile bases of a layout into
+// `pipe` (through the context's pinned staging buffer).
+int upload_layout(DeviceCtx *c, Pipeline &pipe, const BatchLayout &L, hipStream_t s) {
+  const size_t n = L.P.size();
+  const size_t pb = sizeof(DrawParams) * n, ab = sizeof(uint32_t) * 2 * n;
+  if (!n) return SWR_OK;
+  SWR_HIP_CHECK(pipe.params.ensure(pb));
+  SWR_HIP_CHECK(pipe.aux.ensure(ab));
+  if (c->pinnedBytes < pb + ab) {
+    SWR_HIP_CHECK(hipStreamSynchronize(s));
+    if (c->pinned) SWR_HIP_CHECK(hipHostFree(c->pinned));
+    c->pinned = nullptr;
+    c->pinnedBytes = 0;
+    SWR_HIP_CHECK(hipHostMalloc(&c->pinned, pb + ab, hipHostMallocDefault));
+    c->pinnedBytes = pb + ab;
+  }
+  char *h = (char *)c->pinned;
+  memcpy(h, L.P.data(), pb);
+  memcpy(h + pb, L.blockBase.data(), sizeof(uint32_t) * n);
+  memcpy(h + pb + sizeof(uint32_t) * n, L.tileBase.data(), sizeof(uint32_t) * n);
+  SWR_HIP_CHECK(hipMemcpyAsync(pipe.params.ptr, h, pb, hipMemcpyHostToDevice, s));
+  SWR_HIP_CHECK(hipMemcpyAsync(pipe.aux.ptr, h + pb, ab, hipMemcpyHostToDevice, s));
+  // The staging buffer is reused by the next call: wait for the copies.
+  SWR_HIP_CHECK(hipStreamSynchronize(s));
+  return SWR_OK;
+}
+
+uint32_t sort_passes(uint32_t tiles) {
+  int bits = 1;
+  while (bits < 32 && (1ull << bits) < (uint64_t)tiles) bits++;
+  return (uint32_t)((bits + 7) / 8);
+}
+
+// Sizes the per-segment scratch of `pipe` for T slots, `cap` pairs, `tiles` tiles.
+int ensure_pipe(Pipeline &pipe, uint32_t T, uint32_t cap, uint32_t tiles) {
+  const size_t slots = (size_t)T + 1;
+  SWR_HIP_CHECK(pipe.rec.ensure(sizeof(TriRecord) * (size_t)T));
+  SWR_HIP_CHECK(pipe.poly.ensure(sizeof(int) * 2 * kMaxPolygon * (size_t)T));
+  SWR_HIP_CHECK(pipe.cnt.ensure(sizeof(uint32_t) * slots));
+  SWR_HIP_CHECK(pipe.off.ensure(sizeof(uint32_t) * slots));
+  const size_t nb = ((size_t)cap + kSortItems - 1) / kSortItems;
+  const size_t scanBlocks = std::max((slots + kScanItems - 1) / kScanItems, (256 * nb + kScanItems - 1) / kScanItems);
+  SWR_HIP_CHECK(pipe.sums.ensure(sizeof(uint32_t) * std::max<size_t>(scanBlocks, 1)));
+  const size_t pc = std::max<size_t>(cap, 1);
+  SWR_HIP_CHECK(pipe.keysA.ensure(sizeof(uint32_t) * pc));
+  SWR_HIP_CHECK(pipe.keysB.ensure(sizeof(uint32_t) * pc));
+  SWR_HIP_CHECK(pipe.valsA.ensure(sizeof(uint32_t) * pc));
+  SWR_HIP_CHECK(pipe.valsB.ensure(sizeof(uint32_t) * pc));
+  SWR_HIP_CHECK(pipe.hist.ensure(sizeof(uint32_t) * 256 * std::max<size_t>(nb, 1)));
+  SWR_HIP_CHECK(pipe.tStart.ensure(sizeof(uint32_t) * std::max<uint32_t>(tiles, 1)));
+  SWR_HIP_CHECK(pipe.tEnd.ensure(sizeof(uint32_t) * std::max<uint32_t>(tiles, 1)));
+  SWR_HIP_CHECK(pipe.counters.ensure(sizeof(uint32_t) * 16));
+  return SWR_OK;
+}
+
+// Exclusive prefix sum of n u32 (in may equal out).
+void enqueue_scan(const uint32_t *in, uint32_t *out, size_t n, uint32_t *sums, hipStream_t s) {
+  const uint32_t blocks = (uint32_t)((n + kScanItems - 1) / kScanItems);
+  hipLaunchKernelGGL(scan_partial_kernel, dim3(blocks), dim3(256), 0, s, in, out, (uint32_t)n, sums);
+  hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(1024), 0, s, sums, blocks);
+  hipLaunchKernelGGL(scan_add_kernel, dim3(blocks), dim3(256), 0, s, out, (uint32_t)n, (const uint32_t *)sums);
+}
+
+template <int MS, bool FCOL, bool FTEX>
+void launch_raster_batch(dim3 grid, dim3 block, hipStream_t s, const DrawParams *params, const uint32_t *tileBase,
+                         int lo, int hi, uint32_t tLo, uint32_t tHi, uint32_t *counter, const TriRecord *rec,
+                         const int *poly, const uint32_t *bin, const uint32_t *ts, const uint32_t *te) {
+  hipLaunchKernelGGL((raster_batch_kernel<MS, FCOL, FTEX>), grid, block, 0, s, params, tileBase, lo, hi, tLo, tHi,
+                     counter, rec, poly, bin, ts, te);
+}
+
+// Setup and tile-count prefix sum of one segment (off[T] = pair count).
+void enqueue_front(Pipeline &pipe, const BatchLayout &L, const BatchLayout::Seg &sg, hipStream_t s) {
+  const DrawParams *params = pipe.params.as<DrawParams>() + sg.first;
+  const uint32_t *blockBase = pipe.aux.as<uint32_t>() + sg.first;
+  hipLaunchKernelGGL(setup_batch_kernel, dim3(sg.blocks), dim3(256), 0, s, params, blockBase, sg.count,
+                     pipe.rec.as<TriRecord>(), pipe.poly.as<int>(), pipe.cnt.as<uint32_t>());
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(1), 0, s, pipe.cnt.as<uint32_t>() + sg.T, 0u, 1u);
+  enqueue_scan(pipe.cnt.as<uint32_t>(), pipe.off.as<uint32_t>(), (size_t)sg.T + 1, pipe.sums.as<uint32_t>(), s);
+}
+
+// Pair emission, sort, ranges and raster of one segment with pair capacity
+// `cap`; overflow (if any) is reported into *overflowFlag.
+int enqueue_back(DeviceCtx *c, Pipeline &pipe, const BatchLayout &L, const BatchLayout::Seg &sg, uint32_t cap,
+                 const swr_tuning &tuning, uint32_t *overflowFlag, EventSet *es, hipStream_t s) {
+  const DrawParams *params = pipe.params.as<DrawParams>() + sg.first;
+  const uint32_t *tileBase = pipe.aux.as<uint32_t>() + L.P.size() + sg.first;
+  if (cap > 0) {
+    hipLaunchKernelGGL(bin_emit_batch_kernel, dim3((cap + 255) / 256), dim3(256), 0, s, params, tileBase,
+                       (const TriRecord *)pipe.rec.as<TriRecord>(), (const uint32_t *)pipe.off.as<uint32_t>(), sg.T,
+                       cap, pipe.keysA.as<uint32_t>(), pipe.valsA.as<uint32_t>(), overflowFlag);
+    const uint32_t nb = (cap + kSortItems - 1) / kSortItems;
+    const uint32_t passes = sort_passes(sg.tiles);
+    uint32_t *kIn = pipe.keysA.as<uint32_t>(), *vIn = pipe.valsA.as<uint32_t>();
+    uint32_t *kOut = pipe.keysB.as<uint32_t>(), *vOut = pipe.valsB.as<uint32_t>();
+    for (uint32_t p = 0; p < passes; p++) {
+      hipLaunchKernelGGL(radix_hist_kernel, dim3(nb), dim3(256), 0, s, (const uint32_t *)kIn, cap, (int)(8 * p),
+                         pipe.hist.as<uint32_t>(), nb);
+      enqueue_scan(pipe.hist.as<uint32_t>(), pipe.hist.as<uint32_t>(), (size_t)256 * nb, pipe.sums.as<uint32_t>(), s);
+      hipLaunchKernelGGL(radix_scatter_kernel, dim3(nb), dim3(64), 0, s, (const uint32_t *)kIn, (const uint32_t *)vIn,
+                         cap, (int)(8 * p), (const uint32_t *)pipe.hist.as<uint32_t>(), nb, kOut, vOut);
+      std::swap(kIn, kOut);
+      std::swap(vIn, vOut);
+    }
+    const uint32_t tb = (sg.tiles + 255) / 256;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(tb), dim3(256), 0, s, pipe.tStart.as<uint32_t>(), 0u, sg.tiles);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(tb), dim3(256), 0, s, pipe.tEnd.as<uint32_t>(), 0u, sg.tiles);
+    hipLaunchKernelGGL(tile_range_batch_kernel, dim3((cap + 255) / 256), dim3(256), 0, s, (const uint32_t *)kIn, cap,
+                       sg.tiles, pipe.tStart.as<uint32_t>(), pipe.tEnd.as<uint32_t>());
+    SWR_HIP_CHECK(hipGetLastError());
+    if (es) SWR_HIP_CHECK(hipEventRecord(es->ev[EV_BIN], s));
+
+    const uint32_t *bin = vIn;
+    const int wpt = (int)((tuning.tile_width / 8) * (tuning.tile_height / 8));
+    dim3 block(64 * wpt);
+    uint32_t *counters = pipe.counters.as<uint32_t>();
+    const bool persistent = tuning.raster_workgroups_per_cu != ~0u;
+    if (persistent)
+      hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, counters, 0u, (uint32_t)sg.variants.size());
+    for (size_t v = 0; v < sg.variants.size(); v++) {
+      const int lo = sg.variants[v].first, hi = sg.variants[v].second;
+      const uint32_t tLo = L.tileBase[sg.first + lo];
+      const uint32_t tHi = hi < sg.count ? L.tileBase[sg.first + hi] : sg.tiles;
+      if (tHi <= tLo) continue;
+      uint32_t wgs = tHi - tLo;
+      if (persistent) {
+        const uint32_t per = tuning.raster_workgroups_per_cu ? tuning.raster_workgroups_per_cu : 4u;
+        wgs = std::min<uint32_t>(wgs, (uint32_t)c->cus * per);
+      }
+      dim3 grid(wgs);
+      uint32_t *ctr = persistent ? counters + v : nullptr;
+      const TriRecord *rec = pipe.rec.as<TriRecord>();
+      const int *poly = pipe.poly.as<int>();
+      const uint32_t *ts = pipe.tStart.as<uint32_t>(), *te = pipe.tEnd.as<uint32_t>();
+      switch (variant_of(L.P[sg.first + lo])) {
+        case 0: launch_raster_batch<1, false, false>(grid, block, s, params, tileBase, lo, hi, tLo, tHi, ctr, rec, poly, bin, ts, te); break;
+        case 1: launch_raster_batch<1, false, true>(grid, block, s, params, tileBase, lo, hi, tLo, tHi, ctr, rec, poly, bin, ts, te); break;
+        case 2: launch_raster_batch<1, true, false>(grid, block, s, params, tileBase, lo, hi, tLo, tHi, ctr, rec, poly, bin, ts, te); break;
+        case 3: launch_raster_batch<1, true, true>(grid, block, s, params, tileBase, lo, hi, tLo, tHi, ctr, rec, poly, bin, ts, te); break;
+        case 4: launch_raster_batch<4, false, false>(grid, block, s, params, tileBase, lo, hi, tLo, tHi, ctr, rec, poly, bin, ts, te); break;
+        case 5: launch_raster_batch<4, false, true>(grid, block, s, params, tileBase, lo, hi, tLo, tHi, ctr, rec, poly, bin, ts, te); break;
+        case 6: launch_raster_batch<4, true, false>(grid, block, s, params, tileBase, lo, hi, tLo, tHi, ctr, rec, poly, bin, ts, te); break;
+        default: launch_raster_batch<4, true, true>(grid, block, s, params, tileBase, lo, hi, tLo, tHi, ctr, rec, poly, bin, ts, te); break;
+      }
+    }
+  } else if (es) {
+    SWR_HIP_CHECK(hipEventRecord(es->ev[EV_BIN], s));
+  }
+  SWR_HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}
+
+int check_tuning(const swr_tuning &t) {
+  if ((t.tile_width != 8 && t.tile_width != 16) || (t.tile_height != 8 && t.tile_height != 16))
+    return fail(SWR_ERROR_UNSUPPORTED, "tile %ux%u (8 or 16 pixels per side)", t.tile_width, t.tile_height);
+  if (t.raster_workgroups_per_cu != ~0u && t.raster_workgroups_per_cu > 64)
+    return fail(SWR_ERROR_INVALID_ARGUMENT, "raster_workgroups_per_cu %u", t.raster_workgroups_per_cu);
+  return SWR_OK;
+}
+
+// Eager draws: each segment reads its pair count back before sizing the
+// pair buffers (one host wait per segment).
+int draw_eager(DeviceCtx *c, const swr_draw_desc *descs, uint32_t n, hipStream_t s) {
+  BatchLayout L;
+  swr_tuning tuning = c->tuning;
+  int r = layout_batch(descs, n, tuning, L);
   if (r != SWR_OK) return r;
-  const uint32_t T = desc->triangle_count;
-  if (T == 0) return SWR_OK;
-  const uint32_t tiles = (uint32_t)(P.tilesX * P.tilesY);
-
-  SWR_HIP_CHECK(c->rec.ensure(sizeof(TriRecord) * (size_t)T));
-  SWR_HIP_CHECK(c->poly.ensure(sizeof(int) * 2 * kMaxPolygon * (size_t)T));
-  SWR_HIP_CHECK(c->cnt.ensure(sizeof(uint32_t) * ((size_t)T + 1)));
-  SWR_HIP_CHECK(c->off.ensure(sizeof(uint32_t) * ((size_t)T + 1)));
-  size_t scanBytes = 0;
-  SWR_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, scanBytes, c->cnt.as<uint32_t>(), c->off.as<uint32_t>(),
-                                                 (int)(T + 1), s));
-  SWR_HIP_CHECK(c->scanTmp.ensure(scanBytes));
-
-  const int threads = 256;
-  EventSet es{};
+  if (L.segs.empty()) return SWR_OK;
   const bool prof = c->profiling;
   if (prof) {
     SWR_HIP_CHECK(c->stats.ensure(4 * sizeof(unsigned long long)));
-    P.stats = c->stats.as<unsigned long long>();
-    SWR_HIP_CHECK(take_events(c, es));
-    SWR_HIP_CHECK(hipEventRecord(es.ev[EV_START], s));
+    for (DrawParams &P : L.P) P.stats = c->stats.as<unsigned long long>();
   }
-  hipLaunchKernelGGL(setup_kernel, dim3((T + threads - 1) / threads), dim3(threads), 0, s, P, c->rec.as<TriRecord>(),
-                     c->poly.as<int>(), c->cnt.as<uint32_t>());
-  hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(1), 0, s, c->cnt.as<uint32_t>() + T, 0u, 1u);
-  SWR_HIP_CHECK(hipGetLastError());
-  if (prof) SWR_HIP_CHECK(hipEventRecord(es.ev[EV_SETUP], s));
-  SWR_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(c->scanTmp.ptr, scanBytes, c->cnt.as<uint32_t>(),
-                                                 c->off.as<uint32_t>(), (int)(T + 1), s));
-  SWR_HIP_CHECK(hipMemcpyAsync(c->hostTotal, c->off.as<uint32_t>() + T, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  SWR_HIP_CHECK(hipStreamSynchronize(s));
-  const uint32_t pairs = *c->hostTotal;
-  if (pairs == 0) {
-    if (prof) c->pool.push_back(es);
-    return SWR_OK;
-  }
-
-  SWR_HIP_CHECK(c->keysA.ensure(sizeof(uint32_t) * (size_t)pairs));
-  SWR_HIP_CHECK(c->keysB.ensure(sizeof(uint32_t) * (size_t)pairs));
-  SWR_HIP_CHECK(c->valsA.ensure(sizeof(uint32_t) * (size_t)pairs));
-  SWR_HIP_CHECK(c->valsB.ensure(sizeof(uint32_t) * (size_t)pairs));
-  SWR_HIP_CHECK(c->tStart.ensure(sizeof(uint32_t) * (size_t)tiles));
-  SWR_HIP_CHECK(c->tEnd.ensure(sizeof(uint32_t) * (size_t)tiles));
-  int bits = 1;
-  while ((1u << bits) < tiles) bits++;
-  size_t sortBytes = 0;
-  SWR_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, sortBytes, c->keysA.as<uint32_t>(), c->keysB.as<uint32_t>(),
-                                                   c->valsA.as<uint32_t>(), c->valsB.as<uint32_t>(), (int)pairs, 0,
-                                                   bits, s));
-  SWR_HIP_CHECK(c->sortTmp.ensure(sortBytes));
-
-  hipLaunchKernelGGL(bin_emit_kernel, dim3((pairs + threads - 1) / threads), dim3(threads), 0, s, P,
-                     c->rec.as<TriRecord>(), c->off.as<uint32_t>(), pairs, c->keysA.as<uint32_t>(),
-                     c->valsA.as<uint32_t>());
-  SWR_HIP_CHECK(hipGetLastError());
-  SWR_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(c->sortTmp.ptr, sortBytes, c->keysA.as<uint32_t>(),
-                                                   c->keysB.as<uint32_t>(), c->valsA.as<uint32_t>(),
-                                                   c->valsB.as<uint32_t>(), (int)pairs, 0, bits, s));
-  hipLaunchKernelGGL(fill_u32_kernel, dim3((tiles + 255) / 256), dim3(256), 0, s, c->tStart.as<uint32_t>(), 0u, tiles);
-  hipLaunchKernelGGL(fill_u32_kernel, dim3((tiles + 255) / 256), dim3(256), 0, s, c->tEnd.as<uint32_t>(), 0u, tiles);
-  hipLaunchKernelGGL(tile_range_kernel, dim3((pairs + 255) / 256), dim3(256), 0, s, c->keysB.as<uint32_t>(), pairs,
-                     c->tStart.as<uint32_t>(), c->tEnd.as<uint32_t>());
-  SWR_HIP_CHECK(hipGetLastError());
-  if (prof) SWR_HIP_CHECK(hipEventRecord(es.ev[EV_BIN], s));
-
-  dim3 grid(tiles), block(256);  // one workgroup per bin, one wave per 8x8 block
-  const TriRecord *rec = c->rec.as<TriRecord>();
-  const int *poly = c->poly.as<int>();
-  const uint32_t *bin = c->valsB.as<uint32_t>();
-  const uint32_t *ts = c->tStart.as<uint32_t>(), *te = c->tEnd.as<uint32_t>();
-  const bool ftex = P.useUV && P.tex.isFloat;
-  const bool fcol = P.colorFloat != 0;
-  if (P.ms == 1) {
-    if (!fcol && !ftex) launch_raster<1, false, false>(P, grid, block, s, rec, poly, bin, ts, te);
-    else if (!fcol && ftex) launch_raster<1, false, true>(P, grid, block, s, rec, poly, bin, ts, te);
-    else if (fcol && !ftex) launch_raster<1, true, false>(P, grid, block, s, rec, poly, bin, ts, te);
-    else launch_raster<1, true, true>(P, grid, block, s, rec, poly, bin, ts, te);
-  } else {
-    if (!fcol && !ftex) launch_raster<4, false, false>(P, grid, block, s, rec, poly, bin, ts, te);
-    else if (!fcol && ftex) launch_raster<4, false, true>(P, grid, block, s, rec, poly, bin, ts, te);
-    else if (fcol && !ftex) launch_raster<4, true, false>(P, grid, block, s, rec, poly, bin, ts, te);
-    else launch_raster<4, true, true>(P, grid, block, s, rec, poly, bin, ts, te);
-  }
-  SWR_HIP_CHECK(hipGetLastError());
-  if (prof) {
-    SWR_HIP_CHECK(hipEventRecord(es.ev[EV_RASTER], s));
-    es.pairs = pairs;
-    c->pending.push_back(es);
+  Pipeline &pipe = c->pipe;
+  r = upload_layout(c, pipe, L, s);
+  if (r != SWR_OK) return r;
+  for (const auto &sg : L.segs) {
+    EventSet es{};
+    if (prof) {
+      SWR_HIP_CHECK(take_events(c, es));
+      SWR_HIP_CHECK(hipEventRecord(es.ev[EV_START], s));
+    }
+    r = ensure_pipe(pipe, sg.T, 0, sg.tiles);
+    if (r != SWR_OK) return r;
+    enqueue_front(pipe, L, sg, s);
+    SWR_HIP_CHECK(hipGetLastError());
+    if (prof) SWR_HIP_CHECK(hipEventRecord(es.ev[EV_SETUP], s));
+    SWR_HIP_CHECK(hipMemcpyAsync(c->hostTotal, pipe.off.as<uint32_t>() + sg.T, sizeof(uint32_t),
+                                 hipMemcpyDeviceToHost, s));
+    SWR_HIP_CHECK(hipStreamSynchronize(s));
+    const uint32_t pairs = *c->hostTotal;
+    r = ensure_pipe(pipe, sg.T, pairs, sg.tiles);
+    if (r != SWR_OK) return r;
+    r = enqueue_back(c, pipe, L, sg, pairs, tuning, pipe.counters.as<uint32_t>() + 8, prof ? &es : nullptr, s);
+    if (r != SWR_OK) return r;
+    if (prof) {
+      SWR_HIP_CHECK(hipEventRecord(es.ev[EV_RASTER], s));
+      es.pairs = pairs;
+      es.draws = (uint64_t)sg.count;
+      c->pending.push_back(es);
+    }
   }
   return SWR_OK;
 }
@@ -530,16 +737,17 @@ int draw_backward(DeviceCtx *c, const swr_draw_desc *desc, const swr_grad_desc *
   }
   const uint32_t T = desc->triangle_count;
   if (T == 0) return SWR_OK;
-  SWR_HIP_CHECK(c->rec.ensure(sizeof(TriRecord) * (size_t)T));
-  SWR_HIP_CHECK(c->poly.ensure(sizeof(int) * 2 * kMaxPolygon * (size_t)T));
-  SWR_HIP_CHECK(c->cnt.ensure(sizeof(uint32_t) * ((size_t)T + 1)));
-  hipLaunchKernelGGL(setup_kernel, dim3((T + 255) / 256), dim3(256), 0, s, P, c->rec.as<TriRecord>(),
-                     c->poly.as<int>(), c->cnt.as<uint32_t>());
+  Pipeline &pipe = c->pipe;
+  SWR_HIP_CHECK(pipe.rec.ensure(sizeof(TriRecord) * (size_t)T));
+  SWR_HIP_CHECK(pipe.poly.ensure(sizeof(int) * 2 * kMaxPolygon * (size_t)T));
+  SWR_HIP_CHECK(pipe.cnt.ensure(sizeof(uint32_t) * ((size_t)T + 1)));
+  hipLaunchKernelGGL(setup_kernel, dim3((T + 255) / 256), dim3(256), 0, s, P, pipe.rec.as<TriRecord>(),
+                     pipe.poly.as<int>(), pipe.cnt.as<uint32_t>());
   dim3 grid((P.width + 15) / 16, (P.height + 15) / 16), block(256);
   if (P.ms == 1)
-    hipLaunchKernelGGL(backward_kernel<1>, grid, block, 0, s, P, G, (const TriRecord *)c->rec.as<TriRecord>());
+    hipLaunchKernelGGL(backward_kernel<1>, grid, block, 0, s, P, G, (const TriRecord *)pipe.rec.as<TriRecord>());
   else
-    hipLaunchKernelGGL(backward_kernel<4>, grid, block, 0, s, P, G, (const TriRecord *)c->rec.as<TriRecord>());
+    hipLaunchKernelGGL(backward_kernel<4>, grid, block, 0, s, P, G, (const TriRecord *)pipe.rec.as<TriRecord>());
   SWR_HIP_CHECK(hipGetLastError());
   return SWR_OK;
 }
@@ -564,7 +772,173 @@ int check_chain(const swr_texture *t, uint32_t texelBytes) {
 
 }  // namespace
 
+struct swr_batch_plan {
+  int device = -1;
+  std::vector<swr_draw_desc> descs;
+  uint32_t headroom = 10;
+  swr_tuning tuning{};
+  BatchLayout L;
+  Pipeline pipe;
+  Scratch flags;  // per segment: pairs needed when a run overflowed
+};
+
+namespace {
+
+// (Re)builds a plan for its current tuning: layout, upload, one setup pass
+// per segment to size the pair buffers.
+int plan_build(DeviceCtx *c, swr_batch_plan *pl, hipStream_t s) {
+  int r = layout_batch(pl->descs.data(), (uint32_t)pl->descs.size(), pl->tuning, pl->L);
+  if (r != SWR_OK) return r;
+  r = upload_layout(c, pl->pipe, pl->L, s);
+  if (r != SWR_OK) return r;
+  uint32_t maxT = 0, maxCap = 0, maxTiles = 0;
+  for (auto &sg : pl->L.segs) {
+    r = ensure_pipe(pl->pipe, sg.T, 0, sg.tiles);
+    if (r != SWR_OK) return r;
+    enqueue_front(pl->pipe, pl->L, sg, s);
+    SWR_HIP_CHECK(hipGetLastError());
+    SWR_HIP_CHECK(hipMemcpyAsync(c->hostTotal, pl->pipe.off.as<uint32_t>() + sg.T, sizeof(uint32_t),
+                                 hipMemcpyDeviceToHost, s));
+    SWR_HIP_CHECK(hipStreamSynchronize(s));
+    const uint64_t need = *c->hostTotal;
+    const uint64_t cap = need + need * pl->headroom / 100 + 256;
+    if (cap > 0xFFFFFF00ull) return fail(SWR_ERROR_UNSUPPORTED, "batch segment needs %llu pairs", (unsigned long long)cap);
+    sg.pairCap = (uint32_t)cap;
+    maxT = std::max(maxT, sg.T);
+    maxCap = std::max(maxCap, sg.pairCap);
+    maxTiles = std::max(maxTiles, sg.tiles);
+  }
+  r = ensure_pipe(pl->pipe, maxT, maxCap, maxTiles);
+  if (r != SWR_OK) return r;
+  SWR_HIP_CHECK(pl->flags.ensure(sizeof(uint32_t) * std::max<size_t>(pl->L.segs.size(), 1)));
+  SWR_HIP_CHECK(hipMemsetAsync(pl->flags.ptr, 0, sizeof(uint32_t) * std::max<size_t>(pl->L.segs.size(), 1), s));
+  SWR_HIP_CHECK(hipStreamSynchronize(s));
+  return SWR_OK;
+}
+
+int plan_enqueue(DeviceCtx *c, swr_batch_plan *pl, hipStream_t s) {
+  for (size_t i = 0; i < pl->L.segs.size(); i++) {
+    const auto &sg = pl->L.segs[i];
+    enqueue_front(pl->pipe, pl->L, sg, s);
+    int r = enqueue_back(c, pl->pipe, pl->L, sg, sg.pairCap, pl->tuning, pl->flags.as<uint32_t>() + i, nullptr, s);
+    if (r != SWR_OK) return r;
+  }
+  SWR_HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}
+
+int plan_ctx(swr_batch_plan *pl, DeviceCtx **c) {
+  if (!pl) return fail(SWR_ERROR_INVALID_ARGUMENT, "plan is NULL");
+  int dev = 0;
+  SWR_HIP_CHECK(hipGetDevice(&dev));
+  if (dev != pl->device) return fail(SWR_ERROR_INVALID_ARGUMENT, "plan belongs to device %d (current %d)", pl->device, dev);
+  return current_ctx(c);
+}
+
+}  // namespace
+
 extern "C" {
+
+int swr_batch_plan_create(const swr_draw_desc *descs, uint32_t count, uint32_t pair_headroom_percent,
+                          swr_batch_plan **plan, void *stream) {
+  if (!plan) return fail(SWR_ERROR_INVALID_ARGUMENT, "plan is NULL");
+  *plan = nullptr;
+  if (count && !descs) return fail(SWR_ERROR_INVALID_ARGUMENT, "descs is NULL");
+  if (pair_headroom_percent > 1000) return fail(SWR_ERROR_INVALID_ARGUMENT, "headroom above 1000%%");
+  DeviceCtx *c = nullptr;
+  int r = current_ctx(&c);
+  if (r != SWR_OK) return r;
+  swr_batch_plan *pl = new swr_batch_plan();
+  pl->device = c->device;
+  pl->descs.assign(descs, descs + count);
+  pl->headroom = pair_headroom_percent;
+  pl->tuning = c->tuning;
+  r = plan_build(c, pl, reinterpret_cast<hipStream_t>(stream));
+  if (r != SWR_OK) {
+    pl->pipe.release();
+    pl->flags.release();
+    delete pl;
+    return r;
+  }
+  *plan = pl;
+  return SWR_OK;
+}
+
+int swr_batch_plan_run(swr_batch_plan *plan, void *stream) {
+  DeviceCtx *c = nullptr;
+  int r = plan_ctx(plan, &c);
+  if (r != SWR_OK) return r;
+  return plan_enqueue(c, plan, reinterpret_cast<hipStream_t>(stream));
+}
+
+int swr_batch_plan_check(swr_batch_plan *plan, uint32_t *pairs_needed) {
+  DeviceCtx *c = nullptr;
+  int r = plan_ctx(plan, &c);
+  if (r != SWR_OK) return r;
+  SWR_HIP_CHECK(hipDeviceSynchronize());
+  std::vector<uint32_t> f(std::max<size_t>(plan->L.segs.size(), 1), 0);
+  SWR_HIP_CHECK(hipMemcpy(f.data(), plan->flags.ptr, sizeof(uint32_t) * f.size(), hipMemcpyDeviceToHost));
+  uint32_t m = 0;
+  for (uint32_t v : f) m = std::max(m, v);
+  if (pairs_needed) *pairs_needed = m;
+  return SWR_OK;
+}
+
+int swr_batch_plan_autotune(swr_batch_plan *plan, swr_tuning *best, void *stream) {
+  DeviceCtx *c = nullptr;
+  int r = plan_ctx(plan, &c);
+  if (r != SWR_OK) return r;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const uint32_t kPerTile = ~0u;
+  struct Cand { uint32_t w, h, wg; };
+  const Cand cands[] = {{16, 16, kPerTile}, {16, 16, 2}, {16, 16, 3}, {16, 16, 4}, {16, 8, kPerTile}, {16, 8, 4},
+                        {16, 8, 6},         {8, 8, kPerTile}, {8, 8, 8},  {8, 8, 12}};
+  hipEvent_t e0, e1;
+  SWR_HIP_CHECK(hipEventCreate(&e0));
+  SWR_HIP_CHECK(hipEventCreate(&e1));
+  float bestMs = 1e30f;
+  swr_tuning bestT = plan->tuning;
+  for (const Cand &cd : cands) {
+    swr_tuning t{};
+    t.tile_width = cd.w;
+    t.tile_height = cd.h;
+    t.raster_workgroups_per_cu = cd.wg;
+    plan->tuning = t;
+    r = plan_build(c, plan, s);
+    if (r == SWR_OK) r = plan_enqueue(c, plan, s);  // warm-up
+    float ms = 0.0f;
+    if (r == SWR_OK) {
+      SWR_HIP_CHECK(hipEventRecord(e0, s));
+      for (int k = 0; k < 3 && r == SWR_OK; k++) r = plan_enqueue(c, plan, s);
+      SWR_HIP_CHECK(hipEventRecord(e1, s));
+      SWR_HIP_CHECK(hipEventSynchronize(e1));
+      SWR_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    }
+    if (r != SWR_OK) break;
+    if (ms < bestMs) {
+      bestMs = ms;
+      bestT = t;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (r != SWR_OK) return r;
+  plan->tuning = bestT;
+  r = plan_build(c, plan, s);
+  if (r != SWR_OK) return r;
+  if (best) *best = bestT;
+  return SWR_OK;
+}
+
+int swr_batch_plan_destroy(swr_batch_plan *plan) {
+  if (!plan) return SWR_OK;
+  int dev = 0;
+  if (hipGetDevice(&dev) == hipSuccess && dev == plan->device) (void)hipDeviceSynchronize();
+  plan->pipe.release();
+  plan->flags.release();
+  delete plan;
+  return SWR_OK;
+}
 
 int swr_draw_backward(const swr_draw_desc *desc, const swr_grad_desc *grad, void *stream) {
   DeviceCtx *c = nullptr;
@@ -633,9 +1007,7 @@ int swr_release(void) {
   int r = current_ctx(&c);
   if (r != SWR_OK) return r;
   SWR_HIP_CHECK(hipDeviceSynchronize());
-  for (Scratch *s : {&c->rec, &c->poly, &c->cnt, &c->off, &c->keysA, &c->keysB, &c->valsA, &c->valsB, &c->scanTmp,
-                     &c->sortTmp, &c->tStart, &c->tEnd})
-    s->release();
+  c->pipe.release();
   return SWR_OK;
 }
 
@@ -643,7 +1015,8 @@ int swr_draw(const swr_draw_desc *desc, void *stream) {
   DeviceCtx *c = nullptr;
   int r = current_ctx(&c);
   if (r != SWR_OK) return r;
-  return draw_one(c, desc, reinterpret_cast<hipStream_t>(stream));
+  if (!desc) return fail(SWR_ERROR_INVALID_ARGUMENT, "descriptor is NULL");
+  return draw_eager(c, desc, 1, reinterpret_cast<hipStream_t>(stream));
 }
 
 int swr_draw_batch(const swr_draw_desc *descs, uint32_t count, void *stream) {
@@ -651,11 +1024,7 @@ int swr_draw_batch(const swr_draw_desc *descs, uint32_t count, void *stream) {
   DeviceCtx *c = nullptr;
   int r = current_ctx(&c);
   if (r != SWR_OK) return r;
-  for (uint32_t i = 0; i < count; i++) {
-    r = draw_one(c, &descs[i], reinterpret_cast<hipStream_t>(stream));
-    if (r != SWR_OK) return r;
-  }
-  return SWR_OK;
+  return draw_eager(c, descs, count, reinterpret_cast<hipStream_t>(stream));
 }
 
 int swr_resolve(const swr_resolve_desc *d, void *stream) {
@@ -710,7 +1079,7 @@ int swr_profile_read(swr_profile *out) {
     c->acc.bin_ms += b;
     c->acc.raster_ms += d;
     c->acc.total_ms += t;
-    c->acc.draws += 1;
+    c->acc.draws += es.draws;
     c->acc.pairs += es.pairs;
     c->pool.push_back(es);
   }
@@ -731,8 +1100,8 @@ int swr_profile_read(swr_profile *out) {
 
 int swr_set_tuning(const swr_tuning *t) {
   if (!t) return fail(SWR_ERROR_INVALID_ARGUMENT, "tuning is NULL");
-  if (t->tile_width != 16 || t->tile_height != 16)
-    return fail(SWR_ERROR_UNSUPPORTED, "tile %ux%u (16x16 supported)", t->tile_width, t->tile_height);
+  int r0 = check_tuning(*t);
+  if (r0 != SWR_OK) return r0;
   DeviceCtx *c = nullptr;
   int r = current_ctx(&c);
   if (r != SWR_OK) return r;

@@ -97,10 +97,10 @@ __device__ int clip_polygon(ClipVtx (&poly)[kMaxPolygon], int &n, int flagsOr) {
 __device__ __forceinline__ int wadd(int a, int b) { return (int)((unsigned)a + (unsigned)b); }
 __device__ __forceinline__ int wsub(int a, int b) { return (int)((unsigned)a - (unsigned)b); }
 
-__global__ __launch_bounds__(256) void setup_kernel(DrawParams P, TriRecord *__restrict__ rec,
-                                                    int *__restrict__ poly, uint32_t *__restrict__ tileCount) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= P.triCount) return;
+// Setup of primitive t of the draw P, written to record slot g (the
+// primitive's index in the batch); `draw` is the draw's index in the batch.
+__device__ __forceinline__ void setup_one(const DrawParams &P, int t, size_t g, int draw, TriRecord *__restrict__ rec,
+                                          int *__restrict__ poly, uint32_t *__restrict__ tileCount) {
 
   uint32_t vi[3];
 #pragma unroll
@@ -150,9 +150,10 @@ __global__ __launch_bounds__(256) void setup_kernel(DrawParams P, TriRecord *__r
     pw[k] = rhw;
   }
 
-  TriRecord &R = rec[t];
-  tileCount[t] = 0;
+  TriRecord &R = rec[g];
+  tileCount[g] = 0;
   R.n = 0;
+  R.draw = draw;
   // setupSolidTriangles (Renderer.cpp:733-776).
   if ((flags[0] & flags[1] & flags[2]) != CLIP_FINITE) return;
   int flagsOr = flags[0] | flags[1] | flags[2];
@@ -308,13 +309,21 @@ __global__ __launch_bounds__(256) void setup_kernel(DrawParams P, TriRecord *__r
 #pragma unroll
     for (int i = 0; i < 3; i++) { R.X[i] = X[i]; R.Y[i] = Y[i]; }
   } else {
-    int *pp = poly + (size_t)t * (2 * kMaxPolygon);
+    int *pp = poly + g * (2 * kMaxPolygon);
     for (int i = 0; i < n; i++) { pp[i] = X[i]; pp[kMaxPolygon + i] = Y[i]; }
   }
   R.n = n;
   int tx0 = xMin / P.tileW, tx1 = (xMax - 1) / P.tileW;
   int ty0 = yMin / P.tileH, ty1 = (yMax - 1) / P.tileH;
-  tileCount[t] = (uint32_t)((tx1 - tx0 + 1) * (ty1 - ty0 + 1));
+  tileCount[g] = (uint32_t)((tx1 - tx0 + 1) * (ty1 - ty0 + 1));
+}
+
+// One draw: one thread per primitive.
+__global__ __launch_bounds__(256) void setup_kernel(DrawParams P, TriRecord *__restrict__ rec,
+                                                    int *__restrict__ poly, uint32_t *__restrict__ tileCount) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= P.triCount) return;
+  setup_one(P, t, (size_t)t, 0, rec, poly, tileCount);
 }
 
 // ---------------------------------------------------------------------------
@@ -554,7 +563,7 @@ __device__ __forceinline__ void blend_write(const DrawParams &P, PixelState<MS, 
       if (P.colorMask & 2) c.y = o.y;
       if (P.colorMask & 4) c.z = o.z;
       if (P.colorMask & 8) c.w = o.w;
-      S.id[FCOL ? q : 0] = tri + 1u;
+      S.id[FCOL ? q : 0] = tri - (uint32_t)P.triBase + 1u;
     } else {
       uint32_t p = S.cb[FCOL ? 0 : q];
       if (P.colorMask & 1) p = (p & 0xFFFFFF00u) | float_to_unorm8(o.x);
@@ -568,12 +577,13 @@ __device__ __forceinline__ void blend_write(const DrawParams &P, PixelState<MS, 
 
 constexpr int kQueue = 64;  // deferred-shading queue entries per wave
 
+// One screen tile of draw P: `bin` is the tile's index in the draw's tile
+// grid, [start, end) its range of the sorted (tile, primitive) list.  The
+// workgroup has one wave per 8x8 block of the tile.
 template <int MS, bool FCOL, bool FTEX>
-__global__ __launch_bounds__(256) void raster_kernel(DrawParams P, const TriRecord *__restrict__ rec,
-                                                     const int *__restrict__ poly,
-                                                     const uint32_t *__restrict__ binTri,
-                                                     const uint32_t *__restrict__ tileStart,
-                                                     const uint32_t *__restrict__ tileEnd) {
+__device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32_t start, uint32_t end0,
+                                            const TriRecord *__restrict__ rec, const int *__restrict__ poly,
+                                            const uint32_t *__restrict__ binTri) {
   // Per-wave queue of fragments that passed the tests and await shading,
   // in API order: primitive, owner lane | sample mask << 8, shaded colour.
   __shared__ uint32_t qTri[4][kQueue];
@@ -582,19 +592,17 @@ __global__ __launch_bounds__(256) void raster_kernel(DrawParams P, const TriReco
 
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int bin = blockIdx.x;
   const int bx = bin % P.tilesX, by = bin / P.tilesX;
-  const int ox = bx * 16 + 8 * (wave & 1), oy = by * 16 + 8 * (wave >> 1);
+  const int bw = P.tileW >> 3;  // 8x8 blocks per tile row
+  const int ox = bx * P.tileW + 8 * (wave % bw), oy = by * P.tileH + 8 * (wave / bw);
   const int cx = lane_cx(lane), cy = lane_cy(lane);
   const int px = ox + cx, py = oy + cy;
   const int myBit = cy * 8 + cx;
 
-  const uint32_t start = tileStart[bin];
-  if (start >= tileEnd[bin]) return;  // uniform over the workgroup
   // A block outside the scissor walks no list (no early return: the
   // workgroup reduces its counters together at the end).
   const bool waveActive = !(ox >= P.sx1 || oy >= P.sy1 || ox + 8 <= P.sx0 || oy + 8 <= P.sy0);
-  const uint32_t end = waveActive ? tileEnd[bin] : start;
+  const uint32_t end = waveActive ? end0 : start;
 
   uint64_t scissorMask = 0;
   {
@@ -887,13 +895,54 @@ __global__ __launch_bounds__(256) void raster_kernel(DrawParams P, const TriReco
     __syncthreads();
     if (threadIdx.x < 4) {
       const int k = threadIdx.x;
-      const unsigned long long t = red[0][k] + red[1][k] + red[2][k] + red[3][k];
+      const int nw = (int)(blockDim.x >> 6);
+      unsigned long long t = red[0][k];
+      for (int w = 1; w < nw; w++) t += red[w][k];
       if (k == 0) {
         if (P.occlusion && t) atomicAdd(P.occlusion, t);
       } else if (P.stats) {
         atomicAdd(P.stats + (k - 1), t);
       }
     }
+  }
+}
+
+// A batch of draws: persistent workgroups take global tiles [tileLo, tileHi)
+// from an atomic counter (or, with counter == NULL, workgroup b renders tile
+// tileLo + b once); tile t belongs to the draw d in [drawLo, drawHi) with
+// tileBase[d] <= t < tileBase[d + 1].
+template <int MS, bool FCOL, bool FTEX>
+__global__ __launch_bounds__(256) void raster_batch_kernel(const DrawParams *__restrict__ params,
+                                                           const uint32_t *__restrict__ tileBase, int drawLo,
+                                                           int drawHi, uint32_t tileLo, uint32_t tileHi,
+                                                           uint32_t *__restrict__ counter,
+                                                           const TriRecord *__restrict__ rec,
+                                                           const int *__restrict__ poly,
+                                                           const uint32_t *__restrict__ binTri,
+                                                           const uint32_t *__restrict__ tileStart,
+                                                           const uint32_t *__restrict__ tileEnd) {
+  __shared__ uint32_t sTile;
+  for (uint32_t iter = 0;; iter++) {
+    uint32_t t;
+    if (counter) {
+      if (threadIdx.x == 0) sTile = tileLo + atomicAdd(counter, 1u);
+      __syncthreads();
+      t = __builtin_amdgcn_readfirstlane(sTile);
+      __syncthreads();
+    } else {
+      if (iter) break;
+      t = tileLo + blockIdx.x;
+    }
+    if (t >= tileHi) break;
+    int lo = drawLo, hi = drawHi;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (tileBase[mid] <= t) lo = mid;
+      else hi = mid;
+    }
+    lo = __builtin_amdgcn_readfirstlane(lo);
+    const uint32_t s = tileStart[t], e = tileEnd[t];
+    if (s < e) raster_tile<MS, FCOL, FTEX>(params[lo], (int)(t - tileBase[lo]), s, e, rec, poly, binTri);
   }
 }
 
@@ -933,21 +982,30 @@ __global__ void resolve_rgba32f_kernel(const char *src, int srcPitch, long long 
   *reinterpret_cast<float4 *>(dst + (long long)y * dstPitch + 16ll * x) = o;
 }
 
-template __global__ void raster_kernel<1, false, false>(DrawParams, const TriRecord *, const int *, const uint32_t *,
-                                                        const uint32_t *, const uint32_t *);
-template __global__ void raster_kernel<1, false, true>(DrawParams, const TriRecord *, const int *, const uint32_t *,
-                                                       const uint32_t *, const uint32_t *);
-template __global__ void raster_kernel<1, true, false>(DrawParams, const TriRecord *, const int *, const uint32_t *,
-                                                       const uint32_t *, const uint32_t *);
-template __global__ void raster_kernel<1, true, true>(DrawParams, const TriRecord *, const int *, const uint32_t *,
-                                                      const uint32_t *, const uint32_t *);
-template __global__ void raster_kernel<4, false, false>(DrawParams, const TriRecord *, const int *, const uint32_t *,
-                                                        const uint32_t *, const uint32_t *);
-template __global__ void raster_kernel<4, false, true>(DrawParams, const TriRecord *, const int *, const uint32_t *,
-                                                       const uint32_t *, const uint32_t *);
-template __global__ void raster_kernel<4, true, false>(DrawParams, const TriRecord *, const int *, const uint32_t *,
-                                                       const uint32_t *, const uint32_t *);
-template __global__ void raster_kernel<4, true, true>(DrawParams, const TriRecord *, const int *, const uint32_t *,
-                                                      const uint32_t *, const uint32_t *);
+
+template __global__ void raster_batch_kernel<1, false, false>(const DrawParams *, const uint32_t *, int, int, uint32_t,
+                                                              uint32_t, uint32_t *, const TriRecord *, const int *,
+                                                              const uint32_t *, const uint32_t *, const uint32_t *);
+template __global__ void raster_batch_kernel<1, false, true>(const DrawParams *, const uint32_t *, int, int, uint32_t,
+                                                              uint32_t, uint32_t *, const TriRecord *, const int *,
+                                                              const uint32_t *, const uint32_t *, const uint32_t *);
+template __global__ void raster_batch_kernel<1, true, false>(const DrawParams *, const uint32_t *, int, int, uint32_t,
+                                                              uint32_t, uint32_t *, const TriRecord *, const int *,
+                                                              const uint32_t *, const uint32_t *, const uint32_t *);
+template __global__ void raster_batch_kernel<1, true, true>(const DrawParams *, const uint32_t *, int, int, uint32_t,
+                                                              uint32_t, uint32_t *, const TriRecord *, const int *,
+                                                              const uint32_t *, const uint32_t *, const uint32_t *);
+template __global__ void raster_batch_kernel<4, false, false>(const DrawParams *, const uint32_t *, int, int, uint32_t,
+                                                              uint32_t, uint32_t *, const TriRecord *, const int *,
+                                                              const uint32_t *, const uint32_t *, const uint32_t *);
+template __global__ void raster_batch_kernel<4, false, true>(const DrawParams *, const uint32_t *, int, int, uint32_t,
+                                                              uint32_t, uint32_t *, const TriRecord *, const int *,
+                                                              const uint32_t *, const uint32_t *, const uint32_t *);
+template __global__ void raster_batch_kernel<4, true, false>(const DrawParams *, const uint32_t *, int, int, uint32_t,
+                                                              uint32_t, uint32_t *, const TriRecord *, const int *,
+                                                              const uint32_t *, const uint32_t *, const uint32_t *);
+template __global__ void raster_batch_kernel<4, true, true>(const DrawParams *, const uint32_t *, int, int, uint32_t,
+                                                              uint32_t, uint32_t *, const TriRecord *, const int *,
+                                                              const uint32_t *, const uint32_t *, const uint32_t *);
 
 }  // namespace swr

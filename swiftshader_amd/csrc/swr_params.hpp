@@ -87,6 +87,7 @@ struct DrawParams {
   unsigned long long *stats;  // [block tests, block covers, shaded] or NULL
   // Tiling.
   int tileW, tileH, tilesX, tilesY;
+  int triBase;  // first record slot of this draw's primitives in its batch
 };
 
 // Per-primitive setup output.  n == 0 marks a culled primitive.  The first
@@ -103,7 +104,7 @@ struct TriRecord {
   float zA, zB, zC;
   float wA, wB, wC;
   float V[6][3];    // colour rgba, uv
-  int pad;
+  int draw;         // index of the primitive's draw in its batch
 };
 
 static_assert(sizeof(TriRecord) == 160, "TriRecord layout");

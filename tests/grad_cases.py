@@ -58,3 +58,43 @@ def gradient_rel_error(got: np.ndarray, want: np.ndarray) -> float:
     den = np.linalg.norm(want)
     num = np.linalg.norm(got - want)
     return 0.0 if den == 0 and num == 0 else num / max(den, 1e-300)
+
+
+def fd_derivative(view, params, frag, g, kind, idx, rel_h=1e-6):
+    """Central finite difference of G.objective with respect to params[kind][idx].
+
+    The objective is piecewise smooth in vertex parameters: bilinear filtering
+    is piecewise linear in texel coordinates, so a step that moves a footprint
+    across a texel boundary averages two slopes.  The derivative at the point
+    is the limit, so the step is shrunk until two estimates (h and h/4)
+    agree.  For a vertex parameter only the fragments of primitives using the
+    vertex are evaluated (the rest do not change), which keeps rounding small
+    at tiny steps.  Texel parameters enter linearly: one step is exact.
+    """
+    sub = frag
+    if kind != "texture":
+        vi = G._vertex_ids(view, frag.t, len(params["positions"]))
+        m = (vi == idx[0]).any(axis=1)
+        sub = G.Fragments(frag.x[m], frag.y[m], frag.t[m], frag.smask[m], frag.state[m])
+        g = g[m]
+
+    def central(h):
+        p = {k: v.copy() for k, v in params.items()}
+        p[kind][idx] += h
+        fp = G.objective(view, p, sub, g)
+        p[kind][idx] -= 2 * h
+        fm = G.objective(view, p, sub, g)
+        return (fp - fm) / (2 * h)
+
+    scale = max(1.0, abs(params[kind][idx]))
+    if kind == "texture":
+        return central(1e-3 * scale)
+    h = rel_h * scale
+    prev = central(h)
+    for _ in range(6):
+        cur = central(h / 4)
+        if abs(cur - prev) <= 1e-6 * max(abs(cur), 1e-3):
+            return cur
+        h /= 10
+        prev = central(h)
+    return prev

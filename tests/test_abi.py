@@ -81,5 +81,8 @@ def test_version_and_error_paths_without_gpu():
     assert isinstance(lib.swr_last_error(), bytes)
     assert lib.swr_set_tuning(None) < 0
     assert lib.swr_resolve(None, None) < 0
+    assert lib.swr_batch_plan_create(None, 0, 0, None, None) < 0
+    assert lib.swr_batch_plan_run(None, None) < 0
+    assert lib.swr_batch_plan_destroy(None) == 0
     assert lib.swr_generate_mips(None, None) < 0
     assert lib.swr_mip_backward(None, None) < 0

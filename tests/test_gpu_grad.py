@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 import torch
 
-from grad_cases import G, float_scene, gradient_rel_error, params_of, view_of
+from grad_cases import G, fd_derivative, float_scene, gradient_rel_error, params_of, view_of
 from oracle_runner import oracle_render_ids
 from swiftshader_amd import abi, diff, scenes
 
@@ -153,13 +153,7 @@ def test_backward_against_central_differences_64_params():
                "texture": gpu_tex0}
     worst = 0.0
     for kind, idx in checks:
-        p = {k: v.copy() for k, v in params.items()}
-        h = 1e-6 * max(1.0, abs(p[kind][idx]))
-        p[kind][idx] += h
-        fp = G.objective(view, p, frag, g)
-        p[kind][idx] -= 2 * h
-        fm = G.objective(view, p, frag, g)
-        fd = (fp - fm) / (2 * h)
+        fd = fd_derivative(view, params, frag, g, kind, idx)
         got = float(got_all[kind][idx])
         scale = np.abs(got_all[kind]).max()
         err = abs(got - fd) / max(abs(fd), 1e-3 * scale)

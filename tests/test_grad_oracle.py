@@ -12,7 +12,7 @@
 import numpy as np
 import pytest
 
-from grad_cases import G, float_scene, params_of, view_of
+from grad_cases import G, fd_derivative, float_scene, params_of, view_of
 from oracle_runner import oracle_render_ids
 from swiftshader_amd import abi, scenes
 from swiftshader_amd.scenes import DrawData, TextureData
@@ -65,13 +65,7 @@ def test_gradient_matches_central_differences(case):
             checks.append(("texture", (int(nz[k][0]), int(nz[k][1]), int(rng.integers(0, 4)))))
     scale = {k: np.abs(v).max() for k, v in an.items() if k != "texture_levels"}
     for name, idx in checks:
-        p = {k: v.copy() for k, v in params.items()}
-        h = 1e-6 * max(1.0, abs(p[name][idx]))
-        p[name][idx] += h
-        fp = G.objective(view, p, frag, g)
-        p[name][idx] -= 2 * h
-        fm = G.objective(view, p, frag, g)
-        fd = (fp - fm) / (2 * h)
+        fd = fd_derivative(view, params, frag, g, name, idx)
         a = an[name][idx]
         assert abs(fd - a) <= 1e-5 * max(abs(a), 1e-3 * scale[name]) + 1e-9, (name, idx, fd, a)
     # z reaches no colour.
