@@ -9,7 +9,7 @@ per-vertex w ~ U[0.5, 4], UV ~ U[0, 4], a 1024^2 RGBA8 noise texture with an
 SRC_ALPHA / ONE_MINUS_SRC_ALPHA, texture x colour.  Inputs are resident in
 HBM before timing starts.
 
-Reported value: shaded fragments per second, where a fragment is a
+Reported value (headline): shaded fragments per second, where a fragment is a
 (pixel, primitive) pair with non-empty coverage after the scissor -- the
 count is taken with an occlusion query of a depth-ALWAYS counting draw of
 the same scene, outside the timed region.
@@ -25,6 +25,14 @@ Also reported:
   cpu_baseline  the project's scalar CPU restatement of the same path
                 (oracle/, single thread) on a bounded sample: the first
                 40,000 triangles of the same scene at full resolution.
+  batched       512 uneven draws: frames (draws) per second of one
+                autotuned batch plan replayed from a HIP graph, against one
+                swr_draw call per draw.
+  fwd_bwd       16 differentiable 512^2 views: ms per forward + backward
+                step, graph-captured batch against per-draw launches.
+  texture_fit   64-view 4x MSAA, 8x anisotropic texture fitting with Adam:
+                ms per captured step against per-draw launches, and the loss
+                before and after.
 """
 import argparse
 import dataclasses
@@ -49,6 +57,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 256 MiB device copy (calibrates the HBM byte counters)")
+    ap.add_argument("--legs", default="draw,batched,fwd_bwd,fit",
+                    help="comma-separated subset of: draw (1080p frame, the headline), batched (512 uneven draws), "
+                         "fwd_bwd (16 differentiable views), fit (64-view 4x MSAA texture fitting)")
+    ap.add_argument("--fit-steps", type=int, default=200, help="Adam steps of the fitting leg")
     return ap.parse_args()
 
 
@@ -85,6 +97,148 @@ def cpu_baseline(draw, n_tri, gpu_count_fn):
     return {"value": frags / dt / 1e9, "unit": "Gfragments/s", "cores": 1, "kind": "port",
             "sample": f"first {n_tri} of the 100,000 triangles at 1920x1080 ({frags} fragments, {dt:.2f} s, "
                       f"scalar C restatement of the reference path, 1 thread)"}
+
+
+def _ms(fn, reps):
+    """Average milliseconds of fn() over reps, HIP events on the current stream."""
+    import torch
+    st = torch.cuda.current_stream()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    a.record(st)
+    for _ in range(reps):
+        fn()
+    b.record(st)
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def leg_batched(dev, reps):
+    """512 independent draws of uneven size (targets 64..1024 per side, 2k..50k
+    triangles, RGBA8 noise textures 256..2048 with full mips, trilinear,
+    CLAMP_TO_EDGE or REPEAT): one autotuned batch plan, clears + plan run
+    captured in one HIP graph, against one swr_draw per draw (no graph)."""
+    import torch
+    from swiftshader_amd import diff, runtime, scenes
+    from swiftshader_amd.batch import BatchPlan
+    draws = scenes.batched(seed=3, host_textures=False)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(3)
+    dds = []
+    for d in draws:
+        ts = d.texture.size
+        lvl0 = torch.randint(0, 256, (ts, ts, 4), dtype=torch.uint8, device=dev, generator=gen)
+        levels = [lvl0] + diff.mip_levels_like(lvl0)
+        diff.generate_mips(levels)
+        dds.append(runtime.prepare(d, dev, texture_tensors=levels))
+    torch.cuda.synchronize()
+    tris = sum(d.triangle_count for d in draws)
+
+    def clear_all():
+        for dd in dds:
+            dd.targets.color.zero_()
+            dd.targets.depth.fill_(1.0)
+
+    def plain():
+        for dd in dds:
+            dd.targets.color.zero_()
+            dd.targets.depth.fill_(1.0)
+            runtime.draw(dd)
+
+    plain()
+    plain_ms = _ms(plain, 2)
+    plan = BatchPlan([dd.desc for dd in dds])
+    best = plan.autotune()
+    clear_all()
+    plan.run()
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        clear_all()
+        plan.run()
+    graph.replay()
+    batched_ms = _ms(graph.replay, reps)
+    over = plan.pairs_overflow()
+    plan.close()
+    return {"workload": "512 draws, targets 64..1024 px per side, 2k..50k triangles each "
+                        f"({tris} total), RGBA8 textures 256..2048 with mips, trilinear, SRC_ALPHA blend",
+            "draws": len(dds), "batch_ms": round(batched_ms, 3), "frames_per_s": round(1e3 * len(dds) / batched_ms, 1),
+            "plain_batch_ms": round(plain_ms, 3), "plain_frames_per_s": round(1e3 * len(dds) / plain_ms, 1),
+            "speedup_vs_per_draw": round(plain_ms / batched_ms, 2),
+            "tuning": {"tile": f"{best.tile_width}x{best.tile_height}",
+                       "raster_workgroups_per_cu": ("one per tile" if best.raster_workgroups_per_cu == 0xFFFFFFFF
+                                                    else int(best.raster_workgroups_per_cu))},
+            "pair_overflow": over}
+
+
+def _fit_targets(draws, tex_levels0, dev):
+    """Target images: the views rendered with a second texture."""
+    import torch
+    from swiftshader_amd import texfit
+    ref = texfit.MultiViewFit(draws, torch.from_numpy(tex_levels0), device=dev)
+    ref.forward_plain()
+    imgs = []
+    for vb in ref.views:
+        if vb.draw.samples > 1:
+            from swiftshader_amd import runtime
+            runtime.resolve(vb.color, vb.resolved)
+            imgs.append(vb.resolved[0].clone())
+        else:
+            imgs.append(vb.color[0].clone())
+    torch.cuda.synchronize()
+    del ref
+    return imgs
+
+
+def leg_fwd_bwd(dev, reps):
+    """16 views 512^2 RGBA32F, 20k triangles each, 1024^2 float texture
+    (trilinear): forward + backward (texel, colour, texcoord and position
+    gradients) of an L2 loss, per-draw launches vs one graph-captured step."""
+    import numpy as np
+    import torch
+    from swiftshader_amd import scenes, texfit
+    draws = scenes.diff_views(seed=4)
+    rng = np.random.default_rng(40)
+    targets = _fit_targets(draws, scenes.float_texture(rng, 1024)[0], dev)
+    fit = texfit.MultiViewFit(draws, torch.from_numpy(draws[0].texture.levels[0]), targets,
+                              geometry_grads=True, device=dev)
+    fit.step_plain(update=False)
+    plain_ms = _ms(lambda: fit.step_plain(update=False), 3)
+    fit.capture(update=False)
+    fit.step_graph()
+    graph_ms = _ms(fit.step_graph, reps)
+    return {"workload": "16 views 512x512 RGBA32F + D32F, 20k triangles each, 1024^2 RGBA32F texture, trilinear; "
+                        "L2 loss; gradients of texels, colours, texcoords, positions",
+            "plain_ms_per_step": round(plain_ms, 3), "graph_ms_per_step": round(graph_ms, 3),
+            "speedup": round(plain_ms / graph_ms, 2), "loss": fit.current_loss(),
+            "gradient_tolerance": "<= 1e-4 relative to the FP64 oracle (tests/test_gpu_grad.py)"}
+
+
+def leg_fit(dev, steps):
+    """64 views of side 256..1024, 4x MSAA resolved by averaging, 10k..40k
+    triangles, 2048^2 RGBA32F texture with 8x anisotropic trilinear
+    filtering: Adam (lr 1e-2) on the texture against views rendered from a
+    second texture; each step captured in one HIP graph."""
+    import numpy as np
+    import torch
+    from swiftshader_amd import scenes, texfit
+    draws = scenes.msaa_aniso(seed=5)
+    rng = np.random.default_rng(50)
+    targets = _fit_targets(draws, scenes.float_texture(rng, 2048)[0], dev)
+    fit = texfit.MultiViewFit(draws, torch.from_numpy(draws[0].texture.levels[0]), targets, lr=1e-2, device=dev)
+    fit.step_plain(update=False)
+    plain_ms = _ms(lambda: fit.step_plain(update=False), 2)
+    fit.capture(update=True)
+    fit.step_graph()
+    torch.cuda.synchronize()
+    loss0 = fit.current_loss()
+    graph_ms = _ms(fit.step_graph, max(steps - 1, 1))
+    loss1 = fit.current_loss()
+    return {"workload": "64 views 256..1024 px per side, 4x MSAA + resolve, 10k..40k triangles each, "
+                        "2048^2 RGBA32F texture, 8x anisotropic trilinear, Adam lr 1e-2",
+            "steps": steps, "plain_ms_per_step": round(plain_ms, 3), "graph_ms_per_step": round(graph_ms, 3),
+            "speedup": round(plain_ms / graph_ms, 2), "loss_first_step": loss0, "loss_last_step": loss1}
 
 
 def main():
@@ -187,6 +341,21 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    legs = [x.strip() for x in args.legs.split(",") if x.strip()]
+    # The other two parts of the metric (batched frames/s; fwd+bwd ms/step
+    # against per-draw launches); each leg prints a progress line to stderr.
+    for name, fn in (("batched", lambda: leg_batched(dev, max(args.steps // 4, 3))),
+                     ("fwd_bwd", lambda: leg_fwd_bwd(dev, max(args.steps // 4, 3))),
+                     ("texture_fit", lambda: leg_fit(dev, args.fit_steps))):
+        key = {"batched": "batched", "fwd_bwd": "fwd_bwd", "texture_fit": "fit"}[name]
+        if key in legs:
+            print(f"[bench] leg {name}", file=sys.stderr, flush=True)
+            try:
+                out[name] = fn()
+            except Exception as e:  # a failing leg is reported, the headline line still prints
+                out[name] = {"error": f"{type(e).__name__}: {e}"}
+            torch.cuda.synchronize()
+            torch.cuda.empty_cache()
     print(json.dumps(out), flush=True)
 
 

@@ -332,6 +332,17 @@ int swr_batch_plan_check(swr_batch_plan *plan, uint32_t *pairs_needed);
  * Synchronises `stream`. */
 int swr_batch_plan_autotune(swr_batch_plan *plan, swr_tuning *best, void *stream);
 
+/* Sets the gradient targets of a plan's draws (one swr_grad_desc per
+ * planned draw, in the order the draws were given; see swr_grad_desc).  The
+ * buffers they name must stay allocated while the plan is used.
+ * Synchronises `stream`. */
+int swr_batch_plan_set_grads(swr_batch_plan *plan, const swr_grad_desc *grads, uint32_t count, void *stream);
+
+/* Enqueues the gradients of every planned draw (as swr_draw_backward of
+ * each, accumulated into the buffers swr_batch_plan_set_grads named), one
+ * launch per segment and sample count: no allocation, no host wait. */
+int swr_batch_plan_backward(swr_batch_plan *plan, void *stream);
+
 /* Frees the plan (waits for the device first). */
 int swr_batch_plan_destroy(swr_batch_plan *plan);
 

@@ -140,7 +140,7 @@ class Profile(C.Structure):
 EXPORTED_SYMBOLS = (
     "swr_abi_version", "swr_last_error", "swr_init", "swr_release", "swr_draw",
     "swr_draw_batch", "swr_resolve", "swr_batch_plan_create", "swr_batch_plan_run", "swr_batch_plan_check",
-    "swr_batch_plan_autotune", "swr_batch_plan_destroy", "swr_draw_backward", "swr_generate_mips", "swr_mip_backward",
+    "swr_batch_plan_autotune", "swr_batch_plan_set_grads", "swr_batch_plan_backward", "swr_batch_plan_destroy", "swr_draw_backward", "swr_generate_mips", "swr_mip_backward",
     "swr_profile_enable", "swr_profile_read", "swr_set_tuning", "swr_get_tuning",
 )
 
@@ -167,6 +167,10 @@ def bind(lib):
     lib.swr_batch_plan_check.restype = C.c_int
     lib.swr_batch_plan_autotune.argtypes = [C.c_void_p, C.POINTER(Tuning), C.c_void_p]
     lib.swr_batch_plan_autotune.restype = C.c_int
+    lib.swr_batch_plan_set_grads.argtypes = [C.c_void_p, C.POINTER(GradDesc), C.c_uint32, C.c_void_p]
+    lib.swr_batch_plan_set_grads.restype = C.c_int
+    lib.swr_batch_plan_backward.argtypes = [C.c_void_p, C.c_void_p]
+    lib.swr_batch_plan_backward.restype = C.c_int
     lib.swr_batch_plan_destroy.argtypes = [C.c_void_p]
     lib.swr_batch_plan_destroy.restype = C.c_int
     lib.swr_draw_backward.argtypes = [C.POINTER(DrawDesc), C.POINTER(GradDesc), C.c_void_p]

@@ -439,6 +439,7 @@ struct BatchLayout {
     uint32_t pairCap = 0;                       // planned pair capacity (plans only)
   };
   std::vector<DrawParams> P;
+  std::vector<int> orig;                      // per draw: index in the caller's array
   std::vector<uint32_t> blockBase, tileBase;  // per draw, segment-relative
   std::vector<Seg> segs;
 };
@@ -485,6 +486,7 @@ int layout_batch(const swr_draw_desc *descs, uint32_t n, const swr_tuning &tunin
       else
         sg.variants.back().second = (int)k + 1;
       L.P.push_back(P);
+      L.orig.push_back(order[k]);
     }
     sg.count = (int)order.size();
     if (sg.count) L.segs.push_back(sg);
@@ -702,11 +704,10 @@ int draw_eager(DeviceCtx *c, const swr_draw_desc *descs, uint32_t n, hipStream_t
   return SWR_OK;
 }
 
-int draw_backward(DeviceCtx *c, const swr_draw_desc *desc, const swr_grad_desc *g, hipStream_t s) {
+// Validates a gradient descriptor against its draw and derives GradParams.
+int grad_params(const swr_draw_desc *desc, const DrawParams &P, const swr_grad_desc *g, GradParams &G) {
+  G = GradParams{};
   if (!g) return fail(SWR_ERROR_INVALID_ARGUMENT, "grad descriptor is NULL");
-  DrawParams P;
-  int r = build_params(desc, c->tuning, P);
-  if (r != SWR_OK) return r;
   if (!P.primId) return fail(SWR_ERROR_INVALID_ARGUMENT, "backward needs the draw's primitive_id attachment");
   if (P.blendEnable) return fail(SWR_ERROR_UNSUPPORTED, "backward of a blended draw");
   if (!g->d_color) return fail(SWR_ERROR_INVALID_ARGUMENT, "d_color is NULL");
@@ -718,7 +719,6 @@ int draw_backward(DeviceCtx *c, const swr_draw_desc *desc, const swr_grad_desc *
        (P.ms > 1 &&
         g->sample_state_sample_pitch_bytes < (uint64_t)g->sample_state_row_pitch_bytes * desc->framebuffer_height)))
     return fail(SWR_ERROR_INVALID_ARGUMENT, "sample_state pitches too small");
-  GradParams G{};
   G.dColor = (const char *)g->d_color;
   G.dColorPitch = (int)g->d_color_row_pitch_bytes;
   G.dColorSlice = (long long)g->d_color_sample_pitch_bytes;
@@ -735,6 +735,16 @@ int draw_backward(DeviceCtx *c, const swr_draw_desc *desc, const swr_grad_desc *
     const uint32_t n = desc->texture.level_count;
     for (int l = 0; l < kMipLevels; l++) G.dTex[l] = (float *)g->d_texture[std::min<uint32_t>(l, n - 1)];
   }
+  return SWR_OK;
+}
+
+int draw_backward(DeviceCtx *c, const swr_draw_desc *desc, const swr_grad_desc *g, hipStream_t s) {
+  DrawParams P;
+  int r = build_params(desc, c->tuning, P);
+  if (r != SWR_OK) return r;
+  GradParams G;
+  r = grad_params(desc, P, g, G);
+  if (r != SWR_OK) return r;
   const uint32_t T = desc->triangle_count;
   if (T == 0) return SWR_OK;
   Pipeline &pipe = c->pipe;
@@ -780,6 +790,11 @@ struct swr_batch_plan {
   BatchLayout L;
   Pipeline pipe;
   Scratch flags;  // per segment: pairs needed when a run overflowed
+  // Backward: per draw (layout order) gradient targets and 16x16-block bases.
+  std::vector<swr_grad_desc> gradDescs;  // caller order
+  Scratch gparams, gbase;
+  std::vector<uint32_t> bwBase;
+  bool hasGrads = false;
 };
 
 namespace {
@@ -826,6 +841,8 @@ int plan_enqueue(DeviceCtx *c, swr_batch_plan *pl, hipStream_t s) {
   SWR_HIP_CHECK(hipGetLastError());
   return SWR_OK;
 }
+
+int plan_upload_grads(DeviceCtx *c, swr_batch_plan *pl, hipStream_t s);
 
 int plan_ctx(swr_batch_plan *pl, DeviceCtx **c) {
   if (!pl) return fail(SWR_ERROR_INVALID_ARGUMENT, "plan is NULL");
@@ -926,8 +943,102 @@ int swr_batch_plan_autotune(swr_batch_plan *plan, swr_tuning *best, void *stream
   plan->tuning = bestT;
   r = plan_build(c, plan, s);
   if (r != SWR_OK) return r;
+  if (plan->hasGrads) {
+    r = plan_upload_grads(c, plan, s);
+    if (r != SWR_OK) return r;
+  }
   if (best) *best = bestT;
   return SWR_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Uploads the gradient targets of a plan in its layout order.
+int plan_upload_grads(DeviceCtx *c, swr_batch_plan *pl, hipStream_t s) {
+  const size_t n = pl->L.P.size();
+  std::vector<GradParams> G(n);
+  pl->bwBase.assign(n, 0);
+  for (const auto &sg : pl->L.segs) {
+    uint32_t base = 0;
+    for (int k = 0; k < sg.count; k++) {
+      const int i = sg.first + k;
+      const DrawParams &P = pl->L.P[i];
+      const int o = pl->L.orig[i];
+      int r = grad_params(&pl->descs[o], P, &pl->gradDescs[o], G[i]);
+      if (r != SWR_OK) return fail(r, "draw %d: %s", o, g_error.c_str());
+      pl->bwBase[i] = base;
+      base += (uint32_t)(((P.width + 15) / 16) * ((P.height + 15) / 16));
+    }
+  }
+  if (!n) return SWR_OK;
+  SWR_HIP_CHECK(pl->gparams.ensure(sizeof(GradParams) * n));
+  SWR_HIP_CHECK(pl->gbase.ensure(sizeof(uint32_t) * n));
+  SWR_HIP_CHECK(hipMemcpyAsync(pl->gparams.ptr, G.data(), sizeof(GradParams) * n, hipMemcpyHostToDevice, s));
+  SWR_HIP_CHECK(hipMemcpyAsync(pl->gbase.ptr, pl->bwBase.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
+  SWR_HIP_CHECK(hipStreamSynchronize(s));
+  (void)c;
+  return SWR_OK;
+}
+
+int plan_enqueue_backward(swr_batch_plan *pl, hipStream_t s) {
+  Pipeline &pipe = pl->pipe;
+  for (const auto &sg : pl->L.segs) {
+    const DrawParams *params = pipe.params.as<DrawParams>() + sg.first;
+    const uint32_t *blockBase = pipe.aux.as<uint32_t>() + sg.first;
+    hipLaunchKernelGGL(setup_batch_kernel, dim3(sg.blocks), dim3(256), 0, s, params, blockBase, sg.count,
+                       pipe.rec.as<TriRecord>(), pipe.poly.as<int>(), pipe.cnt.as<uint32_t>());
+    const GradParams *grads = pl->gparams.as<GradParams>() + sg.first;
+    const uint32_t *bb = pl->gbase.as<uint32_t>() + sg.first;
+    // Draws of one sample count are contiguous (the layout groups variants).
+    int lo = 0;
+    while (lo < sg.count) {
+      const int ms = pl->L.P[sg.first + lo].ms;
+      int hi = lo + 1;
+      while (hi < sg.count && pl->L.P[sg.first + hi].ms == ms) hi++;
+      const uint32_t bLo = pl->bwBase[sg.first + lo];
+      const DrawParams &Ph = pl->L.P[sg.first + hi - 1];
+      const uint32_t bHi = pl->bwBase[sg.first + hi - 1] + (uint32_t)(((Ph.width + 15) / 16) * ((Ph.height + 15) / 16));
+      if (bHi > bLo) {
+        if (ms == 1)
+          hipLaunchKernelGGL(backward_batch_kernel<1>, dim3(bHi - bLo), dim3(256), 0, s, params, grads, bb, lo, hi, bLo,
+                             (const TriRecord *)pipe.rec.as<TriRecord>());
+        else
+          hipLaunchKernelGGL(backward_batch_kernel<4>, dim3(bHi - bLo), dim3(256), 0, s, params, grads, bb, lo, hi, bLo,
+                             (const TriRecord *)pipe.rec.as<TriRecord>());
+      }
+      lo = hi;
+    }
+  }
+  SWR_HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int swr_batch_plan_set_grads(swr_batch_plan *plan, const swr_grad_desc *grads, uint32_t count, void *stream) {
+  DeviceCtx *c = nullptr;
+  int r = plan_ctx(plan, &c);
+  if (r != SWR_OK) return r;
+  if (!grads || count != plan->descs.size())
+    return fail(SWR_ERROR_INVALID_ARGUMENT, "need one grad descriptor per planned draw (%zu)", plan->descs.size());
+  plan->gradDescs.assign(grads, grads + count);
+  plan->hasGrads = false;
+  r = plan_upload_grads(c, plan, reinterpret_cast<hipStream_t>(stream));
+  if (r != SWR_OK) return r;
+  plan->hasGrads = true;
+  return SWR_OK;
+}
+
+int swr_batch_plan_backward(swr_batch_plan *plan, void *stream) {
+  DeviceCtx *c = nullptr;
+  int r = plan_ctx(plan, &c);
+  if (r != SWR_OK) return r;
+  if (!plan->hasGrads) return fail(SWR_ERROR_INVALID_ARGUMENT, "swr_batch_plan_set_grads has not been called");
+  return plan_enqueue_backward(plan, reinterpret_cast<hipStream_t>(stream));
 }
 
 int swr_batch_plan_destroy(swr_batch_plan *plan) {
@@ -936,6 +1047,8 @@ int swr_batch_plan_destroy(swr_batch_plan *plan) {
   if (hipGetDevice(&dev) == hipSuccess && dev == plan->device) (void)hipDeviceSynchronize();
   plan->pipe.release();
   plan->flags.release();
+  plan->gparams.release();
+  plan->gbase.release();
   delete plan;
   return SWR_OK;
 }

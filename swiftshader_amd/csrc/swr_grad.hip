@@ -281,10 +281,11 @@ __device__ void fragment_backward(const DrawParams &P, const GradParams &G, cons
   }
 }
 
+// Gradients of pixel (px, py) of draw P; rec holds the draw's records
+// (slot t = primitive t).
 template <int MS>
-__global__ __launch_bounds__(256) void backward_kernel(DrawParams P, GradParams G, const TriRecord *__restrict__ rec) {
-  const int px = blockIdx.x * 16 + (threadIdx.x & 15);
-  const int py = blockIdx.y * 16 + (threadIdx.x >> 4);
+__device__ __forceinline__ void backward_pixel(const DrawParams &P, const GradParams &G,
+                                               const TriRecord *__restrict__ rec, int px, int py) {
   if (px >= P.width || py >= P.height) return;
   uint32_t id[MS];
 #pragma unroll
@@ -324,6 +325,35 @@ __global__ __launch_bounds__(256) void backward_kernel(DrawParams P, GradParams 
     }
     fragment_backward(P, G, R, t, px, py, g, L);
   }
+}
+
+// One draw: 16x16 pixels per workgroup.
+template <int MS>
+__global__ __launch_bounds__(256) void backward_kernel(DrawParams P, GradParams G, const TriRecord *__restrict__ rec) {
+  backward_pixel<MS>(P, G, rec, blockIdx.x * 16 + (threadIdx.x & 15), blockIdx.y * 16 + (threadIdx.x >> 4));
+}
+
+// A batch segment: workgroup b renders 16x16 pixels of the draw d in
+// [drawLo, drawHi) with blockBase[d] <= b < blockBase[d + 1] (the draw's
+// blocks in row-major order over its framebuffer).
+template <int MS>
+__global__ __launch_bounds__(256) void backward_batch_kernel(const DrawParams *__restrict__ params,
+                                                             const GradParams *__restrict__ grads,
+                                                             const uint32_t *__restrict__ blockBase, int drawLo,
+                                                             int drawHi, uint32_t blockLo,
+                                                             const TriRecord *__restrict__ rec) {
+  const uint32_t b = blockLo + blockIdx.x;
+  int lo = drawLo, hi = drawHi;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (blockBase[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  const DrawParams &P = params[lo];
+  const uint32_t local = b - blockBase[lo];
+  const uint32_t bx = (uint32_t)(P.width + 15) >> 4;
+  const int px = (int)(local % bx) * 16 + (threadIdx.x & 15), py = (int)(local / bx) * 16 + (threadIdx.x >> 4);
+  backward_pixel<MS>(P, grads[lo], rec + P.triBase, px, py);
 }
 
 // 2x2 box filter (odd sizes: floor, the second texel clamped).
@@ -376,5 +406,9 @@ __global__ void mip_up_rgba32f_kernel(float4 *fine, int fw, int fh, int fp, cons
 
 template __global__ void backward_kernel<1>(DrawParams, GradParams, const TriRecord *);
 template __global__ void backward_kernel<4>(DrawParams, GradParams, const TriRecord *);
+template __global__ void backward_batch_kernel<1>(const DrawParams *, const GradParams *, const uint32_t *, int, int,
+                                                  uint32_t, const TriRecord *);
+template __global__ void backward_batch_kernel<4>(const DrawParams *, const GradParams *, const uint32_t *, int, int,
+                                                  uint32_t, const TriRecord *);
 
 }  // namespace swr

@@ -44,6 +44,7 @@ class TextureData:
     max_anisotropy: float = 1.0
     min_lod: float = 0.0
     max_lod: float = 1000.0            # VK_LOD_CLAMP_NONE; clamped to 13
+    size: int = 0                      # level-0 side when `levels` is left for the device to fill
 
 
 @dataclass
@@ -173,7 +174,10 @@ def _log_uniform_int(rng, lo, hi):
 
 
 def batched(seed=3, draws=512, tri_lo=2000, tri_hi=50000, rt_lo=64, rt_hi=1024, tex_lo=256, tex_hi=2048,
-            scale=1.0):
+            scale=1.0, host_textures=True):
+    """The batched workload.  With host_textures=False the textures' texels
+    are not generated here (levels stays empty, TextureData.size holds the
+    side): the caller fills them on the device."""
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(draws):
@@ -183,8 +187,8 @@ def batched(seed=3, draws=512, tri_lo=2000, tri_hi=50000, rt_lo=64, rt_hi=1024, 
         ts = _log_uniform_int(rng, tex_lo, tex_hi)
         pos, uv, col = screen_triangles(rng, n, w, h, area_hi=min(1e4, w * h / 4))
         mode = abi.ADDRESS_CLAMP_TO_EDGE if rng.integers(0, 2) else abi.ADDRESS_REPEAT
-        tex = TextureData(format=abi.FORMAT_R8G8B8A8_UNORM, levels=box_mip_chain(noise_texture_rgba8(rng, ts)),
-                          address_u=mode, address_v=mode)
+        levels = box_mip_chain(noise_texture_rgba8(rng, ts)) if host_textures else []
+        tex = TextureData(format=abi.FORMAT_R8G8B8A8_UNORM, levels=levels, address_u=mode, address_v=mode, size=ts)
         out.append(DrawData(width=w, height=h, positions=pos, colors=col, texcoords=uv,
                             shader=abi.SHADER_MODULATE, texture=tex, blend_enable=1,
                             blend=(abi.BLEND_FACTOR_SRC_ALPHA, abi.BLEND_FACTOR_ONE_MINUS_SRC_ALPHA,

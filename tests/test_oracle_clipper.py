@@ -89,3 +89,16 @@ def test_clipper_invariants():
             assert (w + y >= -scale).all()
         if flags & CLIP_TOP:
             assert (w - y >= -scale).all()
+
+
+def test_clipper_bit_exact_vs_golden_vectors():
+    """The committed vectors of the compiled reference (tests/golden/make_ref_vectors.py)."""
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "clipper_ref.npz")
+    z = np.load(path)
+    L = oracle()
+    for tri, f, vis, n, poly in zip(z["triangles"], z["flags"], z["visible"], z["count"], z["polygon"]):
+        r, p = run(L.swo_clip_triangle, list(tri), int(f))
+        assert r == vis
+        assert len(p) == n
+        assert np.array_equal(p.view(np.uint32), poly[:n].view(np.uint32))

@@ -127,3 +127,15 @@ def test_constants_match_compiled_reference():
     assert list(cw) == mine[3]
     assert [struct.pack("f", v) for v in uw] == [struct.pack("f", v) for v in mine[4]]
     assert [struct.pack("f", v) for v in us] == [struct.pack("f", v) for v in mine[5]]
+
+
+def test_constants_match_golden_vectors():
+    """The committed tables of the compiled reference (tests/golden/make_ref_vectors.py)."""
+    import os
+    import numpy as np
+    z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "constants_ref.npz"))
+    xf, yf, ms, cw, uw, us = _oracle_constants()
+    assert xf == list(z["xf"]) and yf == list(z["yf"]) and ms == list(z["ms_offsets"])
+    assert cw == list(z["c_weight"])
+    assert np.array_equal(np.array(uw, np.float32).view(np.uint32), z["uv_weight"].view(np.uint32))
+    assert np.array_equal(np.array(us, np.float32).view(np.uint32), z["uv_start"].view(np.uint32))
