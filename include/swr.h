@@ -103,6 +103,12 @@ extern "C" {
 #define SWR_BLEND_OP_REVERSE_SUBTRACT 2
 #define SWR_BLEND_OP_MIN 3
 #define SWR_BLEND_OP_MAX 4
+/* VkPrimitiveTopology / VkProvokingVertexModeEXT */
+#define SWR_TOPOLOGY_TRIANGLE_LIST 3
+#define SWR_TOPOLOGY_TRIANGLE_STRIP 4
+#define SWR_TOPOLOGY_TRIANGLE_FAN 5
+#define SWR_PROVOKING_VERTEX_FIRST 0
+#define SWR_PROVOKING_VERTEX_LAST 1
 /* VkCullModeFlagBits / VkFrontFace */
 #define SWR_CULL_NONE 0
 #define SWR_CULL_FRONT 1
@@ -225,6 +231,15 @@ typedef struct swr_draw_desc {
    * This is the visibility record swr_draw_backward reads.  It has no
    * counterpart in the reference; it requires an RGBA32F colour attachment. */
   swr_attachment primitive_id;
+
+  /* Primitive assembly (setBatchIndices, src/Device/Renderer.cpp:50-145):
+   * the VkPrimitiveTopology -- TRIANGLE_LIST (3; 0 is read as a list too),
+   * TRIANGLE_STRIP (4) or TRIANGLE_FAN (5) -- over the index buffer (or the
+   * vertex sequence when indices is NULL), and the VkProvokingVertexModeEXT
+   * (0 first, 1 last), which rotates each triangle's vertices.  A list needs
+   * 3 * triangle_count indices, a strip or fan triangle_count + 2. */
+  uint32_t topology;
+  uint32_t provoking_vertex;
 } swr_draw_desc;
 
 typedef struct swr_resolve_desc { /* 4x -> 1x colour resolve */

@@ -54,6 +54,8 @@ class View:
     address_v: int = ADDRESS_REPEAT
     indices: Optional[np.ndarray] = None   # (T*3,) or None for 0,1,2,...
     level_count: int = 0                   # mip levels in the view (0 = the full chain)
+    topology: int = 3                      # VkPrimitiveTopology: 3 list, 4 strip, 5 fan
+    provoking_vertex: int = 0              # 0 first, 1 last
 
 
 @dataclass
@@ -146,10 +148,19 @@ def _screen_constants(view: View):
 
 
 def _vertex_ids(view: View, t: np.ndarray, V: int):
-    if view.indices is None:
-        vi = 3 * t[:, None] + np.arange(3)[None, :]
-    else:
-        vi = view.indices.reshape(-1, 3)[t].astype(np.int64)
+    """Vertex indices of primitives t (setBatchIndices, Renderer.cpp:50-145)."""
+    t = np.asarray(t, np.int64)
+    first = view.provoking_vertex == 0
+    if view.topology == 4:      # strip
+        odd = t & 1
+        p = np.stack([t + (0 if first else 2), t + odd + (1 if first else 0), t + (1 - odd) + (1 if first else 0)], 1)
+    elif view.topology == 5:    # fan
+        a, b, z = t + 1, t + 2, np.zeros_like(t)
+        p = np.stack([a, b, z], 1) if first else np.stack([b, z, a], 1)
+    else:                       # list
+        o = [0, 1, 2] if first else [2, 0, 1]
+        p = 3 * t[:, None] + np.array(o)[None, :]
+    vi = p if view.indices is None else np.asarray(view.indices, np.int64)[p]
     return np.minimum(vi, V - 1)
 
 

@@ -1143,6 +1143,11 @@ static int init_ctx(Ctx *c, const swr_draw_desc *d) {
   c->d = d;
   init_constants();
   if (d->samples != 1 && d->samples != 4) return SWR_ERROR_UNSUPPORTED;
+  if (d->topology != 0 && d->topology != SWR_TOPOLOGY_TRIANGLE_LIST && d->topology != SWR_TOPOLOGY_TRIANGLE_STRIP &&
+      d->topology != SWR_TOPOLOGY_TRIANGLE_FAN)
+    return SWR_ERROR_UNSUPPORTED;
+  if (d->provoking_vertex > 1) return SWR_ERROR_INVALID_ARGUMENT;
+  if (d->triangle_count && d->vertex_count == 0) return SWR_ERROR_INVALID_ARGUMENT;
   const swr_viewport *vp = &d->viewport;
   float W = 0.5f * vp->width, H = 0.5f * vp->height;
   float X0 = vp->x + W, Y0 = vp->y + H;
@@ -1247,6 +1252,28 @@ static void fetch_vertex(const swr_draw_desc *d, uint32_t idx, OVertex *v) {
   if (d->texcoords) memcpy(v->v + 4, d->texcoords + 2 * (size_t)idx, 8);
 }
 
+/* Vertex indices of triangle t in the reference's batch order
+ * (setBatchIndices, Renderer.cpp:50-145): list, strip or fan, rotated for
+ * the last-vertex provoking mode. */
+static void assemble(const swr_draw_desc *d, uint32_t t, uint32_t out[3]) {
+  const int first = d->provoking_vertex == SWR_PROVOKING_VERTEX_FIRST;
+  uint32_t p[3];
+  if (d->topology == SWR_TOPOLOGY_TRIANGLE_STRIP) {
+    p[0] = t + (first ? 0u : 2u);
+    p[1] = t + (t & 1u) + (first ? 1u : 0u);
+    p[2] = t + (~t & 1u) + (first ? 1u : 0u);
+  } else if (d->topology == SWR_TOPOLOGY_TRIANGLE_FAN) {
+    p[first ? 0 : 2] = t + 1u;
+    p[first ? 1 : 0] = t + 2u;
+    p[first ? 2 : 1] = 0u;
+  } else {
+    p[0] = 3u * t + (first ? 0u : 2u);
+    p[1] = 3u * t + (first ? 1u : 0u);
+    p[2] = 3u * t + (first ? 2u : 1u);
+  }
+  for (int k = 0; k < 3; k++) out[k] = d->indices ? d->indices[p[k]] : p[k];
+}
+
 /* Renders one draw with host pointers in the descriptor. */
 EXPORT int swo_draw(const swr_draw_desc *d) {
   if (!d || !d->positions) return SWR_ERROR_INVALID_ARGUMENT;
@@ -1257,8 +1284,10 @@ EXPORT int swo_draw(const swr_draw_desc *d) {
   if (r == SWR_OK) {
     for (uint32_t t = 0; t < d->triangle_count; t++) {
       OVertex V[3];
+      uint32_t vidx[3];
+      assemble(d, t, vidx);
       for (int k = 0; k < 3; k++) {
-        uint32_t idx = d->indices ? d->indices[3 * (size_t)t + k] : 3 * t + (uint32_t)k;
+        uint32_t idx = vidx[k] < d->vertex_count ? vidx[k] : d->vertex_count - 1;
         fetch_vertex(d, idx, &V[k]);
         vertex_routine(&c, &V[k]);
       }

@@ -40,6 +40,9 @@ BLEND_FACTOR_SRC_ALPHA_SATURATE = 14
 
 BLEND_OP_ADD, BLEND_OP_SUBTRACT, BLEND_OP_REVERSE_SUBTRACT, BLEND_OP_MIN, BLEND_OP_MAX = 0, 1, 2, 3, 4
 
+TOPOLOGY_TRIANGLE_LIST, TOPOLOGY_TRIANGLE_STRIP, TOPOLOGY_TRIANGLE_FAN = 3, 4, 5
+PROVOKING_VERTEX_FIRST, PROVOKING_VERTEX_LAST = 0, 1
+
 CULL_NONE, CULL_FRONT, CULL_BACK, CULL_FRONT_AND_BACK = 0, 1, 2, 3
 FRONT_FACE_COUNTER_CLOCKWISE, FRONT_FACE_CLOCKWISE = 0, 1
 
@@ -107,6 +110,7 @@ class DrawDesc(C.Structure):
         ("color", Attachment), ("depth", Attachment), ("stencil", Attachment),
         ("occlusion_count", C.c_void_p),
         ("primitive_id", Attachment),
+        ("topology", C.c_uint32), ("provoking_vertex", C.c_uint32),
     ]
 
 

@@ -217,8 +217,18 @@ int build_params(const swr_draw_desc *d, const swr_tuning &tuning, DrawParams &P
       (uint64_t)ra.y + ra.height > d->framebuffer_height)
     return fail(SWR_ERROR_INVALID_ARGUMENT, "render area outside the framebuffer");
   if (d->triangle_count > (1u << 26)) return fail(SWR_ERROR_UNSUPPORTED, "too many triangles (%u)", d->triangle_count);
-  if (!d->indices && (uint64_t)d->triangle_count * 3 > d->vertex_count)
-    return fail(SWR_ERROR_INVALID_ARGUMENT, "non-indexed draw needs 3 vertices per triangle");
+  if (d->topology != 0 && d->topology != SWR_TOPOLOGY_TRIANGLE_LIST && d->topology != SWR_TOPOLOGY_TRIANGLE_STRIP &&
+      d->topology != SWR_TOPOLOGY_TRIANGLE_FAN)
+    return fail(SWR_ERROR_UNSUPPORTED, "topology %u (triangle list, strip and fan are supported)", d->topology);
+  if (d->provoking_vertex > 1) return fail(SWR_ERROR_INVALID_ARGUMENT, "provoking_vertex %u", d->provoking_vertex);
+  P.topology = d->topology == SWR_TOPOLOGY_TRIANGLE_STRIP ? 1 : (d->topology == SWR_TOPOLOGY_TRIANGLE_FAN ? 2 : 0);
+  P.provokeLast = d->provoking_vertex == SWR_PROVOKING_VERTEX_LAST;
+  {
+    const uint64_t need = P.topology == 0 ? (uint64_t)d->triangle_count * 3 : (uint64_t)d->triangle_count + 2;
+    if (!d->indices && d->triangle_count && need > d->vertex_count)
+      return fail(SWR_ERROR_INVALID_ARGUMENT, "non-indexed draw needs %llu vertices", (unsigned long long)need);
+    if (d->triangle_count && d->vertex_count == 0) return fail(SWR_ERROR_INVALID_ARGUMENT, "vertex_count is 0");
+  }
   if (d->shader > SWR_SHADER_MODULATE) return fail(SWR_ERROR_INVALID_ARGUMENT, "unknown shader %u", d->shader);
   if (d->shader != SWR_SHADER_SAMPLE && !d->colors)
     return fail(SWR_ERROR_INVALID_ARGUMENT, "shader reads colours but colors is NULL");

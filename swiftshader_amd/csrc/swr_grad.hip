@@ -156,21 +156,17 @@ __device__ __forceinline__ void texture_backward(const TexParams &T, const GradP
   } while (i < N);
 }
 
-__device__ __forceinline__ uint32_t vertex_index(const DrawParams &P, uint32_t t, int k) {
-  uint32_t vi = P.idx ? P.idx[3 * (size_t)t + k] : 3u * t + (uint32_t)k;
-  return vi < (uint32_t)P.vertexCount ? vi : (uint32_t)P.vertexCount - 1u;
-}
 
 // Gradient of one fragment: primitive t at pixel (px, py), upstream g.
 __device__ void fragment_backward(const DrawParams &P, const GradParams &G, const TriRecord &R, uint32_t t, int px,
                                   int py, const Acc4 &g, const QuadLod &L) {
   uint32_t vi[3];
+  tri_vertices(P, t, vi);
   double X[3], Y[3], rhw[3], x[3], y[3], w[3];
   const double X0 = (double)P.X0xF / 16.0, Y0 = (double)P.Y0xF / 16.0;
   const double Wh = (double)P.WxF / 16.0, Hh = (double)P.HxF / 16.0;
 #pragma unroll
   for (int k = 0; k < 3; k++) {
-    vi[k] = vertex_index(P, t, k);
     const float4 p = reinterpret_cast<const float4 *>(P.pos)[vi[k]];
     x[k] = p.x;
     y[k] = p.y;

@@ -97,18 +97,40 @@ __device__ int clip_polygon(ClipVtx (&poly)[kMaxPolygon], int &n, int flagsOr) {
 __device__ __forceinline__ int wadd(int a, int b) { return (int)((unsigned)a + (unsigned)b); }
 __device__ __forceinline__ int wsub(int a, int b) { return (int)((unsigned)a - (unsigned)b); }
 
+// Vertex indices of primitive t in the reference's batch order
+// (setBatchIndices, Renderer.cpp:50-145); out-of-range indices read the
+// last vertex instead of faulting.
+__device__ __forceinline__ void tri_vertices(const DrawParams &P, uint32_t t, uint32_t vi[3]) {
+  const uint32_t f = P.provokeLast ? 0u : 1u;  // 1: first-vertex mode
+  uint32_t p0, p1, p2;
+  if (P.topology == 1) {
+    p0 = t + (f ? 0u : 2u);
+    p1 = t + (t & 1u) + f;
+    p2 = t + (~t & 1u) + f;
+  } else if (P.topology == 2) {
+    const uint32_t a = t + 1u, b = t + 2u;
+    p0 = f ? a : b;
+    p1 = f ? b : 0u;
+    p2 = f ? 0u : a;
+  } else {
+    p0 = 3u * t + (f ? 0u : 2u);
+    p1 = 3u * t + (f ? 1u : 0u);
+    p2 = 3u * t + (f ? 2u : 1u);
+  }
+  vi[0] = P.idx ? P.idx[p0] : p0;
+  vi[1] = P.idx ? P.idx[p1] : p1;
+  vi[2] = P.idx ? P.idx[p2] : p2;
+#pragma unroll
+  for (int k = 0; k < 3; k++) vi[k] = vi[k] < (uint32_t)P.vertexCount ? vi[k] : (uint32_t)P.vertexCount - 1u;
+}
+
 // Setup of primitive t of the draw P, written to record slot g (the
 // primitive's index in the batch); `draw` is the draw's index in the batch.
 __device__ __forceinline__ void setup_one(const DrawParams &P, int t, size_t g, int draw, TriRecord *__restrict__ rec,
                                           int *__restrict__ poly, uint32_t *__restrict__ tileCount) {
 
   uint32_t vi[3];
-#pragma unroll
-  for (int k = 0; k < 3; k++) {
-    vi[k] = P.idx ? P.idx[3 * t + k] : (uint32_t)(3 * t + k);
-    // Out-of-range indices read the last vertex instead of faulting.
-    vi[k] = vi[k] < (uint32_t)P.vertexCount ? vi[k] : (uint32_t)P.vertexCount - 1;
-  }
+  tri_vertices(P, (uint32_t)t, vi);
 
   float pos[3][4], attr[3][6];
   int flags[3], PX[3], PY[3];

@@ -51,6 +51,8 @@ struct DrawParams {
   const uint32_t *idx;
   int triCount;
   int vertexCount;
+  int topology;     // 0 list, 1 strip, 2 fan
+  int provokeLast;  // VK_PROVOKING_VERTEX_MODE_LAST_VERTEX
   // Viewport / scissor (Renderer.cpp:300-345).
   float WxF, HxF, X0xF, Y0xF, depthRange, depthNear;
   int sx0, sx1, sy0, sy1;

@@ -109,4 +109,6 @@ def build_desc(draw: DrawData, buf: Buffers) -> abi.DrawDesc:
     if buf.stencil:
         d.stencil = abi.Attachment(buf.stencil, abi.FORMAT_S8_UINT, buf.stencil_pitch, buf.stencil_sample_pitch)
     d.occlusion_count = buf.occlusion or None
+    d.topology = draw.topology
+    d.provoking_vertex = draw.provoking_vertex
     return d

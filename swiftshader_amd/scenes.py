@@ -80,12 +80,15 @@ class DrawData:
     clear_stencil: int = 0
     has_depth: bool = True
     has_stencil: bool = False
+    topology: int = abi.TOPOLOGY_TRIANGLE_LIST
+    provoking_vertex: int = abi.PROVOKING_VERTEX_FIRST
 
     @property
     def triangle_count(self):
-        if self.indices is not None:
-            return len(self.indices) // 3
-        return len(self.positions) // 3
+        n = len(self.indices) if self.indices is not None else len(self.positions)
+        if self.topology in (abi.TOPOLOGY_TRIANGLE_STRIP, abi.TOPOLOGY_TRIANGLE_FAN):
+            return max(n - 2, 0)
+        return n // 3
 
 
 def box_mip_chain(level0: np.ndarray) -> List[np.ndarray]:

@@ -109,6 +109,65 @@ def _indexed(seed=17, size=64):
                     shader=abi.SHADER_INTERPOLATE, depth_compare=abi.COMPARE_ALWAYS)
 
 
+def _strip_band(seed=18, size=64, rows=6, provoking=abi.PROVOKING_VERTEX_FIRST, indexed=True, clip=False):
+    """Horizontal bands drawn as one triangle strip (degenerate joins between
+    rows), jittered vertices with perspective w; optionally reaching past
+    the frustum so that clipped polygons start at rotated vertices."""
+    rng = np.random.default_rng(seed)
+    cols = 10
+    span = 1.6 if clip else 1.05
+    xs = np.linspace(-span, span, cols)
+    ys = np.linspace(-span, span, rows + 1)
+    grid = np.stack(np.meshgrid(xs, ys, indexing="xy"), -1).reshape(-1, 2)  # (rows + 1) x cols, row-major
+    grid = grid + rng.uniform(-0.03, 0.03, grid.shape)
+    order = []
+    for r in range(rows):
+        row = []
+        for c in range(cols):
+            row += [r * cols + c, (r + 1) * cols + c]
+        if order:  # degenerate join to the next row
+            order += [order[-1], row[0]]
+        order += row
+    v = grid
+    w = rng.uniform(0.6, 2.0, len(v))
+    z = rng.uniform(-0.2 if clip else 0.05, 0.95, len(v))
+    pos = np.stack([v[:, 0] * w, v[:, 1] * w, z * w, w], 1).astype(np.float32)
+    col = rng.uniform(0, 1, (len(v), 4)).astype(np.float32)
+    uv = rng.uniform(-1, 2, (len(v), 2)).astype(np.float32)
+    tex = TextureData(format=abi.FORMAT_R8G8B8A8_UNORM, levels=scenes.box_mip_chain(scenes.noise_texture_rgba8(rng, 32)))
+    if indexed:
+        idx = np.asarray(order, np.uint32)
+    else:
+        pos, col, uv = pos[order], col[order], uv[order]
+        idx = None
+    return DrawData(width=size, height=size - 5, positions=pos, colors=col, texcoords=uv, indices=idx,
+                    shader=abi.SHADER_MODULATE, texture=tex, topology=abi.TOPOLOGY_TRIANGLE_STRIP,
+                    provoking_vertex=provoking, depth_compare=abi.COMPARE_LESS_OR_EQUAL)
+
+
+def _fan(seed=19, size=72, n=40, provoking=abi.PROVOKING_VERTEX_LAST, samples=1):
+    """A star-shaped polygon drawn as a triangle fan around its centre."""
+    rng = np.random.default_rng(seed)
+    ang = np.sort(rng.uniform(0, 2 * np.pi, n))
+    rad = rng.uniform(0.5, 1.3, n)
+    hub = np.array([[0.0, 0.0]])
+    rim = np.stack([np.cos(ang) * rad, np.sin(ang) * rad], 1)
+    rim = np.concatenate([rim, rim[:1]])
+    v = np.concatenate([hub, rim])
+    w = rng.uniform(0.7, 1.8, len(v))
+    z = rng.uniform(0.1, 0.9, len(v))
+    pos = np.stack([v[:, 0] * w, v[:, 1] * w, z * w, w], 1).astype(np.float32)
+    col = rng.uniform(0, 1, (len(v), 4)).astype(np.float32)
+    return DrawData(width=size, height=size, positions=pos, colors=col, shader=abi.SHADER_INTERPOLATE,
+                    topology=abi.TOPOLOGY_TRIANGLE_FAN, provoking_vertex=provoking, samples=samples)
+
+
+def _list_provoking_last(seed=20):
+    d = _clip_heavy(seed=seed, n=300)
+    d.provoking_vertex = abi.PROVOKING_VERTEX_LAST
+    return d
+
+
 def parity_cases():
     """(name, DrawData) pairs; every one must match the oracle bit for bit."""
     return [
@@ -130,4 +189,10 @@ def parity_cases():
         ("blend_variety", _blend_variety()),
         ("scissor_cull_viewport", _scissor_cull()),
         ("indexed_grid", _indexed()),
+        ("strip_indexed", _strip_band()),
+        ("strip_provoking_last_clipped", _strip_band(seed=29, provoking=abi.PROVOKING_VERTEX_LAST, indexed=False,
+                                                     clip=True)),
+        ("fan_provoking_last", _fan()),
+        ("fan_msaa4", _fan(seed=30, provoking=abi.PROVOKING_VERTEX_FIRST, samples=4)),
+        ("list_provoking_last_clipped", _list_provoking_last()),
     ]

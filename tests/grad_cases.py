@@ -22,7 +22,9 @@ def view_of(d: DrawData) -> "G.View":
         mip = G.MIP_LINEAR if t.mipmap_mode == abi.MIPMAP_MODE_LINEAR else G.MIP_POINT
         au, av, nlev = t.address_u, t.address_v, len(t.levels)
     vp = d.viewport[:4] if d.viewport is not None else (0, 0, d.width, d.height)
-    return G.View(d.width, d.height, tuple(vp), d.shader, filt, mip, au, av, d.indices, nlev)
+    topo = d.topology if d.topology in (abi.TOPOLOGY_TRIANGLE_STRIP, abi.TOPOLOGY_TRIANGLE_FAN) else 3
+    return G.View(d.width, d.height, tuple(vp), d.shader, filt, mip, au, av, d.indices, nlev, topo,
+                  d.provoking_vertex)
 
 
 def params_of(d: DrawData) -> dict:

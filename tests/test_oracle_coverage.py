@@ -105,3 +105,45 @@ def test_shared_edges_are_watertight():
     _, _, s, occ = oracle_render(d)
     assert (s == 1).all()
     assert occ == d.width * d.height
+
+
+def test_strip_and_fan_cover_their_area_once():
+    """A jittered multi-row strip (with degenerate joins) and a fan around an
+    interior hub tile their area: no sample is covered twice (stencil
+    INCREMENT) and, away from the outline, none is missed -- which fails if
+    the strip's alternating winding or the fan's hub order is wrong."""
+    from cases import _fan, _strip_band
+    for d in (_strip_band(seed=40), _fan(seed=41, provoking=abi.PROVOKING_VERTEX_FIRST),
+              _fan(seed=42, provoking=abi.PROVOKING_VERTEX_LAST)):
+        d.has_stencil = True
+        d.stencil_test = 1
+        d.depth_test = 0
+        d.has_depth = False
+        d.front = d.back = dict(compare_op=abi.COMPARE_ALWAYS, pass_op=abi.STENCIL_OP_INCREMENT_AND_WRAP,
+                                fail_op=abi.STENCIL_OP_KEEP, depth_fail_op=abi.STENCIL_OP_INCREMENT_AND_WRAP)
+        d.clear_stencil = 0
+        _, _, s, occ = oracle_render(d)
+        assert s.max() == 1
+        assert occ == int((s == 1).sum()) > 500
+    # The strip spans the whole target: every pixel is covered exactly once.
+    d = _strip_band(seed=43)
+    d.has_stencil, d.stencil_test, d.depth_test, d.has_depth = True, 1, 0, False
+    d.front = d.back = dict(compare_op=abi.COMPARE_ALWAYS, pass_op=abi.STENCIL_OP_INCREMENT_AND_WRAP,
+                            fail_op=abi.STENCIL_OP_KEEP, depth_fail_op=abi.STENCIL_OP_INCREMENT_AND_WRAP)
+    _, _, s, _ = oracle_render(d)
+    assert (s == 1).all()
+
+
+def test_provoking_last_is_the_rotated_list():
+    """A triangle list in last-vertex mode renders as the first-vertex list
+    with each triangle's vertices rotated to (v2, v0, v1) (Renderer.cpp:100-111)."""
+    from cases import _list_provoking_last
+    d = _list_provoking_last()
+    a = oracle_render(d)
+    t = d.triangle_count
+    order = (3 * np.arange(t)[:, None] + np.array([2, 0, 1])[None, :]).reshape(-1)
+    import dataclasses
+    e = dataclasses.replace(d, provoking_vertex=abi.PROVOKING_VERTEX_FIRST, positions=d.positions[order],
+                            colors=d.colors[order], texcoords=d.texcoords[order])
+    b = oracle_render(e)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1].view(np.uint32), b[1].view(np.uint32)) and a[3] == b[3]
