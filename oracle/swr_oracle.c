@@ -588,13 +588,25 @@ static void sample_quad16(const OTexture *t, float u, float v, float lod, int se
   }
 }
 
+/* Row of the anisotropic weight tables for probe count N.  The reference
+ * addresses them as constants + 16 * N (uvWeight, uvStart) and + 8 * N
+ * (cWeight) in wrapping 32-bit Int arithmetic (SamplerCore.cpp:396-398,
+ * 1097-1099).  N is RoundInt(anisotropy) in [0, 16], except that an exactly
+ * singular UV Jacobian gives det = -0 (Abs(x) is x > 0 ? x : -x,
+ * Reactor.cpp:3975), anisotropy = -inf and N = INT_MIN, whose offsets
+ * wrap to 0: row 0. */
+static int aniso_row(int N) {
+  int r = (int)((uint32_t)N << 4) >> 4;
+  return (r < 0 || r > 16) ? 0 : r;
+}
+
 /* sampleAniso, 16-bit (:384-466). */
 static void sample_aniso16(const OTexture *t, float u, float v, float lod, float aniso, float uDelta, float vDelta,
                            int second, uint16_t out[4]) {
   if (t->filter != FILTER_ANISOTROPIC) { sample_quad16(t, u, v, lod, second, out); return; }
   int N = rr_round_int(aniso);
-  float A = kUVWeight[N], B = kUVStart[N];
-  uint16_t cw = kCWeight[N];
+  float A = kUVWeight[aniso_row(N)], B = kUVStart[aniso_row(N)];
+  uint16_t cw = kCWeight[aniso_row(N)];
   uint16_t sum[4] = {0, 0, 0, 0};
   float du = uDelta, dv = vDelta;
   float u0 = u + B * du;
@@ -679,7 +691,7 @@ static void sample_float_aniso(const OTexture *t, float u, float v, float lod, f
                                int second, float out[4]) {
   if (t->filter != FILTER_ANISOTROPIC) { sample_float2d(t, u, v, lod, second, out); return; }
   int N = rr_round_int(aniso);
-  float A = kUVWeight[N], B = kUVStart[N];
+  float A = kUVWeight[aniso_row(N)], B = kUVStart[aniso_row(N)];
   float sum[4] = {0, 0, 0, 0};
   float du = uDelta, dv = vDelta;
   float u0 = u + B * du;

@@ -53,6 +53,17 @@ __device__ __constant__ const uint16_t kCWeight[17] = {0xFFFF, 0xFFFF, 0x8000, 0
                                                       0x2492, 0x2000, 0x1C71, 0x1999, 0x1745, 0x1555, 0x13B1,
                                                       0x1249, 0x1111, 0x1000};
 
+// Row of the anisotropic weight tables for probe count N = RoundInt(aniso).
+// The reference addresses the tables at 16 * N / 8 * N bytes in wrapping
+// 32-bit arithmetic (SamplerCore.cpp:396-398, 1097-1099): N is in [0, 16]
+// except for an exactly singular UV Jacobian (det = -0, anisotropy = -inf,
+// N = INT_MIN), whose offsets wrap to row 0.  Anything else (impossible
+// with maxAnisotropy <= 16) is also sent to row 0, so the table read always
+// stays in bounds.
+__device__ __forceinline__ int aniso_row(int N) {
+  const int r = (int)((uint32_t)N << 4) >> 4;
+  return (r < 0 || r > 16) ? 0 : r;
+}
 __device__ __forceinline__ float uv_weight(int n) { return n <= 1 ? 1.0f : 1.0f / (float)n; }
 __device__ __forceinline__ float uv_start(int n) { return n <= 1 ? -0.0f : -(float)(n - 1) / (float)(2 * n); }
 
@@ -134,8 +145,9 @@ __device__ __forceinline__ void sample_aniso16(const TexParams &T, float u, floa
     return;
   }
   int N = rr_round_int(aniso);
-  float A = uv_weight(N), B = uv_start(N);
-  uint32_t cw = kCWeight[N];
+  const int row = aniso_row(N);
+  float A = uv_weight(row), B = uv_start(row);
+  uint32_t cw = kCWeight[row];
   uint32_t sum[4] = {0, 0, 0, 0};
   float du = uDelta, dv = vDelta;
   float u0 = u + B * du;
@@ -219,7 +231,8 @@ __device__ __forceinline__ float4 sample_float_aniso(const TexParams &T, float u
                                                      float uDelta, float vDelta, int second) {
   if (T.filter != FILT_ANISOTROPIC) return sample_float2d(T, u, v, lod, second);
   int N = rr_round_int(aniso);
-  float A = uv_weight(N), B = uv_start(N);
+  const int row = aniso_row(N);
+  float A = uv_weight(row), B = uv_start(row);
   float4 sum = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   float du = uDelta, dv = vDelta;
   float u0 = u + B * du;

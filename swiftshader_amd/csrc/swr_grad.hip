@@ -137,8 +137,8 @@ __device__ __forceinline__ void texture_backward(const TexParams &T, const GradP
   double A = 1.0, B = 0.0;
   if (T.filter == FILT_ANISOTROPIC) {
     N = rr_round_int(L.aniso);
-    A = (double)uv_weight(N);
-    B = (double)uv_start(N);
+    A = (double)uv_weight(aniso_row(N));
+    B = (double)uv_start(aniso_row(N));
   }
   const int lvl0 = select_mip(T, L.lod, 0);
   double wl0 = 1.0, wl1 = 0.0;

@@ -168,6 +168,28 @@ def _list_provoking_last(seed=20):
     return d
 
 
+def _singular_uv_aniso(seed=31, float_tex=False, samples=1):
+    """Anisotropic filtering where every triangle's v is constant: the quad's
+    UV Jacobian is exactly singular (det = +0 -> -0 through Abs), so the
+    reference's anisotropy is -inf and its table offsets wrap to row 0."""
+    rng = np.random.default_rng(seed)
+    pos, uv, col = scenes.screen_triangles(rng, 300, 80, 64, area_lo=30, area_hi=900)
+    uv = uv.reshape(-1, 3, 2)
+    uv[:, :, 1] = uv[:, :1, 1]
+    uv = uv.reshape(-1, 2).astype(np.float32)
+    if float_tex:
+        tex = TextureData(format=abi.FORMAT_R32G32B32A32_SFLOAT, levels=scenes.float_texture(rng, 64),
+                          anisotropy_enable=1, max_anisotropy=8.0)
+        fmt = abi.FORMAT_R32G32B32A32_SFLOAT
+    else:
+        tex = TextureData(format=abi.FORMAT_R8G8B8A8_UNORM,
+                          levels=scenes.box_mip_chain(scenes.noise_texture_rgba8(rng, 64)),
+                          anisotropy_enable=1, max_anisotropy=8.0)
+        fmt = abi.FORMAT_R8G8B8A8_UNORM
+    return DrawData(width=80, height=64, positions=pos, colors=col, texcoords=uv, samples=samples,
+                    shader=abi.SHADER_MODULATE, texture=tex, color_format=fmt)
+
+
 def parity_cases():
     """(name, DrawData) pairs; every one must match the oracle bit for bit."""
     return [
@@ -195,4 +217,6 @@ def parity_cases():
         ("fan_provoking_last", _fan()),
         ("fan_msaa4", _fan(seed=30, provoking=abi.PROVOKING_VERTEX_FIRST, samples=4)),
         ("list_provoking_last_clipped", _list_provoking_last()),
+        ("singular_uv_aniso_rgba8", _singular_uv_aniso()),
+        ("singular_uv_aniso_float_msaa4", _singular_uv_aniso(seed=32, float_tex=True, samples=4)),
     ]

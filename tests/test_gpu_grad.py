@@ -185,3 +185,33 @@ def test_autograd_path_matches_direct_calls():
         assert gradient_rel_error(a.cpu().numpy(), b.cpu().numpy()) < 1e-6
     want0 = G.box_chain_adjoint([l.cpu().numpy().astype(np.float64) for l in d_lv])
     assert gradient_rel_error(tex0.grad.cpu().numpy(), want0) < 1e-6
+
+
+@pytest.mark.parametrize("which", ["diff_view_512", "msaa_aniso_view"])
+def test_backward_at_full_view_size(which):
+    """One view at the sizes the fitting workloads use: 512^2, 20k triangles,
+    1024^2 trilinear texture; and a 4x MSAA view with 8x anisotropic
+    filtering of a 2048^2 texture.  Every gradient kind within 1e-4 of the
+    FP64 oracle; 16 parameters also against central differences."""
+    if which == "diff_view_512":
+        d = scenes.diff_views(seed=4, views=1)[0]
+    else:
+        d = scenes.msaa_aniso(seed=5, views=1, tri_lo=20000, tri_hi=20001)[0]
+    print(f"[gpu] full-size backward {which} {d.width}x{d.height} s={d.samples} tris={d.triangle_count}", flush=True)
+    gpu, frag, g = run_backward(d, 70)
+    view = view_of(d)
+    params = params_of(d)
+    want = G.gradient(view, params, frag, g)
+    errs = {k: gradient_rel_error(gpu[k], want[k]) for k in ("positions", "colors", "texcoords")}
+    lv = [gradient_rel_error(a, b) for a, b in zip(gpu["texture_levels"], want["texture_levels"])
+          if np.abs(b).max() > 0]
+    print(f"[gpu] relative errors {errs} texture levels max {max(lv):.2e}", flush=True)
+    assert max(errs.values()) < TOL and max(lv) < TOL
+    rng = np.random.default_rng(71)
+    used = np.unique(G._vertex_ids(view, frag.t, len(d.positions)))
+    for i in range(16):
+        kind = ("positions", "colors", "texcoords")[i % 3]
+        idx = (int(rng.choice(used)), int(rng.choice([0, 1, 3]) if kind == "positions" else rng.integers(0, 2)))
+        fd = fd_derivative(view, params, frag, g, kind, idx)
+        got = float(gpu[kind][idx])
+        assert abs(got - fd) / max(abs(fd), 1e-3 * np.abs(gpu[kind]).max()) < TOL, (kind, idx, got, fd)

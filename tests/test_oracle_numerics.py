@@ -181,3 +181,19 @@ def test_constant_texture_is_exact_through_all_filters():
         # 16-bit path: c<<8 weighted by 0.16 weights summing to 0xFFFF, scaled by 1/0xFF00.
         assert np.max(np.abs(got - v / 255.0)) <= 1.5 / 255.0
         assert len(got) > 100
+
+
+def test_singular_uv_jacobian_takes_table_row_zero():
+    """With v constant per triangle the anisotropic LOD's determinant is +0,
+    the reference's Abs turns it into -0 and anisotropy into -inf; RoundInt
+    gives INT_MIN and the wrapped table offsets read row 0 (one probe,
+    weight 0xFFFF).  The oracle renders such scenes on both texture paths
+    with finite output (before the row was wrapped it read the tables out of
+    bounds); the GPU parity scenes of the same name check the bits."""
+    from cases import _singular_uv_aniso
+    d = _singular_uv_aniso()
+    c, _, _, occ = oracle_render(d)
+    assert occ > 1000
+    f = _singular_uv_aniso(seed=32, float_tex=True, samples=4)
+    cf, _, _, occf = oracle_render(f)
+    assert occf > 1000 and np.isfinite(cf).all()
