@@ -11,9 +11,10 @@
 //   * attributes interpolated at the pixel centre with perspective-correct
 //     barycentrics of the UNSNAPPED projected vertices, so that positions
 //     have a gradient (the forward's 28.4 snapping has zero derivative).
-// The per-fragment chain runs in FP64 (screen positions near 2^10 pixels
-// and texel coordinates near 2^12 lose too much in FP32 for 1e-4 gradient
-// agreement); the results are accumulated into FP32 buffers with float
+// The barycentric chain and texel coordinates run in FP64 (screen positions
+// near 2^10 pixels and texel coordinates near 2^12 lose too much in FP32 for
+// 1e-4 gradient agreement); per-tap products run in FP32, and the results
+// are summed per workgroup in LDS, then added to FP32 buffers with float
 // atomics.
 //
 //   backward_kernel    one thread per pixel; per distinct owning primitive
@@ -42,13 +43,6 @@ struct GradParams {
   int statePitch;
   long long stateSlice;
 };
-
-__device__ __forceinline__ void atomic_add4(float *p, double a, double b, double c, double d) {
-  unsafeAtomicAdd(p + 0, (float)a);
-  unsafeAtomicAdd(p + 1, (float)b);
-  unsafeAtomicAdd(p + 2, (float)c);
-  unsafeAtomicAdd(p + 3, (float)d);
-}
 
 // One axis of the float-path address computation (SamplerCore.cpp:2463-2638)
 // in FP64: tap indices x0/x1, the filter fraction f and df/d(uvw).
@@ -94,11 +88,126 @@ __device__ __forceinline__ void axis64(int filter, double uvw, int dim, int clam
 
 struct Acc4 { double x, y, z, w; };
 
+// ---- workgroup aggregation of the gradient scatter ---------------------------
+// The pixels of a workgroup (16x16 of one view) hit few primitives and a
+// compact set of texels, so their float atomics are first summed in LDS
+// hash tables (linear probing on 32-bit keys, LDS float adds) and reach
+// global memory once per distinct key when the workgroup ends.  Keys that
+// find no slot within kProbes go straight to global memory.
+constexpr int kTexSlots = 1024;  // texel entries (key: level << 27 | texel index)
+constexpr int kTriSlots = 64;    // primitive entries
+constexpr int kTriVals = 27;     // per primitive: 3 x (dx, dy, dw), 3 x rgba, 3 x uv
+constexpr int kProbes = 16;
+constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;
+
+struct GradLds {
+  uint32_t texKey[kTexSlots];
+  float4 texVal[kTexSlots];
+  uint32_t triKey[kTriSlots];
+  float triVal[kTriSlots][kTriVals];
+};
+
+__device__ __forceinline__ void lds_init(GradLds &S) {
+  for (int i = threadIdx.x; i < kTexSlots; i += blockDim.x) {
+    S.texKey[i] = kEmptyKey;
+    S.texVal[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+  for (int i = threadIdx.x; i < kTriSlots; i += blockDim.x) S.triKey[i] = kEmptyKey;
+  for (int i = threadIdx.x; i < kTriSlots * kTriVals; i += blockDim.x) (&S.triVal[0][0])[i] = 0.0f;
+  __syncthreads();
+}
+
+__device__ __forceinline__ void texel_add(GradLds &S, float *g, int lvl, uint32_t idx, float a, float b, float c,
+                                          float d) {
+  if (idx < (1u << 27)) {
+    const uint32_t key = ((uint32_t)lvl << 27) | idx;
+    const uint32_t h = (key * 2654435761u) >> 22;
+    for (int p = 0; p < kProbes; p++) {
+      const uint32_t sl = (h + (uint32_t)p) & (kTexSlots - 1);
+      const uint32_t old = atomicCAS(&S.texKey[sl], kEmptyKey, key);
+      if (old == kEmptyKey || old == key) {
+        atomicAdd(&S.texVal[sl].x, a);
+        atomicAdd(&S.texVal[sl].y, b);
+        atomicAdd(&S.texVal[sl].z, c);
+        atomicAdd(&S.texVal[sl].w, d);
+        return;
+      }
+    }
+  }
+  float *o = g + 4 * (size_t)idx;
+  unsafeAtomicAdd(o + 0, a);
+  unsafeAtomicAdd(o + 1, b);
+  unsafeAtomicAdd(o + 2, c);
+  unsafeAtomicAdd(o + 3, d);
+}
+
+// Slot of primitive t in the workgroup table, or NULL when the table is full.
+__device__ __forceinline__ float *tri_slot(GradLds &S, uint32_t t) {
+  const uint32_t h = (t * 2654435761u) >> 26;
+  for (int p = 0; p < kProbes; p++) {
+    const uint32_t sl = (h + (uint32_t)p) & (kTriSlots - 1);
+    const uint32_t old = atomicCAS(&S.triKey[sl], kEmptyKey, t);
+    if (old == kEmptyKey || old == t) return S.triVal[sl];
+  }
+  return nullptr;
+}
+
+// Adds the per-primitive values v[27] of primitive t to the gradient buffers.
+__device__ __forceinline__ void tri_scatter_global(const DrawParams &P, const GradParams &G, uint32_t t,
+                                                   const float *v) {
+  uint32_t vi[3];
+  tri_vertices(P, t, vi);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    if (G.dPos) {
+      float *o = G.dPos + 4 * (size_t)vi[k];
+      unsafeAtomicAdd(o + 0, v[3 * k + 0]);
+      unsafeAtomicAdd(o + 1, v[3 * k + 1]);
+      unsafeAtomicAdd(o + 3, v[3 * k + 2]);
+    }
+    if (G.dCol) {
+      float *o = G.dCol + 4 * (size_t)vi[k];
+      unsafeAtomicAdd(o + 0, v[9 + 4 * k + 0]);
+      unsafeAtomicAdd(o + 1, v[9 + 4 * k + 1]);
+      unsafeAtomicAdd(o + 2, v[9 + 4 * k + 2]);
+      unsafeAtomicAdd(o + 3, v[9 + 4 * k + 3]);
+    }
+    if (G.dUV) {
+      float *o = G.dUV + 2 * (size_t)vi[k];
+      unsafeAtomicAdd(o + 0, v[21 + 2 * k + 0]);
+      unsafeAtomicAdd(o + 1, v[21 + 2 * k + 1]);
+    }
+  }
+}
+
+// Flushes the workgroup tables to global memory (all threads).
+__device__ __forceinline__ void lds_flush(GradLds &S, const DrawParams &P, const GradParams &G) {
+  __syncthreads();
+  for (int i = threadIdx.x; i < kTexSlots; i += blockDim.x) {
+    const uint32_t key = S.texKey[i];
+    if (key == kEmptyKey) continue;
+    float *g = G.dTex[key >> 27];
+    float *o = g + 4 * (size_t)(key & ((1u << 27) - 1));
+    const float4 v = S.texVal[i];
+    unsafeAtomicAdd(o + 0, v.x);
+    unsafeAtomicAdd(o + 1, v.y);
+    unsafeAtomicAdd(o + 2, v.z);
+    unsafeAtomicAdd(o + 3, v.w);
+  }
+  for (int i = threadIdx.x; i < kTriSlots; i += blockDim.x) {
+    const uint32_t t = S.triKey[i];
+    if (t != kEmptyKey) tri_scatter_global(P, G, t, S.triVal[i]);
+  }
+}
+
 // One bilinear (or point) footprint at level `lvl` with weight `wgt`:
 // adds wgt * filter(u, v) to `val`, wgt * d(filter . gTex)/du, /dv to dU, dV,
-// and scatters gTex * wgt * tap weight into the level's texel gradient.
-__device__ __forceinline__ void tap_backward(const TexParams &T, const GradParams &G, int lvl, double u, double v,
-                                             double wgt, const Acc4 &gTex, Acc4 &val, double &dU, double &dV) {
+// and scatters gTex * wgt * tap weight into the level's texel gradient.  The
+// texel coordinate and its fraction are formed in FP64; the per-tap
+// products in FP32.
+__device__ __forceinline__ void tap_backward(GradLds &S, const TexParams &T, const GradParams &G, int lvl, double u,
+                                             double v, double wgt, const float4 &gTex, float4 &val, float &dU,
+                                             float &dV) {
   int x0, x1, y0, y1;
   double fu, fv, dfu, dfv;
   axis64(T.filter, u, T.width[lvl], T.addrU, x0, x1, fu, dfu);
@@ -107,32 +216,34 @@ __device__ __forceinline__ void tap_backward(const TexParams &T, const GradParam
   const float4 *tex = reinterpret_cast<const float4 *>(T.data[lvl]);
   const int i00 = x0 + y0 * pitch, i10 = x1 + y0 * pitch, i01 = x0 + y1 * pitch, i11 = x1 + y1 * pitch;
   const float4 c00 = tex[i00], c10 = tex[i10], c01 = tex[i01], c11 = tex[i11];
-  const double w00 = (1.0 - fu) * (1.0 - fv), w10 = fu * (1.0 - fv), w01 = (1.0 - fu) * fv, w11 = fu * fv;
-  val.x += wgt * (w00 * c00.x + w10 * c10.x + w01 * c01.x + w11 * c11.x);
-  val.y += wgt * (w00 * c00.y + w10 * c10.y + w01 * c01.y + w11 * c11.y);
-  val.z += wgt * (w00 * c00.z + w10 * c10.z + w01 * c01.z + w11 * c11.z);
-  val.w += wgt * (w00 * c00.w + w10 * c10.w + w01 * c01.w + w11 * c11.w);
-  const double d00 = gTex.x * c00.x + gTex.y * c00.y + gTex.z * c00.z + gTex.w * c00.w;
-  const double d10 = gTex.x * c10.x + gTex.y * c10.y + gTex.z * c10.z + gTex.w * c10.w;
-  const double d01 = gTex.x * c01.x + gTex.y * c01.y + gTex.z * c01.z + gTex.w * c01.w;
-  const double d11 = gTex.x * c11.x + gTex.y * c11.y + gTex.z * c11.z + gTex.w * c11.w;
-  dU += wgt * dfu * ((1.0 - fv) * (d10 - d00) + fv * (d11 - d01));
-  dV += wgt * dfv * ((1.0 - fu) * (d01 - d00) + fu * (d11 - d10));
+  const float w00 = (float)(wgt * ((1.0 - fu) * (1.0 - fv))), w10 = (float)(wgt * (fu * (1.0 - fv)));
+  const float w01 = (float)(wgt * ((1.0 - fu) * fv)), w11 = (float)(wgt * (fu * fv));
+  val.x += w00 * c00.x + w10 * c10.x + w01 * c01.x + w11 * c11.x;
+  val.y += w00 * c00.y + w10 * c10.y + w01 * c01.y + w11 * c11.y;
+  val.z += w00 * c00.z + w10 * c10.z + w01 * c01.z + w11 * c11.z;
+  val.w += w00 * c00.w + w10 * c10.w + w01 * c01.w + w11 * c11.w;
+  const float d00 = gTex.x * c00.x + gTex.y * c00.y + gTex.z * c00.z + gTex.w * c00.w;
+  const float d10 = gTex.x * c10.x + gTex.y * c10.y + gTex.z * c10.z + gTex.w * c10.w;
+  const float d01 = gTex.x * c01.x + gTex.y * c01.y + gTex.z * c01.z + gTex.w * c01.w;
+  const float d11 = gTex.x * c11.x + gTex.y * c11.y + gTex.z * c11.z + gTex.w * c11.w;
+  const float ffu = (float)fu, ffv = (float)fv;
+  dU += (float)(wgt * dfu) * ((1.0f - ffv) * (d10 - d00) + ffv * (d11 - d01));
+  dV += (float)(wgt * dfv) * ((1.0f - ffu) * (d01 - d00) + ffu * (d11 - d10));
   float *g = G.dTex[lvl];
   if (g) {
-    const double a = wgt * w00, b = wgt * w10, c = wgt * w01, d = wgt * w11;
-    if (a != 0.0) atomic_add4(g + 4 * (size_t)(uint32_t)i00, gTex.x * a, gTex.y * a, gTex.z * a, gTex.w * a);
-    if (b != 0.0) atomic_add4(g + 4 * (size_t)(uint32_t)i10, gTex.x * b, gTex.y * b, gTex.z * b, gTex.w * b);
-    if (c != 0.0) atomic_add4(g + 4 * (size_t)(uint32_t)i01, gTex.x * c, gTex.y * c, gTex.z * c, gTex.w * c);
-    if (d != 0.0) atomic_add4(g + 4 * (size_t)(uint32_t)i11, gTex.x * d, gTex.y * d, gTex.z * d, gTex.w * d);
+    if (w00 != 0.0f) texel_add(S, g, lvl, (uint32_t)i00, gTex.x * w00, gTex.y * w00, gTex.z * w00, gTex.w * w00);
+    if (w10 != 0.0f) texel_add(S, g, lvl, (uint32_t)i10, gTex.x * w10, gTex.y * w10, gTex.z * w10, gTex.w * w10);
+    if (w01 != 0.0f) texel_add(S, g, lvl, (uint32_t)i01, gTex.x * w01, gTex.y * w01, gTex.z * w01, gTex.w * w01);
+    if (w11 != 0.0f) texel_add(S, g, lvl, (uint32_t)i11, gTex.x * w11, gTex.y * w11, gTex.z * w11, gTex.w * w11);
   }
 }
 
 // The forward filter at (u, v) with the frozen state L: anisotropic probes
 // along (uDelta, vDelta) (sampleFloatAniso, SamplerCore.cpp:1085-1141),
 // each trilinear between two levels (sampleTexture128, :200-253).
-__device__ __forceinline__ void texture_backward(const TexParams &T, const GradParams &G, const QuadLod &L, double u,
-                                                 double v, const Acc4 &gTex, Acc4 &val, double &dU, double &dV) {
+__device__ __forceinline__ void texture_backward(GradLds &S, const TexParams &T, const GradParams &G, const QuadLod &L,
+                                                 double u, double v, const float4 &gTex, float4 &val, float &dU,
+                                                 float &dV) {
   int N = 1;
   double A = 1.0, B = 0.0;
   if (T.filter == FILT_ANISOTROPIC) {
@@ -150,16 +261,15 @@ __device__ __forceinline__ void texture_backward(const TexParams &T, const GradP
   do {
     const double t = B + (double)i * A;
     const double ui = u + t * (double)L.uDelta, vi = v + t * (double)L.vDelta;
-    tap_backward(T, G, lvl0, ui, vi, A * wl0, gTex, val, dU, dV);
-    if (wl1 != 0.0) tap_backward(T, G, lvl0 + 1, ui, vi, A * wl1, gTex, val, dU, dV);
+    tap_backward(S, T, G, lvl0, ui, vi, A * wl0, gTex, val, dU, dV);
+    if (wl1 != 0.0) tap_backward(S, T, G, lvl0 + 1, ui, vi, A * wl1, gTex, val, dU, dV);
     i++;
   } while (i < N);
 }
 
-
 // Gradient of one fragment: primitive t at pixel (px, py), upstream g.
-__device__ void fragment_backward(const DrawParams &P, const GradParams &G, const TriRecord &R, uint32_t t, int px,
-                                  int py, const Acc4 &g, const QuadLod &L) {
+__device__ void fragment_backward(GradLds &S, const DrawParams &P, const GradParams &G, uint32_t t, int px, int py,
+                                  const Acc4 &g, const QuadLod &L) {
   uint32_t vi[3];
   tri_vertices(P, t, vi);
   double X[3], Y[3], rhw[3], x[3], y[3], w[3];
@@ -185,14 +295,14 @@ __device__ void fragment_backward(const DrawParams &P, const GradParams &G, cons
   lam[0] = gx[0] * ex + gy[0] * ey;
   lam[1] = gx[1] * ex + gy[1] * ey;
   lam[2] = 1.0 - lam[0] - lam[1];
-  double r[3], S = 0.0;
+  double r[3], Ssum = 0.0;
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     r[k] = lam[k] * rhw[k];
-    S += r[k];
+    Ssum += r[k];
   }
-  if (!(S != 0.0)) return;
-  const double iS = 1.0 / S;
+  if (!(Ssum != 0.0)) return;
+  const double iS = 1.0 / Ssum;
   double b[3];
 #pragma unroll
   for (int k = 0; k < 3; k++) b[k] = r[k] * iS;
@@ -200,18 +310,19 @@ __device__ void fragment_backward(const DrawParams &P, const GradParams &G, cons
   // Attributes at the fragment.
   float4 vc[3];
   float2 vt[3];
-  Acc4 col = {1.0, 1.0, 1.0, 1.0};
+  float4 col = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
   double u = 0.0, v = 0.0;
   if (P.useColor) {
-    col = {0.0, 0.0, 0.0, 0.0};
+    double cx = 0.0, cy = 0.0, cz = 0.0, cw = 0.0;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
       vc[k] = reinterpret_cast<const float4 *>(P.col)[vi[k]];
-      col.x += b[k] * vc[k].x;
-      col.y += b[k] * vc[k].y;
-      col.z += b[k] * vc[k].z;
-      col.w += b[k] * vc[k].w;
+      cx += b[k] * vc[k].x;
+      cy += b[k] * vc[k].y;
+      cz += b[k] * vc[k].z;
+      cw += b[k] * vc[k].w;
     }
+    col = make_float4((float)cx, (float)cy, (float)cz, (float)cw);
   }
   if (P.useUV) {
 #pragma unroll
@@ -223,64 +334,77 @@ __device__ void fragment_backward(const DrawParams &P, const GradParams &G, cons
   }
 
   // Shader: out = colour | texture | texture * colour.
-  Acc4 dcol = {0.0, 0.0, 0.0, 0.0};
-  double dU = 0.0, dV = 0.0;
+  const float4 gf = make_float4((float)g.x, (float)g.y, (float)g.z, (float)g.w);
+  float4 dcol = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float dU = 0.0f, dV = 0.0f;
   if (P.useUV) {
-    const Acc4 gTex = P.useColor ? Acc4{g.x * col.x, g.y * col.y, g.z * col.z, g.w * col.w} : g;
-    Acc4 tex = {0.0, 0.0, 0.0, 0.0};
-    texture_backward(P.tex, G, L, u, v, gTex, tex, dU, dV);
-    if (P.useColor) dcol = {g.x * tex.x, g.y * tex.y, g.z * tex.z, g.w * tex.w};
+    const float4 gTex = P.useColor ? make_float4(gf.x * col.x, gf.y * col.y, gf.z * col.z, gf.w * col.w) : gf;
+    float4 tex = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    texture_backward(S, P.tex, G, L, u, v, gTex, tex, dU, dV);
+    if (P.useColor) dcol = make_float4(gf.x * tex.x, gf.y * tex.y, gf.z * tex.z, gf.w * tex.w);
   } else {
-    dcol = g;
+    dcol = gf;
   }
 
   // Attribute gradients and dL/db.
+  float val[kTriVals];
+#pragma unroll
+  for (int k = 0; k < kTriVals; k++) val[k] = 0.0f;
   double Gk[3];
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     Gk[k] = 0.0;
     if (P.useColor) {
-      if (G.dCol) atomic_add4(G.dCol + 4 * (size_t)vi[k], b[k] * dcol.x, b[k] * dcol.y, b[k] * dcol.z, b[k] * dcol.w);
-      Gk[k] += dcol.x * vc[k].x + dcol.y * vc[k].y + dcol.z * vc[k].z + dcol.w * vc[k].w;
+      const float bk = (float)b[k];
+      val[9 + 4 * k + 0] = bk * dcol.x;
+      val[9 + 4 * k + 1] = bk * dcol.y;
+      val[9 + 4 * k + 2] = bk * dcol.z;
+      val[9 + 4 * k + 3] = bk * dcol.w;
+      Gk[k] += (double)(dcol.x * vc[k].x + dcol.y * vc[k].y + dcol.z * vc[k].z + dcol.w * vc[k].w);
     }
     if (P.useUV) {
-      if (G.dUV) {
-        unsafeAtomicAdd(G.dUV + 2 * (size_t)vi[k] + 0, (float)(b[k] * dU));
-        unsafeAtomicAdd(G.dUV + 2 * (size_t)vi[k] + 1, (float)(b[k] * dV));
-      }
-      Gk[k] += dU * vt[k].x + dV * vt[k].y;
+      val[21 + 2 * k + 0] = (float)(b[k] * (double)dU);
+      val[21 + 2 * k + 1] = (float)(b[k] * (double)dV);
+      Gk[k] += (double)dU * vt[k].x + (double)dV * vt[k].y;
     }
   }
-  if (!G.dPos) return;
-  // b = r / sum(r), r_k = lambda_k * rhw_k.
-  const double Abar = b[0] * Gk[0] + b[1] * Gk[1] + b[2] * Gk[2];
-  double dlam[3], drhw[3];
+  if (G.dPos) {
+    // b = r / sum(r), r_k = lambda_k * rhw_k.
+    const double Abar = b[0] * Gk[0] + b[1] * Gk[1] + b[2] * Gk[2];
+    double dlam[3], drhw[3];
 #pragma unroll
-  for (int k = 0; k < 3; k++) {
-    const double dr = (Gk[k] - Abar) * iS;
-    dlam[k] = dr * rhw[k];
-    drhw[k] = dr * lam[k];
-  }
-  // d lambda_i / d X_j = -lambda_j * d lambda_i / d px (likewise for Y).
-  const double Gx = dlam[0] * gx[0] + dlam[1] * gx[1] + dlam[2] * gx[2];
-  const double Gy = dlam[0] * gy[0] + dlam[1] * gy[1] + dlam[2] * gy[2];
+    for (int k = 0; k < 3; k++) {
+      const double dr = (Gk[k] - Abar) * iS;
+      dlam[k] = dr * rhw[k];
+      drhw[k] = dr * lam[k];
+    }
+    // d lambda_i / d X_j = -lambda_j * d lambda_i / d px (likewise for Y).
+    const double Gx = dlam[0] * gx[0] + dlam[1] * gx[1] + dlam[2] * gx[2];
+    const double Gy = dlam[0] * gy[0] + dlam[1] * gy[1] + dlam[2] * gy[2];
 #pragma unroll
-  for (int k = 0; k < 3; k++) {
-    const double dX = -lam[k] * Gx, dY = -lam[k] * Gy;
-    const double dx = dX * rhw[k] * Wh, dy = dY * rhw[k] * Hh;
-    const double dr = drhw[k] + dX * x[k] * Wh + dY * y[k] * Hh;
-    const double dw = w[k] != 0.0 ? -dr * rhw[k] * rhw[k] : 0.0;
-    float *o = G.dPos + 4 * (size_t)vi[k];
-    unsafeAtomicAdd(o + 0, (float)dx);
-    unsafeAtomicAdd(o + 1, (float)dy);
-    unsafeAtomicAdd(o + 3, (float)dw);
+    for (int k = 0; k < 3; k++) {
+      const double dX = -lam[k] * Gx, dY = -lam[k] * Gy;
+      const double dr = drhw[k] + dX * x[k] * Wh + dY * y[k] * Hh;
+      val[3 * k + 0] = (float)(dX * rhw[k] * Wh);
+      val[3 * k + 1] = (float)(dY * rhw[k] * Hh);
+      val[3 * k + 2] = (float)(w[k] != 0.0 ? -dr * rhw[k] * rhw[k] : 0.0);
+    }
   }
+  if (!G.dPos && !G.dCol && !G.dUV) return;
+  float *slot = tri_slot(S, t);
+  if (!slot) {
+    tri_scatter_global(P, G, t, val);
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < kTriVals; k++)
+    if (val[k] != 0.0f) atomicAdd(slot + k, val[k]);
 }
 
 // Gradients of pixel (px, py) of draw P; rec holds the draw's records
 // (slot t = primitive t).
 template <int MS>
-__device__ __forceinline__ void backward_pixel(const DrawParams &P, const GradParams &G,
+__device__ __forceinline__ void backward_pixel(GradLds &S, const DrawParams &P, const GradParams &G,
                                                const TriRecord *__restrict__ rec, int px, int py) {
   if (px >= P.width || py >= P.height) return;
   uint32_t id[MS];
@@ -319,14 +443,17 @@ __device__ __forceinline__ void backward_pixel(const DrawParams &P, const GradPa
           *reinterpret_cast<float4 *>(G.state + q2 * G.stateSlice + (long long)py * G.statePitch + 16ll * px) =
               make_float4(L.lod, L.aniso, L.uDelta, L.vDelta);
     }
-    fragment_backward(P, G, R, t, px, py, g, L);
+    fragment_backward(S, P, G, t, px, py, g, L);
   }
 }
 
 // One draw: 16x16 pixels per workgroup.
 template <int MS>
 __global__ __launch_bounds__(256) void backward_kernel(DrawParams P, GradParams G, const TriRecord *__restrict__ rec) {
-  backward_pixel<MS>(P, G, rec, blockIdx.x * 16 + (threadIdx.x & 15), blockIdx.y * 16 + (threadIdx.x >> 4));
+  __shared__ GradLds S;
+  lds_init(S);
+  backward_pixel<MS>(S, P, G, rec, blockIdx.x * 16 + (threadIdx.x & 15), blockIdx.y * 16 + (threadIdx.x >> 4));
+  lds_flush(S, P, G);
 }
 
 // A batch segment: workgroup b renders 16x16 pixels of the draw d in
@@ -338,6 +465,7 @@ __global__ __launch_bounds__(256) void backward_batch_kernel(const DrawParams *_
                                                              const uint32_t *__restrict__ blockBase, int drawLo,
                                                              int drawHi, uint32_t blockLo,
                                                              const TriRecord *__restrict__ rec) {
+  __shared__ GradLds S;
   const uint32_t b = blockLo + blockIdx.x;
   int lo = drawLo, hi = drawHi;
   while (hi - lo > 1) {
@@ -346,10 +474,13 @@ __global__ __launch_bounds__(256) void backward_batch_kernel(const DrawParams *_
     else hi = mid;
   }
   const DrawParams &P = params[lo];
+  const GradParams &G = grads[lo];
+  lds_init(S);
   const uint32_t local = b - blockBase[lo];
   const uint32_t bx = (uint32_t)(P.width + 15) >> 4;
   const int px = (int)(local % bx) * 16 + (threadIdx.x & 15), py = (int)(local / bx) * 16 + (threadIdx.x >> 4);
-  backward_pixel<MS>(P, grads[lo], rec + P.triBase, px, py);
+  backward_pixel<MS>(S, P, G, rec + P.triBase, px, py);
+  lds_flush(S, P, G);
 }
 
 // 2x2 box filter (odd sizes: floor, the second texel clamped).

@@ -451,14 +451,20 @@ __device__ __forceinline__ uint64_t block_coverage(const VFetch &vtx, int n, int
     vtx(ia, Xa, Ya);
     vtx(ib, Xb, Yb);
     const LaneEdge E = lane_edge(Xa - xo, Ya - yo, Xb - xo, Yb - yo, ox, oy);
+    // Rows of the block the edge spans, as a bit mask, routed to the left or
+    // right span; each row then takes (E, dE/dx) under a select mask
+    // (bitfield insert), so the update has no branches.
+    const int r1 = min(max(E.y1 - oy, 0), 8), r2 = min(max(E.y2 - oy, 0), 8);
+    const uint32_t rows = (0xFFu << r1) & ~(0xFFu << r2) & 0xFFu;
+    const uint32_t rowsR = E.right ? rows : 0u, rowsL = rows ^ rowsR;
 #pragma unroll
     for (int r = 0; r < 8; r++) {
-      const int y = oy + r;
-      if (y >= E.y1 && y < E.y2) {
-        const int v = E.e + E.ey * r;
-        if (E.right) { Rv[r] = v; Rx[r] = E.ex; }
-        else { Lv[r] = v; Lx[r] = E.ex; }
-      }
+      const int v = E.e + E.ey * r;
+      const uint32_t ml = 0u - ((rowsL >> r) & 1u), mr = 0u - ((rowsR >> r) & 1u);
+      Lv[r] = (int)(((uint32_t)v & ml) | ((uint32_t)Lv[r] & ~ml));
+      Lx[r] = (int)(((uint32_t)E.ex & ml) | ((uint32_t)Lx[r] & ~ml));
+      Rv[r] = (int)(((uint32_t)v & mr) | ((uint32_t)Rv[r] & ~mr));
+      Rx[r] = (int)(((uint32_t)E.ex & mr) | ((uint32_t)Rx[r] & ~mr));
     }
   }
   uint64_t mask = 0;
