@@ -703,9 +703,9 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
     qCount = 0;
   };
 
-  // One chunk of up to 64 list entries: lanes holding a candidate (box
-  // meets the block, hierarchical depth test passed) compute its exact
-  // coverage; then the covering primitives are applied in list order.
+  // One batch of up to 64 candidates in list order: each lane computes the
+  // exact coverage of its primitive; then the covering primitives are
+  // applied in list order.
   auto process_chunk = [&](bool act, uint32_t tid) __attribute__((always_inline)) {
     // ---- triangle-parallel coverage of this block ----
     uint32_t mlo[MS], mhi[MS];
@@ -840,6 +840,13 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
     }
   };
 
+  // Candidates (box meets the block, hierarchical depth test passed) are
+  // compacted, in list order, into a per-wave ring, and the exact coverage
+  // runs on full batches of 64: a 64-entry chunk of the list yields about a
+  // third of a wave of candidates.
+  __shared__ uint32_t ring[4][128];
+  uint32_t ringHead = 0, ringCount = 0;  // wave-uniform
+
   for (uint32_t base = start; base < end; base += 64) {
     // ---- block depth bound for the hierarchical test ----
     float zmax = __builtin_inff();
@@ -883,7 +890,28 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
       }
     }
     nTests += test ? 1u : 0u;
-    if (__ballot(test)) process_chunk(test, tid);
+    const uint64_t tb = __ballot(test);
+    if (test) {
+      const uint32_t pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(tb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)tb, 0));
+      ring[wave][(ringHead + ringCount + pos) & 127] = tid;
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    ringCount += (uint32_t)__builtin_popcountll(tb);
+    if (ringCount >= 64) {
+      const uint32_t t2 = ring[wave][(ringHead + lane) & 127];
+      ringHead = (ringHead + 64) & 127;
+      ringCount -= 64;
+      process_chunk(true, t2);
+    }
+  }
+  while (ringCount > 0) {
+    const uint32_t nb = ringCount < 64 ? ringCount : 64;
+    const bool act = (uint32_t)lane < nb;
+    const uint32_t t2 = act ? ring[wave][(ringHead + lane) & 127] : 0u;
+    ringHead = (ringHead + nb) & 127;
+    ringCount -= nb;
+    process_chunk(act, t2);
   }
   if (qCount) flush();
 
