@@ -992,7 +992,7 @@ int plan_upload_grads(DeviceCtx *c, swr_batch_plan *pl, hipStream_t s) {
   return SWR_OK;
 }
 
-int plan_enqueue_backward(swr_batch_plan *pl, hipStream_t s) {
+int plan_enqueue_backward(swr_batch_plan *pl, int cus, hipStream_t s) {
   Pipeline &pipe = pl->pipe;
   for (const auto &sg : pl->L.segs) {
     const DrawParams *params = pipe.params.as<DrawParams>() + sg.first;
@@ -1001,7 +1001,11 @@ int plan_enqueue_backward(swr_batch_plan *pl, hipStream_t s) {
                        pipe.rec.as<TriRecord>(), pipe.poly.as<int>(), pipe.cnt.as<uint32_t>());
     const GradParams *grads = pl->gparams.as<GradParams>() + sg.first;
     const uint32_t *bb = pl->gbase.as<uint32_t>() + sg.first;
-    // Draws of one sample count are contiguous (the layout groups variants).
+    // Draws of one sample count are contiguous (the layout groups variants);
+    // each group gets its own work counter (counters[8 + group]).
+    uint32_t *ctr = pipe.counters.as<uint32_t>() + 10;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, ctr, 0u, 2u);
+    int group = 0;
     int lo = 0;
     while (lo < sg.count) {
       const int ms = pl->L.P[sg.first + lo].ms;
@@ -1011,13 +1015,15 @@ int plan_enqueue_backward(swr_batch_plan *pl, hipStream_t s) {
       const DrawParams &Ph = pl->L.P[sg.first + hi - 1];
       const uint32_t bHi = pl->bwBase[sg.first + hi - 1] + (uint32_t)(((Ph.width + 15) / 16) * ((Ph.height + 15) / 16));
       if (bHi > bLo) {
+        const uint32_t wgs = std::min<uint32_t>(bHi - bLo, (uint32_t)cus * 2u);
         if (ms == 1)
-          hipLaunchKernelGGL(backward_batch_kernel<1>, dim3(bHi - bLo), dim3(256), 0, s, params, grads, bb, lo, hi, bLo,
-                             (const TriRecord *)pipe.rec.as<TriRecord>());
+          hipLaunchKernelGGL(backward_batch_kernel<1>, dim3(wgs), dim3(256), 0, s, params, grads, bb, lo, hi, bLo, bHi,
+                             ctr + group, (const TriRecord *)pipe.rec.as<TriRecord>());
         else
-          hipLaunchKernelGGL(backward_batch_kernel<4>, dim3(bHi - bLo), dim3(256), 0, s, params, grads, bb, lo, hi, bLo,
-                             (const TriRecord *)pipe.rec.as<TriRecord>());
+          hipLaunchKernelGGL(backward_batch_kernel<4>, dim3(wgs), dim3(256), 0, s, params, grads, bb, lo, hi, bLo, bHi,
+                             ctr + group, (const TriRecord *)pipe.rec.as<TriRecord>());
       }
+      group++;
       lo = hi;
     }
   }
@@ -1048,7 +1054,7 @@ int swr_batch_plan_backward(swr_batch_plan *plan, void *stream) {
   int r = plan_ctx(plan, &c);
   if (r != SWR_OK) return r;
   if (!plan->hasGrads) return fail(SWR_ERROR_INVALID_ARGUMENT, "swr_batch_plan_set_grads has not been called");
-  return plan_enqueue_backward(plan, reinterpret_cast<hipStream_t>(stream));
+  return plan_enqueue_backward(plan, c->cus, reinterpret_cast<hipStream_t>(stream));
 }
 
 int swr_batch_plan_destroy(swr_batch_plan *plan) {

@@ -94,7 +94,7 @@ struct Acc4 { double x, y, z, w; };
 // hash tables (linear probing on 32-bit keys, LDS float adds) and reach
 // global memory once per distinct key when the workgroup ends.  Keys that
 // find no slot within kProbes go straight to global memory.
-constexpr int kTexSlots = 1024;  // texel entries (key: level << 27 | texel index)
+constexpr int kTexSlots = 2048;  // texel entries (key: level << 27 | texel index)
 constexpr int kTriSlots = 64;    // primitive entries
 constexpr int kTriVals = 27;     // per primitive: 3 x (dx, dy, dw), 3 x rgba, 3 x uv
 constexpr int kProbes = 16;
@@ -105,15 +105,26 @@ struct GradLds {
   float4 texVal[kTexSlots];
   uint32_t triKey[kTriSlots];
   float triVal[kTriSlots][kTriVals];
+  uint32_t texCount, triCount;  // occupied slots
 };
 
-__device__ __forceinline__ void lds_init(GradLds &S) {
+__device__ __forceinline__ void lds_init_tex(GradLds &S) {
   for (int i = threadIdx.x; i < kTexSlots; i += blockDim.x) {
     S.texKey[i] = kEmptyKey;
     S.texVal[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   }
+  if (threadIdx.x == 0) S.texCount = 0;
+}
+
+__device__ __forceinline__ void lds_init_tri(GradLds &S) {
   for (int i = threadIdx.x; i < kTriSlots; i += blockDim.x) S.triKey[i] = kEmptyKey;
   for (int i = threadIdx.x; i < kTriSlots * kTriVals; i += blockDim.x) (&S.triVal[0][0])[i] = 0.0f;
+  if (threadIdx.x == 0) S.triCount = 0;
+}
+
+__device__ __forceinline__ void lds_init(GradLds &S) {
+  lds_init_tex(S);
+  lds_init_tri(S);
   __syncthreads();
 }
 
@@ -125,6 +136,7 @@ __device__ __forceinline__ void texel_add(GradLds &S, float *g, int lvl, uint32_
     for (int p = 0; p < kProbes; p++) {
       const uint32_t sl = (h + (uint32_t)p) & (kTexSlots - 1);
       const uint32_t old = atomicCAS(&S.texKey[sl], kEmptyKey, key);
+      if (old == kEmptyKey) atomicAdd(&S.texCount, 1u);
       if (old == kEmptyKey || old == key) {
         atomicAdd(&S.texVal[sl].x, a);
         atomicAdd(&S.texVal[sl].y, b);
@@ -147,6 +159,7 @@ __device__ __forceinline__ float *tri_slot(GradLds &S, uint32_t t) {
   for (int p = 0; p < kProbes; p++) {
     const uint32_t sl = (h + (uint32_t)p) & (kTriSlots - 1);
     const uint32_t old = atomicCAS(&S.triKey[sl], kEmptyKey, t);
+    if (old == kEmptyKey) atomicAdd(&S.triCount, 1u);
     if (old == kEmptyKey || old == t) return S.triVal[sl];
   }
   return nullptr;
@@ -180,9 +193,9 @@ __device__ __forceinline__ void tri_scatter_global(const DrawParams &P, const Gr
   }
 }
 
-// Flushes the workgroup tables to global memory (all threads).
-__device__ __forceinline__ void lds_flush(GradLds &S, const DrawParams &P, const GradParams &G) {
-  __syncthreads();
+// Flushes the texel table to the gradient buffers of G (all threads, after
+// a barrier).
+__device__ __forceinline__ void lds_flush_tex(GradLds &S, const GradParams &G) {
   for (int i = threadIdx.x; i < kTexSlots; i += blockDim.x) {
     const uint32_t key = S.texKey[i];
     if (key == kEmptyKey) continue;
@@ -194,10 +207,20 @@ __device__ __forceinline__ void lds_flush(GradLds &S, const DrawParams &P, const
     unsafeAtomicAdd(o + 2, v.z);
     unsafeAtomicAdd(o + 3, v.w);
   }
+}
+
+__device__ __forceinline__ void lds_flush_tri(GradLds &S, const DrawParams &P, const GradParams &G) {
   for (int i = threadIdx.x; i < kTriSlots; i += blockDim.x) {
     const uint32_t t = S.triKey[i];
     if (t != kEmptyKey) tri_scatter_global(P, G, t, S.triVal[i]);
   }
+}
+
+// Flushes both tables (all threads).
+__device__ __forceinline__ void lds_flush(GradLds &S, const DrawParams &P, const GradParams &G) {
+  __syncthreads();
+  lds_flush_tex(S, G);
+  lds_flush_tri(S, P, G);
 }
 
 // One bilinear (or point) footprint at level `lvl` with weight `wgt`:
@@ -456,31 +479,66 @@ __global__ __launch_bounds__(256) void backward_kernel(DrawParams P, GradParams 
   lds_flush(S, P, G);
 }
 
-// A batch segment: workgroup b renders 16x16 pixels of the draw d in
-// [drawLo, drawHi) with blockBase[d] <= b < blockBase[d + 1] (the draw's
-// blocks in row-major order over its framebuffer).
+// A batch segment, persistent: workgroups take 16x16-pixel blocks
+// [blockLo, blockHi) from an atomic counter (block b belongs to the draw d in
+// [drawLo, drawHi) with blockBase[d] <= b < blockBase[d + 1], in row-major
+// order over its framebuffer) and keep their LDS tables across blocks: the
+// texel table is flushed when it passes half full or the next block writes
+// other texture-gradient buffers, the primitive table when it passes half
+// full or the next block belongs to another draw, and both at the end.
+// Views that share a texture thus reach its gradient once per table
+// rather than once per block.
 template <int MS>
 __global__ __launch_bounds__(256) void backward_batch_kernel(const DrawParams *__restrict__ params,
                                                              const GradParams *__restrict__ grads,
                                                              const uint32_t *__restrict__ blockBase, int drawLo,
-                                                             int drawHi, uint32_t blockLo,
+                                                             int drawHi, uint32_t blockLo, uint32_t blockHi,
+                                                             uint32_t *__restrict__ counter,
                                                              const TriRecord *__restrict__ rec) {
   __shared__ GradLds S;
-  const uint32_t b = blockLo + blockIdx.x;
-  int lo = drawLo, hi = drawHi;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (blockBase[mid] <= b) lo = mid;
-    else hi = mid;
-  }
-  const DrawParams &P = params[lo];
-  const GradParams &G = grads[lo];
+  __shared__ uint32_t sBlock;
   lds_init(S);
-  const uint32_t local = b - blockBase[lo];
-  const uint32_t bx = (uint32_t)(P.width + 15) >> 4;
-  const int px = (int)(local % bx) * 16 + (threadIdx.x & 15), py = (int)(local / bx) * 16 + (threadIdx.x >> 4);
-  backward_pixel<MS>(S, P, G, rec + P.triBase, px, py);
-  lds_flush(S, P, G);
+  int cur = -1;  // draw whose gradients the tables hold (wave-uniform)
+  for (;;) {
+    if (threadIdx.x == 0) sBlock = blockLo + atomicAdd(counter, 1u);
+    __syncthreads();
+    const uint32_t b = __builtin_amdgcn_readfirstlane(sBlock);
+    const uint32_t texCount = __builtin_amdgcn_readfirstlane(S.texCount);
+    const uint32_t triCount = __builtin_amdgcn_readfirstlane(S.triCount);
+    __syncthreads();
+    int lo = -1;
+    if (b < blockHi) {
+      int l = drawLo, hi = drawHi;
+      while (hi - l > 1) {
+        const int mid = (l + hi) >> 1;
+        if (blockBase[mid] <= b) l = mid;
+        else hi = mid;
+      }
+      lo = __builtin_amdgcn_readfirstlane(l);
+    }
+    if (cur >= 0) {
+      const bool texSwitch = lo < 0 || grads[lo].dTex[0] != grads[cur].dTex[0];
+      const bool triSwitch = lo != cur;
+      const bool ft = texSwitch || texCount > kTexSlots / 2;
+      const bool fr = triSwitch || triCount > kTriSlots / 2;
+      if (ft) lds_flush_tex(S, grads[cur]);
+      if (fr) lds_flush_tri(S, params[cur], grads[cur]);
+      if (ft || fr) {
+        __syncthreads();
+        if (ft) lds_init_tex(S);
+        if (fr) lds_init_tri(S);
+        __syncthreads();
+      }
+    }
+    if (lo < 0) break;
+    cur = lo;
+    const DrawParams &P = params[lo];
+    const uint32_t local = b - blockBase[lo];
+    const uint32_t bx = (uint32_t)(P.width + 15) >> 4;
+    const int px = (int)(local % bx) * 16 + (threadIdx.x & 15), py = (int)(local / bx) * 16 + (threadIdx.x >> 4);
+    backward_pixel<MS>(S, P, grads[lo], rec + P.triBase, px, py);
+    __syncthreads();
+  }
 }
 
 // 2x2 box filter (odd sizes: floor, the second texel clamped).
@@ -534,8 +592,8 @@ __global__ void mip_up_rgba32f_kernel(float4 *fine, int fw, int fh, int fp, cons
 template __global__ void backward_kernel<1>(DrawParams, GradParams, const TriRecord *);
 template __global__ void backward_kernel<4>(DrawParams, GradParams, const TriRecord *);
 template __global__ void backward_batch_kernel<1>(const DrawParams *, const GradParams *, const uint32_t *, int, int,
-                                                  uint32_t, const TriRecord *);
+                                                  uint32_t, uint32_t, uint32_t *, const TriRecord *);
 template __global__ void backward_batch_kernel<4>(const DrawParams *, const GradParams *, const uint32_t *, int, int,
-                                                  uint32_t, const TriRecord *);
+                                                  uint32_t, uint32_t, uint32_t *, const TriRecord *);
 
 }  // namespace swr
