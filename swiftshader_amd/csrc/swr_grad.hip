@@ -88,32 +88,83 @@ __device__ __forceinline__ void axis64(int filter, double uvw, int dim, int clam
 
 struct Acc4 { double x, y, z, w; };
 
-// ---- workgroup aggregation of the gradient scatter ---------------------------
-// The pixels of a workgroup (16x16 of one view) hit few primitives and a
-// compact set of texels, so their float atomics are first summed in LDS
-// hash tables (linear probing on 32-bit keys, LDS float adds) and reach
-// global memory once per distinct key when the workgroup ends.  Keys that
-// find no slot within kProbes go straight to global memory.
-constexpr int kTexSlots = 2048;  // texel entries (key: level << 27 | texel index)
-constexpr int kTriSlots = 64;    // primitive entries
-constexpr int kTriVals = 27;     // per primitive: 3 x (dx, dy, dw), 3 x rgba, 3 x uv
-constexpr int kProbes = 16;
+// ---- aggregation of the gradient scatter --------------------------------------
+// Float atomics to global memory cost one request per (instruction, cache
+// line): lanes that hit the same line in one instruction share a request,
+// lanes that hit different lines do not.  Texel gradients are therefore
+// first summed in LDS, in one hash table per wave, and reach global memory
+// in flushes where 4 adjacent lanes carry the 4 channels of a texel and 16
+// adjacent lanes the 4 texels of an aligned group (x = 4k..4k+3 of a row):
+// the hash keeps the low two bits of the texel index as the low two bits of
+// the slot and probes in steps of 4, so a group's texels sit in 4 adjacent
+// slots.  A wave flushes its own table (no workgroup barrier) whenever it
+// passes kFlushAt entries before a bilinear footprint, so a table never
+// fills; a key that still finds no slot within kProbes goes straight to
+// global memory.  Primitive gradients (27 values per primitive) are summed
+// in one table per workgroup.
+constexpr int kWaves = 4;                // waves per workgroup (256 threads)
+constexpr int kWaveSlots = 512;          // texel entries per wave (key: level << 27 | texel index)
+constexpr uint32_t kFlushAt = 192;       // at most 256 new keys per footprint step: peak < 88 % load
+constexpr int kTriSlots = 64;            // primitive entries per workgroup
+constexpr int kTriVals = 27;             // per primitive: 3 x (dx, dy, dw), 3 x rgba, 3 x uv
+constexpr int kProbes = 8;
 constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;
 
-struct GradLds {
-  uint32_t texKey[kTexSlots];
-  float4 texVal[kTexSlots];
-  uint32_t triKey[kTriSlots];
-  float triVal[kTriSlots][kTriVals];
-  uint32_t texCount, triCount;  // occupied slots
+struct WaveTexTable {
+  uint32_t key[kWaveSlots];
+  float4 val[kWaveSlots];
+  uint32_t count;  // occupied slots
 };
 
-__device__ __forceinline__ void lds_init_tex(GradLds &S) {
-  for (int i = threadIdx.x; i < kTexSlots; i += blockDim.x) {
-    S.texKey[i] = kEmptyKey;
-    S.texVal[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+struct GradLds {
+  WaveTexTable tex[kWaves];
+  uint32_t triKey[kTriSlots];
+  float triVal[kTriSlots][kTriVals];
+  uint32_t triCount;  // occupied slots
+};
+
+__device__ __forceinline__ WaveTexTable &wave_table(GradLds &S) { return S.tex[threadIdx.x >> 6]; }
+
+// Clears the wave's texel table (the whole wave).
+__device__ __forceinline__ void wave_tex_init(WaveTexTable &W) {
+  const int lane = threadIdx.x & 63;
+  for (int i = lane; i < kWaveSlots; i += 64) {
+    W.key[i] = kEmptyKey;
+    W.val[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   }
-  if (threadIdx.x == 0) S.texCount = 0;
+  if (lane == 0) W.count = 0;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Adds the wave's texel table to the gradient buffers of G and clears it.
+// Runs on the wave's active lanes, whichever they are: lane j of n active
+// lanes takes the (slot, channel) pairs j, j + n, ... so that 4 adjacent
+// lanes write the channels of one texel and 16 the texels of one group.
+// LDS operations of a wave complete in order, so the reads precede the
+// clearing writes.
+__device__ __forceinline__ void wave_tex_flush(WaveTexTable &W, const GradParams &G) {
+  const uint64_t act = __ballot(1);
+  const uint32_t n = (uint32_t)__builtin_popcountll(act);
+  const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(act >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0));
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (uint32_t j = rank; j < (uint32_t)kWaveSlots * 4; j += n) {
+    const uint32_t key = W.key[j >> 2];
+    if (key != kEmptyKey) {
+      const float v = reinterpret_cast<const float *>(&W.val[j >> 2])[j & 3];
+      float *g = G.dTex[key >> 27];
+      unsafeAtomicAdd(g + 4 * (size_t)(key & ((1u << 27) - 1)) + (j & 3), v);
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  for (uint32_t j = rank; j < (uint32_t)kWaveSlots; j += n) {
+    W.key[j] = kEmptyKey;
+    W.val[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+  if (rank == 0) W.count = 0;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
 }
 
 __device__ __forceinline__ void lds_init_tri(GradLds &S) {
@@ -123,25 +174,25 @@ __device__ __forceinline__ void lds_init_tri(GradLds &S) {
 }
 
 __device__ __forceinline__ void lds_init(GradLds &S) {
-  lds_init_tex(S);
+  wave_tex_init(wave_table(S));
   lds_init_tri(S);
   __syncthreads();
 }
 
-__device__ __forceinline__ void texel_add(GradLds &S, float *g, int lvl, uint32_t idx, float a, float b, float c,
+__device__ __forceinline__ void texel_add(WaveTexTable &W, float *g, int lvl, uint32_t idx, float a, float b, float c,
                                           float d) {
   if (idx < (1u << 27)) {
     const uint32_t key = ((uint32_t)lvl << 27) | idx;
-    const uint32_t h = (key * 2654435761u) >> 22;
+    const uint32_t h = ((((key >> 2) * 2654435761u) >> 25) << 2) | (key & 3u);
     for (int p = 0; p < kProbes; p++) {
-      const uint32_t sl = (h + (uint32_t)p) & (kTexSlots - 1);
-      const uint32_t old = atomicCAS(&S.texKey[sl], kEmptyKey, key);
-      if (old == kEmptyKey) atomicAdd(&S.texCount, 1u);
+      const uint32_t sl = (h + 4u * (uint32_t)p) & (kWaveSlots - 1);
+      const uint32_t old = atomicCAS(&W.key[sl], kEmptyKey, key);
+      if (old == kEmptyKey) atomicAdd(&W.count, 1u);
       if (old == kEmptyKey || old == key) {
-        atomicAdd(&S.texVal[sl].x, a);
-        atomicAdd(&S.texVal[sl].y, b);
-        atomicAdd(&S.texVal[sl].z, c);
-        atomicAdd(&S.texVal[sl].w, d);
+        atomicAdd(&W.val[sl].x, a);
+        atomicAdd(&W.val[sl].y, b);
+        atomicAdd(&W.val[sl].z, c);
+        atomicAdd(&W.val[sl].w, d);
         return;
       }
     }
@@ -193,22 +244,6 @@ __device__ __forceinline__ void tri_scatter_global(const DrawParams &P, const Gr
   }
 }
 
-// Flushes the texel table to the gradient buffers of G (all threads, after
-// a barrier).
-__device__ __forceinline__ void lds_flush_tex(GradLds &S, const GradParams &G) {
-  for (int i = threadIdx.x; i < kTexSlots; i += blockDim.x) {
-    const uint32_t key = S.texKey[i];
-    if (key == kEmptyKey) continue;
-    float *g = G.dTex[key >> 27];
-    float *o = g + 4 * (size_t)(key & ((1u << 27) - 1));
-    const float4 v = S.texVal[i];
-    unsafeAtomicAdd(o + 0, v.x);
-    unsafeAtomicAdd(o + 1, v.y);
-    unsafeAtomicAdd(o + 2, v.z);
-    unsafeAtomicAdd(o + 3, v.w);
-  }
-}
-
 __device__ __forceinline__ void lds_flush_tri(GradLds &S, const DrawParams &P, const GradParams &G) {
   for (int i = threadIdx.x; i < kTriSlots; i += blockDim.x) {
     const uint32_t t = S.triKey[i];
@@ -218,8 +253,8 @@ __device__ __forceinline__ void lds_flush_tri(GradLds &S, const DrawParams &P, c
 
 // Flushes both tables (all threads).
 __device__ __forceinline__ void lds_flush(GradLds &S, const DrawParams &P, const GradParams &G) {
+  wave_tex_flush(wave_table(S), G);
   __syncthreads();
-  lds_flush_tex(S, G);
   lds_flush_tri(S, P, G);
 }
 
@@ -228,9 +263,9 @@ __device__ __forceinline__ void lds_flush(GradLds &S, const DrawParams &P, const
 // and scatters gTex * wgt * tap weight into the level's texel gradient.  The
 // texel coordinate and its fraction are formed in FP64; the per-tap
 // products in FP32.
-__device__ __forceinline__ void tap_backward(GradLds &S, const TexParams &T, const GradParams &G, int lvl, double u,
-                                             double v, double wgt, const float4 &gTex, float4 &val, float &dU,
-                                             float &dV) {
+__device__ __forceinline__ void tap_backward(WaveTexTable &W, const TexParams &T, const GradParams &G, int lvl,
+                                             double u, double v, double wgt, const float4 &gTex, float4 &val,
+                                             float &dU, float &dV) {
   int x0, x1, y0, y1;
   double fu, fv, dfu, dfv;
   axis64(T.filter, u, T.width[lvl], T.addrU, x0, x1, fu, dfu);
@@ -254,19 +289,21 @@ __device__ __forceinline__ void tap_backward(GradLds &S, const TexParams &T, con
   dV += (float)(wgt * dfv) * ((1.0f - ffu) * (d01 - d00) + ffu * (d11 - d10));
   float *g = G.dTex[lvl];
   if (g) {
-    if (w00 != 0.0f) texel_add(S, g, lvl, (uint32_t)i00, gTex.x * w00, gTex.y * w00, gTex.z * w00, gTex.w * w00);
-    if (w10 != 0.0f) texel_add(S, g, lvl, (uint32_t)i10, gTex.x * w10, gTex.y * w10, gTex.z * w10, gTex.w * w10);
-    if (w01 != 0.0f) texel_add(S, g, lvl, (uint32_t)i01, gTex.x * w01, gTex.y * w01, gTex.z * w01, gTex.w * w01);
-    if (w11 != 0.0f) texel_add(S, g, lvl, (uint32_t)i11, gTex.x * w11, gTex.y * w11, gTex.z * w11, gTex.w * w11);
+    // The active lanes read the same count: the branch is uniform over them.
+    if (__builtin_amdgcn_readfirstlane(W.count) > kFlushAt) wave_tex_flush(W, G);
+    if (w00 != 0.0f) texel_add(W, g, lvl, (uint32_t)i00, gTex.x * w00, gTex.y * w00, gTex.z * w00, gTex.w * w00);
+    if (w10 != 0.0f) texel_add(W, g, lvl, (uint32_t)i10, gTex.x * w10, gTex.y * w10, gTex.z * w10, gTex.w * w10);
+    if (w01 != 0.0f) texel_add(W, g, lvl, (uint32_t)i01, gTex.x * w01, gTex.y * w01, gTex.z * w01, gTex.w * w01);
+    if (w11 != 0.0f) texel_add(W, g, lvl, (uint32_t)i11, gTex.x * w11, gTex.y * w11, gTex.z * w11, gTex.w * w11);
   }
 }
 
 // The forward filter at (u, v) with the frozen state L: anisotropic probes
 // along (uDelta, vDelta) (sampleFloatAniso, SamplerCore.cpp:1085-1141),
 // each trilinear between two levels (sampleTexture128, :200-253).
-__device__ __forceinline__ void texture_backward(GradLds &S, const TexParams &T, const GradParams &G, const QuadLod &L,
-                                                 double u, double v, const float4 &gTex, float4 &val, float &dU,
-                                                 float &dV) {
+__device__ __forceinline__ void texture_backward(WaveTexTable &W, const TexParams &T, const GradParams &G,
+                                                 const QuadLod &L, double u, double v, const float4 &gTex,
+                                                 float4 &val, float &dU, float &dV) {
   int N = 1;
   double A = 1.0, B = 0.0;
   if (T.filter == FILT_ANISOTROPIC) {
@@ -284,8 +321,8 @@ __device__ __forceinline__ void texture_backward(GradLds &S, const TexParams &T,
   do {
     const double t = B + (double)i * A;
     const double ui = u + t * (double)L.uDelta, vi = v + t * (double)L.vDelta;
-    tap_backward(S, T, G, lvl0, ui, vi, A * wl0, gTex, val, dU, dV);
-    if (wl1 != 0.0) tap_backward(S, T, G, lvl0 + 1, ui, vi, A * wl1, gTex, val, dU, dV);
+    tap_backward(W, T, G, lvl0, ui, vi, A * wl0, gTex, val, dU, dV);
+    if (wl1 != 0.0) tap_backward(W, T, G, lvl0 + 1, ui, vi, A * wl1, gTex, val, dU, dV);
     i++;
   } while (i < N);
 }
@@ -363,7 +400,7 @@ __device__ void fragment_backward(GradLds &S, const DrawParams &P, const GradPar
   if (P.useUV) {
     const float4 gTex = P.useColor ? make_float4(gf.x * col.x, gf.y * col.y, gf.z * col.z, gf.w * col.w) : gf;
     float4 tex = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    texture_backward(S, P.tex, G, L, u, v, gTex, tex, dU, dV);
+    texture_backward(wave_table(S), P.tex, G, L, u, v, gTex, tex, dU, dV);
     if (P.useColor) dcol = make_float4(gf.x * tex.x, gf.y * tex.y, gf.z * tex.z, gf.w * tex.w);
   } else {
     dcol = gf;
@@ -424,49 +461,81 @@ __device__ void fragment_backward(GradLds &S, const DrawParams &P, const GradPar
     if (val[k] != 0.0f) atomicAdd(slot + k, val[k]);
 }
 
-// Gradients of pixel (px, py) of draw P; rec holds the draw's records
-// (slot t = primitive t).
+// Per-wave queue of fragments: one entry per (pixel, distinct owning
+// primitive), at most MS per pixel.
+constexpr int kFragQueue = 64 * 4;
+
+struct FragQueue {
+  uint32_t xy[kWaves][kFragQueue];    // x | y << 16
+  uint32_t tri[kWaves][kFragQueue];   // primitive index
+  uint32_t mask[kWaves][kFragQueue];  // samples the fragment owns
+};
+
+// Gradients of the pixels (px, py) of one wave of draw P (rec holds the
+// draw's records: slot t = primitive t).  The wave first queues its pixels'
+// fragments -- the samples of a pixel owned by one primitive share one
+// fragment, shaded once at the pixel centre, whose upstream gradient is the
+// sum of theirs -- then runs them 64 at a time, so edge pixels with several
+// primitives cost extra lanes, not extra passes of the whole wave.
 template <int MS>
-__device__ __forceinline__ void backward_pixel(GradLds &S, const DrawParams &P, const GradParams &G,
-                                               const TriRecord *__restrict__ rec, int px, int py) {
-  if (px >= P.width || py >= P.height) return;
+__device__ __forceinline__ void backward_pixels(GradLds &S, FragQueue &Q, const DrawParams &P, const GradParams &G,
+                                                const TriRecord *__restrict__ rec, int px, int py) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   uint32_t id[MS];
+  const bool inside = px < P.width && py < P.height;
 #pragma unroll
   for (int q = 0; q < MS; q++)
-    id[q] = *reinterpret_cast<const uint32_t *>(P.primId + q * P.primIdSlice + (long long)py * P.primIdPitch + 4ll * px);
+    id[q] = inside ? *reinterpret_cast<const uint32_t *>(P.primId + q * P.primIdSlice + (long long)py * P.primIdPitch +
+                                                         4ll * px)
+                   : 0u;
+  uint32_t count = 0;
 #pragma unroll
   for (int q = 0; q < MS; q++) {
     const uint32_t t1 = id[q];
-    if (!t1 || t1 > (uint32_t)P.triCount) continue;
-    bool seen = false;
+    bool first = t1 != 0u && t1 <= (uint32_t)P.triCount;
 #pragma unroll
-    for (int q2 = 0; q2 < q; q2++) seen |= id[q2] == t1;
-    if (seen) continue;
-    // Samples of this pixel owned by the same primitive share its fragment
-    // (shaded once at the pixel centre): their upstream gradients add.
+    for (int q2 = 0; q2 < q; q2++) first &= id[q2] != t1;
+    uint32_t mask = 0;
+#pragma unroll
+    for (int q2 = q; q2 < MS; q2++) mask |= id[q2] == t1 ? 1u << q2 : 0u;
+    const uint64_t bal = __ballot(first);
+    if (first) {
+      const uint32_t pos = count + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0));
+      Q.xy[wave][pos] = (uint32_t)px | ((uint32_t)py << 16);
+      Q.tri[wave][pos] = t1 - 1u;
+      Q.mask[wave][pos] = mask;
+    }
+    count += (uint32_t)__builtin_popcountll(bal);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (uint32_t base = 0; base < count; base += 64) {
+    if (base + (uint32_t)lane >= count) continue;
+    const uint32_t xy = Q.xy[wave][base + lane], t = Q.tri[wave][base + lane], mask = Q.mask[wave][base + lane];
+    const int x = (int)(xy & 0xFFFFu), y = (int)(xy >> 16);
     Acc4 g = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int q2 = q; q2 < MS; q2++) {
-      if (id[q2] != t1) continue;
-      const float4 d = *reinterpret_cast<const float4 *>(G.dColor + q2 * G.dColorSlice + (long long)py * G.dColorPitch +
-                                                         16ll * px);
+    for (int q = 0; q < MS; q++) {
+      if (!(mask >> q & 1u)) continue;
+      const float4 d = *reinterpret_cast<const float4 *>(G.dColor + q * G.dColorSlice + (long long)y * G.dColorPitch +
+                                                         16ll * x);
       g.x += d.x;
       g.y += d.y;
       g.z += d.z;
       g.w += d.w;
     }
-    const uint32_t t = t1 - 1u;
-    const TriRecord &R = rec[t];
     QuadLod L = {0.0f, 0.0f, 0.0f, 0.0f};
-    if (P.useUV) L = quad_state(P, R, px, py);
+    if (P.useUV) L = quad_state(P, rec[t], x, y);
     if (G.state) {
 #pragma unroll
-      for (int q2 = q; q2 < MS; q2++)
-        if (id[q2] == t1)
-          *reinterpret_cast<float4 *>(G.state + q2 * G.stateSlice + (long long)py * G.statePitch + 16ll * px) =
+      for (int q = 0; q < MS; q++)
+        if (mask >> q & 1u)
+          *reinterpret_cast<float4 *>(G.state + q * G.stateSlice + (long long)y * G.statePitch + 16ll * x) =
               make_float4(L.lod, L.aniso, L.uDelta, L.vDelta);
     }
-    fragment_backward(S, P, G, t, px, py, g, L);
+    fragment_backward(S, P, G, t, x, y, g, L);
   }
 }
 
@@ -474,20 +543,21 @@ __device__ __forceinline__ void backward_pixel(GradLds &S, const DrawParams &P, 
 template <int MS>
 __global__ __launch_bounds__(256) void backward_kernel(DrawParams P, GradParams G, const TriRecord *__restrict__ rec) {
   __shared__ GradLds S;
+  __shared__ FragQueue Q;
   lds_init(S);
-  backward_pixel<MS>(S, P, G, rec, blockIdx.x * 16 + (threadIdx.x & 15), blockIdx.y * 16 + (threadIdx.x >> 4));
+  backward_pixels<MS>(S, Q, P, G, rec, blockIdx.x * 16 + (threadIdx.x & 15), blockIdx.y * 16 + (threadIdx.x >> 4));
   lds_flush(S, P, G);
 }
 
 // A batch segment, persistent: workgroups take 16x16-pixel blocks
 // [blockLo, blockHi) from an atomic counter (block b belongs to the draw d in
 // [drawLo, drawHi) with blockBase[d] <= b < blockBase[d + 1], in row-major
-// order over its framebuffer) and keep their LDS tables across blocks: the
-// texel table is flushed when it passes half full or the next block writes
-// other texture-gradient buffers, the primitive table when it passes half
-// full or the next block belongs to another draw, and both at the end.
-// Views that share a texture thus reach its gradient once per table
-// rather than once per block.
+// order over its framebuffer) and keep their LDS tables across blocks: each
+// wave's texel table flushes itself when it fills and is flushed when the
+// next block writes other texture-gradient buffers; the primitive table is
+// flushed when it passes half full or the next block belongs to another
+// draw; both at the end.  Views that share a texture thus reach its gradient
+// once per table rather than once per block.
 template <int MS>
 __global__ __launch_bounds__(256) void backward_batch_kernel(const DrawParams *__restrict__ params,
                                                              const GradParams *__restrict__ grads,
@@ -496,6 +566,7 @@ __global__ __launch_bounds__(256) void backward_batch_kernel(const DrawParams *_
                                                              uint32_t *__restrict__ counter,
                                                              const TriRecord *__restrict__ rec) {
   __shared__ GradLds S;
+  __shared__ FragQueue Q;
   __shared__ uint32_t sBlock;
   lds_init(S);
   int cur = -1;  // draw whose gradients the tables hold (wave-uniform)
@@ -503,7 +574,6 @@ __global__ __launch_bounds__(256) void backward_batch_kernel(const DrawParams *_
     if (threadIdx.x == 0) sBlock = blockLo + atomicAdd(counter, 1u);
     __syncthreads();
     const uint32_t b = __builtin_amdgcn_readfirstlane(sBlock);
-    const uint32_t texCount = __builtin_amdgcn_readfirstlane(S.texCount);
     const uint32_t triCount = __builtin_amdgcn_readfirstlane(S.triCount);
     __syncthreads();
     int lo = -1;
@@ -517,16 +587,14 @@ __global__ __launch_bounds__(256) void backward_batch_kernel(const DrawParams *_
       lo = __builtin_amdgcn_readfirstlane(l);
     }
     if (cur >= 0) {
-      const bool texSwitch = lo < 0 || grads[lo].dTex[0] != grads[cur].dTex[0];
-      const bool triSwitch = lo != cur;
-      const bool ft = texSwitch || texCount > kTexSlots / 2;
-      const bool fr = triSwitch || triCount > kTriSlots / 2;
-      if (ft) lds_flush_tex(S, grads[cur]);
-      if (fr) lds_flush_tri(S, params[cur], grads[cur]);
-      if (ft || fr) {
+      bool texSwitch = lo < 0;
+      for (int k = 0; k < kMipLevels && !texSwitch; k++) texSwitch = grads[lo].dTex[k] != grads[cur].dTex[k];
+      const bool fr = lo != cur || triCount > kTriSlots / 2;
+      if (texSwitch) wave_tex_flush(wave_table(S), grads[cur]);
+      if (fr) {
+        lds_flush_tri(S, params[cur], grads[cur]);
         __syncthreads();
-        if (ft) lds_init_tex(S);
-        if (fr) lds_init_tri(S);
+        lds_init_tri(S);
         __syncthreads();
       }
     }
@@ -536,7 +604,7 @@ __global__ __launch_bounds__(256) void backward_batch_kernel(const DrawParams *_
     const uint32_t local = b - blockBase[lo];
     const uint32_t bx = (uint32_t)(P.width + 15) >> 4;
     const int px = (int)(local % bx) * 16 + (threadIdx.x & 15), py = (int)(local / bx) * 16 + (threadIdx.x >> 4);
-    backward_pixel<MS>(S, P, grads[lo], rec + P.triBase, px, py);
+    backward_pixels<MS>(S, Q, P, grads[lo], rec + P.triBase, px, py);
     __syncthreads();
   }
 }
