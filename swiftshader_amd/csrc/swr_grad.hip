@@ -104,20 +104,27 @@ struct Acc4 { double x, y, z, w; };
 // in one table per workgroup.
 constexpr int kWaves = 4;                // waves per workgroup (256 threads)
 constexpr int kWaveSlots = 512;          // texel entries per wave (key: level << 27 | texel index)
-constexpr uint32_t kFlushAt = 192;       // at most 256 new keys per footprint step: peak < 88 % load
+constexpr uint32_t kFlushAt = 256;       // at most 256 new keys per footprint step
 constexpr int kTriSlots = 64;            // primitive entries per workgroup
 constexpr int kTriVals = 27;             // per primitive: 3 x (dx, dy, dw), 3 x rgba, 3 x uv
 constexpr int kProbes = 8;
 constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;
 
+// Channel c of slot s sits at val[c * kValStride + s]: one array per
+// channel (a lane's four adds hit four banks that neighbouring slots do not
+// share), padded so that the four channels of one slot fall in different
+// banks when a flush reads them from adjacent lanes.
+constexpr int kValStride = kWaveSlots + 8;
+
 struct WaveTexTable {
   uint32_t key[kWaveSlots];
-  float4 val[kWaveSlots];
+  float val[4 * kValStride];
   uint32_t count;  // occupied slots
 };
 
 struct GradLds {
   WaveTexTable tex[kWaves];
+  float *dTex[kMipLevels];  // texture-gradient buffers of the draw being processed, per level
   uint32_t triKey[kTriSlots];
   float triVal[kTriSlots][kTriVals];
   uint32_t triCount;  // occupied slots
@@ -130,38 +137,39 @@ __device__ __forceinline__ void wave_tex_init(WaveTexTable &W) {
   const int lane = threadIdx.x & 63;
   for (int i = lane; i < kWaveSlots; i += 64) {
     W.key[i] = kEmptyKey;
-    W.val[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+    for (int c = 0; c < 4; c++) W.val[c * kValStride + i] = 0.0f;
   }
   if (lane == 0) W.count = 0;
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
 }
 
-// Adds the wave's texel table to the gradient buffers of G and clears it.
+// Adds the wave's texel table to the gradient buffers dTex (per level, in
+// LDS) and clears it.
 // Runs on the wave's active lanes, whichever they are: lane j of n active
 // lanes takes the (slot, channel) pairs j, j + n, ... so that 4 adjacent
 // lanes write the channels of one texel and 16 the texels of one group.
-// LDS operations of a wave complete in order, so the reads precede the
+// LDS operations of a wave complete in order, so the key reads precede the
 // clearing writes.
-__device__ __forceinline__ void wave_tex_flush(WaveTexTable &W, const GradParams &G) {
+__device__ __forceinline__ void wave_tex_flush(WaveTexTable &W, float *const *dTex) {
   const uint64_t act = __ballot(1);
   const uint32_t n = (uint32_t)__builtin_popcountll(act);
   const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(act >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0));
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // Each value is read and zeroed in one exchange (empty slots hold zeros).
+#pragma unroll 4
   for (uint32_t j = rank; j < (uint32_t)kWaveSlots * 4; j += n) {
     const uint32_t key = W.key[j >> 2];
+    const float v = atomicExch(&W.val[(j & 3) * kValStride + (j >> 2)], 0.0f);
     if (key != kEmptyKey) {
-      const float v = reinterpret_cast<const float *>(&W.val[j >> 2])[j & 3];
-      float *g = G.dTex[key >> 27];
+      float *g = dTex[key >> 27];
       unsafeAtomicAdd(g + 4 * (size_t)(key & ((1u << 27) - 1)) + (j & 3), v);
     }
   }
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
   __builtin_amdgcn_wave_barrier();
-  for (uint32_t j = rank; j < (uint32_t)kWaveSlots; j += n) {
-    W.key[j] = kEmptyKey;
-    W.val[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  }
+  for (uint32_t j = rank; j < (uint32_t)kWaveSlots; j += n) W.key[j] = kEmptyKey;
   if (rank == 0) W.count = 0;
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -173,35 +181,89 @@ __device__ __forceinline__ void lds_init_tri(GradLds &S) {
   if (threadIdx.x == 0) S.triCount = 0;
 }
 
-__device__ __forceinline__ void lds_init(GradLds &S) {
+__device__ __forceinline__ void lds_init(GradLds &S, const GradParams &G) {
   wave_tex_init(wave_table(S));
   lds_init_tri(S);
+  if (threadIdx.x < kMipLevels) S.dTex[threadIdx.x] = G.dTex[threadIdx.x];
   __syncthreads();
 }
 
-__device__ __forceinline__ void texel_add(WaveTexTable &W, float *g, int lvl, uint32_t idx, float a, float b, float c,
-                                          float d) {
-  if (idx < (1u << 27)) {
-    const uint32_t key = ((uint32_t)lvl << 27) | idx;
-    const uint32_t h = ((((key >> 2) * 2654435761u) >> 25) << 2) | (key & 3u);
-    for (int p = 0; p < kProbes; p++) {
-      const uint32_t sl = (h + 4u * (uint32_t)p) & (kWaveSlots - 1);
-      const uint32_t old = atomicCAS(&W.key[sl], kEmptyKey, key);
-      if (old == kEmptyKey) atomicAdd(&W.count, 1u);
-      if (old == kEmptyKey || old == key) {
-        atomicAdd(&W.val[sl].x, a);
-        atomicAdd(&W.val[sl].y, b);
-        atomicAdd(&W.val[sl].z, c);
-        atomicAdd(&W.val[sl].w, d);
-        return;
-      }
+__device__ __forceinline__ uint32_t texel_slot(uint32_t key) {
+  return ((((key >> 2) * 2654435761u) >> 25) << 2) | (key & 3u);
+}
+
+// Adds (a, b, c, d) to texel `key` of the table, probing from the second
+// slot of its sequence (the first was taken); global memory when no slot
+// is found.
+__device__ __forceinline__ void texel_add_slow(WaveTexTable &W, float *g, uint32_t key, float a, float b, float c,
+                                            float d) {
+  const uint32_t h = texel_slot(key);
+  for (int p = 1; p < kProbes; p++) {
+    const uint32_t sl = (h + 4u * (uint32_t)p) & (kWaveSlots - 1);
+    const uint32_t old = atomicCAS(&W.key[sl], kEmptyKey, key);
+    if (old == kEmptyKey) atomicAdd(&W.count, 1u);
+    if (old == kEmptyKey || old == key) {
+      atomicAdd(&W.val[sl], a);
+      atomicAdd(&W.val[kValStride + sl], b);
+      atomicAdd(&W.val[2 * kValStride + sl], c);
+      atomicAdd(&W.val[3 * kValStride + sl], d);
+      return;
     }
   }
-  float *o = g + 4 * (size_t)idx;
+  float *o = g + 4 * (size_t)(key & ((1u << 27) - 1));
   unsafeAtomicAdd(o + 0, a);
   unsafeAtomicAdd(o + 1, b);
   unsafeAtomicAdd(o + 2, c);
   unsafeAtomicAdd(o + 3, d);
+}
+
+// Adds gTex * w[k] to texel idx[k] of level lvl for the 4 taps of a
+// bilinear footprint.  The first probe of all four keys is issued at once
+// (one LDS round trip rather than four); a tap whose key is absent (w = 0,
+// or an index beyond the key range) probes with the empty key, which
+// changes nothing, and adds zeros.  Taps whose first slot holds another
+// key continue in texel_add_slow.
+__device__ __forceinline__ void texels_add4(WaveTexTable &W, float *g, int lvl, const uint32_t idx[4],
+                                            const float w[4], const float4 &gTex) {
+  uint32_t key[4], sl[4], old[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const bool want = w[k] != 0.0f && idx[k] < (1u << 27);
+    key[k] = want ? (((uint32_t)lvl << 27) | idx[k]) : kEmptyKey;
+    sl[k] = texel_slot(key[k]) & (kWaveSlots - 1);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++) old[k] = atomicCAS(&W.key[sl[k]], kEmptyKey, key[k]);
+  uint32_t fresh = 0;
+  bool ok[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    fresh += (key[k] != kEmptyKey && old[k] == kEmptyKey) ? 1u : 0u;
+    ok[k] = key[k] != kEmptyKey && (old[k] == kEmptyKey || old[k] == key[k]);
+  }
+  if (fresh) atomicAdd(&W.count, fresh);
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const float m = ok[k] ? w[k] : 0.0f;
+    atomicAdd(&W.val[sl[k]], gTex.x * m);
+    atomicAdd(&W.val[kValStride + sl[k]], gTex.y * m);
+    atomicAdd(&W.val[2 * kValStride + sl[k]], gTex.z * m);
+    atomicAdd(&W.val[3 * kValStride + sl[k]], gTex.w * m);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    if (w[k] != 0.0f && !ok[k]) {
+      if (idx[k] < (1u << 27)) {
+        texel_add_slow(W, g, key[k], gTex.x * w[k], gTex.y * w[k], gTex.z * w[k], gTex.w * w[k]);
+      } else {
+        float *o = g + 4 * (size_t)idx[k];
+        unsafeAtomicAdd(o + 0, gTex.x * w[k]);
+        unsafeAtomicAdd(o + 1, gTex.y * w[k]);
+        unsafeAtomicAdd(o + 2, gTex.z * w[k]);
+        unsafeAtomicAdd(o + 3, gTex.w * w[k]);
+      }
+    }
+  }
 }
 
 // Slot of primitive t in the workgroup table, or NULL when the table is full.
@@ -253,27 +315,44 @@ __device__ __forceinline__ void lds_flush_tri(GradLds &S, const DrawParams &P, c
 
 // Flushes both tables (all threads).
 __device__ __forceinline__ void lds_flush(GradLds &S, const DrawParams &P, const GradParams &G) {
-  wave_tex_flush(wave_table(S), G);
+  wave_tex_flush(wave_table(S), S.dTex);
   __syncthreads();
   lds_flush_tri(S, P, G);
 }
 
-// One bilinear (or point) footprint at level `lvl` with weight `wgt`:
-// adds wgt * filter(u, v) to `val`, wgt * d(filter . gTex)/du, /dv to dU, dV,
-// and scatters gTex * wgt * tap weight into the level's texel gradient.  The
+// One mip level of the texture and of its gradient, read once per fragment
+// (the level varies per lane: indexing the parameter arrays costs a load).
+struct LevelRef {
+  const float4 *tex;
+  float *grad;
+  int width, height, pitch, lvl;
+};
+
+__device__ __forceinline__ LevelRef level_ref(const TexParams &T, const GradParams &G, int lvl) {
+  LevelRef r;
+  r.tex = reinterpret_cast<const float4 *>(T.data[lvl]);
+  r.grad = G.dTex[lvl];
+  r.width = T.width[lvl];
+  r.height = T.height[lvl];
+  r.pitch = T.pitchP[lvl];
+  r.lvl = lvl;
+  return r;
+}
+
+// One bilinear (or point) footprint at level R with weight `wgt`: adds
+// wgt * filter(u, v) to `val`, wgt * d(filter . gTex)/du, /dv to dU, dV, and
+// scatters gTex * wgt * tap weight into the level's texel gradient.  The
 // texel coordinate and its fraction are formed in FP64; the per-tap
 // products in FP32.
-__device__ __forceinline__ void tap_backward(WaveTexTable &W, const TexParams &T, const GradParams &G, int lvl,
-                                             double u, double v, double wgt, const float4 &gTex, float4 &val,
-                                             float &dU, float &dV) {
+__device__ __forceinline__ void tap_backward(WaveTexTable &W, const TexParams &T, const LevelRef &R, double u,
+                                             double v, double wgt, const float4 &gTex, float4 &val, float &dU,
+                                             float &dV) {
   int x0, x1, y0, y1;
   double fu, fv, dfu, dfv;
-  axis64(T.filter, u, T.width[lvl], T.addrU, x0, x1, fu, dfu);
-  axis64(T.filter, v, T.height[lvl], T.addrV, y0, y1, fv, dfv);
-  const int pitch = T.pitchP[lvl];
-  const float4 *tex = reinterpret_cast<const float4 *>(T.data[lvl]);
-  const int i00 = x0 + y0 * pitch, i10 = x1 + y0 * pitch, i01 = x0 + y1 * pitch, i11 = x1 + y1 * pitch;
-  const float4 c00 = tex[i00], c10 = tex[i10], c01 = tex[i01], c11 = tex[i11];
+  axis64(T.filter, u, R.width, T.addrU, x0, x1, fu, dfu);
+  axis64(T.filter, v, R.height, T.addrV, y0, y1, fv, dfv);
+  const int i00 = x0 + y0 * R.pitch, i10 = x1 + y0 * R.pitch, i01 = x0 + y1 * R.pitch, i11 = x1 + y1 * R.pitch;
+  const float4 c00 = R.tex[i00], c10 = R.tex[i10], c01 = R.tex[i01], c11 = R.tex[i11];
   const float w00 = (float)(wgt * ((1.0 - fu) * (1.0 - fv))), w10 = (float)(wgt * (fu * (1.0 - fv)));
   const float w01 = (float)(wgt * ((1.0 - fu) * fv)), w11 = (float)(wgt * (fu * fv));
   val.x += w00 * c00.x + w10 * c10.x + w01 * c01.x + w11 * c11.x;
@@ -287,23 +366,21 @@ __device__ __forceinline__ void tap_backward(WaveTexTable &W, const TexParams &T
   const float ffu = (float)fu, ffv = (float)fv;
   dU += (float)(wgt * dfu) * ((1.0f - ffv) * (d10 - d00) + ffv * (d11 - d01));
   dV += (float)(wgt * dfv) * ((1.0f - ffu) * (d01 - d00) + ffu * (d11 - d10));
-  float *g = G.dTex[lvl];
+  float *g = R.grad;
   if (g) {
-    // The active lanes read the same count: the branch is uniform over them.
-    if (__builtin_amdgcn_readfirstlane(W.count) > kFlushAt) wave_tex_flush(W, G);
-    if (w00 != 0.0f) texel_add(W, g, lvl, (uint32_t)i00, gTex.x * w00, gTex.y * w00, gTex.z * w00, gTex.w * w00);
-    if (w10 != 0.0f) texel_add(W, g, lvl, (uint32_t)i10, gTex.x * w10, gTex.y * w10, gTex.z * w10, gTex.w * w10);
-    if (w01 != 0.0f) texel_add(W, g, lvl, (uint32_t)i01, gTex.x * w01, gTex.y * w01, gTex.z * w01, gTex.w * w01);
-    if (w11 != 0.0f) texel_add(W, g, lvl, (uint32_t)i11, gTex.x * w11, gTex.y * w11, gTex.z * w11, gTex.w * w11);
+    const uint32_t idx[4] = {(uint32_t)i00, (uint32_t)i10, (uint32_t)i01, (uint32_t)i11};
+    const float w[4] = {w00, w10, w01, w11};
+    texels_add4(W, g, R.lvl, idx, w, gTex);
   }
 }
 
 // The forward filter at (u, v) with the frozen state L: anisotropic probes
 // along (uDelta, vDelta) (sampleFloatAniso, SamplerCore.cpp:1085-1141),
 // each trilinear between two levels (sampleTexture128, :200-253).
-__device__ __forceinline__ void texture_backward(WaveTexTable &W, const TexParams &T, const GradParams &G,
+__device__ __forceinline__ void texture_backward(GradLds &S, const TexParams &T, const GradParams &G,
                                                  const QuadLod &L, double u, double v, const float4 &gTex,
                                                  float4 &val, float &dU, float &dV) {
+  WaveTexTable &W = wave_table(S);
   int N = 1;
   double A = 1.0, B = 0.0;
   if (T.filter == FILT_ANISOTROPIC) {
@@ -317,14 +394,29 @@ __device__ __forceinline__ void texture_backward(WaveTexTable &W, const TexParam
     wl1 = (double)rr_frac(L.lod);
     wl0 = 1.0 - wl1;
   }
-  int i = 0;
-  do {
-    const double t = B + (double)i * A;
-    const double ui = u + t * (double)L.uDelta, vi = v + t * (double)L.vDelta;
-    tap_backward(W, T, G, lvl0, ui, vi, A * wl0, gTex, val, dU, dV);
-    if (wl1 != 0.0) tap_backward(W, T, G, lvl0 + 1, ui, vi, A * wl1, gTex, val, dU, dV);
-    i++;
-  } while (i < N);
+  // Level-major: consecutive probes on one level overlap, so the wave's
+  // table sees repeated keys close together.  The loops run in step over
+  // the wave's active lanes (a lane with fewer probes idles), so that a
+  // table flush, checked once per step, always has the whole set of active
+  // lanes to spread its slots over.
+  const bool anyGrad = G.dTex[0] != nullptr;
+  for (int lv = 0; lv < 2; lv++) {
+    const bool levelOn = lv == 0 || wl1 != 0.0;
+    if (!__ballot(levelOn)) break;
+    const LevelRef R = level_ref(T, G, levelOn ? lvl0 + lv : lvl0);
+    const double wl = lv ? wl1 : wl0;
+    for (int i = 0;; i++) {
+      const bool on = levelOn && i < N;
+      if (!__ballot(on)) break;
+      // The active lanes read the same count: the branch is uniform over them.
+      if (anyGrad && __builtin_amdgcn_readfirstlane(W.count) > kFlushAt) wave_tex_flush(W, S.dTex);
+      if (on) {
+        const double t = B + (double)i * A;
+        const double ui = u + t * (double)L.uDelta, vi = v + t * (double)L.vDelta;
+        tap_backward(W, T, R, ui, vi, A * wl, gTex, val, dU, dV);
+      }
+    }
+  }
 }
 
 // Gradient of one fragment: primitive t at pixel (px, py), upstream g.
@@ -400,7 +492,7 @@ __device__ void fragment_backward(GradLds &S, const DrawParams &P, const GradPar
   if (P.useUV) {
     const float4 gTex = P.useColor ? make_float4(gf.x * col.x, gf.y * col.y, gf.z * col.z, gf.w * col.w) : gf;
     float4 tex = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    texture_backward(wave_table(S), P.tex, G, L, u, v, gTex, tex, dU, dV);
+    texture_backward(S, P.tex, G, L, u, v, gTex, tex, dU, dV);
     if (P.useColor) dcol = make_float4(gf.x * tex.x, gf.y * tex.y, gf.z * tex.z, gf.w * tex.w);
   } else {
     dcol = gf;
@@ -461,26 +553,28 @@ __device__ void fragment_backward(GradLds &S, const DrawParams &P, const GradPar
     if (val[k] != 0.0f) atomicAdd(slot + k, val[k]);
 }
 
-// Per-wave queue of fragments: one entry per (pixel, distinct owning
-// primitive), at most MS per pixel.
-constexpr int kFragQueue = 64 * 4;
+// Per-wave ring of fragments: one entry per (pixel, distinct owning
+// primitive), at most MS per pixel.  A block adds at most 64 * MS entries to
+// fewer than 64 left over from the blocks before.
+constexpr uint32_t kRing = 512;
 
 struct FragQueue {
-  uint32_t xy[kWaves][kFragQueue];    // x | y << 16
-  uint32_t tri[kWaves][kFragQueue];   // primitive index
-  uint32_t mask[kWaves][kFragQueue];  // samples the fragment owns
+  uint32_t xy[kWaves][kRing];    // x | y << 16
+  uint32_t tri[kWaves][kRing];   // primitive index
+  uint32_t mask[kWaves][kRing];  // samples the fragment owns
 };
 
-// Gradients of the pixels (px, py) of one wave of draw P (rec holds the
-// draw's records: slot t = primitive t).  The wave first queues its pixels'
-// fragments -- the samples of a pixel owned by one primitive share one
-// fragment, shaded once at the pixel centre, whose upstream gradient is the
-// sum of theirs -- then runs them 64 at a time, so edge pixels with several
-// primitives cost extra lanes, not extra passes of the whole wave.
+// Wave-uniform position of a wave's ring.
+struct RingPos {
+  uint32_t head, count;
+};
+
+// Appends the fragments of the wave's pixels (px, py) of draw P: the
+// samples of a pixel owned by one primitive share one fragment, shaded once
+// at the pixel centre, whose upstream gradient is the sum of theirs.
 template <int MS>
-__device__ __forceinline__ void backward_pixels(GradLds &S, FragQueue &Q, const DrawParams &P, const GradParams &G,
-                                                const TriRecord *__restrict__ rec, int px, int py) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+__device__ __forceinline__ void enqueue_fragments(FragQueue &Q, RingPos &rp, const DrawParams &P, int px, int py) {
+  const int wave = threadIdx.x >> 6;
   uint32_t id[MS];
   const bool inside = px < P.width && py < P.height;
 #pragma unroll
@@ -488,7 +582,6 @@ __device__ __forceinline__ void backward_pixels(GradLds &S, FragQueue &Q, const 
     id[q] = inside ? *reinterpret_cast<const uint32_t *>(P.primId + q * P.primIdSlice + (long long)py * P.primIdPitch +
                                                          4ll * px)
                    : 0u;
-  uint32_t count = 0;
 #pragma unroll
   for (int q = 0; q < MS; q++) {
     const uint32_t t1 = id[q];
@@ -500,42 +593,58 @@ __device__ __forceinline__ void backward_pixels(GradLds &S, FragQueue &Q, const 
     for (int q2 = q; q2 < MS; q2++) mask |= id[q2] == t1 ? 1u << q2 : 0u;
     const uint64_t bal = __ballot(first);
     if (first) {
-      const uint32_t pos = count + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
-                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0));
+      const uint32_t pos = (rp.head + rp.count + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0))) &
+                           (kRing - 1);
       Q.xy[wave][pos] = (uint32_t)px | ((uint32_t)py << 16);
       Q.tri[wave][pos] = t1 - 1u;
       Q.mask[wave][pos] = mask;
     }
-    count += (uint32_t)__builtin_popcountll(bal);
+    rp.count += (uint32_t)__builtin_popcountll(bal);
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  for (uint32_t base = 0; base < count; base += 64) {
-    if (base + (uint32_t)lane >= count) continue;
-    const uint32_t xy = Q.xy[wave][base + lane], t = Q.tri[wave][base + lane], mask = Q.mask[wave][base + lane];
-    const int x = (int)(xy & 0xFFFFu), y = (int)(xy >> 16);
-    Acc4 g = {0.0, 0.0, 0.0, 0.0};
+}
+
+// Runs the wave's queued fragments of draw P (rec holds the draw's records:
+// slot t = primitive t) 64 at a time, and the rest as well when `all`, so
+// that edge pixels with several primitives cost extra lanes rather than
+// extra passes, and partial batches occur only at draw changes.
+template <int MS>
+__device__ __forceinline__ void drain_fragments(GradLds &S, FragQueue &Q, RingPos &rp, const DrawParams &P,
+                                                const GradParams &G, const TriRecord *__restrict__ rec, bool all) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  while (rp.count >= 64 || (all && rp.count > 0)) {
+    const uint32_t n = rp.count < 64 ? rp.count : 64;
+    const uint32_t e = (rp.head + (uint32_t)lane) & (kRing - 1);
+    if ((uint32_t)lane < n) {
+      const uint32_t xy = Q.xy[wave][e], t = Q.tri[wave][e], mask = Q.mask[wave][e];
+      const int x = (int)(xy & 0xFFFFu), y = (int)(xy >> 16);
+      Acc4 g = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int q = 0; q < MS; q++) {
-      if (!(mask >> q & 1u)) continue;
-      const float4 d = *reinterpret_cast<const float4 *>(G.dColor + q * G.dColorSlice + (long long)y * G.dColorPitch +
-                                                         16ll * x);
-      g.x += d.x;
-      g.y += d.y;
-      g.z += d.z;
-      g.w += d.w;
-    }
-    QuadLod L = {0.0f, 0.0f, 0.0f, 0.0f};
-    if (P.useUV) L = quad_state(P, rec[t], x, y);
-    if (G.state) {
+      for (int q = 0; q < MS; q++) {
+        if (!(mask >> q & 1u)) continue;
+        const float4 d = *reinterpret_cast<const float4 *>(G.dColor + q * G.dColorSlice +
+                                                           (long long)y * G.dColorPitch + 16ll * x);
+        g.x += d.x;
+        g.y += d.y;
+        g.z += d.z;
+        g.w += d.w;
+      }
+      QuadLod L = {0.0f, 0.0f, 0.0f, 0.0f};
+      if (P.useUV) L = quad_state(P, rec[t], x, y);
+      if (G.state) {
 #pragma unroll
-      for (int q = 0; q < MS; q++)
-        if (mask >> q & 1u)
-          *reinterpret_cast<float4 *>(G.state + q * G.stateSlice + (long long)y * G.statePitch + 16ll * x) =
-              make_float4(L.lod, L.aniso, L.uDelta, L.vDelta);
+        for (int q = 0; q < MS; q++)
+          if (mask >> q & 1u)
+            *reinterpret_cast<float4 *>(G.state + q * G.stateSlice + (long long)y * G.statePitch + 16ll * x) =
+                make_float4(L.lod, L.aniso, L.uDelta, L.vDelta);
+      }
+      fragment_backward(S, P, G, t, x, y, g, L);
     }
-    fragment_backward(S, P, G, t, x, y, g, L);
+    rp.head = (rp.head + n) & (kRing - 1);
+    rp.count -= n;
   }
 }
 
@@ -544,8 +653,10 @@ template <int MS>
 __global__ __launch_bounds__(256) void backward_kernel(DrawParams P, GradParams G, const TriRecord *__restrict__ rec) {
   __shared__ GradLds S;
   __shared__ FragQueue Q;
-  lds_init(S);
-  backward_pixels<MS>(S, Q, P, G, rec, blockIdx.x * 16 + (threadIdx.x & 15), blockIdx.y * 16 + (threadIdx.x >> 4));
+  lds_init(S, G);
+  RingPos rp = {0u, 0u};
+  enqueue_fragments<MS>(Q, rp, P, blockIdx.x * 16 + (threadIdx.x & 15), blockIdx.y * 16 + (threadIdx.x >> 4));
+  drain_fragments<MS>(S, Q, rp, P, G, rec, true);
   lds_flush(S, P, G);
 }
 
@@ -556,8 +667,10 @@ __global__ __launch_bounds__(256) void backward_kernel(DrawParams P, GradParams 
 // wave's texel table flushes itself when it fills and is flushed when the
 // next block writes other texture-gradient buffers; the primitive table is
 // flushed when it passes half full or the next block belongs to another
-// draw; both at the end.  Views that share a texture thus reach its gradient
-// once per table rather than once per block.
+// draw; both at the end.  Views that share a texture thus reach its
+// gradient once per table rather than once per block.  Each wave carries
+// the fragments that do not fill a batch of 64 over to its next block of
+// the same draw.
 template <int MS>
 __global__ __launch_bounds__(256) void backward_batch_kernel(const DrawParams *__restrict__ params,
                                                              const GradParams *__restrict__ grads,
@@ -568,8 +681,9 @@ __global__ __launch_bounds__(256) void backward_batch_kernel(const DrawParams *_
   __shared__ GradLds S;
   __shared__ FragQueue Q;
   __shared__ uint32_t sBlock;
-  lds_init(S);
+  lds_init(S, grads[drawLo]);
   int cur = -1;  // draw whose gradients the tables hold (wave-uniform)
+  RingPos rp = {0u, 0u};
   for (;;) {
     if (threadIdx.x == 0) sBlock = blockLo + atomicAdd(counter, 1u);
     __syncthreads();
@@ -586,11 +700,17 @@ __global__ __launch_bounds__(256) void backward_batch_kernel(const DrawParams *_
       }
       lo = __builtin_amdgcn_readfirstlane(l);
     }
+    if (cur >= 0 && lo != cur) {
+      // The rest of draw cur's fragments, before its tables are flushed (the
+      // primitive table is shared: every wave finishes adding first).
+      drain_fragments<MS>(S, Q, rp, params[cur], grads[cur], rec + params[cur].triBase, true);
+      __syncthreads();
+    }
     if (cur >= 0) {
       bool texSwitch = lo < 0;
       for (int k = 0; k < kMipLevels && !texSwitch; k++) texSwitch = grads[lo].dTex[k] != grads[cur].dTex[k];
       const bool fr = lo != cur || triCount > kTriSlots / 2;
-      if (texSwitch) wave_tex_flush(wave_table(S), grads[cur]);
+      if (texSwitch) wave_tex_flush(wave_table(S), S.dTex);
       if (fr) {
         lds_flush_tri(S, params[cur], grads[cur]);
         __syncthreads();
@@ -599,12 +719,17 @@ __global__ __launch_bounds__(256) void backward_batch_kernel(const DrawParams *_
       }
     }
     if (lo < 0) break;
+    if (lo != cur) {
+      if (threadIdx.x < kMipLevels) S.dTex[threadIdx.x] = grads[lo].dTex[threadIdx.x];
+      __syncthreads();
+    }
     cur = lo;
     const DrawParams &P = params[lo];
     const uint32_t local = b - blockBase[lo];
     const uint32_t bx = (uint32_t)(P.width + 15) >> 4;
     const int px = (int)(local % bx) * 16 + (threadIdx.x & 15), py = (int)(local / bx) * 16 + (threadIdx.x >> 4);
-    backward_pixels<MS>(S, Q, P, grads[lo], rec + P.triBase, px, py);
+    enqueue_fragments<MS>(Q, rp, P, px, py);
+    drain_fragments<MS>(S, Q, rp, P, grads[lo], rec + P.triBase, false);
     __syncthreads();
   }
 }
