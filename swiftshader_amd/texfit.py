@@ -30,6 +30,17 @@ from .desc import Buffers, build_desc
 from .scenes import DrawData
 
 
+def _flat(shapes, dtype, device):
+    """One zeroed flat tensor and contiguous views of it with the given shapes."""
+    sizes = [int(torch.Size(sh).numel()) for sh in shapes]
+    flat = torch.zeros(sum(sizes), dtype=dtype, device=device)
+    out, off = [], 0
+    for sh, n in zip(shapes, sizes):
+        out.append(flat[off:off + n].view(sh))
+        off += n
+    return flat, out
+
+
 @dataclass
 class _ViewBuffers:
     draw: DrawData
@@ -56,7 +67,9 @@ class MultiViewFit:
         self.lr = lr
         self.tex0 = texture0.detach().to(self.device, torch.float32).contiguous().clone()
         self.levels = [self.tex0] + diff.mip_levels_like(self.tex0)
-        self.d_levels = [torch.zeros_like(l) for l in self.levels]
+        # Every per-view and per-level buffer is a slice of one flat tensor per
+        # kind, so a clear or an elementwise update of all views is one launch.
+        self._d_levels_flat, self.d_levels = _flat([tuple(l.shape) for l in self.levels], torch.float32, self.device)
         self.m = torch.zeros_like(self.tex0)
         self.v = torch.zeros_like(self.tex0)
         self.t = torch.zeros((), dtype=torch.float32, device=self.device)
@@ -64,27 +77,38 @@ class MultiViewFit:
         self._b2 = torch.full((), 0.999, dtype=torch.float32, device=self.device)
         self.views: List[_ViewBuffers] = []
         self.n_values = 0
-        for i, d in enumerate(draws):
+        for d in draws:
             if d.color_format != abi.FORMAT_R32G32B32A32_SFLOAT or d.blend_enable or d.texture is None:
                 raise ValueError("views render a float texture into RGBA32F targets without blending")
-            S, H, W = d.samples, d.height, d.width
-            dev = self.device
+        dev = self.device
+        samples = [(d.samples, d.height, d.width) for d in draws]
+        self.single_sample = all(d.samples == 1 for d in draws)
+        self._color_flat, colors = _flat([(S, H, W, 4) for S, H, W in samples], torch.float32, dev)
+        self._depth_flat, depths = _flat(samples, torch.float32, dev)
+        self._ids_flat, ids = _flat(samples, torch.int32, dev)
+        self._dcolor_flat, dcolors = _flat([(S, H, W, 4) for S, H, W in samples], torch.float32, dev)
+        self._resolved_flat, resolved = _flat([(1, H, W, 4) for _, H, W in samples], torch.float32, dev)
+        self._target_flat, tgts = _flat([(1, H, W, 4) for _, H, W in samples], torch.float32, dev)
+        geo = [None] * len(draws), [None] * len(draws), [None] * len(draws)
+        self._geo_flat = []
+        if geometry_grads:
+            vcount = [len(d.positions) for d in draws]
+            fp, dp = _flat([(n, 4) for n in vcount], torch.float32, dev)
+            fc, dc = _flat([(n, 4) for n in vcount], torch.float32, dev)
+            fu, du = _flat([(n, 2) for n in vcount], torch.float32, dev)
+            self._geo_flat = [fp, fc, fu]
+            geo = dp, dc, du
+        self._depth_flat.fill_(1.0)
+        for i, d in enumerate(draws):
+            H, W = d.height, d.width
+            if targets is not None:
+                tgts[i].copy_(targets[i].to(dev).reshape(1, H, W, 4))
             vb = _ViewBuffers(
                 draw=d,
                 pos=torch.from_numpy(d.positions).to(dev), col=torch.from_numpy(d.colors).to(dev),
                 uv=torch.from_numpy(d.texcoords).to(dev),
-                color=torch.zeros((S, H, W, 4), dtype=torch.float32, device=dev),
-                depth=torch.ones((S, H, W), dtype=torch.float32, device=dev),
-                ids=torch.zeros((S, H, W), dtype=torch.int32, device=dev),
-                resolved=torch.zeros((1, H, W, 4), dtype=torch.float32, device=dev),
-                target=(targets[i].to(dev).reshape(1, H, W, 4).contiguous() if targets is not None
-                        else torch.zeros((1, H, W, 4), dtype=torch.float32, device=dev)),
-                d_color=torch.zeros((S, H, W, 4), dtype=torch.float32, device=dev),
-                d_pos=None, d_col=None, d_uv=None)
-            if geometry_grads:
-                vb.d_pos = torch.zeros_like(vb.pos)
-                vb.d_col = torch.zeros_like(vb.col)
-                vb.d_uv = torch.zeros_like(vb.uv)
+                color=colors[i], depth=depths[i], ids=ids[i], resolved=resolved[i], target=tgts[i],
+                d_color=dcolors[i], d_pos=geo[0][i], d_col=geo[1][i], d_uv=geo[2][i])
             vb.desc = self._desc(vb)
             self.views.append(vb)
             self.n_values += H * W * 4
@@ -114,10 +138,9 @@ class MultiViewFit:
 
     # ---- pieces of a step -----------------------------------------------------
     def _clear(self):
-        for vb in self.views:
-            vb.color.zero_()
-            vb.depth.fill_(1.0)
-            vb.ids.zero_()
+        self._color_flat.zero_()
+        self._depth_flat.fill_(1.0)
+        self._ids_flat.zero_()
 
     def _image(self, vb: _ViewBuffers) -> torch.Tensor:
         return vb.resolved if vb.draw.samples > 1 else vb.color
@@ -127,6 +150,11 @@ class MultiViewFit:
         of (image - target)^2 (elementwise ops only: no reductions, whose
         library kernels may record memsets into a captured graph)."""
         inv = 1.0 / self.n_values
+        if self.single_sample:
+            # Images are the colour buffers themselves, laid out like the targets.
+            torch.sub(self._color_flat, self._target_flat, out=self._dcolor_flat)
+            self._dcolor_flat.mul_(2.0 * inv)
+            return
         for vb in self.views:
             if vb.draw.samples > 1:
                 runtime.resolve(vb.color, vb.resolved)
@@ -145,13 +173,9 @@ class MultiViewFit:
         return total / self.n_values
 
     def _zero_grads(self):
-        for t in self.d_levels:
+        self._d_levels_flat.zero_()
+        for t in self._geo_flat:
             t.zero_()
-        if self.geometry_grads:
-            for vb in self.views:
-                vb.d_pos.zero_()
-                vb.d_col.zero_()
-                vb.d_uv.zero_()
 
     def _mip_adjoint(self):
         t = diff._texture_struct(self.d_levels, abi.FORMAT_R32G32B32A32_SFLOAT)
