@@ -84,11 +84,12 @@ struct Pipeline {
   Scratch params, aux;                      // DrawParams[n]; blockBase[n], tileBase[n]
   Scratch rec, poly, cnt, off, sums;        // per primitive slot
   Scratch keysA, keysB, valsA, valsB, hist; // per pair
+  Scratch first;                            // per block of emitted pairs
   Scratch tStart, tEnd;                     // per tile
   Scratch counters;                         // work counters (per variant), overflow flags
   void release() {
-    for (Scratch *s : {&params, &aux, &rec, &poly, &cnt, &off, &sums, &keysA, &keysB, &valsA, &valsB, &hist, &tStart,
-                       &tEnd, &counters})
+    for (Scratch *s : {&params, &aux, &rec, &poly, &cnt, &off, &sums, &keysA, &keysB, &valsA, &valsB, &hist, &first,
+                       &tStart, &tEnd, &counters})
       s->release();
   }
 };
@@ -558,6 +559,7 @@ int ensure_pipe(Pipeline &pipe, uint32_t T, uint32_t cap, uint32_t tiles) {
   SWR_HIP_CHECK(pipe.valsA.ensure(sizeof(uint32_t) * pc));
   SWR_HIP_CHECK(pipe.valsB.ensure(sizeof(uint32_t) * pc));
   SWR_HIP_CHECK(pipe.hist.ensure(sizeof(uint32_t) * 256 * std::max<size_t>(nb, 1)));
+  SWR_HIP_CHECK(pipe.first.ensure(sizeof(uint32_t) * ((size_t)cap / kEmitBlock + 2)));
   SWR_HIP_CHECK(pipe.tStart.ensure(sizeof(uint32_t) * std::max<uint32_t>(tiles, 1)));
   SWR_HIP_CHECK(pipe.tEnd.ensure(sizeof(uint32_t) * std::max<uint32_t>(tiles, 1)));
   SWR_HIP_CHECK(pipe.counters.ensure(sizeof(uint32_t) * 16));
@@ -597,9 +599,13 @@ int enqueue_back(DeviceCtx *c, Pipeline &pipe, const BatchLayout &L, const Batch
   const DrawParams *params = pipe.params.as<DrawParams>() + sg.first;
   const uint32_t *tileBase = pipe.aux.as<uint32_t>() + L.P.size() + sg.first;
   if (cap > 0) {
-    hipLaunchKernelGGL(bin_emit_batch_kernel, dim3((cap + 255) / 256), dim3(256), 0, s, params, tileBase,
-                       (const TriRecord *)pipe.rec.as<TriRecord>(), (const uint32_t *)pipe.off.as<uint32_t>(), sg.T,
-                       cap, pipe.keysA.as<uint32_t>(), pipe.valsA.as<uint32_t>(), overflowFlag);
+    if (sg.T > 0)
+      hipLaunchKernelGGL(emit_index_kernel, dim3((sg.T + 255) / 256), dim3(256), 0, s,
+                         (const uint32_t *)pipe.off.as<uint32_t>(), sg.T, cap, pipe.first.as<uint32_t>());
+    hipLaunchKernelGGL(bin_emit_batch_kernel, dim3((cap + kEmitBlock - 1) / kEmitBlock), dim3(kEmitBlock), 0, s, params,
+                       tileBase, (const TriRecord *)pipe.rec.as<TriRecord>(), (const uint32_t *)pipe.off.as<uint32_t>(),
+                       sg.T, (const uint32_t *)pipe.first.as<uint32_t>(), cap, pipe.keysA.as<uint32_t>(),
+                       pipe.valsA.as<uint32_t>(), overflowFlag);
     const uint32_t nb = (cap + kSortItems - 1) / kSortItems;
     const uint32_t passes = sort_passes(sg.tiles);
     uint32_t *kIn = pipe.keysA.as<uint32_t>(), *vIn = pipe.valsA.as<uint32_t>();
@@ -608,7 +614,7 @@ int enqueue_back(DeviceCtx *c, Pipeline &pipe, const BatchLayout &L, const Batch
       hipLaunchKernelGGL(radix_hist_kernel, dim3(nb), dim3(256), 0, s, (const uint32_t *)kIn, cap, (int)(8 * p),
                          pipe.hist.as<uint32_t>(), nb);
       enqueue_scan(pipe.hist.as<uint32_t>(), pipe.hist.as<uint32_t>(), (size_t)256 * nb, pipe.sums.as<uint32_t>(), s);
-      hipLaunchKernelGGL(radix_scatter_kernel, dim3(nb), dim3(64), 0, s, (const uint32_t *)kIn, (const uint32_t *)vIn,
+      hipLaunchKernelGGL(radix_scatter_kernel, dim3(nb), dim3(256), 0, s, (const uint32_t *)kIn, (const uint32_t *)vIn,
                          cap, (int)(8 * p), (const uint32_t *)pipe.hist.as<uint32_t>(), nb, kOut, vOut);
       std::swap(kIn, kOut);
       std::swap(vIn, vOut);

@@ -8,14 +8,17 @@
 //   setup_batch_kernel    setup of every primitive of every draw
 //   scan_*                exclusive prefix sum of the per-primitive tile
 //                         counts (three passes, no memset, fixed sizes)
+//   emit_index_kernel     the first primitive slot of every 256-pair block
 //   bin_emit_batch_kernel one thread per (tile, primitive) pair up to a
 //                         fixed capacity; entries past the real count get
 //                         the key 0xFFFFFFFF so that they sort last
 //   radix_*               stable LSD radix sort of the pairs by tile, 8 bits
 //                         per pass: per-block digit histograms, a scan, and
-//                         an in-order scatter (one wave per block ranks its
-//                         keys with ballots, so equal keys keep list order,
-//                         which keeps every tile's primitives in API order)
+//                         an in-order scatter (waves rank their keys with
+//                         ballots, so equal keys keep list order, which
+//                         keeps every tile's primitives in API order; the
+//                         block is staged in LDS sorted, then written in
+//                         runs)
 //   tile_range_kernel     start / end of each tile's run of pairs
 //
 // Every size is fixed when the work is enqueued and nothing is cleared with
@@ -29,7 +32,8 @@
 namespace swr {
 
 constexpr int kScanItems = 1024;   // items per scan block (256 threads x 4)
-constexpr int kSortItems = 1024;   // items per radix-sort block (64 lanes x 16)
+constexpr int kSortItems = 4096;   // items per radix-sort block (4 waves x 16 steps x 64 lanes)
+constexpr int kEmitBlock = 256;    // pairs per emission workgroup
 
 // Largest d in [0, n) with base[d] <= v (base non-decreasing, base[0] <= v).
 __device__ __forceinline__ int find_segment(const uint32_t *__restrict__ base, int n, uint32_t v) {
@@ -122,27 +126,72 @@ __global__ __launch_bounds__(256) void scan_add_kernel(uint32_t *__restrict__ ou
 
 // ---- pair emission ---------------------------------------------------------
 // off: exclusive prefix of the tile counts over the T record slots (off[T]
-// = total pairs).  Pairs beyond `capacity` are dropped and the needed count
-// is reported in *overflow (left untouched when everything fits).
-__global__ __launch_bounds__(256) void bin_emit_batch_kernel(const DrawParams *__restrict__ params,
-                                                             const uint32_t *__restrict__ tileBase,
-                                                             const TriRecord *__restrict__ rec,
-                                                             const uint32_t *__restrict__ off, uint32_t T,
-                                                             uint32_t capacity, uint32_t *__restrict__ keys,
-                                                             uint32_t *__restrict__ vals,
-                                                             uint32_t *__restrict__ overflow) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+// = total pairs).  Pair k belongs to the slot g with off[g] <= k <
+// off[g + 1].  first[b] is the slot of pair b * kEmitBlock (written for the
+// blocks that start below the total): slot g writes it for the blocks whose
+// first pair falls in its range.
+__global__ __launch_bounds__(256) void emit_index_kernel(const uint32_t *__restrict__ off, uint32_t T,
+                                                         uint32_t capacity, uint32_t *__restrict__ first) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= T) return;
+  const uint32_t lo = off[g], hi = off[g + 1] < capacity ? off[g + 1] : capacity;
+  for (uint32_t b = (lo + kEmitBlock - 1) / kEmitBlock; b * kEmitBlock < hi; b++) first[b] = g;
+}
+
+// Pairs beyond `capacity` are dropped and the needed count is reported in
+// *overflow (left untouched when everything fits).  A workgroup emits
+// kEmitBlock consecutive pairs; their slots lie in [first[b], first[b + 1]]
+// (the last block searches up to T - 1), a range each thread searches,
+// staged in LDS when it fits.
+constexpr int kEmitSpan = 1024;
+
+__global__ __launch_bounds__(kEmitBlock) void bin_emit_batch_kernel(const DrawParams *__restrict__ params,
+                                                                    const uint32_t *__restrict__ tileBase,
+                                                                    const TriRecord *__restrict__ rec,
+                                                                    const uint32_t *__restrict__ off, uint32_t T,
+                                                                    const uint32_t *__restrict__ first,
+                                                                    uint32_t capacity, uint32_t *__restrict__ keys,
+                                                                    uint32_t *__restrict__ vals,
+                                                                    uint32_t *__restrict__ overflow) {
+  __shared__ uint32_t sOff[kEmitSpan + 1];
+  const uint32_t k = blockIdx.x * kEmitBlock + threadIdx.x;
   const uint32_t total = off[T];
   if (k == 0 && total > capacity) *overflow = total;
+  const uint32_t k0 = blockIdx.x * kEmitBlock;
+  const bool any = k0 < total && k0 < capacity;  // workgroup-uniform
+  int t0 = 0, span = 0;
+  if (any) {
+    t0 = (int)first[blockIdx.x];
+    const uint32_t next = k0 + kEmitBlock;
+    const int t1 = (next < total && next < capacity) ? (int)first[blockIdx.x + 1] : (int)T - 1;
+    span = t1 - t0 + 1;
+    if (span <= kEmitSpan)
+      for (int i = threadIdx.x; i <= span; i += kEmitBlock) sOff[i] = off[t0 + i];
+  }
+  __syncthreads();
   if (k >= capacity) return;
   if (k >= total) {
     keys[k] = 0xFFFFFFFFu;
     vals[k] = 0u;
     return;
   }
-  const int g = find_segment(off, (int)T, k);
-  // Skip empty slots that share the prefix value (find the last slot <= k
-  // whose range is non-empty: off[g] <= k < off[g + 1] holds for it).
+  // Largest slot g in [t0, t0 + span) with off[g] <= k: the slot whose
+  // range holds k (empty slots share their prefix value with the next).
+  int lo = 0, hi = span;
+  if (span <= kEmitSpan) {
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (sOff[mid] <= k) lo = mid;
+      else hi = mid;
+    }
+  } else {
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (off[t0 + mid] <= k) lo = mid;
+      else hi = mid;
+    }
+  }
+  const int g = t0 + lo;
   const TriRecord &R = rec[g];
   const DrawParams &P = params[R.draw];
   const uint32_t i = k - off[g];
@@ -170,28 +219,42 @@ __global__ __launch_bounds__(256) void radix_hist_kernel(const uint32_t *__restr
   hist[threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
 }
 
-// One wave per block of kSortItems keys, taken 64 at a time in order: each
-// lane's rank among the lanes with its digit comes from eight ballots, so
-// keys with equal digits keep their input order.
-__global__ __launch_bounds__(64) void radix_scatter_kernel(const uint32_t *__restrict__ keysIn,
-                                                           const uint32_t *__restrict__ valsIn, uint32_t n, int shift,
-                                                           const uint32_t *__restrict__ histScan, uint32_t nb,
-                                                           uint32_t *__restrict__ keysOut,
-                                                           uint32_t *__restrict__ valsOut) {
-  __shared__ uint32_t next[256];
-  const int lane = threadIdx.x;
-#pragma unroll
-  for (int k = 0; k < 4; k++) next[lane + 64 * k] = histScan[(lane + 64 * k) * nb + blockIdx.x];
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+// One workgroup of 4 waves per block of kSortItems keys.  Each wave ranks
+// its 1024 consecutive keys 64 at a time in order (a lane's rank among the
+// lanes with its digit comes from eight ballots, so equal digits keep their
+// input order); the waves' counts then give every key its place in the
+// block sorted by digit, the block is staged in LDS in that order, and
+// written out with consecutive threads on consecutive positions of each
+// digit's run.
+__global__ __launch_bounds__(256) void radix_scatter_kernel(const uint32_t *__restrict__ keysIn,
+                                                            const uint32_t *__restrict__ valsIn, uint32_t n,
+                                                            int shift, const uint32_t *__restrict__ histScan,
+                                                            uint32_t nb, uint32_t *__restrict__ keysOut,
+                                                            uint32_t *__restrict__ valsOut) {
+  constexpr int kSteps = kSortItems / 256;
+  __shared__ uint32_t cnt[4][256];  // per wave and digit: running count, then the wave's offset
+  __shared__ uint32_t lstart[256];  // block-local start of each digit
+  __shared__ uint32_t gstart[256];  // output start of each digit for this block
+  __shared__ uint32_t wsum[4];
+  __shared__ uint32_t skey[kSortItems], sval[kSortItems];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < 4 * 256; i += 256) (&cnt[0][0])[i] = 0u;
+  gstart[threadIdx.x] = histScan[threadIdx.x * nb + blockIdx.x];
+  __syncthreads();
+
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const uint32_t base = blockIdx.x * kSortItems;
-  for (int step = 0; step < kSortItems / 64; step++) {
+  const uint32_t base = blockIdx.x * kSortItems + wave * (kSortItems / 4);
+  uint32_t keyv[kSteps], valv[kSteps], rankv[kSteps];
+#pragma unroll
+  for (int step = 0; step < kSteps; step++) {
     const uint32_t i = base + step * 64 + lane;
-    const bool valid = i < n;
-    const uint32_t key = valid ? keysIn[i] : 0u;
-    const uint32_t val = valid ? valsIn[i] : 0u;
-    const uint32_t digit = (key >> shift) & 0xFFu;
+    keyv[step] = i < n ? keysIn[i] : 0u;
+    valv[step] = i < n ? valsIn[i] : 0u;
+  }
+#pragma unroll
+  for (int step = 0; step < kSteps; step++) {
+    const bool valid = base + step * 64 + lane < n;
+    const uint32_t digit = (keyv[step] >> shift) & 0xFFu;
     uint64_t peers = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; b++) {
@@ -199,18 +262,57 @@ __global__ __launch_bounds__(64) void radix_scatter_kernel(const uint32_t *__res
       peers &= ((digit >> b) & 1u) ? m : ~m;
     }
     const uint32_t rank = (uint32_t)__builtin_popcountll(peers & lt);
-    const uint32_t cnt = (uint32_t)__builtin_popcountll(peers);
-    uint32_t pos = 0;
-    if (valid) pos = next[digit] + rank;
+    const uint32_t c = (uint32_t)__builtin_popcountll(peers);
+    rankv[step] = valid ? cnt[wave][digit] + rank : 0u;
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    if (valid && rank + 1 == cnt) next[digit] += cnt;
+    if (valid && rank + 1 == c) cnt[wave][digit] += c;
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    if (valid) {
-      keysOut[pos] = key;
-      valsOut[pos] = val;
+  }
+  __syncthreads();
+
+  // Thread d: the waves' offsets within digit d, and the block-local start
+  // of digit d (exclusive scan of the digit totals).
+  {
+    const int d = threadIdx.x;
+    const uint32_t c0 = cnt[0][d], c1 = cnt[1][d], c2 = cnt[2][d], c3 = cnt[3][d];
+    const uint32_t tot = c0 + c1 + c2 + c3;
+    uint32_t incl = tot;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t v = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += v;
     }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    uint32_t pre = 0;
+    for (int w = 0; w < wave; w++) pre += wsum[w];
+    lstart[d] = pre + incl - tot;
+    cnt[0][d] = 0u;
+    cnt[1][d] = c0;
+    cnt[2][d] = c0 + c1;
+    cnt[3][d] = c0 + c1 + c2;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int step = 0; step < kSteps; step++) {
+    if (base + step * 64 + lane < n) {
+      const uint32_t digit = (keyv[step] >> shift) & 0xFFu;
+      const uint32_t pos = lstart[digit] + cnt[wave][digit] + rankv[step];
+      skey[pos] = keyv[step];
+      sval[pos] = valv[step];
+    }
+  }
+  __syncthreads();
+  const uint32_t b0 = blockIdx.x * kSortItems;
+  const uint32_t valid = n - b0 < (uint32_t)kSortItems ? n - b0 : (uint32_t)kSortItems;
+  for (uint32_t e = threadIdx.x; e < valid; e += 256) {
+    const uint32_t key = skey[e];
+    const uint32_t digit = (key >> shift) & 0xFFu;
+    const uint32_t o = gstart[digit] + (e - lstart[digit]);
+    keysOut[o] = key;
+    valsOut[o] = sval[e];
   }
 }
 
