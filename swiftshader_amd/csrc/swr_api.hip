@@ -135,7 +135,7 @@ int current_ctx(DeviceCtx **out) {
     c->device = dev;
     c->tuning.tile_width = 16;
     c->tuning.tile_height = 16;
-    c->tuning.raster_workgroups_per_cu = 4;
+    c->tuning.raster_workgroups_per_cu = ~0u;  // one workgroup per tile: fastest for a single large draw
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
       c->cus = prop.multiProcessorCount;

@@ -703,10 +703,32 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
     qCount = 0;
   };
 
+  // Largest stored depth of the block (+inf when any is NaN): a primitive
+  // whose depth is bounded below by zl cannot pass where zl >= it (LESS) or
+  // zl > it (LESS_OR_EQUAL).
+  auto block_zmax = [&]() __attribute__((always_inline)) {
+    float v = -__builtin_inff();
+    bool nan = false;
+    if (inside) {
+#pragma unroll
+      for (int q = 0; q < MS; q++) {
+        v = fmaxf(v, S.zb[q]);
+        nan |= S.zb[q] != S.zb[q];
+      }
+    }
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+    return __ballot(nan) ? __builtin_inff() : v;
+  };
+
   // One batch of up to 64 candidates in list order: each lane computes the
   // exact coverage of its primitive; then the covering primitives are
-  // applied in list order.
-  auto process_chunk = [&](bool act, uint32_t tid) __attribute__((always_inline)) {
+  // applied in list order.  A candidate waited in the ring while earlier
+  // ones wrote depth, so its depth bound zl is tested again first.
+  auto process_chunk = [&](bool act, uint32_t tid, float zl) __attribute__((always_inline)) {
+    if (hiz) {
+      const float zmax = block_zmax();
+      act = act && !(P.depthCompare == 1 ? zl >= zmax : zl > zmax);
+    }
     // ---- triangle-parallel coverage of this block ----
     uint32_t mlo[MS], mhi[MS];
     int flags = 0;
@@ -845,28 +867,17 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
   // runs on full batches of 64: a 64-entry chunk of the list yields about a
   // third of a wave of candidates.
   __shared__ uint32_t ring[4][128];
+  __shared__ float ringZ[4][128];
   uint32_t ringHead = 0, ringCount = 0;  // wave-uniform
 
   for (uint32_t base = start; base < end; base += 64) {
     // ---- block depth bound for the hierarchical test ----
-    float zmax = __builtin_inff();
-    if (hiz) {
-      float v = -__builtin_inff();
-      bool nan = false;
-      if (inside) {
-#pragma unroll
-        for (int q = 0; q < MS; q++) {
-          v = fmaxf(v, S.zb[q]);
-          nan |= S.zb[q] != S.zb[q];
-        }
-      }
-      for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
-      zmax = __ballot(nan) ? __builtin_inff() : v;
-    }
+    const float zmax = hiz ? block_zmax() : __builtin_inff();
     // ---- candidate test: box overlap, then the hierarchical depth bound ----
     const uint32_t e = base + lane;
     bool test = false;
     uint32_t tid = 0;
+    float zlo = -__builtin_inff();
     if (e < end) {
       tid = binTri[e];
       const int4 *rp = reinterpret_cast<const int4 *>(rec + tid);
@@ -887,6 +898,7 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
         if (!(zl == zl)) zl = -__builtin_inff();
         zl = fminf(zl, 1.0f);
         test = !(P.depthCompare == 1 ? zl >= zmax : zl > zmax);
+        zlo = zl;
       }
     }
     nTests += test ? 1u : 0u;
@@ -894,24 +906,27 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
     if (test) {
       const uint32_t pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(tb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)tb, 0));
       ring[wave][(ringHead + ringCount + pos) & 127] = tid;
+      ringZ[wave][(ringHead + ringCount + pos) & 127] = zlo;
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     ringCount += (uint32_t)__builtin_popcountll(tb);
     if (ringCount >= 64) {
       const uint32_t t2 = ring[wave][(ringHead + lane) & 127];
+      const float z2 = ringZ[wave][(ringHead + lane) & 127];
       ringHead = (ringHead + 64) & 127;
       ringCount -= 64;
-      process_chunk(true, t2);
+      process_chunk(true, t2, z2);
     }
   }
   while (ringCount > 0) {
     const uint32_t nb = ringCount < 64 ? ringCount : 64;
     const bool act = (uint32_t)lane < nb;
     const uint32_t t2 = act ? ring[wave][(ringHead + lane) & 127] : 0u;
+    const float z2 = act ? ringZ[wave][(ringHead + lane) & 127] : -__builtin_inff();
     ringHead = (ringHead + nb) & 127;
     ringCount -= nb;
-    process_chunk(act, t2);
+    process_chunk(act, t2, z2);
   }
   if (qCount) flush();
 
@@ -968,7 +983,7 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
 // tileLo + b once); tile t belongs to the draw d in [drawLo, drawHi) with
 // tileBase[d] <= t < tileBase[d + 1].
 template <int MS, bool FCOL, bool FTEX>
-__global__ __launch_bounds__(256) void raster_batch_kernel(const DrawParams *__restrict__ params,
+__global__ __launch_bounds__(256, MS == 1 ? 4 : 1) void raster_batch_kernel(const DrawParams *__restrict__ params,
                                                            const uint32_t *__restrict__ tileBase, int drawLo,
                                                            int drawHi, uint32_t tileLo, uint32_t tileHi,
                                                            uint32_t *__restrict__ counter,
