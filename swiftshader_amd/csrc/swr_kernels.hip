@@ -982,8 +982,12 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
 // from an atomic counter (or, with counter == NULL, workgroup b renders tile
 // tileLo + b once); tile t belongs to the draw d in [drawLo, drawHi) with
 // tileBase[d] <= t < tileBase[d + 1].
+// Occupancy: 4 waves per SIMD (128 VGPRs) for the single-sample kernels
+// (the float-colour ones spill 12 bytes there and still run faster than at
+// 3 waves), 3 for the multisample ones.
 template <int MS, bool FCOL, bool FTEX>
-__global__ __launch_bounds__(256, MS == 1 ? 4 : 1) void raster_batch_kernel(const DrawParams *__restrict__ params,
+__global__ __launch_bounds__(256, MS == 1 ? 4 : 3) void raster_batch_kernel(
+                                                           const DrawParams *__restrict__ params,
                                                            const uint32_t *__restrict__ tileBase, int drawLo,
                                                            int drawHi, uint32_t tileLo, uint32_t tileHi,
                                                            uint32_t *__restrict__ counter,
