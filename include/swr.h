@@ -399,7 +399,8 @@ int swr_profile_enable(int enable);
  * and resets them. */
 int swr_profile_read(swr_profile *out);
 
-/* Sets / reads the rasteriser tuning used by later draws on this device. */
+/* Sets / reads the rasteriser tuning used by later draws and new batch plans
+ * on this device.  The default is 16x16 tiles with one workgroup per tile. */
 int swr_set_tuning(const swr_tuning *tuning);
 int swr_get_tuning(swr_tuning *tuning);
 
