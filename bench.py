@@ -137,13 +137,11 @@ def leg_batched(dev, reps):
 
     def clear_all():
         for dd in dds:
-            dd.targets.color.zero_()
-            dd.targets.depth.fill_(1.0)
+            runtime.clear_targets(dd.draw, dd.targets, reset_occlusion=False)
 
     def plain():
         for dd in dds:
-            dd.targets.color.zero_()
-            dd.targets.depth.fill_(1.0)
+            runtime.clear_targets(dd.draw, dd.targets, reset_occlusion=False)
             runtime.draw(dd)
 
     plain()
@@ -271,9 +269,7 @@ def main():
     color, depth, occ = dd.targets.color, dd.targets.depth, dd.targets.occlusion
 
     def step():
-        color.zero_()
-        depth.fill_(1.0)
-        occ.zero_()
+        runtime.clear_targets(draw, dd.targets)  # render-pass clear: swr_clear of colour and depth
         runtime.draw(dd)
 
     for _ in range(args.warmup):

@@ -46,7 +46,7 @@
 extern "C" {
 #endif
 
-#define SWR_ABI_VERSION 2
+#define SWR_ABI_VERSION 3
 
 /* ---- status codes -------------------------------------------------------- */
 #define SWR_OK 0
@@ -253,6 +253,23 @@ typedef struct swr_resolve_desc { /* 4x -> 1x colour resolve */
   uint32_t dst_row_pitch_bytes;
 } swr_resolve_desc;
 
+typedef union swr_clear_value { /* VkClearValue */
+  float color[4];                 /* RGBA8 / RGBA32F targets */
+  struct {
+    float depth;                  /* D32F targets */
+    uint32_t stencil;             /* S8 targets */
+  } depth_stencil;
+} swr_clear_value;
+
+typedef struct swr_clear_desc {
+  swr_attachment target;          /* format: RGBA8, RGBA32F, D32F or S8 */
+  uint32_t width, height;         /* image extent in pixels */
+  uint32_t samples;               /* sample planes: 1 or 4 */
+  uint32_t use_area;              /* 0: the whole image; 1: `area` (clipped to the image) */
+  swr_rect area;
+  swr_clear_value value;
+} swr_clear_desc;
+
 /* Gradients of a draw (differentiable rendering; no counterpart in the
  * reference).  The draw is the one a swr_draw_desc describes, rendered with
  * its primitive_id attachment; blending must be off, the colour attachment
@@ -327,6 +344,14 @@ int swr_draw_batch(const swr_draw_desc *descs, uint32_t count, void *stream);
 
 /* MSAA colour resolve (sw::Blitter::resolve, src/Device/Blitter.cpp:2035). */
 int swr_resolve(const swr_resolve_desc *desc, void *stream);
+
+/* Attachment clear (vk::Image::clear, src/Vulkan/VkImage.cpp:1105-1124, ->
+ * sw::Blitter::clear / fastClear, src/Device/Blitter.cpp:55-326): every
+ * sample of the area.  RGBA8 channels are clamped to [0, 1] (a NaN passes
+ * the clamp and packs as 0) and packed as (uint32_t)(255 * c + 0.5f);
+ * RGBA32F is written as given; D32F gets the depth's bits unclamped (the
+ * format is not normalised); S8 the stencil's low byte. */
+int swr_clear(const swr_clear_desc *desc, void *stream);
 
 /* Prepares `count` draws (as swr_draw_batch would render them) for
  * repeated runs; pair_headroom_percent (e.g. 10) over-provisions the pair

@@ -25,6 +25,7 @@
  *                    src/Vulkan/VkDescriptorSetLayout.cpp:298-334,468-503
  *   blend state      src/Device/Context.cpp:1090-1311
  *   resolve          src/Device/Blitter.cpp:1524-1545, 2076-2200
+ *   clear            src/Device/Blitter.cpp:55-326, src/Vulkan/VkImage.cpp:1105-1124
  *   numerics         src/Reactor/SubzeroReactor.cpp (RoundInt, Int(Float),
  *                    Min/Max, Frac, Floor, MulHigh, Short4(Int4)),
  *                    src/Pipeline/ShaderCore.cpp:691-722
@@ -1327,6 +1328,64 @@ EXPORT int swo_draw(const swr_draw_desc *d) {
 /* ===================================================================== */
 /* Resolve (Blitter.cpp:1524-1545 float; :2076-2200 RGBA8 4x fast path)  */
 /* ===================================================================== */
+
+/* Attachment clear: sw::Blitter::clear's clamp of a normalised format
+ * (src/Device/Blitter.cpp:64-86; sw::clamp, src/System/Math.hpp:153-160, is
+ * two compares, so a NaN passes), then fastClear's packing (:170-230):
+ * (uint32_t)(255 * c + 0.5f) per RGBA8 channel (x86-64 converts a NaN to
+ * 0), D32F as the float's bits, S8 as the low byte, written to every sample
+ * of the area (:276-318).  RGBA32F, which fastClear does not take, goes
+ * through the blit routine as a float-to-float copy. */
+static uint32_t clear_unorm8(float c) {
+  if (c < 0.0f) c = 0.0f;
+  if (c > 1.0f) c = 1.0f;
+  float t = 255.0f * c;
+  t = t + 0.5f;
+  return t == t ? (uint32_t)t : 0u;
+}
+
+EXPORT int swo_clear(const swr_clear_desc *d) {
+  if (!d || !d->target.data || (d->samples != 1 && d->samples != 4)) return SWR_ERROR_INVALID_ARGUMENT;
+  uint32_t bpp;
+  uint8_t bytes[16];
+  switch (d->target.format) {
+    case SWR_FORMAT_R8G8B8A8_UNORM: {
+      uint32_t p = clear_unorm8(d->value.color[0]) | (clear_unorm8(d->value.color[1]) << 8) |
+                   (clear_unorm8(d->value.color[2]) << 16) | (clear_unorm8(d->value.color[3]) << 24);
+      memcpy(bytes, &p, 4);
+      bpp = 4;
+      break;
+    }
+    case SWR_FORMAT_R32G32B32A32_SFLOAT:
+      memcpy(bytes, d->value.color, 16);
+      bpp = 16;
+      break;
+    case SWR_FORMAT_D32_SFLOAT:
+      memcpy(bytes, &d->value.depth_stencil.depth, 4);
+      bpp = 4;
+      break;
+    case SWR_FORMAT_S8_UINT:
+      bytes[0] = (uint8_t)d->value.depth_stencil.stencil;
+      bpp = 1;
+      break;
+    default:
+      return SWR_ERROR_UNSUPPORTED;
+  }
+  int64_t x0 = 0, y0 = 0, x1 = d->width, y1 = d->height;
+  if (d->use_area) {
+    if (d->area.x > x0) x0 = d->area.x;
+    if (d->area.y > y0) y0 = d->area.y;
+    if ((int64_t)d->area.x + d->area.width < x1) x1 = (int64_t)d->area.x + d->area.width;
+    if ((int64_t)d->area.y + d->area.height < y1) y1 = (int64_t)d->area.y + d->area.height;
+  }
+  for (uint32_t q = 0; q < d->samples; q++)
+    for (int64_t y = y0; y < y1; y++)
+      for (int64_t x = x0; x < x1; x++)
+        memcpy((uint8_t *)d->target.data + q * d->target.sample_pitch_bytes + (size_t)y * d->target.row_pitch_bytes +
+                   (size_t)x * bpp,
+               bytes, bpp);
+  return SWR_OK;
+}
 
 EXPORT int swo_resolve(const swr_resolve_desc *r) {
   if (!r || r->src_samples != 4) return SWR_ERROR_UNSUPPORTED;

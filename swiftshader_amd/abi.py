@@ -7,7 +7,7 @@ DrawData in the reference (src/Device/Renderer.hpp:58-113, 211-213).
 """
 import ctypes as C
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 OK = 0
 ERROR_INVALID_ARGUMENT = -1
@@ -120,6 +120,19 @@ class ResolveDesc(C.Structure):
                 ("src_samples", C.c_uint32), ("dst", C.c_void_p), ("dst_row_pitch_bytes", C.c_uint32)]
 
 
+class _DepthStencilValue(C.Structure):
+    _fields_ = [("depth", C.c_float), ("stencil", C.c_uint32)]
+
+
+class ClearValue(C.Union):
+    _fields_ = [("color", C.c_float * 4), ("depth_stencil", _DepthStencilValue)]
+
+
+class ClearDesc(C.Structure):
+    _fields_ = [("target", Attachment), ("width", C.c_uint32), ("height", C.c_uint32), ("samples", C.c_uint32),
+                ("use_area", C.c_uint32), ("area", Rect), ("value", ClearValue)]
+
+
 class GradDesc(C.Structure):
     _fields_ = [("d_color", C.c_void_p), ("d_color_row_pitch_bytes", C.c_uint32),
                 ("d_color_sample_pitch_bytes", C.c_uint64),
@@ -143,7 +156,7 @@ class Profile(C.Structure):
 # Symbols declared by include/swr.h, in declaration order.
 EXPORTED_SYMBOLS = (
     "swr_abi_version", "swr_last_error", "swr_init", "swr_release", "swr_draw",
-    "swr_draw_batch", "swr_resolve", "swr_batch_plan_create", "swr_batch_plan_run", "swr_batch_plan_check",
+    "swr_draw_batch", "swr_resolve", "swr_clear", "swr_batch_plan_create", "swr_batch_plan_run", "swr_batch_plan_check",
     "swr_batch_plan_autotune", "swr_batch_plan_set_grads", "swr_batch_plan_backward", "swr_batch_plan_destroy", "swr_draw_backward", "swr_generate_mips", "swr_mip_backward",
     "swr_profile_enable", "swr_profile_read", "swr_set_tuning", "swr_get_tuning",
 )
@@ -162,6 +175,8 @@ def bind(lib):
     lib.swr_draw_batch.restype = C.c_int
     lib.swr_resolve.argtypes = [C.POINTER(ResolveDesc), C.c_void_p]
     lib.swr_resolve.restype = C.c_int
+    lib.swr_clear.argtypes = [C.POINTER(ClearDesc), C.c_void_p]
+    lib.swr_clear.restype = C.c_int
     lib.swr_batch_plan_create.argtypes = [C.POINTER(DrawDesc), C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p),
                                           C.c_void_p]
     lib.swr_batch_plan_create.restype = C.c_int

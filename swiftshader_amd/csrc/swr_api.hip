@@ -1186,6 +1186,72 @@ int swr_resolve(const swr_resolve_desc *d, void *stream) {
   return SWR_OK;
 }
 
+namespace {
+// The clear value's packed form for `format` (fastClear, Blitter.cpp:
+// 170-230, after the clamp of Blitter::clear, :64-86; sw::clamp lets a NaN
+// through, and x86-64 converts a NaN to 0 here).
+uint32_t unorm8_clear(float c) {
+  if (c < 0.0f) c = 0.0f;
+  if (c > 1.0f) c = 1.0f;
+  volatile float t = 255.0f * c;  // two roundings, as the reference's unfused expression
+  const float u = t + 0.5f;
+  return u == u ? (uint32_t)u : 0u;
+}
+}  // namespace
+
+int swr_clear(const swr_clear_desc *d, void *stream) {
+  if (!d || !d->target.data) return fail(SWR_ERROR_INVALID_ARGUMENT, "clear descriptor incomplete");
+  if (d->samples != 1 && d->samples != 4) return fail(SWR_ERROR_UNSUPPORTED, "clear supports 1 or 4 samples");
+  uint32_t bpp = 0;
+  uint4 v = make_uint4(0, 0, 0, 0);
+  switch (d->target.format) {
+    case SWR_FORMAT_R8G8B8A8_UNORM:
+      bpp = 4;
+      v.x = unorm8_clear(d->value.color[0]) | (unorm8_clear(d->value.color[1]) << 8) |
+            (unorm8_clear(d->value.color[2]) << 16) | (unorm8_clear(d->value.color[3]) << 24);
+      break;
+    case SWR_FORMAT_R32G32B32A32_SFLOAT:
+      bpp = 16;
+      memcpy(&v, d->value.color, 16);
+      break;
+    case SWR_FORMAT_D32_SFLOAT:
+      bpp = 4;
+      memcpy(&v.x, &d->value.depth_stencil.depth, 4);
+      break;
+    case SWR_FORMAT_S8_UINT:
+      bpp = 1;
+      v.x = d->value.depth_stencil.stencil & 0xFFu;
+      break;
+    default:
+      return fail(SWR_ERROR_UNSUPPORTED, "clear format %u", d->target.format);
+  }
+  if (d->target.row_pitch_bytes < d->width * bpp ||
+      (d->samples > 1 && d->target.sample_pitch_bytes < (uint64_t)d->target.row_pitch_bytes * d->height))
+    return fail(SWR_ERROR_INVALID_ARGUMENT, "clear pitches too small");
+  int64_t x0 = 0, y0 = 0, x1 = d->width, y1 = d->height;
+  if (d->use_area) {
+    x0 = std::max<int64_t>(x0, d->area.x);
+    y0 = std::max<int64_t>(y0, d->area.y);
+    x1 = std::min<int64_t>(x1, (int64_t)d->area.x + d->area.width);
+    y1 = std::min<int64_t>(y1, (int64_t)d->area.y + d->area.height);
+  }
+  if (x1 <= x0 || y1 <= y0) return SWR_OK;
+  const int w = (int)(x1 - x0), h = (int)(y1 - y0);
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  dim3 block(256), grid((w + 255) / 256, h, d->samples);
+  char *base = (char *)d->target.data;
+  const int pitch = (int)d->target.row_pitch_bytes;
+  const long long slice = (long long)d->target.sample_pitch_bytes;
+  if (bpp == 16)
+    hipLaunchKernelGGL(clear_rect_kernel<16>, grid, block, 0, s, base, pitch, slice, (int)x0, (int)y0, w, h, v);
+  else if (bpp == 4)
+    hipLaunchKernelGGL(clear_rect_kernel<4>, grid, block, 0, s, base, pitch, slice, (int)x0, (int)y0, w, h, v);
+  else
+    hipLaunchKernelGGL(clear_rect_kernel<1>, grid, block, 0, s, base, pitch, slice, (int)x0, (int)y0, w, h, v);
+  SWR_HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}
+
 int swr_profile_enable(int enable) {
   DeviceCtx *c = nullptr;
   int r = current_ctx(&c);

@@ -1024,6 +1024,20 @@ __global__ __launch_bounds__(256, MS == 1 ? 4 : 3) void raster_batch_kernel(
 // ---------------------------------------------------------------------------
 // MSAA resolve (Blitter.cpp:2076-2200 for RGBA8, :1524-1545 for float)
 // ---------------------------------------------------------------------------
+// Fills a rectangle of every sample plane with one BYTES-wide value (the
+// packed clear value).  Grid: x over the rectangle's width, y over its
+// rows, z over the sample planes.
+template <int BYTES>
+__global__ void clear_rect_kernel(char *base, int rowPitch, long long slice, int x0, int y0, int w, int h,
+                                  uint4 value) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= w || y >= h) return;
+  char *p = base + (long long)blockIdx.z * slice + (long long)(y0 + y) * rowPitch + (long long)BYTES * (x0 + x);
+  if (BYTES == 16) *reinterpret_cast<uint4 *>(p) = value;
+  else if (BYTES == 4) *reinterpret_cast<uint32_t *>(p) = value.x;
+  else *reinterpret_cast<uint8_t *>(p) = (uint8_t)value.x;
+}
+
 __global__ void resolve_rgba8_kernel(const char *src, int srcPitch, long long slice, char *dst, int dstPitch, int w,
                                      int h) {
   int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;

@@ -57,26 +57,33 @@ class Targets:
     occlusion: torch.Tensor          # (1,) int64
 
 
-def clear_value_rgba8(c):
-    return [int(np.clip(np.rint(np.float32(v) * np.float32(255.0)), 0, 255)) for v in c]
+def clear_targets(draw: DrawData, targets: Targets, stream=None, reset_occlusion: bool = True):
+    """Clears the draw's attachments to its clear values (render-pass load op
+    CLEAR) with the library's swr_clear; resets the occlusion counter unless
+    told not to."""
+    if targets.color is not None:
+        clear(targets.color, draw.color_format, draw.clear_color, stream=stream)
+    if targets.depth is not None:
+        clear(targets.depth, abi.FORMAT_D32_SFLOAT, draw.clear_depth, stream=stream)
+    if targets.stencil is not None:
+        clear(targets.stencil, abi.FORMAT_S8_UINT, draw.clear_stencil, stream=stream)
+    if reset_occlusion:
+        targets.occlusion.zero_()
 
 
 def make_targets(draw: DrawData, device) -> Targets:
     S, H, W = draw.samples, draw.height, draw.width
     if draw.color_format == abi.FORMAT_R32G32B32A32_SFLOAT:
         color = torch.empty((S, H, W, 4), dtype=torch.float32, device=device)
-        color[:] = torch.tensor(draw.clear_color, dtype=torch.float32, device=device)
     else:
         color = torch.empty((S, H, W, 4), dtype=torch.uint8, device=device)
-        color[:] = torch.tensor(clear_value_rgba8(draw.clear_color), dtype=torch.uint8, device=device)
-    depth = None
-    if draw.has_depth:
-        depth = torch.full((S, H, W), float(draw.clear_depth), dtype=torch.float32, device=device)
-    stencil = None
-    if draw.has_stencil:
-        stencil = torch.full((S, H, W), int(draw.clear_stencil), dtype=torch.uint8, device=device)
+    depth = torch.empty((S, H, W), dtype=torch.float32, device=device) if draw.has_depth else None
+    stencil = torch.empty((S, H, W), dtype=torch.uint8, device=device) if draw.has_stencil else None
     occ = torch.zeros(1, dtype=torch.int64, device=device)
-    return Targets(color, depth, stencil, occ)
+    t = Targets(color, depth, stencil, occ)
+    with torch.cuda.device(torch.device(device)):
+        clear_targets(draw, t)
+    return t
 
 
 @dataclass
@@ -150,6 +157,36 @@ def resolve(src: torch.Tensor, dst: torch.Tensor, stream=None):
     bpp = 16 if src.dtype == torch.float32 else 4
     r = abi.ResolveDesc(W, H, fmt, src.data_ptr(), W * bpp, H * W * bpp, S, dst.data_ptr(), W * bpp)
     check(lib().swr_resolve(ctypes.byref(r), _stream_handle(stream)), "swr_resolve")
+
+
+def clear_desc(target: torch.Tensor, fmt: int, value, area=None) -> abi.ClearDesc:
+    """A swr_clear_desc for a (S, H, W[, C]) target tensor of format fmt.
+
+    value: 4 floats (colour), one float (D32F depth) or one int (S8 stencil);
+    area: (x, y, width, height) or None for the whole image."""
+    S, H, W = target.shape[:3]
+    bpp = {abi.FORMAT_R8G8B8A8_UNORM: 4, abi.FORMAT_R32G32B32A32_SFLOAT: 16, abi.FORMAT_D32_SFLOAT: 4,
+           abi.FORMAT_S8_UINT: 1}[fmt]
+    d = abi.ClearDesc()
+    d.target = abi.Attachment(target.data_ptr(), fmt, W * bpp, H * W * bpp)
+    d.width, d.height, d.samples = W, H, S
+    if area is not None:
+        d.use_area = 1
+        d.area = abi.Rect(*area)
+    if fmt in (abi.FORMAT_R8G8B8A8_UNORM, abi.FORMAT_R32G32B32A32_SFLOAT):
+        for k in range(4):
+            d.value.color[k] = float(value[k])
+    elif fmt == abi.FORMAT_D32_SFLOAT:
+        d.value.depth_stencil.depth = float(value)
+    else:
+        d.value.depth_stencil.stencil = int(value) & 0xFFFFFFFF
+    return d
+
+
+def clear(target: torch.Tensor, fmt: int, value, area=None, stream=None):
+    """Attachment clear (vk::Image::clear -> sw::Blitter::clear) of every sample of `area`."""
+    d = clear_desc(target, fmt, value, area)
+    check(lib().swr_clear(ctypes.byref(d), _stream_handle(stream)), "swr_clear")
 
 
 def profile_enable(enable: bool = True):

@@ -35,6 +35,8 @@ def oracle():
         L.swo_draw.restype = ctypes.c_int
         L.swo_resolve.argtypes = [ctypes.POINTER(abi.ResolveDesc)]
         L.swo_resolve.restype = ctypes.c_int
+        L.swo_clear.argtypes = [ctypes.POINTER(abi.ClearDesc)]
+        L.swo_clear.restype = ctypes.c_int
         for f in ("swo_round_int", "swo_round_int_clamped", "swo_trunc_int"):
             getattr(L, f).argtypes = [ctypes.c_float]
             getattr(L, f).restype = ctypes.c_int32
@@ -66,16 +68,19 @@ def reference_lib():
 
 
 def host_targets(draw):
+    """Host attachments cleared by the oracle's clear (the same values the
+    library's swr_clear writes)."""
     S, H, W = draw.samples, draw.height, draw.width
     if draw.color_format == abi.FORMAT_R32G32B32A32_SFLOAT:
         color = np.empty((S, H, W, 4), np.float32)
-        color[:] = np.asarray(draw.clear_color, np.float32)
     else:
         color = np.empty((S, H, W, 4), np.uint8)
-        cc = [int(np.clip(np.rint(np.float32(v) * np.float32(255.0)), 0, 255)) for v in draw.clear_color]
-        color[:] = np.asarray(cc, np.uint8)
-    depth = np.full((S, H, W), np.float32(draw.clear_depth), np.float32) if draw.has_depth else None
-    stencil = np.full((S, H, W), draw.clear_stencil, np.uint8) if draw.has_stencil else None
+    oracle_clear(color, draw.color_format, draw.clear_color)
+    depth = stencil = None
+    if draw.has_depth:
+        depth = oracle_clear(np.empty((S, H, W), np.float32), abi.FORMAT_D32_SFLOAT, draw.clear_depth)
+    if draw.has_stencil:
+        stencil = oracle_clear(np.empty((S, H, W), np.uint8), abi.FORMAT_S8_UINT, draw.clear_stencil)
     return color, depth, stencil
 
 
@@ -143,3 +148,30 @@ def oracle_resolve(src):
     if rc != 0:
         raise RuntimeError(f"oracle swo_resolve failed: {rc}")
     return dst
+
+
+def oracle_clear(target, fmt, value, area=None):
+    """Clears a (S, H, W[, C]) host array in place with the oracle (same
+    descriptor as the library's swr_clear)."""
+    L = oracle()
+    assert target.flags["C_CONTIGUOUS"]
+    S, H, W = target.shape[:3]
+    bpp = {abi.FORMAT_R8G8B8A8_UNORM: 4, abi.FORMAT_R32G32B32A32_SFLOAT: 16, abi.FORMAT_D32_SFLOAT: 4,
+           abi.FORMAT_S8_UINT: 1}[fmt]
+    d = abi.ClearDesc()
+    d.target = abi.Attachment(target.ctypes.data, fmt, W * bpp, H * W * bpp)
+    d.width, d.height, d.samples = W, H, S
+    if area is not None:
+        d.use_area = 1
+        d.area = abi.Rect(*area)
+    if fmt in (abi.FORMAT_R8G8B8A8_UNORM, abi.FORMAT_R32G32B32A32_SFLOAT):
+        for k in range(4):
+            d.value.color[k] = float(value[k])
+    elif fmt == abi.FORMAT_D32_SFLOAT:
+        d.value.depth_stencil.depth = float(value)
+    else:
+        d.value.depth_stencil.stencil = int(value) & 0xFFFFFFFF
+    rc = L.swo_clear(ctypes.byref(d))
+    if rc != 0:
+        raise RuntimeError(f"oracle swo_clear failed: {rc}")
+    return target
