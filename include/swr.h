@@ -240,6 +240,19 @@ typedef struct swr_draw_desc {
    * 3 * triangle_count indices, a strip or fan triangle_count + 2. */
   uint32_t topology;
   uint32_t provoking_vertex;
+
+  /* Depth bias (VkPipelineRasterizationStateCreateInfo): the setup adds
+   * r * constant + max(|dz/dx|, |dz/dy|) * slope, clamped by depth_bias_clamp
+   * when it is non-zero (a NaN clamp is no clamp), to every sample's depth
+   * before the depth clamp and test (SetupRoutine.cpp:417-475,
+   * PixelRoutine.cpp:141-144, Context.cpp:640-661).  D32F: r = 2^(e - 23),
+   * e the largest exponent of the depth plane's constant and of
+   * (z1 - z0) * range + near, (z2 - z0) * range + near, as the reference
+   * forms them. */
+  uint32_t depth_bias_enable;
+  float depth_bias_constant_factor;
+  float depth_bias_clamp;
+  float depth_bias_slope_factor;
 } swr_draw_desc;
 
 typedef struct swr_resolve_desc { /* 4x -> 1x colour resolve */

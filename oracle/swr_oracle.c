@@ -24,6 +24,8 @@
  *                    src/Vulkan/VkSampler.hpp:33-36,
  *                    src/Vulkan/VkDescriptorSetLayout.cpp:298-334,468-503
  *   blend state      src/Device/Context.cpp:1090-1311
+ *   depth bias       src/Pipeline/SetupRoutine.cpp:417-475, PixelRoutine.cpp:141-144,
+ *                    src/Device/Context.cpp:640-661
  *   resolve          src/Device/Blitter.cpp:1524-1545, 2076-2200
  *   clear            src/Device/Blitter.cpp:55-326, src/Vulkan/VkImage.cpp:1105-1124
  *   numerics         src/Reactor/SubzeroReactor.cpp (RoundInt, Int(Float),
@@ -190,6 +192,7 @@ typedef struct {
   int yMin, yMax;
   float x0, y0;
   Plane z, w, V[NATTR];
+  float zBias;
   int frontFacing;
 } OPrim;
 
@@ -215,6 +218,9 @@ typedef struct {
   int enableMS;
   unsigned msMask;
   int depthTestActive, depthWriteEnable, depthCompare;
+  /* depth bias (SetupProcessor.cpp:75-77, PixelProcessor.cpp:119) */
+  int biasConstant, biasSlope, biasClamp, depthBias;
+  float constantDepthBias, slopeDepthBias, depthBiasClamp;
   int stencilActive;
   int colorWriteActive; /* 4-bit mask, 0 if no colour attachment */
   int colorIsFloat;
@@ -513,6 +519,37 @@ static int setup_routine(Ctx *c, OPrim *prim, const OVertex *tri[3], const Polyg
     prim->z.A = (fy2 * z1 - fy1 * z2) * D;
     prim->z.B = (fx1 * z2 - fx2 * z1) * D;
     prim->z.C = z0 * c->depthRange + c->depthNear;
+    /* Depth bias, floating-point depth buffer (SetupRoutine.cpp:417-475).
+     * Z1, Z2 are formed from z1, z2 after they became differences from z0
+     * (:384-385), as the reference does; Abs is Reactor's (x > 0 ? x : -x,
+     * Reactor.cpp:3975-3978), Min/Max NaN-ordered. */
+    float bias = 0.0f;
+    if (c->biasConstant) {
+      float Z0 = prim->z.C;
+      float Z1 = z1 * c->depthRange + c->depthNear;
+      float Z2 = z2 * c->depthRange + c->depthNear;
+      int32_t e0, e1, e2;
+      memcpy(&e0, &Z0, 4);
+      memcpy(&e1, &Z1, 4);
+      memcpy(&e2, &Z2, 4);
+      e0 &= 0x7F800000;
+      e1 &= 0x7F800000;
+      e2 &= 0x7F800000;
+      int32_t e = imax(imax(e0, e1), e2);
+      float r;
+      memcpy(&r, &e, 4);
+      r = r * (1.0f / (1 << 23));
+      bias = r * c->constantDepthBias;
+    }
+    if (c->biasSlope) {
+      float aA = prim->z.A > 0.0f ? prim->z.A : -prim->z.A;
+      float aB = prim->z.B > 0.0f ? prim->z.B : -prim->z.B;
+      float m = rr_max(aA, aB);
+      bias = bias + m * c->slopeDepthBias;
+    }
+    if ((c->biasConstant || c->biasSlope) && c->biasClamp)
+      bias = c->depthBiasClamp > 0.0f ? rr_min(bias, c->depthBiasClamp) : rr_max(bias, c->depthBiasClamp);
+    prim->zBias = bias;
   }
   /* Perspective-correct gradients (setupGradient, SetupRoutine.cpp:514-548). */
   for (int k = 0; k < NATTR; k++) {
@@ -947,6 +984,7 @@ static void quad(Ctx *c, const OPrim *prim, int x, int y, const int cMask[4], co
         float xx = xFragment[i];
         if (c->enableMS) xx = xx - kSampleLocationsX[q];
         z[q][i] = xx * prim->z.A + Dz[q][i];
+        if (c->depthBias) z[q][i] = z[q][i] + prim->zBias; /* PixelRoutine.cpp:141-144 */
       }
     }
   }
@@ -1183,6 +1221,18 @@ static int init_ctx(Ctx *c, const swr_draw_desc *d) {
   c->depthTestActive = d->depth.data && d->depth_test_enable;
   c->depthWriteEnable = c->depthTestActive && d->depth_write_enable;
   c->depthCompare = (int)d->depth_compare_op;
+  {
+    /* Context.cpp:640-661: factors only when enabled, a NaN clamp is 0. */
+    int en = d->depth_bias_enable != 0;
+    c->constantDepthBias = en ? d->depth_bias_constant_factor : 0.0f;
+    c->slopeDepthBias = en ? d->depth_bias_slope_factor : 0.0f;
+    c->depthBiasClamp = en ? d->depth_bias_clamp : 0.0f;
+    if (c->depthBiasClamp != c->depthBiasClamp) c->depthBiasClamp = 0.0f;
+    c->biasConstant = c->constantDepthBias != 0.0f;
+    c->biasSlope = c->slopeDepthBias != 0.0f;
+    c->biasClamp = c->depthBiasClamp != 0.0f;
+    c->depthBias = c->depthTestActive && (c->biasConstant || c->biasSlope);
+  }
   c->stencilActive = d->stencil.data && d->stencil_test_enable;
 
   c->colorIsFloat = d->color.format == SWR_FORMAT_R32G32B32A32_SFLOAT;

@@ -111,6 +111,8 @@ class DrawDesc(C.Structure):
         ("occlusion_count", C.c_void_p),
         ("primitive_id", Attachment),
         ("topology", C.c_uint32), ("provoking_vertex", C.c_uint32),
+        ("depth_bias_enable", C.c_uint32), ("depth_bias_constant_factor", C.c_float),
+        ("depth_bias_clamp", C.c_float), ("depth_bias_slope_factor", C.c_float),
     ]
 
 

@@ -279,6 +279,23 @@ int build_params(const swr_draw_desc *d, const swr_tuning &tuning, DrawParams &P
   P.depthTest = (d->depth.data && d->depth_test_enable) ? 1 : 0;
   P.depthWrite = (P.depthTest && d->depth_write_enable) ? 1 : 0;
   P.depthCompare = (int)d->depth_compare_op;
+  // Depth bias state (Context.cpp:640-661: a NaN clamp is no clamp;
+  // SetupProcessor.cpp:75-77 and PixelProcessor.cpp:119: each term only
+  // when its factor is non-zero).
+  {
+    const bool en = d->depth_bias_enable != 0;
+    const float cst = en ? d->depth_bias_constant_factor : 0.0f;
+    const float slope = en ? d->depth_bias_slope_factor : 0.0f;
+    float clampv = en ? d->depth_bias_clamp : 0.0f;
+    if (clampv != clampv) clampv = 0.0f;
+    P.biasConstant = cst != 0.0f;
+    P.biasSlope = slope != 0.0f;
+    P.biasClamp = clampv != 0.0f;
+    P.depthBias = P.depthTest && (P.biasConstant || P.biasSlope);
+    P.constantDepthBias = cst;
+    P.slopeDepthBias = slope;
+    P.depthBiasClamp = clampv;
+  }
   P.stencilActive = (d->stencil.data && d->stencil_test_enable) ? 1 : 0;
   int r;
   if ((r = stencil_params(d->front, P.front)) != SWR_OK) return r;

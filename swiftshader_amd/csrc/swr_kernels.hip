@@ -174,7 +174,7 @@ __device__ __forceinline__ void setup_one(const DrawParams &P, int t, size_t g, 
 
   TriRecord &R = rec[g];
   tileCount[g] = 0;
-  R.n = 0;
+  R.nflags = 0;
   R.draw = draw;
   // setupSolidTriangles (Renderer.cpp:733-776).
   if ((flags[0] & flags[1] & flags[2]) != CLIP_FINITE) return;
@@ -289,8 +289,30 @@ __device__ __forceinline__ void setup_one(const DrawParams &P, int t, size_t g, 
     R.zA = (fy2 * z1 - fy1 * z2) * D;
     R.zB = (fx1 * z2 - fx2 * z1) * D;
     R.zC = z0 * P.depthRange + P.depthNear;
+    // Depth bias (SetupRoutine.cpp:417-475), floating-point depth buffer:
+    // r = 2^(e - 23) for the largest exponent e among the plane constant and
+    // z1, z2 mapped through the depth range -- z1, z2 being the differences
+    // from z0 formed above, as in the reference.
+    float bias = 0.0f;
+    if (P.biasConstant) {
+      const float Z1 = z1 * P.depthRange + P.depthNear, Z2 = z2 * P.depthRange + P.depthNear;
+      const int e0 = __float_as_int(R.zC) & 0x7F800000, e1 = __float_as_int(Z1) & 0x7F800000;
+      const int e2 = __float_as_int(Z2) & 0x7F800000;
+      const int e = max(max(e0, e1), e2);
+      const float r = __int_as_float(e) * (1.0f / (float)(1 << 23));
+      bias = r * P.constantDepthBias;
+    }
+    if (P.biasSlope) {
+      const float aA = R.zA > 0.0f ? R.zA : -R.zA, aB = R.zB > 0.0f ? R.zB : -R.zB;  // Reactor's Abs
+      const float m = rr_max(aA, aB);
+      bias = bias + m * P.slopeDepthBias;
+    }
+    if ((P.biasConstant || P.biasSlope) && P.biasClamp)
+      bias = P.depthBiasClamp > 0.0f ? rr_min(bias, P.depthBiasClamp) : rr_max(bias, P.depthBiasClamp);
+    R.zBias = bias;
   } else {
     R.zA = R.zB = R.zC = 0.0f;
+    R.zBias = 0.0f;
   }
 #pragma unroll
   for (int k = 0; k < 6; k++) {
@@ -305,12 +327,16 @@ __device__ __forceinline__ void setup_one(const DrawParams &P, int t, size_t g, 
   // the affine minimum at a corner, less a margin of 8 ulp of the largest
   // term.  Lets a tile skip primitives that cannot pass its depth test.
   float zlo = -__builtin_inff();
-  if (P.depthTest && __builtin_isfinite(R.zA) && __builtin_isfinite(R.zB) && __builtin_isfinite(R.zC)) {
+  if (P.depthTest && __builtin_isfinite(R.zA) && __builtin_isfinite(R.zB) && __builtin_isfinite(R.zC) &&
+      __builtin_isfinite(R.zBias)) {
     const double xa = (double)xMin - 0.5 - (double)R.x0, xb = (double)(xMax - 1) + 0.5 - (double)R.x0;
     const double ya = (double)yMin - 0.5 - (double)R.y0, yb = (double)(yMax - 1) + 0.5 - (double)R.y0;
     const double A = R.zA, B = R.zB, C = R.zC;
-    const double m = fmin(fmin(A * xa + B * ya, A * xb + B * ya), fmin(A * xa + B * yb, A * xb + B * yb)) + C;
-    const double mag = fabs(A) * fmax(fabs(xa), fabs(xb)) + fabs(C) + 2.0 * fabs(B) * fmax(fabs(ya), fabs(yb));
+    const double Zb = P.depthBias ? (double)R.zBias : 0.0;
+    const double m =
+        fmin(fmin(A * xa + B * ya, A * xb + B * ya), fmin(A * xa + B * yb, A * xb + B * yb)) + C + Zb;
+    const double mag =
+        fabs(A) * fmax(fabs(xa), fabs(xb)) + fabs(C) + 2.0 * fabs(B) * fmax(fabs(ya), fabs(yb)) + 2.0 * fabs(Zb);
     const double lo = m - mag * 0x1p-21 - 1e-30;
     if (__builtin_isfinite(lo)) {
       float f = (float)lo;
@@ -322,7 +348,6 @@ __device__ __forceinline__ void setup_one(const DrawParams &P, int t, size_t g, 
     }
   }
   R.zlo = zlo;
-  R.flags = d | (frontFacing ? 2 : 0);
   R.yMin = yMin;
   R.yMax = yMax;
   R.xMin = xMin;
@@ -334,7 +359,7 @@ __device__ __forceinline__ void setup_one(const DrawParams &P, int t, size_t g, 
     int *pp = poly + g * (2 * kMaxPolygon);
     for (int i = 0; i < n; i++) { pp[i] = X[i]; pp[kMaxPolygon + i] = Y[i]; }
   }
-  R.n = n;
+  R.nflags = n | (d << 8) | (frontFacing ? 1 << 9 : 0);
   int tx0 = xMin / P.tileW, tx1 = (xMax - 1) / P.tileW;
   int ty0 = yMin / P.tileH, ty1 = (yMax - 1) / P.tileH;
   tileCount[g] = (uint32_t)((tx1 - tx0 + 1) * (ty1 - ty0 + 1));
@@ -732,24 +757,24 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
     // ---- triangle-parallel coverage of this block ----
     uint32_t mlo[MS], mhi[MS];
     int flags = 0;
-    float x0 = 0, y0 = 0, zA = 0, zB = 0, zC = 0;
+    float x0 = 0, y0 = 0, zA = 0, zB = 0, zC = 0, zBias = 0;
 #pragma unroll
     for (int q = 0; q < MS; q++) { mlo[q] = 0; mhi[q] = 0; }
     if (act) {
       const int4 *rp = reinterpret_cast<const int4 *>(rec + tid);
-      const int4 h0 = rp[0];  // n, flags, yMin, yMax
-      const int4 h1 = rp[1];  // xMin, xMax, X0, X1
-      const int4 h2 = rp[2];  // X2, Y0, Y1, Y2
-      const float4 h3 = reinterpret_cast<const float4 *>(rp)[3];  // zlo, x0, y0, zA
-      const float4 h4 = reinterpret_cast<const float4 *>(rp)[4];  // zB, zC, wA, wB
-      const int n = h0.x, d = h0.y & 1;
-      flags = h0.y;
-      x0 = h3.y; y0 = h3.z; zA = h3.w; zB = h4.x; zC = h4.y;
+      const int4 h0 = rp[0];  // nflags, yMin, yMax, xMin
+      const int4 h1 = rp[1];  // xMax, X0, X1, X2
+      const int4 h2 = rp[2];  // Y0, Y1, Y2, zlo
+      const float4 h3 = reinterpret_cast<const float4 *>(rp)[3];  // x0, y0, zA, zB
+      const float4 h4 = reinterpret_cast<const float4 *>(rp)[4];  // zC, zBias, wA, wB
+      const int n = h0.x & 0xFF, d = (h0.x >> 8) & 1;
+      flags = h0.x >> 8;
+      x0 = h3.x; y0 = h3.y; zA = h3.z; zB = h3.w; zC = h4.x; zBias = h4.y;
       const int *pp = poly + (size_t)tid * (2 * kMaxPolygon);
       auto vtx = [&](int i, int &Xo, int &Yo) {
         if (n == 3) {
-          Xo = i == 0 ? h1.z : (i == 1 ? h1.w : h2.x);
-          Yo = i == 0 ? h2.y : (i == 1 ? h2.z : h2.w);
+          Xo = i == 0 ? h1.y : (i == 1 ? h1.z : h1.w);
+          Yo = i == 0 ? h2.x : (i == 1 ? h2.y : h2.z);
         } else {
           Xo = pp[i];
           Yo = pp[kMaxPolygon + i];
@@ -795,6 +820,7 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
         const float xf = (float)px - rdl(x0, j);  // xFragment (PixelRoutine.cpp:125)
         const float yf = (float)py - rdl(y0, j);  // yFragment (QuadRasterizer.cpp:126)
         const float a = rdl(zA, j), b = rdl(zB, j), c = rdl(zC, j);
+        const float zbias = P.depthBias ? rdl(zBias, j) : 0.0f;
         zMask = 0;
 #pragma unroll
         for (int q = 0; q < MS; q++) {
@@ -802,6 +828,7 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
           const float xx = (MS > 1) ? xf - kSampleX[q] : xf;
           const float Dz = c + yy * b;
           float z = xx * a + Dz;
+          if (P.depthBias) z = z + zbias;  // PixelRoutine.cpp:141-144, before the clamp
           z = rr_min(rr_max(z, 0.0f), 1.0f);
           zq[q] = z;
           const float zb = S.zb[q];
@@ -881,20 +908,25 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
     if (e < end) {
       tid = binTri[e];
       const int4 *rp = reinterpret_cast<const int4 *>(rec + tid);
-      const int4 h0 = rp[0];  // n, flags, yMin, yMax
-      const int4 h1 = rp[1];  // xMin, xMax, ...
-      test = h0.z < oy + 8 && h0.w > oy && h1.x < ox + 8 && h1.y > ox;
+      const int4 h0 = rp[0];  // nflags, yMin, yMax, xMin
+      const int4 h1 = rp[1];  // xMax, ...
+      test = h0.y < oy + 8 && h0.z > oy && h0.w < ox + 8 && h1.x > ox;
       if (hiz && test) {
-        const float4 h3 = reinterpret_cast<const float4 *>(rp)[3];  // zlo, x0, y0, zA
-        const float4 h4 = reinterpret_cast<const float4 *>(rp)[4];  // zB, zC, ...
-        const float x0 = h3.y, y0 = h3.z, zA = h3.w, zB = h4.x, zC = h4.y;
+        const float zlo0 = __int_as_float(rp[2].w);                  // zlo
+        const float4 h3 = reinterpret_cast<const float4 *>(rp)[3];  // x0, y0, zA, zB
+        const float4 h4 = reinterpret_cast<const float4 *>(rp)[4];  // zC, zBias, ...
+        const float x0 = h3.x, y0 = h3.y, zA = h3.z, zB = h3.w, zC = h4.x;
+        const float zBias = P.depthBias ? h4.y : 0.0f;
         // Lower bound of the evaluated plane over this block (sample offsets
         // included), tightened with the primitive's bound over its box.
         const float xa = (float)ox - 0.5f - x0, xb = (float)(ox + 7) + 0.5f - x0;
         const float ya = (float)oy - 0.5f - y0, yb = (float)(oy + 7) + 0.5f - y0;
-        const float m = fminf(fminf(zA * xa + zB * ya, zA * xb + zB * ya), fminf(zA * xa + zB * yb, zA * xb + zB * yb)) + zC;
-        const float mag = fabsf(zA) * fmaxf(fabsf(xa), fabsf(xb)) + fabsf(zC) + 2.0f * fabsf(zB) * fmaxf(fabsf(ya), fabsf(yb));
-        float zl = fmaxf(h3.x, m - mag * 0x1p-19f);
+        const float m =
+            fminf(fminf(zA * xa + zB * ya, zA * xb + zB * ya), fminf(zA * xa + zB * yb, zA * xb + zB * yb)) + zC +
+            zBias;
+        const float mag = fabsf(zA) * fmaxf(fabsf(xa), fabsf(xb)) + fabsf(zC) +
+                          2.0f * fabsf(zB) * fmaxf(fabsf(ya), fabsf(yb)) + 2.0f * fabsf(zBias);
+        float zl = fmaxf(zlo0, m - mag * 0x1p-19f);
         if (!(zl == zl)) zl = -__builtin_inff();
         zl = fminf(zl, 1.0f);
         test = !(P.depthCompare == 1 ? zl >= zmax : zl > zmax);

@@ -62,6 +62,9 @@ struct DrawParams {
   int ms, msMask;
   // Depth / stencil.
   int depthTest, depthWrite, depthCompare;
+  // Depth bias: which terms the setup applies, and whether samples get it.
+  int biasConstant, biasSlope, biasClamp, depthBias;
+  float constantDepthBias, slopeDepthBias, depthBiasClamp;
   int stencilActive, stencilWrite;
   StencilParams front, back;
   // Colour.
@@ -96,14 +99,15 @@ struct DrawParams {
 // 80 bytes are what the coverage and depth passes read per lane; the rest
 // (w plane and attribute planes) is read when fragments are shaded.
 struct TriRecord {
-  int n;            // polygon vertex count (3 unless clipped)
-  int flags;        // bit0: d (winding), bit1: front facing
+  int nflags;       // bits 0-7: polygon vertex count n (3 unless clipped; 0: culled);
+                    // bit 8: d (winding); bit 9: front facing
   int yMin, yMax;   // rows with possible coverage (scissor-clamped)
   int xMin, xMax;   // pixel columns with possible coverage
   int X[3], Y[3];   // 28.4 polygon vertices when n == 3
-  float zlo;        // lower bound of the evaluated depth over the bounding box
+  float zlo;        // lower bound of the evaluated depth (bias included) over the bounding box
   float x0, y0;
   float zA, zB, zC;
+  float zBias;      // depth bias added to every sample's depth (0 when off)
   float wA, wB, wC;
   float V[6][3];    // colour rgba, uv
   int draw;         // index of the primitive's draw in its batch

@@ -111,4 +111,7 @@ def build_desc(draw: DrawData, buf: Buffers) -> abi.DrawDesc:
     d.occlusion_count = buf.occlusion or None
     d.topology = draw.topology
     d.provoking_vertex = draw.provoking_vertex
+    if draw.depth_bias is not None:
+        d.depth_bias_enable = 1
+        (d.depth_bias_constant_factor, d.depth_bias_clamp, d.depth_bias_slope_factor) = draw.depth_bias
     return d

@@ -82,6 +82,7 @@ class DrawData:
     has_stencil: bool = False
     topology: int = abi.TOPOLOGY_TRIANGLE_LIST
     provoking_vertex: int = abi.PROVOKING_VERTEX_FIRST
+    depth_bias: Optional[tuple] = None  # (constant_factor, clamp, slope_factor); None = disabled
 
     @property
     def triangle_count(self):

@@ -190,6 +190,14 @@ def _singular_uv_aniso(seed=31, float_tex=False, samples=1):
                     shader=abi.SHADER_MODULATE, texture=tex, color_format=fmt)
 
 
+def _depth_bias(seed=33, bias=(800.0, 0.0, 1.5), samples=1, size=88, tris=700):
+    """Overlapping triangles of varied slope and depth with a depth bias."""
+    rng = np.random.default_rng(seed)
+    pos, uv, col = scenes.screen_triangles(rng, tris, size, size - 6, area_hi=900)
+    return DrawData(width=size, height=size - 6, positions=pos, colors=col, texcoords=uv, samples=samples,
+                    shader=abi.SHADER_INTERPOLATE, depth_bias=bias, depth_compare=abi.COMPARE_LESS_OR_EQUAL)
+
+
 def parity_cases():
     """(name, DrawData) pairs; every one must match the oracle bit for bit."""
     return [
@@ -219,4 +227,7 @@ def parity_cases():
         ("list_provoking_last_clipped", _list_provoking_last()),
         ("singular_uv_aniso_rgba8", _singular_uv_aniso()),
         ("singular_uv_aniso_float_msaa4", _singular_uv_aniso(seed=32, float_tex=True, samples=4)),
+        ("depth_bias_constant_slope", _depth_bias()),
+        ("depth_bias_clamp_negative_msaa4", _depth_bias(seed=34, bias=(-3000.0, -0.002, -2.0), samples=4)),
+        ("depth_bias_clamp_positive", _depth_bias(seed=35, bias=(1e6, 0.004, 0.0))),
     ]
