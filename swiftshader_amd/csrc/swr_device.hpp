@@ -85,11 +85,30 @@ __device__ __forceinline__ uint32_t offset16(uint32_t uvw, uint32_t off, int wra
 
 struct Texel16 { uint32_t c[4]; };
 
-__device__ __forceinline__ void fetch16(const TexParams &T, int lvl, uint32_t u, uint32_t v, uint32_t out[4]) {
-  uint32_t uu = mulhi16(u, (uint32_t)T.width[lvl]);
-  uint32_t vv = mulhi16(v, (uint32_t)T.height[lvl]);
-  int32_t index = (int32_t)(int16_t)uu + (int32_t)(int16_t)vv * (int32_t)(int16_t)T.pitchP[lvl];
-  uint32_t p = reinterpret_cast<const uint32_t *>(T.data[lvl])[(uint32_t)index];
+// One mip level's parameters, read once per sampling call: the level
+// varies per lane, so each read of the parameter arrays is a memory load.
+struct TexLevel {
+  const void *data;
+  int width, height, pitch;
+  uint32_t uHalf, vHalf;
+};
+
+__device__ __forceinline__ TexLevel tex_level(const TexParams &T, int lvl) {
+  TexLevel L;
+  L.data = T.data[lvl];
+  L.width = T.width[lvl];
+  L.height = T.height[lvl];
+  L.pitch = T.pitchP[lvl];
+  L.uHalf = T.uHalf[lvl];
+  L.vHalf = T.vHalf[lvl];
+  return L;
+}
+
+__device__ __forceinline__ void fetch16(const TexLevel &L, uint32_t u, uint32_t v, uint32_t out[4]) {
+  uint32_t uu = mulhi16(u, (uint32_t)L.width);
+  uint32_t vv = mulhi16(v, (uint32_t)L.height);
+  int32_t index = (int32_t)(int16_t)uu + (int32_t)(int16_t)vv * (int32_t)(int16_t)L.pitch;
+  uint32_t p = reinterpret_cast<const uint32_t *>(L.data)[(uint32_t)index];
   out[0] = (p & 0xFFu) << 8;
   out[1] = ((p >> 8) & 0xFFu) << 8;
   out[2] = ((p >> 16) & 0xFFu) << 8;
@@ -103,16 +122,15 @@ __device__ __forceinline__ int select_mip(const TexParams &T, float lod, int sec
 }
 
 // sampleQuad2D + bilinearInterpolate, RGBA8 16-bit path (:668-909, :516-666).
-__device__ __forceinline__ void sample_quad16(const TexParams &T, float u, float v, float lod, int second,
+__device__ __forceinline__ void sample_quad16(const TexParams &T, const TexLevel &L, float u, float v, float lod,
                                               uint32_t out[4]) {
-  int lvl = select_mip(T, lod, second);
   uint32_t uu = address16(u, T.addrU);
   uint32_t vv = address16(v, T.addrV);
   if (T.filter == FILT_POINT) {
-    fetch16(T, lvl, uu, vv, out);
+    fetch16(L, uu, vv, out);
     return;
   }
-  uint32_t uh = T.uHalf[lvl], vh = T.vHalf[lvl];
+  uint32_t uh = L.uHalf, vh = L.vHalf;
   if (T.filter == FILT_MIN_LINEAR_MAG_POINT) {
     if (lod <= 0.0f) { uh = 0; vh = 0; }        // offset &= CmpNLE(lod, 0)
   } else if (T.filter == FILT_MIN_POINT_MAG_LINEAR) {
@@ -121,12 +139,12 @@ __device__ __forceinline__ void sample_quad16(const TexParams &T, float u, float
   uint32_t u0 = offset16(uu, uh, !T.addrU, -1), v0 = offset16(vv, vh, !T.addrV, -1);
   uint32_t u1 = offset16(uu, uh, !T.addrU, +1), v1 = offset16(vv, vh, !T.addrV, +1);
   uint32_t c00[4], c10[4], c01[4], c11[4];
-  fetch16(T, lvl, u0, v0, c00);
-  fetch16(T, lvl, u1, v0, c10);
-  fetch16(T, lvl, u0, v1, c01);
-  fetch16(T, lvl, u1, v1, c11);
-  uint32_t f0u = (u0 * (uint32_t)T.width[lvl]) & 0xFFFFu;
-  uint32_t f0v = (v0 * (uint32_t)T.height[lvl]) & 0xFFFFu;
+  fetch16(L, u0, v0, c00);
+  fetch16(L, u1, v0, c10);
+  fetch16(L, u0, v1, c01);
+  fetch16(L, u1, v1, c11);
+  uint32_t f0u = (u0 * (uint32_t)L.width) & 0xFFFFu;
+  uint32_t f0v = (v0 * (uint32_t)L.height) & 0xFFFFu;
   uint32_t f1u = f0u ^ 0xFFFFu, f1v = f0v ^ 0xFFFFu;
   uint32_t f0u0v = mulhi16(f0u, f0v), f1u0v = mulhi16(f1u, f0v);
   uint32_t f0u1v = mulhi16(f0u, f1v), f1u1v = mulhi16(f1u, f1v);
@@ -140,8 +158,9 @@ __device__ __forceinline__ void sample_quad16(const TexParams &T, float u, float
 
 __device__ __forceinline__ void sample_aniso16(const TexParams &T, float u, float v, float lod, float aniso,
                                                float uDelta, float vDelta, int second, uint32_t out[4]) {
+  const TexLevel L = tex_level(T, select_mip(T, lod, second));
   if (T.filter != FILT_ANISOTROPIC) {
-    sample_quad16(T, u, v, lod, second, out);
+    sample_quad16(T, L, u, v, lod, out);
     return;
   }
   int N = rr_round_int(aniso);
@@ -157,7 +176,7 @@ __device__ __forceinline__ void sample_aniso16(const TexParams &T, float u, floa
   int i = 0;
   do {
     uint32_t c[4];
-    sample_quad16(T, u0, v0, lod, second, c);
+    sample_quad16(T, L, u0, v0, lod, c);
     u0 = u0 + du;
     v0 = v0 + dv;
 #pragma unroll
@@ -202,22 +221,21 @@ __device__ __forceinline__ void address_float(const TexParams &T, float uvw, int
   }
 }
 
-__device__ __forceinline__ float4 fetchf(const TexParams &T, int lvl, int x, int yoff) {
-  return reinterpret_cast<const float4 *>(T.data[lvl])[(uint32_t)(x + yoff)];
+__device__ __forceinline__ float4 fetchf(const TexLevel &L, int x, int yoff) {
+  return reinterpret_cast<const float4 *>(L.data)[(uint32_t)(x + yoff)];
 }
 
-__device__ __forceinline__ float4 sample_float2d(const TexParams &T, float u, float v, float lod, int second) {
-  int lvl = select_mip(T, lod, second);
+__device__ __forceinline__ float4 sample_float2d(const TexParams &T, const TexLevel &L, float u, float v, float lod) {
   int x0, x1, y0, y1;
   float fu, fv;
-  address_float(T, u, T.width[lvl], T.addrU, lod, x0, x1, fu);
-  address_float(T, v, T.height[lvl], T.addrV, lod, y0, y1, fv);
-  int pitch = T.pitchP[lvl];
+  address_float(T, u, L.width, T.addrU, lod, x0, x1, fu);
+  address_float(T, v, L.height, T.addrV, lod, y0, y1, fv);
+  int pitch = L.pitch;
   y0 = (int)((uint32_t)y0 * (uint32_t)pitch);
-  if (T.filter == FILT_POINT) return fetchf(T, lvl, x0, y0);
+  if (T.filter == FILT_POINT) return fetchf(L, x0, y0);
   y1 = (int)((uint32_t)y1 * (uint32_t)pitch);
-  float4 c00 = fetchf(T, lvl, x0, y0), c10 = fetchf(T, lvl, x1, y0);
-  float4 c01 = fetchf(T, lvl, x0, y1), c11 = fetchf(T, lvl, x1, y1);
+  float4 c00 = fetchf(L, x0, y0), c10 = fetchf(L, x1, y0);
+  float4 c01 = fetchf(L, x0, y1), c11 = fetchf(L, x1, y1);
   float4 r;
   float a, b;
   a = c00.x + fu * (c10.x - c00.x); b = c01.x + fu * (c11.x - c01.x); r.x = a + fv * (b - a);
@@ -229,7 +247,8 @@ __device__ __forceinline__ float4 sample_float2d(const TexParams &T, float u, fl
 
 __device__ __forceinline__ float4 sample_float_aniso(const TexParams &T, float u, float v, float lod, float aniso,
                                                      float uDelta, float vDelta, int second) {
-  if (T.filter != FILT_ANISOTROPIC) return sample_float2d(T, u, v, lod, second);
+  const TexLevel L = tex_level(T, select_mip(T, lod, second));
+  if (T.filter != FILT_ANISOTROPIC) return sample_float2d(T, L, u, v, lod);
   int N = rr_round_int(aniso);
   const int row = aniso_row(N);
   float A = uv_weight(row), B = uv_start(row);
@@ -241,7 +260,7 @@ __device__ __forceinline__ float4 sample_float_aniso(const TexParams &T, float u
   dv = dv * A;
   int i = 0;
   do {
-    float4 c = sample_float2d(T, u0, v0, lod, second);
+    float4 c = sample_float2d(T, L, u0, v0, lod);
     u0 = u0 + du;
     v0 = v0 + dv;
     sum.x = sum.x + c.x * A;

@@ -773,8 +773,11 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
       const int *pp = poly + (size_t)tid * (2 * kMaxPolygon);
       auto vtx = [&](int i, int &Xo, int &Yo) {
         if (n == 3) {
-          Xo = i == 0 ? h1.y : (i == 1 ? h1.z : h1.w);
-          Yo = i == 0 ? h2.x : (i == 1 ? h2.y : h2.z);
+          // Mask selects: a conditional pick among the components of one
+          // vector would become an indexed access through scratch memory.
+          const int m0 = -(int)(i == 0), m1 = -(int)(i == 1), m2 = -(int)(i == 2);
+          Xo = (h1.y & m0) | (h1.z & m1) | (h1.w & m2);
+          Yo = (h2.x & m0) | (h2.y & m1) | (h2.z & m2);
         } else {
           Xo = pp[i];
           Yo = pp[kMaxPolygon + i];
