@@ -46,7 +46,7 @@
 extern "C" {
 #endif
 
-#define SWR_ABI_VERSION 3
+#define SWR_ABI_VERSION 4
 
 /* ---- status codes -------------------------------------------------------- */
 #define SWR_OK 0
@@ -104,9 +104,15 @@ extern "C" {
 #define SWR_BLEND_OP_MIN 3
 #define SWR_BLEND_OP_MAX 4
 /* VkPrimitiveTopology / VkProvokingVertexModeEXT */
+#define SWR_TOPOLOGY_LINE_LIST 1
+#define SWR_TOPOLOGY_LINE_STRIP 2
 #define SWR_TOPOLOGY_TRIANGLE_LIST 3
 #define SWR_TOPOLOGY_TRIANGLE_STRIP 4
 #define SWR_TOPOLOGY_TRIANGLE_FAN 5
+#define SWR_LINE_RASTERIZATION_DEFAULT 0 /* VkLineRasterizationModeEXT */
+#define SWR_LINE_RASTERIZATION_RECTANGULAR 1
+#define SWR_LINE_RASTERIZATION_BRESENHAM 2
+#define SWR_LINE_RASTERIZATION_RECTANGULAR_SMOOTH 3
 #define SWR_PROVOKING_VERTEX_FIRST 0
 #define SWR_PROVOKING_VERTEX_LAST 1
 /* VkCullModeFlagBits / VkFrontFace */
@@ -189,7 +195,7 @@ typedef struct swr_draw_desc {
   const float *colors;      /* float4 (shader input location 0) or NULL */
   const float *texcoords;   /* float2 (shader input location 1) or NULL */
   uint32_t vertex_count;
-  uint32_t triangle_count;  /* TRIANGLE_LIST primitives */
+  uint32_t triangle_count;  /* primitives (triangles, or lines for the line topologies) */
   const uint32_t *indices;  /* 3 per triangle, or NULL for 0,1,2,3,...  */
 
   /* Pre-rasterisation state. */
@@ -233,11 +239,14 @@ typedef struct swr_draw_desc {
   swr_attachment primitive_id;
 
   /* Primitive assembly (setBatchIndices, src/Device/Renderer.cpp:50-145):
-   * the VkPrimitiveTopology -- TRIANGLE_LIST (3; 0 is read as a list too),
-   * TRIANGLE_STRIP (4) or TRIANGLE_FAN (5) -- over the index buffer (or the
-   * vertex sequence when indices is NULL), and the VkProvokingVertexModeEXT
-   * (0 first, 1 last), which rotates each triangle's vertices.  A list needs
-   * 3 * triangle_count indices, a strip or fan triangle_count + 2. */
+   * the VkPrimitiveTopology -- LINE_LIST (1), LINE_STRIP (2), TRIANGLE_LIST
+   * (3; 0 is read as a list too), TRIANGLE_STRIP (4) or TRIANGLE_FAN (5) --
+   * over the index buffer (or the vertex sequence when indices is NULL), and
+   * the VkProvokingVertexModeEXT (0 first, 1 last), which rotates each
+   * primitive's vertices.  triangle_count counts primitives (lines for the
+   * line topologies): a triangle list needs 3 * triangle_count indices, a
+   * line list 2 * triangle_count, a strip or fan triangle_count + 2, a line
+   * strip triangle_count + 1. */
   uint32_t topology;
   uint32_t provoking_vertex;
 
@@ -253,6 +262,19 @@ typedef struct swr_draw_desc {
   float depth_bias_constant_factor;
   float depth_bias_clamp;
   float depth_bias_slope_factor;
+
+  /* Lines (DrawCall::setupLine, Renderer.cpp:920-1163).  line_width is
+   * lineWidth (0 reads as 1; accepted in (0, 64] -- the reference device
+   * exposes only 1, VkPhysicalDevice.cpp:53,820, and builds wider lines the
+   * same way).  line_rasterization_mode is the
+   * VkLineRasterizationModeEXT: BRESENHAM (2) sets up the parallelogram
+   * approximating a Bresenham line and needs samples == 1 (the reference
+   * turns multisampling off for it); every other mode sets up the
+   * rectangle of width line_width centred on the segment.  Either shape is
+   * clipped against the frustum and rasterised as a polygon.  Depth bias
+   * applies to triangles only and is refused for lines. */
+  float line_width;
+  uint32_t line_rasterization_mode;
 } swr_draw_desc;
 
 typedef struct swr_resolve_desc { /* 4x -> 1x colour resolve */

@@ -218,6 +218,9 @@ typedef struct {
   int enableMS;
   unsigned msMask;
   int depthTestActive, depthWriteEnable, depthCompare;
+  int line;           /* LINE_LIST / LINE_STRIP: primitives are lines */
+  int bresenham;      /* lines set up as Bresenham parallelograms */
+  float lineWidth;
   /* depth bias (SetupProcessor.cpp:75-77, PixelProcessor.cpp:119) */
   int biasConstant, biasSlope, biasClamp, depthBias;
   float constantDepthBias, slopeDepthBias, depthBiasClamp;
@@ -393,11 +396,14 @@ static void setup_edge(Ctx *c, int q, int Xa, int Ya, int Xb, int Yb) {
 
 /* Returns 1 if the primitive is visible. */
 static int setup_routine(Ctx *c, OPrim *prim, const OVertex *tri[3], const Polygon *poly) {
+  const int line = c->line;
   int X[17], Y[17];
   X[0] = tri[0]->X; X[1] = tri[1]->X; X[2] = tri[2]->X;
   Y[0] = tri[0]->Y; Y[1] = tri[1]->Y; Y[2] = tri[2]->Y;
   int d = 1;
-  {
+  if (line) {
+    prim->frontFacing = 1; /* not a triangle: front-facing masks, d = 1 */
+  } else {
     float x0 = (float)X[0], x1 = (float)X[1], x2 = (float)X[2];
     float y0 = (float)Y[0], y1 = (float)Y[1], y2 = (float)Y[2];
     float A = (y0 - y2) * x1 + (y2 - y1) * x0 + (y1 - y0) * x2;
@@ -411,7 +417,7 @@ static int setup_routine(Ctx *c, OPrim *prim, const OVertex *tri[3], const Polyg
   }
   int n = poly->n;
   int m = poly->i;
-  if (m != 0) {
+  if (m != 0 || line) { /* clipped, or not a triangle: reproject the polygon */
     for (int i = 0; i < n; i++) {
       const float *v = poly->P[m][i];
       float w = v[3];
@@ -460,16 +466,16 @@ static int setup_routine(Ctx *c, OPrim *prim, const OVertex *tri[3], const Polyg
   prim->yMin = yMin;
   prim->yMax = yMax;
 
-  /* Sort by minimum y (clip-space y), then by maximum w. */
+  /* Sort by minimum y (clip-space y), then by maximum w (triangles only). */
   const OVertex *v0 = tri[0], *v1 = tri[1], *v2 = tri[2];
-  {
+  if (!line) {
     float y0 = v0->pos[1], y1 = v1->pos[1], y2 = v2->pos[1];
     float ym = rr_min(rr_min(y0, y1), y2);
     const OVertex *vX;
     if (ym == y1) { vX = v0; v0 = v1; v1 = v2; v2 = vX; }
     if (ym == y2) { vX = v2; v2 = v1; v1 = v0; v0 = vX; }
   }
-  {
+  if (!line) {
     float w0 = v0->pos[3], w1 = v1->pos[3], w2 = v2->pos[3];
     float wm = rr_max(rr_max(w0, w1), w2);
     const OVertex *vX;
@@ -480,6 +486,10 @@ static int setup_routine(Ctx *c, OPrim *prim, const OVertex *tri[3], const Polyg
   float rhw0 = v0->pw;
   int X0 = v0->X, X1 = v1->X, X2 = v2->X;
   int Y0 = v0->Y, Y1 = v1->Y, Y2 = v2->Y;
+  if (line) { /* a third point across the line (SetupRoutine.cpp:317-321) */
+    X2 = wsub(wadd(X1, Y1), Y0);
+    Y2 = wsub(wadd(Y1, X0), X1);
+  }
   float fx0 = (float)X0 * (1.0f / 16.0f);
   float fy0 = (float)Y0 * (1.0f / 16.0f);
   prim->x0 = fx0;
@@ -1195,7 +1205,21 @@ static int init_ctx(Ctx *c, const swr_draw_desc *d) {
   init_constants();
   if (d->samples != 1 && d->samples != 4) return SWR_ERROR_UNSUPPORTED;
   if (d->topology != 0 && d->topology != SWR_TOPOLOGY_TRIANGLE_LIST && d->topology != SWR_TOPOLOGY_TRIANGLE_STRIP &&
-      d->topology != SWR_TOPOLOGY_TRIANGLE_FAN)
+      d->topology != SWR_TOPOLOGY_TRIANGLE_FAN && d->topology != SWR_TOPOLOGY_LINE_LIST &&
+      d->topology != SWR_TOPOLOGY_LINE_STRIP)
+    return SWR_ERROR_UNSUPPORTED;
+  c->line = d->topology == SWR_TOPOLOGY_LINE_LIST || d->topology == SWR_TOPOLOGY_LINE_STRIP;
+  c->lineWidth = d->line_width == 0.0f ? 1.0f : d->line_width;
+  if (!(c->lineWidth > 0.0f) || c->lineWidth > 64.0f) return SWR_ERROR_INVALID_ARGUMENT;
+  if (d->line_rasterization_mode > SWR_LINE_RASTERIZATION_RECTANGULAR_SMOOTH) return SWR_ERROR_INVALID_ARGUMENT;
+  c->bresenham = c->line && d->line_rasterization_mode == SWR_LINE_RASTERIZATION_BRESENHAM;
+  /* The reference sets up a Bresenham line without multisampling
+   * (SetupProcessor.cpp:83-86); that mode with samples is not built. */
+  if (c->bresenham && d->samples != 1) return SWR_ERROR_UNSUPPORTED;
+  /* The setup writes the depth bias of triangles only; for lines the
+   * reference's pixel routine would add an unwritten field. */
+  if (c->line && d->depth_bias_enable &&
+      (d->depth_bias_constant_factor != 0.0f || d->depth_bias_slope_factor != 0.0f))
     return SWR_ERROR_UNSUPPORTED;
   if (d->provoking_vertex > 1) return SWR_ERROR_INVALID_ARGUMENT;
   if (d->triangle_count && d->vertex_count == 0) return SWR_ERROR_INVALID_ARGUMENT;
@@ -1321,7 +1345,13 @@ static void fetch_vertex(const swr_draw_desc *d, uint32_t idx, OVertex *v) {
 static void assemble(const swr_draw_desc *d, uint32_t t, uint32_t out[3]) {
   const int first = d->provoking_vertex == SWR_PROVOKING_VERTEX_FIRST;
   uint32_t p[3];
-  if (d->topology == SWR_TOPOLOGY_TRIANGLE_STRIP) {
+  if (d->topology == SWR_TOPOLOGY_LINE_LIST || d->topology == SWR_TOPOLOGY_LINE_STRIP) {
+    /* (v0, v1, v1), v0 and v1 exchanged in last-vertex mode */
+    const uint32_t b = d->topology == SWR_TOPOLOGY_LINE_LIST ? 2u * t : t;
+    p[0] = b + (first ? 0u : 1u);
+    p[1] = b + (first ? 1u : 0u);
+    p[2] = b + 1u;
+  } else if (d->topology == SWR_TOPOLOGY_TRIANGLE_STRIP) {
     p[0] = t + (first ? 0u : 2u);
     p[1] = t + (t & 1u) + (first ? 1u : 0u);
     p[2] = t + (~t & 1u) + (first ? 1u : 0u);
@@ -1335,6 +1365,61 @@ static void assemble(const swr_draw_desc *d, uint32_t t, uint32_t out[3]) {
     p[2] = 3u * t + (first ? 2u : 1u);
   }
   for (int k = 0; k < 3; k++) out[k] = d->indices ? d->indices[p[k]] : p[k];
+}
+
+/* DrawCall::setupLine (Renderer.cpp:920-1163): the rectangle of width
+ * lineWidth centred on the segment v0-v1 (every mode but Bresenham), or the
+ * parallelogram approximating a Bresenham line -- v0's and v1's diamonds of
+ * half-width lineWidth / 2 joined along the major axis -- built in clip
+ * space, then clipped against every frustum plane (CLIP_FRUSTUM, depth clip
+ * on).  cullMask is always on here.  Returns 0 for a rejected line. */
+static int setup_line_polygon(const Ctx *c, const OVertex *v0, const OVertex *v1, float L[4][4], Polygon *poly) {
+  const float *P0 = v0->pos, *P1 = v1->pos;
+  if (P0[3] <= 0.0f && P1[3] <= 0.0f) return 0;
+  const float W = c->WxF * (1.0f / 16.0f);
+  const float H = c->HxF * (1.0f / 16.0f);
+  float dx = W * (P1[0] / P1[3] - P0[0] / P0[3]);
+  float dy = H * (P1[1] / P1[3] - P0[1] / P0[3]);
+  if (dx == 0.0f && dy == 0.0f) return 0;
+  if (c->bresenham) {
+    /* Diamond corners: 0 left, 1 up (+y), 2 right, 3 down; 4-7 the same
+     * around P1. */
+    const float dx0 = c->lineWidth * 0.5f * P0[3] / W, dy0 = c->lineWidth * 0.5f * P0[3] / H;
+    const float dx1 = c->lineWidth * 0.5f * P1[3] / W, dy1 = c->lineWidth * 0.5f * P1[3] / H;
+    float D[8][4];
+    for (int k = 0; k < 8; k++)
+      for (int j = 0; j < 4; j++) D[k][j] = (k < 4 ? P0 : P1)[j];
+    D[0][0] += -dx0; D[1][1] += +dy0; D[2][0] += +dx0; D[3][1] += -dy0;
+    D[4][0] += -dx1; D[5][1] += +dy1; D[6][0] += +dx1; D[7][1] += -dy1;
+    /* The four corners kept per major direction (Renderer.cpp:1117-1150). */
+    static const int pick[4][4] = {{1, 5, 7, 3}, {0, 4, 6, 2}, {0, 2, 6, 4}, {1, 3, 7, 5}};
+    const int dir = dx > -dy ? (dx > dy ? 0 : 1) : (dx > dy ? 2 : 3);
+    for (int k = 0; k < 4; k++)
+      for (int j = 0; j < 4; j++) L[k][j] = D[pick[dir][k]][j];
+  } else {
+    const float scale = c->lineWidth * 0.5f / sqrtf(dx * dx + dy * dy);
+    dx *= scale;
+    dy *= scale;
+    const float dx0h = dx * P0[3] / H, dy0w = dy * P0[3] / W;
+    const float dx1h = dx * P1[3] / H, dy1w = dy * P1[3] / W;
+    for (int k = 0; k < 4; k++) {
+      const float *src = (k == 0 || k == 3) ? P0 : P1;
+      for (int j = 0; j < 4; j++) L[k][j] = src[j];
+    }
+    L[0][0] += -dy0w;
+    L[0][1] += +dx0h;
+    L[1][0] += -dy1w;
+    L[1][1] += +dx1h;
+    L[2][0] += +dy1w;
+    L[2][1] += -dx1h;
+    L[3][0] += +dy0w;
+    L[3][1] += -dx0h;
+  }
+  for (int k = 0; k < 4; k++) poly->P[0][k] = L[k];
+  poly->n = 4;
+  poly->i = 0;
+  poly->b = 0;
+  return clipper_clip(poly, CLIP_FRUSTUM);
 }
 
 /* Renders one draw with host pointers in the descriptor. */
@@ -1354,14 +1439,20 @@ EXPORT int swo_draw(const swr_draw_desc *d) {
         fetch_vertex(d, idx, &V[k]);
         vertex_routine(&c, &V[k]);
       }
-      /* setupSolidTriangles (Renderer.cpp:733-776); cullMask is always 15 here. */
-      if ((V[0].clipFlags & V[1].clipFlags & V[2].clipFlags) != CLIP_FINITE) continue;
       Polygon poly;
-      poly.P[0][0] = V[0].pos; poly.P[0][1] = V[1].pos; poly.P[0][2] = V[2].pos;
-      poly.n = 3; poly.i = 0; poly.b = 0;
-      int flagsOr = V[0].clipFlags | V[1].clipFlags | V[2].clipFlags;
-      if (flagsOr != CLIP_FINITE && !clipper_clip(&poly, flagsOr)) continue;
-      const OVertex *tri[3] = {&V[0], &V[1], &V[2]};
+      float L[4][4];
+      if (c.line) {
+        if (!setup_line_polygon(&c, &V[0], &V[1], L, &poly)) continue;
+      } else {
+        /* setupSolidTriangles (Renderer.cpp:733-776); cullMask is always 15 here. */
+        if ((V[0].clipFlags & V[1].clipFlags & V[2].clipFlags) != CLIP_FINITE) continue;
+        poly.P[0][0] = V[0].pos; poly.P[0][1] = V[1].pos; poly.P[0][2] = V[2].pos;
+        poly.n = 3; poly.i = 0; poly.b = 0;
+        int flagsOr = V[0].clipFlags | V[1].clipFlags | V[2].clipFlags;
+        if (flagsOr != CLIP_FINITE && !clipper_clip(&poly, flagsOr)) continue;
+      }
+      /* A line's third setup vertex is its second (SetupRoutine.cpp:52-54). */
+      const OVertex *tri[3] = {&V[0], &V[1], c.line ? &V[1] : &V[2]};
       OPrim prim;
       memset(&prim, 0, sizeof(prim));
       if (!setup_routine(&c, &prim, tri, &poly)) continue;

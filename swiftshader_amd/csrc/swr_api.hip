@@ -218,14 +218,36 @@ int build_params(const swr_draw_desc *d, const swr_tuning &tuning, DrawParams &P
       (uint64_t)ra.y + ra.height > d->framebuffer_height)
     return fail(SWR_ERROR_INVALID_ARGUMENT, "render area outside the framebuffer");
   if (d->triangle_count > (1u << 26)) return fail(SWR_ERROR_UNSUPPORTED, "too many triangles (%u)", d->triangle_count);
-  if (d->topology != 0 && d->topology != SWR_TOPOLOGY_TRIANGLE_LIST && d->topology != SWR_TOPOLOGY_TRIANGLE_STRIP &&
-      d->topology != SWR_TOPOLOGY_TRIANGLE_FAN)
-    return fail(SWR_ERROR_UNSUPPORTED, "topology %u (triangle list, strip and fan are supported)", d->topology);
+  switch (d->topology) {
+    case 0:
+    case SWR_TOPOLOGY_TRIANGLE_LIST: P.topology = 0; break;
+    case SWR_TOPOLOGY_TRIANGLE_STRIP: P.topology = 1; break;
+    case SWR_TOPOLOGY_TRIANGLE_FAN: P.topology = 2; break;
+    case SWR_TOPOLOGY_LINE_LIST: P.topology = 3; break;
+    case SWR_TOPOLOGY_LINE_STRIP: P.topology = 4; break;
+    default: return fail(SWR_ERROR_UNSUPPORTED, "topology %u (line and triangle lists and strips, fans)", d->topology);
+  }
   if (d->provoking_vertex > 1) return fail(SWR_ERROR_INVALID_ARGUMENT, "provoking_vertex %u", d->provoking_vertex);
-  P.topology = d->topology == SWR_TOPOLOGY_TRIANGLE_STRIP ? 1 : (d->topology == SWR_TOPOLOGY_TRIANGLE_FAN ? 2 : 0);
   P.provokeLast = d->provoking_vertex == SWR_PROVOKING_VERTEX_LAST;
+  P.line = P.topology >= 3;
   {
-    const uint64_t need = P.topology == 0 ? (uint64_t)d->triangle_count * 3 : (uint64_t)d->triangle_count + 2;
+    // Lines: width 0 reads as 1; the rectangle or, for the Bresenham mode
+    // without samples, the parallelogram (Renderer.cpp:920-1163).
+    P.lineWidth = d->line_width == 0.0f ? 1.0f : d->line_width;
+    if (!(P.lineWidth > 0.0f) || P.lineWidth > 64.0f)
+      return fail(SWR_ERROR_INVALID_ARGUMENT, "line_width %g outside (0, 64]", (double)d->line_width);
+    if (d->line_rasterization_mode > SWR_LINE_RASTERIZATION_RECTANGULAR_SMOOTH)
+      return fail(SWR_ERROR_INVALID_ARGUMENT, "line_rasterization_mode %u", d->line_rasterization_mode);
+    P.bresenham = P.line && d->line_rasterization_mode == SWR_LINE_RASTERIZATION_BRESENHAM;
+    if (P.bresenham && d->samples != 1)
+      return fail(SWR_ERROR_UNSUPPORTED, "Bresenham lines with samples (the reference turns multisampling off)");
+    if (P.line && d->depth_bias_enable &&
+        (d->depth_bias_constant_factor != 0.0f || d->depth_bias_slope_factor != 0.0f))
+      return fail(SWR_ERROR_UNSUPPORTED, "depth bias of lines (the reference applies it to triangles only)");
+  }
+  {
+    const uint64_t T = d->triangle_count;
+    const uint64_t need = P.topology == 0 ? T * 3 : P.topology == 3 ? T * 2 : P.topology == 4 ? T + 1 : T + 2;
     if (!d->indices && d->triangle_count && need > d->vertex_count)
       return fail(SWR_ERROR_INVALID_ARGUMENT, "non-indexed draw needs %llu vertices", (unsigned long long)need);
     if (d->triangle_count && d->vertex_count == 0) return fail(SWR_ERROR_INVALID_ARGUMENT, "vertex_count is 0");
@@ -742,6 +764,7 @@ int grad_params(const swr_draw_desc *desc, const DrawParams &P, const swr_grad_d
   G = GradParams{};
   if (!g) return fail(SWR_ERROR_INVALID_ARGUMENT, "grad descriptor is NULL");
   if (!P.primId) return fail(SWR_ERROR_INVALID_ARGUMENT, "backward needs the draw's primitive_id attachment");
+  if (P.line) return fail(SWR_ERROR_UNSUPPORTED, "backward of lines");
   if (P.blendEnable) return fail(SWR_ERROR_UNSUPPORTED, "backward of a blended draw");
   if (!g->d_color) return fail(SWR_ERROR_INVALID_ARGUMENT, "d_color is NULL");
   if (g->d_color_row_pitch_bytes < desc->framebuffer_width * 16 ||

@@ -103,7 +103,14 @@ __device__ __forceinline__ int wsub(int a, int b) { return (int)((unsigned)a - (
 __device__ __forceinline__ void tri_vertices(const DrawParams &P, uint32_t t, uint32_t vi[3]) {
   const uint32_t f = P.provokeLast ? 0u : 1u;  // 1: first-vertex mode
   uint32_t p0, p1, p2;
-  if (P.topology == 1) {
+  if (P.topology >= 3) {
+    // A line's vertices, exchanged in last-vertex mode; the setup's third
+    // vertex is the second (SetupRoutine.cpp:52-54).
+    const uint32_t b = P.topology == 3 ? 2u * t : t;
+    p0 = b + (f ? 0u : 1u);
+    p1 = b + f;
+    p2 = p1;
+  } else if (P.topology == 1) {
     p0 = t + (f ? 0u : 2u);
     p1 = t + (t & 1u) + f;
     p2 = t + (~t & 1u) + f;
@@ -122,6 +129,56 @@ __device__ __forceinline__ void tri_vertices(const DrawParams &P, uint32_t t, ui
   vi[2] = P.idx ? P.idx[p2] : p2;
 #pragma unroll
   for (int k = 0; k < 3; k++) vi[k] = vi[k] < (uint32_t)P.vertexCount ? vi[k] : (uint32_t)P.vertexCount - 1u;
+}
+
+// DrawCall::setupLine (Renderer.cpp:920-1163): the rectangle of width
+// P.lineWidth centred on the segment P0-P1, or, in the Bresenham mode, the
+// parallelogram joining P0's and P1's diamonds of half-width lineWidth / 2
+// along the major axis, in clip space.  Returns false for a rejected line.
+__device__ __forceinline__ bool line_polygon(const DrawParams &P, const float (&P0)[4], const float (&P1)[4],
+                                             ClipVtx (&cp)[kMaxPolygon]) {
+  if (P0[3] <= 0.0f && P1[3] <= 0.0f) return false;
+  const float W = P.WxF * (1.0f / 16.0f);
+  const float H = P.HxF * (1.0f / 16.0f);
+  float dx = W * (P1[0] / P1[3] - P0[0] / P0[3]);
+  float dy = H * (P1[1] / P1[3] - P0[1] / P0[3]);
+  if (dx == 0.0f && dy == 0.0f) return false;
+  if (P.bresenham) {
+    const float dx0 = P.lineWidth * 0.5f * P0[3] / W, dy0 = P.lineWidth * 0.5f * P0[3] / H;
+    const float dx1 = P.lineWidth * 0.5f * P1[3] / W, dy1 = P.lineWidth * 0.5f * P1[3] / H;
+    // Corners kept per major direction: right and down run P0, P1, P1, P0
+    // along one diamond axis; up and left run P0, P0, P1, P1.
+    const bool alongFirst = dx > -dy;
+    const bool xMajor = dx > dy;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const bool first = alongFirst ? (k == 0 || k == 3) : (k < 2);
+      for (int j = 0; j < 4; j++) cp[k].v[j] = first ? P0[j] : P1[j];
+    }
+    if (alongFirst && xMajor) {  // right: up and down corners
+      cp[0].v[1] += +dy0; cp[1].v[1] += +dy1; cp[2].v[1] += -dy1; cp[3].v[1] += -dy0;
+    } else if (alongFirst) {     // down: left and right corners
+      cp[0].v[0] += -dx0; cp[1].v[0] += -dx1; cp[2].v[0] += +dx1; cp[3].v[0] += +dx0;
+    } else if (xMajor) {         // up: left and right corners
+      cp[0].v[0] += -dx0; cp[1].v[0] += +dx0; cp[2].v[0] += +dx1; cp[3].v[0] += -dx1;
+    } else {                     // left: up and down corners
+      cp[0].v[1] += +dy0; cp[1].v[1] += -dy0; cp[2].v[1] += -dy1; cp[3].v[1] += +dy1;
+    }
+  } else {
+    const float scale = P.lineWidth * 0.5f / sqrtf(dx * dx + dy * dy);
+    dx = dx * scale;
+    dy = dy * scale;
+    const float dx0h = dx * P0[3] / H, dy0w = dy * P0[3] / W;
+    const float dx1h = dx * P1[3] / H, dy1w = dy * P1[3] / W;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      for (int j = 0; j < 4; j++) cp[k].v[j] = (k == 0 || k == 3) ? P0[j] : P1[j];
+    cp[0].v[0] += -dy0w; cp[0].v[1] += +dx0h;
+    cp[1].v[0] += -dy1w; cp[1].v[1] += +dx1h;
+    cp[2].v[0] += +dy1w; cp[2].v[1] += -dx1h;
+    cp[3].v[0] += +dy0w; cp[3].v[1] += -dx0h;
+  }
+  return true;
 }
 
 // Setup of primitive t of the draw P, written to record slot g (the
@@ -176,24 +233,30 @@ __device__ __forceinline__ void setup_one(const DrawParams &P, int t, size_t g, 
   tileCount[g] = 0;
   R.nflags = 0;
   R.draw = draw;
-  // setupSolidTriangles (Renderer.cpp:733-776).
-  if ((flags[0] & flags[1] & flags[2]) != CLIP_FINITE) return;
-  int flagsOr = flags[0] | flags[1] | flags[2];
   ClipVtx cp[kMaxPolygon];
   int n = 3;
   int stages = 0;
-  if (flagsOr != CLIP_FINITE) {
-#pragma unroll
-    for (int k = 0; k < 3; k++)
-      for (int c = 0; c < 4; c++) cp[k].v[c] = pos[k][c];
-    stages = clip_polygon(cp, n, flagsOr);
-    if (n < 3) return;
-  }
-
-  // Culling and facing (SetupRoutine.cpp:71-120).
   int d = 1;
-  int frontFacing;
-  {
+  int frontFacing = 1;
+  if (P.line) {
+    // setupLine: the line's polygon, clipped against every frustum plane;
+    // no culling, front-facing, d = 1 (SetupRoutine.cpp:115-119).
+    if (!line_polygon(P, pos[0], pos[1], cp)) return;
+    n = 4;
+    stages = clip_polygon(cp, n, CLIP_FRUSTUM);
+    if (n < 3) return;
+  } else {
+    // setupSolidTriangles (Renderer.cpp:733-776).
+    if ((flags[0] & flags[1] & flags[2]) != CLIP_FINITE) return;
+    int flagsOr = flags[0] | flags[1] | flags[2];
+    if (flagsOr != CLIP_FINITE) {
+#pragma unroll
+      for (int k = 0; k < 3; k++)
+        for (int c = 0; c < 4; c++) cp[k].v[c] = pos[k][c];
+      stages = clip_polygon(cp, n, flagsOr);
+      if (n < 3) return;
+    }
+    // Culling and facing (SetupRoutine.cpp:71-120).
     float x0 = (float)PX[0], x1 = (float)PX[1], x2 = (float)PX[2];
     float y0 = (float)PY[0], y1 = (float)PY[1], y2 = (float)PY[2];
     float A = (y0 - y2) * x1 + (y2 - y1) * x0 + (y1 - y0) * x2;
@@ -205,9 +268,10 @@ __device__ __forceinline__ void setup_one(const DrawParams &P, int t, size_t g, 
     d = (A > 0.0f) ? 1 : 0;
   }
 
-  // Polygon in 28.4 (reprojected when clipped, SetupRoutine.cpp:122-143).
+  // Polygon in 28.4 (reprojected when clipped or a line,
+  // SetupRoutine.cpp:122-143).
   int X[kMaxPolygon], Y[kMaxPolygon];
-  if (stages != 0) {
+  if (stages != 0 || P.line) {
     for (int i = 0; i < n; i++) {
       float w = cp[i].v[3];
       float rhw = (w != 0.0f) ? 1.0f / w : 1.0f;
@@ -239,15 +303,16 @@ __device__ __forceinline__ void setup_one(const DrawParams &P, int t, size_t g, 
   int xMax = min(wadd(wadd(xHi, sxo), 15) >> 4, P.sx1);
   if (xMin >= xMax) return;
 
-  // Sort by minimum y, then by maximum w (SetupRoutine.cpp:270-294).
+  // Sort by minimum y, then by maximum w (SetupRoutine.cpp:270-294),
+  // triangles only.
   int o0 = 0, o1 = 1, o2 = 2;
-  {
+  if (!P.line) {
     float y0 = pos[0][1], y1 = pos[1][1], y2 = pos[2][1];
     float ym = rr_min(rr_min(y0, y1), y2);
     if (ym == y1) { int tt = o0; o0 = o1; o1 = o2; o2 = tt; }
     if (ym == y2) { int tt = o2; o2 = o1; o1 = o0; o0 = tt; }
   }
-  {
+  if (!P.line) {
     float w0 = pos[o0][3], w1 = pos[o1][3], w2 = pos[o2][3];
     float wm = rr_max(rr_max(w0, w1), w2);
     if (wm == w1) { int tt = o0; o0 = o1; o1 = o2; o2 = tt; }
@@ -258,6 +323,10 @@ __device__ __forceinline__ void setup_one(const DrawParams &P, int t, size_t g, 
   float rhw0 = pw[o0];
   int X0 = PX[o0], X1 = PX[o1], X2 = PX[o2];
   int Y0 = PY[o0], Y1 = PY[o1], Y2 = PY[o2];
+  if (P.line) {  // a third point across the line (SetupRoutine.cpp:317-321)
+    X2 = wsub(wadd(X1, Y1), Y0);
+    Y2 = wsub(wadd(Y1, X0), X1);
+  }
   R.x0 = (float)X0 * (1.0f / 16.0f);
   R.y0 = (float)Y0 * (1.0f / 16.0f);
   X1 = wsub(X1, X0); Y1 = wsub(Y1, Y0);

@@ -51,7 +51,9 @@ struct DrawParams {
   const uint32_t *idx;
   int triCount;
   int vertexCount;
-  int topology;     // 0 list, 1 strip, 2 fan
+  int topology;     // 0 triangle list, 1 triangle strip, 2 fan, 3 line list, 4 line strip
+  int line, bresenham;  // lines; set up as Bresenham parallelograms
+  float lineWidth;
   int provokeLast;  // VK_PROVOKING_VERTEX_MODE_LAST_VERTEX
   // Viewport / scissor (Renderer.cpp:300-345).
   float WxF, HxF, X0xF, Y0xF, depthRange, depthNear;

@@ -114,4 +114,6 @@ def build_desc(draw: DrawData, buf: Buffers) -> abi.DrawDesc:
     if draw.depth_bias is not None:
         d.depth_bias_enable = 1
         (d.depth_bias_constant_factor, d.depth_bias_clamp, d.depth_bias_slope_factor) = draw.depth_bias
+    d.line_width = draw.line_width
+    d.line_rasterization_mode = draw.line_rasterization_mode
     return d

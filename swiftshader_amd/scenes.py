@@ -83,12 +83,19 @@ class DrawData:
     topology: int = abi.TOPOLOGY_TRIANGLE_LIST
     provoking_vertex: int = abi.PROVOKING_VERTEX_FIRST
     depth_bias: Optional[tuple] = None  # (constant_factor, clamp, slope_factor); None = disabled
+    line_width: float = 1.0
+    line_rasterization_mode: int = abi.LINE_RASTERIZATION_DEFAULT
 
     @property
     def triangle_count(self):
+        """Primitives in the draw (lines for the line topologies)."""
         n = len(self.indices) if self.indices is not None else len(self.positions)
         if self.topology in (abi.TOPOLOGY_TRIANGLE_STRIP, abi.TOPOLOGY_TRIANGLE_FAN):
             return max(n - 2, 0)
+        if self.topology == abi.TOPOLOGY_LINE_STRIP:
+            return max(n - 1, 0)
+        if self.topology == abi.TOPOLOGY_LINE_LIST:
+            return n // 2
         return n // 3
 
 

@@ -198,6 +198,59 @@ def _depth_bias(seed=33, bias=(800.0, 0.0, 1.5), samples=1, size=88, tris=700):
                     shader=abi.SHADER_INTERPOLATE, depth_bias=bias, depth_compare=abi.COMPARE_LESS_OR_EQUAL)
 
 
+def _lines(seed=36, size=(88, 70), n=300, topology=abi.TOPOLOGY_LINE_LIST, width=1.0,
+           mode=abi.LINE_RASTERIZATION_DEFAULT, samples=1, provoking=abi.PROVOKING_VERTEX_FIRST, clip=False,
+           textured=False, float_target=False, stencil=False):
+    """Random line segments (a list, or one strip through random points); a
+    list's first quarter is axis-aligned on the pixel grid (edges through
+    pixel centres and between them)."""
+    rng = np.random.default_rng(seed)
+    W, H = size
+    m = 2 * n if topology == abi.TOPOLOGY_LINE_LIST else n + 1
+    if clip:
+        xy = rng.uniform(-2.0, 2.0, (m, 2))
+        w = rng.uniform(-0.4, 2.5, (m, 1))
+        z = rng.uniform(-0.6, 1.5, (m, 1))
+    else:
+        xy = rng.uniform(-1.1, 1.1, (m, 2))
+        w = rng.uniform(0.6, 2.0, (m, 1))
+        z = rng.uniform(0.05, 0.95, (m, 1))
+    if topology == abi.TOPOLOGY_LINE_LIST and not clip:
+        k = n // 4
+        px = rng.integers(0, 2 * W + 1, (k, 2)) * 0.5  # pixel coordinates on a half-pixel grid
+        py = rng.integers(0, 2 * H + 1, (k, 2)) * 0.5
+        vert = rng.uniform(size=k) < 0.5
+        py[vert, 1] = rng.integers(0, 2 * H + 1, vert.sum()) * 0.5
+        px[vert, 1] = px[vert, 0]
+        py[~vert, 1] = py[~vert, 0]
+        seg = np.stack([px * 2.0 / W - 1.0, py * 2.0 / H - 1.0], -1).reshape(-1, 2)  # (2k, 2) in NDC
+        xy[: 2 * k] = seg
+        w[: 2 * k] = 1.0
+    pos = np.concatenate([xy * np.abs(w), z * np.abs(w), w], axis=1).astype(np.float32)
+    col = rng.uniform(0, 1, (m, 4)).astype(np.float32)
+    uv = rng.uniform(-1, 3, (m, 2)).astype(np.float32)
+    tex = None
+    shader = abi.SHADER_INTERPOLATE
+    if textured:
+        tex = TextureData(format=abi.FORMAT_R8G8B8A8_UNORM,
+                          levels=scenes.box_mip_chain(scenes.noise_texture_rgba8(rng, 64)))
+        shader = abi.SHADER_MODULATE
+    extra = {}
+    if stencil:
+        extra = dict(has_stencil=True, stencil_test=1, clear_stencil=1,
+                     front=dict(fail_op=abi.STENCIL_OP_KEEP, pass_op=abi.STENCIL_OP_INCREMENT_AND_CLAMP,
+                                depth_fail_op=abi.STENCIL_OP_INVERT, compare_op=abi.COMPARE_ALWAYS,
+                                compare_mask=0xFF, write_mask=0xFF, reference=0),
+                     back=dict(fail_op=abi.STENCIL_OP_ZERO, pass_op=abi.STENCIL_OP_ZERO,
+                               depth_fail_op=abi.STENCIL_OP_ZERO, compare_op=abi.COMPARE_NEVER,
+                               compare_mask=0xFF, write_mask=0xFF, reference=0))
+    return DrawData(width=W, height=H, positions=pos, colors=col, texcoords=uv, samples=samples, shader=shader,
+                    texture=tex, topology=topology, line_width=width, line_rasterization_mode=mode,
+                    provoking_vertex=provoking, depth_compare=abi.COMPARE_LESS_OR_EQUAL,
+                    color_format=abi.FORMAT_R32G32B32A32_SFLOAT if float_target else abi.FORMAT_R8G8B8A8_UNORM,
+                    **extra)
+
+
 def parity_cases():
     """(name, DrawData) pairs; every one must match the oracle bit for bit."""
     return [
@@ -230,4 +283,12 @@ def parity_cases():
         ("depth_bias_constant_slope", _depth_bias()),
         ("depth_bias_clamp_negative_msaa4", _depth_bias(seed=34, bias=(-3000.0, -0.002, -2.0), samples=4)),
         ("depth_bias_clamp_positive", _depth_bias(seed=35, bias=(1e6, 0.004, 0.0))),
+        ("lines_list_gouraud", _lines()),
+        ("lines_strip_wide_msaa4_textured", _lines(seed=37, topology=abi.TOPOLOGY_LINE_STRIP, n=120, width=3.5,
+                                                   samples=4, textured=True, provoking=abi.PROVOKING_VERTEX_LAST)),
+        ("lines_bresenham_float", _lines(seed=38, mode=abi.LINE_RASTERIZATION_BRESENHAM, float_target=True)),
+        ("lines_bresenham_wide_strip", _lines(seed=39, topology=abi.TOPOLOGY_LINE_STRIP, n=150, width=2.5,
+                                              mode=abi.LINE_RASTERIZATION_BRESENHAM, textured=True)),
+        ("lines_clipped_last_msaa4_stencil", _lines(seed=40, clip=True, width=1.5, samples=4, stencil=True,
+                                                    provoking=abi.PROVOKING_VERTEX_LAST)),
     ]

@@ -2,9 +2,9 @@
 
 Every case renders the same seeded scene through libswr.so (C ABI) on the
 GPU and through the oracle on the CPU and requires identical colour,
-depth, stencil bytes and occlusion count (the tolerances north of exact
-that the brief allows -- depth 2^-22 relative, UNORM8 1 LSB -- are not
-needed: the kernels reproduce the reference's arithmetic exactly).
+depth, stencil bytes and occlusion count (looser tolerances -- depth
+2^-22 relative, UNORM8 1 LSB -- are not needed: the kernels reproduce the
+reference's arithmetic exactly).
 """
 import sys
 
