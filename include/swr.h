@@ -46,7 +46,7 @@
 extern "C" {
 #endif
 
-#define SWR_ABI_VERSION 4
+#define SWR_ABI_VERSION 5
 
 /* ---- status codes -------------------------------------------------------- */
 #define SWR_OK 0
@@ -104,6 +104,7 @@ extern "C" {
 #define SWR_BLEND_OP_MIN 3
 #define SWR_BLEND_OP_MAX 4
 /* VkPrimitiveTopology / VkProvokingVertexModeEXT */
+#define SWR_TOPOLOGY_POINT_LIST 0
 #define SWR_TOPOLOGY_LINE_LIST 1
 #define SWR_TOPOLOGY_LINE_STRIP 2
 #define SWR_TOPOLOGY_TRIANGLE_LIST 3
@@ -239,14 +240,14 @@ typedef struct swr_draw_desc {
   swr_attachment primitive_id;
 
   /* Primitive assembly (setBatchIndices, src/Device/Renderer.cpp:50-145):
-   * the VkPrimitiveTopology -- LINE_LIST (1), LINE_STRIP (2), TRIANGLE_LIST
-   * (3; 0 is read as a list too), TRIANGLE_STRIP (4) or TRIANGLE_FAN (5) --
-   * over the index buffer (or the vertex sequence when indices is NULL), and
-   * the VkProvokingVertexModeEXT (0 first, 1 last), which rotates each
-   * primitive's vertices.  triangle_count counts primitives (lines for the
-   * line topologies): a triangle list needs 3 * triangle_count indices, a
-   * line list 2 * triangle_count, a strip or fan triangle_count + 2, a line
-   * strip triangle_count + 1. */
+   * the VkPrimitiveTopology -- POINT_LIST (0), LINE_LIST (1), LINE_STRIP
+   * (2), TRIANGLE_LIST (3), TRIANGLE_STRIP (4) or TRIANGLE_FAN (5) -- over
+   * the index buffer (or the vertex sequence when indices is NULL), and the
+   * VkProvokingVertexModeEXT (0 first, 1 last), which rotates each
+   * primitive's vertices.  triangle_count counts primitives (points or lines
+   * for those topologies): a triangle list needs 3 * triangle_count indices,
+   * a line list 2 * triangle_count, a strip or fan triangle_count + 2, a
+   * line strip triangle_count + 1, a point list triangle_count. */
   uint32_t topology;
   uint32_t provoking_vertex;
 
@@ -275,6 +276,13 @@ typedef struct swr_draw_desc {
    * applies to triangles only and is refused for lines. */
   float line_width;
   uint32_t line_rasterization_mode;
+
+  /* Points (DrawCall::setupPoint, Renderer.cpp:1166-1210): each point is
+   * the square of side PointSize -- the vertex's point_sizes entry (the
+   * vertex stage's PointSize output), 1 when point_sizes is NULL, clamped
+   * to [1, 1023] -- centred on the vertex, clipped and rasterised as a
+   * polygon with a constant depth.  Depth bias is refused for points. */
+  const float *point_sizes; /* one per vertex, optional */
 } swr_draw_desc;
 
 typedef struct swr_resolve_desc { /* 4x -> 1x colour resolve */

@@ -184,6 +184,7 @@ typedef struct {
   int X, Y;
   float pz, pw;
   float v[NATTR];
+  float pointSize;
 } OVertex;
 
 typedef struct { float A, B, C; } Plane;
@@ -220,6 +221,7 @@ typedef struct {
   int depthTestActive, depthWriteEnable, depthCompare;
   int line;           /* LINE_LIST / LINE_STRIP: primitives are lines */
   int bresenham;      /* lines set up as Bresenham parallelograms */
+  int point;          /* POINT_LIST: primitives are points */
   float lineWidth;
   /* depth bias (SetupProcessor.cpp:75-77, PixelProcessor.cpp:119) */
   int biasConstant, biasSlope, biasClamp, depthBias;
@@ -396,12 +398,12 @@ static void setup_edge(Ctx *c, int q, int Xa, int Ya, int Xb, int Yb) {
 
 /* Returns 1 if the primitive is visible. */
 static int setup_routine(Ctx *c, OPrim *prim, const OVertex *tri[3], const Polygon *poly) {
-  const int line = c->line;
+  const int line = c->line, point = c->point;
   int X[17], Y[17];
   X[0] = tri[0]->X; X[1] = tri[1]->X; X[2] = tri[2]->X;
   Y[0] = tri[0]->Y; Y[1] = tri[1]->Y; Y[2] = tri[2]->Y;
   int d = 1;
-  if (line) {
+  if (line || point) {
     prim->frontFacing = 1; /* not a triangle: front-facing masks, d = 1 */
   } else {
     float x0 = (float)X[0], x1 = (float)X[1], x2 = (float)X[2];
@@ -417,7 +419,7 @@ static int setup_routine(Ctx *c, OPrim *prim, const OVertex *tri[3], const Polyg
   }
   int n = poly->n;
   int m = poly->i;
-  if (m != 0 || line) { /* clipped, or not a triangle: reproject the polygon */
+  if (m != 0 || line || point) { /* clipped, or not a triangle: reproject the polygon */
     for (int i = 0; i < n; i++) {
       const float *v = poly->P[m][i];
       float w = v[3];
@@ -468,14 +470,14 @@ static int setup_routine(Ctx *c, OPrim *prim, const OVertex *tri[3], const Polyg
 
   /* Sort by minimum y (clip-space y), then by maximum w (triangles only). */
   const OVertex *v0 = tri[0], *v1 = tri[1], *v2 = tri[2];
-  if (!line) {
+  if (!line && !point) {
     float y0 = v0->pos[1], y1 = v1->pos[1], y2 = v2->pos[1];
     float ym = rr_min(rr_min(y0, y1), y2);
     const OVertex *vX;
     if (ym == y1) { vX = v0; v0 = v1; v1 = v2; v2 = vX; }
     if (ym == y2) { vX = v2; v2 = v1; v1 = v0; v0 = vX; }
   }
-  if (!line) {
+  if (!line && !point) {
     float w0 = v0->pos[3], w1 = v1->pos[3], w2 = v2->pos[3];
     float wm = rr_max(rr_max(w0, w1), w2);
     const OVertex *vX;
@@ -525,9 +527,14 @@ static int setup_routine(Ctx *c, OPrim *prim, const OVertex *tri[3], const Polyg
     float fy1 = (float)Y1 * (1.0f / 16.0f);
     float fx2 = (float)X2 * (1.0f / 16.0f);
     float fy2 = (float)Y2 * (1.0f / 16.0f);
-    float D = c->depthRange / (fx1 * fy2 - fx2 * fy1);
-    prim->z.A = (fy2 * z1 - fy1 * z2) * D;
-    prim->z.B = (fx1 * z2 - fx2 * z1) * D;
+    if (!point) {
+      float D = c->depthRange / (fx1 * fy2 - fx2 * fy1);
+      prim->z.A = (fy2 * z1 - fy1 * z2) * D;
+      prim->z.B = (fx1 * z2 - fx2 * z1) * D;
+    } else { /* a point's depth is constant (SetupRoutine.cpp:393-409) */
+      prim->z.A = 0.0f;
+      prim->z.B = 0.0f;
+    }
     prim->z.C = z0 * c->depthRange + c->depthNear;
     /* Depth bias, floating-point depth buffer (SetupRoutine.cpp:417-475).
      * Z1, Z2 are formed from z1, z2 after they became differences from z0
@@ -1204,11 +1211,9 @@ static int init_ctx(Ctx *c, const swr_draw_desc *d) {
   c->d = d;
   init_constants();
   if (d->samples != 1 && d->samples != 4) return SWR_ERROR_UNSUPPORTED;
-  if (d->topology != 0 && d->topology != SWR_TOPOLOGY_TRIANGLE_LIST && d->topology != SWR_TOPOLOGY_TRIANGLE_STRIP &&
-      d->topology != SWR_TOPOLOGY_TRIANGLE_FAN && d->topology != SWR_TOPOLOGY_LINE_LIST &&
-      d->topology != SWR_TOPOLOGY_LINE_STRIP)
-    return SWR_ERROR_UNSUPPORTED;
+  if (d->topology > SWR_TOPOLOGY_TRIANGLE_FAN) return SWR_ERROR_UNSUPPORTED;
   c->line = d->topology == SWR_TOPOLOGY_LINE_LIST || d->topology == SWR_TOPOLOGY_LINE_STRIP;
+  c->point = d->topology == SWR_TOPOLOGY_POINT_LIST;
   c->lineWidth = d->line_width == 0.0f ? 1.0f : d->line_width;
   if (!(c->lineWidth > 0.0f) || c->lineWidth > 64.0f) return SWR_ERROR_INVALID_ARGUMENT;
   if (d->line_rasterization_mode > SWR_LINE_RASTERIZATION_RECTANGULAR_SMOOTH) return SWR_ERROR_INVALID_ARGUMENT;
@@ -1216,9 +1221,9 @@ static int init_ctx(Ctx *c, const swr_draw_desc *d) {
   /* The reference sets up a Bresenham line without multisampling
    * (SetupProcessor.cpp:83-86); that mode with samples is not built. */
   if (c->bresenham && d->samples != 1) return SWR_ERROR_UNSUPPORTED;
-  /* The setup writes the depth bias of triangles only; for lines the
-   * reference's pixel routine would add an unwritten field. */
-  if (c->line && d->depth_bias_enable &&
+  /* The setup writes the depth bias of triangles only; for lines and
+   * points the reference's pixel routine would add an unwritten field. */
+  if ((c->line || c->point) && d->depth_bias_enable &&
       (d->depth_bias_constant_factor != 0.0f || d->depth_bias_slope_factor != 0.0f))
     return SWR_ERROR_UNSUPPORTED;
   if (d->provoking_vertex > 1) return SWR_ERROR_INVALID_ARGUMENT;
@@ -1337,6 +1342,7 @@ static void fetch_vertex(const swr_draw_desc *d, uint32_t idx, OVertex *v) {
   for (int k = 0; k < NATTR; k++) v->v[k] = 0.0f;
   if (d->colors) memcpy(v->v, d->colors + 4 * (size_t)idx, 16);
   if (d->texcoords) memcpy(v->v + 4, d->texcoords + 2 * (size_t)idx, 8);
+  v->pointSize = d->point_sizes ? d->point_sizes[idx] : 1.0f;
 }
 
 /* Vertex indices of triangle t in the reference's batch order
@@ -1345,7 +1351,9 @@ static void fetch_vertex(const swr_draw_desc *d, uint32_t idx, OVertex *v) {
 static void assemble(const swr_draw_desc *d, uint32_t t, uint32_t out[3]) {
   const int first = d->provoking_vertex == SWR_PROVOKING_VERTEX_FIRST;
   uint32_t p[3];
-  if (d->topology == SWR_TOPOLOGY_LINE_LIST || d->topology == SWR_TOPOLOGY_LINE_STRIP) {
+  if (d->topology == SWR_TOPOLOGY_POINT_LIST) {
+    p[0] = p[1] = p[2] = t;
+  } else if (d->topology == SWR_TOPOLOGY_LINE_LIST || d->topology == SWR_TOPOLOGY_LINE_STRIP) {
     /* (v0, v1, v1), v0 and v1 exchanged in last-vertex mode */
     const uint32_t b = d->topology == SWR_TOPOLOGY_LINE_LIST ? 2u * t : t;
     p[0] = b + (first ? 0u : 1u);
@@ -1422,6 +1430,31 @@ static int setup_line_polygon(const Ctx *c, const OVertex *v0, const OVertex *v1
   return clipper_clip(poly, CLIP_FRUSTUM);
 }
 
+/* DrawCall::setupPoint (Renderer.cpp:1166-1210): the square of side
+ * PointSize (clamped to [1, 1023] by two compares, so a NaN passes; Math.hpp
+ * clamp) centred on v, offsets pSize * w * halfPixel with halfPixel
+ * = 0.5 / (viewport extent / 2), clipped against every frustum plane. */
+static int setup_point_polygon(const Ctx *c, const OVertex *v, float L[4][4], Polygon *poly) {
+  float pSize = v->pointSize;
+  if (pSize < 1.0f) pSize = 1.0f;
+  if (pSize > 1023.0f) pSize = 1023.0f;
+  const float halfPixelX = 0.5f / (c->WxF * (1.0f / 16.0f));
+  const float halfPixelY = 0.5f / (c->HxF * (1.0f / 16.0f));
+  const float X = pSize * v->pos[3] * halfPixelX;
+  const float Y = pSize * v->pos[3] * halfPixelY;
+  for (int k = 0; k < 4; k++)
+    for (int j = 0; j < 4; j++) L[k][j] = v->pos[j];
+  L[0][0] -= X; L[0][1] += Y;
+  L[1][0] += X; L[1][1] += Y;
+  L[2][0] += X; L[2][1] -= Y;
+  L[3][0] -= X; L[3][1] -= Y;
+  for (int k = 0; k < 4; k++) poly->P[0][k] = L[k];
+  poly->n = 4;
+  poly->i = 0;
+  poly->b = 0;
+  return clipper_clip(poly, CLIP_FRUSTUM);
+}
+
 /* Renders one draw with host pointers in the descriptor. */
 EXPORT int swo_draw(const swr_draw_desc *d) {
   if (!d || !d->positions) return SWR_ERROR_INVALID_ARGUMENT;
@@ -1441,7 +1474,9 @@ EXPORT int swo_draw(const swr_draw_desc *d) {
       }
       Polygon poly;
       float L[4][4];
-      if (c.line) {
+      if (c.point) {
+        if (!setup_point_polygon(&c, &V[0], L, &poly)) continue;
+      } else if (c.line) {
         if (!setup_line_polygon(&c, &V[0], &V[1], L, &poly)) continue;
       } else {
         /* setupSolidTriangles (Renderer.cpp:733-776); cullMask is always 15 here. */
@@ -1451,8 +1486,9 @@ EXPORT int swo_draw(const swr_draw_desc *d) {
         int flagsOr = V[0].clipFlags | V[1].clipFlags | V[2].clipFlags;
         if (flagsOr != CLIP_FINITE && !clipper_clip(&poly, flagsOr)) continue;
       }
-      /* A line's third setup vertex is its second (SetupRoutine.cpp:52-54). */
-      const OVertex *tri[3] = {&V[0], &V[1], c.line ? &V[1] : &V[2]};
+      /* A line's third setup vertex is its second, a point's every setup
+       * vertex its one (SetupRoutine.cpp:52-54). */
+      const OVertex *tri[3] = {&V[0], c.point ? &V[0] : &V[1], c.line ? &V[1] : (c.point ? &V[0] : &V[2])};
       OPrim prim;
       memset(&prim, 0, sizeof(prim));
       if (!setup_routine(&c, &prim, tri, &poly)) continue;

@@ -7,7 +7,7 @@ DrawData in the reference (src/Device/Renderer.hpp:58-113, 211-213).
 """
 import ctypes as C
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 OK = 0
 ERROR_INVALID_ARGUMENT = -1
@@ -40,7 +40,7 @@ BLEND_FACTOR_SRC_ALPHA_SATURATE = 14
 
 BLEND_OP_ADD, BLEND_OP_SUBTRACT, BLEND_OP_REVERSE_SUBTRACT, BLEND_OP_MIN, BLEND_OP_MAX = 0, 1, 2, 3, 4
 
-TOPOLOGY_LINE_LIST, TOPOLOGY_LINE_STRIP = 1, 2
+TOPOLOGY_POINT_LIST, TOPOLOGY_LINE_LIST, TOPOLOGY_LINE_STRIP = 0, 1, 2
 TOPOLOGY_TRIANGLE_LIST, TOPOLOGY_TRIANGLE_STRIP, TOPOLOGY_TRIANGLE_FAN = 3, 4, 5
 LINE_RASTERIZATION_DEFAULT, LINE_RASTERIZATION_RECTANGULAR = 0, 1
 LINE_RASTERIZATION_BRESENHAM, LINE_RASTERIZATION_RECTANGULAR_SMOOTH = 2, 3
@@ -117,6 +117,7 @@ class DrawDesc(C.Structure):
         ("depth_bias_enable", C.c_uint32), ("depth_bias_constant_factor", C.c_float),
         ("depth_bias_clamp", C.c_float), ("depth_bias_slope_factor", C.c_float),
         ("line_width", C.c_float), ("line_rasterization_mode", C.c_uint32),
+        ("point_sizes", C.c_void_p),
     ]
 
 

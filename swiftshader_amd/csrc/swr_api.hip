@@ -219,17 +219,19 @@ int build_params(const swr_draw_desc *d, const swr_tuning &tuning, DrawParams &P
     return fail(SWR_ERROR_INVALID_ARGUMENT, "render area outside the framebuffer");
   if (d->triangle_count > (1u << 26)) return fail(SWR_ERROR_UNSUPPORTED, "too many triangles (%u)", d->triangle_count);
   switch (d->topology) {
-    case 0:
+    case SWR_TOPOLOGY_POINT_LIST: P.topology = 5; break;
     case SWR_TOPOLOGY_TRIANGLE_LIST: P.topology = 0; break;
     case SWR_TOPOLOGY_TRIANGLE_STRIP: P.topology = 1; break;
     case SWR_TOPOLOGY_TRIANGLE_FAN: P.topology = 2; break;
     case SWR_TOPOLOGY_LINE_LIST: P.topology = 3; break;
     case SWR_TOPOLOGY_LINE_STRIP: P.topology = 4; break;
-    default: return fail(SWR_ERROR_UNSUPPORTED, "topology %u (line and triangle lists and strips, fans)", d->topology);
+    default: return fail(SWR_ERROR_UNSUPPORTED, "topology %u (points, line and triangle lists and strips, fans)", d->topology);
   }
   if (d->provoking_vertex > 1) return fail(SWR_ERROR_INVALID_ARGUMENT, "provoking_vertex %u", d->provoking_vertex);
   P.provokeLast = d->provoking_vertex == SWR_PROVOKING_VERTEX_LAST;
-  P.line = P.topology >= 3;
+  P.line = P.topology == 3 || P.topology == 4;
+  P.point = P.topology == 5;
+  P.psize = d->point_sizes;
   {
     // Lines: width 0 reads as 1; the rectangle or, for the Bresenham mode
     // without samples, the parallelogram (Renderer.cpp:920-1163).
@@ -241,13 +243,17 @@ int build_params(const swr_draw_desc *d, const swr_tuning &tuning, DrawParams &P
     P.bresenham = P.line && d->line_rasterization_mode == SWR_LINE_RASTERIZATION_BRESENHAM;
     if (P.bresenham && d->samples != 1)
       return fail(SWR_ERROR_UNSUPPORTED, "Bresenham lines with samples (the reference turns multisampling off)");
-    if (P.line && d->depth_bias_enable &&
+    if ((P.line || P.point) && d->depth_bias_enable &&
         (d->depth_bias_constant_factor != 0.0f || d->depth_bias_slope_factor != 0.0f))
-      return fail(SWR_ERROR_UNSUPPORTED, "depth bias of lines (the reference applies it to triangles only)");
+      return fail(SWR_ERROR_UNSUPPORTED, "depth bias of lines or points (the reference applies it to triangles only)");
   }
   {
     const uint64_t T = d->triangle_count;
-    const uint64_t need = P.topology == 0 ? T * 3 : P.topology == 3 ? T * 2 : P.topology == 4 ? T + 1 : T + 2;
+    const uint64_t need = P.topology == 0   ? T * 3
+                          : P.topology == 3 ? T * 2
+                          : P.topology == 4 ? T + 1
+                          : P.topology == 5 ? T
+                                            : T + 2;
     if (!d->indices && d->triangle_count && need > d->vertex_count)
       return fail(SWR_ERROR_INVALID_ARGUMENT, "non-indexed draw needs %llu vertices", (unsigned long long)need);
     if (d->triangle_count && d->vertex_count == 0) return fail(SWR_ERROR_INVALID_ARGUMENT, "vertex_count is 0");
@@ -764,7 +770,7 @@ int grad_params(const swr_draw_desc *desc, const DrawParams &P, const swr_grad_d
   G = GradParams{};
   if (!g) return fail(SWR_ERROR_INVALID_ARGUMENT, "grad descriptor is NULL");
   if (!P.primId) return fail(SWR_ERROR_INVALID_ARGUMENT, "backward needs the draw's primitive_id attachment");
-  if (P.line) return fail(SWR_ERROR_UNSUPPORTED, "backward of lines");
+  if (P.line || P.point) return fail(SWR_ERROR_UNSUPPORTED, "backward of lines and points");
   if (P.blendEnable) return fail(SWR_ERROR_UNSUPPORTED, "backward of a blended draw");
   if (!g->d_color) return fail(SWR_ERROR_INVALID_ARGUMENT, "d_color is NULL");
   if (g->d_color_row_pitch_bytes < desc->framebuffer_width * 16 ||

@@ -103,7 +103,9 @@ __device__ __forceinline__ int wsub(int a, int b) { return (int)((unsigned)a - (
 __device__ __forceinline__ void tri_vertices(const DrawParams &P, uint32_t t, uint32_t vi[3]) {
   const uint32_t f = P.provokeLast ? 0u : 1u;  // 1: first-vertex mode
   uint32_t p0, p1, p2;
-  if (P.topology >= 3) {
+  if (P.topology == 5) {
+    p0 = p1 = p2 = t;  // a point's every setup vertex is its one
+  } else if (P.topology >= 3) {
     // A line's vertices, exchanged in last-vertex mode; the setup's third
     // vertex is the second (SetupRoutine.cpp:52-54).
     const uint32_t b = P.topology == 3 ? 2u * t : t;
@@ -181,6 +183,28 @@ __device__ __forceinline__ bool line_polygon(const DrawParams &P, const float (&
   return true;
 }
 
+// DrawCall::setupPoint (Renderer.cpp:1166-1210): the square of side
+// PointSize (clamped to [1, 1023] by two compares, so a NaN passes) centred
+// on the vertex, offsets pSize * w * halfPixel, halfPixel = 0.5 / (viewport
+// extent / 2).
+__device__ __forceinline__ void point_polygon(const DrawParams &P, const float (&V)[4], float size,
+                                              ClipVtx (&cp)[kMaxPolygon]) {
+  float pSize = size;
+  if (pSize < 1.0f) pSize = 1.0f;
+  if (pSize > 1023.0f) pSize = 1023.0f;
+  const float halfPixelX = 0.5f / (P.WxF * (1.0f / 16.0f));
+  const float halfPixelY = 0.5f / (P.HxF * (1.0f / 16.0f));
+  const float X = pSize * V[3] * halfPixelX;
+  const float Y = pSize * V[3] * halfPixelY;
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    for (int j = 0; j < 4; j++) cp[k].v[j] = V[j];
+  cp[0].v[0] -= X; cp[0].v[1] += Y;
+  cp[1].v[0] += X; cp[1].v[1] += Y;
+  cp[2].v[0] += X; cp[2].v[1] -= Y;
+  cp[3].v[0] -= X; cp[3].v[1] -= Y;
+}
+
 // Setup of primitive t of the draw P, written to record slot g (the
 // primitive's index in the batch); `draw` is the draw's index in the batch.
 __device__ __forceinline__ void setup_one(const DrawParams &P, int t, size_t g, int draw, TriRecord *__restrict__ rec,
@@ -238,7 +262,14 @@ __device__ __forceinline__ void setup_one(const DrawParams &P, int t, size_t g, 
   int stages = 0;
   int d = 1;
   int frontFacing = 1;
-  if (P.line) {
+  if (P.point) {
+    // setupPoint: the point's square, clipped against every frustum plane;
+    // no culling, front-facing, d = 1 (SetupRoutine.cpp:115-119).
+    point_polygon(P, pos[0], P.psize ? P.psize[vi[0]] : 1.0f, cp);
+    n = 4;
+    stages = clip_polygon(cp, n, CLIP_FRUSTUM);
+    if (n < 3) return;
+  } else if (P.line) {
     // setupLine: the line's polygon, clipped against every frustum plane;
     // no culling, front-facing, d = 1 (SetupRoutine.cpp:115-119).
     if (!line_polygon(P, pos[0], pos[1], cp)) return;
@@ -271,7 +302,7 @@ __device__ __forceinline__ void setup_one(const DrawParams &P, int t, size_t g, 
   // Polygon in 28.4 (reprojected when clipped or a line,
   // SetupRoutine.cpp:122-143).
   int X[kMaxPolygon], Y[kMaxPolygon];
-  if (stages != 0 || P.line) {
+  if (stages != 0 || P.line || P.point) {
     for (int i = 0; i < n; i++) {
       float w = cp[i].v[3];
       float rhw = (w != 0.0f) ? 1.0f / w : 1.0f;
@@ -306,13 +337,13 @@ __device__ __forceinline__ void setup_one(const DrawParams &P, int t, size_t g, 
   // Sort by minimum y, then by maximum w (SetupRoutine.cpp:270-294),
   // triangles only.
   int o0 = 0, o1 = 1, o2 = 2;
-  if (!P.line) {
+  if (!P.line && !P.point) {
     float y0 = pos[0][1], y1 = pos[1][1], y2 = pos[2][1];
     float ym = rr_min(rr_min(y0, y1), y2);
     if (ym == y1) { int tt = o0; o0 = o1; o1 = o2; o2 = tt; }
     if (ym == y2) { int tt = o2; o2 = o1; o1 = o0; o0 = tt; }
   }
-  if (!P.line) {
+  if (!P.line && !P.point) {
     float w0 = pos[o0][3], w1 = pos[o1][3], w2 = pos[o2][3];
     float wm = rr_max(rr_max(w0, w1), w2);
     if (wm == w1) { int tt = o0; o0 = o1; o1 = o2; o2 = tt; }
@@ -354,9 +385,14 @@ __device__ __forceinline__ void setup_one(const DrawParams &P, int t, size_t g, 
     float z0 = pz[o0], z1 = pz[o1] - z0, z2 = pz[o2] - z0;
     float fx1 = (float)X1 * (1.0f / 16.0f), fy1 = (float)Y1 * (1.0f / 16.0f);
     float fx2 = (float)X2 * (1.0f / 16.0f), fy2 = (float)Y2 * (1.0f / 16.0f);
-    float D = P.depthRange / (fx1 * fy2 - fx2 * fy1);
-    R.zA = (fy2 * z1 - fy1 * z2) * D;
-    R.zB = (fx1 * z2 - fx2 * z1) * D;
+    if (!P.point) {
+      float D = P.depthRange / (fx1 * fy2 - fx2 * fy1);
+      R.zA = (fy2 * z1 - fy1 * z2) * D;
+      R.zB = (fx1 * z2 - fx2 * z1) * D;
+    } else {  // a point's depth is constant (SetupRoutine.cpp:393-409)
+      R.zA = 0.0f;
+      R.zB = 0.0f;
+    }
     R.zC = z0 * P.depthRange + P.depthNear;
     // Depth bias (SetupRoutine.cpp:417-475), floating-point depth buffer:
     // r = 2^(e - 23) for the largest exponent e among the plane constant and

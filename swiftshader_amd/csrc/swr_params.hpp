@@ -51,9 +51,11 @@ struct DrawParams {
   const uint32_t *idx;
   int triCount;
   int vertexCount;
-  int topology;     // 0 triangle list, 1 triangle strip, 2 fan, 3 line list, 4 line strip
+  int topology;     // 0 triangle list, 1 triangle strip, 2 fan, 3 line list, 4 line strip, 5 points
   int line, bresenham;  // lines; set up as Bresenham parallelograms
   float lineWidth;
+  int point;
+  const float *psize;  // PointSize per vertex, or null (1)
   int provokeLast;  // VK_PROVOKING_VERTEX_MODE_LAST_VERTEX
   // Viewport / scissor (Renderer.cpp:300-345).
   float WxF, HxF, X0xF, Y0xF, depthRange, depthNear;

@@ -18,6 +18,7 @@ class Buffers:
     colors: int = 0
     texcoords: int = 0
     indices: int = 0
+    point_sizes: int = 0
     texture_levels: Optional[List[Tuple[int, int, int, int]]] = None  # (addr, width, height, row_pitch_bytes)
     color: int = 0
     color_pitch: int = 0
@@ -53,6 +54,7 @@ def build_desc(draw: DrawData, buf: Buffers) -> abi.DrawDesc:
     d.colors = buf.colors or None
     d.texcoords = buf.texcoords or None
     d.indices = buf.indices or None
+    d.point_sizes = buf.point_sizes or None
     d.vertex_count = len(draw.positions)
     d.triangle_count = draw.triangle_count
     if draw.viewport is None:

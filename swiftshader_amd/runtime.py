@@ -121,6 +121,10 @@ def prepare(draw: DrawData, device, targets: Optional[Targets] = None,
         idx = _dev(draw.indices.astype(np.uint32).view(np.int32), device)
         keep.append(idx)
         buf.indices = idx.data_ptr()
+    if draw.point_sizes is not None:
+        ps = _dev(draw.point_sizes.astype(np.float32), device)
+        keep.append(ps)
+        buf.point_sizes = ps.data_ptr()
     if draw.texture is not None:
         levels = texture_tensors if texture_tensors is not None else upload_texture(draw.texture, device)
         keep.extend(levels)

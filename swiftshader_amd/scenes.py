@@ -85,6 +85,7 @@ class DrawData:
     depth_bias: Optional[tuple] = None  # (constant_factor, clamp, slope_factor); None = disabled
     line_width: float = 1.0
     line_rasterization_mode: int = abi.LINE_RASTERIZATION_DEFAULT
+    point_sizes: Optional[np.ndarray] = None  # (V,) float32 PointSize per vertex; None = 1
 
     @property
     def triangle_count(self):
@@ -96,6 +97,8 @@ class DrawData:
             return max(n - 1, 0)
         if self.topology == abi.TOPOLOGY_LINE_LIST:
             return n // 2
+        if self.topology == abi.TOPOLOGY_POINT_LIST:
+            return n
         return n // 3
 
 

@@ -251,6 +251,49 @@ def _lines(seed=36, size=(88, 70), n=300, topology=abi.TOPOLOGY_LINE_LIST, width
                     **extra)
 
 
+def _points(seed=41, size=(80, 64), n=700, samples=1, sizes=True, indexed=False, textured=False, clip=False,
+            stencil=False):
+    """Random points: sizes spread over [0.25, 14] with a few below 1, above
+    the target and NaN; optionally indexed, textured, outside the frustum."""
+    rng = np.random.default_rng(seed)
+    W, H = size
+    if clip:
+        xy = rng.uniform(-1.6, 1.6, (n, 2))
+        w = rng.uniform(-0.3, 2.5, (n, 1))
+        z = rng.uniform(-0.5, 1.5, (n, 1))
+    else:
+        xy = rng.uniform(-1.05, 1.05, (n, 2))
+        w = rng.uniform(0.4, 3.0, (n, 1))
+        z = rng.uniform(0.05, 0.95, (n, 1))
+    pos = np.concatenate([xy * np.abs(w), z * np.abs(w), w], axis=1).astype(np.float32)
+    col = rng.uniform(0, 1, (n, 4)).astype(np.float32)
+    uv = rng.uniform(-1, 3, (n, 2)).astype(np.float32)
+    ps = None
+    if sizes:
+        ps = rng.uniform(0.25, 14.0, n).astype(np.float32)
+        ps[rng.integers(0, n, 12)] = np.float32(np.nan)
+        ps[rng.integers(0, n, 3)] = np.float32(2000.0)
+        ps[rng.integers(0, n, 20)] = rng.integers(1, 6, 20).astype(np.float32)  # whole sizes: edges on the grid
+    idx = None
+    if indexed:
+        idx = rng.integers(0, n, n + n // 2).astype(np.uint32)
+    tex, shader = None, abi.SHADER_INTERPOLATE
+    if textured:
+        tex = TextureData(format=abi.FORMAT_R8G8B8A8_UNORM,
+                          levels=scenes.box_mip_chain(scenes.noise_texture_rgba8(rng, 64)))
+        shader = abi.SHADER_MODULATE
+    extra = {}
+    if stencil:
+        extra = dict(has_stencil=True, stencil_test=1, clear_stencil=2,
+                     front=dict(fail_op=abi.STENCIL_OP_KEEP, pass_op=abi.STENCIL_OP_INCREMENT_AND_WRAP,
+                                depth_fail_op=abi.STENCIL_OP_DECREMENT_AND_CLAMP, compare_op=abi.COMPARE_ALWAYS,
+                                compare_mask=0xFF, write_mask=0xFF, reference=0),
+                     back=dict(compare_op=abi.COMPARE_NEVER, fail_op=abi.STENCIL_OP_ZERO))
+    return DrawData(width=W, height=H, positions=pos, colors=col, texcoords=uv, indices=idx, samples=samples,
+                    shader=shader, texture=tex, topology=abi.TOPOLOGY_POINT_LIST, point_sizes=ps,
+                    depth_compare=abi.COMPARE_LESS_OR_EQUAL, **extra)
+
+
 def parity_cases():
     """(name, DrawData) pairs; every one must match the oracle bit for bit."""
     return [
@@ -291,4 +334,7 @@ def parity_cases():
                                               mode=abi.LINE_RASTERIZATION_BRESENHAM, textured=True)),
         ("lines_clipped_last_msaa4_stencil", _lines(seed=40, clip=True, width=1.5, samples=4, stencil=True,
                                                     provoking=abi.PROVOKING_VERTEX_LAST)),
+        ("points_sizes", _points()),
+        ("points_sizes_msaa4_clipped_stencil", _points(seed=42, samples=4, clip=True, stencil=True)),
+        ("points_indexed_textured_unsized", _points(seed=43, sizes=False, indexed=True, textured=True)),
     ]

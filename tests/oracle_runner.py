@@ -110,6 +110,8 @@ def oracle_render(draw, targets=None, ids=None):
         buf.texcoords = arr(draw.texcoords, np.float32)
     if draw.indices is not None:
         buf.indices = arr(draw.indices, np.uint32)
+    if draw.point_sizes is not None:
+        buf.point_sizes = arr(draw.point_sizes, np.float32)
     if draw.texture is not None:
         levels = []
         for l in draw.texture.levels:
