@@ -219,6 +219,8 @@ typedef struct {
   int enableMS;
   unsigned msMask;
   int depthTestActive, depthWriteEnable, depthCompare;
+  int colorBGRA;      /* B8G8R8A8_UNORM colour attachment */
+  int depth16;        /* D16_UNORM depth attachment */
   int line;           /* LINE_LIST / LINE_STRIP: primitives are lines */
   int bresenham;      /* lines set up as Bresenham parallelograms */
   int point;          /* POINT_LIST: primitives are points */
@@ -556,6 +558,8 @@ static int setup_routine(Ctx *c, OPrim *prim, const OVertex *tri[3], const Polyg
       float r;
       memcpy(&r, &e, 4);
       r = r * (1.0f / (1 << 23));
+      /* D16: minimumResolvableDepthDifference (Renderer.cpp:427-430) */
+      if (c->depth16) r = 1.01f / 65535.0f;
       bias = r * c->constantDepthBias;
     }
     if (c->biasSlope) {
@@ -864,10 +868,39 @@ static uint8_t *stencil_at(Ctx *c, int q, int x, int y) {
   return (uint8_t *)c->d->stencil.data + (size_t)q * c->d->stencil.sample_pitch_bytes +
          (size_t)y * c->d->stencil.row_pitch_bytes + (size_t)x;
 }
-static float *depth_at(Ctx *c, int q, int x, int y) {
-  return (float *)((uint8_t *)c->d->depth.data + (size_t)q * c->d->depth.sample_pitch_bytes +
-                   (size_t)y * c->d->depth.row_pitch_bytes + (size_t)x * 4);
+static uint8_t *depth_at(Ctx *c, int q, int x, int y) {
+  return (uint8_t *)c->d->depth.data + (size_t)q * c->d->depth.sample_pitch_bytes +
+         (size_t)y * c->d->depth.row_pitch_bytes + (size_t)x * (c->depth16 ? 2 : 4);
 }
+/* readDepth32F / readDepth16 (PixelRoutine.cpp:451-481): D16 reads as the
+ * float of its integer. */
+static float read_depth(Ctx *c, int q, int x, int y) {
+  const uint8_t *p = depth_at(c, q, x, y);
+  if (c->depth16) {
+    uint16_t u;
+    memcpy(&u, p, 2);
+    return (float)u;
+  }
+  float f;
+  memcpy(&f, p, 4);
+  return f;
+}
+/* writeDepth32F / writeDepth16 (PixelRoutine.cpp:658-711); z is the value
+ * the test compared, already Round(z * 0xFFFF) for D16 and in [0, 65535]
+ * (UShort4(Float4, saturate), LLVMReactor.cpp:2150-2171). */
+static void write_depth(Ctx *c, int q, int x, int y, float z) {
+  uint8_t *p = depth_at(c, q, x, y);
+  if (c->depth16) {
+    uint16_t u = (uint16_t)z;
+    memcpy(p, &u, 2);
+  } else {
+    memcpy(p, &z, 4);
+  }
+}
+/* Memory byte of colour channel k: B8G8R8A8 swaps R and B (PixelRoutine.cpp
+ * readPixel :1091-1110, writeColor :2130-2143 with the write mask flipped to
+ * memory order, so the logical mask still selects logical channels). */
+static int chan(const Ctx *c, int k) { return c->colorBGRA && (k & 1) == 0 ? 2 - k : k; }
 static uint8_t *color_at(Ctx *c, int q, int x, int y) {
   return (uint8_t *)c->d->color.data + (size_t)q * c->d->color.sample_pitch_bytes +
          (size_t)y * c->d->color.row_pitch_bytes + (size_t)x * (c->colorIsFloat ? 16 : 4);
@@ -937,7 +970,7 @@ static void alpha_blend(Ctx *c, const uint8_t *buf, const float src[4], float ou
   if (c->colorIsFloat) {
     memcpy(dst, buf, 16);
   } else {
-    for (int k = 0; k < 4; k++) dst[k] = (float)(buf[k] * 257) * (1.0f / 65535.0f);
+    for (int k = 0; k < 4; k++) dst[k] = (float)(buf[chan(c, k)] * 257) * (1.0f / 65535.0f);
   }
   float sf[4], df[4];
   for (int k = 0; k < 3; k++) {
@@ -961,7 +994,7 @@ static void write_color(Ctx *c, uint8_t *buf, const float col[4]) {
       float v = rr_min(rr_max(col[k], 0.0f), 1.0f);
       int32_t iv = rr_round_int(v * 255.0f);
       uint8_t b = (uint8_t)(iv < 0 ? 0 : (iv > 255 ? 255 : iv)); /* PackUnsigned saturation */
-      if (wm & (1 << k)) buf[k] = b;
+      if (wm & (1 << k)) buf[chan(c, k)] = b;
     }
   }
 }
@@ -1038,10 +1071,12 @@ static void quad(Ctx *c, const OPrim *prim, int x, int y, const int cMask[4], co
     if (!c->depthTestActive) { depthPass = 1; continue; }
     int m = 0;
     for (int i = 0; i < 4; i++) {
-      z[q][i] = rr_min(rr_max(z[q][i], 0.0f), 1.0f); /* clampDepth: D32F without unrestricted range */
+      z[q][i] = rr_min(rr_max(z[q][i], 0.0f), 1.0f); /* clampDepth: [0, 1] without unrestricted range */
+      /* D16 compares and stores Min(Max(Round(z * 0xFFFF), 0), 0xFFFF) (PixelRoutine.cpp:506-510) */
+      if (c->depth16) z[q][i] = rr_min(rr_max(rintf(z[q][i] * 65535.0f), 0.0f), 65535.0f);
       float Z = z[q][i];
       if (!(cMask[q] & (1 << i))) continue; /* uncovered lanes are masked out */
-      float zb = *depth_at(c, q, x + (i & 1), y + (i >> 1));
+      float zb = read_depth(c, q, x + (i & 1), y + (i >> 1));
       int t;
       switch (c->depthCompare) {
         case SWR_COMPARE_ALWAYS: t = 1; break;
@@ -1067,7 +1102,7 @@ static void quad(Ctx *c, const OPrim *prim, int x, int y, const int cMask[4], co
       for (int s = 0; s < ns; s++) {
         int q = samples[s];
         for (int i = 0; i < 4; i++)
-          if (zMask[q] & (1 << i)) *depth_at(c, q, x + (i & 1), y + (i >> 1)) = z[q][i];
+          if (zMask[q] & (1 << i)) write_depth(c, q, x + (i & 1), y + (i >> 1), z[q][i]);
       }
     }
     for (int s = 0; s < ns; s++) {
@@ -1265,6 +1300,8 @@ static int init_ctx(Ctx *c, const swr_draw_desc *d) {
   c->stencilActive = d->stencil.data && d->stencil_test_enable;
 
   c->colorIsFloat = d->color.format == SWR_FORMAT_R32G32B32A32_SFLOAT;
+  c->colorBGRA = d->color.format == SWR_FORMAT_B8G8R8A8_UNORM;
+  c->depth16 = d->depth.data && d->depth.format == SWR_FORMAT_D16_UNORM;
   if (d->primitive_id.data && (!c->colorIsFloat || d->primitive_id.format != SWR_FORMAT_R32_UINT))
     return SWR_ERROR_UNSUPPORTED;
   const swr_blend_state *b = &d->blend;
@@ -1521,6 +1558,20 @@ static uint32_t clear_unorm8(float c) {
   return t == t ? (uint32_t)t : 0u;
 }
 
+/* A D16 clear runs the blit routine from a D32F pixel: sw::clamp to [0, 1]
+ * (a NaN passes), the 0xFFFF / 1 scale, Min(., 0xFFFF) and Max(., 0) with
+ * Reactor's ordered selects (a NaN becomes 0xFFFF), a second Max(., 0),
+ * then UShort(RoundInt(.)) -- round to nearest even. */
+static uint16_t clear_d16(float d) {
+  if (d < 0.0f) d = 0.0f;
+  if (d > 1.0f) d = 1.0f;
+  float v = d * 65535.0f;
+  v = rr_min(v, 65535.0f);
+  v = rr_max(v, 0.0f);
+  v = rr_max(v, 0.0f);
+  return (uint16_t)rr_round_int(v);
+}
+
 EXPORT int swo_clear(const swr_clear_desc *d) {
   if (!d || !d->target.data || (d->samples != 1 && d->samples != 4)) return SWR_ERROR_INVALID_ARGUMENT;
   uint32_t bpp;
@@ -1533,10 +1584,23 @@ EXPORT int swo_clear(const swr_clear_desc *d) {
       bpp = 4;
       break;
     }
+    case SWR_FORMAT_B8G8R8A8_UNORM: { /* fastClear's B8G8R8A8 packing (Blitter.cpp:222-227) */
+      uint32_t p = clear_unorm8(d->value.color[2]) | (clear_unorm8(d->value.color[1]) << 8) |
+                   (clear_unorm8(d->value.color[0]) << 16) | (clear_unorm8(d->value.color[3]) << 24);
+      memcpy(bytes, &p, 4);
+      bpp = 4;
+      break;
+    }
     case SWR_FORMAT_R32G32B32A32_SFLOAT:
       memcpy(bytes, d->value.color, 16);
       bpp = 16;
       break;
+    case SWR_FORMAT_D16_UNORM: { /* blit routine: D32F source to D16 (Blitter.cpp:1427-1484, 1085-1087) */
+      uint16_t u = clear_d16(d->value.depth_stencil.depth);
+      memcpy(bytes, &u, 2);
+      bpp = 2;
+      break;
+    }
     case SWR_FORMAT_D32_SFLOAT:
       memcpy(bytes, &d->value.depth_stencil.depth, 4);
       bpp = 4;
@@ -1570,7 +1634,7 @@ EXPORT int swo_resolve(const swr_resolve_desc *r) {
   _mm_setcsr(csr | 0x8040);
   for (uint32_t y = 0; y < r->height; y++) {
     for (uint32_t x = 0; x < r->width; x++) {
-      if (r->format == SWR_FORMAT_R8G8B8A8_UNORM) {
+      if (r->format == SWR_FORMAT_R8G8B8A8_UNORM || r->format == SWR_FORMAT_B8G8R8A8_UNORM) {
         const uint8_t *s[4];
         for (int q = 0; q < 4; q++)
           s[q] = (const uint8_t *)r->src + q * r->src_sample_pitch_bytes + (size_t)y * r->src_row_pitch_bytes + x * 4;

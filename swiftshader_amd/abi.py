@@ -17,8 +17,10 @@ ERROR_DEVICE = -4
 
 FORMAT_UNDEFINED = 0
 FORMAT_R8G8B8A8_UNORM = 37
+FORMAT_B8G8R8A8_UNORM = 44
 FORMAT_R32_UINT = 98
 FORMAT_R32G32B32A32_SFLOAT = 109
+FORMAT_D16_UNORM = 124
 FORMAT_D32_SFLOAT = 126
 FORMAT_S8_UINT = 127
 

@@ -11,6 +11,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -299,8 +300,9 @@ int build_params(const swr_draw_desc *d, const swr_tuning &tuning, DrawParams &P
   P.msMask = (int)(d->sample_mask & (0xFFFFFFFFu >> (32 - d->samples)));
 
   // Depth / stencil state (PixelProcessor.cpp:99-137, Context.cpp:930-944).
-  if (d->depth.data && d->depth.format != SWR_FORMAT_D32_SFLOAT)
-    return fail(SWR_ERROR_UNSUPPORTED, "depth format %u (only D32_SFLOAT)", d->depth.format);
+  if (d->depth.data && d->depth.format != SWR_FORMAT_D32_SFLOAT && d->depth.format != SWR_FORMAT_D16_UNORM)
+    return fail(SWR_ERROR_UNSUPPORTED, "depth format %u (D32_SFLOAT or D16_UNORM)", d->depth.format);
+  P.depth16 = d->depth.data && d->depth.format == SWR_FORMAT_D16_UNORM;
   if (d->stencil.data && d->stencil.format != SWR_FORMAT_S8_UINT)
     return fail(SWR_ERROR_UNSUPPORTED, "stencil format %u (only S8_UINT)", d->stencil.format);
   if (!valid_compare(d->depth_compare_op)) return fail(SWR_ERROR_INVALID_ARGUMENT, "invalid depth compare op");
@@ -337,9 +339,11 @@ int build_params(const swr_draw_desc *d, const swr_tuning &tuning, DrawParams &P
 
   // Colour output and blend state (Context.cpp:1090-1311).
   bool hasColor = d->color.data && d->color.format != SWR_FORMAT_UNDEFINED;
-  if (hasColor && d->color.format != SWR_FORMAT_R8G8B8A8_UNORM && d->color.format != SWR_FORMAT_R32G32B32A32_SFLOAT)
+  if (hasColor && d->color.format != SWR_FORMAT_R8G8B8A8_UNORM && d->color.format != SWR_FORMAT_B8G8R8A8_UNORM &&
+      d->color.format != SWR_FORMAT_R32G32B32A32_SFLOAT)
     return fail(SWR_ERROR_UNSUPPORTED, "colour format %u", d->color.format);
   P.colorFloat = d->color.format == SWR_FORMAT_R32G32B32A32_SFLOAT;
+  P.colorBGRA = hasColor && d->color.format == SWR_FORMAT_B8G8R8A8_UNORM;
   const swr_blend_state &b = d->blend;
   if (!valid_factor(b.src_color_blend_factor) || !valid_factor(b.dst_color_blend_factor) ||
       !valid_factor(b.src_alpha_blend_factor) || !valid_factor(b.dst_alpha_blend_factor) ||
@@ -435,7 +439,8 @@ int build_params(const swr_draw_desc *d, const swr_tuning &tuning, DrawParams &P
       return fail(SWR_ERROR_INVALID_ARGUMENT, "colour sample pitch too small");
   }
   if (P.depthTest) {
-    if (d->depth.row_pitch_bytes < d->framebuffer_width * 4) return fail(SWR_ERROR_INVALID_ARGUMENT, "depth row pitch too small");
+    if (d->depth.row_pitch_bytes < d->framebuffer_width * (P.depth16 ? 2u : 4u))
+      return fail(SWR_ERROR_INVALID_ARGUMENT, "depth row pitch too small");
     if (d->samples > 1 && d->depth.sample_pitch_bytes < (uint64_t)d->depth.row_pitch_bytes * d->framebuffer_height)
       return fail(SWR_ERROR_INVALID_ARGUMENT, "depth sample pitch too small");
   }
@@ -1211,10 +1216,12 @@ int swr_draw_batch(const swr_draw_desc *descs, uint32_t count, void *stream) {
 int swr_resolve(const swr_resolve_desc *d, void *stream) {
   if (!d || !d->src || !d->dst) return fail(SWR_ERROR_INVALID_ARGUMENT, "resolve descriptor incomplete");
   if (d->src_samples != 4) return fail(SWR_ERROR_UNSUPPORTED, "resolve supports 4 samples");
-  if (d->format != SWR_FORMAT_R8G8B8A8_UNORM && d->format != SWR_FORMAT_R32G32B32A32_SFLOAT)
+  // B8G8R8A8 takes the same byte-wise path as R8G8B8A8 (Blitter.cpp:2142).
+  if (d->format != SWR_FORMAT_R8G8B8A8_UNORM && d->format != SWR_FORMAT_B8G8R8A8_UNORM &&
+      d->format != SWR_FORMAT_R32G32B32A32_SFLOAT)
     return fail(SWR_ERROR_UNSUPPORTED, "resolve format %u", d->format);
   if (d->width == 0 || d->height == 0) return SWR_OK;
-  uint32_t bpp = d->format == SWR_FORMAT_R8G8B8A8_UNORM ? 4 : 16;
+  uint32_t bpp = d->format == SWR_FORMAT_R32G32B32A32_SFLOAT ? 16 : 4;
   if (d->src_row_pitch_bytes < d->width * bpp || d->dst_row_pitch_bytes < d->width * bpp ||
       d->src_sample_pitch_bytes < (uint64_t)d->src_row_pitch_bytes * d->height)
     return fail(SWR_ERROR_INVALID_ARGUMENT, "resolve pitches too small");
@@ -1243,6 +1250,19 @@ uint32_t unorm8_clear(float c) {
   const float u = t + 0.5f;
   return u == u ? (uint32_t)u : 0u;
 }
+
+// A D16 clear runs the blit routine from a D32F pixel (Blitter.cpp:64-86,
+// 1427-1484, 1085-1087): the clamp above, the 0xFFFF scale, Min(., 0xFFFF)
+// and Max(., 0) as ordered selects (a NaN becomes 0xFFFF), then RoundInt,
+// to nearest even.
+uint32_t d16_clear(float d) {
+  if (d < 0.0f) d = 0.0f;
+  if (d > 1.0f) d = 1.0f;
+  float v = d * 65535.0f;
+  v = v < 65535.0f ? v : 65535.0f;
+  v = v > 0.0f ? v : 0.0f;
+  return (uint32_t)std::nearbyint(v) & 0xFFFFu;
+}
 }  // namespace
 
 int swr_clear(const swr_clear_desc *d, void *stream) {
@@ -1256,9 +1276,18 @@ int swr_clear(const swr_clear_desc *d, void *stream) {
       v.x = unorm8_clear(d->value.color[0]) | (unorm8_clear(d->value.color[1]) << 8) |
             (unorm8_clear(d->value.color[2]) << 16) | (unorm8_clear(d->value.color[3]) << 24);
       break;
+    case SWR_FORMAT_B8G8R8A8_UNORM:
+      bpp = 4;
+      v.x = unorm8_clear(d->value.color[2]) | (unorm8_clear(d->value.color[1]) << 8) |
+            (unorm8_clear(d->value.color[0]) << 16) | (unorm8_clear(d->value.color[3]) << 24);
+      break;
     case SWR_FORMAT_R32G32B32A32_SFLOAT:
       bpp = 16;
       memcpy(&v, d->value.color, 16);
+      break;
+    case SWR_FORMAT_D16_UNORM:
+      bpp = 2;
+      v.x = d16_clear(d->value.depth_stencil.depth);
       break;
     case SWR_FORMAT_D32_SFLOAT:
       bpp = 4;
@@ -1292,6 +1321,8 @@ int swr_clear(const swr_clear_desc *d, void *stream) {
     hipLaunchKernelGGL(clear_rect_kernel<16>, grid, block, 0, s, base, pitch, slice, (int)x0, (int)y0, w, h, v);
   else if (bpp == 4)
     hipLaunchKernelGGL(clear_rect_kernel<4>, grid, block, 0, s, base, pitch, slice, (int)x0, (int)y0, w, h, v);
+  else if (bpp == 2)
+    hipLaunchKernelGGL(clear_rect_kernel<2>, grid, block, 0, s, base, pitch, slice, (int)x0, (int)y0, w, h, v);
   else
     hipLaunchKernelGGL(clear_rect_kernel<1>, grid, block, 0, s, base, pitch, slice, (int)x0, (int)y0, w, h, v);
   SWR_HIP_CHECK(hipGetLastError());

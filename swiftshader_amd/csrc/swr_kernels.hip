@@ -97,6 +97,18 @@ __device__ int clip_polygon(ClipVtx (&poly)[kMaxPolygon], int &n, int flagsOr) {
 __device__ __forceinline__ int wadd(int a, int b) { return (int)((unsigned)a + (unsigned)b); }
 __device__ __forceinline__ int wsub(int a, int b) { return (int)((unsigned)a - (unsigned)b); }
 
+// D16 compares and stores Min(Max(Round(z * 0xFFFF), 0), 0xFFFF) of the
+// clamped depth (PixelRoutine.cpp:506-510, 686-711; Round is to nearest even).
+__device__ __forceinline__ float d16_value(float z) {
+  return rr_min(rr_max(__builtin_rintf(z * 65535.0f), 0.0f), 65535.0f);
+}
+
+// B8G8R8A8 keeps R and B exchanged in memory (PixelRoutine.cpp:1091-1110,
+// 2130-2143): the tile holds R8G8B8A8 order.
+__device__ __forceinline__ uint32_t bgra_swap(int bgra, uint32_t v) {
+  return bgra ? (v & 0xFF00FF00u) | ((v >> 16) & 0xFFu) | ((v & 0xFFu) << 16) : v;
+}
+
 // Vertex indices of primitive t in the reference's batch order
 // (setBatchIndices, Renderer.cpp:50-145); out-of-range indices read the
 // last vertex instead of faulting.
@@ -404,7 +416,8 @@ __device__ __forceinline__ void setup_one(const DrawParams &P, int t, size_t g, 
       const int e0 = __float_as_int(R.zC) & 0x7F800000, e1 = __float_as_int(Z1) & 0x7F800000;
       const int e2 = __float_as_int(Z2) & 0x7F800000;
       const int e = max(max(e0, e1), e2);
-      const float r = __int_as_float(e) * (1.0f / (float)(1 << 23));
+      // D16: minimumResolvableDepthDifference (Renderer.cpp:427-430).
+      const float r = P.depth16 ? 1.01f / 65535.0f : __int_as_float(e) * (1.0f / (float)(1 << 23));
       bias = r * P.constantDepthBias;
     }
     if (P.biasSlope) {
@@ -786,16 +799,20 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
     if (FCOL && P.primId)
       S.id[FCOL ? q : 0] =
           *reinterpret_cast<const uint32_t *>(P.primId + q * P.primIdSlice + (long long)py * P.primIdPitch + 4ll * px);
-    if (P.depthTest)
-      S.zb[q] = *reinterpret_cast<const float *>(P.depth + q * P.depthSlice + (long long)py * P.depthPitch + 4ll * px);
+    if (P.depthTest)  // D16 holds the float of its integer (readDepth16, PixelRoutine.cpp:466-481)
+      S.zb[q] = P.depth16 ? (float)*reinterpret_cast<const uint16_t *>(P.depth + q * P.depthSlice +
+                                                                        (long long)py * P.depthPitch + 2ll * px)
+                          : *reinterpret_cast<const float *>(P.depth + q * P.depthSlice +
+                                                             (long long)py * P.depthPitch + 4ll * px);
     if (P.stencilActive) S.sb[q] = (uint8_t)P.stencil[q * P.stencilSlice + (long long)py * P.stencilPitch + px];
     if (P.colorMask) {
       if (FCOL)
         S.cf[FCOL ? q : 0] =
             *reinterpret_cast<const float4 *>(P.color + q * P.colorSlice + (long long)py * P.colorPitch + 16ll * px);
       else
-        S.cb[FCOL ? 0 : q] =
-            *reinterpret_cast<const uint32_t *>(P.color + q * P.colorSlice + (long long)py * P.colorPitch + 4ll * px);
+        S.cb[FCOL ? 0 : q] = bgra_swap(
+            P.colorBGRA,
+            *reinterpret_cast<const uint32_t *>(P.color + q * P.colorSlice + (long long)py * P.colorPitch + 4ll * px));
     }
   }
 
@@ -938,6 +955,7 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
           float z = xx * a + Dz;
           if (P.depthBias) z = z + zbias;  // PixelRoutine.cpp:141-144, before the clamp
           z = rr_min(rr_max(z, 0.0f), 1.0f);
+          if (P.depth16) z = d16_value(z);
           zq[q] = z;
           const float zb = S.zb[q];
           bool pass;
@@ -1037,6 +1055,7 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
         float zl = fmaxf(zlo0, m - mag * 0x1p-19f);
         if (!(zl == zl)) zl = -__builtin_inff();
         zl = fminf(zl, 1.0f);
+        if (P.depth16) zl = d16_value(fmaxf(zl, 0.0f));  // the bound in stored units (monotone map)
         test = !(P.depthCompare == 1 ? zl >= zmax : zl > zmax);
         zlo = zl;
       }
@@ -1074,8 +1093,13 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
   if (inside) {
 #pragma unroll
     for (int q = 0; q < MS; q++) {
-      if (P.depthWrite)
-        *reinterpret_cast<float *>(P.depth + q * P.depthSlice + (long long)py * P.depthPitch + 4ll * px) = S.zb[q];
+      if (P.depthWrite) {
+        if (P.depth16)
+          *reinterpret_cast<uint16_t *>(P.depth + q * P.depthSlice + (long long)py * P.depthPitch + 2ll * px) =
+              (uint16_t)S.zb[q];
+        else
+          *reinterpret_cast<float *>(P.depth + q * P.depthSlice + (long long)py * P.depthPitch + 4ll * px) = S.zb[q];
+      }
       if (P.stencilWrite) P.stencil[q * P.stencilSlice + (long long)py * P.stencilPitch + px] = (char)S.sb[q];
       if (FCOL && P.primId)
         *reinterpret_cast<uint32_t *>(P.primId + q * P.primIdSlice + (long long)py * P.primIdPitch + 4ll * px) =
@@ -1086,7 +1110,7 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
               S.cf[FCOL ? q : 0];
         else
           *reinterpret_cast<uint32_t *>(P.color + q * P.colorSlice + (long long)py * P.colorPitch + 4ll * px) =
-              S.cb[FCOL ? 0 : q];
+              bgra_swap(P.colorBGRA, S.cb[FCOL ? 0 : q]);
       }
     }
   }
@@ -1175,6 +1199,7 @@ __global__ void clear_rect_kernel(char *base, int rowPitch, long long slice, int
   char *p = base + (long long)blockIdx.z * slice + (long long)(y0 + y) * rowPitch + (long long)BYTES * (x0 + x);
   if (BYTES == 16) *reinterpret_cast<uint4 *>(p) = value;
   else if (BYTES == 4) *reinterpret_cast<uint32_t *>(p) = value.x;
+  else if (BYTES == 2) *reinterpret_cast<uint16_t *>(p) = (uint16_t)value.x;
   else *reinterpret_cast<uint8_t *>(p) = (uint8_t)value.x;
 }
 

@@ -66,6 +66,7 @@ struct DrawParams {
   int ms, msMask;
   // Depth / stencil.
   int depthTest, depthWrite, depthCompare;
+  int depth16;  // D16_UNORM depth attachment (else D32_SFLOAT)
   // Depth bias: which terms the setup applies, and whether samples get it.
   int biasConstant, biasSlope, biasClamp, depthBias;
   float constantDepthBias, slopeDepthBias, depthBiasClamp;
@@ -74,6 +75,7 @@ struct DrawParams {
   // Colour.
   int colorMask;   // canonical 4-bit write mask; 0 = no colour output
   int colorFloat;  // RGBA32F target
+  int colorBGRA;   // B8G8R8A8_UNORM target (R and B exchanged in memory)
   int blendEnable;
   int srcF, dstF, colorOp, srcFA, dstFA, alphaOp;
   float blendConst[4], invBlendConst[4];

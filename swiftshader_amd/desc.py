@@ -48,6 +48,17 @@ def color_bytes_per_pixel(fmt: int) -> int:
     return 16 if fmt == abi.FORMAT_R32G32B32A32_SFLOAT else 4
 
 
+def depth_bytes_per_pixel(fmt: int) -> int:
+    return 2 if fmt == abi.FORMAT_D16_UNORM else 4
+
+
+# Bytes per pixel of every attachment format the library clears.
+FORMAT_BYTES = {abi.FORMAT_R8G8B8A8_UNORM: 4, abi.FORMAT_B8G8R8A8_UNORM: 4, abi.FORMAT_R32G32B32A32_SFLOAT: 16,
+                abi.FORMAT_D16_UNORM: 2, abi.FORMAT_D32_SFLOAT: 4, abi.FORMAT_S8_UINT: 1}
+COLOR_FORMATS = (abi.FORMAT_R8G8B8A8_UNORM, abi.FORMAT_B8G8R8A8_UNORM, abi.FORMAT_R32G32B32A32_SFLOAT)
+DEPTH_FORMATS = (abi.FORMAT_D16_UNORM, abi.FORMAT_D32_SFLOAT)
+
+
 def build_desc(draw: DrawData, buf: Buffers) -> abi.DrawDesc:
     d = abi.DrawDesc()
     d.positions = buf.positions
@@ -107,7 +118,7 @@ def build_desc(draw: DrawData, buf: Buffers) -> abi.DrawDesc:
     if buf.color:
         d.color = abi.Attachment(buf.color, draw.color_format, buf.color_pitch, buf.color_sample_pitch)
     if buf.depth:
-        d.depth = abi.Attachment(buf.depth, abi.FORMAT_D32_SFLOAT, buf.depth_pitch, buf.depth_sample_pitch)
+        d.depth = abi.Attachment(buf.depth, draw.depth_format, buf.depth_pitch, buf.depth_sample_pitch)
     if buf.stencil:
         d.stencil = abi.Attachment(buf.stencil, abi.FORMAT_S8_UINT, buf.stencil_pitch, buf.stencil_sample_pitch)
     d.occlusion_count = buf.occlusion or None

@@ -97,6 +97,8 @@ class View:
             raise ValueError("differentiable draws render into RGBA32F")
         if draw.blend_enable:
             raise ValueError("differentiable draws do not blend")
+        if draw.depth_format != abi.FORMAT_D32_SFLOAT:
+            raise ValueError("differentiable draws use a D32F depth attachment")
         self.draw = draw
         self.device = torch.device(device)
         S, H, W = draw.samples, draw.height, draw.width

@@ -14,7 +14,8 @@ import numpy as np
 import torch
 
 from . import abi
-from .desc import Buffers, build_desc, color_bytes_per_pixel
+from .desc import COLOR_FORMATS, DEPTH_FORMATS, FORMAT_BYTES, Buffers, build_desc, color_bytes_per_pixel, \
+    depth_bytes_per_pixel
 from .scenes import DrawData
 
 _LIB = None
@@ -52,7 +53,7 @@ def _stream_handle(stream) -> Optional[int]:
 class Targets:
     """Render targets of one draw: sample-major planes, row-major pixels."""
     color: Optional[torch.Tensor]    # (S, H, W, 4) uint8 or float32
-    depth: Optional[torch.Tensor]    # (S, H, W) float32
+    depth: Optional[torch.Tensor]    # (S, H, W) float32 (D32F) or int16 holding D16's bits
     stencil: Optional[torch.Tensor]  # (S, H, W) uint8
     occlusion: torch.Tensor          # (1,) int64
 
@@ -64,7 +65,7 @@ def clear_targets(draw: DrawData, targets: Targets, stream=None, reset_occlusion
     if targets.color is not None:
         clear(targets.color, draw.color_format, draw.clear_color, stream=stream)
     if targets.depth is not None:
-        clear(targets.depth, abi.FORMAT_D32_SFLOAT, draw.clear_depth, stream=stream)
+        clear(targets.depth, draw.depth_format, draw.clear_depth, stream=stream)
     if targets.stencil is not None:
         clear(targets.stencil, abi.FORMAT_S8_UINT, draw.clear_stencil, stream=stream)
     if reset_occlusion:
@@ -77,7 +78,8 @@ def make_targets(draw: DrawData, device) -> Targets:
         color = torch.empty((S, H, W, 4), dtype=torch.float32, device=device)
     else:
         color = torch.empty((S, H, W, 4), dtype=torch.uint8, device=device)
-    depth = torch.empty((S, H, W), dtype=torch.float32, device=device) if draw.has_depth else None
+    ddt = torch.int16 if draw.depth_format == abi.FORMAT_D16_UNORM else torch.float32  # D16: 16-bit storage
+    depth = torch.empty((S, H, W), dtype=ddt, device=device) if draw.has_depth else None
     stencil = torch.empty((S, H, W), dtype=torch.uint8, device=device) if draw.has_stencil else None
     occ = torch.zeros(1, dtype=torch.int64, device=device)
     t = Targets(color, depth, stencil, occ)
@@ -137,7 +139,8 @@ def prepare(draw: DrawData, device, targets: Optional[Targets] = None,
     if targets.color is not None:
         buf.color, buf.color_pitch, buf.color_sample_pitch = targets.color.data_ptr(), W * bpp, H * W * bpp
     if targets.depth is not None:
-        buf.depth, buf.depth_pitch, buf.depth_sample_pitch = targets.depth.data_ptr(), W * 4, H * W * 4
+        dbpp = depth_bytes_per_pixel(draw.depth_format)
+        buf.depth, buf.depth_pitch, buf.depth_sample_pitch = targets.depth.data_ptr(), W * dbpp, H * W * dbpp
     if targets.stencil is not None:
         buf.stencil, buf.stencil_pitch, buf.stencil_sample_pitch = targets.stencil.data_ptr(), W, H * W
     buf.occlusion = targets.occlusion.data_ptr()
@@ -166,21 +169,20 @@ def resolve(src: torch.Tensor, dst: torch.Tensor, stream=None):
 def clear_desc(target: torch.Tensor, fmt: int, value, area=None) -> abi.ClearDesc:
     """A swr_clear_desc for a (S, H, W[, C]) target tensor of format fmt.
 
-    value: 4 floats (colour), one float (D32F depth) or one int (S8 stencil);
-    area: (x, y, width, height) or None for the whole image."""
+    value: 4 floats (colour), one float (D32F or D16 depth) or one int (S8
+    stencil); area: (x, y, width, height) or None for the whole image."""
     S, H, W = target.shape[:3]
-    bpp = {abi.FORMAT_R8G8B8A8_UNORM: 4, abi.FORMAT_R32G32B32A32_SFLOAT: 16, abi.FORMAT_D32_SFLOAT: 4,
-           abi.FORMAT_S8_UINT: 1}[fmt]
+    bpp = FORMAT_BYTES[fmt]
     d = abi.ClearDesc()
     d.target = abi.Attachment(target.data_ptr(), fmt, W * bpp, H * W * bpp)
     d.width, d.height, d.samples = W, H, S
     if area is not None:
         d.use_area = 1
         d.area = abi.Rect(*area)
-    if fmt in (abi.FORMAT_R8G8B8A8_UNORM, abi.FORMAT_R32G32B32A32_SFLOAT):
+    if fmt in COLOR_FORMATS:
         for k in range(4):
             d.value.color[k] = float(value[k])
-    elif fmt == abi.FORMAT_D32_SFLOAT:
+    elif fmt in DEPTH_FORMATS:
         d.value.depth_stencil.depth = float(value)
     else:
         d.value.depth_stencil.stencil = int(value) & 0xFFFFFFFF

@@ -58,6 +58,7 @@ class DrawData:
     samples: int = 1
     sample_mask: int = 0xFFFFFFFF
     color_format: int = abi.FORMAT_R8G8B8A8_UNORM
+    depth_format: int = abi.FORMAT_D32_SFLOAT  # or FORMAT_D16_UNORM
     shader: int = abi.SHADER_INTERPOLATE
     cull_mode: int = abi.CULL_NONE
     front_face: int = abi.FRONT_FACE_COUNTER_CLOCKWISE

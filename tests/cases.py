@@ -294,6 +294,37 @@ def _points(seed=41, size=(80, 64), n=700, samples=1, sizes=True, indexed=False,
                     depth_compare=abi.COMPARE_LESS_OR_EQUAL, **extra)
 
 
+def _formats(seed=44, d16=True, bgra=False, samples=1, bias=None, stencil=False, textured=False, blend=False):
+    """Overlapping triangles into a D16 depth and/or B8G8R8A8 colour
+    attachment (optionally blended with a partial write mask)."""
+    rng = np.random.default_rng(seed)
+    W, H = 76, 58
+    pos, uv, col = scenes.screen_triangles(rng, 900, W, H, area_hi=700)
+    col = (col * 1.2 - 0.1).astype(np.float32)
+    tex, shader = None, abi.SHADER_INTERPOLATE
+    if textured:
+        tex = TextureData(format=abi.FORMAT_R8G8B8A8_UNORM,
+                          levels=scenes.box_mip_chain(scenes.noise_texture_rgba8(rng, 64)), anisotropy_enable=1,
+                          max_anisotropy=4.0)
+        shader = abi.SHADER_MODULATE
+    extra = {}
+    if blend:
+        extra.update(blend_enable=1, color_write_mask=0xD,
+                     blend=(abi.BLEND_FACTOR_SRC_ALPHA, abi.BLEND_FACTOR_ONE_MINUS_SRC_ALPHA, abi.BLEND_OP_ADD,
+                            abi.BLEND_FACTOR_ONE_MINUS_DST_ALPHA, abi.BLEND_FACTOR_ONE, abi.BLEND_OP_ADD),
+                     clear_color=(0.7, 0.2, 0.9, 0.4))
+    if stencil:
+        extra.update(has_stencil=True, stencil_test=1, clear_stencil=9,
+                     front=dict(pass_op=abi.STENCIL_OP_INCREMENT_AND_CLAMP, depth_fail_op=abi.STENCIL_OP_DECREMENT_AND_WRAP,
+                                compare_op=abi.COMPARE_GREATER, reference=40),
+                     back=dict(pass_op=abi.STENCIL_OP_INVERT, depth_fail_op=abi.STENCIL_OP_REPLACE,
+                               compare_op=abi.COMPARE_ALWAYS, reference=3))
+    return DrawData(width=W, height=H, positions=pos, colors=col, texcoords=uv, samples=samples, shader=shader,
+                    texture=tex, depth_format=abi.FORMAT_D16_UNORM if d16 else abi.FORMAT_D32_SFLOAT,
+                    color_format=abi.FORMAT_B8G8R8A8_UNORM if bgra else abi.FORMAT_R8G8B8A8_UNORM,
+                    depth_compare=abi.COMPARE_LESS, depth_bias=bias, clear_depth=0.75, **extra)
+
+
 def parity_cases():
     """(name, DrawData) pairs; every one must match the oracle bit for bit."""
     return [
@@ -337,4 +368,8 @@ def parity_cases():
         ("points_sizes", _points()),
         ("points_sizes_msaa4_clipped_stencil", _points(seed=42, samples=4, clip=True, stencil=True)),
         ("points_indexed_textured_unsized", _points(seed=43, sizes=False, indexed=True, textured=True)),
+        ("d16_depth", _formats()),
+        ("d16_msaa4_bias_stencil", _formats(seed=45, samples=4, bias=(-3.0, 0.0, 1.25), stencil=True)),
+        ("bgra8_blend_mask", _formats(seed=46, d16=False, bgra=True, blend=True)),
+        ("bgra8_d16_msaa4_textured", _formats(seed=47, bgra=True, samples=4, textured=True)),
     ]

@@ -9,7 +9,8 @@ import subprocess
 import numpy as np
 
 from swiftshader_amd import abi
-from swiftshader_amd.desc import Buffers, build_desc, color_bytes_per_pixel
+from swiftshader_amd.desc import (COLOR_FORMATS, DEPTH_FORMATS, FORMAT_BYTES, Buffers, build_desc,
+                                  color_bytes_per_pixel, depth_bytes_per_pixel)
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
@@ -78,7 +79,8 @@ def host_targets(draw):
     oracle_clear(color, draw.color_format, draw.clear_color)
     depth = stencil = None
     if draw.has_depth:
-        depth = oracle_clear(np.empty((S, H, W), np.float32), abi.FORMAT_D32_SFLOAT, draw.clear_depth)
+        ddt = np.uint16 if draw.depth_format == abi.FORMAT_D16_UNORM else np.float32
+        depth = oracle_clear(np.empty((S, H, W), ddt), draw.depth_format, draw.clear_depth)
     if draw.has_stencil:
         stencil = oracle_clear(np.empty((S, H, W), np.uint8), abi.FORMAT_S8_UINT, draw.clear_stencil)
     return color, depth, stencil
@@ -123,7 +125,8 @@ def oracle_render(draw, targets=None, ids=None):
     bpp = color_bytes_per_pixel(draw.color_format)
     buf.color, buf.color_pitch, buf.color_sample_pitch = color.ctypes.data, W * bpp, H * W * bpp
     if depth is not None:
-        buf.depth, buf.depth_pitch, buf.depth_sample_pitch = depth.ctypes.data, W * 4, H * W * 4
+        dbpp = depth_bytes_per_pixel(draw.depth_format)
+        buf.depth, buf.depth_pitch, buf.depth_sample_pitch = depth.ctypes.data, W * dbpp, H * W * dbpp
     if stencil is not None:
         buf.stencil, buf.stencil_pitch, buf.stencil_sample_pitch = stencil.ctypes.data, W, H * W
     occ = np.zeros(1, np.uint64)
@@ -158,18 +161,17 @@ def oracle_clear(target, fmt, value, area=None):
     L = oracle()
     assert target.flags["C_CONTIGUOUS"]
     S, H, W = target.shape[:3]
-    bpp = {abi.FORMAT_R8G8B8A8_UNORM: 4, abi.FORMAT_R32G32B32A32_SFLOAT: 16, abi.FORMAT_D32_SFLOAT: 4,
-           abi.FORMAT_S8_UINT: 1}[fmt]
+    bpp = FORMAT_BYTES[fmt]
     d = abi.ClearDesc()
     d.target = abi.Attachment(target.ctypes.data, fmt, W * bpp, H * W * bpp)
     d.width, d.height, d.samples = W, H, S
     if area is not None:
         d.use_area = 1
         d.area = abi.Rect(*area)
-    if fmt in (abi.FORMAT_R8G8B8A8_UNORM, abi.FORMAT_R32G32B32A32_SFLOAT):
+    if fmt in COLOR_FORMATS:
         for k in range(4):
             d.value.color[k] = float(value[k])
-    elif fmt == abi.FORMAT_D32_SFLOAT:
+    elif fmt in DEPTH_FORMATS:
         d.value.depth_stencil.depth = float(value)
     else:
         d.value.depth_stencil.stencil = int(value) & 0xFFFFFFFF

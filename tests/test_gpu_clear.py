@@ -14,6 +14,9 @@ CASES = [
     (abi.FORMAT_R32G32B32A32_SFLOAT, (4,), np.float32, [0.1, -2.0, 1e30, float("nan")]),
     (abi.FORMAT_D32_SFLOAT, (), np.float32, 1.25),
     (abi.FORMAT_S8_UINT, (), np.uint8, 0x1A5),
+    (abi.FORMAT_B8G8R8A8_UNORM, (4,), np.uint8, [0.25, 1.5, float("nan"), 0.5019608]),
+    (abi.FORMAT_D16_UNORM, (), np.uint16, 0.3337),
+    (abi.FORMAT_D16_UNORM, (), np.uint16, float("nan")),
 ]
 AREAS = [None, (3, 2, 20, 9), (-5, -4, 12, 40), (30, 0, 10, 10), (4, 4, 0, 5)]
 
@@ -25,11 +28,11 @@ def test_clear_matches_the_oracle(case, samples):
     H, W = 23, 37
     rng = np.random.default_rng(case * 10 + samples)
     shape = (samples, H, W) + tail
-    if dtype == np.uint8:
-        host = rng.integers(0, 256, shape, dtype=np.uint8)
+    if dtype in (np.uint8, np.uint16):
+        host = rng.integers(0, np.iinfo(dtype).max + 1, shape, dtype=dtype)
     else:
         host = rng.standard_normal(shape).astype(np.float32)
-    dev = torch.from_numpy(host.copy()).to("cuda:0")
+    dev = torch.from_numpy(host.copy().view(np.int16) if dtype == np.uint16 else host.copy()).to("cuda:0")
     want = host.copy()
     for area in AREAS:
         print(f"[gpu clear] format {fmt} samples {samples} area {area}", flush=True)

@@ -41,7 +41,8 @@ def _compare(name, ref, got):
         bad = rc != gc
     assert not bad.any(), f"{name}: {int(bad.sum())} colour components differ; first at {np.argwhere(bad)[0]}"
     if rz is not None:
-        badz = rz.view(np.uint32) != gz.view(np.uint32)
+        badz = rz.view(np.uint8).reshape(rz.shape + (-1,)) != gz.view(np.uint8).reshape(gz.shape + (-1,))
+        badz = badz.any(-1)
         assert not badz.any(), f"{name}: {int(badz.sum())} depth values differ; first at {np.argwhere(badz)[0]}"
     if rs is not None:
         assert np.array_equal(rs, gs), f"{name}: stencil differs"

@@ -4,7 +4,9 @@ src/Device/Blitter.cpp:55-326) against closed forms.
 RGBA8 packing is pinned by the exact codes k/255 -> k, by its distance to
 the real 255 c (at most one half, so the packing is a rounding to nearest),
 by monotonicity, and by the clamp's special cases; D32F, S8 and RGBA32F by
-bit patterns that a conversion or a clamp would change.
+bit patterns that a conversion or a clamp would change.  D16 (the blit
+routine from a D32F pixel) is pinned the same way as RGBA8, plus the NaN
+case the ordered Min decides; B8G8R8A8 is R8G8B8A8 with R and B exchanged.
 """
 import numpy as np
 
@@ -79,3 +81,43 @@ def test_area_is_clipped_and_covers_every_sample():
     for q in range(4):
         assert (t[q][inside] == 200).all()
         assert (t[q][~inside] == before[q][~inside]).all()
+
+
+def _d16(v):
+    t = np.full((1, 1, 2), 0x1234, np.uint16)
+    oracle_clear(t, abi.FORMAT_D16_UNORM, v)
+    assert t[0, 0, 0] == t[0, 0, 1]
+    return int(t[0, 0, 0])
+
+
+def test_d16_exact_codes_and_rounding():
+    # k / 65535 -> k for every code; the packing is within one half of
+    # 65535 d (plus the float product's half ulp, 2^-9 near 65535) and
+    # monotone: a rounding to nearest.
+    ks = np.arange(65536)
+    got = np.array([_d16(np.float32(k) / np.float32(65535)) for k in ks[::97]] + [_d16(1.0)])
+    assert np.array_equal(got, np.append(ks[::97], 65535))
+    xs = np.linspace(0.0, 1.0, 5001, dtype=np.float32)
+    got = np.array([_d16(x) for x in xs])
+    assert np.all(np.diff(got) >= 0)
+    assert np.max(np.abs(got - 65535.0 * xs.astype(np.float64))) <= 0.5 + 2.0 ** -8
+
+
+def test_d16_clamp_and_special_values():
+    assert _d16(-0.25) == 0
+    assert _d16(3.0) == 65535
+    assert _d16(float("-inf")) == 0
+    assert _d16(float("inf")) == 65535
+    # A NaN passes sw::clamp, then Min(NaN, 0xFFFF) selects 0xFFFF.
+    assert _d16(float("nan")) == 65535
+
+
+def test_bgra8_is_rgba8_with_red_and_blue_exchanged():
+    rng = np.random.default_rng(4)
+    for _ in range(50):
+        c = rng.uniform(-0.2, 1.2, 4).astype(np.float32)
+        a = np.zeros((1, 1, 1, 4), np.uint8)
+        b = np.zeros((1, 1, 1, 4), np.uint8)
+        oracle_clear(a, abi.FORMAT_R8G8B8A8_UNORM, c)
+        oracle_clear(b, abi.FORMAT_B8G8R8A8_UNORM, c)
+        assert list(b[0, 0, 0]) == [a[0, 0, 0, 2], a[0, 0, 0, 1], a[0, 0, 0, 0], a[0, 0, 0, 3]]

@@ -1,0 +1,107 @@
+"""The oracle's D16_UNORM depth and B8G8R8A8_UNORM colour attachments
+(PixelRoutine.cpp:466-481, 506-510, 686-711 for D16; :1091-1110, 2130-2143
+for B8G8R8A8; Renderer.cpp:427-430 for D16's depth bias unit) against
+closed forms.
+
+- A flat quad at depth z stores Round(65535 z), to nearest even.
+- D16 tests quantised depths: two depths that round to one code are equal
+  under LESS and LESS_OR_EQUAL, whatever their float difference.
+- A constant depth bias of -1 on D16 is 1.01 codes: a coplanar redraw then
+  passes LESS wherever the code's fraction keeps clear of a rounding tie,
+  and ten units carry a fraction of .55 across one.
+- B8G8R8A8 renders R8G8B8A8's bytes with R and B exchanged, blending and
+  write masks included.
+"""
+import numpy as np
+
+from oracle_runner import oracle_render
+from swiftshader_amd import abi, scenes
+from swiftshader_amd.scenes import DrawData
+
+W, H = 24, 20
+
+
+def _quad(z):
+    xy = np.array([[-1, -1], [1, -1], [1, 1], [-1, -1], [1, 1], [-1, 1]], np.float64)
+    return np.concatenate([xy, np.full((6, 1), z), np.ones((6, 1))], axis=1).astype(np.float32)
+
+
+def _d16(pos, compare=abi.COMPARE_ALWAYS, bias=None, clear_depth=1.0, colors=None):
+    col = np.ones((len(pos), 4), np.float32) if colors is None else colors
+    d = DrawData(width=W, height=H, positions=pos, colors=col, depth_format=abi.FORMAT_D16_UNORM,
+                 depth_compare=compare, depth_bias=bias, clear_depth=clear_depth)
+    color, depth, _, occ = oracle_render(d)
+    return color, depth, occ
+
+
+def test_flat_depth_is_rounded_to_the_nearest_code():
+    rng = np.random.default_rng(1)
+    zs = list(rng.uniform(0, 1, 40).astype(np.float32))
+    zs += [np.float32((k + 0.5) / 65535.0) for k in (10, 11, 30000, 30001)]  # near ties
+    zs += [np.float32(0.0), np.float32(1.0)]  # outside [0, 1] the quad is clipped away
+    for z in zs:
+        _, depth, _ = _d16(_quad(z))
+        zc = min(max(np.float32(z), np.float32(0)), np.float32(1))
+        want = np.rint(np.float32(zc) * np.float32(65535))  # numpy rint: to nearest even
+        assert depth.dtype == np.uint16 and (depth == want).all(), (z, depth[0, 0, 0], want)
+
+
+def test_depths_of_one_code_are_equal():
+    k = 20000
+    za = np.float32((k - 0.3) / 65535.0)
+    zb = np.float32((k + 0.3) / 65535.0)  # larger in float, the same code
+    assert za < zb
+    quads = np.concatenate([_quad(zb), _quad(za)])
+    _, _, occ_less = _d16(quads, compare=abi.COMPARE_LESS)
+    _, _, occ_leq = _d16(quads, compare=abi.COMPARE_LESS_OR_EQUAL)
+    assert occ_less == W * H  # only the first quad passes against the clear
+    assert occ_leq == 2 * W * H
+
+
+def test_constant_bias_moves_one_code():
+    k = 40000
+    z = np.float32((k + 0.2) / 65535.0)
+    quads = np.concatenate([_quad(z), _quad(z)])
+    d = DrawData(width=W, height=H, positions=quads, colors=np.ones((12, 4), np.float32),
+                 depth_format=abi.FORMAT_D16_UNORM, depth_compare=abi.COMPARE_LESS, depth_bias=(-1.0, 0.0, 0.0))
+    _, depth, _, occ = oracle_render(d)
+    # Both quads are biased by the same -1.01 codes: the second fails LESS.
+    assert occ == W * H and (depth == k - 1).all()
+    first = _d16(_quad(z), compare=abi.COMPARE_LESS)[1]
+    d2 = DrawData(width=W, height=H, positions=_quad(z), colors=np.ones((6, 4), np.float32),
+                  depth_format=abi.FORMAT_D16_UNORM, depth_compare=abi.COMPARE_LESS, depth_bias=(-1.0, 0.0, 0.0))
+    from oracle_runner import host_targets
+    color, _, stencil = host_targets(d2)
+    _, depth2, _, occ2 = oracle_render(d2, targets=(color, first.copy(), stencil))
+    assert occ2 == W * H and (depth2 == k - 1).all() and (first == k).all()
+
+
+def _blend_scene(fmt, seed=2):
+    rng = np.random.default_rng(seed)
+    pos, uv, col = scenes.screen_triangles(rng, 200, W, H, area_hi=300)
+    col = (col * 1.3 - 0.15).astype(np.float32)
+    return DrawData(width=W, height=H, positions=pos, colors=col, shader=abi.SHADER_INTERPOLATE, samples=4,
+                    color_format=fmt, blend_enable=1,
+                    blend=(abi.BLEND_FACTOR_SRC_ALPHA, abi.BLEND_FACTOR_ONE_MINUS_SRC_ALPHA, abi.BLEND_OP_ADD,
+                           abi.BLEND_FACTOR_DST_ALPHA, abi.BLEND_FACTOR_ONE, abi.BLEND_OP_REVERSE_SUBTRACT),
+                    color_write_mask=0xB, clear_color=(0.1, 0.6, 0.3, 0.9), depth_compare=abi.COMPARE_LESS_OR_EQUAL)
+
+
+def test_bgra8_is_rgba8_with_red_and_blue_exchanged():
+    a = oracle_render(_blend_scene(abi.FORMAT_R8G8B8A8_UNORM))
+    b = oracle_render(_blend_scene(abi.FORMAT_B8G8R8A8_UNORM))
+    assert np.array_equal(b[0][..., [2, 1, 0, 3]], a[0])
+    assert np.array_equal(a[1], b[1]) and a[3] == b[3]
+    assert (a[0][..., 2] == a[0][0, 0, 0, 2]).all()  # blue is masked off: still the clear value
+
+
+def test_bias_unit_is_1_01_codes():
+    """r = 1.01 / 65535: ten units move a code fraction of .55 across the
+    tie (k + .55 - 10.1 rounds to k - 10, not k - 9 as a unit of one code
+    would give)."""
+    k = 30000
+    z = np.float32((k + 0.55) / 65535.0)
+    _, depth, _ = _d16(_quad(z), bias=(-10.0, 0.0, 0.0))
+    assert (depth == k - 10).all(), depth[0, 0, 0]
+    _, depth, _ = _d16(_quad(z), bias=(10.0, 0.0, 0.0))  # k + .55 + 10.1 -> k + 11
+    assert (depth == k + 11).all(), depth[0, 0, 0]
