@@ -187,7 +187,7 @@ typedef struct swr_texture { /* combined image sampler, 2D view */
 
 typedef struct swr_attachment {
   void *data;                  /* device pointer to sample 0, pixel (0,0) */
-  uint32_t format;             /* colour: RGBA8 / RGBA32F; depth: D32F; stencil: S8 */
+  uint32_t format;             /* colour: RGBA8 / BGRA8 / RGBA32F; depth: D32F / D16; stencil: S8 */
   uint32_t row_pitch_bytes;
   uint64_t sample_pitch_bytes; /* distance between sample planes (4x MSAA) */
 } swr_attachment;
@@ -289,7 +289,7 @@ typedef struct swr_draw_desc {
 
 typedef struct swr_resolve_desc { /* 4x -> 1x colour resolve */
   uint32_t width, height;
-  uint32_t format;               /* RGBA8 or RGBA32F */
+  uint32_t format;               /* RGBA8, BGRA8 or RGBA32F */
   const void *src;               /* multisampled source, sample 0 plane */
   uint32_t src_row_pitch_bytes;
   uint64_t src_sample_pitch_bytes;
@@ -299,15 +299,15 @@ typedef struct swr_resolve_desc { /* 4x -> 1x colour resolve */
 } swr_resolve_desc;
 
 typedef union swr_clear_value { /* VkClearValue */
-  float color[4];                 /* RGBA8 / RGBA32F targets */
+  float color[4];                 /* RGBA8 / BGRA8 / RGBA32F targets */
   struct {
-    float depth;                  /* D32F targets */
+    float depth;                  /* D32F / D16 targets */
     uint32_t stencil;             /* S8 targets */
   } depth_stencil;
 } swr_clear_value;
 
 typedef struct swr_clear_desc {
-  swr_attachment target;          /* format: RGBA8, RGBA32F, D32F or S8 */
+  swr_attachment target;          /* format: RGBA8, BGRA8, RGBA32F, D32F, D16 or S8 */
   uint32_t width, height;         /* image extent in pixels */
   uint32_t samples;               /* sample planes: 1 or 4 */
   uint32_t use_area;              /* 0: the whole image; 1: `area` (clipped to the image) */
