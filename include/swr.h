@@ -392,10 +392,12 @@ int swr_resolve(const swr_resolve_desc *desc, void *stream);
 
 /* Attachment clear (vk::Image::clear, src/Vulkan/VkImage.cpp:1105-1124, ->
  * sw::Blitter::clear / fastClear, src/Device/Blitter.cpp:55-326): every
- * sample of the area.  RGBA8 channels are clamped to [0, 1] (a NaN passes
- * the clamp and packs as 0) and packed as (uint32_t)(255 * c + 0.5f);
- * RGBA32F is written as given; D32F gets the depth's bits unclamped (the
- * format is not normalised); S8 the stencil's low byte. */
+ * sample of the area.  RGBA8 and BGRA8 channels are clamped to [0, 1] (a
+ * NaN passes the clamp and packs as 0) and packed as (uint32_t)(255 * c +
+ * 0.5f), BGRA8 with R and B exchanged; RGBA32F is written as given; D32F
+ * gets the depth's bits unclamped (the format is not normalised); D16 the
+ * blit routine's Round(65535 * clamp(d)) (a NaN becomes 0xFFFF,
+ * Blitter.cpp:1427-1484); S8 the stencil's low byte. */
 int swr_clear(const swr_clear_desc *desc, void *stream);
 
 /* Prepares `count` draws (as swr_draw_batch would render them) for
