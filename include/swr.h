@@ -59,7 +59,9 @@ extern "C" {
 /* VkFormat */
 #define SWR_FORMAT_UNDEFINED 0
 #define SWR_FORMAT_R8G8B8A8_UNORM 37
+#define SWR_FORMAT_R8G8B8A8_SRGB 43
 #define SWR_FORMAT_B8G8R8A8_UNORM 44
+#define SWR_FORMAT_B8G8R8A8_SRGB 50
 #define SWR_FORMAT_R32_UINT 98
 #define SWR_FORMAT_R32G32B32A32_SFLOAT 109
 #define SWR_FORMAT_D16_UNORM 124

@@ -219,7 +219,8 @@ typedef struct {
   int enableMS;
   unsigned msMask;
   int depthTestActive, depthWriteEnable, depthCompare;
-  int colorBGRA;      /* B8G8R8A8_UNORM colour attachment */
+  int colorBGRA;      /* B8G8R8A8 colour attachment */
+  int colorSRGB;      /* sRGB-encoded colour attachment */
   int depth16;        /* D16_UNORM depth attachment */
   int line;           /* LINE_LIST / LINE_STRIP: primitives are lines */
   int bresenham;      /* lines set up as Bresenham parallelograms */
@@ -897,6 +898,49 @@ static void write_depth(Ctx *c, int q, int x, int y, float z) {
     memcpy(p, &z, 4);
   }
 }
+/* sRGB transfer functions of the legacy-precision build: linearToSRGB and
+ * sRGBtoLinear (ShaderCore.cpp:673-689) over Pow<Mediump> = Exp2(Log2(x) * y)
+ * (:472-477), Exp2's clamp (:335-346) and the legacy Exp2 / Log2
+ * approximations (:300-314, :388-402).  MulAdd is unfused; RoundInt rounds
+ * to nearest even; Min / Max are ordered selects. */
+static float log2_legacy(float x) {
+  const uint32_t ix = f2u(x);
+  float x1 = u2f(((ix & 0x7F800000u) >> 8) | 0x3F800000u);
+  x1 = (x1 - 1.4960938f) * 256.0f;
+  const float x0 = u2f((ix & 0x007FFFFFu) | 0x3F800000u);
+  const float x2 = (9.5428179e-2f * x0 + 4.7779095e-1f) * x0 + 1.9782813e-1f;
+  const float x3 = ((1.6618466e-2f * x0 + 2.0350508e-1f) * x0 + 2.7382900e-1f) * x0 + 4.0496687e-2f;
+  x1 += (x0 - 1.0f) * (x2 / x3);
+  return ix == 0x7F800000u ? x : x1;
+}
+static float exp2_legacy(float x) {
+  float x0 = rr_min(x, 128.0f);
+  x0 = rr_max(x0, u2f(0xC2FDFFFFu));
+  const int32_t i = rr_round_int(x0 - 0.5f);
+  const float ii = u2f((uint32_t)(i + 127) << 23);
+  const float f = x0 - (float)i;
+  float ff = u2f(0x3AF61905u);
+  ff = ff * f + u2f(0x3C134806u);
+  ff = ff * f + u2f(0x3D64AA23u);
+  ff = ff * f + u2f(0x3E75EAD4u);
+  ff = ff * f + u2f(0x3F31727Bu);
+  ff = ff * f + 1.0f;
+  return ii * ff;
+}
+static float pow_mediump(float x, float y) { return exp2_legacy(log2_legacy(x) * y); }
+static float linear_to_srgb(float c) {
+  const float lc = c * 12.92f;
+  const float ec = 1.055f * pow_mediump(c, 1.0f / 2.4f) + -0.055f;
+  return c < 0.0031308f ? lc : ec;
+}
+static float srgb_to_linear(float c) {
+  const float lc = c * (1.0f / 12.92f);
+  const float ec = pow_mediump(c * (1.0f / 1.055f) + 0.055f / 1.055f, 2.4f);
+  return c < 0.04045f ? lc : ec;
+}
+EXPORT float swo_linear_to_srgb(float c) { return linear_to_srgb(c); }
+EXPORT float swo_srgb_to_linear(float c) { return srgb_to_linear(c); }
+
 /* Memory byte of colour channel k: B8G8R8A8 swaps R and B (PixelRoutine.cpp
  * readPixel :1091-1110, writeColor :2130-2143 with the write mask flipped to
  * memory order, so the logical mask still selects logical channels). */
@@ -971,6 +1015,9 @@ static void alpha_blend(Ctx *c, const uint8_t *buf, const float src[4], float ou
     memcpy(dst, buf, 16);
   } else {
     for (int k = 0; k < 4; k++) dst[k] = (float)(buf[chan(c, k)] * 257) * (1.0f / 65535.0f);
+    /* an sRGB destination is blended in linear space (PixelRoutine.cpp:1811-1827) */
+    if (c->colorSRGB)
+      for (int k = 0; k < 3; k++) dst[k] = srgb_to_linear(dst[k]);
   }
   float sf[4], df[4];
   for (int k = 0; k < 3; k++) {
@@ -991,7 +1038,9 @@ static void write_color(Ctx *c, uint8_t *buf, const float col[4]) {
       if (wm & (1 << k)) p[k] = col[k];
   } else {
     for (int k = 0; k < 4; k++) {
-      float v = rr_min(rr_max(col[k], 0.0f), 1.0f);
+      /* writeColor converts R, G, B of an sRGB target first (PixelRoutine.cpp:1963-1969) */
+      float v = c->colorSRGB && k < 3 ? linear_to_srgb(col[k]) : col[k];
+      v = rr_min(rr_max(v, 0.0f), 1.0f);
       int32_t iv = rr_round_int(v * 255.0f);
       uint8_t b = (uint8_t)(iv < 0 ? 0 : (iv > 255 ? 255 : iv)); /* PackUnsigned saturation */
       if (wm & (1 << k)) buf[chan(c, k)] = b;
@@ -1300,7 +1349,8 @@ static int init_ctx(Ctx *c, const swr_draw_desc *d) {
   c->stencilActive = d->stencil.data && d->stencil_test_enable;
 
   c->colorIsFloat = d->color.format == SWR_FORMAT_R32G32B32A32_SFLOAT;
-  c->colorBGRA = d->color.format == SWR_FORMAT_B8G8R8A8_UNORM;
+  c->colorBGRA = d->color.format == SWR_FORMAT_B8G8R8A8_UNORM || d->color.format == SWR_FORMAT_B8G8R8A8_SRGB;
+  c->colorSRGB = d->color.format == SWR_FORMAT_R8G8B8A8_SRGB || d->color.format == SWR_FORMAT_B8G8R8A8_SRGB;
   c->depth16 = d->depth.data && d->depth.format == SWR_FORMAT_D16_UNORM;
   if (d->primitive_id.data && (!c->colorIsFloat || d->primitive_id.format != SWR_FORMAT_R32_UINT))
     return SWR_ERROR_UNSUPPORTED;
@@ -1572,6 +1622,20 @@ static uint16_t clear_d16(float d) {
   return (uint16_t)rr_round_int(v);
 }
 
+/* An sRGB clear runs the blit routine from an RGBA32F pixel with sRGB
+ * conversion (Blitter.cpp:64-86, 1427-1484, 647-661, 676-690): the clamp to
+ * [0, 1] (a NaN passes), linearToSRGB of R, G, B, the 255 scale, Min(., 255)
+ * and Max(., 0) as ordered selects, then RoundInt to nearest even. */
+static uint32_t clear_srgb8(float c, int k) {
+  if (c < 0.0f) c = 0.0f;
+  if (c > 1.0f) c = 1.0f;
+  float v = k < 3 ? linear_to_srgb(c) : c;
+  v = v * 255.0f;
+  v = rr_min(v, 255.0f);
+  v = rr_max(v, 0.0f);
+  return (uint32_t)rr_round_int(v);
+}
+
 EXPORT int swo_clear(const swr_clear_desc *d) {
   if (!d || !d->target.data || (d->samples != 1 && d->samples != 4)) return SWR_ERROR_INVALID_ARGUMENT;
   uint32_t bpp;
@@ -1588,6 +1652,13 @@ EXPORT int swo_clear(const swr_clear_desc *d) {
       uint32_t p = clear_unorm8(d->value.color[2]) | (clear_unorm8(d->value.color[1]) << 8) |
                    (clear_unorm8(d->value.color[0]) << 16) | (clear_unorm8(d->value.color[3]) << 24);
       memcpy(bytes, &p, 4);
+      bpp = 4;
+      break;
+    }
+    case SWR_FORMAT_R8G8B8A8_SRGB:
+    case SWR_FORMAT_B8G8R8A8_SRGB: {
+      const int bgra = d->target.format == SWR_FORMAT_B8G8R8A8_SRGB;
+      for (int k = 0; k < 4; k++) bytes[bgra && (k & 1) == 0 ? 2 - k : k] = (uint8_t)clear_srgb8(d->value.color[k], k);
       bpp = 4;
       break;
     }
@@ -1630,6 +1701,10 @@ EXPORT int swo_clear(const swr_clear_desc *d) {
 
 EXPORT int swo_resolve(const swr_resolve_desc *r) {
   if (!r || r->src_samples != 4) return SWR_ERROR_UNSUPPORTED;
+  /* sRGB sources resolve through the blit routine in linear space, not built */
+  if (r->format != SWR_FORMAT_R8G8B8A8_UNORM && r->format != SWR_FORMAT_B8G8R8A8_UNORM &&
+      r->format != SWR_FORMAT_R32G32B32A32_SFLOAT)
+    return SWR_ERROR_UNSUPPORTED;
   unsigned csr = _mm_getcsr();
   _mm_setcsr(csr | 0x8040);
   for (uint32_t y = 0; y < r->height; y++) {

@@ -17,7 +17,9 @@ ERROR_DEVICE = -4
 
 FORMAT_UNDEFINED = 0
 FORMAT_R8G8B8A8_UNORM = 37
+FORMAT_R8G8B8A8_SRGB = 43
 FORMAT_B8G8R8A8_UNORM = 44
+FORMAT_B8G8R8A8_SRGB = 50
 FORMAT_R32_UINT = 98
 FORMAT_R32G32B32A32_SFLOAT = 109
 FORMAT_D16_UNORM = 124

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "../../include/swr.h"
+#include "swr_srgb.hpp"
 #include "swr_kernels.hip"
 #include "swr_batch.hip"
 #include "swr_grad.hip"
@@ -340,10 +341,12 @@ int build_params(const swr_draw_desc *d, const swr_tuning &tuning, DrawParams &P
   // Colour output and blend state (Context.cpp:1090-1311).
   bool hasColor = d->color.data && d->color.format != SWR_FORMAT_UNDEFINED;
   if (hasColor && d->color.format != SWR_FORMAT_R8G8B8A8_UNORM && d->color.format != SWR_FORMAT_B8G8R8A8_UNORM &&
+      d->color.format != SWR_FORMAT_R8G8B8A8_SRGB && d->color.format != SWR_FORMAT_B8G8R8A8_SRGB &&
       d->color.format != SWR_FORMAT_R32G32B32A32_SFLOAT)
     return fail(SWR_ERROR_UNSUPPORTED, "colour format %u", d->color.format);
   P.colorFloat = d->color.format == SWR_FORMAT_R32G32B32A32_SFLOAT;
-  P.colorBGRA = hasColor && d->color.format == SWR_FORMAT_B8G8R8A8_UNORM;
+  P.colorBGRA = hasColor && (d->color.format == SWR_FORMAT_B8G8R8A8_UNORM || d->color.format == SWR_FORMAT_B8G8R8A8_SRGB);
+  P.colorSRGB = hasColor && (d->color.format == SWR_FORMAT_R8G8B8A8_SRGB || d->color.format == SWR_FORMAT_B8G8R8A8_SRGB);
   const swr_blend_state &b = d->blend;
   if (!valid_factor(b.src_color_blend_factor) || !valid_factor(b.dst_color_blend_factor) ||
       !valid_factor(b.src_alpha_blend_factor) || !valid_factor(b.dst_alpha_blend_factor) ||
@@ -1263,6 +1266,20 @@ uint32_t d16_clear(float d) {
   v = v > 0.0f ? v : 0.0f;
   return (uint32_t)std::nearbyint(v) & 0xFFFFu;
 }
+
+// An sRGB clear runs the blit routine from an RGBA32F pixel with sRGB
+// conversion (Blitter.cpp:64-86, 1427-1484, 647-661, 676-690): the clamp
+// above, linearToSRGB of R, G, B, the 255 scale, Min(., 255) and Max(., 0)
+// as ordered selects, then RoundInt to nearest even.
+uint32_t srgb8_clear(float c, int k) {
+  if (c < 0.0f) c = 0.0f;
+  if (c > 1.0f) c = 1.0f;
+  float v = k < 3 ? swr::linear_to_srgb(c) : c;
+  v = v * 255.0f;
+  v = v < 255.0f ? v : 255.0f;
+  v = v > 0.0f ? v : 0.0f;
+  return (uint32_t)std::nearbyint(v);
+}
 }  // namespace
 
 int swr_clear(const swr_clear_desc *d, void *stream) {
@@ -1281,6 +1298,13 @@ int swr_clear(const swr_clear_desc *d, void *stream) {
       v.x = unorm8_clear(d->value.color[2]) | (unorm8_clear(d->value.color[1]) << 8) |
             (unorm8_clear(d->value.color[0]) << 16) | (unorm8_clear(d->value.color[3]) << 24);
       break;
+    case SWR_FORMAT_R8G8B8A8_SRGB:
+    case SWR_FORMAT_B8G8R8A8_SRGB: {
+      bpp = 4;
+      const int bgra = d->target.format == SWR_FORMAT_B8G8R8A8_SRGB;
+      for (int k = 0; k < 4; k++) v.x |= srgb8_clear(d->value.color[k], k) << (8 * (bgra && (k & 1) == 0 ? 2 - k : k));
+      break;
+    }
     case SWR_FORMAT_R32G32B32A32_SFLOAT:
       bpp = 16;
       memcpy(&v, d->value.color, 16);

@@ -24,6 +24,7 @@
 
 #include "swr_device.hpp"
 #include "swr_params.hpp"
+#include "swr_srgb.hpp"
 
 namespace swr {
 
@@ -726,8 +727,18 @@ __device__ __forceinline__ void blend_write(const DrawParams &P, PixelState<MS, 
       const uint32_t p = S.cb[FCOL ? 0 : q];
       dst = make_float4(unorm8_to_float(p & 0xFFu), unorm8_to_float((p >> 8) & 0xFFu),
                         unorm8_to_float((p >> 16) & 0xFFu), unorm8_to_float(p >> 24));
+      if (P.colorSRGB && P.blendEnable) {  // blended in linear space (PixelRoutine.cpp:1811-1827)
+        dst.x = srgb_to_linear(dst.x);
+        dst.y = srgb_to_linear(dst.y);
+        dst.z = srgb_to_linear(dst.z);
+      }
     }
-    const float4 o = P.blendEnable ? alpha_blend(P, s, dst) : s;
+    float4 o = P.blendEnable ? alpha_blend(P, s, dst) : s;
+    if (!FCOL && P.colorSRGB) {  // writeColor encodes R, G, B first (PixelRoutine.cpp:1963-1969)
+      o.x = linear_to_srgb(o.x);
+      o.y = linear_to_srgb(o.y);
+      o.z = linear_to_srgb(o.z);
+    }
     if (FCOL) {
       float4 &c = S.cf[FCOL ? q : 0];
       if (P.colorMask & 1) c.x = o.x;

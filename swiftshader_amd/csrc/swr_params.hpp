@@ -75,7 +75,8 @@ struct DrawParams {
   // Colour.
   int colorMask;   // canonical 4-bit write mask; 0 = no colour output
   int colorFloat;  // RGBA32F target
-  int colorBGRA;   // B8G8R8A8_UNORM target (R and B exchanged in memory)
+  int colorBGRA;   // B8G8R8A8 target (R and B exchanged in memory)
+  int colorSRGB;   // sRGB-encoded 8-bit target
   int blendEnable;
   int srcF, dstF, colorOp, srcFA, dstFA, alphaOp;
   float blendConst[4], invBlendConst[4];

@@ -53,9 +53,11 @@ def depth_bytes_per_pixel(fmt: int) -> int:
 
 
 # Bytes per pixel of every attachment format the library clears.
-FORMAT_BYTES = {abi.FORMAT_R8G8B8A8_UNORM: 4, abi.FORMAT_B8G8R8A8_UNORM: 4, abi.FORMAT_R32G32B32A32_SFLOAT: 16,
-                abi.FORMAT_D16_UNORM: 2, abi.FORMAT_D32_SFLOAT: 4, abi.FORMAT_S8_UINT: 1}
-COLOR_FORMATS = (abi.FORMAT_R8G8B8A8_UNORM, abi.FORMAT_B8G8R8A8_UNORM, abi.FORMAT_R32G32B32A32_SFLOAT)
+FORMAT_BYTES = {abi.FORMAT_R8G8B8A8_UNORM: 4, abi.FORMAT_B8G8R8A8_UNORM: 4, abi.FORMAT_R8G8B8A8_SRGB: 4,
+                abi.FORMAT_B8G8R8A8_SRGB: 4, abi.FORMAT_R32G32B32A32_SFLOAT: 16, abi.FORMAT_D16_UNORM: 2,
+                abi.FORMAT_D32_SFLOAT: 4, abi.FORMAT_S8_UINT: 1}
+COLOR_FORMATS = (abi.FORMAT_R8G8B8A8_UNORM, abi.FORMAT_B8G8R8A8_UNORM, abi.FORMAT_R8G8B8A8_SRGB,
+                 abi.FORMAT_B8G8R8A8_SRGB, abi.FORMAT_R32G32B32A32_SFLOAT)
 DEPTH_FORMATS = (abi.FORMAT_D16_UNORM, abi.FORMAT_D32_SFLOAT)
 
 

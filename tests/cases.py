@@ -294,7 +294,8 @@ def _points(seed=41, size=(80, 64), n=700, samples=1, sizes=True, indexed=False,
                     depth_compare=abi.COMPARE_LESS_OR_EQUAL, **extra)
 
 
-def _formats(seed=44, d16=True, bgra=False, samples=1, bias=None, stencil=False, textured=False, blend=False):
+def _formats(seed=44, d16=True, bgra=False, samples=1, bias=None, stencil=False, textured=False, blend=False,
+             srgb=False):
     """Overlapping triangles into a D16 depth and/or B8G8R8A8 colour
     attachment (optionally blended with a partial write mask)."""
     rng = np.random.default_rng(seed)
@@ -321,7 +322,8 @@ def _formats(seed=44, d16=True, bgra=False, samples=1, bias=None, stencil=False,
                                compare_op=abi.COMPARE_ALWAYS, reference=3))
     return DrawData(width=W, height=H, positions=pos, colors=col, texcoords=uv, samples=samples, shader=shader,
                     texture=tex, depth_format=abi.FORMAT_D16_UNORM if d16 else abi.FORMAT_D32_SFLOAT,
-                    color_format=abi.FORMAT_B8G8R8A8_UNORM if bgra else abi.FORMAT_R8G8B8A8_UNORM,
+                    color_format=((abi.FORMAT_B8G8R8A8_SRGB if bgra else abi.FORMAT_R8G8B8A8_SRGB) if srgb else
+                                  (abi.FORMAT_B8G8R8A8_UNORM if bgra else abi.FORMAT_R8G8B8A8_UNORM)),
                     depth_compare=abi.COMPARE_LESS, depth_bias=bias, clear_depth=0.75, **extra)
 
 
@@ -372,4 +374,8 @@ def parity_cases():
         ("d16_msaa4_bias_stencil", _formats(seed=45, samples=4, bias=(-3.0, 0.0, 1.25), stencil=True)),
         ("bgra8_blend_mask", _formats(seed=46, d16=False, bgra=True, blend=True)),
         ("bgra8_d16_msaa4_textured", _formats(seed=47, bgra=True, samples=4, textured=True)),
+        ("srgb8_blend_mask", _formats(seed=48, d16=False, srgb=True, blend=True)),
+        ("bgra8_srgb_msaa4_textured_blend", _formats(seed=49, bgra=True, srgb=True, samples=4, textured=True,
+                                                    blend=True)),
+        ("srgb8_opaque_d16", _formats(seed=50, srgb=True)),
     ]

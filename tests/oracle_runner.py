@@ -44,7 +44,7 @@ def oracle():
         for f in ("swo_mulhigh_u16x2", "swo_mulhigh_s16x2"):
             getattr(L, f).argtypes = [ctypes.c_uint32, ctypes.c_uint32]
             getattr(L, f).restype = ctypes.c_uint32
-        for f in ("swo_log2sqrt", "swo_frac"):
+        for f in ("swo_log2sqrt", "swo_frac", "swo_linear_to_srgb", "swo_srgb_to_linear"):
             getattr(L, f).argtypes = [ctypes.c_float]
             getattr(L, f).restype = ctypes.c_float
         L.swo_constants.restype = None

@@ -17,6 +17,8 @@ CASES = [
     (abi.FORMAT_B8G8R8A8_UNORM, (4,), np.uint8, [0.25, 1.5, float("nan"), 0.5019608]),
     (abi.FORMAT_D16_UNORM, (), np.uint16, 0.3337),
     (abi.FORMAT_D16_UNORM, (), np.uint16, float("nan")),
+    (abi.FORMAT_R8G8B8A8_SRGB, (4,), np.uint8, [0.0021, 0.5, float("nan"), 0.25]),
+    (abi.FORMAT_B8G8R8A8_SRGB, (4,), np.uint8, [0.8, -0.1, 0.04, 1.2]),
 ]
 AREAS = [None, (3, 2, 20, 9), (-5, -4, 12, 40), (30, 0, 10, 10), (4, 4, 0, 5)]
 
@@ -41,3 +43,13 @@ def test_clear_matches_the_oracle(case, samples):
         torch.cuda.synchronize()
         got = dev.cpu().numpy()
         assert np.array_equal(got.view(np.uint8), want.view(np.uint8)), area
+
+
+def test_srgb_resolve_is_refused():
+    """An sRGB source resolves through the reference's blit routine in linear
+    space, which is not built; the library says so instead of averaging codes."""
+    src = torch.zeros((4, 8, 8, 4), dtype=torch.uint8, device="cuda:0")
+    dst = torch.zeros((1, 8, 8, 4), dtype=torch.uint8, device="cuda:0")
+    r = abi.ResolveDesc(8, 8, abi.FORMAT_R8G8B8A8_SRGB, src.data_ptr(), 32, 256, 4, dst.data_ptr(), 32)
+    import ctypes
+    assert runtime.lib().swr_resolve(ctypes.byref(r), None) == abi.ERROR_UNSUPPORTED

@@ -1,0 +1,127 @@
+"""The oracle's sRGB colour attachments (ShaderCore.cpp:300-314, 388-402,
+472-477, 673-689; PixelRoutine.cpp:1811-1827, 1963-1969; Blitter.cpp clear
+path) against the sRGB transfer function in FP64.
+
+- linearToSRGB and sRGBtoLinear, restated from the legacy Exp2 / Log2
+  approximations, stay within 1e-6 of the exact curves over [0, 1] (the
+  approximations reach 2.4e-7 and 6.1e-7; a wrong coefficient or constant
+  moves them by orders of magnitude), and are exact below the linear
+  segment's knee.
+- Every 8-bit code survives decode, encode and quantisation.
+- A flat colour written to an sRGB target is the exact encoding's nearest
+  code (away from rounding ties); blending happens in linear space.
+- R8G8B8A8_SRGB and B8G8R8A8_SRGB differ by the R / B exchange only; clears
+  encode R, G, B and leave alpha linear.
+"""
+import numpy as np
+
+from oracle_runner import oracle, oracle_clear, oracle_render
+from swiftshader_amd import abi
+from swiftshader_amd.scenes import DrawData
+
+
+def _enc(c):
+    c = np.asarray(c, np.float64)
+    return np.where(c < 0.0031308, 12.92 * c, 1.055 * np.power(np.maximum(c, 0), 1 / 2.4) - 0.055)
+
+
+def _dec(c):
+    c = np.asarray(c, np.float64)
+    return np.where(c < 0.04045, c / 12.92, np.power((c + 0.055) / 1.055, 2.4))
+
+
+def test_transfer_functions_match_the_curves():
+    L = oracle()
+    xs = np.linspace(0.0, 1.0, 40001, dtype=np.float32)
+    enc = np.array([L.swo_linear_to_srgb(float(x)) for x in xs])
+    dec = np.array([L.swo_srgb_to_linear(float(x)) for x in xs])
+    assert np.abs(enc - _enc(xs)).max() < 1e-6
+    assert np.abs(dec - _dec(xs)).max() < 1e-6
+    lo = xs[xs < 0.0031308]
+    assert all(L.swo_linear_to_srgb(float(x)) == np.float32(x * np.float32(12.92)) for x in lo[:200])
+
+
+def test_every_code_round_trips():
+    L = oracle()
+    for k in range(256):
+        lin = L.swo_srgb_to_linear(float(np.float32(k) / np.float32(255)))
+        back = L.swo_linear_to_srgb(lin)
+        assert int(np.rint(min(max(back, 0.0), 1.0) * 255)) == k
+
+
+W, H = 16, 12
+
+
+def _quad(colors, z=0.5):
+    xy = np.array([[-1, -1], [1, -1], [1, 1], [-1, -1], [1, 1], [-1, 1]], np.float64)
+    pos = np.concatenate([xy, np.full((6, 1), z), np.ones((6, 1))], axis=1).astype(np.float32)
+    return pos, np.repeat(np.asarray(colors, np.float32)[None], 6, 0)
+
+
+def test_flat_colours_take_the_nearest_code():
+    rng = np.random.default_rng(2)
+    checked = 0
+    for _ in range(40):
+        c = rng.uniform(0, 1, 4).astype(np.float32)
+        pos, col = _quad(c)
+        d = DrawData(width=W, height=H, positions=pos, colors=col, color_format=abi.FORMAT_R8G8B8A8_SRGB)
+        got = oracle_render(d)[0][0, 0, 0].astype(int)
+        want = np.append(_enc(c[:3]), c[3]) * 255.0
+        for k in range(4):
+            if abs(want[k] - np.floor(want[k]) - 0.5) > 0.01:
+                assert got[k] == int(np.rint(want[k])), (c, k, got, want)
+                checked += 1
+    assert checked > 140
+
+
+def test_blending_happens_in_linear_space():
+    a = np.array([0.8, 0.1, 0.5, 1.0], np.float32)
+    b = np.array([0.05, 0.9, 0.3, 0.5], np.float32)
+    pa, ca = _quad(a, 0.6)
+    pb, cb = _quad(b, 0.4)
+    d = DrawData(width=W, height=H, positions=np.concatenate([pa, pb]), colors=np.concatenate([ca, cb]),
+                 color_format=abi.FORMAT_R8G8B8A8_SRGB, blend_enable=1, depth_compare=abi.COMPARE_ALWAYS,
+                 blend=(abi.BLEND_FACTOR_SRC_ALPHA, abi.BLEND_FACTOR_ONE_MINUS_SRC_ALPHA, abi.BLEND_OP_ADD,
+                        abi.BLEND_FACTOR_ONE, abi.BLEND_FACTOR_ZERO, abi.BLEND_OP_ADD))
+    got = oracle_render(d)[0][0, 0, 0].astype(int)
+    first = np.rint(_enc(a[:3]) * 255)              # codes after the first quad
+    lin = 0.5 * b[:3] + 0.5 * _dec(first / 255.0)   # the blend of linear values
+    want = _enc(lin) * 255
+    assert np.all(np.abs(got[:3] - want) <= 0.51), (got, want)
+    assert got[3] == int(np.rint(b[3] * 255))
+    srgb_space = (0.5 * _enc(b[:3]) + 0.5 * first / 255.0) * 255  # the mistake: blending the codes
+    assert np.abs(got[:3] - srgb_space).max() > 5
+
+
+def test_bgra_srgb_is_rgba_srgb_with_red_and_blue_exchanged():
+    rng = np.random.default_rng(3)
+    pos = rng.uniform(-1, 1, (90, 4)).astype(np.float32)
+    pos[:, 2] = rng.uniform(0.1, 0.9, 90)
+    pos[:, 3] = 1.0
+    col = rng.uniform(-0.1, 1.1, (90, 4)).astype(np.float32)
+    out = []
+    for fmt in (abi.FORMAT_R8G8B8A8_SRGB, abi.FORMAT_B8G8R8A8_SRGB):
+        d = DrawData(width=W, height=H, positions=pos, colors=col, color_format=fmt, blend_enable=1, samples=4,
+                     blend=(abi.BLEND_FACTOR_SRC_ALPHA, abi.BLEND_FACTOR_ONE_MINUS_SRC_ALPHA, abi.BLEND_OP_ADD,
+                            abi.BLEND_FACTOR_ONE, abi.BLEND_FACTOR_ONE_MINUS_SRC_ALPHA, abi.BLEND_OP_ADD),
+                     clear_color=(0.3, 0.6, 0.9, 0.5), depth_compare=abi.COMPARE_LESS_OR_EQUAL)
+        out.append(oracle_render(d)[0])
+    assert np.array_equal(out[1][..., [2, 1, 0, 3]], out[0])
+
+
+def test_clear_encodes_colour_and_leaves_alpha_linear():
+    rng = np.random.default_rng(4)
+    for _ in range(60):
+        c = rng.uniform(0, 1, 4).astype(np.float32)
+        t = np.zeros((1, 1, 1, 4), np.uint8)
+        oracle_clear(t, abi.FORMAT_R8G8B8A8_SRGB, c)
+        want = np.append(_enc(c[:3]), c[3]) * 255.0
+        for k in range(4):
+            if abs(want[k] - np.floor(want[k]) - 0.5) > 0.01:
+                assert t[0, 0, 0, k] == int(np.rint(want[k])), (c, k)
+        u = np.zeros((1, 1, 1, 4), np.uint8)
+        oracle_clear(u, abi.FORMAT_B8G8R8A8_SRGB, c)
+        assert list(u[0, 0, 0]) == [t[0, 0, 0, 2], t[0, 0, 0, 1], t[0, 0, 0, 0], t[0, 0, 0, 3]]
+    t = np.zeros((1, 1, 1, 4), np.uint8)
+    oracle_clear(t, abi.FORMAT_R8G8B8A8_SRGB, [float("nan"), -1.0, 2.0, float("nan")])
+    assert list(t[0, 0, 0]) == [255, 0, 255, 255]  # NaN passes the clamp; Min(NaN, 255) selects 255
