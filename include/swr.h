@@ -189,7 +189,7 @@ typedef struct swr_texture { /* combined image sampler, 2D view */
 
 typedef struct swr_attachment {
   void *data;                  /* device pointer to sample 0, pixel (0,0) */
-  uint32_t format;             /* colour: RGBA8 / BGRA8 / RGBA32F; depth: D32F / D16; stencil: S8 */
+  uint32_t format;             /* colour: RGBA8 / BGRA8 (UNORM, sRGB) / RGBA32F; depth: D32F / D16; stencil: S8 */
   uint32_t row_pitch_bytes;
   uint64_t sample_pitch_bytes; /* distance between sample planes (4x MSAA) */
 } swr_attachment;
@@ -309,7 +309,7 @@ typedef union swr_clear_value { /* VkClearValue */
 } swr_clear_value;
 
 typedef struct swr_clear_desc {
-  swr_attachment target;          /* format: RGBA8, BGRA8, RGBA32F, D32F, D16 or S8 */
+  swr_attachment target;          /* format: any attachment format swr_draw takes */
   uint32_t width, height;         /* image extent in pixels */
   uint32_t samples;               /* sample planes: 1 or 4 */
   uint32_t use_area;              /* 0: the whole image; 1: `area` (clipped to the image) */
@@ -396,7 +396,9 @@ int swr_resolve(const swr_resolve_desc *desc, void *stream);
  * sw::Blitter::clear / fastClear, src/Device/Blitter.cpp:55-326): every
  * sample of the area.  RGBA8 and BGRA8 channels are clamped to [0, 1] (a
  * NaN passes the clamp and packs as 0) and packed as (uint32_t)(255 * c +
- * 0.5f), BGRA8 with R and B exchanged; RGBA32F is written as given; D32F
+ * 0.5f), BGRA8 with R and B exchanged; the sRGB formats through the blit
+ * routine (clamp, linearToSRGB of R, G, B, Round(255 * c), a NaN becoming
+ * 255); RGBA32F is written as given; D32F
  * gets the depth's bits unclamped (the format is not normalised); D16 the
  * blit routine's Round(65535 * clamp(d)) (a NaN becomes 0xFFFF,
  * Blitter.cpp:1427-1484); S8 the stencil's low byte. */
