@@ -404,6 +404,27 @@ int swr_resolve(const swr_resolve_desc *desc, void *stream);
  * Blitter.cpp:1427-1484); S8 the stencil's low byte. */
 int swr_clear(const swr_clear_desc *desc, void *stream);
 
+/* Compressed-texture decoding: BC_Decoder::Decode (src/Device/
+ * BC_Decoder.hpp:29), which vk::Image::decodeBC calls per mip level and
+ * layer (src/Vulkan/VkImage.cpp:1381-1399) to fill the decompressed image
+ * the sampler reads.  `src` holds ceil(width / 4) x ceil(height / 4) blocks
+ * in row-major order (8 bytes for BC1 and BC4, 16 otherwise); pixels
+ * outside the image are not written.  BC1-BC3 write whole B8G8R8A8 pixels
+ * (dst and its pitch 4-byte aligned, dst_bytes_per_pixel 4); BC4 writes
+ * byte 0 and BC5 bytes 0 and 1 of each pixel, leaving the others as they
+ * are.  n is 1..5; BC6H and BC7 return SWR_ERROR_UNSUPPORTED. */
+typedef struct swr_bc_desc {
+  const void *src;                /* device pointer, 8-byte aligned */
+  void *dst;                      /* device pointer to pixel (0, 0) */
+  uint32_t width, height;
+  uint32_t dst_row_pitch_bytes;
+  uint32_t dst_bytes_per_pixel;
+  uint32_t n;                     /* the n of BCn */
+  uint32_t no_alpha_or_unsigned;  /* BC1: RGB without alpha; BC4 / BC5: UNORM (else SNORM) */
+} swr_bc_desc;
+
+int swr_decode_bc(const swr_bc_desc *desc, void *stream);
+
 /* Prepares `count` draws (as swr_draw_batch would render them) for
  * repeated runs; pair_headroom_percent (e.g. 10) over-provisions the pair
  * buffers beyond the planned geometry.  Synchronises `stream`. */

@@ -1,12 +1,14 @@
 // ref_harness.cpp -- C entry points into reference sources compiled as-is.
 //
 // TEST INFRASTRUCTURE ONLY (see oracle/Makefile, target `ref`).  This file
-// is this project's own code; it is linked with three unmodified reference
+// is this project's own code; it is linked with unmodified reference
 // translation units that compile standalone with g++ and the reference's
 // own include directories:
 //   src/Device/Clipper.cpp    (Sutherland-Hodgman clipping)
 //   src/Pipeline/Constants.cpp (sample offsets and anisotropy tables)
 //   src/System/Half.cpp       (needed by Constants.cpp)
+//   src/Device/BC_Decoder.cpp (BC1-BC7 block decoding)
+//   src/System/Debug.cpp      (needed by BC_Decoder.cpp)
 // The result, oracle/_ref/libswr_ref.so, lets the tests check the oracle's
 // clipper bit-exactly and its constant tables value by value.  The rest of
 // the draw path is JIT-generated through Reactor/Subzero and needs the
@@ -16,6 +18,7 @@
 #include <new>
 #include <xmmintrin.h>
 
+#include "Device/BC_Decoder.hpp"
 #include "Device/Clipper.hpp"
 #include "Device/Polygon.hpp"
 #include "Device/Renderer.hpp"
@@ -60,6 +63,12 @@ __attribute__((visibility("default"))) void ref_constants(int *xf, int *yf, int 
     uvweight[n] = constants->uvWeight[n][0];
     uvstart[n] = constants->uvStart[n][0];
   }
+}
+
+// BC_Decoder::Decode as the reference's Image::decodeBC calls it.
+__attribute__((visibility("default"))) int ref_decode_bc(const uint8_t *src, uint8_t *dst, int w, int h, int dstPitch,
+                                                          int dstBpp, int n, int isNoAlphaU) {
+  return BC_Decoder::Decode(src, dst, w, h, dstPitch, dstBpp, n, isNoAlphaU != 0) ? 1 : 0;
 }
 
 }  // extern "C"

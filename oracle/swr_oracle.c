@@ -204,6 +204,7 @@ typedef struct {
   uint16_t uHalf[SWR_MAX_MIP_LEVELS], vHalf[SWR_MAX_MIP_LEVELS];
   float wwhh[4];
   int isFloat;
+  int bgra; /* B8G8R8A8_UNORM texels */
   /* sw::Sampler subset */
   int filter, mipmap, addrU, addrV;
   float mipLodBias, maxAnisotropy, minLod, maxLod;
@@ -824,7 +825,8 @@ static void sample_texture(const OTexture *t, const float u[4], const float v[4]
         for (int k = 0; k < 4; k++) c[k] = (uint16_t)(mulhi_u16(c[k], inv) + mulhi_u16(cc[k], utri));
       }
       const float scale = 1.0f / 65280.0f;
-      for (int k = 0; k < 4; k++) out[lane][k] = (float)c[k] * scale;
+      /* B8G8R8A8 texels: the channels filter alike, R and B come back exchanged */
+      for (int k = 0; k < 4; k++) out[lane][k] = (float)c[t->bgra && (k & 1) == 0 ? 2 - k : k] * scale;
     }
   }
 }
@@ -1384,6 +1386,7 @@ static int init_ctx(Ctx *c, const swr_draw_desc *d) {
     const swr_sampler_state *s = &d->sampler;
     if (tx->level_count < 1 || tx->level_count > SWR_MAX_MIP_LEVELS) return SWR_ERROR_INVALID_ARGUMENT;
     t->isFloat = tx->format == SWR_FORMAT_R32G32B32A32_SFLOAT;
+    t->bgra = tx->format == SWR_FORMAT_B8G8R8A8_UNORM;
     int texelBytes = t->isFloat ? 16 : 4;
     for (int l = 0; l < SWR_MAX_MIP_LEVELS; l++) {
       int vl = imin(l, (int)tx->level_count - 1);
@@ -1606,6 +1609,111 @@ static uint32_t clear_unorm8(float c) {
   float t = 255.0f * c;
   t = t + 0.5f;
   return t == t ? (uint32_t)t : 0u;
+}
+
+/* ===================================================================== */
+/* BC1-BC5 block decoding (BC_Decoder.cpp:31-217, 1662-1777)              */
+/* ===================================================================== */
+
+/* Colour block (BC1, and the colour half of BC2/BC3): two 565 endpoints
+ * widened by bit replication, stored blue in byte 0 and red in byte 2 (the
+ * decoded image is B8G8R8A8); 4-colour mode (c0 > c1, or always with a
+ * separate alpha block) interpolates thirds, 3-colour mode halves, and its
+ * fourth code is black with alpha 0 when the format has alpha. */
+static void bc_color_palette(const uint8_t *blk, int hasAlpha, int separateAlpha, uint32_t pal[4]) {
+  const uint32_t c0 = (uint32_t)blk[0] | ((uint32_t)blk[1] << 8), c1 = (uint32_t)blk[2] | ((uint32_t)blk[3] << 8);
+  int e[2][4];
+  const uint32_t cs[2] = {c0, c1};
+  for (int k = 0; k < 2; k++) {
+    const uint32_t c = cs[k];
+    e[k][0] = (int)(((c & 0x1Fu) << 3) | ((c & 0x1Cu) >> 2));
+    e[k][1] = (int)(((c & 0x7E0u) >> 3) | ((c & 0x600u) >> 9));
+    e[k][2] = (int)(((c & 0xF800u) >> 8) | ((c & 0xE000u) >> 13));
+    e[k][3] = 255;
+  }
+  int p[4][4];
+  for (int ch = 0; ch < 4; ch++) {
+    p[0][ch] = e[0][ch];
+    p[1][ch] = e[1][ch];
+    if (separateAlpha || c0 > c1) {
+      p[2][ch] = (2 * e[0][ch] + e[1][ch]) / 3;
+      p[3][ch] = (2 * e[1][ch] + e[0][ch]) / 3;
+    } else {
+      p[2][ch] = (e[0][ch] + e[1][ch]) >> 1;
+      p[3][ch] = ch == 3 ? (hasAlpha ? 0 : 255) : 0;
+    }
+  }
+  for (int i = 0; i < 4; i++)
+    pal[i] = (uint32_t)p[i][0] | ((uint32_t)p[i][1] << 8) | ((uint32_t)p[i][2] << 16) | ((uint32_t)p[i][3] << 24);
+}
+
+/* Single-channel block (BC3 alpha, BC4, BC5): two 8-bit endpoints (signed
+ * for the SNORM formats), 8 codes when c0 > c1 (sevenths), else 6 codes
+ * (fifths) plus the format's minimum and maximum; 3-bit indices from bit 16.
+ * Divisions truncate toward zero. */
+static void bc_channel_palette(const uint8_t *blk, int isSigned, int pal[8]) {
+  const int c0 = isSigned ? (int)(int8_t)blk[0] : (int)blk[0];
+  const int c1 = isSigned ? (int)(int8_t)blk[1] : (int)blk[1];
+  pal[0] = c0;
+  pal[1] = c1;
+  if (c0 > c1) {
+    for (int i = 2; i < 8; i++) pal[i] = ((8 - i) * c0 + (i - 1) * c1) / 7;
+  } else {
+    for (int i = 2; i < 6; i++) pal[i] = ((6 - i) * c0 + (i - 1) * c1) / 5;
+    pal[6] = isSigned ? -128 : 0;
+    pal[7] = isSigned ? 127 : 255;
+  }
+}
+
+static uint64_t bc_u64(const uint8_t *p) {
+  uint64_t v = 0;
+  for (int k = 7; k >= 0; k--) v = (v << 8) | p[k];
+  return v;
+}
+
+EXPORT int swo_decode_bc(const uint8_t *src, uint8_t *dst, int w, int h, int dstPitch, int dstBpp, int n,
+                         int isNoAlphaU) {
+  if (n < 1 || n > 5) return 0; /* BC6H and BC7 are not restated */
+  const int hasAlpha = n == 1 && !isNoAlphaU;
+  const int isSigned = (n == 4 || n == 5) && !isNoAlphaU;
+  const int blockBytes = (n == 1 || n == 4) ? 8 : 16;
+  const int bw = (w + 3) / 4;
+  for (int by = 0; by * 4 < h; by++) {
+    for (int bx = 0; bx * 4 < w; bx++) {
+      const uint8_t *blk = src + (size_t)(by * bw + bx) * blockBytes;
+      for (int j = 0; j < 4 && by * 4 + j < h; j++) {
+        for (int i = 0; i < 4 && bx * 4 + i < w; i++) {
+          const int t = j * 4 + i;
+          uint8_t *px = dst + (size_t)(by * 4 + j) * dstPitch + (size_t)(bx * 4 + i) * dstBpp;
+          if (n <= 3) {
+            const uint8_t *cb = n == 1 ? blk : blk + 8;
+            uint32_t pal[4];
+            bc_color_palette(cb, hasAlpha, n != 1, pal);
+            const uint32_t idx = (uint32_t)cb[4] | ((uint32_t)cb[5] << 8) | ((uint32_t)cb[6] << 16) | ((uint32_t)cb[7] << 24);
+            uint32_t v = pal[(idx >> (2 * t)) & 3u];
+            if (n == 2) { /* explicit 4-bit alpha, replicated */
+              const uint32_t a = (uint32_t)((bc_u64(blk) >> (4 * t)) & 0xFu);
+              v = (v & 0x00FFFFFFu) | ((a | (a << 4)) << 24);
+            } else if (n == 3) {
+              int apal[8];
+              bc_channel_palette(blk, 0, apal);
+              const uint32_t code = (uint32_t)((bc_u64(blk) >> (16 + 3 * t)) & 7u);
+              v = (v & 0x00FFFFFFu) | ((uint32_t)(uint8_t)apal[code] << 24);
+            }
+            memcpy(px, &v, 4);
+          } else {
+            for (int ch = 0; ch < n - 3; ch++) { /* BC4: red; BC5: red, green */
+              int pal[8];
+              bc_channel_palette(blk + 8 * ch, isSigned, pal);
+              const uint32_t code = (uint32_t)((bc_u64(blk + 8 * ch) >> (16 + 3 * t)) & 7u);
+              px[ch] = (uint8_t)pal[code];
+            }
+          }
+        }
+      }
+    }
+  }
+  return 1;
 }
 
 /* A D16 clear runs the blit routine from a D32F pixel: sw::clamp to [0, 1]

@@ -386,9 +386,11 @@ int build_params(const swr_draw_desc *d, const swr_tuning &tuning, DrawParams &P
     TexParams &T = P.tex;
     if (tx.level_count < 1 || tx.level_count > SWR_MAX_MIP_LEVELS)
       return fail(SWR_ERROR_INVALID_ARGUMENT, "texture level_count %u outside 1..15", tx.level_count);
-    if (tx.format != SWR_FORMAT_R8G8B8A8_UNORM && tx.format != SWR_FORMAT_R32G32B32A32_SFLOAT)
+    if (tx.format != SWR_FORMAT_R8G8B8A8_UNORM && tx.format != SWR_FORMAT_B8G8R8A8_UNORM &&
+        tx.format != SWR_FORMAT_R32G32B32A32_SFLOAT)
       return fail(SWR_ERROR_UNSUPPORTED, "texture format %u", tx.format);
     T.isFloat = tx.format == SWR_FORMAT_R32G32B32A32_SFLOAT;
+    T.bgra = tx.format == SWR_FORMAT_B8G8R8A8_UNORM;  // e.g. decoded BC1-BC3
     uint32_t texelBytes = T.isFloat ? 16 : 4;
     for (int l = 0; l < SWR_MAX_MIP_LEVELS; l++) {
       const swr_image_level &lv = tx.levels[std::min<uint32_t>(l, tx.level_count - 1)];
@@ -1140,7 +1142,8 @@ int swr_draw_backward(const swr_draw_desc *desc, const swr_grad_desc *grad, void
 int swr_generate_mips(const swr_texture *tex, void *stream) {
   if (!tex) return fail(SWR_ERROR_INVALID_ARGUMENT, "texture is NULL");
   const bool f = tex->format == SWR_FORMAT_R32G32B32A32_SFLOAT;
-  if (!f && tex->format != SWR_FORMAT_R8G8B8A8_UNORM) return fail(SWR_ERROR_UNSUPPORTED, "format %u", tex->format);
+  if (!f && tex->format != SWR_FORMAT_R8G8B8A8_UNORM && tex->format != SWR_FORMAT_B8G8R8A8_UNORM)
+    return fail(SWR_ERROR_UNSUPPORTED, "format %u", tex->format);  // the box filter is byte-wise
   const uint32_t tb = f ? 16 : 4;
   int r = check_chain(tex, tb);
   if (r != SWR_OK) return r;
@@ -1349,6 +1352,31 @@ int swr_clear(const swr_clear_desc *d, void *stream) {
     hipLaunchKernelGGL(clear_rect_kernel<2>, grid, block, 0, s, base, pitch, slice, (int)x0, (int)y0, w, h, v);
   else
     hipLaunchKernelGGL(clear_rect_kernel<1>, grid, block, 0, s, base, pitch, slice, (int)x0, (int)y0, w, h, v);
+  SWR_HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}
+
+int swr_decode_bc(const swr_bc_desc *d, void *stream) {
+  if (!d || !d->src || !d->dst) return fail(SWR_ERROR_INVALID_ARGUMENT, "BC descriptor incomplete");
+  if (d->n == 6 || d->n == 7) return fail(SWR_ERROR_UNSUPPORTED, "BC%u decoding is not built", d->n);
+  if (d->n < 1 || d->n > 7) return fail(SWR_ERROR_INVALID_ARGUMENT, "BC%u is not a BC format", d->n);
+  if (d->width == 0 || d->height == 0) return SWR_OK;
+  if (d->width > 16384 || d->height > 16384) return fail(SWR_ERROR_INVALID_ARGUMENT, "image larger than 16384");
+  if (reinterpret_cast<uintptr_t>(d->src) & 7u) return fail(SWR_ERROR_INVALID_ARGUMENT, "src is not 8-byte aligned");
+  const uint32_t bpp = d->dst_bytes_per_pixel;
+  if (d->n <= 3) {
+    if (bpp != 4) return fail(SWR_ERROR_INVALID_ARGUMENT, "BC%u writes 4-byte pixels", d->n);
+    if ((reinterpret_cast<uintptr_t>(d->dst) | d->dst_row_pitch_bytes) & 3u)
+      return fail(SWR_ERROR_INVALID_ARGUMENT, "BC%u needs a 4-byte aligned destination and pitch", d->n);
+  } else if (bpp < d->n - 3u) {
+    return fail(SWR_ERROR_INVALID_ARGUMENT, "BC%u needs %u bytes per pixel", d->n, d->n - 3u);
+  }
+  if ((uint64_t)d->dst_row_pitch_bytes < (uint64_t)d->width * bpp)
+    return fail(SWR_ERROR_INVALID_ARGUMENT, "destination pitch too small");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  dim3 block(256), grid((d->width + 255) / 256, d->height);
+  hipLaunchKernelGGL(decode_bc_kernel, grid, block, 0, s, (const uint8_t *)d->src, (uint8_t *)d->dst, (int)d->width,
+                     (int)d->height, (int)d->dst_row_pitch_bytes, (int)bpp, (int)d->n, (int)d->no_alpha_or_unsigned);
   SWR_HIP_CHECK(hipGetLastError());
   return SWR_OK;
 }

@@ -338,7 +338,10 @@ __device__ __forceinline__ float4 sample_lane(const TexParams &T, float u, float
     for (int k = 0; k < 4; k++) c[k] = (mulhi16(c[k], inv) + mulhi16(cc[k], utri)) & 0xFFFFu;
   }
   const float scale = 1.0f / 65280.0f;
-  return make_float4((float)c[0] * scale, (float)c[1] * scale, (float)c[2] * scale, (float)c[3] * scale);
+  // B8G8R8A8 texels filter channel by channel like R8G8B8A8; only the
+  // order of R and B differs (SamplerCore's per-format swizzle).
+  const uint32_t r = T.bgra ? c[2] : c[0], b = T.bgra ? c[0] : c[2];
+  return make_float4((float)r * scale, (float)c[1] * scale, (float)b * scale, (float)c[3] * scale);
 }
 
 // ---------------------------------------------------------------------------

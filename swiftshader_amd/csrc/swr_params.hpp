@@ -31,6 +31,7 @@ struct TexParams {
   uint32_t vHalf[kMipLevels];
   float wwhh[4];
   int isFloat;
+  int bgra;  // B8G8R8A8_UNORM texels: R and B exchanged in memory
   int filter, mipmap, addrU, addrV;  // addr: 0 wrap, 1 clamp
   int skipLod;
   float mipLodBias, maxAnisotropy, minLod, maxLod;

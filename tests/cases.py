@@ -327,6 +327,34 @@ def _formats(seed=44, d16=True, bgra=False, samples=1, bias=None, stencil=False,
                     depth_compare=abi.COMPARE_LESS, depth_bias=bias, clear_depth=0.75, **extra)
 
 
+def bc_texture_levels(rng, size, n=1, flag=0):
+    """A mip chain of random BCn blocks (n = 1..3) and its B8G8R8A8
+    decoding by the oracle: [(blocks, decoded (h, w, 4) uint8)] per level."""
+    from oracle_runner import bc_block_bytes, oracle_decode_bc
+    out = []
+    w = h = size
+    while True:
+        nb = ((w + 3) // 4) * ((h + 3) // 4)
+        blocks = rng.integers(0, 256, nb * bc_block_bytes(n), dtype=np.uint8)
+        dec = np.zeros((h, w, 4), np.uint8)
+        oracle_decode_bc(blocks, dec.reshape(-1), w, h, w * 4, 4, n, flag)
+        out.append((blocks, dec))
+        if w == 1 and h == 1:
+            return out
+        w, h = max(w // 2, 1), max(h // 2, 1)
+
+
+def _bc_textured(seed=51, n=1, aniso=False, samples=1):
+    """Triangles sampling a decoded BCn texture (B8G8R8A8 texels)."""
+    rng = np.random.default_rng(seed)
+    levels = [dec for _, dec in bc_texture_levels(rng, 64, n)]
+    tex = TextureData(format=abi.FORMAT_B8G8R8A8_UNORM, levels=levels, anisotropy_enable=1 if aniso else 0,
+                      max_anisotropy=8.0 if aniso else 1.0)
+    pos, uv, col = scenes.screen_triangles(rng, 1200, 96, 72, area_hi=1500)
+    return DrawData(width=96, height=72, positions=pos, colors=col, texcoords=uv, samples=samples,
+                    shader=abi.SHADER_MODULATE, texture=tex)
+
+
 def parity_cases():
     """(name, DrawData) pairs; every one must match the oracle bit for bit."""
     return [
@@ -378,4 +406,6 @@ def parity_cases():
         ("bgra8_srgb_msaa4_textured_blend", _formats(seed=49, bgra=True, srgb=True, samples=4, textured=True,
                                                     blend=True)),
         ("srgb8_opaque_d16", _formats(seed=50, srgb=True)),
+        ("bc1_texture_bgra", _bc_textured()),
+        ("bc3_texture_bgra_aniso_msaa4", _bc_textured(seed=52, n=3, aniso=True, samples=4)),
     ]

@@ -48,6 +48,8 @@ def oracle():
             getattr(L, f).argtypes = [ctypes.c_float]
             getattr(L, f).restype = ctypes.c_float
         L.swo_constants.restype = None
+        L.swo_decode_bc.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 6
+        L.swo_decode_bc.restype = ctypes.c_int
         fp = ctypes.POINTER(ctypes.c_float)
         L.swo_clip_triangle.argtypes = [fp, fp, fp, ctypes.c_int, fp, ctypes.POINTER(ctypes.c_int)]
         L.swo_clip_triangle.restype = ctypes.c_int
@@ -64,6 +66,8 @@ def reference_lib():
         L.ref_clip_triangle.argtypes = [fp, fp, fp, ctypes.c_int, fp, ctypes.POINTER(ctypes.c_int)]
         L.ref_clip_triangle.restype = ctypes.c_int
         L.ref_constants.restype = None
+        L.ref_decode_bc.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 6
+        L.ref_decode_bc.restype = ctypes.c_int
         _ref = L
     return _ref
 
@@ -179,3 +183,17 @@ def oracle_clear(target, fmt, value, area=None):
     if rc != 0:
         raise RuntimeError(f"oracle swo_clear failed: {rc}")
     return target
+
+
+def bc_block_bytes(n):
+    return 8 if n in (1, 4) else 16
+
+
+def oracle_decode_bc(src, dst, w, h, pitch, bpp, n, no_alpha_or_unsigned, lib=None):
+    """Decodes BCn blocks `src` (uint8) into `dst` (uint8, pitch bytes per
+    row) in place with the oracle, or with `lib`'s ref_decode_bc."""
+    src = np.ascontiguousarray(src, np.uint8)
+    assert dst.flags["C_CONTIGUOUS"] and dst.dtype == np.uint8
+    if lib is None:
+        return oracle().swo_decode_bc(src.ctypes.data, dst.ctypes.data, w, h, pitch, bpp, n, no_alpha_or_unsigned)
+    return lib.ref_decode_bc(src.ctypes.data, dst.ctypes.data, w, h, pitch, bpp, n, no_alpha_or_unsigned)

@@ -53,7 +53,8 @@ def test_struct_layout_against_the_header(tmp_path):
     if cc is None:
         pytest.skip("gcc not available")
     structs = [("swr_draw_desc", abi.DrawDesc), ("swr_grad_desc", abi.GradDesc), ("swr_texture", abi.Texture),
-               ("swr_resolve_desc", abi.ResolveDesc), ("swr_profile", abi.Profile), ("swr_clear_desc", abi.ClearDesc)]
+               ("swr_resolve_desc", abi.ResolveDesc), ("swr_profile", abi.Profile), ("swr_clear_desc", abi.ClearDesc),
+               ("swr_bc_desc", abi.BCDesc)]
     src = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
     want = []
     for cname, py in structs:
@@ -82,6 +83,7 @@ def test_version_and_error_paths_without_gpu():
     assert lib.swr_set_tuning(None) < 0
     assert lib.swr_resolve(None, None) < 0
     assert lib.swr_clear(None, None) < 0
+    assert lib.swr_decode_bc(None, None) < 0
     assert lib.swr_batch_plan_create(None, 0, 0, None, None) < 0
     assert lib.swr_batch_plan_run(None, None) < 0
     assert lib.swr_batch_plan_destroy(None) == 0

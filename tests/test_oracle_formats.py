@@ -105,3 +105,21 @@ def test_bias_unit_is_1_01_codes():
     assert (depth == k - 10).all(), depth[0, 0, 0]
     _, depth, _ = _d16(_quad(z), bias=(10.0, 0.0, 0.0))  # k + .55 + 10.1 -> k + 11
     assert (depth == k + 11).all(), depth[0, 0, 0]
+
+
+def test_bgra8_texture_samples_like_rgba8():
+    """A B8G8R8A8 texture renders exactly as the R8G8B8A8 texture holding
+    the same texels (trilinear and anisotropic, both address modes)."""
+    rng = np.random.default_rng(9)
+    lv = scenes.box_mip_chain(scenes.noise_texture_rgba8(rng, 32))
+    pos, uv, col = scenes.screen_triangles(rng, 150, W, H, area_hi=200)
+    for aniso, addr in ((False, abi.ADDRESS_REPEAT), (True, abi.ADDRESS_CLAMP_TO_EDGE)):
+        out = []
+        for fmt, levels in ((abi.FORMAT_R8G8B8A8_UNORM, lv),
+                            (abi.FORMAT_B8G8R8A8_UNORM, [np.ascontiguousarray(l[..., [2, 1, 0, 3]]) for l in lv])):
+            tex = scenes.TextureData(format=fmt, levels=levels, anisotropy_enable=int(aniso),
+                                     max_anisotropy=8.0 if aniso else 1.0, address_u=addr, address_v=addr)
+            d = DrawData(width=W, height=H, positions=pos, colors=col, texcoords=uv, shader=abi.SHADER_MODULATE,
+                         texture=tex)
+            out.append(oracle_render(d)[0])
+        assert np.array_equal(out[0], out[1])
