@@ -182,7 +182,7 @@ typedef struct swr_image_level { /* one mip level of a 2D image view */
 } swr_image_level;
 
 typedef struct swr_texture { /* combined image sampler, 2D view */
-  uint32_t format;          /* R8G8B8A8_UNORM or R32G32B32A32_SFLOAT    */
+  uint32_t format;          /* R8G8B8A8_UNORM, B8G8R8A8_UNORM or R32G32B32A32_SFLOAT */
   uint32_t level_count;     /* levels in the view, 1..15                */
   swr_image_level levels[SWR_MAX_MIP_LEVELS];
 } swr_texture;
@@ -464,7 +464,7 @@ int swr_draw_backward(const swr_draw_desc *desc, const swr_grad_desc *grad, void
 
 /* Fills levels 1..level_count-1 of `tex` from level 0 by 2x2 box filtering
  * (level k+1 = 0.25 * ((a + b) + (c + d)) for RGBA32F; (a+b+c+d+2)>>2 per
- * channel for RGBA8; odd sizes halve with floor and clamp the second
+ * channel for RGBA8 and BGRA8; odd sizes halve with floor and clamp the second
  * texel).  The mip chain a differentiable texture is sampled through. */
 int swr_generate_mips(const swr_texture *tex, void *stream);
 
