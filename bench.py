@@ -246,7 +246,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
-              "(shard whole draws or screen tiles across GPUs: see DESIGN.md)", file=sys.stderr)
+              "(whole draws or screen tiles would shard across GPUs)", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -88,7 +88,7 @@ struct Pipeline {
   Scratch keysA, keysB, valsA, valsB, hist; // per pair
   Scratch first;                            // per block of emitted pairs
   Scratch tStart, tEnd;                     // per tile
-  Scratch counters;                         // work counters (per variant), overflow flags
+  Scratch counters;  // [0, 16) raster work counters per variant, [16] overflow flag, [20, 22) backward groups
   void release() {
     for (Scratch *s : {&params, &aux, &rec, &poly, &cnt, &off, &sums, &keysA, &keysB, &valsA, &valsB, &hist, &first,
                        &tStart, &tEnd, &counters})
@@ -487,10 +487,14 @@ int build_params(const swr_draw_desc *d, const swr_tuning &tuning, DrawParams &P
 // ---------------------------------------------------------------------------
 // Batched pipeline
 // ---------------------------------------------------------------------------
-// Raster variant of a draw: samples, float colour target, float texture.
+// Raster variant of a draw: samples, float colour target, float texture,
+// and whether it uses a format outside the common RGBA8 / RGBA32F / D32F set
+// (D16 depth, BGRA8 or sRGB colour, BGRA8 texture), which the common
+// variants compile out.
 int variant_of(const DrawParams &P) {
   const bool ftex = P.useUV && P.tex.isFloat;
-  return (P.ms > 1 ? 4 : 0) | (P.colorFloat ? 2 : 0) | (ftex ? 1 : 0);
+  const bool xf = P.depth16 || P.colorBGRA || P.colorSRGB || (P.useUV && P.tex.bgra);
+  return (xf ? 8 : 0) | (P.ms > 1 ? 4 : 0) | (P.colorFloat ? 2 : 0) | (ftex ? 1 : 0);
 }
 
 // Host plan of a batch: per-draw state in enqueue order, split into
@@ -617,7 +621,7 @@ int ensure_pipe(Pipeline &pipe, uint32_t T, uint32_t cap, uint32_t tiles) {
   SWR_HIP_CHECK(pipe.first.ensure(sizeof(uint32_t) * ((size_t)cap / kEmitBlock + 2)));
   SWR_HIP_CHECK(pipe.tStart.ensure(sizeof(uint32_t) * std::max<uint32_t>(tiles, 1)));
   SWR_HIP_CHECK(pipe.tEnd.ensure(sizeof(uint32_t) * std::max<uint32_t>(tiles, 1)));
-  SWR_HIP_CHECK(pipe.counters.ensure(sizeof(uint32_t) * 16));
+  SWR_HIP_CHECK(pipe.counters.ensure(sizeof(uint32_t) * 32));
   return SWR_OK;
 }
 
@@ -629,12 +633,30 @@ void enqueue_scan(const uint32_t *in, uint32_t *out, size_t n, uint32_t *sums, h
   hipLaunchKernelGGL(scan_add_kernel, dim3(blocks), dim3(256), 0, s, out, (uint32_t)n, (const uint32_t *)sums);
 }
 
-template <int MS, bool FCOL, bool FTEX>
+template <int MS, bool FCOL, bool FTEX, bool XF>
 void launch_raster_batch(dim3 grid, dim3 block, hipStream_t s, const DrawParams *params, const uint32_t *tileBase,
                          int lo, int hi, uint32_t tLo, uint32_t tHi, uint32_t *counter, const TriRecord *rec,
                          const int *poly, const uint32_t *bin, const uint32_t *ts, const uint32_t *te) {
-  hipLaunchKernelGGL((raster_batch_kernel<MS, FCOL, FTEX>), grid, block, 0, s, params, tileBase, lo, hi, tLo, tHi,
-                     counter, rec, poly, bin, ts, te);
+  hipLaunchKernelGGL((raster_batch_kernel<MS, FCOL, FTEX, XF>), grid, block, 0, s, params, tileBase, lo, hi, tLo,
+                     tHi, counter, rec, poly, bin, ts, te);
+}
+
+// The raster kernel of variant v (variant_of).
+template <bool XF>
+void launch_raster_variant(int v, dim3 grid, dim3 block, hipStream_t s, const DrawParams *params,
+                           const uint32_t *tileBase, int lo, int hi, uint32_t tLo, uint32_t tHi, uint32_t *ctr,
+                           const TriRecord *rec, const int *poly, const uint32_t *bin, const uint32_t *ts,
+                           const uint32_t *te) {
+  switch (v & 7) {
+    case 0: launch_raster_batch<1, false, false, XF>(grid, block, s, params, tileBase, lo, hi, tLo, tHi, ctr, rec, poly, bin, ts, te); break;
+    case 1: launch_raster_batch<1, false, true, XF>(grid, block, s, params, tileBase, lo, hi, tLo, tHi, ctr, rec, poly, bin, ts, te); break;
+    case 2: launch_raster_batch<1, true, false, XF>(grid, block, s, params, tileBase, lo, hi, tLo, tHi, ctr, rec, poly, bin, ts, te); break;
+    case 3: launch_raster_batch<1, true, true, XF>(grid, block, s, params, tileBase, lo, hi, tLo, tHi, ctr, rec, poly, bin, ts, te); break;
+    case 4: launch_raster_batch<4, false, false, XF>(grid, block, s, params, tileBase, lo, hi, tLo, tHi, ctr, rec, poly, bin, ts, te); break;
+    case 5: launch_raster_batch<4, false, true, XF>(grid, block, s, params, tileBase, lo, hi, tLo, tHi, ctr, rec, poly, bin, ts, te); break;
+    case 6: launch_raster_batch<4, true, false, XF>(grid, block, s, params, tileBase, lo, hi, tLo, tHi, ctr, rec, poly, bin, ts, te); break;
+    default: launch_raster_batch<4, true, true, XF>(grid, block, s, params, tileBase, lo, hi, tLo, tHi, ctr, rec, poly, bin, ts, te); break;
+  }
 }
 
 // Setup and tile-count prefix sum of one segment (off[T] = pair count).
@@ -704,16 +726,11 @@ int enqueue_back(DeviceCtx *c, Pipeline &pipe, const BatchLayout &L, const Batch
       const TriRecord *rec = pipe.rec.as<TriRecord>();
       const int *poly = pipe.poly.as<int>();
       const uint32_t *ts = pipe.tStart.as<uint32_t>(), *te = pipe.tEnd.as<uint32_t>();
-      switch (variant_of(L.P[sg.first + lo])) {
-        case 0: launch_raster_batch<1, false, false>(grid, block, s, params, tileBase, lo, hi, tLo, tHi, ctr, rec, poly, bin, ts, te); break;
-        case 1: launch_raster_batch<1, false, true>(grid, block, s, params, tileBase, lo, hi, tLo, tHi, ctr, rec, poly, bin, ts, te); break;
-        case 2: launch_raster_batch<1, true, false>(grid, block, s, params, tileBase, lo, hi, tLo, tHi, ctr, rec, poly, bin, ts, te); break;
-        case 3: launch_raster_batch<1, true, true>(grid, block, s, params, tileBase, lo, hi, tLo, tHi, ctr, rec, poly, bin, ts, te); break;
-        case 4: launch_raster_batch<4, false, false>(grid, block, s, params, tileBase, lo, hi, tLo, tHi, ctr, rec, poly, bin, ts, te); break;
-        case 5: launch_raster_batch<4, false, true>(grid, block, s, params, tileBase, lo, hi, tLo, tHi, ctr, rec, poly, bin, ts, te); break;
-        case 6: launch_raster_batch<4, true, false>(grid, block, s, params, tileBase, lo, hi, tLo, tHi, ctr, rec, poly, bin, ts, te); break;
-        default: launch_raster_batch<4, true, true>(grid, block, s, params, tileBase, lo, hi, tLo, tHi, ctr, rec, poly, bin, ts, te); break;
-      }
+      const int var = variant_of(L.P[sg.first + lo]);
+      if (var & 8)
+        launch_raster_variant<true>(var, grid, block, s, params, tileBase, lo, hi, tLo, tHi, ctr, rec, poly, bin, ts, te);
+      else
+        launch_raster_variant<false>(var, grid, block, s, params, tileBase, lo, hi, tLo, tHi, ctr, rec, poly, bin, ts, te);
     }
   } else if (es) {
     SWR_HIP_CHECK(hipEventRecord(es->ev[EV_BIN], s));
@@ -763,7 +780,7 @@ int draw_eager(DeviceCtx *c, const swr_draw_desc *descs, uint32_t n, hipStream_t
     const uint32_t pairs = *c->hostTotal;
     r = ensure_pipe(pipe, sg.T, pairs, sg.tiles);
     if (r != SWR_OK) return r;
-    r = enqueue_back(c, pipe, L, sg, pairs, tuning, pipe.counters.as<uint32_t>() + 8, prof ? &es : nullptr, s);
+    r = enqueue_back(c, pipe, L, sg, pairs, tuning, pipe.counters.as<uint32_t>() + 16, prof ? &es : nullptr, s);
     if (r != SWR_OK) return r;
     if (prof) {
       SWR_HIP_CHECK(hipEventRecord(es.ev[EV_RASTER], s));
@@ -1064,8 +1081,8 @@ int plan_enqueue_backward(swr_batch_plan *pl, int cus, hipStream_t s) {
     const GradParams *grads = pl->gparams.as<GradParams>() + sg.first;
     const uint32_t *bb = pl->gbase.as<uint32_t>() + sg.first;
     // Draws of one sample count are contiguous (the layout groups variants);
-    // each group gets its own work counter (counters[8 + group]).
-    uint32_t *ctr = pipe.counters.as<uint32_t>() + 10;
+    // each group gets its own work counter (counters[20 + group]).
+    uint32_t *ctr = pipe.counters.as<uint32_t>() + 20;
     hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, ctr, 0u, 2u);
     int group = 0;
     int lo = 0;

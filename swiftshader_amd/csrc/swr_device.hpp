@@ -313,7 +313,7 @@ __device__ __forceinline__ QuadLod quad_lod(const TexParams &T, const float u[4]
 
 // One lane of sampleTexture128 after the LOD is known (FTEX: RGBA32F
 // texture, 32-bit float path; otherwise RGBA8, 16-bit fixed-point path).
-template <bool FTEX>
+template <bool FTEX, bool XF>
 __device__ __forceinline__ float4 sample_lane(const TexParams &T, float u, float v, const QuadLod &L) {
   if (FTEX) {
     float4 c = sample_float_aniso(T, u, v, L.lod, L.aniso, L.uDelta, L.vDelta, 0);
@@ -340,7 +340,8 @@ __device__ __forceinline__ float4 sample_lane(const TexParams &T, float u, float
   const float scale = 1.0f / 65280.0f;
   // B8G8R8A8 texels filter channel by channel like R8G8B8A8; only the
   // order of R and B differs (SamplerCore's per-format swizzle).
-  const uint32_t r = T.bgra ? c[2] : c[0], b = T.bgra ? c[0] : c[2];
+  const bool bgra = XF && T.bgra;
+  const uint32_t r = bgra ? c[2] : c[0], b = bgra ? c[0] : c[2];
   return make_float4((float)r * scale, (float)c[1] * scale, (float)b * scale, (float)c[3] * scale);
 }
 

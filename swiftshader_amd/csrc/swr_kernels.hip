@@ -680,7 +680,7 @@ __device__ __forceinline__ QuadLod quad_state(const DrawParams &P, const TriReco
 // of primitive R, with the texture LOD of the 2x2 quad the pixel belongs
 // to (helper lanes evaluated from the plane equations).  Output clamped
 // for UNORM targets (PixelProgram::clampColor).
-template <bool FCOL, bool FTEX>
+template <bool FCOL, bool FTEX, bool XF>
 __device__ __forceinline__ float4 shade(const DrawParams &P, const TriRecord &R, int px, int py) {
   const float xf = (float)px - R.x0, yf = (float)py - R.y0;
   const float rhw = quad_rhw(R, xf, yf);
@@ -694,7 +694,7 @@ __device__ __forceinline__ float4 shade(const DrawParams &P, const TriRecord &R,
   if (P.shader != SHADER_INTERPOLATE) {
     const float u = interp(R, 4, xf, yf, rhw), v = interp(R, 5, xf, yf, rhw);
     const QuadLod Lq = quad_state(P, R, px, py);
-    float4 c = sample_lane<FTEX>(P.tex, u, v, Lq);
+    float4 c = sample_lane<FTEX, XF>(P.tex, u, v, Lq);
     if (P.shader == SHADER_MODULATE) {
       c.x = c.x * col.x;
       c.y = c.y * col.y;
@@ -714,7 +714,7 @@ __device__ __forceinline__ float4 shade(const DrawParams &P, const TriRecord &R,
 
 // Blend a shaded colour into the samples of this lane's pixel selected by
 // xMask, then write (PixelRoutine.cpp:1617-2936).
-template <int MS, bool FCOL>
+template <int MS, bool FCOL, bool XF>
 __device__ __forceinline__ void blend_write(const DrawParams &P, PixelState<MS, FCOL> &S, float4 s, int xMask,
                                             uint32_t tri) {
 #pragma unroll
@@ -727,14 +727,14 @@ __device__ __forceinline__ void blend_write(const DrawParams &P, PixelState<MS, 
       const uint32_t p = S.cb[FCOL ? 0 : q];
       dst = make_float4(unorm8_to_float(p & 0xFFu), unorm8_to_float((p >> 8) & 0xFFu),
                         unorm8_to_float((p >> 16) & 0xFFu), unorm8_to_float(p >> 24));
-      if (P.colorSRGB && P.blendEnable) {  // blended in linear space (PixelRoutine.cpp:1811-1827)
+      if ((XF && P.colorSRGB) && P.blendEnable) {  // blended in linear space (PixelRoutine.cpp:1811-1827)
         dst.x = srgb_to_linear(dst.x);
         dst.y = srgb_to_linear(dst.y);
         dst.z = srgb_to_linear(dst.z);
       }
     }
     float4 o = P.blendEnable ? alpha_blend(P, s, dst) : s;
-    if (!FCOL && P.colorSRGB) {  // writeColor encodes R, G, B first (PixelRoutine.cpp:1963-1969)
+    if (!FCOL && (XF && P.colorSRGB)) {  // writeColor encodes R, G, B first (PixelRoutine.cpp:1963-1969)
       o.x = linear_to_srgb(o.x);
       o.y = linear_to_srgb(o.y);
       o.z = linear_to_srgb(o.z);
@@ -762,7 +762,7 @@ constexpr int kQueue = 64;  // deferred-shading queue entries per wave
 // One screen tile of draw P: `bin` is the tile's index in the draw's tile
 // grid, [start, end) its range of the sorted (tile, primitive) list.  The
 // workgroup has one wave per 8x8 block of the tile.
-template <int MS, bool FCOL, bool FTEX>
+template <int MS, bool FCOL, bool FTEX, bool XF>
 __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32_t start, uint32_t end0,
                                             const TriRecord *__restrict__ rec, const int *__restrict__ poly,
                                             const uint32_t *__restrict__ binTri) {
@@ -811,7 +811,7 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
       S.id[FCOL ? q : 0] =
           *reinterpret_cast<const uint32_t *>(P.primId + q * P.primIdSlice + (long long)py * P.primIdPitch + 4ll * px);
     if (P.depthTest)  // D16 holds the float of its integer (readDepth16, PixelRoutine.cpp:466-481)
-      S.zb[q] = P.depth16 ? (float)*reinterpret_cast<const uint16_t *>(P.depth + q * P.depthSlice +
+      S.zb[q] = (XF && P.depth16) ? (float)*reinterpret_cast<const uint16_t *>(P.depth + q * P.depthSlice +
                                                                         (long long)py * P.depthPitch + 2ll * px)
                           : *reinterpret_cast<const float *>(P.depth + q * P.depthSlice +
                                                              (long long)py * P.depthPitch + 4ll * px);
@@ -822,7 +822,7 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
             *reinterpret_cast<const float4 *>(P.color + q * P.colorSlice + (long long)py * P.colorPitch + 16ll * px);
       else
         S.cb[FCOL ? 0 : q] = bgra_swap(
-            P.colorBGRA,
+            (XF && P.colorBGRA),
             *reinterpret_cast<const uint32_t *>(P.color + q * P.colorSlice + (long long)py * P.colorPitch + 4ll * px));
     }
   }
@@ -844,7 +844,7 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
       const uint32_t t = qTri[wave][lane];
       const uint32_t pix = qPix[wave][lane];
       const int l = (int)(pix & 63u);
-      qCol[wave][lane] = shade<FCOL, FTEX>(P, rec[t], ox + lane_cx(l), oy + lane_cy(l));
+      qCol[wave][lane] = shade<FCOL, FTEX, XF>(P, rec[t], ox + lane_cx(l), oy + lane_cy(l));
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
@@ -853,7 +853,7 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
     while (m) {
       const int k = __builtin_ctzll(m);
       m &= m - 1;
-      blend_write<MS, FCOL>(P, S, qCol[wave][k], (int)(qPix[wave][k] >> 8), qTri[wave][k]);
+      blend_write<MS, FCOL, XF>(P, S, qCol[wave][k], (int)(qPix[wave][k] >> 8), qTri[wave][k]);
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
@@ -966,7 +966,7 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
           float z = xx * a + Dz;
           if (P.depthBias) z = z + zbias;  // PixelRoutine.cpp:141-144, before the clamp
           z = rr_min(rr_max(z, 0.0f), 1.0f);
-          if (P.depth16) z = d16_value(z);
+          if ((XF && P.depth16)) z = d16_value(z);
           zq[q] = z;
           const float zb = S.zb[q];
           bool pass;
@@ -1066,7 +1066,7 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
         float zl = fmaxf(zlo0, m - mag * 0x1p-19f);
         if (!(zl == zl)) zl = -__builtin_inff();
         zl = fminf(zl, 1.0f);
-        if (P.depth16) zl = d16_value(fmaxf(zl, 0.0f));  // the bound in stored units (monotone map)
+        if ((XF && P.depth16)) zl = d16_value(fmaxf(zl, 0.0f));  // the bound in stored units (monotone map)
         test = !(P.depthCompare == 1 ? zl >= zmax : zl > zmax);
         zlo = zl;
       }
@@ -1105,7 +1105,7 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
 #pragma unroll
     for (int q = 0; q < MS; q++) {
       if (P.depthWrite) {
-        if (P.depth16)
+        if ((XF && P.depth16))
           *reinterpret_cast<uint16_t *>(P.depth + q * P.depthSlice + (long long)py * P.depthPitch + 2ll * px) =
               (uint16_t)S.zb[q];
         else
@@ -1121,7 +1121,7 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
               S.cf[FCOL ? q : 0];
         else
           *reinterpret_cast<uint32_t *>(P.color + q * P.colorSlice + (long long)py * P.colorPitch + 4ll * px) =
-              bgra_swap(P.colorBGRA, S.cb[FCOL ? 0 : q]);
+              bgra_swap((XF && P.colorBGRA), S.cb[FCOL ? 0 : q]);
       }
     }
   }
@@ -1160,7 +1160,7 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
 // Occupancy: 4 waves per SIMD (128 VGPRs) for the single-sample kernels
 // (the float-colour ones spill 12 bytes there and still run faster than at
 // 3 waves), 3 for the multisample ones.
-template <int MS, bool FCOL, bool FTEX>
+template <int MS, bool FCOL, bool FTEX, bool XF>
 __global__ __launch_bounds__(256, MS == 1 ? 4 : 3) void raster_batch_kernel(
                                                            const DrawParams *__restrict__ params,
                                                            const uint32_t *__restrict__ tileBase, int drawLo,
@@ -1192,7 +1192,7 @@ __global__ __launch_bounds__(256, MS == 1 ? 4 : 3) void raster_batch_kernel(
     }
     lo = __builtin_amdgcn_readfirstlane(lo);
     const uint32_t s = tileStart[t], e = tileEnd[t];
-    if (s < e) raster_tile<MS, FCOL, FTEX>(params[lo], (int)(t - tileBase[lo]), s, e, rec, poly, binTri);
+    if (s < e) raster_tile<MS, FCOL, FTEX, XF>(params[lo], (int)(t - tileBase[lo]), s, e, rec, poly, binTri);
   }
 }
 
@@ -1321,29 +1321,26 @@ __global__ void resolve_rgba32f_kernel(const char *src, int srcPitch, long long 
 }
 
 
-template __global__ void raster_batch_kernel<1, false, false>(const DrawParams *, const uint32_t *, int, int, uint32_t,
-                                                              uint32_t, uint32_t *, const TriRecord *, const int *,
-                                                              const uint32_t *, const uint32_t *, const uint32_t *);
-template __global__ void raster_batch_kernel<1, false, true>(const DrawParams *, const uint32_t *, int, int, uint32_t,
-                                                              uint32_t, uint32_t *, const TriRecord *, const int *,
-                                                              const uint32_t *, const uint32_t *, const uint32_t *);
-template __global__ void raster_batch_kernel<1, true, false>(const DrawParams *, const uint32_t *, int, int, uint32_t,
-                                                              uint32_t, uint32_t *, const TriRecord *, const int *,
-                                                              const uint32_t *, const uint32_t *, const uint32_t *);
-template __global__ void raster_batch_kernel<1, true, true>(const DrawParams *, const uint32_t *, int, int, uint32_t,
-                                                              uint32_t, uint32_t *, const TriRecord *, const int *,
-                                                              const uint32_t *, const uint32_t *, const uint32_t *);
-template __global__ void raster_batch_kernel<4, false, false>(const DrawParams *, const uint32_t *, int, int, uint32_t,
-                                                              uint32_t, uint32_t *, const TriRecord *, const int *,
-                                                              const uint32_t *, const uint32_t *, const uint32_t *);
-template __global__ void raster_batch_kernel<4, false, true>(const DrawParams *, const uint32_t *, int, int, uint32_t,
-                                                              uint32_t, uint32_t *, const TriRecord *, const int *,
-                                                              const uint32_t *, const uint32_t *, const uint32_t *);
-template __global__ void raster_batch_kernel<4, true, false>(const DrawParams *, const uint32_t *, int, int, uint32_t,
-                                                              uint32_t, uint32_t *, const TriRecord *, const int *,
-                                                              const uint32_t *, const uint32_t *, const uint32_t *);
-template __global__ void raster_batch_kernel<4, true, true>(const DrawParams *, const uint32_t *, int, int, uint32_t,
-                                                              uint32_t, uint32_t *, const TriRecord *, const int *,
-                                                              const uint32_t *, const uint32_t *, const uint32_t *);
+#define SWR_RASTER_INSTANCE(MS, FCOL, FTEX, XF)                                                                  \
+  template __global__ void raster_batch_kernel<MS, FCOL, FTEX, XF>(                                             \
+      const DrawParams *, const uint32_t *, int, int, uint32_t, uint32_t, uint32_t *, const TriRecord *, const int *, \
+      const uint32_t *, const uint32_t *, const uint32_t *);
+SWR_RASTER_INSTANCE(1, false, false, false)
+SWR_RASTER_INSTANCE(1, false, true, false)
+SWR_RASTER_INSTANCE(1, true, false, false)
+SWR_RASTER_INSTANCE(1, true, true, false)
+SWR_RASTER_INSTANCE(4, false, false, false)
+SWR_RASTER_INSTANCE(4, false, true, false)
+SWR_RASTER_INSTANCE(4, true, false, false)
+SWR_RASTER_INSTANCE(4, true, true, false)
+SWR_RASTER_INSTANCE(1, false, false, true)
+SWR_RASTER_INSTANCE(1, false, true, true)
+SWR_RASTER_INSTANCE(1, true, false, true)
+SWR_RASTER_INSTANCE(1, true, true, true)
+SWR_RASTER_INSTANCE(4, false, false, true)
+SWR_RASTER_INSTANCE(4, false, true, true)
+SWR_RASTER_INSTANCE(4, true, false, true)
+SWR_RASTER_INSTANCE(4, true, true, true)
+#undef SWR_RASTER_INSTANCE
 
 }  // namespace swr
