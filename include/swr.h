@@ -107,6 +107,23 @@ extern "C" {
 #define SWR_BLEND_OP_REVERSE_SUBTRACT 2
 #define SWR_BLEND_OP_MIN 3
 #define SWR_BLEND_OP_MAX 4
+/* VK_EXT_blend_operation_advanced, the modes the reference implements
+ * (PixelRoutine.cpp:1506-1615); the colour and alpha ops must be equal. */
+#define SWR_BLEND_OP_MULTIPLY_EXT 1000148012
+#define SWR_BLEND_OP_SCREEN_EXT 1000148013
+#define SWR_BLEND_OP_OVERLAY_EXT 1000148014
+#define SWR_BLEND_OP_DARKEN_EXT 1000148015
+#define SWR_BLEND_OP_LIGHTEN_EXT 1000148016
+#define SWR_BLEND_OP_COLORDODGE_EXT 1000148017
+#define SWR_BLEND_OP_COLORBURN_EXT 1000148018
+#define SWR_BLEND_OP_HARDLIGHT_EXT 1000148019
+#define SWR_BLEND_OP_SOFTLIGHT_EXT 1000148020
+#define SWR_BLEND_OP_DIFFERENCE_EXT 1000148021
+#define SWR_BLEND_OP_EXCLUSION_EXT 1000148022
+#define SWR_BLEND_OP_HSL_HUE_EXT 1000148031
+#define SWR_BLEND_OP_HSL_SATURATION_EXT 1000148032
+#define SWR_BLEND_OP_HSL_COLOR_EXT 1000148033
+#define SWR_BLEND_OP_HSL_LUMINOSITY_EXT 1000148034
 /* VkPrimitiveTopology / VkProvokingVertexModeEXT */
 #define SWR_TOPOLOGY_POINT_LIST 0
 #define SWR_TOPOLOGY_LINE_LIST 1

@@ -996,6 +996,95 @@ static float blend_factor_alpha(const Ctx *c, int f, const float *src, const flo
   }
 }
 
+/* Advanced blend modes (computeAdvancedBlendMode, PixelRoutine.cpp:
+ * 1395-1615): factors are ONE, both colours are divided by their alpha
+ * (0 where alpha is 0), the mode combines them per channel or through the
+ * HSL helpers, and the result is weighted by the uncorrelated overlap
+ * terms; alpha is src + dst - src * dst (:1937-1953).  Min / Max are ordered
+ * selects, Abs clears the sign bit, Sqrt is correctly rounded. */
+static int is_advanced_op(int op) {
+  return (op >= SWR_BLEND_OP_MULTIPLY_EXT && op <= SWR_BLEND_OP_EXCLUSION_EXT) ||
+         (op >= SWR_BLEND_OP_HSL_HUE_EXT && op <= SWR_BLEND_OP_HSL_LUMINOSITY_EXT);
+}
+static float adv_overlay(float s, float d) { return d > 0.5f ? 1.0f - (2.0f * (1.0f - s) * (1.0f - d)) : 2.0f * s * d; }
+static float adv_hardlight(float s, float d) { return s > 0.5f ? 1.0f - (2.0f * (1.0f - s) * (1.0f - d)) : 2.0f * s * d; }
+static float adv_colordodge(float s, float d) {
+  if (!(d > 0.0f)) return 0.0f;
+  return s < 1.0f ? rr_min(1.0f, d / (1.0f - s)) : 1.0f;
+}
+static float adv_colorburn(float s, float d) {
+  if (!(d < 1.0f)) return 1.0f;
+  return s > 0.0f ? 1.0f - rr_min(1.0f, (1.0f - d) / s) : 0.0f;
+}
+static float adv_softlight(float s, float d) {
+  if (!(s > 0.5f)) return d - ((1.0f - (2.0f * s)) * d * (1.0f - d));
+  if (!(d > 0.25f)) return d + (((2.0f * s) - 1.0f) * d * ((((16.0f * d) - 12.0f) * d) + 3.0f));
+  return d + (((2.0f * s) - 1.0f) * (sqrtf(d) - d));
+}
+static float adv_lum(const float c[3]) { return c[0] * 0.3f + c[1] * 0.59f + c[2] * 0.11f; }
+static float adv_min3(const float c[3]) { return rr_min(rr_min(c[0], c[1]), c[2]); }
+static float adv_max3(const float c[3]) { return rr_max(rr_max(c[0], c[1]), c[2]); }
+static void adv_set_lum(const float cbase[3], const float clum[3], float out[3]) {
+  const float ldiff = adv_lum(clum) - adv_lum(cbase);
+  float col[3];
+  for (int k = 0; k < 3; k++) col[k] = cbase[k] + ldiff;
+  const float lum = adv_lum(col), mincol = adv_min3(col), maxcol = adv_max3(col);
+  const int negative = mincol < 0.0f, aboveOne = maxcol > 1.0f;
+  for (int k = 0; k < 3; k++) {
+    if (negative) out[k] = lum + ((col[k] - lum) * lum) / (lum - mincol);
+    else if (aboveOne) out[k] = lum + ((col[k] - lum) * (1.0f - lum)) / (maxcol - lum);
+    else out[k] = col[k];
+  }
+}
+static void adv_set_lum_sat(const float cbase[3], const float csat[3], const float clum[3], float out[3]) {
+  const float minbase = adv_min3(cbase);
+  const float sbase = adv_max3(cbase) - minbase;
+  const float ssat = adv_max3(csat) - adv_min3(csat);
+  float col[3];
+  for (int k = 0; k < 3; k++) col[k] = sbase > 0.0f ? (cbase[k] - minbase) * ssat / sbase : 0.0f;
+  adv_set_lum(col, clum, out);
+}
+static void advanced_blend(int op, const float src[4], const float dst[4], float out[4]) {
+  float s[4], d[4];
+  for (int k = 0; k < 4; k++) { s[k] = src[k] * 1.0f; d[k] = dst[k] * 1.0f; }
+  for (int k = 0; k < 3; k++) { /* premultiply(): divide by alpha */
+    s[k] = s[3] != 0.0f ? s[k] / s[3] : 0.0f;
+    d[k] = d[3] != 0.0f ? d[k] / d[3] : 0.0f;
+  }
+  float b[3];
+  switch (op) {
+    case SWR_BLEND_OP_HSL_HUE_EXT: adv_set_lum_sat(s, d, d, b); break;
+    case SWR_BLEND_OP_HSL_SATURATION_EXT: adv_set_lum_sat(d, s, d, b); break;
+    case SWR_BLEND_OP_HSL_COLOR_EXT: adv_set_lum(s, d, b); break;
+    case SWR_BLEND_OP_HSL_LUMINOSITY_EXT: adv_set_lum(d, s, b); break;
+    default:
+      for (int k = 0; k < 3; k++) {
+        const float x = s[k], y = d[k];
+        switch (op) {
+          case SWR_BLEND_OP_MULTIPLY_EXT: b[k] = x * y; break;
+          case SWR_BLEND_OP_SCREEN_EXT: b[k] = x + y - (x * y); break;
+          case SWR_BLEND_OP_OVERLAY_EXT: b[k] = adv_overlay(x, y); break;
+          case SWR_BLEND_OP_DARKEN_EXT: b[k] = rr_min(x, y); break;
+          case SWR_BLEND_OP_LIGHTEN_EXT: b[k] = rr_max(x, y); break;
+          case SWR_BLEND_OP_COLORDODGE_EXT: b[k] = adv_colordodge(x, y); break;
+          case SWR_BLEND_OP_COLORBURN_EXT: b[k] = adv_colorburn(x, y); break;
+          case SWR_BLEND_OP_HARDLIGHT_EXT: b[k] = adv_hardlight(x, y); break;
+          case SWR_BLEND_OP_SOFTLIGHT_EXT: b[k] = adv_softlight(x, y); break;
+          case SWR_BLEND_OP_DIFFERENCE_EXT: b[k] = u2f(f2u(x - y) & 0x7FFFFFFFu); break;
+          default: b[k] = x + y - (x * y * 2.0f); break; /* EXCLUSION */
+        }
+      }
+  }
+  float p = s[3] * d[3];
+  for (int k = 0; k < 3; k++) b[k] *= p;
+  p = s[3] * (1.0f - d[3]);
+  for (int k = 0; k < 3; k++) b[k] += s[k] * p;
+  p = d[3] * (1.0f - s[3]);
+  for (int k = 0; k < 3; k++) b[k] += d[k] * p;
+  for (int k = 0; k < 3; k++) out[k] = b[k];
+  out[3] = src[3] + dst[3] - (src[3] * dst[3]);
+}
+
 static float blend_apply(int op, float s, float sf, float d, float df) {
   switch (op) {
     case SWR_BLEND_OP_ADD: return s * sf + d * df;
@@ -1028,6 +1117,10 @@ static void alpha_blend(Ctx *c, const uint8_t *buf, const float src[4], float ou
   }
   sf[3] = blend_factor_alpha(c, c->srcFA, src, dst);
   df[3] = blend_factor_alpha(c, c->dstFA, src, dst);
+  if (is_advanced_op(c->colorOp)) { /* its factors are ONE */
+    advanced_blend(c->colorOp, src, dst, out);
+    return;
+  }
   for (int k = 0; k < 3; k++) out[k] = blend_apply(c->colorOp, src[k], sf[k], dst[k], df[k]);
   out[3] = blend_apply(c->alphaOp, src[3], sf[3], dst[3], df[3]);
 }
@@ -1287,7 +1380,7 @@ static int blend_operation(int op, int sf, int df, int isFloat) {
 }
 
 static int blend_factor(int op, int f) {
-  return (op == SWR_BLEND_OP_MIN || op == SWR_BLEND_OP_MAX) ? SWR_BLEND_FACTOR_ONE : f;
+  return (op == SWR_BLEND_OP_MIN || op == SWR_BLEND_OP_MAX || is_advanced_op(op)) ? SWR_BLEND_FACTOR_ONE : f;
 }
 
 static float clampf(float x, float lo, float hi) { return x < lo ? lo : (x > hi ? hi : x); }
@@ -1357,6 +1450,12 @@ static int init_ctx(Ctx *c, const swr_draw_desc *d) {
   if (d->primitive_id.data && (!c->colorIsFloat || d->primitive_id.format != SWR_FORMAT_R32_UINT))
     return SWR_ERROR_UNSUPPORTED;
   const swr_blend_state *b = &d->blend;
+  {
+    const int cr = (int)b->color_blend_op, ar = (int)b->alpha_blend_op;
+    if ((cr > SWR_BLEND_OP_MAX && !is_advanced_op(cr)) || (ar > SWR_BLEND_OP_MAX && !is_advanced_op(ar)))
+      return SWR_ERROR_INVALID_ARGUMENT;
+    if ((is_advanced_op(cr) || is_advanced_op(ar)) && cr != ar) return SWR_ERROR_INVALID_ARGUMENT;
+  }
   int cop = blend_operation((int)b->color_blend_op, (int)b->src_color_blend_factor, (int)b->dst_color_blend_factor, c->colorIsFloat);
   int aop = blend_operation((int)b->alpha_blend_op, (int)b->src_alpha_blend_factor, (int)b->dst_alpha_blend_factor, c->colorIsFloat);
   if (d->color.data && d->color.format != SWR_FORMAT_UNDEFINED)

@@ -46,6 +46,14 @@ BLEND_OP_ADD, BLEND_OP_SUBTRACT, BLEND_OP_REVERSE_SUBTRACT, BLEND_OP_MIN, BLEND_
 
 TOPOLOGY_POINT_LIST, TOPOLOGY_LINE_LIST, TOPOLOGY_LINE_STRIP = 0, 1, 2
 TOPOLOGY_TRIANGLE_LIST, TOPOLOGY_TRIANGLE_STRIP, TOPOLOGY_TRIANGLE_FAN = 3, 4, 5
+BLEND_OP_MULTIPLY_EXT, BLEND_OP_SCREEN_EXT, BLEND_OP_OVERLAY_EXT, BLEND_OP_DARKEN_EXT = (1000148012, 1000148013,
+                                                                                        1000148014, 1000148015)
+BLEND_OP_LIGHTEN_EXT, BLEND_OP_COLORDODGE_EXT, BLEND_OP_COLORBURN_EXT = 1000148016, 1000148017, 1000148018
+BLEND_OP_HARDLIGHT_EXT, BLEND_OP_SOFTLIGHT_EXT, BLEND_OP_DIFFERENCE_EXT = 1000148019, 1000148020, 1000148021
+BLEND_OP_EXCLUSION_EXT = 1000148022
+BLEND_OP_HSL_HUE_EXT, BLEND_OP_HSL_SATURATION_EXT = 1000148031, 1000148032
+BLEND_OP_HSL_COLOR_EXT, BLEND_OP_HSL_LUMINOSITY_EXT = 1000148033, 1000148034
+ADVANCED_BLEND_OPS = tuple(range(1000148012, 1000148023)) + tuple(range(1000148031, 1000148035))
 LINE_RASTERIZATION_DEFAULT, LINE_RASTERIZATION_RECTANGULAR = 0, 1
 LINE_RASTERIZATION_BRESENHAM, LINE_RASTERIZATION_RECTANGULAR_SMOOTH = 2, 3
 PROVOKING_VERTEX_FIRST, PROVOKING_VERTEX_LAST = 0, 1

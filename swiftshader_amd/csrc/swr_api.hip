@@ -204,7 +204,11 @@ int stencil_params(const swr_stencil_op_state &s, StencilParams &o) {
 
 bool valid_compare(uint32_t op) { return op <= 7; }
 bool valid_factor(uint32_t f) { return f <= 14; }
-bool valid_blend_op(uint32_t op) { return op <= 4; }
+bool is_advanced_blend_op(uint32_t op) {
+  return (op >= SWR_BLEND_OP_MULTIPLY_EXT && op <= SWR_BLEND_OP_EXCLUSION_EXT) ||
+         (op >= SWR_BLEND_OP_HSL_HUE_EXT && op <= SWR_BLEND_OP_HSL_LUMINOSITY_EXT);
+}
+bool valid_blend_op(uint32_t op) { return op <= 4 || is_advanced_blend_op(op); }
 
 int build_params(const swr_draw_desc *d, const swr_tuning &tuning, DrawParams &P) {
   memset(&P, 0, sizeof(P));
@@ -352,17 +356,23 @@ int build_params(const swr_draw_desc *d, const swr_tuning &tuning, DrawParams &P
       !valid_factor(b.src_alpha_blend_factor) || !valid_factor(b.dst_alpha_blend_factor) ||
       !valid_blend_op(b.color_blend_op) || !valid_blend_op(b.alpha_blend_op))
     return fail(SWR_ERROR_UNSUPPORTED, "blend state outside the supported VkBlendFactor/VkBlendOp range");
+  if ((is_advanced_blend_op(b.color_blend_op) || is_advanced_blend_op(b.alpha_blend_op)) &&
+      b.color_blend_op != b.alpha_blend_op)
+    return fail(SWR_ERROR_INVALID_ARGUMENT, "an advanced blend op must be both the colour and the alpha op");
   int cop = blend_operation((int)b.color_blend_op, (int)b.src_color_blend_factor, (int)b.dst_color_blend_factor, P.colorFloat);
   int aop = blend_operation((int)b.alpha_blend_op, (int)b.src_alpha_blend_factor, (int)b.dst_alpha_blend_factor, P.colorFloat);
   if (hasColor) P.colorMask = (cop == BLEND_OP_DST && aop == BLEND_OP_DST) ? 0 : (int)(b.color_write_mask & 0xFu);
   P.blendEnable = (hasColor && b.blend_enable && P.colorMask && (cop != BLEND_OP_SRC || aop != BLEND_OP_SRC)) ? 1 : 0;
+  // Blend factors: ONE for MIN, MAX and the advanced modes (Context.cpp:1149-1179).
   auto bf = [](uint32_t op, uint32_t f) {
-    return (op == SWR_BLEND_OP_MIN || op == SWR_BLEND_OP_MAX) ? (int)SWR_BLEND_FACTOR_ONE : (int)f;
+    return (op == SWR_BLEND_OP_MIN || op == SWR_BLEND_OP_MAX || is_advanced_blend_op(op)) ? (int)SWR_BLEND_FACTOR_ONE
+                                                                                          : (int)f;
   };
   if (P.blendEnable) {
     P.srcF = bf(b.color_blend_op, b.src_color_blend_factor);
     P.dstF = bf(b.color_blend_op, b.dst_color_blend_factor);
     P.colorOp = cop;
+    P.advancedBlend = is_advanced_blend_op((uint32_t)cop);
     P.srcFA = bf(b.alpha_blend_op, b.src_alpha_blend_factor);
     P.dstFA = bf(b.alpha_blend_op, b.dst_alpha_blend_factor);
     P.alphaOp = aop;
@@ -489,11 +499,11 @@ int build_params(const swr_draw_desc *d, const swr_tuning &tuning, DrawParams &P
 // ---------------------------------------------------------------------------
 // Raster variant of a draw: samples, float colour target, float texture,
 // and whether it uses a format outside the common RGBA8 / RGBA32F / D32F set
-// (D16 depth, BGRA8 or sRGB colour, BGRA8 texture), which the common
-// variants compile out.
+// (D16 depth, BGRA8 or sRGB colour, BGRA8 texture) or an advanced blend
+// mode, which the common variants compile out.
 int variant_of(const DrawParams &P) {
   const bool ftex = P.useUV && P.tex.isFloat;
-  const bool xf = P.depth16 || P.colorBGRA || P.colorSRGB || (P.useUV && P.tex.bgra);
+  const bool xf = P.depth16 || P.colorBGRA || P.colorSRGB || (P.useUV && P.tex.bgra) || P.advancedBlend;
   return (xf ? 8 : 0) | (P.ms > 1 ? 4 : 0) | (P.colorFloat ? 2 : 0) | (ftex ? 1 : 0);
 }
 

@@ -401,7 +401,88 @@ __device__ __forceinline__ float blend_apply(int op, float s, float sf, float d,
   }
 }
 
+// VK_EXT_blend_operation_advanced modes (computeAdvancedBlendMode,
+// PixelRoutine.cpp:1395-1615, alpha :1937-1953): factors are ONE; both
+// colours divided by their alpha (0 where alpha is 0); per-channel modes or
+// the HSL helpers; then the uncorrelated-overlap weighting.  Abs clears the
+// sign bit; Min / Max are ordered selects.
+__device__ __forceinline__ bool is_advanced_op(int op) {
+  return (op >= 1000148012 && op <= 1000148022) || (op >= 1000148031 && op <= 1000148034);
+}
+__device__ __forceinline__ float adv_lum(float r, float g, float b) { return r * 0.3f + g * 0.59f + b * 0.11f; }
+__device__ __forceinline__ float3 adv_set_lum(float3 cb, float3 cl) {
+  const float ldiff = adv_lum(cl.x, cl.y, cl.z) - adv_lum(cb.x, cb.y, cb.z);
+  const float3 c = make_float3(cb.x + ldiff, cb.y + ldiff, cb.z + ldiff);
+  const float lum = adv_lum(c.x, c.y, c.z);
+  const float mn = rr_min(rr_min(c.x, c.y), c.z), mx = rr_max(rr_max(c.x, c.y), c.z);
+  if (mn < 0.0f) {
+    return make_float3(lum + ((c.x - lum) * lum) / (lum - mn), lum + ((c.y - lum) * lum) / (lum - mn),
+                       lum + ((c.z - lum) * lum) / (lum - mn));
+  }
+  if (mx > 1.0f) {
+    return make_float3(lum + ((c.x - lum) * (1.0f - lum)) / (mx - lum), lum + ((c.y - lum) * (1.0f - lum)) / (mx - lum),
+                       lum + ((c.z - lum) * (1.0f - lum)) / (mx - lum));
+  }
+  return c;
+}
+__device__ __forceinline__ float3 adv_set_lum_sat(float3 cb, float3 cs, float3 cl) {
+  const float mnb = rr_min(rr_min(cb.x, cb.y), cb.z);
+  const float sb = rr_max(rr_max(cb.x, cb.y), cb.z) - mnb;
+  const float ss = rr_max(rr_max(cs.x, cs.y), cs.z) - rr_min(rr_min(cs.x, cs.y), cs.z);
+  const bool nz = sb > 0.0f;
+  const float3 c = make_float3(nz ? (cb.x - mnb) * ss / sb : 0.0f, nz ? (cb.y - mnb) * ss / sb : 0.0f,
+                               nz ? (cb.z - mnb) * ss / sb : 0.0f);
+  return adv_set_lum(c, cl);
+}
+__device__ __forceinline__ float adv_channel(int op, float x, float y) {
+  switch (op) {
+    case 1000148012: return x * y;                                   // MULTIPLY
+    case 1000148013: return x + y - (x * y);                         // SCREEN
+    case 1000148014:                                                 // OVERLAY
+      return y > 0.5f ? 1.0f - (2.0f * (1.0f - x) * (1.0f - y)) : 2.0f * x * y;
+    case 1000148015: return rr_min(x, y);                            // DARKEN
+    case 1000148016: return rr_max(x, y);                            // LIGHTEN
+    case 1000148017:                                                 // COLORDODGE
+      return !(y > 0.0f) ? 0.0f : (x < 1.0f ? rr_min(1.0f, y / (1.0f - x)) : 1.0f);
+    case 1000148018:                                                 // COLORBURN
+      return !(y < 1.0f) ? 1.0f : (x > 0.0f ? 1.0f - rr_min(1.0f, (1.0f - y) / x) : 0.0f);
+    case 1000148019:                                                 // HARDLIGHT
+      return x > 0.5f ? 1.0f - (2.0f * (1.0f - x) * (1.0f - y)) : 2.0f * x * y;
+    case 1000148020:                                                 // SOFTLIGHT
+      if (!(x > 0.5f)) return y - ((1.0f - (2.0f * x)) * y * (1.0f - y));
+      if (!(y > 0.25f)) return y + (((2.0f * x) - 1.0f) * y * ((((16.0f * y) - 12.0f) * y) + 3.0f));
+      return y + (((2.0f * x) - 1.0f) * (__builtin_sqrtf(y) - y));
+    case 1000148021: return __uint_as_float(__float_as_uint(x - y) & 0x7FFFFFFFu);  // DIFFERENCE
+    default: return x + y - (x * y * 2.0f);                          // EXCLUSION
+  }
+}
+__device__ __noinline__ float4 advanced_blend(int op, float4 src, float4 dst) {
+  float4 s = src, d = dst;
+  s.x = s.w != 0.0f ? s.x / s.w : 0.0f;
+  s.y = s.w != 0.0f ? s.y / s.w : 0.0f;
+  s.z = s.w != 0.0f ? s.z / s.w : 0.0f;
+  d.x = d.w != 0.0f ? d.x / d.w : 0.0f;
+  d.y = d.w != 0.0f ? d.y / d.w : 0.0f;
+  d.z = d.w != 0.0f ? d.z / d.w : 0.0f;
+  const float3 sc = make_float3(s.x, s.y, s.z), dc = make_float3(d.x, d.y, d.z);
+  float3 b;
+  if (op == 1000148031) b = adv_set_lum_sat(sc, dc, dc);       // HSL_HUE
+  else if (op == 1000148032) b = adv_set_lum_sat(dc, sc, dc);  // HSL_SATURATION
+  else if (op == 1000148033) b = adv_set_lum(sc, dc);          // HSL_COLOR
+  else if (op == 1000148034) b = adv_set_lum(dc, sc);          // HSL_LUMINOSITY
+  else b = make_float3(adv_channel(op, s.x, d.x), adv_channel(op, s.y, d.y), adv_channel(op, s.z, d.z));
+  float p = s.w * d.w;
+  b.x = b.x * p; b.y = b.y * p; b.z = b.z * p;
+  p = s.w * (1.0f - d.w);
+  b.x = b.x + s.x * p; b.y = b.y + s.y * p; b.z = b.z + s.z * p;
+  p = d.w * (1.0f - s.w);
+  b.x = b.x + d.x * p; b.y = b.y + d.y * p; b.z = b.z + d.z * p;
+  return make_float4(b.x, b.y, b.z, src.w + dst.w - (src.w * dst.w));
+}
+
+template <bool XF>
 __device__ __forceinline__ float4 alpha_blend(const DrawParams &P, float4 s, float4 d) {
+  if (XF && is_advanced_op(P.colorOp)) return advanced_blend(P.colorOp, s, d);
   float sfr = blend_factor_rgb(P, P.srcF, s.x, s.w, d.x, d.w, 0);
   float sfg = blend_factor_rgb(P, P.srcF, s.y, s.w, d.y, d.w, 1);
   float sfb = blend_factor_rgb(P, P.srcF, s.z, s.w, d.z, d.w, 2);

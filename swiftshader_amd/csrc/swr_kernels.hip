@@ -733,7 +733,7 @@ __device__ __forceinline__ void blend_write(const DrawParams &P, PixelState<MS, 
         dst.z = srgb_to_linear(dst.z);
       }
     }
-    float4 o = P.blendEnable ? alpha_blend(P, s, dst) : s;
+    float4 o = P.blendEnable ? alpha_blend<XF>(P, s, dst) : s;
     if (!FCOL && (XF && P.colorSRGB)) {  // writeColor encodes R, G, B first (PixelRoutine.cpp:1963-1969)
       o.x = linear_to_srgb(o.x);
       o.y = linear_to_srgb(o.y);

@@ -79,6 +79,7 @@ struct DrawParams {
   int colorBGRA;   // B8G8R8A8 target (R and B exchanged in memory)
   int colorSRGB;   // sRGB-encoded 8-bit target
   int blendEnable;
+  int advancedBlend;  // colorOp (= alphaOp) is a VK_EXT_blend_operation_advanced mode
   int srcF, dstF, colorOp, srcFA, dstFA, alphaOp;
   float blendConst[4], invBlendConst[4];
   int shader, useColor, useUV;

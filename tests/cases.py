@@ -355,6 +355,25 @@ def _bc_textured(seed=51, n=1, aniso=False, samples=1):
                     shader=abi.SHADER_MODULATE, texture=tex)
 
 
+def _advanced_blend(op, k):
+    """Overlapping translucent triangles blended with one advanced mode;
+    the target format and sample count rotate with k."""
+    rng = np.random.default_rng(60 + k)
+    W, H = 48, 40
+    pos, uv, col = scenes.screen_triangles(rng, 260, W, H, area_hi=500)
+    col = col.astype(np.float32)
+    col[:, 3] = rng.uniform(0.0, 1.0, len(col)).astype(np.float32)
+    col[::7, 3] = 0.0  # zero alpha: colours divide to zero
+    if k % 3 == 2:
+        col = (col * 1.3 - 0.1).astype(np.float32)  # out of range on a float target
+    fmt = (abi.FORMAT_R8G8B8A8_UNORM, abi.FORMAT_B8G8R8A8_SRGB, abi.FORMAT_R32G32B32A32_SFLOAT)[k % 3]
+    return DrawData(width=W, height=H, positions=pos, colors=col, shader=abi.SHADER_INTERPOLATE,
+                    samples=4 if k % 4 == 1 else 1, color_format=fmt, blend_enable=1,
+                    blend=(abi.BLEND_FACTOR_SRC_ALPHA, abi.BLEND_FACTOR_ZERO, op, abi.BLEND_FACTOR_ONE,
+                           abi.BLEND_FACTOR_ZERO, op),
+                    clear_color=(0.2, 0.5, 0.8, 0.6), depth_compare=abi.COMPARE_LESS_OR_EQUAL)
+
+
 def parity_cases():
     """(name, DrawData) pairs; every one must match the oracle bit for bit."""
     return [
@@ -408,4 +427,4 @@ def parity_cases():
         ("srgb8_opaque_d16", _formats(seed=50, srgb=True)),
         ("bc1_texture_bgra", _bc_textured()),
         ("bc3_texture_bgra_aniso_msaa4", _bc_textured(seed=52, n=3, aniso=True, samples=4)),
-    ]
+    ] + [(f"advanced_blend_{op - 1000148000}", _advanced_blend(op, k)) for k, op in enumerate(abi.ADVANCED_BLEND_OPS)]
