@@ -65,6 +65,12 @@ __attribute__((visibility("default"))) void ref_constants(int *xf, int *yf, int 
   }
 }
 
+// sw::Constants' sRGB decode table for 8-bit sampled texels.
+__attribute__((visibility("default"))) void ref_srgb_lut(uint16_t *out) {
+  static sw::Constants *constants = new sw::Constants();
+  for (int i = 0; i < 256; i++) out[i] = constants->sRGBtoLinearFF_FF00[i];
+}
+
 // BC_Decoder::Decode as the reference's Image::decodeBC calls it.
 __attribute__((visibility("default"))) int ref_decode_bc(const uint8_t *src, uint8_t *dst, int w, int h, int dstPitch,
                                                           int dstBpp, int n, int isNoAlphaU) {

@@ -130,8 +130,10 @@ static int kConstantsReady = 0;
 
 static int to_fixed_point(float v, int p) { return (int)(v * (float)(1 << p)); }
 
+static void init_srgb_lut(void);
 static void init_constants(void) {
   if (kConstantsReady) return;
+  init_srgb_lut();
   for (int q = 0; q < 4; q++) {
     kXf[q] = to_fixed_point(kSampleLocationsX[q], 4);
     kYf[q] = to_fixed_point(kSampleLocationsY[q], 4);
@@ -204,7 +206,8 @@ typedef struct {
   uint16_t uHalf[SWR_MAX_MIP_LEVELS], vHalf[SWR_MAX_MIP_LEVELS];
   float wwhh[4];
   int isFloat;
-  int bgra; /* B8G8R8A8_UNORM texels */
+  int bgra; /* B8G8R8A8 texels */
+  int srgb; /* sRGB texels */
   /* sw::Sampler subset */
   int filter, mipmap, addrU, addrV;
   float mipLodBias, maxAnisotropy, minLod, maxLod;
@@ -588,7 +591,24 @@ static int setup_routine(Ctx *c, OPrim *prim, const OVertex *tri[3], const Polyg
 /* Sampler (SamplerCore.cpp)                                              */
 /* ===================================================================== */
 
-/* 16-bit texel fetch, RGBA8 -> (c << 8) per channel (sampleTexel, :1678-1800). */
+/* The sampler's sRGB decode table (Constants.cpp:273-276): sw::sRGBtoLinear
+ * (Math.hpp:326-336, libm powf) of i / 255, scaled to 0xFF00, rounded half
+ * up by truncation of x + 0.5. */
+static uint16_t g_srgb_lut[256];
+static void init_srgb_lut(void) {
+  for (int i = 0; i < 256; i++) {
+    volatile float c = (float)i / 255.0f; /* a runtime powf, as the reference's */
+    const float lin = c <= 0.04045f ? c / 12.92f : powf((c + 0.055f) / 1.055f, 2.4f);
+    g_srgb_lut[i] = (uint16_t)(lin * 65280.0f + 0.5f);
+  }
+}
+EXPORT void swo_srgb_lut(uint16_t *out) {
+  init_srgb_lut();
+  memcpy(out, g_srgb_lut, sizeof(g_srgb_lut));
+}
+
+/* 16-bit texel fetch, RGBA8 -> (c << 8) per channel (sampleTexel, :1678-1800);
+ * sRGB texels take R, G, B through the decode table (:1966-1978, 2670-2680). */
 static void texel16(const OTexture *t, int level, uint16_t u, uint16_t v, uint16_t out[4]) {
   uint16_t uu = mulhi_u16(u, (uint16_t)t->width[level]);
   uint16_t vv = mulhi_u16(v, (uint16_t)t->height[level]);
@@ -596,6 +616,8 @@ static void texel16(const OTexture *t, int level, uint16_t u, uint16_t v, uint16
   int32_t index = (int32_t)(int16_t)uu * 1 + (int32_t)(int16_t)vv * (int32_t)(int16_t)t->pitchP[level];
   const uint8_t *p = t->data[level] + (size_t)(uint32_t)index * 4;
   for (int k = 0; k < 4; k++) out[k] = (uint16_t)(p[k] << 8);
+  if (t->srgb)
+    for (int k = 0; k < 3; k++) out[k] = g_srgb_lut[p[k]];
 }
 
 static uint16_t address16(float uw, int mode) {
@@ -1485,7 +1507,8 @@ static int init_ctx(Ctx *c, const swr_draw_desc *d) {
     const swr_sampler_state *s = &d->sampler;
     if (tx->level_count < 1 || tx->level_count > SWR_MAX_MIP_LEVELS) return SWR_ERROR_INVALID_ARGUMENT;
     t->isFloat = tx->format == SWR_FORMAT_R32G32B32A32_SFLOAT;
-    t->bgra = tx->format == SWR_FORMAT_B8G8R8A8_UNORM;
+    t->bgra = tx->format == SWR_FORMAT_B8G8R8A8_UNORM || tx->format == SWR_FORMAT_B8G8R8A8_SRGB;
+    t->srgb = tx->format == SWR_FORMAT_R8G8B8A8_SRGB || tx->format == SWR_FORMAT_B8G8R8A8_SRGB;
     int texelBytes = t->isFloat ? 16 : 4;
     for (int l = 0; l < SWR_MAX_MIP_LEVELS; l++) {
       int vl = imin(l, (int)tx->level_count - 1);

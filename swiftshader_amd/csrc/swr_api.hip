@@ -202,6 +202,22 @@ int stencil_params(const swr_stencil_op_state &s, StencilParams &o) {
   return SWR_OK;
 }
 
+// The sampler's sRGB decode table, as the reference's Constants builds it
+// (Constants.cpp:273-276): sw::sRGBtoLinear (Math.hpp:326-336, the host's
+// powf) of i / 255, scaled to 0xFF00 and rounded half up.
+const uint16_t *srgb_decode_table() {
+  static const std::vector<uint16_t> table = [] {
+    std::vector<uint16_t> t(256);
+    for (int i = 0; i < 256; i++) {
+      volatile float c = (float)i / 0xFF;  // keeps powf a call to the host's libm
+      const float lin = c <= 0.04045f ? c / 12.92f : powf((c + 0.055f) / 1.055f, 2.4f);
+      t[i] = (uint16_t)(lin * 0xFF00 + 0.5f);
+    }
+    return t;
+  }();
+  return table.data();
+}
+
 bool valid_compare(uint32_t op) { return op <= 7; }
 bool valid_factor(uint32_t f) { return f <= 14; }
 bool is_advanced_blend_op(uint32_t op) {
@@ -397,10 +413,14 @@ int build_params(const swr_draw_desc *d, const swr_tuning &tuning, DrawParams &P
     if (tx.level_count < 1 || tx.level_count > SWR_MAX_MIP_LEVELS)
       return fail(SWR_ERROR_INVALID_ARGUMENT, "texture level_count %u outside 1..15", tx.level_count);
     if (tx.format != SWR_FORMAT_R8G8B8A8_UNORM && tx.format != SWR_FORMAT_B8G8R8A8_UNORM &&
+        tx.format != SWR_FORMAT_R8G8B8A8_SRGB && tx.format != SWR_FORMAT_B8G8R8A8_SRGB &&
         tx.format != SWR_FORMAT_R32G32B32A32_SFLOAT)
       return fail(SWR_ERROR_UNSUPPORTED, "texture format %u", tx.format);
     T.isFloat = tx.format == SWR_FORMAT_R32G32B32A32_SFLOAT;
-    T.bgra = tx.format == SWR_FORMAT_B8G8R8A8_UNORM;  // e.g. decoded BC1-BC3
+    // B8G8R8A8 (e.g. decoded BC1-BC3) keeps R and B exchanged in memory.
+    T.bgra = tx.format == SWR_FORMAT_B8G8R8A8_UNORM || tx.format == SWR_FORMAT_B8G8R8A8_SRGB;
+    T.srgb = tx.format == SWR_FORMAT_R8G8B8A8_SRGB || tx.format == SWR_FORMAT_B8G8R8A8_SRGB;
+    if (T.srgb) memcpy(T.srgbLut, srgb_decode_table(), sizeof(T.srgbLut));
     uint32_t texelBytes = T.isFloat ? 16 : 4;
     for (int l = 0; l < SWR_MAX_MIP_LEVELS; l++) {
       const swr_image_level &lv = tx.levels[std::min<uint32_t>(l, tx.level_count - 1)];
@@ -503,7 +523,7 @@ int build_params(const swr_draw_desc *d, const swr_tuning &tuning, DrawParams &P
 // mode, which the common variants compile out.
 int variant_of(const DrawParams &P) {
   const bool ftex = P.useUV && P.tex.isFloat;
-  const bool xf = P.depth16 || P.colorBGRA || P.colorSRGB || (P.useUV && P.tex.bgra) || P.advancedBlend;
+  const bool xf = P.depth16 || P.colorBGRA || P.colorSRGB || (P.useUV && (P.tex.bgra || P.tex.srgb)) || P.advancedBlend;
   return (xf ? 8 : 0) | (P.ms > 1 ? 4 : 0) | (P.colorFloat ? 2 : 0) | (ftex ? 1 : 0);
 }
 

@@ -91,6 +91,7 @@ struct TexLevel {
   const void *data;
   int width, height, pitch;
   uint32_t uHalf, vHalf;
+  const uint16_t *srgb;  // sRGB texels: the decode table, else null
 };
 
 __device__ __forceinline__ TexLevel tex_level(const TexParams &T, int lvl) {
@@ -101,9 +102,11 @@ __device__ __forceinline__ TexLevel tex_level(const TexParams &T, int lvl) {
   L.pitch = T.pitchP[lvl];
   L.uHalf = T.uHalf[lvl];
   L.vHalf = T.vHalf[lvl];
+  L.srgb = T.srgb ? T.srgbLut : nullptr;
   return L;
 }
 
+template <bool XF>
 __device__ __forceinline__ void fetch16(const TexLevel &L, uint32_t u, uint32_t v, uint32_t out[4]) {
   uint32_t uu = mulhi16(u, (uint32_t)L.width);
   uint32_t vv = mulhi16(v, (uint32_t)L.height);
@@ -113,6 +116,11 @@ __device__ __forceinline__ void fetch16(const TexLevel &L, uint32_t u, uint32_t 
   out[1] = ((p >> 8) & 0xFFu) << 8;
   out[2] = ((p >> 16) & 0xFFu) << 8;
   out[3] = ((p >> 24) & 0xFFu) << 8;
+  if (XF && L.srgb) {  // sRGBtoLinearFF00 (SamplerCore.cpp:1966-1978, 2670-2680): R, G, B through the table
+    out[0] = L.srgb[out[0] >> 8];
+    out[1] = L.srgb[out[1] >> 8];
+    out[2] = L.srgb[out[2] >> 8];
+  }
 }
 
 __device__ __forceinline__ int select_mip(const TexParams &T, float lod, int second) {
@@ -122,12 +130,13 @@ __device__ __forceinline__ int select_mip(const TexParams &T, float lod, int sec
 }
 
 // sampleQuad2D + bilinearInterpolate, RGBA8 16-bit path (:668-909, :516-666).
+template <bool XF>
 __device__ __forceinline__ void sample_quad16(const TexParams &T, const TexLevel &L, float u, float v, float lod,
                                               uint32_t out[4]) {
   uint32_t uu = address16(u, T.addrU);
   uint32_t vv = address16(v, T.addrV);
   if (T.filter == FILT_POINT) {
-    fetch16(L, uu, vv, out);
+    fetch16<XF>(L, uu, vv, out);
     return;
   }
   uint32_t uh = L.uHalf, vh = L.vHalf;
@@ -139,10 +148,10 @@ __device__ __forceinline__ void sample_quad16(const TexParams &T, const TexLevel
   uint32_t u0 = offset16(uu, uh, !T.addrU, -1), v0 = offset16(vv, vh, !T.addrV, -1);
   uint32_t u1 = offset16(uu, uh, !T.addrU, +1), v1 = offset16(vv, vh, !T.addrV, +1);
   uint32_t c00[4], c10[4], c01[4], c11[4];
-  fetch16(L, u0, v0, c00);
-  fetch16(L, u1, v0, c10);
-  fetch16(L, u0, v1, c01);
-  fetch16(L, u1, v1, c11);
+  fetch16<XF>(L, u0, v0, c00);
+  fetch16<XF>(L, u1, v0, c10);
+  fetch16<XF>(L, u0, v1, c01);
+  fetch16<XF>(L, u1, v1, c11);
   uint32_t f0u = (u0 * (uint32_t)L.width) & 0xFFFFu;
   uint32_t f0v = (v0 * (uint32_t)L.height) & 0xFFFFu;
   uint32_t f1u = f0u ^ 0xFFFFu, f1v = f0v ^ 0xFFFFu;
@@ -156,11 +165,12 @@ __device__ __forceinline__ void sample_quad16(const TexParams &T, const TexLevel
   }
 }
 
+template <bool XF>
 __device__ __forceinline__ void sample_aniso16(const TexParams &T, float u, float v, float lod, float aniso,
                                                float uDelta, float vDelta, int second, uint32_t out[4]) {
   const TexLevel L = tex_level(T, select_mip(T, lod, second));
   if (T.filter != FILT_ANISOTROPIC) {
-    sample_quad16(T, L, u, v, lod, out);
+    sample_quad16<XF>(T, L, u, v, lod, out);
     return;
   }
   int N = rr_round_int(aniso);
@@ -176,7 +186,7 @@ __device__ __forceinline__ void sample_aniso16(const TexParams &T, float u, floa
   int i = 0;
   do {
     uint32_t c[4];
-    sample_quad16(T, L, u0, v0, lod, c);
+    sample_quad16<XF>(T, L, u0, v0, lod, c);
     u0 = u0 + du;
     v0 = v0 + dv;
 #pragma unroll
@@ -328,10 +338,10 @@ __device__ __forceinline__ float4 sample_lane(const TexParams &T, float u, float
     return c;
   }
   uint32_t c[4];
-  sample_aniso16(T, u, v, L.lod, L.aniso, L.uDelta, L.vDelta, 0, c);
+  sample_aniso16<XF>(T, u, v, L.lod, L.aniso, L.uDelta, L.vDelta, 0, c);
   if (T.mipmap == MIP_LINEAR) {
     uint32_t cc[4];
-    sample_aniso16(T, u, v, L.lod, L.aniso, L.uDelta, L.vDelta, 1, cc);
+    sample_aniso16<XF>(T, u, v, L.lod, L.aniso, L.uDelta, L.vDelta, 1, cc);
     uint32_t utri = (uint32_t)rr_trunc_int(L.lod * 65536.0f) & 0xFFFFu;
     uint32_t inv = utri ^ 0xFFFFu;
 #pragma unroll

@@ -31,7 +31,9 @@ struct TexParams {
   uint32_t vHalf[kMipLevels];
   float wwhh[4];
   int isFloat;
-  int bgra;  // B8G8R8A8_UNORM texels: R and B exchanged in memory
+  int bgra;  // B8G8R8A8 texels: R and B exchanged in memory
+  int srgb;  // sRGB texels, decoded through srgbLut
+  uint16_t srgbLut[256];  // sRGBtoLinearFF_FF00 (Constants.cpp:273-276)
   int filter, mipmap, addrU, addrV;  // addr: 0 wrap, 1 clamp
   int skipLod;
   float mipLodBias, maxAnisotropy, minLod, maxLod;

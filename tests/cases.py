@@ -374,6 +374,20 @@ def _advanced_blend(op, k):
                     clear_color=(0.2, 0.5, 0.8, 0.6), depth_compare=abi.COMPARE_LESS_OR_EQUAL)
 
 
+def _srgb_textured(seed=53, bgra=False, aniso=False, samples=1, target=abi.FORMAT_R8G8B8A8_UNORM):
+    """Triangles sampling an sRGB-encoded texture (decoded per texel before
+    filtering)."""
+    rng = np.random.default_rng(seed)
+    levels = scenes.box_mip_chain(scenes.noise_texture_rgba8(rng, 64))
+    if bgra:
+        levels = [np.ascontiguousarray(l[..., [2, 1, 0, 3]]) for l in levels]
+    tex = TextureData(format=abi.FORMAT_B8G8R8A8_SRGB if bgra else abi.FORMAT_R8G8B8A8_SRGB, levels=levels,
+                      anisotropy_enable=1 if aniso else 0, max_anisotropy=8.0 if aniso else 1.0)
+    pos, uv, col = scenes.screen_triangles(rng, 1000, 88, 66, area_hi=1500)
+    return DrawData(width=88, height=66, positions=pos, colors=col, texcoords=uv, samples=samples,
+                    shader=abi.SHADER_MODULATE, texture=tex, color_format=target)
+
+
 def parity_cases():
     """(name, DrawData) pairs; every one must match the oracle bit for bit."""
     return [
@@ -427,4 +441,7 @@ def parity_cases():
         ("srgb8_opaque_d16", _formats(seed=50, srgb=True)),
         ("bc1_texture_bgra", _bc_textured()),
         ("bc3_texture_bgra_aniso_msaa4", _bc_textured(seed=52, n=3, aniso=True, samples=4)),
+        ("srgb_texture_trilinear", _srgb_textured()),
+        ("bgra_srgb_texture_aniso_msaa4_srgb_target", _srgb_textured(seed=54, bgra=True, aniso=True, samples=4,
+                                                                     target=abi.FORMAT_R8G8B8A8_SRGB)),
     ] + [(f"advanced_blend_{op - 1000148000}", _advanced_blend(op, k)) for k, op in enumerate(abi.ADVANCED_BLEND_OPS)]

@@ -48,6 +48,8 @@ def oracle():
             getattr(L, f).argtypes = [ctypes.c_float]
             getattr(L, f).restype = ctypes.c_float
         L.swo_constants.restype = None
+        L.swo_srgb_lut.argtypes = [ctypes.c_void_p]
+        L.swo_srgb_lut.restype = None
         L.swo_decode_bc.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 6
         L.swo_decode_bc.restype = ctypes.c_int
         fp = ctypes.POINTER(ctypes.c_float)
@@ -66,6 +68,8 @@ def reference_lib():
         L.ref_clip_triangle.argtypes = [fp, fp, fp, ctypes.c_int, fp, ctypes.POINTER(ctypes.c_int)]
         L.ref_clip_triangle.restype = ctypes.c_int
         L.ref_constants.restype = None
+        L.ref_srgb_lut.argtypes = [ctypes.c_void_p]
+        L.ref_srgb_lut.restype = None
         L.ref_decode_bc.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 6
         L.ref_decode_bc.restype = ctypes.c_int
         _ref = L

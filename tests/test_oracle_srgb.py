@@ -125,3 +125,43 @@ def test_clear_encodes_colour_and_leaves_alpha_linear():
     t = np.zeros((1, 1, 1, 4), np.uint8)
     oracle_clear(t, abi.FORMAT_R8G8B8A8_SRGB, [float("nan"), -1.0, 2.0, float("nan")])
     assert list(t[0, 0, 0]) == [255, 0, 255, 255]  # NaN passes the clamp; Min(NaN, 255) selects 255
+
+
+def test_texture_decode_table_matches_the_reference_and_the_curve():
+    """sRGB texels decode through a 256-entry table in 0xFF00 units: equal to
+    the reference's Constants table (compiled in oracle/_ref) and within one
+    unit of the exact curve, exact at 0 and 255."""
+    from oracle_runner import reference_lib
+    got = np.zeros(256, np.uint16)
+    oracle().swo_srgb_lut(got.ctypes.data)
+    exact = _dec(np.arange(256) / 255.0) * 65280.0
+    assert np.abs(got - exact).max() <= 0.51  # rounding to nearest, less float and powf error
+    assert got[0] == 0 and got[255] == 65280 and np.all(np.diff(got.astype(int)) >= 0)
+    ref = reference_lib()
+    if ref is not None:
+        want = np.zeros(256, np.uint16)
+        ref.ref_srgb_lut(want.ctypes.data)
+        assert np.array_equal(got, want)
+
+
+def test_srgb_texture_samples_the_decoded_texels():
+    """An sRGB texture renders as the RGBA16-linear texture it decodes to:
+    with nearest filtering and one mip, each fragment is its texel's table
+    entry / 0xFF00 (alpha linear)."""
+    from swiftshader_amd import scenes
+    rng = np.random.default_rng(5)
+    tex8 = rng.integers(0, 256, (8, 8, 4), dtype=np.uint8)
+    lut = np.zeros(256, np.uint16)
+    oracle().swo_srgb_lut(lut.ctypes.data)
+    xy = np.array([[-1, -1], [1, -1], [1, 1], [-1, -1], [1, 1], [-1, 1]], np.float64)
+    pos = np.concatenate([xy, np.full((6, 1), 0.5), np.ones((6, 1))], axis=1).astype(np.float32)
+    uv = ((xy + 1) / 2).astype(np.float32)
+    for fmt, data in ((abi.FORMAT_R8G8B8A8_SRGB, tex8), (abi.FORMAT_B8G8R8A8_SRGB, tex8[..., [2, 1, 0, 3]].copy())):
+        tex = scenes.TextureData(format=fmt, levels=[data], mag_filter=abi.FILTER_NEAREST,
+                                 min_filter=abi.FILTER_NEAREST, mipmap_mode=abi.MIPMAP_MODE_NEAREST)
+        d = DrawData(width=8, height=8, positions=pos, texcoords=uv, colors=np.ones((6, 4), np.float32),
+                     shader=abi.SHADER_SAMPLE, texture=tex, color_format=abi.FORMAT_R32G32B32A32_SFLOAT)
+        got = oracle_render(d)[0][0]
+        want = np.concatenate([lut[tex8[..., :3]] / np.float32(65280), (tex8[..., 3:].astype(np.float32) * 256)
+                               / np.float32(65280)], -1)
+        assert np.allclose(got, want, atol=1e-7), fmt
