@@ -199,7 +199,7 @@ typedef struct swr_image_level { /* one mip level of a 2D image view */
 } swr_image_level;
 
 typedef struct swr_texture { /* combined image sampler, 2D view */
-  uint32_t format;          /* R8G8B8A8_UNORM, B8G8R8A8_UNORM or R32G32B32A32_SFLOAT */
+  uint32_t format;          /* R8G8B8A8 / B8G8R8A8 (UNORM or SRGB), or R32G32B32A32_SFLOAT */
   uint32_t level_count;     /* levels in the view, 1..15                */
   swr_image_level levels[SWR_MAX_MIP_LEVELS];
 } swr_texture;
