@@ -1929,9 +1929,47 @@ EXPORT int swo_clear(const swr_clear_desc *d) {
   return SWR_OK;
 }
 
+/* 4x resolve of one sRGB channel through the blit routine (Blitter.cpp:
+ * 1505-1545 sample(), 1427-1484 ApplyScaleAndClamp, 676-690 write): each
+ * sample's byte is unscaled (x (1 / 255)), decoded to linear (R, G, B) and
+ * rescaled (x 255); the four are summed in order and scaled by 0.25; the
+ * sum is unscaled, encoded (R, G, B), rescaled and rounded to nearest even,
+ * saturated to a byte. */
+static uint8_t resolve_srgb8_channel(const uint8_t s[4], int rgb) {
+  const float un = 1.0f / 255.0f;
+  float acc = 0.0f;
+  for (int q = 0; q < 4; q++) {
+    float v = (float)s[q] * un;
+    if (rgb) v = srgb_to_linear(v);
+    v = v * 255.0f;
+    acc = q == 0 ? v : acc + v;
+  }
+  float c = acc * 0.25f;
+  c = c * un;
+  if (rgb) c = linear_to_srgb(c);
+  c = c * 255.0f;
+  const int32_t i = rr_round_int(c);
+  return (uint8_t)(i < 0 ? 0 : (i > 255 ? 255 : i));
+}
+
 EXPORT int swo_resolve(const swr_resolve_desc *r) {
   if (!r || r->src_samples != 4) return SWR_ERROR_UNSUPPORTED;
-  /* sRGB sources resolve through the blit routine in linear space, not built */
+  if (r->format == SWR_FORMAT_R8G8B8A8_SRGB || r->format == SWR_FORMAT_B8G8R8A8_SRGB) {
+    unsigned csr = _mm_getcsr();
+    _mm_setcsr(csr | 0x8040);
+    for (uint32_t y = 0; y < r->height; y++)
+      for (uint32_t x = 0; x < r->width; x++) {
+        uint8_t *o = (uint8_t *)r->dst + (size_t)y * r->dst_row_pitch_bytes + x * 4;
+        for (int k = 0; k < 4; k++) {
+          uint8_t s[4];
+          for (int q = 0; q < 4; q++)
+            s[q] = ((const uint8_t *)r->src)[q * r->src_sample_pitch_bytes + (size_t)y * r->src_row_pitch_bytes + x * 4 + k];
+          o[k] = resolve_srgb8_channel(s, k < 3);
+        }
+      }
+    _mm_setcsr(csr);
+    return SWR_OK;
+  }
   if (r->format != SWR_FORMAT_R8G8B8A8_UNORM && r->format != SWR_FORMAT_B8G8R8A8_UNORM &&
       r->format != SWR_FORMAT_R32G32B32A32_SFLOAT)
     return SWR_ERROR_UNSUPPORTED;

@@ -1270,7 +1270,9 @@ int swr_resolve(const swr_resolve_desc *d, void *stream) {
   if (!d || !d->src || !d->dst) return fail(SWR_ERROR_INVALID_ARGUMENT, "resolve descriptor incomplete");
   if (d->src_samples != 4) return fail(SWR_ERROR_UNSUPPORTED, "resolve supports 4 samples");
   // B8G8R8A8 takes the same byte-wise path as R8G8B8A8 (Blitter.cpp:2142).
-  if (d->format != SWR_FORMAT_R8G8B8A8_UNORM && d->format != SWR_FORMAT_B8G8R8A8_UNORM &&
+  // sRGB attachments resolve in linear space, as the reference's blit routine does.
+  const bool srgb = d->format == SWR_FORMAT_R8G8B8A8_SRGB || d->format == SWR_FORMAT_B8G8R8A8_SRGB;
+  if (d->format != SWR_FORMAT_R8G8B8A8_UNORM && d->format != SWR_FORMAT_B8G8R8A8_UNORM && !srgb &&
       d->format != SWR_FORMAT_R32G32B32A32_SFLOAT)
     return fail(SWR_ERROR_UNSUPPORTED, "resolve format %u", d->format);
   if (d->width == 0 || d->height == 0) return SWR_OK;
@@ -1280,7 +1282,11 @@ int swr_resolve(const swr_resolve_desc *d, void *stream) {
     return fail(SWR_ERROR_INVALID_ARGUMENT, "resolve pitches too small");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   dim3 block(256), grid((d->width + 255) / 256, d->height);
-  if (bpp == 4)
+  if (srgb)
+    hipLaunchKernelGGL(resolve_srgb8_kernel, grid, block, 0, s, (const char *)d->src, (int)d->src_row_pitch_bytes,
+                       (long long)d->src_sample_pitch_bytes, (char *)d->dst, (int)d->dst_row_pitch_bytes,
+                       (int)d->width, (int)d->height);
+  else if (bpp == 4)
     hipLaunchKernelGGL(resolve_rgba8_kernel, grid, block, 0, s, (const char *)d->src, (int)d->src_row_pitch_bytes,
                        (long long)d->src_sample_pitch_bytes, (char *)d->dst, (int)d->dst_row_pitch_bytes,
                        (int)d->width, (int)d->height);

@@ -1287,6 +1287,41 @@ __global__ void decode_bc_kernel(const uint8_t *__restrict__ src, uint8_t *__res
   *reinterpret_cast<uint32_t *>(px) = (v & 0x00FFFFFFu) | (a << 24);
 }
 
+// 4x resolve of an sRGB attachment: the reference's blit routine (Blitter.cpp
+// 1505-1545, 1427-1484, 676-690) decodes each sample to linear, averages in
+// linear space and encodes the average: per R, G, B channel
+// byte * (1 / 255) -> sRGBtoLinear -> * 255, summed in sample order, * 0.25,
+// * (1 / 255) -> linearToSRGB -> * 255, rounded to nearest even and
+// saturated; alpha takes the same path without the conversions.
+__global__ void resolve_srgb8_kernel(const char *src, int srcPitch, long long slice, char *dst, int dstPitch, int w,
+                                     int h) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= w || y >= h) return;
+  uint32_t s[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) s[q] = *reinterpret_cast<const uint32_t *>(src + q * slice + (long long)y * srcPitch + 4ll * x);
+  const float un = 1.0f / 255.0f;
+  uint32_t o = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    float acc = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      float v = (float)((s[q] >> (8 * k)) & 0xFFu) * un;
+      if (k < 3) v = srgb_to_linear(v);
+      v = v * 255.0f;
+      acc = q == 0 ? v : acc + v;
+    }
+    float c = acc * 0.25f;
+    c = c * un;
+    if (k < 3) c = linear_to_srgb(c);
+    c = c * 255.0f;
+    const int32_t i = rr_round_int(c);
+    o |= (uint32_t)(i < 0 ? 0 : (i > 255 ? 255 : i)) << (8 * k);
+  }
+  *reinterpret_cast<uint32_t *>(dst + (long long)y * dstPitch + 4ll * x) = o;
+}
+
 __global__ void resolve_rgba8_kernel(const char *src, int srcPitch, long long slice, char *dst, int dstPitch, int w,
                                      int h) {
   int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;

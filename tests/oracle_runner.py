@@ -148,13 +148,15 @@ def oracle_render(draw, targets=None, ids=None):
     return color, depth, stencil, int(occ[0])
 
 
-def oracle_resolve(src):
-    """4x -> 1x resolve of a (4, H, W, 4) array with the oracle."""
+def oracle_resolve(src, fmt=None):
+    """4x -> 1x resolve of a (4, H, W, 4) array with the oracle (format by
+    dtype unless given)."""
     L = oracle()
     S, H, W, _ = src.shape
     src = np.ascontiguousarray(src)
     bpp = 16 if src.dtype == np.float32 else 4
-    fmt = abi.FORMAT_R32G32B32A32_SFLOAT if src.dtype == np.float32 else abi.FORMAT_R8G8B8A8_UNORM
+    if fmt is None:
+        fmt = abi.FORMAT_R32G32B32A32_SFLOAT if src.dtype == np.float32 else abi.FORMAT_R8G8B8A8_UNORM
     dst = np.zeros((1, H, W, 4), src.dtype)
     r = abi.ResolveDesc(W, H, fmt, src.ctypes.data, W * bpp, H * W * bpp, S, dst.ctypes.data, W * bpp)
     rc = L.swo_resolve(ctypes.byref(r))

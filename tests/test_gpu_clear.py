@@ -45,11 +45,20 @@ def test_clear_matches_the_oracle(case, samples):
         assert np.array_equal(got.view(np.uint8), want.view(np.uint8)), area
 
 
-def test_srgb_resolve_is_refused():
-    """An sRGB source resolves through the reference's blit routine in linear
-    space, which is not built; the library says so instead of averaging codes."""
-    src = torch.zeros((4, 8, 8, 4), dtype=torch.uint8, device="cuda:0")
-    dst = torch.zeros((1, 8, 8, 4), dtype=torch.uint8, device="cuda:0")
-    r = abi.ResolveDesc(8, 8, abi.FORMAT_R8G8B8A8_SRGB, src.data_ptr(), 32, 256, 4, dst.data_ptr(), 32)
+@pytest.mark.parametrize("fmt", [abi.FORMAT_R8G8B8A8_SRGB, abi.FORMAT_B8G8R8A8_SRGB])
+def test_srgb_resolve_matches_the_oracle(fmt):
+    """sRGB attachments resolve in linear space (the reference's blit routine)."""
+    from oracle_runner import oracle_resolve
     import ctypes
-    assert runtime.lib().swr_resolve(ctypes.byref(r), None) == abi.ERROR_UNSUPPORTED
+    rng = np.random.default_rng(fmt)
+    src = rng.integers(0, 256, (4, 21, 35, 4), dtype=np.uint8)
+    src[:, :5] = src[:1, :5]  # equal samples: the resolve returns them
+    s = torch.from_numpy(src).to("cuda:0")
+    d = torch.zeros((1, 21, 35, 4), dtype=torch.uint8, device="cuda:0")
+    r = abi.ResolveDesc(35, 21, fmt, s.data_ptr(), 35 * 4, 21 * 35 * 4, 4, d.data_ptr(), 35 * 4)
+    assert runtime.lib().swr_resolve(ctypes.byref(r), None) == 0
+    torch.cuda.synchronize()
+    want = oracle_resolve(src, fmt)
+    got = d.cpu().numpy()
+    assert np.array_equal(got, want)
+    assert np.array_equal(got[0, :5], src[0, :5])

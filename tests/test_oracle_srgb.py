@@ -165,3 +165,25 @@ def test_srgb_texture_samples_the_decoded_texels():
         want = np.concatenate([lut[tex8[..., :3]] / np.float32(65280), (tex8[..., 3:].astype(np.float32) * 256)
                                / np.float32(65280)], -1)
         assert np.allclose(got, want, atol=1e-7), fmt
+
+
+def test_resolve_averages_in_linear_space():
+    """The 4x resolve of an sRGB attachment decodes, averages and encodes:
+    equal samples come back unchanged (every code, both formats), and mixed
+    samples give the code nearest the encoded linear mean (alpha: the plain
+    mean), within the approximations' error."""
+    from oracle_runner import oracle_resolve
+    codes = np.arange(256, dtype=np.uint8)
+    src = np.broadcast_to(np.stack([codes] * 4, -1)[None, None], (4, 1, 256, 4)).copy()
+    for fmt in (abi.FORMAT_R8G8B8A8_SRGB, abi.FORMAT_B8G8R8A8_SRGB):
+        assert np.array_equal(oracle_resolve(src, fmt)[0, 0], src[0, 0])
+    rng = np.random.default_rng(6)
+    src = rng.integers(0, 256, (4, 8, 32, 4), dtype=np.uint8)
+    got = oracle_resolve(src, abi.FORMAT_R8G8B8A8_SRGB)[0].astype(int)
+    lin = _dec(src[..., :3] / 255.0).mean(0)
+    want_rgb = _enc(lin) * 255
+    want_a = src[..., 3].astype(np.float64).mean(0)
+    assert np.abs(got[..., :3] - want_rgb).max() <= 0.51
+    assert np.abs(got[..., 3] - want_a).max() <= 0.51
+    unorm = oracle_resolve(src)[0].astype(int)  # the byte average differs
+    assert np.abs(unorm[..., :3] - got[..., :3]).max() > 10
