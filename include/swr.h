@@ -308,7 +308,7 @@ typedef struct swr_draw_desc {
 
 typedef struct swr_resolve_desc { /* 4x -> 1x colour resolve */
   uint32_t width, height;
-  uint32_t format;               /* RGBA8, BGRA8 or RGBA32F */
+  uint32_t format;               /* RGBA8, BGRA8 (UNORM: byte average; sRGB: linear-space average) or RGBA32F */
   const void *src;               /* multisampled source, sample 0 plane */
   uint32_t src_row_pitch_bytes;
   uint64_t src_sample_pitch_bytes;
