@@ -218,6 +218,9 @@ __device__ __forceinline__ void point_polygon(const DrawParams &P, const float (
   cp[3].v[0] -= X; cp[3].v[1] -= Y;
 }
 
+template <typename T>
+__device__ __forceinline__ T pick3(T a, T b, T c, int i) { return i == 0 ? a : (i == 1 ? b : c); }
+
 // Setup of primitive t of the draw P, written to record slot g (the
 // primitive's index in the batch); `draw` is the draw's index in the batch.
 __device__ __forceinline__ void setup_one(const DrawParams &P, int t, size_t g, int draw, TriRecord *__restrict__ rec,
@@ -314,22 +317,28 @@ __device__ __forceinline__ void setup_one(const DrawParams &P, int t, size_t g, 
 
   // Polygon in 28.4 (reprojected when clipped or a line,
   // SetupRoutine.cpp:122-143).
+  // An unclipped triangle keeps its three vertices in registers; only a
+  // clipped polygon, a line or a point goes through the n-vertex arrays.
+  const bool isPolygon = stages != 0 || P.line || P.point;
   int X[kMaxPolygon], Y[kMaxPolygon];
-  if (stages != 0 || P.line || P.point) {
+  int yMin, yMax, xLo, xHi;
+  if (isPolygon) {
     for (int i = 0; i < n; i++) {
       float w = cp[i].v[3];
       float rhw = (w != 0.0f) ? 1.0f / w : 1.0f;
       X[i] = rr_round_int(P.X0xF + (cp[i].v[0] * rhw) * P.WxF);
       Y[i] = rr_round_int(P.Y0xF + (cp[i].v[1] * rhw) * P.HxF);
     }
+    yMin = Y[0]; yMax = Y[0]; xLo = X[0]; xHi = X[0];
+    for (int i = 1; i < n; i++) {
+      yMin = min(yMin, Y[i]); yMax = max(yMax, Y[i]);
+      xLo = min(xLo, X[i]); xHi = max(xHi, X[i]);
+    }
   } else {
-#pragma unroll
-    for (int i = 0; i < 3; i++) { X[i] = PX[i]; Y[i] = PY[i]; }
-  }
-  int yMin = Y[0], yMax = Y[0], xLo = X[0], xHi = X[0];
-  for (int i = 1; i < n; i++) {
-    yMin = min(yMin, Y[i]); yMax = max(yMax, Y[i]);
-    xLo = min(xLo, X[i]); xHi = max(xHi, X[i]);
+    yMin = min(min(PY[0], PY[1]), PY[2]);
+    yMax = max(max(PY[0], PY[1]), PY[2]);
+    xLo = min(min(PX[0], PX[1]), PX[2]);
+    xHi = max(max(PX[0], PX[1]), PX[2]);
   }
   if (P.ms > 1) {
     yMin = wadd(yMin, 9) >> 4;   // yMinMultiSampleOffset (Constants.hpp:51)
@@ -357,16 +366,20 @@ __device__ __forceinline__ void setup_one(const DrawParams &P, int t, size_t g, 
     if (ym == y2) { int tt = o2; o2 = o1; o1 = o0; o0 = tt; }
   }
   if (!P.line && !P.point) {
-    float w0 = pos[o0][3], w1 = pos[o1][3], w2 = pos[o2][3];
+    float w0 = pick3(pos[0][3], pos[1][3], pos[2][3], o0), w1 = pick3(pos[0][3], pos[1][3], pos[2][3], o1),
+          w2 = pick3(pos[0][3], pos[1][3], pos[2][3], o2);
     float wm = rr_max(rr_max(w0, w1), w2);
     if (wm == w1) { int tt = o0; o0 = o1; o1 = o2; o2 = tt; }
     if (wm == w2) { int tt = o2; o2 = o1; o1 = o0; o0 = tt; }
   }
   // Plane equations (SetupRoutine.cpp:296-548).
-  float w1 = pos[o1][3], w2 = pos[o2][3];
-  float rhw0 = pw[o0];
-  int X0 = PX[o0], X1 = PX[o1], X2 = PX[o2];
-  int Y0 = PY[o0], Y1 = PY[o1], Y2 = PY[o2];
+  // The rotated vertices are picked by selects, not by indexing the
+  // per-vertex arrays with a run-time index (which would place them in
+  // scratch memory).
+  float w1 = pick3(pos[0][3], pos[1][3], pos[2][3], o1), w2 = pick3(pos[0][3], pos[1][3], pos[2][3], o2);
+  float rhw0 = pick3(pw[0], pw[1], pw[2], o0);
+  int X0 = pick3(PX[0], PX[1], PX[2], o0), X1 = pick3(PX[0], PX[1], PX[2], o1), X2 = pick3(PX[0], PX[1], PX[2], o2);
+  int Y0 = pick3(PY[0], PY[1], PY[2], o0), Y1 = pick3(PY[0], PY[1], PY[2], o1), Y2 = pick3(PY[0], PY[1], PY[2], o2);
   if (P.line) {  // a third point across the line (SetupRoutine.cpp:317-321)
     X2 = wsub(wadd(X1, Y1), Y0);
     Y2 = wsub(wadd(Y1, X0), X1);
@@ -395,7 +408,8 @@ __device__ __forceinline__ void setup_one(const DrawParams &P, int t, size_t g, 
   R.wB = (M01 + M11) + M21;
   R.wC = (M02 + 0.0f) + 0.0f;
   if (P.depthTest) {
-    float z0 = pz[o0], z1 = pz[o1] - z0, z2 = pz[o2] - z0;
+    float z0 = pick3(pz[0], pz[1], pz[2], o0), z1 = pick3(pz[0], pz[1], pz[2], o1) - z0,
+          z2 = pick3(pz[0], pz[1], pz[2], o2) - z0;
     float fx1 = (float)X1 * (1.0f / 16.0f), fy1 = (float)Y1 * (1.0f / 16.0f);
     float fx2 = (float)X2 * (1.0f / 16.0f), fy2 = (float)Y2 * (1.0f / 16.0f);
     if (!P.point) {
@@ -435,7 +449,8 @@ __device__ __forceinline__ void setup_one(const DrawParams &P, int t, size_t g, 
   }
 #pragma unroll
   for (int k = 0; k < 6; k++) {
-    float i0 = attr[o0][k], i1 = attr[o1][k], i2 = attr[o2][k];
+    float i0 = pick3(attr[0][k], attr[1][k], attr[2][k], o0), i1 = pick3(attr[0][k], attr[1][k], attr[2][k], o1),
+          i2 = pick3(attr[0][k], attr[1][k], attr[2][k], o2);
     R.V[k][0] = (i0 * M00 + i1 * M10) + i2 * M20;
     R.V[k][1] = (i0 * M01 + i1 * M11) + i2 * M21;
     R.V[k][2] = (i0 * M02 + i1 * 0.0f) + i2 * 0.0f;
@@ -471,8 +486,10 @@ __device__ __forceinline__ void setup_one(const DrawParams &P, int t, size_t g, 
   R.yMax = yMax;
   R.xMin = xMin;
   R.xMax = xMax;
-  if (n == 3) {
+  if (!isPolygon) {
 #pragma unroll
+    for (int i = 0; i < 3; i++) { R.X[i] = PX[i]; R.Y[i] = PY[i]; }
+  } else if (n == 3) {
     for (int i = 0; i < 3; i++) { R.X[i] = X[i]; R.Y[i] = Y[i]; }
   } else {
     int *pp = poly + g * (2 * kMaxPolygon);
