@@ -46,7 +46,7 @@
 extern "C" {
 #endif
 
-#define SWR_ABI_VERSION 5
+#define SWR_ABI_VERSION 6
 
 /* ---- status codes -------------------------------------------------------- */
 #define SWR_OK 0
@@ -135,6 +135,9 @@ extern "C" {
 #define SWR_LINE_RASTERIZATION_RECTANGULAR 1
 #define SWR_LINE_RASTERIZATION_BRESENHAM 2
 #define SWR_LINE_RASTERIZATION_RECTANGULAR_SMOOTH 3
+#define SWR_INDEX_TYPE_UINT16 0 /* VkIndexType */
+#define SWR_INDEX_TYPE_UINT32 1
+#define SWR_INDEX_TYPE_UINT8 1000265000
 #define SWR_PROVOKING_VERTEX_FIRST 0
 #define SWR_PROVOKING_VERTEX_LAST 1
 /* VkCullModeFlagBits / VkFrontFace */
@@ -218,7 +221,7 @@ typedef struct swr_draw_desc {
   const float *texcoords;   /* float2 (shader input location 1) or NULL */
   uint32_t vertex_count;
   uint32_t triangle_count;  /* primitives (triangles, or lines for the line topologies) */
-  const uint32_t *indices;  /* 3 per triangle, or NULL for 0,1,2,3,...  */
+  const void *indices;      /* of index_type, or NULL for 0,1,2,3,...   */
 
   /* Pre-rasterisation state. */
   swr_viewport viewport;
@@ -304,6 +307,13 @@ typedef struct swr_draw_desc {
    * to [1, 1023] -- centred on the vertex, clipped and rasterised as a
    * polygon with a constant depth.  Depth bias is refused for points. */
   const float *point_sizes; /* one per vertex, optional */
+
+  /* VkIndexType of `indices` (setBatchIndices is instantiated for each,
+   * Renderer.cpp:696-716): UINT16 (0), UINT32 (1) or UINT8_EXT
+   * (1000265000).  Primitive restart is the caller's: the reference splits
+   * the index buffer into separate draws before Renderer::draw
+   * (Context.cpp:48-90, 249-285). */
+  uint32_t index_type;
 } swr_draw_desc;
 
 typedef struct swr_resolve_desc { /* 4x -> 1x colour resolve */

@@ -1413,6 +1413,9 @@ static int init_ctx(Ctx *c, const swr_draw_desc *d) {
   init_constants();
   if (d->samples != 1 && d->samples != 4) return SWR_ERROR_UNSUPPORTED;
   if (d->topology > SWR_TOPOLOGY_TRIANGLE_FAN) return SWR_ERROR_UNSUPPORTED;
+  if (d->index_type != SWR_INDEX_TYPE_UINT16 && d->index_type != SWR_INDEX_TYPE_UINT32 &&
+      d->index_type != SWR_INDEX_TYPE_UINT8)
+    return SWR_ERROR_INVALID_ARGUMENT;
   c->line = d->topology == SWR_TOPOLOGY_LINE_LIST || d->topology == SWR_TOPOLOGY_LINE_STRIP;
   c->point = d->topology == SWR_TOPOLOGY_POINT_LIST;
   c->lineWidth = d->line_width == 0.0f ? 1.0f : d->line_width;
@@ -1584,7 +1587,12 @@ static void assemble(const swr_draw_desc *d, uint32_t t, uint32_t out[3]) {
     p[1] = 3u * t + (first ? 1u : 0u);
     p[2] = 3u * t + (first ? 2u : 1u);
   }
-  for (int k = 0; k < 3; k++) out[k] = d->indices ? d->indices[p[k]] : p[k];
+  for (int k = 0; k < 3; k++) {
+    if (!d->indices) out[k] = p[k];
+    else if (d->index_type == SWR_INDEX_TYPE_UINT16) out[k] = ((const uint16_t *)d->indices)[p[k]];
+    else if (d->index_type == SWR_INDEX_TYPE_UINT8) out[k] = ((const uint8_t *)d->indices)[p[k]];
+    else out[k] = ((const uint32_t *)d->indices)[p[k]];
+  }
 }
 
 /* DrawCall::setupLine (Renderer.cpp:920-1163): the rectangle of width

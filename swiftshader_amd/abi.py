@@ -7,7 +7,7 @@ DrawData in the reference (src/Device/Renderer.hpp:58-113, 211-213).
 """
 import ctypes as C
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 OK = 0
 ERROR_INVALID_ARGUMENT = -1
@@ -57,6 +57,7 @@ ADVANCED_BLEND_OPS = tuple(range(1000148012, 1000148023)) + tuple(range(10001480
 LINE_RASTERIZATION_DEFAULT, LINE_RASTERIZATION_RECTANGULAR = 0, 1
 LINE_RASTERIZATION_BRESENHAM, LINE_RASTERIZATION_RECTANGULAR_SMOOTH = 2, 3
 PROVOKING_VERTEX_FIRST, PROVOKING_VERTEX_LAST = 0, 1
+INDEX_TYPE_UINT16, INDEX_TYPE_UINT32, INDEX_TYPE_UINT8 = 0, 1, 1000265000
 
 CULL_NONE, CULL_FRONT, CULL_BACK, CULL_FRONT_AND_BACK = 0, 1, 2, 3
 FRONT_FACE_COUNTER_CLOCKWISE, FRONT_FACE_CLOCKWISE = 0, 1
@@ -129,7 +130,7 @@ class DrawDesc(C.Structure):
         ("depth_bias_enable", C.c_uint32), ("depth_bias_constant_factor", C.c_float),
         ("depth_bias_clamp", C.c_float), ("depth_bias_slope_factor", C.c_float),
         ("line_width", C.c_float), ("line_rasterization_mode", C.c_uint32),
-        ("point_sizes", C.c_void_p),
+        ("point_sizes", C.c_void_p), ("index_type", C.c_uint32),
     ]
 
 

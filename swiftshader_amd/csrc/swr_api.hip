@@ -290,6 +290,12 @@ int build_params(const swr_draw_desc *d, const swr_tuning &tuning, DrawParams &P
   P.col = d->colors;
   P.uv = d->texcoords;
   P.idx = d->indices;
+  switch (d->index_type) {
+    case SWR_INDEX_TYPE_UINT16: P.idxBytes = 2; break;
+    case SWR_INDEX_TYPE_UINT32: P.idxBytes = 4; break;
+    case SWR_INDEX_TYPE_UINT8: P.idxBytes = 1; break;
+    default: return fail(SWR_ERROR_INVALID_ARGUMENT, "index_type %u", d->index_type);
+  }
   P.triCount = (int)d->triangle_count;
   if (d->vertex_count == 0 && d->triangle_count) return fail(SWR_ERROR_INVALID_ARGUMENT, "vertex_count is 0");
   P.vertexCount = (int)std::min<uint32_t>(d->vertex_count, INT32_MAX);

@@ -110,6 +110,13 @@ __device__ __forceinline__ uint32_t bgra_swap(int bgra, uint32_t v) {
   return bgra ? (v & 0xFF00FF00u) | ((v >> 16) & 0xFFu) | ((v & 0xFFu) << 16) : v;
 }
 
+// Index p of the draw's index buffer, of its VkIndexType.
+__device__ __forceinline__ uint32_t load_index(const DrawParams &P, uint32_t p) {
+  if (P.idxBytes == 2) return reinterpret_cast<const uint16_t *>(P.idx)[p];
+  if (P.idxBytes == 1) return reinterpret_cast<const uint8_t *>(P.idx)[p];
+  return reinterpret_cast<const uint32_t *>(P.idx)[p];
+}
+
 // Vertex indices of primitive t in the reference's batch order
 // (setBatchIndices, Renderer.cpp:50-145); out-of-range indices read the
 // last vertex instead of faulting.
@@ -139,9 +146,9 @@ __device__ __forceinline__ void tri_vertices(const DrawParams &P, uint32_t t, ui
     p1 = 3u * t + (f ? 1u : 0u);
     p2 = 3u * t + (f ? 2u : 1u);
   }
-  vi[0] = P.idx ? P.idx[p0] : p0;
-  vi[1] = P.idx ? P.idx[p1] : p1;
-  vi[2] = P.idx ? P.idx[p2] : p2;
+  vi[0] = P.idx ? load_index(P, p0) : p0;
+  vi[1] = P.idx ? load_index(P, p1) : p1;
+  vi[2] = P.idx ? load_index(P, p2) : p2;
 #pragma unroll
   for (int k = 0; k < 3; k++) vi[k] = vi[k] < (uint32_t)P.vertexCount ? vi[k] : (uint32_t)P.vertexCount - 1u;
 }

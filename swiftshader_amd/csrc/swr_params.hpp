@@ -51,7 +51,8 @@ struct DrawParams {
   const float *pos;
   const float *col;
   const float *uv;
-  const uint32_t *idx;
+  const void *idx;
+  int idxBytes;  // 1, 2 or 4 (VkIndexType UINT8_EXT, UINT16, UINT32)
   int triCount;
   int vertexCount;
   int topology;     // 0 triangle list, 1 triangle strip, 2 fan, 3 line list, 4 line strip, 5 points

@@ -8,6 +8,8 @@ sw::Renderer::draw (src/Device/Renderer.cpp:183-490).
 from dataclasses import dataclass
 from typing import List, Optional, Tuple
 
+import numpy as np
+
 from . import abi
 from .scenes import DrawData
 
@@ -44,6 +46,15 @@ def _stencil_state(d: dict) -> abi.StencilOpState:
     return s
 
 
+def index_type_of(indices) -> int:
+    """VkIndexType of an index array by its dtype (uint8, uint16, else uint32)."""
+    if indices is not None and indices.dtype == np.uint8:
+        return abi.INDEX_TYPE_UINT8
+    if indices is not None and indices.dtype == np.uint16:
+        return abi.INDEX_TYPE_UINT16
+    return abi.INDEX_TYPE_UINT32
+
+
 def color_bytes_per_pixel(fmt: int) -> int:
     return 16 if fmt == abi.FORMAT_R32G32B32A32_SFLOAT else 4
 
@@ -68,6 +79,7 @@ def build_desc(draw: DrawData, buf: Buffers) -> abi.DrawDesc:
     d.texcoords = buf.texcoords or None
     d.indices = buf.indices or None
     d.point_sizes = buf.point_sizes or None
+    d.index_type = index_type_of(draw.indices)
     d.vertex_count = len(draw.positions)
     d.triangle_count = draw.triangle_count
     if draw.viewport is None:

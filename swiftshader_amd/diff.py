@@ -127,6 +127,7 @@ class View:
             tex = TextureData(format=abi.FORMAT_R32G32B32A32_SFLOAT, levels=[])
         dd = build_desc(replace(d, texture=tex) if tex is not d.texture else d, buf)
         dd.vertex_count = pos.shape[0]
+        dd.index_type = abi.INDEX_TYPE_UINT32  # the view uploads its indices as uint32
         dd.primitive_id = abi.Attachment(self.ids.data_ptr(), abi.FORMAT_R32_UINT, W * 4, H * W * 4)
         return dd
 

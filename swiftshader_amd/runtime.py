@@ -120,7 +120,13 @@ def prepare(draw: DrawData, device, targets: Optional[Targets] = None,
         keep.append(uv)
         buf.texcoords = uv.data_ptr()
     if draw.indices is not None:
-        idx = _dev(draw.indices.astype(np.uint32).view(np.int32), device)
+        ind = np.ascontiguousarray(draw.indices)
+        if ind.dtype == np.uint16:
+            idx = _dev(ind.view(np.int16), device)
+        elif ind.dtype == np.uint8:
+            idx = _dev(ind, device)
+        else:
+            idx = _dev(ind.astype(np.uint32).view(np.int32), device)
         keep.append(idx)
         buf.indices = idx.data_ptr()
     if draw.point_sizes is not None:

@@ -388,6 +388,14 @@ def _srgb_textured(seed=53, bgra=False, aniso=False, samples=1, target=abi.FORMA
                     shader=abi.SHADER_MODULATE, texture=tex, color_format=target)
 
 
+def _with_index_type(draw, dtype):
+    """The same indexed draw with its indices narrowed to uint16 or uint8
+    (VkIndexType UINT16 / UINT8_EXT)."""
+    assert draw.indices.max() <= np.iinfo(dtype).max
+    draw.indices = draw.indices.astype(dtype)
+    return draw
+
+
 def parity_cases():
     """(name, DrawData) pairs; every one must match the oracle bit for bit."""
     return [
@@ -441,6 +449,8 @@ def parity_cases():
         ("srgb8_opaque_d16", _formats(seed=50, srgb=True)),
         ("bc1_texture_bgra", _bc_textured()),
         ("bc3_texture_bgra_aniso_msaa4", _bc_textured(seed=52, n=3, aniso=True, samples=4)),
+        ("indexed_grid_uint16", _with_index_type(_indexed(seed=55), np.uint16)),
+        ("strip_indexed_uint8", _with_index_type(_strip_band(seed=56), np.uint8)),
         ("srgb_texture_trilinear", _srgb_textured()),
         ("bgra_srgb_texture_aniso_msaa4_srgb_target", _srgb_textured(seed=54, bgra=True, aniso=True, samples=4,
                                                                      target=abi.FORMAT_R8G8B8A8_SRGB)),

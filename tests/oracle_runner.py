@@ -119,7 +119,8 @@ def oracle_render(draw, targets=None, ids=None):
     if draw.texcoords is not None:
         buf.texcoords = arr(draw.texcoords, np.float32)
     if draw.indices is not None:
-        buf.indices = arr(draw.indices, np.uint32)
+        ind = draw.indices
+        buf.indices = arr(ind, ind.dtype if ind.dtype in (np.uint8, np.uint16) else np.uint32)
     if draw.point_sizes is not None:
         buf.point_sizes = arr(draw.point_sizes, np.float32)
     if draw.texture is not None:

@@ -12,7 +12,7 @@ import struct
 import numpy as np
 import pytest
 
-from oracle_runner import oracle, reference_lib
+from oracle_runner import oracle, oracle_render, reference_lib
 
 
 def f32(x):
@@ -139,3 +139,18 @@ def test_constants_match_golden_vectors():
     assert cw == list(z["c_weight"])
     assert np.array_equal(np.array(uw, np.float32).view(np.uint32), z["uv_weight"].view(np.uint32))
     assert np.array_equal(np.array(us, np.float32).view(np.uint32), z["uv_start"].view(np.uint32))
+
+
+def test_index_types_read_the_same_indices():
+    """UINT32, UINT16 and UINT8_EXT index buffers holding the same values
+    (Renderer.cpp:696-716) render the same image bit for bit."""
+    from cases import _indexed, _strip_band
+    for make in (lambda: _indexed(seed=57), lambda: _strip_band(seed=58)):
+        out = []
+        for dt in (np.uint32, np.uint16, np.uint8):
+            d = make()
+            d.indices = d.indices.astype(dt)
+            out.append(oracle_render(d))
+        for o in out[1:]:
+            assert np.array_equal(o[0], out[0][0]) and np.array_equal(o[1].view(np.uint32), out[0][1].view(np.uint32))
+            assert o[3] == out[0][3]
