@@ -46,7 +46,7 @@
 extern "C" {
 #endif
 
-#define SWR_ABI_VERSION 6
+#define SWR_ABI_VERSION 7
 
 /* ---- status codes -------------------------------------------------------- */
 #define SWR_OK 0
@@ -314,6 +314,16 @@ typedef struct swr_draw_desc {
    * the index buffer into separate draws before Renderer::draw
    * (Context.cpp:48-90, 249-285). */
   uint32_t index_type;
+
+  /* Depth clamp and depth clip (Context.cpp:647-648, 686-693;
+   * PixelProcessor.cpp:120-136; VertexRoutine.cpp:137-143; Renderer.cpp:940,
+   * 1176).  depth_clamp_enable clamps every sample's depth to the viewport's
+   * [min, max] depth range instead of [0, 1].  depth_clip_control is the
+   * VK_EXT_depth_clip_enable state: 0 uses the default (clip unless
+   * depth_clamp_enable), 1 clips, 2 does not clip against the near and far
+   * planes. */
+  uint32_t depth_clamp_enable;
+  uint32_t depth_clip_control;
 } swr_draw_desc;
 
 typedef struct swr_resolve_desc { /* 4x -> 1x colour resolve */

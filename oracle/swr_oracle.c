@@ -172,7 +172,7 @@ EXPORT void swo_constants(int *xf, int *yf, int *ms_offsets, uint16_t *cweight, 
 /* ===================================================================== */
 
 enum { CLIP_RIGHT = 1, CLIP_TOP = 2, CLIP_FAR = 4, CLIP_LEFT = 8, CLIP_BOTTOM = 16, CLIP_NEAR = 32,
-       CLIP_FRUSTUM = 63, CLIP_FINITE = 128 };
+       CLIP_FRUSTUM = 63, CLIP_SIDES = 27, CLIP_FINITE = 128 };
 enum { FILTER_POINT, FILTER_MIN_POINT_MAG_LINEAR, FILTER_MIN_LINEAR_MAG_POINT, FILTER_LINEAR, FILTER_ANISOTROPIC };
 enum { MIPMAP_NONE, MIPMAP_POINT, MIPMAP_LINEAR };
 enum { ADDR_WRAP, ADDR_CLAMP };
@@ -226,6 +226,8 @@ typedef struct {
   int colorBGRA;      /* B8G8R8A8 colour attachment */
   int colorSRGB;      /* sRGB-encoded colour attachment */
   int depth16;        /* D16_UNORM depth attachment */
+  int depthClip;      /* clip against the near and far planes */
+  float depthClampLo, depthClampHi;
   int line;           /* LINE_LIST / LINE_STRIP: primitives are lines */
   int bresenham;      /* lines set up as Bresenham parallelograms */
   int point;          /* POINT_LIST: primitives are points */
@@ -264,9 +266,10 @@ static void vertex_routine(const Ctx *c, OVertex *v) {
   if (w < y) flags |= CLIP_TOP;
   if (!(-w <= x)) flags |= CLIP_LEFT;
   if (!(-w <= y)) flags |= CLIP_BOTTOM;
-  /* depthClipEnable defaults to !depthClampEnable = true */
-  if (w < z) flags |= CLIP_FAR;
-  if (!(0.0f <= z)) flags |= CLIP_NEAR;
+  if (c->depthClip) { /* near and far only with depth clipping (VertexRoutine.cpp:137-143) */
+    if (w < z) flags |= CLIP_FAR;
+    if (!(0.0f <= z)) flags |= CLIP_NEAR;
+  }
   const float maxPos = u2f(0x7F7FFFFFu);
   if (fabsf(x) <= maxPos && fabsf(y) <= maxPos && fabsf(z) <= maxPos) flags |= CLIP_FINITE;
   v->clipFlags = flags;
@@ -1237,7 +1240,8 @@ static void quad(Ctx *c, const OPrim *prim, int x, int y, const int cMask[4], co
     if (!c->depthTestActive) { depthPass = 1; continue; }
     int m = 0;
     for (int i = 0; i < 4; i++) {
-      z[q][i] = rr_min(rr_max(z[q][i], 0.0f), 1.0f); /* clampDepth: [0, 1] without unrestricted range */
+      /* clampDepth: the viewport's depth range with depth clamp, else [0, 1] (PixelProcessor.cpp:120-136) */
+      z[q][i] = rr_min(rr_max(z[q][i], c->depthClampLo), c->depthClampHi);
       /* D16 compares and stores Min(Max(Round(z * 0xFFFF), 0), 0xFFFF) (PixelRoutine.cpp:506-510) */
       if (c->depth16) z[q][i] = rr_min(rr_max(rintf(z[q][i] * 65535.0f), 0.0f), 65535.0f);
       float Z = z[q][i];
@@ -1413,6 +1417,16 @@ static int init_ctx(Ctx *c, const swr_draw_desc *d) {
   init_constants();
   if (d->samples != 1 && d->samples != 4) return SWR_ERROR_UNSUPPORTED;
   if (d->topology > SWR_TOPOLOGY_TRIANGLE_FAN) return SWR_ERROR_UNSUPPORTED;
+  if (d->depth_clip_control > 2) return SWR_ERROR_INVALID_ARGUMENT;
+  c->depthClip = d->depth_clip_control == 0 ? !d->depth_clamp_enable : d->depth_clip_control == 1;
+  if (d->depth_clamp_enable) {
+    const float n = d->viewport.min_depth, f = d->viewport.max_depth;
+    c->depthClampLo = n < f ? n : f; /* std::min / std::max */
+    c->depthClampHi = n < f ? f : n;
+  } else {
+    c->depthClampLo = 0.0f;
+    c->depthClampHi = 1.0f;
+  }
   if (d->index_type != SWR_INDEX_TYPE_UINT16 && d->index_type != SWR_INDEX_TYPE_UINT32 &&
       d->index_type != SWR_INDEX_TYPE_UINT8)
     return SWR_ERROR_INVALID_ARGUMENT;
@@ -1647,7 +1661,7 @@ static int setup_line_polygon(const Ctx *c, const OVertex *v0, const OVertex *v1
   poly->n = 4;
   poly->i = 0;
   poly->b = 0;
-  return clipper_clip(poly, CLIP_FRUSTUM);
+  return clipper_clip(poly, c->depthClip ? CLIP_FRUSTUM : CLIP_SIDES);
 }
 
 /* DrawCall::setupPoint (Renderer.cpp:1166-1210): the square of side
@@ -1672,7 +1686,7 @@ static int setup_point_polygon(const Ctx *c, const OVertex *v, float L[4][4], Po
   poly->n = 4;
   poly->i = 0;
   poly->b = 0;
-  return clipper_clip(poly, CLIP_FRUSTUM);
+  return clipper_clip(poly, c->depthClip ? CLIP_FRUSTUM : CLIP_SIDES);
 }
 
 /* Renders one draw with host pointers in the descriptor. */

@@ -7,7 +7,7 @@ DrawData in the reference (src/Device/Renderer.hpp:58-113, 211-213).
 """
 import ctypes as C
 
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 OK = 0
 ERROR_INVALID_ARGUMENT = -1
@@ -131,6 +131,7 @@ class DrawDesc(C.Structure):
         ("depth_bias_clamp", C.c_float), ("depth_bias_slope_factor", C.c_float),
         ("line_width", C.c_float), ("line_rasterization_mode", C.c_uint32),
         ("point_sizes", C.c_void_p), ("index_type", C.c_uint32),
+        ("depth_clamp_enable", C.c_uint32), ("depth_clip_control", C.c_uint32),
     ]
 
 

@@ -330,6 +330,17 @@ int build_params(const swr_draw_desc *d, const swr_tuning &tuning, DrawParams &P
   if (d->depth.data && d->depth.format != SWR_FORMAT_D32_SFLOAT && d->depth.format != SWR_FORMAT_D16_UNORM)
     return fail(SWR_ERROR_UNSUPPORTED, "depth format %u (D32_SFLOAT or D16_UNORM)", d->depth.format);
   P.depth16 = d->depth.data && d->depth.format == SWR_FORMAT_D16_UNORM;
+  // Depth clip follows depth clamp unless set (Context.cpp:647-648, 686-693);
+  // depth clamp takes the viewport's range (PixelProcessor.cpp:120-136).
+  if (d->depth_clip_control > 2) return fail(SWR_ERROR_INVALID_ARGUMENT, "depth_clip_control %u", d->depth_clip_control);
+  P.depthClip = d->depth_clip_control == 0 ? !d->depth_clamp_enable : d->depth_clip_control == 1;
+  if (d->depth_clamp_enable) {
+    P.depthClampLo = std::min(d->viewport.min_depth, d->viewport.max_depth);
+    P.depthClampHi = std::max(d->viewport.min_depth, d->viewport.max_depth);
+  } else {
+    P.depthClampLo = 0.0f;
+    P.depthClampHi = 1.0f;
+  }
   if (d->stencil.data && d->stencil.format != SWR_FORMAT_S8_UINT)
     return fail(SWR_ERROR_UNSUPPORTED, "stencil format %u (only S8_UINT)", d->stencil.format);
   if (!valid_compare(d->depth_compare_op)) return fail(SWR_ERROR_INVALID_ARGUMENT, "invalid depth compare op");

@@ -29,7 +29,7 @@
 namespace swr {
 
 enum : int { CLIP_RIGHT = 1, CLIP_TOP = 2, CLIP_FAR = 4, CLIP_LEFT = 8, CLIP_BOTTOM = 16, CLIP_NEAR = 32,
-             CLIP_FRUSTUM = 63, CLIP_FINITE = 128 };
+             CLIP_FRUSTUM = 63, CLIP_SIDES = 27, CLIP_FINITE = 128 };
 
 // Standard 4x sample locations relative to the pixel centre
 // (Constants.hpp:26-47) and their 28.4 offsets Xf/Yf (Constants.cpp:291-297).
@@ -262,8 +262,10 @@ __device__ __forceinline__ void setup_one(const DrawParams &P, int t, size_t g, 
     if (w < y) f |= CLIP_TOP;
     if (!(-w <= x)) f |= CLIP_LEFT;
     if (!(-w <= y)) f |= CLIP_BOTTOM;
-    if (w < z) f |= CLIP_FAR;
-    if (!(0.0f <= z)) f |= CLIP_NEAR;
+    if (P.depthClip) {  // near and far only with depth clipping (VertexRoutine.cpp:137-143)
+      if (w < z) f |= CLIP_FAR;
+      if (!(0.0f <= z)) f |= CLIP_NEAR;
+    }
     const float maxPos = __uint_as_float(0x7F7FFFFFu);
     if (__builtin_fabsf(x) <= maxPos && __builtin_fabsf(y) <= maxPos && __builtin_fabsf(z) <= maxPos) f |= CLIP_FINITE;
     flags[k] = f;
@@ -290,14 +292,14 @@ __device__ __forceinline__ void setup_one(const DrawParams &P, int t, size_t g, 
     // no culling, front-facing, d = 1 (SetupRoutine.cpp:115-119).
     point_polygon(P, pos[0], P.psize ? P.psize[vi[0]] : 1.0f, cp);
     n = 4;
-    stages = clip_polygon(cp, n, CLIP_FRUSTUM);
+    stages = clip_polygon(cp, n, P.depthClip ? CLIP_FRUSTUM : CLIP_SIDES);
     if (n < 3) return;
   } else if (P.line) {
     // setupLine: the line's polygon, clipped against every frustum plane;
     // no culling, front-facing, d = 1 (SetupRoutine.cpp:115-119).
     if (!line_polygon(P, pos[0], pos[1], cp)) return;
     n = 4;
-    stages = clip_polygon(cp, n, CLIP_FRUSTUM);
+    stages = clip_polygon(cp, n, P.depthClip ? CLIP_FRUSTUM : CLIP_SIDES);
     if (n < 3) return;
   } else {
     // setupSolidTriangles (Renderer.cpp:733-776).
@@ -989,7 +991,7 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
           const float Dz = c + yy * b;
           float z = xx * a + Dz;
           if (P.depthBias) z = z + zbias;  // PixelRoutine.cpp:141-144, before the clamp
-          z = rr_min(rr_max(z, 0.0f), 1.0f);
+          z = rr_min(rr_max(z, P.depthClampLo), P.depthClampHi);  // clampDepth
           if ((XF && P.depth16)) z = d16_value(z);
           zq[q] = z;
           const float zb = S.zb[q];
@@ -1089,7 +1091,7 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
                           2.0f * fabsf(zB) * fmaxf(fabsf(ya), fabsf(yb)) + 2.0f * fabsf(zBias);
         float zl = fmaxf(zlo0, m - mag * 0x1p-19f);
         if (!(zl == zl)) zl = -__builtin_inff();
-        zl = fminf(zl, 1.0f);
+        zl = fminf(zl, P.depthClampHi);
         if ((XF && P.depth16)) zl = d16_value(fmaxf(zl, 0.0f));  // the bound in stored units (monotone map)
         test = !(P.depthCompare == 1 ? zl >= zmax : zl > zmax);
         zlo = zl;

@@ -71,6 +71,8 @@ struct DrawParams {
   // Depth / stencil.
   int depthTest, depthWrite, depthCompare;
   int depth16;  // D16_UNORM depth attachment (else D32_SFLOAT)
+  int depthClip;                    // clip against the near and far planes
+  float depthClampLo, depthClampHi;  // every sample's depth range
   // Depth bias: which terms the setup applies, and whether samples get it.
   int biasConstant, biasSlope, biasClamp, depthBias;
   float constantDepthBias, slopeDepthBias, depthBiasClamp;

@@ -80,6 +80,8 @@ def build_desc(draw: DrawData, buf: Buffers) -> abi.DrawDesc:
     d.indices = buf.indices or None
     d.point_sizes = buf.point_sizes or None
     d.index_type = index_type_of(draw.indices)
+    d.depth_clamp_enable = draw.depth_clamp_enable
+    d.depth_clip_control = draw.depth_clip_control
     d.vertex_count = len(draw.positions)
     d.triangle_count = draw.triangle_count
     if draw.viewport is None:

@@ -87,6 +87,8 @@ class DrawData:
     line_width: float = 1.0
     line_rasterization_mode: int = abi.LINE_RASTERIZATION_DEFAULT
     point_sizes: Optional[np.ndarray] = None  # (V,) float32 PointSize per vertex; None = 1
+    depth_clamp_enable: int = 0
+    depth_clip_control: int = 0  # 0 default (clip unless clamping), 1 clip, 2 no clip
 
     @property
     def triangle_count(self):

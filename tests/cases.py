@@ -396,6 +396,16 @@ def _with_index_type(draw, dtype):
     return draw
 
 
+def _depth_clamped(draw, clamp, clip, near=0.15, far=0.8, compare=None):
+    """The draw with depth clamp / depth clip state and a viewport depth
+    range inside [0, 1], so that clamping to it is visible."""
+    draw.depth_clamp_enable, draw.depth_clip_control = clamp, clip
+    if compare is not None:
+        draw.depth_compare = compare
+    draw.viewport = (0.0, 0.0, float(draw.width), float(draw.height), near, far)
+    return draw
+
+
 def parity_cases():
     """(name, DrawData) pairs; every one must match the oracle bit for bit."""
     return [
@@ -451,6 +461,10 @@ def parity_cases():
         ("bc3_texture_bgra_aniso_msaa4", _bc_textured(seed=52, n=3, aniso=True, samples=4)),
         ("indexed_grid_uint16", _with_index_type(_indexed(seed=55), np.uint16)),
         ("strip_indexed_uint8", _with_index_type(_strip_band(seed=56), np.uint8)),
+        ("depth_clamp_clip_heavy", _depth_clamped(_clip_heavy(seed=59), clamp=1, clip=0,
+                                                  compare=abi.COMPARE_ALWAYS)),
+        ("depth_clip_off_lines_msaa4", _depth_clamped(_lines(seed=61, clip=True, samples=4), clamp=0, clip=2)),
+        ("depth_clamp_clip_on_points", _depth_clamped(_points(seed=62, clip=True), clamp=1, clip=1)),
         ("srgb_texture_trilinear", _srgb_textured()),
         ("bgra_srgb_texture_aniso_msaa4_srgb_target", _srgb_textured(seed=54, bgra=True, aniso=True, samples=4,
                                                                      target=abi.FORMAT_R8G8B8A8_SRGB)),
