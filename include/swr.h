@@ -153,7 +153,9 @@ extern "C" {
 #define SWR_MIPMAP_MODE_NEAREST 0
 #define SWR_MIPMAP_MODE_LINEAR 1
 #define SWR_ADDRESS_REPEAT 0
+#define SWR_ADDRESS_MIRRORED_REPEAT 1
 #define SWR_ADDRESS_CLAMP_TO_EDGE 2
+#define SWR_ADDRESS_MIRROR_CLAMP_TO_EDGE 4
 
 /* ---- fixed fragment-shader family ---------------------------------------- */
 /* Inputs: location 0 = vec4 colour, location 1 = vec2 texcoord, both

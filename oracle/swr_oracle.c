@@ -175,7 +175,7 @@ enum { CLIP_RIGHT = 1, CLIP_TOP = 2, CLIP_FAR = 4, CLIP_LEFT = 8, CLIP_BOTTOM = 
        CLIP_FRUSTUM = 63, CLIP_SIDES = 27, CLIP_FINITE = 128 };
 enum { FILTER_POINT, FILTER_MIN_POINT_MAG_LINEAR, FILTER_MIN_LINEAR_MAG_POINT, FILTER_LINEAR, FILTER_ANISOTROPIC };
 enum { MIPMAP_NONE, MIPMAP_POINT, MIPMAP_LINEAR };
-enum { ADDR_WRAP, ADDR_CLAMP };
+enum { ADDR_WRAP, ADDR_CLAMP, ADDR_MIRROR, ADDR_MIRRORONCE };
 enum { BLEND_OP_SRC_EXT = 1000148000, BLEND_OP_DST_EXT = 1000148001, BLEND_OP_ZERO_EXT = 1000148002 };
 
 #define NATTR 6 /* colour rgba + uv */
@@ -623,10 +623,24 @@ static void texel16(const OTexture *t, int level, uint16_t u, uint16_t v, uint16
     for (int k = 0; k < 3; k++) out[k] = g_srgb_lut[p[k]];
 }
 
+/* SamplerCore::address (:2398-2435): clamp, mirror (bit 16 of the 16.16
+ * coordinate flips the low bits), mirror once (|u| saturated through a
+ * signed 16-bit pack), else wrap. */
 static uint16_t address16(float uw, int mode) {
   if (mode == ADDR_CLAMP) {
     float cl = rr_min(rr_max(uw, 0.0f), 65535.0f / 65536.0f);
     return (uint16_t)rr_trunc_int(cl * 65536.0f);
+  }
+  if (mode == ADDR_MIRROR) {
+    int32_t conv = rr_trunc_int(uw * 65536.0f);
+    int32_t mirror = (int32_t)((uint32_t)conv << 15) >> 31;
+    return (uint16_t)(conv ^ mirror);
+  }
+  if (mode == ADDR_MIRRORONCE) {
+    float a = u2f(f2u(uw * 65536.0f) & 0x7FFFFFFFu);
+    int32_t conv = (int32_t)((uint32_t)rr_trunc_int(a) - 0x8000u);
+    int32_t packed = conv < -32768 ? -32768 : (conv > 32767 ? 32767 : conv);
+    return (uint16_t)(packed + 0x8000);
   }
   return (uint16_t)rr_trunc_int(uw * 65536.0f);
 }
@@ -719,8 +733,16 @@ static void address_float(const OTexture *t, float uvw, int dim, int mode, float
   else if (t->filter == FILTER_MIN_POINT_MAG_LINEAR) filterMask = (lod <= 0.0f) ? -1 : 0;
   else filterMask = -1;
   float coord;
-  if (mode == ADDR_CLAMP) coord = rr_min(rr_max(uvw, 0.0f), u2f(0x3F7FFFFFu));
-  else coord = rr_frac(uvw);
+  if (mode == ADDR_CLAMP) {
+    coord = rr_min(rr_max(uvw, 0.0f), u2f(0x3F7FFFFFu));
+  } else if (mode == ADDR_MIRROR) { /* 2 |u/2 - round(u/2)|, under 1 */
+    float h = uvw * 0.5f;
+    coord = rr_min(2.0f * u2f(f2u(h - rintf(h)) & 0x7FFFFFFFu), u2f(0x3F7FFFFFu));
+  } else if (mode == ADDR_MIRRORONCE) {
+    coord = rr_min(u2f(f2u(uvw) & 0x7FFFFFFFu), u2f(0x3F7FFFFFu));
+  } else {
+    coord = rr_frac(uvw);
+  }
   coord = coord * (float)dim;
   if (t->filter == FILTER_POINT) {
     *x0 = rr_trunc_int(coord);
@@ -736,7 +758,7 @@ static void address_float(const OTexture *t, float uvw, int dim, int mode, float
   }
   *x1 = *x0 - filterMask;
   if (t->filter != FILTER_POINT) {
-    if (mode == ADDR_CLAMP) {
+    if (mode != ADDR_WRAP) {
       *x0 = imax(*x0, 0);
       *x1 = imin(*x1, dim - 1);
     } else {
@@ -1405,6 +1427,18 @@ static int blend_operation(int op, int sf, int df, int isFloat) {
   return op;
 }
 
+/* VkSamplerAddressMode -> addressing mode (SpirvShaderSampling.cpp:419-430);
+ * CLAMP_TO_BORDER is not restated. */
+static int address_code(uint32_t m) {
+  switch (m) {
+    case SWR_ADDRESS_REPEAT: return ADDR_WRAP;
+    case SWR_ADDRESS_MIRRORED_REPEAT: return ADDR_MIRROR;
+    case SWR_ADDRESS_CLAMP_TO_EDGE: return ADDR_CLAMP;
+    case SWR_ADDRESS_MIRROR_CLAMP_TO_EDGE: return ADDR_MIRRORONCE;
+    default: return -1;
+  }
+}
+
 static int blend_factor(int op, int f) {
   return (op == SWR_BLEND_OP_MIN || op == SWR_BLEND_OP_MAX || is_advanced_op(op)) ? SWR_BLEND_FACTOR_ONE : f;
 }
@@ -1542,8 +1576,9 @@ static int init_ctx(Ctx *c, const swr_draw_desc *d) {
     else if (s->mag_filter == SWR_FILTER_NEAREST) t->filter = s->min_filter == SWR_FILTER_LINEAR ? FILTER_MIN_LINEAR_MAG_POINT : FILTER_POINT;
     else t->filter = s->min_filter == SWR_FILTER_NEAREST ? FILTER_MIN_POINT_MAG_LINEAR : FILTER_LINEAR;
     t->mipmap = s->mipmap_mode == SWR_MIPMAP_MODE_LINEAR ? MIPMAP_LINEAR : MIPMAP_POINT;
-    t->addrU = s->address_mode_u == SWR_ADDRESS_CLAMP_TO_EDGE ? ADDR_CLAMP : ADDR_WRAP;
-    t->addrV = s->address_mode_v == SWR_ADDRESS_CLAMP_TO_EDGE ? ADDR_CLAMP : ADDR_WRAP;
+    t->addrU = address_code(s->address_mode_u);
+    t->addrV = address_code(s->address_mode_v);
+    if (t->addrU < 0 || t->addrV < 0) return SWR_ERROR_UNSUPPORTED;
     t->mipLodBias = s->mip_lod_bias;
     t->maxAnisotropy = s->max_anisotropy;
     t->minLod = clampf(s->min_lod, 0.0f, 13.0f);

@@ -64,7 +64,7 @@ FRONT_FACE_COUNTER_CLOCKWISE, FRONT_FACE_CLOCKWISE = 0, 1
 
 FILTER_NEAREST, FILTER_LINEAR = 0, 1
 MIPMAP_MODE_NEAREST, MIPMAP_MODE_LINEAR = 0, 1
-ADDRESS_REPEAT, ADDRESS_CLAMP_TO_EDGE = 0, 2
+ADDRESS_REPEAT, ADDRESS_MIRRORED_REPEAT, ADDRESS_CLAMP_TO_EDGE, ADDRESS_MIRROR_CLAMP_TO_EDGE = 0, 1, 2, 4
 
 SHADER_INTERPOLATE, SHADER_SAMPLE, SHADER_MODULATE = 0, 1, 2
 

@@ -457,8 +457,11 @@ int build_params(const swr_draw_desc *d, const swr_tuning &tuning, DrawParams &P
     T.wwhh[2] = T.wwhh[3] = (float)tx.levels[0].height;
     if (s.mag_filter > 1 || s.min_filter > 1 || s.mipmap_mode > 1)
       return fail(SWR_ERROR_INVALID_ARGUMENT, "invalid filter state");
-    if ((s.address_mode_u != SWR_ADDRESS_REPEAT && s.address_mode_u != SWR_ADDRESS_CLAMP_TO_EDGE) ||
-        (s.address_mode_v != SWR_ADDRESS_REPEAT && s.address_mode_v != SWR_ADDRESS_CLAMP_TO_EDGE))
+    auto addr_ok = [](uint32_t m) {
+      return m == SWR_ADDRESS_REPEAT || m == SWR_ADDRESS_CLAMP_TO_EDGE || m == SWR_ADDRESS_MIRRORED_REPEAT ||
+             m == SWR_ADDRESS_MIRROR_CLAMP_TO_EDGE;
+    };
+    if (!addr_ok(s.address_mode_u) || !addr_ok(s.address_mode_v))
       return fail(SWR_ERROR_UNSUPPORTED, "address mode (REPEAT and CLAMP_TO_EDGE are supported)");
     // convertFilterMode (SpirvShaderSampling.cpp:275-324).
     if (s.anisotropy_enable) T.filter = FILT_ANISOTROPIC;
@@ -467,8 +470,13 @@ int build_params(const swr_draw_desc *d, const swr_tuning &tuning, DrawParams &P
     if (s.anisotropy_enable && !(s.max_anisotropy >= 1.0f && s.max_anisotropy <= 16.0f))
       return fail(SWR_ERROR_INVALID_ARGUMENT, "max_anisotropy must be in [1, 16]");
     T.mipmap = (s.mipmap_mode == SWR_MIPMAP_MODE_LINEAR) ? MIP_LINEAR : MIP_POINT;
-    T.addrU = s.address_mode_u == SWR_ADDRESS_CLAMP_TO_EDGE;
-    T.addrV = s.address_mode_v == SWR_ADDRESS_CLAMP_TO_EDGE;
+    // Address modes: 0 wrap, 1 clamp to edge, 2 mirrored repeat, 3 mirror clamp to edge.
+    auto addr_code = [](uint32_t m) {
+      return m == SWR_ADDRESS_CLAMP_TO_EDGE ? 1 : m == SWR_ADDRESS_MIRRORED_REPEAT ? 2
+                                                : m == SWR_ADDRESS_MIRROR_CLAMP_TO_EDGE ? 3 : 0;
+    };
+    T.addrU = addr_code(s.address_mode_u);
+    T.addrV = addr_code(s.address_mode_v);
     T.mipLodBias = s.mip_lod_bias;
     T.maxAnisotropy = s.max_anisotropy;
     // vk::SamplerState::ClampLod (VkSampler.hpp:33-36).
@@ -536,11 +544,13 @@ int build_params(const swr_draw_desc *d, const swr_tuning &tuning, DrawParams &P
 // ---------------------------------------------------------------------------
 // Raster variant of a draw: samples, float colour target, float texture,
 // and whether it uses a format outside the common RGBA8 / RGBA32F / D32F set
-// (D16 depth, BGRA8 or sRGB colour, BGRA8 texture) or an advanced blend
-// mode, which the common variants compile out.
+// (D16 depth, BGRA8 or sRGB colour, BGRA8 or sRGB texture), a mirrored
+// address mode or an advanced blend mode, which the common variants compile
+// out.
 int variant_of(const DrawParams &P) {
   const bool ftex = P.useUV && P.tex.isFloat;
-  const bool xf = P.depth16 || P.colorBGRA || P.colorSRGB || (P.useUV && (P.tex.bgra || P.tex.srgb)) || P.advancedBlend;
+  const bool xf = P.depth16 || P.colorBGRA || P.colorSRGB || P.advancedBlend ||
+                  (P.useUV && (P.tex.bgra || P.tex.srgb || P.tex.addrU > 1 || P.tex.addrV > 1));
   return (xf ? 8 : 0) | (P.ms > 1 ? 4 : 0) | (P.colorFloat ? 2 : 0) | (ftex ? 1 : 0);
 }
 
@@ -845,6 +855,8 @@ int grad_params(const swr_draw_desc *desc, const DrawParams &P, const swr_grad_d
   if (!g) return fail(SWR_ERROR_INVALID_ARGUMENT, "grad descriptor is NULL");
   if (!P.primId) return fail(SWR_ERROR_INVALID_ARGUMENT, "backward needs the draw's primitive_id attachment");
   if (P.line || P.point) return fail(SWR_ERROR_UNSUPPORTED, "backward of lines and points");
+  if (P.useUV && (P.tex.addrU > 1 || P.tex.addrV > 1))
+    return fail(SWR_ERROR_UNSUPPORTED, "backward of mirrored address modes");
   if (P.blendEnable) return fail(SWR_ERROR_UNSUPPORTED, "backward of a blended draw");
   if (!g->d_color) return fail(SWR_ERROR_INVALID_ARGUMENT, "d_color is NULL");
   if (g->d_color_row_pitch_bytes < desc->framebuffer_width * 16 ||

@@ -68,10 +68,25 @@ __device__ __forceinline__ float uv_weight(int n) { return n <= 1 ? 1.0f : 1.0f 
 __device__ __forceinline__ float uv_start(int n) { return n <= 1 ? -0.0f : -(float)(n - 1) / (float)(2 * n); }
 
 // 0.16 fixed-point coordinate (address, :2399-2435).
-__device__ __forceinline__ uint32_t address16(float uw, int clampMode) {
-  if (clampMode) {
+// SamplerCore::address (SamplerCore.cpp:2398-2435) for the address modes
+// 0 wrap, 1 clamp to edge, 2 mirrored repeat, 3 mirror clamp to edge.
+// The mirrored modes exist only in the extended (XF) kernel variants.
+template <bool XF>
+__device__ __forceinline__ uint32_t address16(float uw, int mode) {
+  if (mode == 1) {
     float cl = rr_min(rr_max(uw, 0.0f), 65535.0f / 65536.0f);
     return (uint32_t)rr_trunc_int(cl * 65536.0f) & 0xFFFFu;
+  }
+  if (XF && mode == 2) {  // bit 16 of the fixed-point coordinate flips the low 16 bits
+    const int32_t conv = rr_trunc_int(uw * 65536.0f);
+    const int32_t mirror = (int32_t)((uint32_t)conv << 15) >> 31;
+    return (uint32_t)(conv ^ mirror) & 0xFFFFu;
+  }
+  if (XF && mode == 3) {  // |u|, saturated to [0, 0xFFFF] through a signed 16-bit pack
+    const float a = __uint_as_float(__float_as_uint(uw * 65536.0f) & 0x7FFFFFFFu);
+    int32_t c = (int32_t)((uint32_t)rr_trunc_int(a) - 0x8000u);
+    c = c < -32768 ? -32768 : (c > 32767 ? 32767 : c);
+    return (uint32_t)(c + 0x8000) & 0xFFFFu;
   }
   return (uint32_t)rr_trunc_int(uw * 65536.0f) & 0xFFFFu;
 }
@@ -133,8 +148,8 @@ __device__ __forceinline__ int select_mip(const TexParams &T, float lod, int sec
 template <bool XF>
 __device__ __forceinline__ void sample_quad16(const TexParams &T, const TexLevel &L, float u, float v, float lod,
                                               uint32_t out[4]) {
-  uint32_t uu = address16(u, T.addrU);
-  uint32_t vv = address16(v, T.addrV);
+  uint32_t uu = address16<XF>(u, T.addrU);
+  uint32_t vv = address16<XF>(v, T.addrV);  // offsets wrap only in wrap mode (:720-723)
   if (T.filter == FILT_POINT) {
     fetch16<XF>(L, uu, vv, out);
     return;
@@ -145,8 +160,8 @@ __device__ __forceinline__ void sample_quad16(const TexParams &T, const TexLevel
   } else if (T.filter == FILT_MIN_POINT_MAG_LINEAR) {
     if (!(lod <= 0.0f)) { uh = 0; vh = 0; }     // offset &= CmpLE(lod, 0)
   }
-  uint32_t u0 = offset16(uu, uh, !T.addrU, -1), v0 = offset16(vv, vh, !T.addrV, -1);
-  uint32_t u1 = offset16(uu, uh, !T.addrU, +1), v1 = offset16(vv, vh, !T.addrV, +1);
+  uint32_t u0 = offset16(uu, uh, T.addrU == 0, -1), v0 = offset16(vv, vh, T.addrV == 0, -1);
+  uint32_t u1 = offset16(uu, uh, T.addrU == 0, +1), v1 = offset16(vv, vh, T.addrV == 0, +1);
   uint32_t c00[4], c10[4], c01[4], c11[4];
   fetch16<XF>(L, u0, v0, c00);
   fetch16<XF>(L, u1, v0, c10);
@@ -198,14 +213,27 @@ __device__ __forceinline__ void sample_aniso16(const TexParams &T, float u, floa
 }
 
 // Float path: address (:2463-2638), sampleFloat2D (:1155-1248).
-__device__ __forceinline__ void address_float(const TexParams &T, float uvw, int dim, int clampMode, float lod, int &x0,
+// computeIndices (SamplerCore.cpp:2480-2630) for one axis; mode as address16.
+template <bool XF>
+__device__ __forceinline__ void address_float(const TexParams &T, float uvw, int dim, int mode, float lod, int &x0,
                                               int &x1, float &f) {
   int filterMask;
   if (T.filter == FILT_POINT) filterMask = 0;
   else if (T.filter == FILT_MIN_LINEAR_MAG_POINT) filterMask = !(lod <= 0.0f) ? -1 : 0;
   else if (T.filter == FILT_MIN_POINT_MAG_LINEAR) filterMask = (lod <= 0.0f) ? -1 : 0;
   else filterMask = -1;
-  float coord = clampMode ? rr_min(rr_max(uvw, 0.0f), __uint_as_float(0x3F7FFFFFu)) : rr_frac(uvw);
+  const float one = __uint_as_float(0x3F7FFFFFu);  // just under 1
+  float coord;
+  if (mode == 1) {
+    coord = rr_min(rr_max(uvw, 0.0f), one);
+  } else if (XF && mode == 2) {
+    const float h = uvw * 0.5f;
+    coord = rr_min(2.0f * __uint_as_float(__float_as_uint(h - __builtin_rintf(h)) & 0x7FFFFFFFu), one);
+  } else if (XF && mode == 3) {
+    coord = rr_min(__uint_as_float(__float_as_uint(uvw) & 0x7FFFFFFFu), one);
+  } else {
+    coord = rr_frac(uvw);
+  }
   coord = coord * (float)dim;
   if (T.filter == FILT_POINT) {
     x0 = rr_trunc_int(coord);
@@ -221,7 +249,7 @@ __device__ __forceinline__ void address_float(const TexParams &T, float uvw, int
   }
   x1 = x0 - filterMask;
   if (T.filter != FILT_POINT) {
-    if (clampMode) {
+    if (mode != 0) {
       x0 = x0 > 0 ? x0 : 0;
       x1 = x1 < dim - 1 ? x1 : dim - 1;
     } else {
@@ -235,11 +263,12 @@ __device__ __forceinline__ float4 fetchf(const TexLevel &L, int x, int yoff) {
   return reinterpret_cast<const float4 *>(L.data)[(uint32_t)(x + yoff)];
 }
 
+template <bool XF>
 __device__ __forceinline__ float4 sample_float2d(const TexParams &T, const TexLevel &L, float u, float v, float lod) {
   int x0, x1, y0, y1;
   float fu, fv;
-  address_float(T, u, L.width, T.addrU, lod, x0, x1, fu);
-  address_float(T, v, L.height, T.addrV, lod, y0, y1, fv);
+  address_float<XF>(T, u, L.width, T.addrU, lod, x0, x1, fu);
+  address_float<XF>(T, v, L.height, T.addrV, lod, y0, y1, fv);
   int pitch = L.pitch;
   y0 = (int)((uint32_t)y0 * (uint32_t)pitch);
   if (T.filter == FILT_POINT) return fetchf(L, x0, y0);
@@ -255,10 +284,11 @@ __device__ __forceinline__ float4 sample_float2d(const TexParams &T, const TexLe
   return r;
 }
 
+template <bool XF>
 __device__ __forceinline__ float4 sample_float_aniso(const TexParams &T, float u, float v, float lod, float aniso,
                                                      float uDelta, float vDelta, int second) {
   const TexLevel L = tex_level(T, select_mip(T, lod, second));
-  if (T.filter != FILT_ANISOTROPIC) return sample_float2d(T, L, u, v, lod);
+  if (T.filter != FILT_ANISOTROPIC) return sample_float2d<XF>(T, L, u, v, lod);
   int N = rr_round_int(aniso);
   const int row = aniso_row(N);
   float A = uv_weight(row), B = uv_start(row);
@@ -270,7 +300,7 @@ __device__ __forceinline__ float4 sample_float_aniso(const TexParams &T, float u
   dv = dv * A;
   int i = 0;
   do {
-    float4 c = sample_float2d(T, L, u0, v0, lod);
+    float4 c = sample_float2d<XF>(T, L, u0, v0, lod);
     u0 = u0 + du;
     v0 = v0 + dv;
     sum.x = sum.x + c.x * A;
@@ -326,9 +356,9 @@ __device__ __forceinline__ QuadLod quad_lod(const TexParams &T, const float u[4]
 template <bool FTEX, bool XF>
 __device__ __forceinline__ float4 sample_lane(const TexParams &T, float u, float v, const QuadLod &L) {
   if (FTEX) {
-    float4 c = sample_float_aniso(T, u, v, L.lod, L.aniso, L.uDelta, L.vDelta, 0);
+    float4 c = sample_float_aniso<XF>(T, u, v, L.lod, L.aniso, L.uDelta, L.vDelta, 0);
     if (T.mipmap == MIP_LINEAR) {
-      float4 cc = sample_float_aniso(T, u, v, L.lod, L.aniso, L.uDelta, L.vDelta, 1);
+      float4 cc = sample_float_aniso<XF>(T, u, v, L.lod, L.aniso, L.uDelta, L.vDelta, 1);
       float l4 = rr_frac(L.lod);
       c.x = (cc.x - c.x) * l4 + c.x;
       c.y = (cc.y - c.y) * l4 + c.y;

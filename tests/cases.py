@@ -465,6 +465,12 @@ def parity_cases():
                                                   compare=abi.COMPARE_ALWAYS)),
         ("depth_clip_off_lines_msaa4", _depth_clamped(_lines(seed=61, clip=True, samples=4), clamp=0, clip=2)),
         ("depth_clamp_clip_on_points", _depth_clamped(_points(seed=62, clip=True), clamp=1, clip=1)),
+        ("rgba8_mirrored_npot_aniso", _rgba8_texture_case(address=abi.ADDRESS_MIRRORED_REPEAT, npot=(100, 37),
+                                                          aniso=True, seed=63)),
+        ("rgba8_mirror_clamp_msaa4", _rgba8_texture_case(address=abi.ADDRESS_MIRROR_CLAMP_TO_EDGE, samples=4, seed=64,
+                                                         size=(70, 50), tris=900)),
+        ("float_mirrored_aniso", _float_target(address=abi.ADDRESS_MIRRORED_REPEAT, aniso=True, seed=65)),
+        ("float_mirror_clamp", _float_target(address=abi.ADDRESS_MIRROR_CLAMP_TO_EDGE, seed=66)),
         ("srgb_texture_trilinear", _srgb_textured()),
         ("bgra_srgb_texture_aniso_msaa4_srgb_target", _srgb_textured(seed=54, bgra=True, aniso=True, samples=4,
                                                                      target=abi.FORMAT_R8G8B8A8_SRGB)),
