@@ -462,7 +462,7 @@ int build_params(const swr_draw_desc *d, const swr_tuning &tuning, DrawParams &P
              m == SWR_ADDRESS_MIRROR_CLAMP_TO_EDGE;
     };
     if (!addr_ok(s.address_mode_u) || !addr_ok(s.address_mode_v))
-      return fail(SWR_ERROR_UNSUPPORTED, "address mode (REPEAT and CLAMP_TO_EDGE are supported)");
+      return fail(SWR_ERROR_UNSUPPORTED, "address mode (CLAMP_TO_BORDER is not supported)");
     // convertFilterMode (SpirvShaderSampling.cpp:275-324).
     if (s.anisotropy_enable) T.filter = FILT_ANISOTROPIC;
     else if (s.mag_filter == SWR_FILTER_NEAREST) T.filter = (s.min_filter == SWR_FILTER_LINEAR) ? FILT_MIN_LINEAR_MAG_POINT : FILT_POINT;
