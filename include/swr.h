@@ -46,7 +46,7 @@
 extern "C" {
 #endif
 
-#define SWR_ABI_VERSION 7
+#define SWR_ABI_VERSION 8
 
 /* ---- status codes -------------------------------------------------------- */
 #define SWR_OK 0
@@ -326,6 +326,16 @@ typedef struct swr_draw_desc {
    * planes. */
   uint32_t depth_clamp_enable;
   uint32_t depth_clip_control;
+
+  /* Depth bounds test (PixelRoutine.cpp:576-641; Context.cpp:946-962).  With
+   * a depth attachment, a sample whose stored depth (D16: code / 0xFFFF) lies
+   * outside [min_depth_bounds, max_depth_bounds] loses its depth-pass bit when
+   * the depth test is on (no depth or colour write; the stencil takes its
+   * depth-fail op) and its coverage when it is off (no colour or stencil
+   * write; the occlusion count still includes it). */
+  uint32_t depth_bounds_test_enable;
+  float min_depth_bounds;
+  float max_depth_bounds;
 } swr_draw_desc;
 
 typedef struct swr_resolve_desc { /* 4x -> 1x colour resolve */

@@ -227,6 +227,8 @@ typedef struct {
   int colorSRGB;      /* sRGB-encoded colour attachment */
   int depth16;        /* D16_UNORM depth attachment */
   int depthClip;      /* clip against the near and far planes */
+  int depthBoundsActive;
+  float minDepthBounds, maxDepthBounds;
   float depthClampLo, depthClampHi;
   int line;           /* LINE_LIST / LINE_STRIP: primitives are lines */
   int bresenham;      /* lines set up as Bresenham parallelograms */
@@ -935,6 +937,22 @@ static float read_depth(Ctx *c, int q, int x, int y) {
   memcpy(&f, p, 4);
   return f;
 }
+/* depthBoundsTest (PixelRoutine.cpp:576-641): the stored depth of each
+ * covered pixel of the quad (D16: code * (1 / 0xFFFF)) within
+ * [minDepthBounds, maxDepthBounds]; a miss clears zMask with the depth test
+ * on, cMask with it off. */
+static void depth_bounds_test(Ctx *c, int q, int x, int y, int *zMask, int *cMask) {
+  if (!c->depthBoundsActive) return;
+  int t = 0;
+  for (int i = 0; i < 4; i++) {
+    if (!(*cMask & (1 << i))) continue;
+    float zv = read_depth(c, q, x + (i & 1), y + (i >> 1));
+    if (c->depth16) zv = zv * (1.0f / 65535.0f);
+    if (c->minDepthBounds <= zv && zv <= c->maxDepthBounds) t |= 1 << i;
+  }
+  if (!c->depthTestActive) *cMask &= *zMask & t;
+  else *zMask &= *cMask & t;
+}
 /* writeDepth32F / writeDepth16 (PixelRoutine.cpp:658-711); z is the value
  * the test compared, already Round(z * 0xFFFF) for D16 and in [0, 65535]
  * (UShort4(Float4, saturate), LLVMReactor.cpp:2150-2171). */
@@ -1259,7 +1277,11 @@ static void quad(Ctx *c, const OPrim *prim, int x, int y, const int cMask[4], co
   int depthPass = 0;
   for (int s = 0; s < ns; s++) {
     int q = samples[s];
-    if (!c->depthTestActive) { depthPass = 1; continue; }
+    if (!c->depthTestActive) {
+      depthPass = 1;
+      depth_bounds_test(c, q, x, y, &zMask[q], &cMask[q]);
+      continue;
+    }
     int m = 0;
     for (int i = 0; i < 4; i++) {
       /* clampDepth: the viewport's depth range with depth clamp, else [0, 1] (PixelProcessor.cpp:120-136) */
@@ -1287,6 +1309,7 @@ static void quad(Ctx *c, const OPrim *prim, int x, int y, const int cMask[4], co
     if (c->depthCompare == SWR_COMPARE_NEVER) zMask[q] = 0;
     if (c->stencilActive) zMask[q] &= sMask[q];
     if (zMask[q] != 0) depthPass = 1;
+    depth_bounds_test(c, q, x, y, &zMask[q], &cMask[q]);
   }
 
   if (depthPass) {
@@ -1501,6 +1524,9 @@ static int init_ctx(Ctx *c, const swr_draw_desc *d) {
 
   c->depthTestActive = d->depth.data && d->depth_test_enable;
   c->depthWriteEnable = c->depthTestActive && d->depth_write_enable;
+  c->depthBoundsActive = d->depth.data && d->depth_bounds_test_enable; /* Context.cpp:946-949 */
+  c->minDepthBounds = d->min_depth_bounds;
+  c->maxDepthBounds = d->max_depth_bounds;
   c->depthCompare = (int)d->depth_compare_op;
   {
     /* Context.cpp:640-661: factors only when enabled, a NaN clamp is 0. */

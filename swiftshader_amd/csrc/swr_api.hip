@@ -346,6 +346,10 @@ int build_params(const swr_draw_desc *d, const swr_tuning &tuning, DrawParams &P
   if (!valid_compare(d->depth_compare_op)) return fail(SWR_ERROR_INVALID_ARGUMENT, "invalid depth compare op");
   P.depthTest = (d->depth.data && d->depth_test_enable) ? 1 : 0;
   P.depthWrite = (P.depthTest && d->depth_write_enable) ? 1 : 0;
+  // depthBoundsTestActive (Context.cpp:946-949).
+  P.depthBounds = (d->depth.data && d->depth_bounds_test_enable) ? 1 : 0;
+  P.depthBoundsMin = d->min_depth_bounds;
+  P.depthBoundsMax = d->max_depth_bounds;
   P.depthCompare = (int)d->depth_compare_op;
   // Depth bias state (Context.cpp:640-661: a NaN clamp is no clamp;
   // SetupProcessor.cpp:75-77 and PixelProcessor.cpp:119: each term only
@@ -545,11 +549,11 @@ int build_params(const swr_draw_desc *d, const swr_tuning &tuning, DrawParams &P
 // Raster variant of a draw: samples, float colour target, float texture,
 // and whether it uses a format outside the common RGBA8 / RGBA32F / D32F set
 // (D16 depth, BGRA8 or sRGB colour, BGRA8 or sRGB texture), a mirrored
-// address mode or an advanced blend mode, which the common variants compile
+// address mode, the depth bounds test or an advanced blend mode, which the common variants compile
 // out.
 int variant_of(const DrawParams &P) {
   const bool ftex = P.useUV && P.tex.isFloat;
-  const bool xf = P.depth16 || P.colorBGRA || P.colorSRGB || P.advancedBlend ||
+  const bool xf = P.depth16 || P.colorBGRA || P.colorSRGB || P.advancedBlend || P.depthBounds ||
                   (P.useUV && (P.tex.bgra || P.tex.srgb || P.tex.addrU > 1 || P.tex.addrV > 1));
   return (xf ? 8 : 0) | (P.ms > 1 ? 4 : 0) | (P.colorFloat ? 2 : 0) | (ftex ? 1 : 0);
 }
@@ -857,6 +861,7 @@ int grad_params(const swr_draw_desc *desc, const DrawParams &P, const swr_grad_d
   if (P.line || P.point) return fail(SWR_ERROR_UNSUPPORTED, "backward of lines and points");
   if (P.useUV && (P.tex.addrU > 1 || P.tex.addrV > 1))
     return fail(SWR_ERROR_UNSUPPORTED, "backward of mirrored address modes");
+  if (P.depthBounds) return fail(SWR_ERROR_UNSUPPORTED, "backward of a draw with the depth bounds test");
   if (P.blendEnable) return fail(SWR_ERROR_UNSUPPORTED, "backward of a blended draw");
   if (!g->d_color) return fail(SWR_ERROR_INVALID_ARGUMENT, "d_color is NULL");
   if (g->d_color_row_pitch_bytes < desc->framebuffer_width * 16 ||

@@ -836,7 +836,7 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
     if (FCOL && P.primId)
       S.id[FCOL ? q : 0] =
           *reinterpret_cast<const uint32_t *>(P.primId + q * P.primIdSlice + (long long)py * P.primIdPitch + 4ll * px);
-    if (P.depthTest)  // D16 holds the float of its integer (readDepth16, PixelRoutine.cpp:466-481)
+    if (P.depthTest || (XF && P.depthBounds))  // D16 holds the float of its integer (readDepth16, PixelRoutine.cpp:466-481)
       S.zb[q] = (XF && P.depth16) ? (float)*reinterpret_cast<const uint16_t *>(P.depth + q * P.depthSlice +
                                                                         (long long)py * P.depthPitch + 2ll * px)
                           : *reinterpret_cast<const float *>(P.depth + q * P.depthSlice +
@@ -1010,6 +1010,19 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
         }
         zMask &= cMask;
         if (P.stencilActive) zMask &= sMask;
+      }
+      if (XF && P.depthBounds && cMask) {
+        // depthBoundsTest (PixelRoutine.cpp:576-641) on the stored depth, D16
+        // as its code / 0xFFFF. With the depth test off the coverage loses
+        // the sample while zMask keeps it, so the occlusion count does too.
+        int bMask = 0;
+#pragma unroll
+        for (int q = 0; q < MS; q++) {
+          const float zs = P.depth16 ? S.zb[q] * (1.0f / 65535.0f) : S.zb[q];
+          if (P.depthBoundsMin <= zs && zs <= P.depthBoundsMax) bMask |= 1 << q;
+        }
+        if (P.depthTest) zMask &= bMask;
+        else cMask &= bMask;
       }
       if (P.stencilWrite && cMask) {
 #pragma unroll

@@ -72,6 +72,8 @@ struct DrawParams {
   int depthTest, depthWrite, depthCompare;
   int depth16;  // D16_UNORM depth attachment (else D32_SFLOAT)
   int depthClip;                    // clip against the near and far planes
+  int depthBounds;                  // depth bounds test on the stored depth
+  float depthBoundsMin, depthBoundsMax;
   float depthClampLo, depthClampHi;  // every sample's depth range
   // Depth bias: which terms the setup applies, and whether samples get it.
   int biasConstant, biasSlope, biasClamp, depthBias;

@@ -82,6 +82,9 @@ def build_desc(draw: DrawData, buf: Buffers) -> abi.DrawDesc:
     d.index_type = index_type_of(draw.indices)
     d.depth_clamp_enable = draw.depth_clamp_enable
     d.depth_clip_control = draw.depth_clip_control
+    if draw.depth_bounds is not None:
+        d.depth_bounds_test_enable = 1
+        d.min_depth_bounds, d.max_depth_bounds = draw.depth_bounds
     d.vertex_count = len(draw.positions)
     d.triangle_count = draw.triangle_count
     if draw.viewport is None:

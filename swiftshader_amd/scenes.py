@@ -23,7 +23,7 @@ Every generator accepts a `scale` factor that shrinks triangle counts (and
 target sizes where stated) for fast parity tests.
 """
 from dataclasses import dataclass, field
-from typing import List, Optional
+from typing import List, Optional, Tuple
 
 import numpy as np
 
@@ -89,6 +89,7 @@ class DrawData:
     point_sizes: Optional[np.ndarray] = None  # (V,) float32 PointSize per vertex; None = 1
     depth_clamp_enable: int = 0
     depth_clip_control: int = 0  # 0 default (clip unless clamping), 1 clip, 2 no clip
+    depth_bounds: Optional[Tuple[float, float]] = None  # (min, max) enables the depth bounds test
 
     @property
     def triangle_count(self):

@@ -406,6 +406,15 @@ def _depth_clamped(draw, clamp, clip, near=0.15, far=0.8, compare=None):
     return draw
 
 
+def _depth_bounded(draw, lo, hi, clear_depth, depth_test=1, depth_format=None):
+    """The draw with the depth bounds test on [lo, hi], cleared to a depth
+    inside them so that the first triangles can pass."""
+    draw.depth_bounds, draw.clear_depth, draw.depth_test = (lo, hi), clear_depth, depth_test
+    if depth_format is not None:
+        draw.depth_format = depth_format
+    return draw
+
+
 def parity_cases():
     """(name, DrawData) pairs; every one must match the oracle bit for bit."""
     return [
@@ -465,6 +474,12 @@ def parity_cases():
                                                   compare=abi.COMPARE_ALWAYS)),
         ("depth_clip_off_lines_msaa4", _depth_clamped(_lines(seed=61, clip=True, samples=4), clamp=0, clip=2)),
         ("depth_clamp_clip_on_points", _depth_clamped(_points(seed=62, clip=True), clamp=1, clip=1)),
+        ("depth_bounds_gouraud", _depth_bounded(scenes.plain_gouraud(), 0.2, 0.6, clear_depth=0.55)),
+        ("depth_bounds_d16_textured",
+         _depth_bounded(scenes.textured_1080p(triangles=3000, width=192, height=108, tex_size=128), 0.1, 0.5,
+                        clear_depth=0.45, depth_format=abi.FORMAT_D16_UNORM)),
+        ("depth_bounds_depth_test_off", _depth_bounded(scenes.plain_gouraud(), 0.6, 0.9, clear_depth=0.5,
+                                                       depth_test=0)),
         ("rgba8_mirrored_npot_aniso", _rgba8_texture_case(address=abi.ADDRESS_MIRRORED_REPEAT, npot=(100, 37),
                                                           aniso=True, seed=63)),
         ("rgba8_mirror_clamp_msaa4", _rgba8_texture_case(address=abi.ADDRESS_MIRROR_CLAMP_TO_EDGE, samples=4, seed=64,
