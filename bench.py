@@ -120,19 +120,9 @@ def leg_batched(dev, reps):
     CLAMP_TO_EDGE or REPEAT): one autotuned batch plan, clears + plan run
     captured in one HIP graph, against one swr_draw per draw (no graph)."""
     import torch
-    from swiftshader_amd import diff, runtime, scenes
+    from swiftshader_amd import runtime, workloads
     from swiftshader_amd.batch import BatchPlan
-    draws = scenes.batched(seed=3, host_textures=False)
-    gen = torch.Generator(device=dev)
-    gen.manual_seed(3)
-    dds = []
-    for d in draws:
-        ts = d.texture.size
-        lvl0 = torch.randint(0, 256, (ts, ts, 4), dtype=torch.uint8, device=dev, generator=gen)
-        levels = [lvl0] + diff.mip_levels_like(lvl0)
-        diff.generate_mips(levels)
-        dds.append(runtime.prepare(d, dev, texture_tensors=levels))
-    torch.cuda.synchronize()
+    draws, dds, _ = workloads.batched_device(dev, seed=3)
     tris = sum(d.triangle_count for d in draws)
 
     def clear_all():

@@ -412,8 +412,13 @@ typedef struct swr_tuning {
  * so a run can be captured into a HIP graph and replayed.  The memory the
  * descriptors point to is read at run time: vertex and texel contents and
  * attachments may change between runs, triangle counts and sizes may not.
- * A run whose geometry needs more pairs than planned renders incompletely
- * and is reported by swr_batch_plan_check. */
+ * No primitive is ever dropped: a run whose geometry needs more (tile,
+ * primitive) pairs than planned bins the primitives whose pairs fit and
+ * renders the rest, in API order after them, by an unbinned pass in which
+ * each tile walks those primitives directly (as the reference re-batches
+ * any count, Renderer.cpp:250-268).  That pass is slower, so such a run is
+ * reported by swr_batch_plan_check; rebuilding the plan (destroy + create,
+ * then capturing any graph again) restores the binned speed. */
 typedef struct swr_batch_plan swr_batch_plan;
 
 /* Returns SWR_ABI_VERSION. */
@@ -483,8 +488,9 @@ int swr_batch_plan_create(const swr_draw_desc *descs, uint32_t count, uint32_t p
 /* Enqueues the planned draws on `stream` (no allocation, no host wait). */
 int swr_batch_plan_run(swr_batch_plan *plan, void *stream);
 
-/* Waits for the device and reports the largest pair count any run needed
- * beyond the plan's capacity (0: every run fit). */
+/* Waits for the device and reports, in *pairs_needed, the largest pair
+ * count any run needed when it exceeded the plan's capacity (0: every run
+ * fit).  Such runs still rendered every primitive (see swr_batch_plan). */
 int swr_batch_plan_check(swr_batch_plan *plan, uint32_t *pairs_needed);
 
 /* Times the plan under candidate tunings (tile size, raster workgroups per

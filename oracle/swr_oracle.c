@@ -43,7 +43,9 @@
  * every primitive to completion before the next, which yields the same
  * per-pixel order.
  */
+#define _GNU_SOURCE /* pthread barriers under -std=c11 */
 #include <math.h>
+#include <pthread.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -251,6 +253,8 @@ typedef struct {
   uint16_t *left[4], *right[4];
   uint64_t occlusion;
   uint32_t curTri; /* primitive index in draw order (for the primitive_id record) */
+  int clusterId, clusterCount; /* rasterise only row pairs (y / 2) % clusterCount == clusterId (if > 1) */
+  float *stateOut; /* optional (S, H, W, 4) record of the colour-writing fragment's SampleState */
 } Ctx;
 
 #define OUTLINE_OFFSET 4
@@ -820,8 +824,13 @@ static void sample_float_aniso(const OTexture *t, float u, float v, float lod, f
   memcpy(out, sum, sizeof(sum));
 }
 
-/* sampleTexture128 for an implicit-LOD 2D sample of one quad (4 lanes). */
-static void sample_texture(const OTexture *t, const float u[4], const float v[4], float out[4][4]) {
+/* A quad's sampling state: LOD, anisotropy (probe count before RoundInt)
+ * and the major-axis lane difference the probes step along. */
+typedef struct { float lod, aniso, uDelta, vDelta; } SampleState;
+
+/* sampleTexture128 for an implicit-LOD 2D sample of one quad (4 lanes);
+ * the quad's sampling state goes to *st. */
+static void sample_texture(const OTexture *t, const float u[4], const float v[4], float out[4][4], SampleState *st) {
   float lod, aniso = 0.0f, uDelta = 0.0f, vDelta = 0.0f;
   int singleMip = (t->minLod == t->maxLod);
   int skipLod = singleMip && t->filter != FILTER_ANISOTROPIC;
@@ -851,6 +860,10 @@ static void sample_texture(const OTexture *t, const float u[4], const float v[4]
     lod = rr_max(lod, t->minLod);
     lod = rr_min(lod, t->maxLod);
   }
+  st->lod = lod;
+  st->aniso = aniso;
+  st->uDelta = uDelta;
+  st->vDelta = vDelta;
   for (int lane = 0; lane < 4; lane++) {
     if (t->isFloat) {
       float c[4];
@@ -1020,6 +1033,11 @@ static uint8_t *color_at(Ctx *c, int q, int x, int y) {
 static uint32_t *id_at(Ctx *c, int q, int x, int y) {
   const swr_attachment *a = &c->d->primitive_id;
   return (uint32_t *)((uint8_t *)a->data + q * a->sample_pitch_bytes + (size_t)y * a->row_pitch_bytes + (size_t)x * 4);
+}
+
+static float *state_at(Ctx *c, int q, int x, int y) {
+  const size_t W = c->d->framebuffer_width, H = c->d->framebuffer_height;
+  return c->stateOut + 4 * (((size_t)q * H + (size_t)y) * W + (size_t)x);
 }
 
 static float blend_factor_rgb(const Ctx *c, int f, int comp, const float *src, const float *dst) {
@@ -1208,7 +1226,7 @@ static void write_color(Ctx *c, uint8_t *buf, const float col[4]) {
   }
 }
 
-static void quad(Ctx *c, const OPrim *prim, int x, int y, const int cMask[4], const float Dz[4][4], const float Dw[4],
+static void quad(Ctx *c, const OPrim *prim, int x, int y, int cMask[4], const float Dz[4][4], const float Dw[4],
                  const float Dv[NATTR][4]) {
   static const float xm[4] = {0, 1, 0, 1};
   int zMask[4] = {0, 0, 0, 0}, sMask[4] = {0, 0, 0, 0};
@@ -1259,12 +1277,13 @@ static void quad(Ctx *c, const OPrim *prim, int x, int y, const int cMask[4], co
     for (int i = 0; i < 4; i++) in[k][i] = (xFragment[i] * prim->V[k].A + Dv[k][i]) * rhw[i];
   }
   float col[4][4];
+  SampleState st = {0.0f, 0.0f, 0.0f, 0.0f};
   int shader = c->d->shader;
   if (shader == SWR_SHADER_INTERPOLATE) {
     for (int i = 0; i < 4; i++)
       for (int k = 0; k < 4; k++) col[i][k] = in[k][i];
   } else {
-    sample_texture(&c->tex, in[4], in[5], col);
+    sample_texture(&c->tex, in[4], in[5], col, &st);
     if (shader == SWR_SHADER_MODULATE)
       for (int i = 0; i < 4; i++)
         for (int k = 0; k < 4; k++) col[i][k] = col[i][k] * in[k][i];
@@ -1336,6 +1355,7 @@ static void quad(Ctx *c, const OPrim *prim, int x, int y, const int cMask[4], co
           alpha_blend(c, buf, col[i], bl);
           write_color(c, buf, bl);
           if (c->d->primitive_id.data) *id_at(c, q, x + (i & 1), y + (i >> 1)) = c->curTri + 1u;
+          if (c->stateOut) memcpy(state_at(c, q, x + (i & 1), y + (i >> 1)), &st, sizeof(st));
         }
       }
     }
@@ -1370,6 +1390,7 @@ static void quad(Ctx *c, const OPrim *prim, int x, int y, const int cMask[4], co
 /* QuadRasterizer::rasterize (QuadRasterizer.cpp:71-233) for all row pairs. */
 static void rasterize(Ctx *c, const OPrim *prim) {
   for (int y = prim->yMin & ~1; y < prim->yMax; y += 2) {
+    if (c->clusterCount > 1 && (y >> 1) % c->clusterCount != c->clusterId) continue;
     int x0 = imin(*LEFT(c, 0, y), *LEFT(c, 0, y + 1));
     for (int q = 1; q < c->ms; q++) x0 = imin(x0, imin(*LEFT(c, q, y), *LEFT(c, q, y + 1)));
     x0 &= ~1;
@@ -1751,46 +1772,215 @@ static int setup_point_polygon(const Ctx *c, const OVertex *v, float L[4][4], Po
 }
 
 /* Renders one draw with host pointers in the descriptor. */
-EXPORT int swo_draw(const swr_draw_desc *d) {
+/* Primitive t of the draw through the vertex tail, assembly, clipping and
+ * setup (Renderer.cpp:733-776 and the line / point set-up); returns 1 when
+ * it is visible, with its planes in *prim and its outline in c's span rows. */
+static int setup_primitive(Ctx *c, uint32_t t, OPrim *prim) {
+  const swr_draw_desc *d = c->d;
+  OVertex V[3];
+  uint32_t vidx[3];
+  assemble(d, t, vidx);
+  for (int k = 0; k < 3; k++) {
+    uint32_t idx = vidx[k] < d->vertex_count ? vidx[k] : d->vertex_count - 1;
+    fetch_vertex(d, idx, &V[k]);
+    vertex_routine(c, &V[k]);
+  }
+  Polygon poly;
+  float L[4][4];
+  if (c->point) {
+    if (!setup_point_polygon(c, &V[0], L, &poly)) return 0;
+  } else if (c->line) {
+    if (!setup_line_polygon(c, &V[0], &V[1], L, &poly)) return 0;
+  } else {
+    /* setupSolidTriangles (Renderer.cpp:733-776); cullMask is always 15 here. */
+    if ((V[0].clipFlags & V[1].clipFlags & V[2].clipFlags) != CLIP_FINITE) return 0;
+    poly.P[0][0] = V[0].pos; poly.P[0][1] = V[1].pos; poly.P[0][2] = V[2].pos;
+    poly.n = 3; poly.i = 0; poly.b = 0;
+    int flagsOr = V[0].clipFlags | V[1].clipFlags | V[2].clipFlags;
+    if (flagsOr != CLIP_FINITE && !clipper_clip(&poly, flagsOr)) return 0;
+  }
+  /* A line's third setup vertex is its second, a point's every setup
+   * vertex its one (SetupRoutine.cpp:52-54). */
+  const OVertex *tri[3] = {&V[0], c->point ? &V[0] : &V[1], c->line ? &V[1] : (c->point ? &V[0] : &V[2])};
+  memset(prim, 0, sizeof(*prim));
+  return setup_routine(c, prim, tri, &poly);
+}
+
+static int draw_single(const swr_draw_desc *d, float *state) {
   if (!d || !d->positions) return SWR_ERROR_INVALID_ARGUMENT;
   unsigned csr = _mm_getcsr();
   _mm_setcsr(csr | 0x8040); /* FTZ | DAZ, as the reference's worker threads */
   Ctx c;
   int r = init_ctx(&c, d);
+  c.stateOut = state;
   if (r == SWR_OK) {
     for (uint32_t t = 0; t < d->triangle_count; t++) {
-      OVertex V[3];
-      uint32_t vidx[3];
-      assemble(d, t, vidx);
-      for (int k = 0; k < 3; k++) {
-        uint32_t idx = vidx[k] < d->vertex_count ? vidx[k] : d->vertex_count - 1;
-        fetch_vertex(d, idx, &V[k]);
-        vertex_routine(&c, &V[k]);
-      }
-      Polygon poly;
-      float L[4][4];
-      if (c.point) {
-        if (!setup_point_polygon(&c, &V[0], L, &poly)) continue;
-      } else if (c.line) {
-        if (!setup_line_polygon(&c, &V[0], &V[1], L, &poly)) continue;
-      } else {
-        /* setupSolidTriangles (Renderer.cpp:733-776); cullMask is always 15 here. */
-        if ((V[0].clipFlags & V[1].clipFlags & V[2].clipFlags) != CLIP_FINITE) continue;
-        poly.P[0][0] = V[0].pos; poly.P[0][1] = V[1].pos; poly.P[0][2] = V[2].pos;
-        poly.n = 3; poly.i = 0; poly.b = 0;
-        int flagsOr = V[0].clipFlags | V[1].clipFlags | V[2].clipFlags;
-        if (flagsOr != CLIP_FINITE && !clipper_clip(&poly, flagsOr)) continue;
-      }
-      /* A line's third setup vertex is its second, a point's every setup
-       * vertex its one (SetupRoutine.cpp:52-54). */
-      const OVertex *tri[3] = {&V[0], c.point ? &V[0] : &V[1], c.line ? &V[1] : (c.point ? &V[0] : &V[2])};
       OPrim prim;
-      memset(&prim, 0, sizeof(prim));
-      if (!setup_routine(&c, &prim, tri, &poly)) continue;
+      if (!setup_primitive(&c, t, &prim)) continue;
       c.curTri = t;
       rasterize(&c, &prim);
     }
     if (d->occlusion_count) *d->occlusion_count += c.occlusion;
+  }
+  free_ctx(&c);
+  _mm_setcsr(csr);
+  return r;
+}
+
+/* ===================================================================== */
+/* Multithreaded draw: the reference's worker split (Renderer.cpp:551-662) */
+/* ===================================================================== */
+/* The reference sets primitives up in batches on its marl workers and then
+ * hands every batch to each of its clusters, cluster k rasterising the row
+ * pairs y with (y / 2) mod clusterCount == k (QuadRasterizer.cpp:46-49,
+ * 230), each cluster walking the batch's primitives in order.  The same
+ * split here: per batch of kMtBatch primitives, the workers set up
+ * primitive slots j = w, w + n, ... into per-slot outlines, wait at a
+ * barrier, then each rasterises its row pairs of every visible primitive of
+ * the batch in order, and wait again.  A pixel is touched by one worker
+ * only, in primitive order, so the images equal swo_draw's. */
+enum { kMtBatch = 128, kMtMaxThreads = 64 };
+
+typedef struct {
+  const swr_draw_desc *d;
+  int n;
+  pthread_barrier_t bar;
+  OPrim prim[kMtBatch];
+  int visible[kMtBatch];
+  uint16_t *left[kMtBatch][4], *right[kMtBatch][4];
+  int failed;
+} MtShared;
+
+typedef struct {
+  MtShared *sh;
+  int id;
+  Ctx c;
+  uint16_t *ownLeft[4], *ownRight[4];
+} MtWorker;
+
+static void mt_point_outline(MtWorker *w, int slot) {
+  for (int q = 0; q < w->c.ms; q++) {
+    w->c.left[q] = w->sh->left[slot][q];
+    w->c.right[q] = w->sh->right[slot][q];
+  }
+}
+
+static void *mt_worker(void *arg) {
+  MtWorker *w = (MtWorker *)arg;
+  MtShared *sh = w->sh;
+  unsigned csr = _mm_getcsr();
+  _mm_setcsr(csr | 0x8040); /* FTZ | DAZ */
+  const uint32_t T = sh->d->triangle_count;
+  for (uint32_t base = 0; base < T; base += kMtBatch) {
+    const int count = (int)(T - base < (uint32_t)kMtBatch ? T - base : (uint32_t)kMtBatch);
+    for (int j = w->id; j < count; j += sh->n) {
+      mt_point_outline(w, j);
+      sh->visible[j] = setup_primitive(&w->c, base + (uint32_t)j, &sh->prim[j]);
+    }
+    pthread_barrier_wait(&sh->bar);
+    for (int j = 0; j < count; j++) {
+      if (!sh->visible[j]) continue;
+      mt_point_outline(w, j);
+      w->c.curTri = base + (uint32_t)j;
+      rasterize(&w->c, &sh->prim[j]);
+    }
+    pthread_barrier_wait(&sh->bar);
+  }
+  _mm_setcsr(csr);
+  return NULL;
+}
+
+EXPORT int swo_draw(const swr_draw_desc *d) { return draw_single(d, NULL); }
+
+/* swo_draw on `threads` host threads (1..64); same images and counts. */
+static int draw_mt(const swr_draw_desc *d, int threads, float *state) {
+  if (!d || !d->positions) return SWR_ERROR_INVALID_ARGUMENT;
+  if (threads <= 1) return draw_single(d, state);
+  if (threads > kMtMaxThreads) threads = kMtMaxThreads;
+  MtShared *sh = (MtShared *)calloc(1, sizeof(MtShared));
+  MtWorker *w = (MtWorker *)calloc((size_t)threads, sizeof(MtWorker));
+  pthread_t tid[kMtMaxThreads];
+  int r = (sh && w) ? SWR_OK : SWR_ERROR_OUT_OF_MEMORY;
+  int started = 0, ready = 0;
+  if (r == SWR_OK) {
+    sh->d = d;
+    sh->n = threads;
+    /* Contexts are initialised here, on one thread: init_ctx also fills
+     * the shared constant tables on first use. */
+    for (; ready < threads && r == SWR_OK; ready++) {
+      w[ready].sh = sh;
+      w[ready].id = ready;
+      r = init_ctx(&w[ready].c, d);
+      w[ready].c.clusterId = ready;
+      w[ready].c.clusterCount = threads;
+      w[ready].c.stateOut = state;
+      for (int q = 0; q < 4; q++) { w[ready].ownLeft[q] = w[ready].c.left[q]; w[ready].ownRight[q] = w[ready].c.right[q]; }
+    }
+  }
+  if (r == SWR_OK) {
+    const size_t rows = (size_t)w[0].c.outlineRows;
+    for (int j = 0; j < kMtBatch && r == SWR_OK; j++)
+      for (int q = 0; q < w[0].c.ms; q++) {
+        sh->left[j][q] = (uint16_t *)calloc(rows, 2);
+        sh->right[j][q] = (uint16_t *)calloc(rows, 2);
+        if (!sh->left[j][q] || !sh->right[j][q]) r = SWR_ERROR_OUT_OF_MEMORY;
+      }
+  }
+  if (r == SWR_OK && pthread_barrier_init(&sh->bar, NULL, (unsigned)threads) != 0) r = SWR_ERROR_OUT_OF_MEMORY;
+  if (r == SWR_OK) {
+    for (; started < threads; started++)
+      if (pthread_create(&tid[started], NULL, mt_worker, &w[started]) != 0) break;
+    if (started < threads) {
+      /* Workers already running would wait for the missing ones at the
+       * barrier: this cannot be recovered, so it is reported as fatal. */
+      abort();
+    }
+    for (int k = 0; k < started; k++) pthread_join(tid[k], NULL);
+    pthread_barrier_destroy(&sh->bar);
+    uint64_t occ = 0;
+    for (int k = 0; k < threads; k++) occ += w[k].c.occlusion;
+    if (d->occlusion_count) *d->occlusion_count += occ;
+  }
+  if (w) {
+    for (int k = 0; k < ready; k++) {
+      for (int q = 0; q < 4; q++) { w[k].c.left[q] = w[k].ownLeft[q]; w[k].c.right[q] = w[k].ownRight[q]; }
+      free_ctx(&w[k].c);
+    }
+  }
+  if (sh)
+    for (int j = 0; j < kMtBatch; j++)
+      for (int q = 0; q < 4; q++) { free(sh->left[j][q]); free(sh->right[j][q]); }
+  free(sh);
+  free(w);
+  return r;
+}
+
+EXPORT int swo_draw_mt(const swr_draw_desc *d, int threads) { return draw_mt(d, threads, NULL); }
+
+/* swo_draw (threads > 1: swo_draw_mt) that also records, per sample, the
+ * sampling state (lod, anisotropy, uDelta, vDelta) of the fragment that
+ * last wrote its colour into `state` (S x H x W x 4 floats, caller-zeroed). */
+EXPORT int swo_draw_state(const swr_draw_desc *d, float *state, int threads) { return draw_mt(d, threads, state); }
+
+/* sampleTexture128 of one quad under the draw's sampler state: u[4], v[4]
+ * are lanes (0,0), (1,0), (0,1), (1,1) of the quad; out[16] receives the
+ * lanes' RGBA (RGBA8 textures: the 16-bit result / 0xFF00), state[4] the
+ * quad's (lod, anisotropy, uDelta, vDelta). */
+EXPORT int swo_sample_quad(const swr_draw_desc *d, const float *u, const float *v, float *out, float *state) {
+  if (!d || !d->texture.level_count) return SWR_ERROR_INVALID_ARGUMENT;
+  unsigned csr = _mm_getcsr();
+  _mm_setcsr(csr | 0x8040);
+  Ctx c;
+  int r = init_ctx(&c, d);
+  if (r == SWR_OK) {
+    float col[4][4];
+    SampleState st;
+    sample_texture(&c.tex, u, v, col, &st);
+    memcpy(out, col, sizeof(col));
+    state[0] = st.lod;
+    state[1] = st.aniso;
+    state[2] = st.uDelta;
+    state[3] = st.vDelta;
   }
   free_ctx(&c);
   _mm_setcsr(csr);

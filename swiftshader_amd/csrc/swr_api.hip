@@ -559,9 +559,10 @@ int variant_of(const DrawParams &P) {
 }
 
 // Host plan of a batch: per-draw state in enqueue order, split into
-// segments of draws that share no attachment (so that the draws of a
-// segment can be rendered concurrently), each segment's draws grouped by
-// raster variant.
+// segments of draws none of which depends on an earlier one of its segment
+// (depends_on: no memory one writes meets memory another reads or writes),
+// so that the draws of a segment can be rendered concurrently; each
+// segment's draws grouped by raster variant.
 struct BatchLayout {
   struct Seg {
     int first = 0, count = 0;  // draws [first, first + count) of P
@@ -575,25 +576,84 @@ struct BatchLayout {
   std::vector<Seg> segs;
 };
 
-bool shares_attachment(const DrawParams &a, const DrawParams &b) {
-  auto same = [](const void *x, const void *y) { return x && x == y; };
-  return same(a.color, b.color) || same(a.depth, b.depth) || same(a.stencil, b.stencil) || same(a.primId, b.primId);
+// Device byte ranges [lo, hi) a draw writes (its attachments, read and
+// written) and only reads (vertex streams, indices, point sizes, texture
+// levels).  Draws of one segment run concurrently, so a later draw joins the
+// segment only if none of its ranges meets a range an earlier draw of the
+// segment writes, and none of its writes meets an earlier draw's reads --
+// the ordering of a batch must equal count swr_draw calls on one stream
+// (draw tickets, Renderer.cpp:551-662), also when a texture is an earlier
+// draw's attachment or two attachments overlap without sharing a base.
+struct Extents {
+  std::vector<std::pair<uintptr_t, uintptr_t>> writes, reads;
+};
+
+void add_range(std::vector<std::pair<uintptr_t, uintptr_t>> &v, const void *p, uint64_t bytes) {
+  if (p && bytes) v.push_back({(uintptr_t)p, (uintptr_t)p + (uintptr_t)bytes});
+}
+
+uint64_t image_bytes(uint32_t w, uint32_t h, uint32_t samples, uint64_t rowPitch, uint64_t slice, uint32_t bpp) {
+  return (uint64_t)(samples - 1) * slice + (uint64_t)(h - 1) * rowPitch + (uint64_t)w * bpp;
+}
+
+Extents extents_of(const swr_draw_desc *d, const DrawParams &P) {
+  Extents e;
+  const uint32_t W = d->framebuffer_width, H = d->framebuffer_height, S = d->samples;
+  if (P.color) add_range(e.writes, P.color, image_bytes(W, H, S, P.colorPitch, P.colorSlice, P.colorFloat ? 16 : 4));
+  if (P.depth) add_range(e.writes, P.depth, image_bytes(W, H, S, P.depthPitch, P.depthSlice, P.depth16 ? 2 : 4));
+  if (P.stencil) add_range(e.writes, P.stencil, image_bytes(W, H, S, P.stencilPitch, P.stencilSlice, 1));
+  if (P.primId) add_range(e.writes, P.primId, image_bytes(W, H, S, P.primIdPitch, P.primIdSlice, 4));
+  const uint64_t V = (uint64_t)d->vertex_count;
+  add_range(e.reads, P.pos, V * 16);
+  add_range(e.reads, P.col, V * 16);
+  add_range(e.reads, P.uv, V * 8);
+  add_range(e.reads, P.psize, V * 4);
+  if (P.idx) {
+    const uint64_t T = (uint64_t)d->triangle_count;
+    const uint64_t n = P.topology == 0 ? T * 3 : P.topology == 3 ? T * 2 : P.topology == 4 ? T + 1
+                       : P.topology == 5 ? T : T + 2;
+    add_range(e.reads, P.idx, n * (uint64_t)P.idxBytes);
+  }
+  if (P.useUV) {
+    const uint32_t tb = P.tex.isFloat ? 16 : 4;
+    for (uint32_t l = 0; l < d->texture.level_count && l < (uint32_t)kMipLevels; l++) {
+      const swr_image_level &lv = d->texture.levels[l];
+      add_range(e.reads, lv.data, image_bytes(lv.width, lv.height, 1, lv.row_pitch_bytes, 0, tb));
+    }
+  }
+  return e;
+}
+
+bool ranges_meet(const std::vector<std::pair<uintptr_t, uintptr_t>> &a,
+                 const std::vector<std::pair<uintptr_t, uintptr_t>> &b) {
+  for (const auto &x : a)
+    for (const auto &y : b)
+      if (x.first < y.second && y.first < x.second) return true;
+  return false;
+}
+
+// Whether draw `later` must wait for draw `earlier` of the same batch.
+bool depends_on(const Extents &earlier, const Extents &later) {
+  return ranges_meet(earlier.writes, later.writes) || ranges_meet(earlier.writes, later.reads) ||
+         ranges_meet(earlier.reads, later.writes);
 }
 
 int layout_batch(const swr_draw_desc *descs, uint32_t n, const swr_tuning &tuning, BatchLayout &L) {
   L = BatchLayout{};
   std::vector<DrawParams> all(n);
+  std::vector<Extents> ext(n);
   for (uint32_t i = 0; i < n; i++) {
     int r = build_params(&descs[i], tuning, all[i]);
     if (r != SWR_OK) return fail(r, "draw %u: %s", i, g_error.c_str());
+    ext[i] = extents_of(&descs[i], all[i]);
   }
   uint32_t i = 0;
   while (i < n) {
-    // Extend the segment while the next draw shares no attachment with it.
+    // Extend the segment while the next draw depends on none of its draws.
     uint32_t j = i + 1;
     while (j < n) {
       bool clash = false;
-      for (uint32_t k = i; k < j && !clash; k++) clash = shares_attachment(all[k], all[j]);
+      for (uint32_t k = i; k < j && !clash; k++) clash = depends_on(ext[k], ext[j]);
       if (clash) break;
       j++;
     }
@@ -657,6 +717,13 @@ int upload_layout(DeviceCtx *c, Pipeline &pipe, const BatchLayout &L, hipStream_
   return SWR_OK;
 }
 
+// Device words of Pipeline::counters: the persistent raster workgroups'
+// tile counters (one per raster variant and pass, 16 variants x 2 passes),
+// the eager draws' overflow word and the segment's split slot.
+constexpr int kOverflowSlot = 32;
+constexpr int kSplitSlot = 33;
+constexpr int kCounterWords = 40;
+
 uint32_t sort_passes(uint32_t tiles) {
   int bits = 1;
   while (bits < 32 && (1ull << bits) < (uint64_t)tiles) bits++;
@@ -682,7 +749,7 @@ int ensure_pipe(Pipeline &pipe, uint32_t T, uint32_t cap, uint32_t tiles) {
   SWR_HIP_CHECK(pipe.first.ensure(sizeof(uint32_t) * ((size_t)cap / kEmitBlock + 2)));
   SWR_HIP_CHECK(pipe.tStart.ensure(sizeof(uint32_t) * std::max<uint32_t>(tiles, 1)));
   SWR_HIP_CHECK(pipe.tEnd.ensure(sizeof(uint32_t) * std::max<uint32_t>(tiles, 1)));
-  SWR_HIP_CHECK(pipe.counters.ensure(sizeof(uint32_t) * 32));
+  SWR_HIP_CHECK(pipe.counters.ensure(sizeof(uint32_t) * kCounterWords));
   return SWR_OK;
 }
 
@@ -694,29 +761,39 @@ void enqueue_scan(const uint32_t *in, uint32_t *out, size_t n, uint32_t *sums, h
   hipLaunchKernelGGL(scan_add_kernel, dim3(blocks), dim3(256), 0, s, out, (uint32_t)n, (const uint32_t *)sums);
 }
 
+// Arguments of one raster launch over tiles [tLo, tHi) of draws [lo, hi).
+struct RasterArgs {
+  const DrawParams *params;
+  const uint32_t *tileBase;
+  int lo, hi;
+  uint32_t tLo, tHi;
+  uint32_t *counter;
+  const TriRecord *rec;
+  const int *poly;
+  const uint32_t *bin;  // NULL: the unbinned pass
+  const uint32_t *ts, *te;
+  const uint32_t *split;
+  uint32_t segT;
+};
+
 template <int MS, bool FCOL, bool FTEX, bool XF>
-void launch_raster_batch(dim3 grid, dim3 block, hipStream_t s, const DrawParams *params, const uint32_t *tileBase,
-                         int lo, int hi, uint32_t tLo, uint32_t tHi, uint32_t *counter, const TriRecord *rec,
-                         const int *poly, const uint32_t *bin, const uint32_t *ts, const uint32_t *te) {
-  hipLaunchKernelGGL((raster_batch_kernel<MS, FCOL, FTEX, XF>), grid, block, 0, s, params, tileBase, lo, hi, tLo,
-                     tHi, counter, rec, poly, bin, ts, te);
+void launch_raster_batch(dim3 grid, dim3 block, hipStream_t s, const RasterArgs &a) {
+  hipLaunchKernelGGL((raster_batch_kernel<MS, FCOL, FTEX, XF>), grid, block, 0, s, a.params, a.tileBase, a.lo, a.hi,
+                     a.tLo, a.tHi, a.counter, a.rec, a.poly, a.bin, a.ts, a.te, a.split, a.segT);
 }
 
 // The raster kernel of variant v (variant_of).
 template <bool XF>
-void launch_raster_variant(int v, dim3 grid, dim3 block, hipStream_t s, const DrawParams *params,
-                           const uint32_t *tileBase, int lo, int hi, uint32_t tLo, uint32_t tHi, uint32_t *ctr,
-                           const TriRecord *rec, const int *poly, const uint32_t *bin, const uint32_t *ts,
-                           const uint32_t *te) {
+void launch_raster_variant(int v, dim3 grid, dim3 block, hipStream_t s, const RasterArgs &a) {
   switch (v & 7) {
-    case 0: launch_raster_batch<1, false, false, XF>(grid, block, s, params, tileBase, lo, hi, tLo, tHi, ctr, rec, poly, bin, ts, te); break;
-    case 1: launch_raster_batch<1, false, true, XF>(grid, block, s, params, tileBase, lo, hi, tLo, tHi, ctr, rec, poly, bin, ts, te); break;
-    case 2: launch_raster_batch<1, true, false, XF>(grid, block, s, params, tileBase, lo, hi, tLo, tHi, ctr, rec, poly, bin, ts, te); break;
-    case 3: launch_raster_batch<1, true, true, XF>(grid, block, s, params, tileBase, lo, hi, tLo, tHi, ctr, rec, poly, bin, ts, te); break;
-    case 4: launch_raster_batch<4, false, false, XF>(grid, block, s, params, tileBase, lo, hi, tLo, tHi, ctr, rec, poly, bin, ts, te); break;
-    case 5: launch_raster_batch<4, false, true, XF>(grid, block, s, params, tileBase, lo, hi, tLo, tHi, ctr, rec, poly, bin, ts, te); break;
-    case 6: launch_raster_batch<4, true, false, XF>(grid, block, s, params, tileBase, lo, hi, tLo, tHi, ctr, rec, poly, bin, ts, te); break;
-    default: launch_raster_batch<4, true, true, XF>(grid, block, s, params, tileBase, lo, hi, tLo, tHi, ctr, rec, poly, bin, ts, te); break;
+    case 0: launch_raster_batch<1, false, false, XF>(grid, block, s, a); break;
+    case 1: launch_raster_batch<1, false, true, XF>(grid, block, s, a); break;
+    case 2: launch_raster_batch<1, true, false, XF>(grid, block, s, a); break;
+    case 3: launch_raster_batch<1, true, true, XF>(grid, block, s, a); break;
+    case 4: launch_raster_batch<4, false, false, XF>(grid, block, s, a); break;
+    case 5: launch_raster_batch<4, false, true, XF>(grid, block, s, a); break;
+    case 6: launch_raster_batch<4, true, false, XF>(grid, block, s, a); break;
+    default: launch_raster_batch<4, true, true, XF>(grid, block, s, a); break;
   }
 }
 
@@ -731,19 +808,25 @@ void enqueue_front(Pipeline &pipe, const BatchLayout &L, const BatchLayout::Seg 
 }
 
 // Pair emission, sort, ranges and raster of one segment with pair capacity
-// `cap`; overflow (if any) is reported into *overflowFlag.
+// `cap`.  When a run needs more pairs than `cap` (a plan whose geometry grew
+// since it was built), the total needed goes to *overflowFlag and, with
+// `unbinned`, the primitives whose pairs did not fit are rendered by an
+// unbinned raster pass after the binned one, so that none is dropped.
 int enqueue_back(DeviceCtx *c, Pipeline &pipe, const BatchLayout &L, const BatchLayout::Seg &sg, uint32_t cap,
-                 const swr_tuning &tuning, uint32_t *overflowFlag, EventSet *es, hipStream_t s) {
+                 const swr_tuning &tuning, uint32_t *overflowFlag, bool unbinned, EventSet *es, hipStream_t s) {
   const DrawParams *params = pipe.params.as<DrawParams>() + sg.first;
   const uint32_t *tileBase = pipe.aux.as<uint32_t>() + L.P.size() + sg.first;
+  uint32_t *split = pipe.counters.as<uint32_t>() + kSplitSlot;
   if (cap > 0) {
     if (sg.T > 0)
       hipLaunchKernelGGL(emit_index_kernel, dim3((sg.T + 255) / 256), dim3(256), 0, s,
-                         (const uint32_t *)pipe.off.as<uint32_t>(), sg.T, cap, pipe.first.as<uint32_t>());
+                         (const uint32_t *)pipe.off.as<uint32_t>(), sg.T, cap, pipe.first.as<uint32_t>(), split);
+    else
+      hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(1), 0, s, split, 0u, 1u);
     hipLaunchKernelGGL(bin_emit_batch_kernel, dim3((cap + kEmitBlock - 1) / kEmitBlock), dim3(kEmitBlock), 0, s, params,
                        tileBase, (const TriRecord *)pipe.rec.as<TriRecord>(), (const uint32_t *)pipe.off.as<uint32_t>(),
                        sg.T, (const uint32_t *)pipe.first.as<uint32_t>(), cap, pipe.keysA.as<uint32_t>(),
-                       pipe.valsA.as<uint32_t>(), overflowFlag);
+                       pipe.valsA.as<uint32_t>(), overflowFlag, (const uint32_t *)split);
     const uint32_t nb = (cap + kSortItems - 1) / kSortItems;
     const uint32_t passes = sort_passes(sg.tiles);
     uint32_t *kIn = pipe.keysA.as<uint32_t>(), *vIn = pipe.valsA.as<uint32_t>();
@@ -765,33 +848,46 @@ int enqueue_back(DeviceCtx *c, Pipeline &pipe, const BatchLayout &L, const Batch
     SWR_HIP_CHECK(hipGetLastError());
     if (es) SWR_HIP_CHECK(hipEventRecord(es->ev[EV_BIN], s));
 
-    const uint32_t *bin = vIn;
     const int wpt = (int)((tuning.tile_width / 8) * (tuning.tile_height / 8));
     dim3 block(64 * wpt);
     uint32_t *counters = pipe.counters.as<uint32_t>();
     const bool persistent = tuning.raster_workgroups_per_cu != ~0u;
+    const uint32_t nv = (uint32_t)sg.variants.size();
     if (persistent)
-      hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, counters, 0u, (uint32_t)sg.variants.size());
-    for (size_t v = 0; v < sg.variants.size(); v++) {
-      const int lo = sg.variants[v].first, hi = sg.variants[v].second;
-      const uint32_t tLo = L.tileBase[sg.first + lo];
-      const uint32_t tHi = hi < sg.count ? L.tileBase[sg.first + hi] : sg.tiles;
-      if (tHi <= tLo) continue;
-      uint32_t wgs = tHi - tLo;
-      if (persistent) {
-        const uint32_t per = tuning.raster_workgroups_per_cu ? tuning.raster_workgroups_per_cu : 4u;
-        wgs = std::min<uint32_t>(wgs, (uint32_t)c->cus * per);
+      hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, counters, 0u, unbinned ? 2 * nv : nv);
+    // Binned pass, then (plans) the unbinned pass of the slots left out.
+    for (int pass = 0; pass < (unbinned ? 2 : 1); pass++) {
+      for (uint32_t v = 0; v < nv; v++) {
+        const int lo = sg.variants[v].first, hi = sg.variants[v].second;
+        const uint32_t tLo = L.tileBase[sg.first + lo];
+        const uint32_t tHi = hi < sg.count ? L.tileBase[sg.first + hi] : sg.tiles;
+        if (tHi <= tLo) continue;
+        uint32_t wgs = tHi - tLo;
+        if (persistent) {
+          const uint32_t per = tuning.raster_workgroups_per_cu ? tuning.raster_workgroups_per_cu : 4u;
+          wgs = std::min<uint32_t>(wgs, (uint32_t)c->cus * per);
+        }
+        RasterArgs a;
+        a.params = params;
+        a.tileBase = tileBase;
+        a.lo = lo;
+        a.hi = hi;
+        a.tLo = tLo;
+        a.tHi = tHi;
+        a.counter = persistent ? counters + pass * nv + v : nullptr;
+        a.rec = pipe.rec.as<TriRecord>();
+        a.poly = pipe.poly.as<int>();
+        a.bin = pass == 0 ? vIn : nullptr;
+        a.ts = pipe.tStart.as<uint32_t>();
+        a.te = pipe.tEnd.as<uint32_t>();
+        a.split = split;
+        a.segT = sg.T;
+        const int var = variant_of(L.P[sg.first + lo]);
+        if (var & 8)
+          launch_raster_variant<true>(var, dim3(wgs), block, s, a);
+        else
+          launch_raster_variant<false>(var, dim3(wgs), block, s, a);
       }
-      dim3 grid(wgs);
-      uint32_t *ctr = persistent ? counters + v : nullptr;
-      const TriRecord *rec = pipe.rec.as<TriRecord>();
-      const int *poly = pipe.poly.as<int>();
-      const uint32_t *ts = pipe.tStart.as<uint32_t>(), *te = pipe.tEnd.as<uint32_t>();
-      const int var = variant_of(L.P[sg.first + lo]);
-      if (var & 8)
-        launch_raster_variant<true>(var, grid, block, s, params, tileBase, lo, hi, tLo, tHi, ctr, rec, poly, bin, ts, te);
-      else
-        launch_raster_variant<false>(var, grid, block, s, params, tileBase, lo, hi, tLo, tHi, ctr, rec, poly, bin, ts, te);
     }
   } else if (es) {
     SWR_HIP_CHECK(hipEventRecord(es->ev[EV_BIN], s));
@@ -841,7 +937,8 @@ int draw_eager(DeviceCtx *c, const swr_draw_desc *descs, uint32_t n, hipStream_t
     const uint32_t pairs = *c->hostTotal;
     r = ensure_pipe(pipe, sg.T, pairs, sg.tiles);
     if (r != SWR_OK) return r;
-    r = enqueue_back(c, pipe, L, sg, pairs, tuning, pipe.counters.as<uint32_t>() + 16, prof ? &es : nullptr, s);
+    r = enqueue_back(c, pipe, L, sg, pairs, tuning, pipe.counters.as<uint32_t>() + kOverflowSlot, false,
+                     prof ? &es : nullptr, s);
     if (r != SWR_OK) return r;
     if (prof) {
       SWR_HIP_CHECK(hipEventRecord(es.ev[EV_RASTER], s));
@@ -988,7 +1085,7 @@ int plan_enqueue(DeviceCtx *c, swr_batch_plan *pl, hipStream_t s) {
   for (size_t i = 0; i < pl->L.segs.size(); i++) {
     const auto &sg = pl->L.segs[i];
     enqueue_front(pl->pipe, pl->L, sg, s);
-    int r = enqueue_back(c, pl->pipe, pl->L, sg, sg.pairCap, pl->tuning, pl->flags.as<uint32_t>() + i, nullptr, s);
+    int r = enqueue_back(c, pl->pipe, pl->L, sg, sg.pairCap, pl->tuning, pl->flags.as<uint32_t>() + i, true, nullptr, s);
     if (r != SWR_OK) return r;
   }
   SWR_HIP_CHECK(hipGetLastError());

@@ -130,16 +130,29 @@ __global__ __launch_bounds__(256) void scan_add_kernel(uint32_t *__restrict__ ou
 // off[g + 1].  first[b] is the slot of pair b * kEmitBlock (written for the
 // blocks that start below the total): slot g writes it for the blocks whose
 // first pair falls in its range.
+//
+// *split receives the first slot whose pairs do not all fit below
+// `capacity` (T when everything fits): exactly one thread writes it, the
+// slot whose non-empty pair range holds pair `capacity`, or slot T - 1 when
+// the total fits.  The binned pass renders the pairs of slots below the
+// split; the slots from the split on are rendered afterwards by the
+// unbinned pass of the raster kernel, so a run whose geometry outgrew its
+// plan still renders every primitive, in order.
 __global__ __launch_bounds__(256) void emit_index_kernel(const uint32_t *__restrict__ off, uint32_t T,
-                                                         uint32_t capacity, uint32_t *__restrict__ first) {
+                                                         uint32_t capacity, uint32_t *__restrict__ first,
+                                                         uint32_t *__restrict__ split) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= T) return;
-  const uint32_t lo = off[g], hi = off[g + 1] < capacity ? off[g + 1] : capacity;
+  const uint32_t lo = off[g], hi0 = off[g + 1];
+  const uint32_t hi = hi0 < capacity ? hi0 : capacity;
   for (uint32_t b = (lo + kEmitBlock - 1) / kEmitBlock; b * kEmitBlock < hi; b++) first[b] = g;
+  if (lo <= capacity && capacity < hi0) *split = g;
+  if (g == T - 1 && hi0 <= capacity) *split = T;
 }
 
-// Pairs beyond `capacity` are dropped and the needed count is reported in
-// *overflow (left untouched when everything fits).  A workgroup emits
+// Pairs of the slots from *split on are not emitted (their keys, like those
+// of the unused capacity, are 0xFFFFFFFF) and the total needed is reported
+// in *overflow (left untouched when everything fits).  A workgroup emits
 // kEmitBlock consecutive pairs; their slots lie in [first[b], first[b + 1]]
 // (the last block searches up to T - 1), a range each thread searches,
 // staged in LDS when it fits.
@@ -152,11 +165,13 @@ __global__ __launch_bounds__(kEmitBlock) void bin_emit_batch_kernel(const DrawPa
                                                                     const uint32_t *__restrict__ first,
                                                                     uint32_t capacity, uint32_t *__restrict__ keys,
                                                                     uint32_t *__restrict__ vals,
-                                                                    uint32_t *__restrict__ overflow) {
+                                                                    uint32_t *__restrict__ overflow,
+                                                                    const uint32_t *__restrict__ split) {
   __shared__ uint32_t sOff[kEmitSpan + 1];
   const uint32_t k = blockIdx.x * kEmitBlock + threadIdx.x;
-  const uint32_t total = off[T];
-  if (k == 0 && total > capacity) *overflow = total;
+  const uint32_t needed = off[T];
+  const uint32_t total = off[*split];  // <= capacity
+  if (k == 0 && needed > capacity) *overflow = needed;
   const uint32_t k0 = blockIdx.x * kEmitBlock;
   const bool any = k0 < total && k0 < capacity;  // workgroup-uniform
   int t0 = 0, span = 0;

@@ -1082,7 +1082,7 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
     uint32_t tid = 0;
     float zlo = -__builtin_inff();
     if (e < end) {
-      tid = binTri[e];
+      tid = binTri ? binTri[e] : e;
       const int4 *rp = reinterpret_cast<const int4 *>(rec + tid);
       const int4 h0 = rp[0];  // nflags, yMin, yMax, xMin
       const int4 h1 = rp[1];  // xMax, ...
@@ -1196,6 +1196,12 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
 // from an atomic counter (or, with counter == NULL, workgroup b renders tile
 // tileLo + b once); tile t belongs to the draw d in [drawLo, drawHi) with
 // tileBase[d] <= t < tileBase[d + 1].
+//
+// binTri == NULL selects the unbinned pass: every tile walks the record
+// slots of its draw from *split on (the primitives the binned pass left out
+// because their pairs exceeded the planned capacity) in slot order, testing
+// each against its blocks as it would a binned list.  It does nothing when
+// *split >= segT (everything was binned).
 // Occupancy: 4 waves per SIMD (128 VGPRs) for the single-sample kernels
 // (the float-colour ones spill 12 bytes there and still run faster than at
 // 3 waves), 3 for the multisample ones.
@@ -1209,8 +1215,11 @@ __global__ __launch_bounds__(256, MS == 1 ? 4 : 3) void raster_batch_kernel(
                                                            const int *__restrict__ poly,
                                                            const uint32_t *__restrict__ binTri,
                                                            const uint32_t *__restrict__ tileStart,
-                                                           const uint32_t *__restrict__ tileEnd) {
+                                                           const uint32_t *__restrict__ tileEnd,
+                                                           const uint32_t *__restrict__ split, uint32_t segT) {
   __shared__ uint32_t sTile;
+  const uint32_t from = binTri ? 0u : *split;
+  if (!binTri && from >= segT) return;
   for (uint32_t iter = 0;; iter++) {
     uint32_t t;
     if (counter) {
@@ -1230,7 +1239,15 @@ __global__ __launch_bounds__(256, MS == 1 ? 4 : 3) void raster_batch_kernel(
       else hi = mid;
     }
     lo = __builtin_amdgcn_readfirstlane(lo);
-    const uint32_t s = tileStart[t], e = tileEnd[t];
+    uint32_t s, e;
+    if (binTri) {
+      s = tileStart[t];
+      e = tileEnd[t];
+    } else {
+      const uint32_t b = (uint32_t)params[lo].triBase;
+      s = from > b ? from : b;
+      e = b + (uint32_t)params[lo].triCount;
+    }
     if (s < e) raster_tile<MS, FCOL, FTEX, XF>(params[lo], (int)(t - tileBase[lo]), s, e, rec, poly, binTri);
   }
 }
@@ -1398,7 +1415,7 @@ __global__ void resolve_rgba32f_kernel(const char *src, int srcPitch, long long 
 #define SWR_RASTER_INSTANCE(MS, FCOL, FTEX, XF)                                                                  \
   template __global__ void raster_batch_kernel<MS, FCOL, FTEX, XF>(                                             \
       const DrawParams *, const uint32_t *, int, int, uint32_t, uint32_t, uint32_t *, const TriRecord *, const int *, \
-      const uint32_t *, const uint32_t *, const uint32_t *);
+      const uint32_t *, const uint32_t *, const uint32_t *, const uint32_t *, uint32_t);
 SWR_RASTER_INSTANCE(1, false, false, false)
 SWR_RASTER_INSTANCE(1, false, true, false)
 SWR_RASTER_INSTANCE(1, true, false, false)

@@ -9,6 +9,7 @@ import subprocess
 import numpy as np
 
 from swiftshader_amd import abi
+from swiftshader_amd.scenes import DrawData
 from swiftshader_amd.desc import (COLOR_FORMATS, DEPTH_FORMATS, FORMAT_BYTES, Buffers, build_desc,
                                   color_bytes_per_pixel, depth_bytes_per_pixel)
 
@@ -34,6 +35,12 @@ def oracle():
         L = ctypes.CDLL(ORACLE_SO)
         L.swo_draw.argtypes = [ctypes.POINTER(abi.DrawDesc)]
         L.swo_draw.restype = ctypes.c_int
+        L.swo_draw_mt.argtypes = [ctypes.POINTER(abi.DrawDesc), ctypes.c_int]
+        L.swo_draw_mt.restype = ctypes.c_int
+        L.swo_draw_state.argtypes = [ctypes.POINTER(abi.DrawDesc), ctypes.c_void_p, ctypes.c_int]
+        L.swo_draw_state.restype = ctypes.c_int
+        L.swo_sample_quad.argtypes = [ctypes.POINTER(abi.DrawDesc)] + [ctypes.c_void_p] * 4
+        L.swo_sample_quad.restype = ctypes.c_int
         L.swo_resolve.argtypes = [ctypes.POINTER(abi.ResolveDesc)]
         L.swo_resolve.restype = ctypes.c_int
         L.swo_clear.argtypes = [ctypes.POINTER(abi.ClearDesc)]
@@ -94,16 +101,76 @@ def host_targets(draw):
     return color, depth, stencil
 
 
-def oracle_render_ids(draw):
+def host_threads():
+    """Host threads for the multithreaded oracle: the CPUs this process may
+    use, at most 16."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def oracle_render_ids(draw, threads=1):
     """Renders `draw` (RGBA32F target) with the primitive_id record; returns
     (color, depth, ids (S, H, W) uint32, occlusion)."""
     ids = np.zeros((draw.samples, draw.height, draw.width), np.uint32)
-    color, depth, _, occ = oracle_render(draw, ids=ids)
+    color, depth, _, occ = oracle_render(draw, ids=ids, threads=threads)
     return color, depth, ids, occ
 
 
-def oracle_render(draw, targets=None, ids=None):
-    """Renders `draw` with the oracle; returns (color, depth, stencil, occlusion)."""
+def oracle_render_state(draw, threads=1):
+    """Renders `draw` (RGBA32F target) with the primitive_id record and the
+    per-sample sampling state of the colour-writing fragment; returns
+    (color, depth, ids (S, H, W) uint32, state (S, H, W, 4) float32
+    (lod, anisotropy, uDelta, vDelta), occlusion)."""
+    ids = np.zeros((draw.samples, draw.height, draw.width), np.uint32)
+    state = np.zeros((draw.samples, draw.height, draw.width, 4), np.float32)
+    color, depth, _, occ = oracle_render(draw, ids=ids, threads=threads, state=state)
+    return color, depth, ids, state, occ
+
+
+def _texture_desc(texture):
+    """A draw descriptor holding only `texture` (for swo_sample_quad) and the arrays it points to."""
+    keep = []
+    levels = []
+    for l in texture.levels:
+        a = np.ascontiguousarray(l)
+        keep.append(a)
+        levels.append((a.ctypes.data, a.shape[1], a.shape[0], a.shape[1] * a.shape[2] * a.itemsize))
+    pos = np.zeros((3, 4), np.float32)
+    keep.append(pos)
+    d = DrawData(width=8, height=8, positions=pos, texcoords=np.zeros((3, 2), np.float32), texture=texture,
+                 shader=abi.SHADER_SAMPLE, color_format=abi.FORMAT_R32G32B32A32_SFLOAT, has_depth=False,
+                 depth_test=0)
+    buf = Buffers(positions=pos.ctypes.data, texcoords=pos.ctypes.data)
+    buf.texture_levels = levels
+    return build_desc(d, buf), keep
+
+
+def oracle_sample_quad(texture, u, v):
+    """sampleTexture128 of one quad: u, v are the 4 lanes (0,0), (1,0), (0,1),
+    (1,1).  Returns (rgba (4, 4) float32, state (lod, anisotropy, uDelta,
+    vDelta) float32); RGBA8 textures return the 16-bit result / 0xFF00."""
+    L = oracle()
+    desc, keep = _texture_desc(texture)
+    uu = np.ascontiguousarray(u, np.float32)
+    vv = np.ascontiguousarray(v, np.float32)
+    out = np.zeros((4, 4), np.float32)
+    st = np.zeros(4, np.float32)
+    rc = L.swo_sample_quad(ctypes.byref(desc), uu.ctypes.data, vv.ctypes.data, out.ctypes.data, st.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"oracle swo_sample_quad failed: {rc}")
+    return out, st
+
+
+def oracle_render(draw, targets=None, ids=None, threads=1, state=None):
+    """Renders `draw` with the oracle; returns (color, depth, stencil, occlusion).
+
+    threads > 1 runs swo_draw_mt (row pairs split over host threads, the
+    reference's cluster split), which renders the same bytes.  `state`, an
+    (S, H, W, 4) float32 array, receives the sampling state of each sample's
+    colour-writing fragment."""
     L = oracle()
     color, depth, stencil = targets if targets is not None else host_targets(draw)
     keep = []
@@ -143,7 +210,13 @@ def oracle_render(draw, targets=None, ids=None):
     desc = build_desc(draw, buf)
     if ids is not None:
         desc.primitive_id = abi.Attachment(ids.ctypes.data, abi.FORMAT_R32_UINT, W * 4, H * W * 4)
-    rc = L.swo_draw(ctypes.byref(desc))
+    if state is not None:
+        assert state.shape == (S, H, W, 4) and state.dtype == np.float32 and state.flags["C_CONTIGUOUS"]
+        rc = L.swo_draw_state(ctypes.byref(desc), state.ctypes.data, int(threads))
+    elif threads > 1:
+        rc = L.swo_draw_mt(ctypes.byref(desc), int(threads))
+    else:
+        rc = L.swo_draw(ctypes.byref(desc))
     if rc != 0:
         raise RuntimeError(f"oracle swo_draw failed: {rc}")
     return color, depth, stencil, int(occ[0])

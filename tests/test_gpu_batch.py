@@ -179,3 +179,132 @@ def test_autotune_picks_a_candidate_and_keeps_results():
             _same(want[i], g, names[i])
     finally:
         lib.swr_batch_plan_destroy(plan)
+
+
+def _hazard_draws():
+    """A renders into image IA; B samples IA as its texture; C and D render
+    blended into two overlapping 48-column windows of one 64x64 image (base
+    pointers 16 pixels apart, one row pitch)."""
+    from swiftshader_amd.scenes import DrawData, TextureData
+    rng = np.random.default_rng(21)
+    pos, uv, col = scenes.screen_triangles(rng, 300, 64, 64, area_hi=500)
+    a = DrawData(width=64, height=64, positions=pos[:450], colors=col[:450], shader=abi.SHADER_INTERPOLATE,
+                 clear_color=(0.1, 0.2, 0.3, 1.0))
+    b = DrawData(width=64, height=64, positions=pos[450:], texcoords=uv[450:] * 0.5, shader=abi.SHADER_SAMPLE,
+                 texture=TextureData(format=abi.FORMAT_R8G8B8A8_UNORM, levels=[np.zeros((64, 64, 4), np.uint8)],
+                                     mipmap_mode=abi.MIPMAP_MODE_NEAREST))
+    pos2, _, col2 = scenes.screen_triangles(rng, 200, 48, 64, area_hi=400)
+    blend = (abi.BLEND_FACTOR_SRC_ALPHA, abi.BLEND_FACTOR_ONE_MINUS_SRC_ALPHA, abi.BLEND_OP_ADD,
+             abi.BLEND_FACTOR_ONE, abi.BLEND_FACTOR_ZERO, abi.BLEND_OP_ADD)
+    c = DrawData(width=48, height=64, positions=pos2[:300], colors=col2[:300], shader=abi.SHADER_INTERPOLATE,
+                 has_depth=False, depth_test=0, blend_enable=1, blend=blend)
+    d = DrawData(width=48, height=64, positions=pos2[300:], colors=col2[300:], shader=abi.SHADER_INTERPOLATE,
+                 has_depth=False, depth_test=0, blend_enable=1, blend=blend)
+    return a, b, c, d
+
+
+def _hazard_device(a, b, c, d):
+    da = runtime.prepare(a, DEV)
+    db = runtime.prepare(b, DEV, texture_tensors=[da.targets.color[0]])
+    image = torch.zeros((1, 64, 64, 4), dtype=torch.uint8, device=DEV)
+    dc = runtime.prepare(c, DEV)
+    dd = runtime.prepare(d, DEV)
+    for dev_draw, x0 in ((dc, 0), (dd, 16)):
+        dev_draw.desc.color = abi.Attachment(image.data_ptr() + 4 * x0, abi.FORMAT_R8G8B8A8_UNORM, 64 * 4, 64 * 64 * 4)
+    return [da, db, dc, dd], image
+
+
+def test_batch_orders_draws_that_read_or_overlap_earlier_writes():
+    """A batch equals its draws issued one by one on one stream (and the
+    oracle) when a draw samples an earlier draw's colour attachment and when
+    two attachments overlap without sharing a base pointer."""
+    a, b, c, d = _hazard_draws()
+    seq, img_seq = _hazard_device(a, b, c, d)
+    for x in seq:
+        runtime.draw(x)
+    bat, img_bat = _hazard_device(a, b, c, d)
+    print("[gpu batch] hazard batch", flush=True)
+    runtime.draw_batch(bat)
+    torch.cuda.synchronize()
+    assert np.array_equal(seq[0].targets.color.cpu().numpy(), bat[0].targets.color.cpu().numpy())
+    assert np.array_equal(seq[1].targets.color.cpu().numpy(), bat[1].targets.color.cpu().numpy())
+    assert np.array_equal(img_seq.cpu().numpy(), img_bat.cpu().numpy())
+    # The oracle in API order.
+    ra = oracle_render(a)
+    import dataclasses
+    b_host = dataclasses.replace(b, texture=dataclasses.replace(b.texture, levels=[ra[0][0].copy()]))
+    rb = oracle_render(b_host)
+    assert np.array_equal(rb[0], bat[1].targets.color.cpu().numpy())
+    img = np.zeros((1, 64, 64, 4), np.uint8)
+    for x0, dr in ((0, c), (16, d)):
+        win = np.ascontiguousarray(img[:, :, x0:x0 + 48])
+        r = oracle_render(dr, (win, None, None))
+        img[:, :, x0:x0 + 48] = r[0]
+    assert np.array_equal(img, img_bat.cpu().numpy())
+    # The same batch through a plan.
+    plan, keep = _plan(bat)
+    lib = runtime.lib()
+    try:
+        fresh, img2 = _hazard_device(a, b, c, d)
+        for x, f in zip(bat, fresh):
+            x.targets.color.copy_(f.targets.color)
+            if x.targets.depth is not None:
+                x.targets.depth.copy_(f.targets.depth)
+        img_bat.zero_()
+        runtime.check(lib.swr_batch_plan_run(plan, runtime._stream_handle(None)))
+        torch.cuda.synchronize()
+        assert np.array_equal(rb[0], bat[1].targets.color.cpu().numpy())
+        assert np.array_equal(img, img_bat.cpu().numpy())
+    finally:
+        lib.swr_batch_plan_destroy(plan)
+
+
+def test_plan_whose_geometry_outgrows_it_still_renders_every_primitive():
+    """A plan built with no headroom, whose vertices then move so that the
+    draws need several times the planned (tile, primitive) pairs: the run
+    (eager and replayed from a graph) still equals the oracle bit for bit,
+    and swr_batch_plan_check reports the pairs the run needed."""
+    from swiftshader_amd.scenes import DrawData
+    rng = np.random.default_rng(31)
+    small, big = [], []
+    for k, (w, h, samples) in enumerate(((96, 80, 1), (128, 96, 4), (64, 64, 1))):
+        ps, _, cs = scenes.screen_triangles(rng, 600, w, h, area_lo=2.0, area_hi=20.0)
+        pb, _, cb = scenes.screen_triangles(rng, 600, w, h, area_lo=200.0, area_hi=2000.0)
+        kw = dict(width=w, height=h, samples=samples, shader=abi.SHADER_INTERPOLATE, blend_enable=1,
+                  blend=(abi.BLEND_FACTOR_SRC_ALPHA, abi.BLEND_FACTOR_ONE_MINUS_SRC_ALPHA, abi.BLEND_OP_ADD,
+                         abi.BLEND_FACTOR_ONE, abi.BLEND_FACTOR_ZERO, abi.BLEND_OP_ADD))
+        small.append(DrawData(positions=ps, colors=cs, **kw))
+        big.append(DrawData(positions=pb, colors=cb, **kw))
+    dds = [runtime.prepare(d, DEV) for d in small]
+    plan, keep = _plan(dds, headroom=0)
+    lib = runtime.lib()
+    try:
+        # Same counts, new vertex contents: the plan reads them at run time.
+        for dd, b in zip(dds, big):
+            dd.tensors[0].copy_(torch.from_numpy(b.positions))
+            dd.tensors[1].copy_(torch.from_numpy(b.colors))
+        want = [oracle_render(b) for b in big]
+        _reset(dds)
+        runtime.check(lib.swr_batch_plan_run(plan, runtime._stream_handle(None)))
+        torch.cuda.synchronize()
+        for i, g in enumerate(_snap(dds)):
+            _same(want[i], g, f"grown draw {i}")
+        need = ctypes.c_uint32(0)
+        runtime.check(lib.swr_batch_plan_check(plan, ctypes.byref(need)))
+        assert need.value > 0
+        fresh = [runtime.make_targets(dd.draw, DEV) for dd in dds]
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            for dd, f in zip(dds, fresh):
+                dd.targets.color.copy_(f.color)
+                dd.targets.depth.copy_(f.depth)
+                dd.targets.occlusion.zero_()
+            runtime.check(lib.swr_batch_plan_run(plan, runtime._stream_handle(None)))
+        for rep in range(2):
+            print(f"[gpu batch] grown plan graph replay {rep}", flush=True)
+            graph.replay()
+            torch.cuda.synchronize()
+            for i, g in enumerate(_snap(dds)):
+                _same(want[i], g, f"grown draw {i} replay {rep}")
+    finally:
+        lib.swr_batch_plan_destroy(plan)

@@ -126,3 +126,50 @@ def test_shade_matches_oracle_image_where_snapping_is_exact(samples, shader):
         sel = frag.smask[:, q]
         want = color[q][frag.y[sel], frag.x[sel]].astype(np.float64)
         assert np.max(np.abs(got[sel] - want)) < 2e-5
+
+
+def _grid_scene_mips(seed, samples, aniso, address=abi.ADDRESS_REPEAT, size=64, tex=64, uv_hi=6.0):
+    """_grid_scene with a full mip chain, texture coordinates spread so that
+    the LOD varies over several levels, and optional 8x anisotropy."""
+    rng = np.random.default_rng(seed)
+    n = 40
+    pts = rng.integers(16, (size - 1) * 16, (n * 3, 2)) / 16.0
+    w = rng.choice([0.5, 1.0, 2.0, 4.0], n * 3)
+    ndc = (pts - (size / 2 - 0.5)) / (size / 2)
+    z = rng.uniform(0.1, 0.9, n * 3)
+    pos = np.stack([ndc[:, 0] * w, ndc[:, 1] * w, z * w, w], 1).astype(np.float32)
+    assert np.all(pos[:, 0].astype(np.float64) / pos[:, 3] == ndc[:, 0])
+    col = rng.uniform(0.2, 1.0, (n * 3, 4)).astype(np.float32)
+    uv = rng.uniform(-2.0, uv_hi, (n * 3, 2)).astype(np.float32)
+    t = TextureData(format=abi.FORMAT_R32G32B32A32_SFLOAT, levels=scenes.float_texture(rng, tex),
+                    address_u=address, address_v=address, anisotropy_enable=1 if aniso else 0,
+                    max_anisotropy=8.0 if aniso else 1.0)
+    return DrawData(width=size, height=size, positions=pos, colors=col, texcoords=uv, samples=samples,
+                    shader=abi.SHADER_MODULATE, texture=t, color_format=abi.FORMAT_R32G32B32A32_SFLOAT,
+                    clear_color=(-1, -1, -1, -1))
+
+
+STATE_CASES = [dict(samples=1, aniso=False), dict(samples=1, aniso=True), dict(samples=4, aniso=False),
+               dict(samples=4, aniso=True, address=abi.ADDRESS_CLAMP_TO_EDGE)]
+
+
+@pytest.mark.parametrize("case", range(len(STATE_CASES)))
+def test_shade_at_the_forward_sampling_state_matches_the_oracle_image(case):
+    """The FP64 model evaluated at the sampling state the oracle's forward
+    used for each fragment (trilinear, 8x anisotropic, 4x MSAA) reproduces
+    the oracle's image where the 28.4 snap is exact: the gradient is that
+    of the function the forward renders."""
+    from oracle_runner import oracle_render_state
+    d = _grid_scene_mips(70 + case, **STATE_CASES[case])
+    color, _, ids, state, _ = oracle_render_state(d)
+    frag = G.fragments(ids, state)
+    assert len(frag.t) > 500
+    lods = frag.state[:, 0]
+    assert lods.max() - lods.min() > 1.5  # several levels in play
+    if STATE_CASES[case]["aniso"]:
+        assert (np.rint(frag.state[:, 1]) >= 2).mean() > 0.2
+    got = G.shade(view_of(d), params_of(d), frag)
+    for q in range(d.samples):
+        sel = frag.smask[:, q]
+        want = color[q][frag.y[sel], frag.x[sel]].astype(np.float64)
+        assert np.max(np.abs(got[sel] - want)) < 2e-5
