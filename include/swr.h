@@ -46,7 +46,7 @@
 extern "C" {
 #endif
 
-#define SWR_ABI_VERSION 8
+#define SWR_ABI_VERSION 9
 
 /* ---- status codes -------------------------------------------------------- */
 #define SWR_OK 0
@@ -336,6 +336,16 @@ typedef struct swr_draw_desc {
   uint32_t depth_bounds_test_enable;
   float min_depth_bounds;
   float max_depth_bounds;
+
+  /* Alpha-to-coverage (VkPipelineMultisampleStateCreateInfo::
+   * alphaToCoverageEnable; PixelProgram.cpp:243-259, PixelRoutine.cpp:
+   * 317-337, 643-657, thresholds Renderer.cpp:391-411): the fragment is
+   * shaded before the depth test and sample q keeps its coverage only where
+   * the shaded alpha is not below 0.5 (1 sample) or 0.2, 0.4, 0.6, 0.8
+   * (4 samples) -- the alpha clamped to [0, 1] when the colour attachment
+   * is UNORM, as is, NaN included, otherwise (a NaN passes).  The samples
+   * it removes take no depth, stencil, colour or occlusion effect. */
+  uint32_t alpha_to_coverage_enable;
 } swr_draw_desc;
 
 typedef struct swr_resolve_desc { /* 4x -> 1x colour resolve */

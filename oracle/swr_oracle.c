@@ -231,6 +231,8 @@ typedef struct {
   int depthClip;      /* clip against the near and far planes */
   int depthBoundsActive;
   float minDepthBounds, maxDepthBounds;
+  int alphaToCoverage; /* VkPipelineMultisampleStateCreateInfo::alphaToCoverageEnable */
+  float a2c[4];        /* per-sample alpha thresholds */
   float depthClampLo, depthClampHi;
   int line;           /* LINE_LIST / LINE_STRIP: primitives are lines */
   int bresenham;      /* lines set up as Bresenham parallelograms */
@@ -1288,13 +1290,34 @@ static void quad(Ctx *c, const OPrim *prim, int x, int y, int cMask[4], const fl
       for (int i = 0; i < 4; i++)
         for (int k = 0; k < 4; k++) col[i][k] = col[i][k] * in[k][i];
   }
-  /* clampColor for UNORM targets */
-  if (!c->colorIsFloat && c->colorWriteActive)
+  /* clampColor for UNORM targets: attachment 0 is clamped when it writes
+   * colour or feeds alpha-to-coverage (PixelProgram.cpp:286-364) */
+  if (c->d->color.data && !c->colorIsFloat && (c->colorWriteActive || c->alphaToCoverage))
     for (int i = 0; i < 4; i++)
       for (int k = 0; k < 4; k++) col[i][k] = rr_min(rr_max(col[i][k], 0.0f), 1.0f);
 
+  /* alphaTest (PixelProgram.cpp:243-259): sample q keeps the lanes whose
+   * alpha is not below its threshold (alphaToCoverage, PixelRoutine.cpp:
+   * 643-657); the depth and stencil masks follow the coverage
+   * (PixelRoutine.cpp:317-325), and without a covered sample the depth and
+   * colour stages are skipped (:327). */
+  int alphaPass = 1;
+  if (c->alphaToCoverage) {
+    alphaPass = 0;
+    for (int s = 0; s < ns; s++) {
+      int q = samples[s];
+      int m = 0;
+      for (int i = 0; i < 4; i++)
+        if (!(col[i][3] < c->a2c[q])) m |= 1 << i; /* CmpNLT: a NaN alpha passes */
+      cMask[q] &= m;
+      zMask[q] &= cMask[q];
+      sMask[q] &= cMask[q];
+      alphaPass |= cMask[q];
+    }
+  }
+
   int depthPass = 0;
-  for (int s = 0; s < ns; s++) {
+  for (int s = 0; s < ns && alphaPass; s++) {
     int q = samples[s];
     if (!c->depthTestActive) {
       depthPass = 1;
@@ -1331,7 +1354,7 @@ static void quad(Ctx *c, const OPrim *prim, int x, int y, int cMask[4], const fl
     depth_bounds_test(c, q, x, y, &zMask[q], &cMask[q]);
   }
 
-  if (depthPass) {
+  if (alphaPass && depthPass) {
     if (c->depthWriteEnable) {
       for (int s = 0; s < ns; s++) {
         int q = samples[s];
@@ -1546,6 +1569,13 @@ static int init_ctx(Ctx *c, const swr_draw_desc *d) {
   c->depthTestActive = d->depth.data && d->depth_test_enable;
   c->depthWriteEnable = c->depthTestActive && d->depth_write_enable;
   c->depthBoundsActive = d->depth.data && d->depth_bounds_test_enable; /* Context.cpp:946-949 */
+  /* Alpha-to-coverage thresholds (Renderer.cpp:391-411). */
+  if (d->alpha_to_coverage_enable > 1) return SWR_ERROR_INVALID_ARGUMENT;
+  c->alphaToCoverage = (int)d->alpha_to_coverage_enable;
+  c->a2c[0] = d->samples == 4 ? 0.2f : 0.5f;
+  c->a2c[1] = 0.4f;
+  c->a2c[2] = 0.6f;
+  c->a2c[3] = 0.8f;
   c->minDepthBounds = d->min_depth_bounds;
   c->maxDepthBounds = d->max_depth_bounds;
   c->depthCompare = (int)d->depth_compare_op;

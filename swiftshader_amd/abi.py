@@ -7,7 +7,7 @@ DrawData in the reference (src/Device/Renderer.hpp:58-113, 211-213).
 """
 import ctypes as C
 
-ABI_VERSION = 8
+ABI_VERSION = 9
 
 OK = 0
 ERROR_INVALID_ARGUMENT = -1
@@ -133,6 +133,7 @@ class DrawDesc(C.Structure):
         ("point_sizes", C.c_void_p), ("index_type", C.c_uint32),
         ("depth_clamp_enable", C.c_uint32), ("depth_clip_control", C.c_uint32),
         ("depth_bounds_test_enable", C.c_uint32), ("min_depth_bounds", C.c_float), ("max_depth_bounds", C.c_float),
+        ("alpha_to_coverage_enable", C.c_uint32),
     ]
 
 

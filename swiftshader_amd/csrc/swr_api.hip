@@ -348,6 +348,14 @@ int build_params(const swr_draw_desc *d, const swr_tuning &tuning, DrawParams &P
   P.depthWrite = (P.depthTest && d->depth_write_enable) ? 1 : 0;
   // depthBoundsTestActive (Context.cpp:946-949).
   P.depthBounds = (d->depth.data && d->depth_bounds_test_enable) ? 1 : 0;
+  // Alpha-to-coverage thresholds per sample (Renderer.cpp:391-411).
+  if (d->alpha_to_coverage_enable > 1) return fail(SWR_ERROR_INVALID_ARGUMENT, "alpha_to_coverage_enable %u", d->alpha_to_coverage_enable);
+  P.alphaToCoverage = (int)d->alpha_to_coverage_enable;
+  if (d->samples == 4) {
+    P.a2c[0] = 0.2f; P.a2c[1] = 0.4f; P.a2c[2] = 0.6f; P.a2c[3] = 0.8f;
+  } else {
+    P.a2c[0] = 0.5f;
+  }
   P.depthBoundsMin = d->min_depth_bounds;
   P.depthBoundsMax = d->max_depth_bounds;
   P.depthCompare = (int)d->depth_compare_op;
@@ -553,7 +561,7 @@ int build_params(const swr_draw_desc *d, const swr_tuning &tuning, DrawParams &P
 // out.
 int variant_of(const DrawParams &P) {
   const bool ftex = P.useUV && P.tex.isFloat;
-  const bool xf = P.depth16 || P.colorBGRA || P.colorSRGB || P.advancedBlend || P.depthBounds ||
+  const bool xf = P.depth16 || P.colorBGRA || P.colorSRGB || P.advancedBlend || P.depthBounds || P.alphaToCoverage ||
                   (P.useUV && (P.tex.bgra || P.tex.srgb || P.tex.addrU > 1 || P.tex.addrV > 1));
   return (xf ? 8 : 0) | (P.ms > 1 ? 4 : 0) | (P.colorFloat ? 2 : 0) | (ftex ? 1 : 0);
 }
@@ -959,6 +967,7 @@ int grad_params(const swr_draw_desc *desc, const DrawParams &P, const swr_grad_d
   if (P.useUV && (P.tex.addrU > 1 || P.tex.addrV > 1))
     return fail(SWR_ERROR_UNSUPPORTED, "backward of mirrored address modes");
   if (P.depthBounds) return fail(SWR_ERROR_UNSUPPORTED, "backward of a draw with the depth bounds test");
+  if (P.alphaToCoverage) return fail(SWR_ERROR_UNSUPPORTED, "backward of a draw with alpha-to-coverage");
   if (P.blendEnable) return fail(SWR_ERROR_UNSUPPORTED, "backward of a blended draw");
   if (!g->d_color) return fail(SWR_ERROR_INVALID_ARGUMENT, "d_color is NULL");
   if (g->d_color_row_pitch_bytes < desc->framebuffer_width * 16 ||

@@ -707,7 +707,8 @@ __device__ __forceinline__ QuadLod quad_state(const DrawParams &P, const TriReco
 // to (helper lanes evaluated from the plane equations).  Output clamped
 // for UNORM targets (PixelProgram::clampColor).
 template <bool FCOL, bool FTEX, bool XF>
-__device__ __forceinline__ float4 shade(const DrawParams &P, const TriRecord &R, int px, int py) {
+__device__ __forceinline__ float4 shade(const DrawParams &P, const TriRecord &R, int px, int py,
+                                        bool clampUnorm = true) {
   const float xf = (float)px - R.x0, yf = (float)py - R.y0;
   const float rhw = quad_rhw(R, xf, yf);
   float4 col = make_float4(0, 0, 0, 0);
@@ -729,7 +730,7 @@ __device__ __forceinline__ float4 shade(const DrawParams &P, const TriRecord &R,
     }
     col = c;
   }
-  if (!FCOL) {
+  if (!FCOL && clampUnorm) {
     col.x = rr_min(rr_max(col.x, 0.0f), 1.0f);
     col.y = rr_min(rr_max(col.y, 0.0f), 1.0f);
     col.z = rr_min(rr_max(col.z, 0.0f), 1.0f);
@@ -976,6 +977,24 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
 #pragma unroll
         for (int q = 0; q < MS; q++)
           if (((cMask >> q) & 1) && !stencil_compare(SP, S.sb[q])) sMask &= ~(1 << q);
+      }
+      if (XF && P.alphaToCoverage) {
+        // The reference shades before the depth test and keeps sample q
+        // only where !(alpha < a2c[q]) (alphaTest / alphaToCoverage,
+        // PixelProgram.cpp:243-259, PixelRoutine.cpp:317-337, 643-657); the
+        // alpha is clamped when the colour attachment is UNORM.  The
+        // fragment is shaded again, identically, if it reaches the queue.
+        if (cMask) {
+          const uint32_t tj = __builtin_amdgcn_readlane(tid, j);
+          const float a = shade<FCOL, FTEX, XF>(P, rec[tj], px, py, P.color != nullptr).w;
+          int aMask = 0;
+#pragma unroll
+          for (int q = 0; q < MS; q++)
+            if (!(a < P.a2c[q])) aMask |= 1 << q;
+          cMask &= aMask;
+        }
+        sMask &= cMask;
+        zMask &= cMask;
       }
       float zq[MS];
       if (P.depthTest) {

@@ -73,6 +73,8 @@ struct DrawParams {
   int depth16;  // D16_UNORM depth attachment (else D32_SFLOAT)
   int depthClip;                    // clip against the near and far planes
   int depthBounds;                  // depth bounds test on the stored depth
+  int alphaToCoverage;              // sample q kept where the shaded alpha is not below a2c[q]
+  float a2c[4];
   float depthBoundsMin, depthBoundsMax;
   float depthClampLo, depthClampHi;  // every sample's depth range
   // Depth bias: which terms the setup applies, and whether samples get it.

@@ -85,6 +85,7 @@ def build_desc(draw: DrawData, buf: Buffers) -> abi.DrawDesc:
     if draw.depth_bounds is not None:
         d.depth_bounds_test_enable = 1
         d.min_depth_bounds, d.max_depth_bounds = draw.depth_bounds
+    d.alpha_to_coverage_enable = 1 if draw.alpha_to_coverage else 0
     d.vertex_count = len(draw.positions)
     d.triangle_count = draw.triangle_count
     if draw.viewport is None:

@@ -90,6 +90,7 @@ class DrawData:
     depth_clamp_enable: int = 0
     depth_clip_control: int = 0  # 0 default (clip unless clamping), 1 clip, 2 no clip
     depth_bounds: Optional[Tuple[float, float]] = None  # (min, max) enables the depth bounds test
+    alpha_to_coverage: int = 0
 
     @property
     def triangle_count(self):

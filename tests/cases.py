@@ -415,6 +415,50 @@ def _depth_bounded(draw, lo, hi, clear_depth, depth_test=1, depth_format=None):
     return draw
 
 
+def _alpha_to_coverage(seed=70, samples=4, shader=abi.SHADER_MODULATE, float_target=False, stencil=False,
+                       blend=False, size=(72, 56), tris=700):
+    """Alpha-to-coverage over random alphas (some outside [0, 1]), with
+    depth LESS, optionally stencil ops, blending and a float target."""
+    rng = np.random.default_rng(seed)
+    pos, uv, col = scenes.screen_triangles(rng, tris, size[0], size[1], area_hi=900)
+    col[:, 3] = rng.uniform(-0.2, 1.2, len(col)).astype(np.float32)
+    tex = TextureData(format=abi.FORMAT_R8G8B8A8_UNORM,
+                      levels=scenes.box_mip_chain(scenes.noise_texture_rgba8(rng, 64)))
+    d = DrawData(width=size[0], height=size[1], positions=pos, colors=col, texcoords=uv, samples=samples,
+                 shader=shader, texture=tex if shader != abi.SHADER_INTERPOLATE else None, alpha_to_coverage=1,
+                 color_format=abi.FORMAT_R32G32B32A32_SFLOAT if float_target else abi.FORMAT_R8G8B8A8_UNORM)
+    if stencil:
+        d.has_stencil, d.stencil_test, d.clear_stencil = True, 1, 1
+        d.front = dict(fail_op=abi.STENCIL_OP_INVERT, pass_op=abi.STENCIL_OP_INCREMENT_AND_WRAP,
+                       depth_fail_op=abi.STENCIL_OP_DECREMENT_AND_CLAMP, compare_op=abi.COMPARE_NOT_EQUAL,
+                       compare_mask=0xFF, write_mask=0xFF, reference=6)
+        d.back = dict(fail_op=abi.STENCIL_OP_ZERO, pass_op=abi.STENCIL_OP_REPLACE,
+                      depth_fail_op=abi.STENCIL_OP_INCREMENT_AND_CLAMP, compare_op=abi.COMPARE_ALWAYS,
+                      compare_mask=0xFF, write_mask=0x7F, reference=9)
+    if blend:
+        d.blend_enable = 1
+        d.blend = (abi.BLEND_FACTOR_SRC_ALPHA, abi.BLEND_FACTOR_ONE_MINUS_SRC_ALPHA, abi.BLEND_OP_ADD,
+                   abi.BLEND_FACTOR_ONE, abi.BLEND_FACTOR_ONE_MINUS_SRC_ALPHA, abi.BLEND_OP_ADD)
+    return d
+
+
+def _bounds_ramp(samples=1, depth_test=1, depth_format=abi.FORMAT_D32_SFLOAT, bounds=(0.3, 0.7)):
+    """A depth ramp (z = (x + 1) / 2) then a flat quad at 1/2 drawn with
+    ALWAYS under the depth bounds test, with stencil ops; at 4 samples a
+    column's samples straddle the bounds differently."""
+    xy = np.array([[-1, -1], [1, -1], [1, 1], [-1, -1], [1, 1], [-1, 1]], np.float64)
+    ramp = np.concatenate([xy, (xy[:, :1] + 1) / 2, np.ones((6, 1))], axis=1)
+    flat = np.concatenate([xy, np.full((6, 1), 0.5), np.ones((6, 1))], axis=1)
+    pos = np.concatenate([ramp, flat]).astype(np.float32)
+    col = np.array([[1, 0, 0, 1]] * 6 + [[0, 1, 0, 1]] * 6, np.float32)
+    st = dict(fail_op=abi.STENCIL_OP_ZERO, pass_op=abi.STENCIL_OP_REPLACE,
+              depth_fail_op=abi.STENCIL_OP_INCREMENT_AND_CLAMP, compare_op=abi.COMPARE_ALWAYS,
+              compare_mask=0xFF, write_mask=0xFF, reference=7)
+    return DrawData(width=45, height=6, positions=pos, colors=col, samples=samples, depth_test=depth_test,
+                    depth_compare=abi.COMPARE_ALWAYS, clear_depth=0.5, depth_format=depth_format,
+                    has_stencil=True, stencil_test=1, front=st, back=st, depth_bounds=bounds)
+
+
 def parity_cases():
     """(name, DrawData) pairs; every one must match the oracle bit for bit."""
     return [
@@ -489,4 +533,14 @@ def parity_cases():
         ("srgb_texture_trilinear", _srgb_textured()),
         ("bgra_srgb_texture_aniso_msaa4_srgb_target", _srgb_textured(seed=54, bgra=True, aniso=True, samples=4,
                                                                      target=abi.FORMAT_R8G8B8A8_SRGB)),
+        ("a2c_msaa4_modulate", _alpha_to_coverage()),
+        ("a2c_msaa4_stencil_blend", _alpha_to_coverage(seed=71, stencil=True, blend=True)),
+        ("a2c_1x_gouraud", _alpha_to_coverage(seed=72, samples=1, shader=abi.SHADER_INTERPOLATE)),
+        ("a2c_msaa4_float_target", _alpha_to_coverage(seed=73, float_target=True, shader=abi.SHADER_INTERPOLATE)),
+        ("depth_bounds_ramp_stencil_d32f", _bounds_ramp()),
+        ("depth_bounds_ramp_stencil_d16", _bounds_ramp(depth_format=abi.FORMAT_D16_UNORM)),
+        ("depth_bounds_ramp_stencil_test_off", _bounds_ramp(depth_test=0)),
+        ("depth_bounds_ramp_stencil_msaa4", _bounds_ramp(samples=4)),
+        ("depth_bounds_ramp_stencil_msaa4_d16_test_off", _bounds_ramp(samples=4, depth_test=0,
+                                                                     depth_format=abi.FORMAT_D16_UNORM)),
     ] + [(f"advanced_blend_{op - 1000148000}", _advanced_blend(op, k)) for k, op in enumerate(abi.ADVANCED_BLEND_OPS)]
