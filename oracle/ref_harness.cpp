@@ -8,6 +8,7 @@
 //   src/Pipeline/Constants.cpp (sample offsets and anisotropy tables)
 //   src/System/Half.cpp       (needed by Constants.cpp)
 //   src/Device/BC_Decoder.cpp (BC1-BC7 block decoding)
+//   src/Device/ETC_Decoder.cpp (ETC2 / EAC block decoding)
 //   src/System/Debug.cpp      (needed by BC_Decoder.cpp)
 // The result, oracle/_ref/libswr_ref.so, lets the tests check the oracle's
 // clipper bit-exactly and its constant tables value by value.  The rest of
@@ -19,6 +20,7 @@
 #include <xmmintrin.h>
 
 #include "Device/BC_Decoder.hpp"
+#include "Device/ETC_Decoder.hpp"
 #include "Device/Clipper.hpp"
 #include "Device/Polygon.hpp"
 #include "Device/Renderer.hpp"
@@ -75,6 +77,12 @@ __attribute__((visibility("default"))) void ref_srgb_lut(uint16_t *out) {
 __attribute__((visibility("default"))) int ref_decode_bc(const uint8_t *src, uint8_t *dst, int w, int h, int dstPitch,
                                                           int dstBpp, int n, int isNoAlphaU) {
   return BC_Decoder::Decode(src, dst, w, h, dstPitch, dstBpp, n, isNoAlphaU != 0) ? 1 : 0;
+}
+
+// ETC_Decoder::Decode; `type` is its InputType (0 R11 signed .. 6 RGBA).
+__attribute__((visibility("default"))) int ref_decode_etc(const uint8_t *src, uint8_t *dst, int w, int h,
+                                                           int dstPitch, int dstBpp, int type) {
+  return ETC_Decoder::Decode(src, dst, w, h, dstPitch, dstBpp, static_cast<ETC_Decoder::InputType>(type)) ? 1 : 0;
 }
 
 }  // extern "C"

@@ -2096,9 +2096,13 @@ static uint64_t bc_u64(const uint8_t *p) {
   return v;
 }
 
+int swo_decode_bptc(const uint8_t *src, uint8_t *dst, int w, int h, int dstPitch, int dstBpp, int n, int isSigned);
+
 EXPORT int swo_decode_bc(const uint8_t *src, uint8_t *dst, int w, int h, int dstPitch, int dstBpp, int n,
                          int isNoAlphaU) {
-  if (n < 1 || n > 5) return 0; /* BC6H and BC7 are not restated */
+  /* BC6H (signed unless isNoAlphaU) and BC7: swr_oracle_bptc.c */
+  if (n == 6 || n == 7) return swo_decode_bptc(src, dst, w, h, dstPitch, dstBpp, n, n == 6 && !isNoAlphaU);
+  if (n < 1 || n > 5) return 0;
   const int hasAlpha = n == 1 && !isNoAlphaU;
   const int isSigned = (n == 4 || n == 5) && !isNoAlphaU;
   const int blockBytes = (n == 1 || n == 4) ? 8 : 16;

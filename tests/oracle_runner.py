@@ -59,6 +59,8 @@ def oracle():
         L.swo_srgb_lut.restype = None
         L.swo_decode_bc.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 6
         L.swo_decode_bc.restype = ctypes.c_int
+        L.swo_decode_etc.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 5
+        L.swo_decode_etc.restype = ctypes.c_int
         fp = ctypes.POINTER(ctypes.c_float)
         L.swo_clip_triangle.argtypes = [fp, fp, fp, ctypes.c_int, fp, ctypes.POINTER(ctypes.c_int)]
         L.swo_clip_triangle.restype = ctypes.c_int
@@ -79,6 +81,8 @@ def reference_lib():
         L.ref_srgb_lut.restype = None
         L.ref_decode_bc.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 6
         L.ref_decode_bc.restype = ctypes.c_int
+        L.ref_decode_etc.argtypes = [ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_int] * 5
+        L.ref_decode_etc.restype = ctypes.c_int
         _ref = L
     return _ref
 
@@ -277,3 +281,24 @@ def oracle_decode_bc(src, dst, w, h, pitch, bpp, n, no_alpha_or_unsigned, lib=No
     if lib is None:
         return oracle().swo_decode_bc(src.ctypes.data, dst.ctypes.data, w, h, pitch, bpp, n, no_alpha_or_unsigned)
     return lib.ref_decode_bc(src.ctypes.data, dst.ctypes.data, w, h, pitch, bpp, n, no_alpha_or_unsigned)
+
+
+ETC_TYPES = {"r11s": 0, "r11": 1, "rg11s": 2, "rg11": 3, "rgb8": 4, "rgb8a1": 5, "rgba8": 6}
+
+
+def etc_block_bytes(t):
+    return 16 if t in (2, 3, 6) else 8
+
+
+def etc_out_bpp(t):
+    return 2 if t in (0, 1) else 4
+
+
+def oracle_decode_etc(src, dst, w, h, pitch, bpp, t, lib=None):
+    """Decodes ETC2 / EAC blocks `src` into `dst` (uint8, pitch bytes per
+    row) in place with the oracle, or with `lib`'s ref_decode_etc."""
+    src = np.ascontiguousarray(src, np.uint8)
+    assert dst.flags["C_CONTIGUOUS"] and dst.dtype == np.uint8
+    if lib is None:
+        return oracle().swo_decode_etc(src.ctypes.data, dst.ctypes.data, w, h, pitch, bpp, t)
+    return lib.ref_decode_etc(src.ctypes.data, dst.ctypes.data, w, h, pitch, bpp, t)
