@@ -727,37 +727,53 @@ int upload_layout(DeviceCtx *c, Pipeline &pipe, const BatchLayout &L, hipStream_
 
 // Device words of Pipeline::counters: the persistent raster workgroups'
 // tile counters (one per raster variant and pass, 16 variants x 2 passes),
-// the eager draws' overflow word and the segment's split slot.
+// the eager draws' overflow word, the segment's split slot and the number
+// of pairs the segment emitted.
 constexpr int kOverflowSlot = 32;
 constexpr int kSplitSlot = 33;
+constexpr int kEmittedSlot = 34;
 constexpr int kCounterWords = 40;
 
+// 8-bit radix passes that order every tile key below the key 0xFFFFFFFF of
+// unused capacity and dropped pairs: 2^(8 passes) > tiles, so the sorted
+// digits of a real key never all equal the sentinel's.
 uint32_t sort_passes(uint32_t tiles) {
   int bits = 1;
-  while (bits < 32 && (1ull << bits) < (uint64_t)tiles) bits++;
+  while (bits < 32 && (1ull << bits) <= (uint64_t)tiles) bits++;
   return (uint32_t)((bits + 7) / 8);
+}
+
+// Calls f(buffer, bytes) for every per-segment scratch buffer of `pipe`
+// sized for T slots, `cap` pairs and `tiles` tiles.
+template <typename F>
+void pipe_buffers(Pipeline &pipe, uint32_t T, uint32_t cap, uint32_t tiles, F &&f) {
+  const size_t slots = (size_t)T + 1;
+  const size_t nb = ((size_t)cap + kSortItems - 1) / kSortItems;
+  const size_t scanBlocks = std::max((slots + kScanItems - 1) / kScanItems, (256 * nb + kScanItems - 1) / kScanItems);
+  const size_t pc = std::max<size_t>(cap, 1);
+  f(pipe.counters, sizeof(uint32_t) * kCounterWords);
+  f(pipe.rec, sizeof(TriRecord) * (size_t)T);
+  f(pipe.poly, sizeof(int) * 2 * kMaxPolygon * (size_t)T);
+  f(pipe.cnt, sizeof(uint32_t) * slots);
+  f(pipe.off, sizeof(uint32_t) * slots);
+  f(pipe.sums, sizeof(uint32_t) * std::max<size_t>(scanBlocks, 1));
+  f(pipe.tStart, sizeof(uint32_t) * std::max<uint32_t>(tiles, 1));
+  f(pipe.tEnd, sizeof(uint32_t) * std::max<uint32_t>(tiles, 1));
+  f(pipe.keysA, sizeof(uint32_t) * pc);
+  f(pipe.keysB, sizeof(uint32_t) * pc);
+  f(pipe.valsA, sizeof(uint32_t) * pc);
+  f(pipe.valsB, sizeof(uint32_t) * pc);
+  f(pipe.hist, sizeof(uint32_t) * 256 * std::max<size_t>(nb, 1));
+  f(pipe.first, sizeof(uint32_t) * ((size_t)cap / kEmitBlock + 2));
 }
 
 // Sizes the per-segment scratch of `pipe` for T slots, `cap` pairs, `tiles` tiles.
 int ensure_pipe(Pipeline &pipe, uint32_t T, uint32_t cap, uint32_t tiles) {
-  const size_t slots = (size_t)T + 1;
-  SWR_HIP_CHECK(pipe.rec.ensure(sizeof(TriRecord) * (size_t)T));
-  SWR_HIP_CHECK(pipe.poly.ensure(sizeof(int) * 2 * kMaxPolygon * (size_t)T));
-  SWR_HIP_CHECK(pipe.cnt.ensure(sizeof(uint32_t) * slots));
-  SWR_HIP_CHECK(pipe.off.ensure(sizeof(uint32_t) * slots));
-  const size_t nb = ((size_t)cap + kSortItems - 1) / kSortItems;
-  const size_t scanBlocks = std::max((slots + kScanItems - 1) / kScanItems, (256 * nb + kScanItems - 1) / kScanItems);
-  SWR_HIP_CHECK(pipe.sums.ensure(sizeof(uint32_t) * std::max<size_t>(scanBlocks, 1)));
-  const size_t pc = std::max<size_t>(cap, 1);
-  SWR_HIP_CHECK(pipe.keysA.ensure(sizeof(uint32_t) * pc));
-  SWR_HIP_CHECK(pipe.keysB.ensure(sizeof(uint32_t) * pc));
-  SWR_HIP_CHECK(pipe.valsA.ensure(sizeof(uint32_t) * pc));
-  SWR_HIP_CHECK(pipe.valsB.ensure(sizeof(uint32_t) * pc));
-  SWR_HIP_CHECK(pipe.hist.ensure(sizeof(uint32_t) * 256 * std::max<size_t>(nb, 1)));
-  SWR_HIP_CHECK(pipe.first.ensure(sizeof(uint32_t) * ((size_t)cap / kEmitBlock + 2)));
-  SWR_HIP_CHECK(pipe.tStart.ensure(sizeof(uint32_t) * std::max<uint32_t>(tiles, 1)));
-  SWR_HIP_CHECK(pipe.tEnd.ensure(sizeof(uint32_t) * std::max<uint32_t>(tiles, 1)));
-  SWR_HIP_CHECK(pipe.counters.ensure(sizeof(uint32_t) * kCounterWords));
+  hipError_t err = hipSuccess;
+  pipe_buffers(pipe, T, cap, tiles, [&](Scratch &b, size_t bytes) {
+    if (err == hipSuccess) err = b.ensure(bytes);
+  });
+  SWR_HIP_CHECK(err);
   return SWR_OK;
 }
 
@@ -825,6 +841,7 @@ int enqueue_back(DeviceCtx *c, Pipeline &pipe, const BatchLayout &L, const Batch
   const DrawParams *params = pipe.params.as<DrawParams>() + sg.first;
   const uint32_t *tileBase = pipe.aux.as<uint32_t>() + L.P.size() + sg.first;
   uint32_t *split = pipe.counters.as<uint32_t>() + kSplitSlot;
+  uint32_t *emitted = pipe.counters.as<uint32_t>() + kEmittedSlot;
   if (cap > 0) {
     if (sg.T > 0)
       hipLaunchKernelGGL(emit_index_kernel, dim3((sg.T + 255) / 256), dim3(256), 0, s,
@@ -834,25 +851,26 @@ int enqueue_back(DeviceCtx *c, Pipeline &pipe, const BatchLayout &L, const Batch
     hipLaunchKernelGGL(bin_emit_batch_kernel, dim3((cap + kEmitBlock - 1) / kEmitBlock), dim3(kEmitBlock), 0, s, params,
                        tileBase, (const TriRecord *)pipe.rec.as<TriRecord>(), (const uint32_t *)pipe.off.as<uint32_t>(),
                        sg.T, (const uint32_t *)pipe.first.as<uint32_t>(), cap, pipe.keysA.as<uint32_t>(),
-                       pipe.valsA.as<uint32_t>(), overflowFlag, (const uint32_t *)split);
+                       pipe.valsA.as<uint32_t>(), overflowFlag, (const uint32_t *)split, emitted);
     const uint32_t nb = (cap + kSortItems - 1) / kSortItems;
     const uint32_t passes = sort_passes(sg.tiles);
     uint32_t *kIn = pipe.keysA.as<uint32_t>(), *vIn = pipe.valsA.as<uint32_t>();
     uint32_t *kOut = pipe.keysB.as<uint32_t>(), *vOut = pipe.valsB.as<uint32_t>();
     for (uint32_t p = 0; p < passes; p++) {
-      hipLaunchKernelGGL(radix_hist_kernel, dim3(nb), dim3(256), 0, s, (const uint32_t *)kIn, cap, (int)(8 * p),
-                         pipe.hist.as<uint32_t>(), nb);
+      hipLaunchKernelGGL(radix_hist_kernel, dim3(nb), dim3(256), 0, s, (const uint32_t *)kIn,
+                         (const uint32_t *)emitted, (int)(8 * p), pipe.hist.as<uint32_t>(), nb);
       enqueue_scan(pipe.hist.as<uint32_t>(), pipe.hist.as<uint32_t>(), (size_t)256 * nb, pipe.sums.as<uint32_t>(), s);
       hipLaunchKernelGGL(radix_scatter_kernel, dim3(nb), dim3(256), 0, s, (const uint32_t *)kIn, (const uint32_t *)vIn,
-                         cap, (int)(8 * p), (const uint32_t *)pipe.hist.as<uint32_t>(), nb, kOut, vOut);
+                         (const uint32_t *)emitted, (int)(8 * p), (const uint32_t *)pipe.hist.as<uint32_t>(), nb,
+                         kOut, vOut);
       std::swap(kIn, kOut);
       std::swap(vIn, vOut);
     }
     const uint32_t tb = (sg.tiles + 255) / 256;
     hipLaunchKernelGGL(fill_u32_kernel, dim3(tb), dim3(256), 0, s, pipe.tStart.as<uint32_t>(), 0u, sg.tiles);
     hipLaunchKernelGGL(fill_u32_kernel, dim3(tb), dim3(256), 0, s, pipe.tEnd.as<uint32_t>(), 0u, sg.tiles);
-    hipLaunchKernelGGL(tile_range_batch_kernel, dim3((cap + 255) / 256), dim3(256), 0, s, (const uint32_t *)kIn, cap,
-                       sg.tiles, pipe.tStart.as<uint32_t>(), pipe.tEnd.as<uint32_t>());
+    hipLaunchKernelGGL(tile_range_batch_kernel, dim3((cap + 255) / 256), dim3(256), 0, s, (const uint32_t *)kIn,
+                       (const uint32_t *)emitted, sg.tiles, pipe.tStart.as<uint32_t>(), pipe.tEnd.as<uint32_t>());
     SWR_HIP_CHECK(hipGetLastError());
     if (es) SWR_HIP_CHECK(hipEventRecord(es->ev[EV_BIN], s));
 

@@ -9,9 +9,12 @@
 //   scan_*                exclusive prefix sum of the per-primitive tile
 //                         counts (three passes, no memset, fixed sizes)
 //   emit_index_kernel     the first primitive slot of every 256-pair block
-//   bin_emit_batch_kernel one thread per (tile, primitive) pair up to a
-//                         fixed capacity; entries past the real count get
-//                         the key 0xFFFFFFFF so that they sort last
+//   bin_emit_batch_kernel one thread per (tile, primitive) pair of the
+//                         primitives' bounding boxes up to a fixed capacity,
+//                         with the 8x8 blocks of the tile the primitive's
+//                         edges may reach; pairs that reach none, and
+//                         entries past the real count, get the key
+//                         0xFFFFFFFF so that they sort last
 //   radix_*               stable LSD radix sort of the pairs by tile, 8 bits
 //                         per pass: per-block digit histograms, a scan, and
 //                         an in-order scatter (waves rank their keys with
@@ -19,7 +22,7 @@
 //                         keeps every tile's primitives in API order; the
 //                         block is staged in LDS sorted, then written in
 //                         runs)
-//   tile_range_kernel     start / end of each tile's run of pairs
+//   tile_range_batch_kernel  start / end of each tile's run of pairs
 //
 // Every size is fixed when the work is enqueued and nothing is cleared with
 // a memset, so the whole front end can be captured into a HIP graph and
@@ -150,12 +153,54 @@ __global__ __launch_bounds__(256) void emit_index_kernel(const uint32_t *__restr
   if (g == T - 1 && hi0 <= capacity) *split = T;
 }
 
+// The 8x8 blocks of a tile that a triangle may cover, as bits 0-3 (block b
+// of the tile in row-major order, tileW / 8 per row).  All sample points of
+// a block's pixels lie within 6/16 px of their centres (28.4 coordinate
+// 16 x + xo); a block is dropped when one edge of the triangle has the
+// block's centres grown by half a pixel on every side strictly on its outer
+// side.  Covered samples lie in the closed triangle (the span rule takes
+// x >= ceil(Xleft / 16) and x < ceil(Xright / 16) of the spanning edges),
+// so no covered sample is dropped.  Together with the bounding-box test of
+// the list walk this is the separating-axis test of the triangle against
+// the block.  Clipped polygons, lines and points (n != 3) and degenerate
+// triangles keep every block.
+__device__ __forceinline__ uint32_t tile_block_mask(const TriRecord &R, int tx, int ty, int tileW, int tileH) {
+  const int bw = tileW >> 3, nb = bw * (tileH >> 3);
+  const uint32_t all = (1u << nb) - 1u;
+  if ((R.nflags & 0xFF) != 3) return all;
+  const long long X0 = R.X[0], X1 = R.X[1], X2 = R.X[2], Y0 = R.Y[0], Y1 = R.Y[1], Y2 = R.Y[2];
+  const long long area = (X1 - X0) * (Y2 - Y0) - (Y1 - Y0) * (X2 - X0);
+  if (area == 0) return all;
+  const long long s = area > 0 ? 1 : -1;
+  uint32_t mask = 0;
+  for (int b = 0; b < nb; b++) {
+    const long long bx0 = 16ll * (tx * tileW + 8 * (b % bw)) - 8, by0 = 16ll * (ty * tileH + 8 * (b / bw)) - 8;
+    const long long bx1 = bx0 + 16 * 8, by1 = by0 + 16 * 8;
+    bool out = false;
+#pragma unroll
+    for (int e = 0; e < 3; e++) {
+      const long long xa = e == 0 ? X0 : (e == 1 ? X1 : X2), ya = e == 0 ? Y0 : (e == 1 ? Y1 : Y2);
+      const long long xb = e == 0 ? X1 : (e == 1 ? X2 : X0), yb = e == 0 ? Y1 : (e == 1 ? Y2 : Y0);
+      // s * f(x, y), f = (xb - xa)(y - ya) - (yb - ya)(x - xa), is positive
+      // inside; its largest value over the block is at the corner below.
+      const long long cx = -s * (yb - ya), cy = s * (xb - xa);
+      const long long x = cx > 0 ? bx1 : bx0, y = cy > 0 ? by1 : by0;
+      out |= cx * (x - xa) + cy * (y - ya) < 0;
+    }
+    if (!out) mask |= 1u << b;
+  }
+  return mask;
+}
+
 // Pairs of the slots from *split on are not emitted (their keys, like those
 // of the unused capacity, are 0xFFFFFFFF) and the total needed is reported
 // in *overflow (left untouched when everything fits).  A workgroup emits
 // kEmitBlock consecutive pairs; their slots lie in [first[b], first[b + 1]]
 // (the last block searches up to T - 1), a range each thread searches,
-// staged in LDS when it fits.
+// staged in LDS when it fits.  A pair's value is its slot with, in bits
+// 28-31, the blocks of its tile the primitive may cover (tile_block_mask);
+// a pair that covers none of them gets the key 0xFFFFFFFF and leaves the
+// tile's list.
 constexpr int kEmitSpan = 1024;
 
 __global__ __launch_bounds__(kEmitBlock) void bin_emit_batch_kernel(const DrawParams *__restrict__ params,
@@ -166,12 +211,14 @@ __global__ __launch_bounds__(kEmitBlock) void bin_emit_batch_kernel(const DrawPa
                                                                     uint32_t capacity, uint32_t *__restrict__ keys,
                                                                     uint32_t *__restrict__ vals,
                                                                     uint32_t *__restrict__ overflow,
-                                                                    const uint32_t *__restrict__ split) {
+                                                                    const uint32_t *__restrict__ split,
+                                                                    uint32_t *__restrict__ emitted) {
   __shared__ uint32_t sOff[kEmitSpan + 1];
   const uint32_t k = blockIdx.x * kEmitBlock + threadIdx.x;
   const uint32_t needed = off[T];
   const uint32_t total = off[*split];  // <= capacity
   if (k == 0 && needed > capacity) *overflow = needed;
+  if (k == 0) *emitted = total;
   const uint32_t k0 = blockIdx.x * kEmitBlock;
   const bool any = k0 < total && k0 < capacity;  // workgroup-uniform
   int t0 = 0, span = 0;
@@ -213,15 +260,23 @@ __global__ __launch_bounds__(kEmitBlock) void bin_emit_batch_kernel(const DrawPa
   const int tx0 = R.xMin / P.tileW, tx1 = (R.xMax - 1) / P.tileW;
   const int ty0 = R.yMin / P.tileH;
   const uint32_t w = (uint32_t)(tx1 - tx0 + 1);
-  keys[k] = tileBase[R.draw] + (uint32_t)((ty0 + (int)(i / w)) * P.tilesX + tx0 + (int)(i % w));
-  vals[k] = (uint32_t)g;
+  const int tx = tx0 + (int)(i % w), ty = ty0 + (int)(i / w);
+  const uint32_t bm = tile_block_mask(R, tx, ty, P.tileW, P.tileH);
+  keys[k] = bm ? tileBase[R.draw] + (uint32_t)(ty * P.tilesX + tx) : 0xFFFFFFFFu;
+  vals[k] = (uint32_t)g | (bm << 28);
 }
 
 // ---- stable LSD radix sort, 8 bits per pass ---------------------------------
+// The sort and the tile ranges cover the *count pairs that were emitted
+// (bin_emit_batch_kernel's total, at most the capacity the grids are sized
+// for): blocks past it write empty histograms and nothing else, so unused
+// capacity costs no sorting.
 // hist is digit-major: hist[digit * nb + block].
-__global__ __launch_bounds__(256) void radix_hist_kernel(const uint32_t *__restrict__ keys, uint32_t n, int shift,
+__global__ __launch_bounds__(256) void radix_hist_kernel(const uint32_t *__restrict__ keys,
+                                                         const uint32_t *__restrict__ count, int shift,
                                                          uint32_t *__restrict__ hist, uint32_t nb) {
   __shared__ uint32_t h[256];
+  const uint32_t n = *count;
   h[threadIdx.x] = 0;
   __syncthreads();
   const uint32_t base = blockIdx.x * kSortItems;
@@ -242,11 +297,14 @@ __global__ __launch_bounds__(256) void radix_hist_kernel(const uint32_t *__restr
 // written out with consecutive threads on consecutive positions of each
 // digit's run.
 __global__ __launch_bounds__(256) void radix_scatter_kernel(const uint32_t *__restrict__ keysIn,
-                                                            const uint32_t *__restrict__ valsIn, uint32_t n,
-                                                            int shift, const uint32_t *__restrict__ histScan,
-                                                            uint32_t nb, uint32_t *__restrict__ keysOut,
+                                                            const uint32_t *__restrict__ valsIn,
+                                                            const uint32_t *__restrict__ count, int shift,
+                                                            const uint32_t *__restrict__ histScan, uint32_t nb,
+                                                            uint32_t *__restrict__ keysOut,
                                                             uint32_t *__restrict__ valsOut) {
   constexpr int kSteps = kSortItems / 256;
+  const uint32_t n = *count;
+  if (blockIdx.x * (uint32_t)kSortItems >= n) return;  // workgroup-uniform
   __shared__ uint32_t cnt[4][256];  // per wave and digit: running count, then the wave's offset
   __shared__ uint32_t lstart[256];  // block-local start of each digit
   __shared__ uint32_t gstart[256];  // output start of each digit for this block
@@ -331,9 +389,11 @@ __global__ __launch_bounds__(256) void radix_scatter_kernel(const uint32_t *__re
   }
 }
 
-// Tile ranges over sorted keys; keys >= tiles (unused capacity) are skipped.
-__global__ void tile_range_batch_kernel(const uint32_t *__restrict__ keys, uint32_t n, uint32_t tiles,
-                                        uint32_t *__restrict__ start, uint32_t *__restrict__ end) {
+// Tile ranges over the *count sorted keys; keys >= tiles (dropped pairs) are
+// skipped.
+__global__ void tile_range_batch_kernel(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ count,
+                                        uint32_t tiles, uint32_t *__restrict__ start, uint32_t *__restrict__ end) {
+  const uint32_t n = *count;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t k = keys[i];

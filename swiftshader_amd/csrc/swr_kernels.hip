@@ -1,18 +1,18 @@
 // swr_kernels.hip -- HIP kernels of the fixed-function draw path (gfx950).
 //
-// Pipeline of one draw:
+// Kernels of a draw (the front end between setup and raster, binning and
+// the sort, is in swr_batch.hip):
 //   setup_kernel     one thread per triangle: vertex tail (clip flags,
 //                    28.4 projection), trivial reject, clipping, culling,
-//                    plane equations, pixel/tile bounding box.
-//   bin_emit_kernel  one thread per visible triangle: (tile, triangle)
-//                    pairs, triangle-major, at offsets from a prefix sum.
-//   (stable radix sort of the pairs by tile keeps API order inside a tile)
-//   tile_range_kernel  per-tile [start, end) into the sorted pair list.
-//   raster_kernel    one wavefront per 16x16 screen tile, one 2x2 quad per
-//                    lane; the tile's depth / stencil / colour live in
-//                    registers while the wave walks its triangle list in
-//                    order: coverage, stencil, depth, shading, blend.
-//   resolve_kernel   4x -> 1x colour resolve.
+//                    plane equations, pixel/tile bounding box (setup_one,
+//                    shared with the batch front end).
+//   raster_batch_kernel  one workgroup per screen tile, one wave per 8x8
+//                    block, one pixel per lane (a 2x2 quad in lanes
+//                    4k..4k+3); the block's depth / stencil / colour live in
+//                    registers while the wave walks the tile's primitive
+//                    list in order: coverage, stencil, depth, shading, blend.
+//   clear_rect_kernel, decode_bc_kernel, resolve_*_kernel  clears, BC
+//                    decoding and the 4x -> 1x resolve.
 //
 // Reference semantics: src/Pipeline/VertexRoutine.cpp:116-154,601-611,
 // src/Device/Clipper.cpp, src/Pipeline/SetupRoutine.cpp,
@@ -518,44 +518,9 @@ __global__ __launch_bounds__(256) void setup_kernel(DrawParams P, TriRecord *__r
   setup_one(P, t, (size_t)t, 0, rec, poly, tileCount);
 }
 
-// ---------------------------------------------------------------------------
-// Binning
-// ---------------------------------------------------------------------------
-// One thread per (tile, primitive) pair: the pair's primitive is found by
-// binary search over the prefix sum of per-primitive tile counts, so the
-// pairs of large primitives are written in parallel and coalesced.
-__global__ __launch_bounds__(256) void bin_emit_kernel(DrawParams P, const TriRecord *__restrict__ rec,
-                                                       const uint32_t *__restrict__ tileOffset, uint32_t pairs,
-                                                       uint32_t *__restrict__ keys, uint32_t *__restrict__ vals) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= pairs) return;
-  int lo = 0, hi = P.triCount;  // tileOffset[lo] <= k < tileOffset[hi]
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (tileOffset[mid] <= k) lo = mid;
-    else hi = mid;
-  }
-  const TriRecord &R = rec[lo];
-  const uint32_t i = k - tileOffset[lo];
-  const int tx0 = R.xMin / P.tileW, tx1 = (R.xMax - 1) / P.tileW;
-  const int ty0 = R.yMin / P.tileH;
-  const uint32_t w = (uint32_t)(tx1 - tx0 + 1);
-  keys[k] = (uint32_t)((ty0 + (int)(i / w)) * P.tilesX + tx0 + (int)(i % w));
-  vals[k] = (uint32_t)lo;
-}
-
 __global__ void fill_u32_kernel(uint32_t *p, uint32_t v, uint32_t n) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
-}
-
-__global__ void tile_range_kernel(const uint32_t *__restrict__ keys, uint32_t n, uint32_t *__restrict__ start,
-                                  uint32_t *__restrict__ end) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  uint32_t k = keys[i];
-  if (i == 0 || keys[i - 1] != k) start[k] = i;
-  if (i == n - 1 || keys[i + 1] != k) end[k] = i + 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -1100,8 +1065,19 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
     bool test = false;
     uint32_t tid = 0;
     float zlo = -__builtin_inff();
+    // A binned pair carries the blocks of its tile that the primitive's
+    // edges may reach (bits 28-31); the box test completes the overlap test.
+    bool reach = true;
     if (e < end) {
-      tid = binTri ? binTri[e] : e;
+      if (binTri) {
+        const uint32_t v = binTri[e];
+        tid = v & kSlotMask;
+        reach = (v >> (28 + wave)) & 1u;
+      } else {
+        tid = e;
+      }
+    }
+    if (e < end && reach) {
       const int4 *rp = reinterpret_cast<const int4 *>(rec + tid);
       const int4 h0 = rp[0];  // nflags, yMin, yMax, xMin
       const int4 h1 = rp[1];  // xMax, ...

@@ -21,6 +21,9 @@ enum : int { SHADER_INTERPOLATE = 0, SHADER_SAMPLE = 1, SHADER_MODULATE = 2 };
 
 constexpr int kMipLevels = 15;
 constexpr int kMaxPolygon = 16;
+// A binned (tile, primitive) pair's value: the primitive's record slot in
+// bits 0-27, the 8x8 blocks of the tile it may cover in bits 28-31.
+constexpr uint32_t kSlotMask = (1u << 28) - 1u;
 
 struct TexParams {
   const void *data[kMipLevels];
