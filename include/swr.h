@@ -46,7 +46,7 @@
 extern "C" {
 #endif
 
-#define SWR_ABI_VERSION 9
+#define SWR_ABI_VERSION 10
 
 /* ---- status codes -------------------------------------------------------- */
 #define SWR_OK 0
@@ -447,6 +447,23 @@ int swr_release(void);
 /* Renders one draw (sw::Renderer::draw).  Reads and writes only the memory
  * the descriptor names plus library scratch.  Asynchronous on `stream`. */
 int swr_draw(const swr_draw_desc *desc, void *stream);
+
+/* Renders one draw as swr_draw does, without allocating.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory --
+ * code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  It calls no hipMalloc, hipFree or host-pinned allocation and
+ * never waits on the host.  Its scratch (primitive records, the (tile,
+ * primitive) pairs and their sort) is one device array of
+ * SWR_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends and shared
+ * by every call on that GPU, so calls that overlap on one GPU (other
+ * streams, other threads) race.  Each primitive takes 296 bytes and each
+ * 16x16 screen tile 8; the pairs get what is left, and the pairs beyond it
+ * are rendered by the unbinned pass (slower, never dropped).  A draw that
+ * leaves room for fewer than 65,536 pairs (about 330,000 primitives)
+ * returns SWR_ERROR_UNSUPPORTED.  swr_init(device) locates the device's
+ * array; otherwise the first call on that device does. */
+#define SWR_NOALLOC_SCRATCH_BYTES (96u << 20)
+int swr_draw_noalloc(const swr_draw_desc *desc, void *stream);
 
 /* Renders `count` independent draws in one submission, in array order
  * (draws may target different attachments).  Equivalent to `count` calls

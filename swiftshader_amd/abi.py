@@ -7,7 +7,7 @@ DrawData in the reference (src/Device/Renderer.hpp:58-113, 211-213).
 """
 import ctypes as C
 
-ABI_VERSION = 9
+ABI_VERSION = 10
 
 OK = 0
 ERROR_INVALID_ARGUMENT = -1
@@ -184,7 +184,7 @@ class Profile(C.Structure):
 
 # Symbols declared by include/swr.h, in declaration order.
 EXPORTED_SYMBOLS = (
-    "swr_abi_version", "swr_last_error", "swr_init", "swr_release", "swr_draw",
+    "swr_abi_version", "swr_last_error", "swr_init", "swr_release", "swr_draw", "swr_draw_noalloc",
     "swr_draw_batch", "swr_resolve", "swr_clear", "swr_decode_bc", "swr_batch_plan_create", "swr_batch_plan_run", "swr_batch_plan_check",
     "swr_batch_plan_autotune", "swr_batch_plan_set_grads", "swr_batch_plan_backward", "swr_batch_plan_destroy", "swr_draw_backward", "swr_generate_mips", "swr_mip_backward",
     "swr_profile_enable", "swr_profile_read", "swr_set_tuning", "swr_get_tuning",
@@ -200,6 +200,8 @@ def bind(lib):
     lib.swr_release.restype = C.c_int
     lib.swr_draw.argtypes = [C.POINTER(DrawDesc), C.c_void_p]
     lib.swr_draw.restype = C.c_int
+    lib.swr_draw_noalloc.argtypes = [C.POINTER(DrawDesc), C.c_void_p]
+    lib.swr_draw_noalloc.restype = C.c_int
     lib.swr_draw_batch.argtypes = [C.POINTER(DrawDesc), C.c_uint32, C.c_void_p]
     lib.swr_draw_batch.restype = C.c_int
     lib.swr_resolve.argtypes = [C.POINTER(ResolveDesc), C.c_void_p]

@@ -976,6 +976,120 @@ int draw_eager(DeviceCtx *c, const swr_draw_desc *descs, uint32_t n, hipStream_t
   return SWR_OK;
 }
 
+// ---- swr_draw_noalloc: one draw whose scratch is a fixed device array ----
+// Callers that cannot allocate device memory render through `scratch`, a
+// file-scope device array, instead of the grow-on-demand pipeline buffers.
+// Host code never uses the array's name as an address (on the host it is a
+// stand-in, not device memory): each device's copy is located once with
+// hipGetSymbolAddress, with that device current, and its address kept in
+// g_static.  The pair capacity is whatever the array leaves after the
+// per-primitive buffers; pairs beyond it go through the unbinned raster
+// pass, so no host wait is needed and nothing is allocated per call.
+constexpr size_t kStaticScratchBytes = SWR_NOALLOC_SCRATCH_BYTES;
+constexpr uint32_t kStaticMinPairs = 65536;  // smallest pair capacity a draw may be left with
+
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+struct StaticDev {
+  char *base = nullptr;  // this device's copy of `scratch`
+  int cus = 0;
+};
+StaticDev g_static[64];  // guarded by g_mutex
+
+// Locates device `dev`'s copy of the array (once); `dev` must be current.
+int static_setup(int dev) {
+  if (dev < 0 || dev >= 64) return fail(SWR_ERROR_DEVICE, "device index %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_mutex);
+  StaticDev &sd = g_static[dev];
+  if (sd.base) return SWR_OK;
+  void *p = nullptr;
+  SWR_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) return fail(SWR_ERROR_DEVICE, "no device address for the static scratch array");
+  int cus = 0;
+  SWR_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  sd.cus = cus > 0 ? cus : 256;
+  sd.base = (char *)p;
+  return SWR_OK;
+}
+
+// Places the buffers of one segment (n draws, T slots, `cap` pairs, `tiles`
+// tiles) at `base`, 256-byte aligned; returns the bytes they span (`base`
+// NULL: sizes only).
+size_t place_pipe(char *base, Pipeline &pipe, uint32_t n, uint32_t T, uint32_t cap, uint32_t tiles) {
+  size_t at = 0;
+  auto put = [&](Scratch &b, size_t bytes) {
+    at = (at + 255) & ~size_t(255);
+    b.ptr = base ? base + at : nullptr;
+    b.bytes = bytes;
+    at += bytes;
+  };
+  put(pipe.params, sizeof(DrawParams) * n);
+  put(pipe.aux, sizeof(uint32_t) * 2 * n);
+  pipe_buffers(pipe, T, cap, tiles, put);
+  return at;
+}
+
+// Writes a one-draw segment's parameters and bases into the scratch (a
+// kernel, so that nothing goes through host memory).
+static_assert(sizeof(DrawParams) % 4 == 0 && sizeof(DrawParams) + 32 <= 4096, "DrawParams as a kernel argument");
+__global__ void store_params_kernel(DrawParams P, DrawParams *dst, uint32_t *aux) {
+  const uint32_t *src = reinterpret_cast<const uint32_t *>(&P);
+  uint32_t *d = reinterpret_cast<uint32_t *>(dst);
+  for (uint32_t i = threadIdx.x; i < sizeof(DrawParams) / 4; i += blockDim.x) d[i] = src[i];
+  if (threadIdx.x == 0) {
+    aux[0] = 0u;  // blockBase
+    aux[1] = 0u;  // tileBase
+  }
+}
+
+int draw_noalloc(const swr_draw_desc *desc, hipStream_t s) {
+  int dev = 0;
+  SWR_HIP_CHECK(hipGetDevice(&dev));
+  int r = static_setup(dev);
+  if (r != SWR_OK) return r;
+  swr_tuning tuning{};
+  tuning.tile_width = 16;
+  tuning.tile_height = 16;
+  tuning.raster_workgroups_per_cu = ~0u;
+  {
+    std::lock_guard<std::mutex> lock(g_mutex);
+    if (g_ctx[dev]) tuning = g_ctx[dev]->tuning;  // as swr_set_tuning left it for swr_draw
+  }
+  BatchLayout L;
+  r = layout_batch(desc, 1, tuning, L);
+  if (r != SWR_OK) return r;
+  if (L.segs.empty()) return SWR_OK;
+  const BatchLayout::Seg &sg = L.segs[0];
+  Pipeline pipe;
+  const size_t fixed = place_pipe(nullptr, pipe, 1, sg.T, 0, sg.tiles);
+  uint32_t cap = 0;
+  if (fixed < kStaticScratchBytes) {
+    uint64_t lo = 0, hi = (kStaticScratchBytes - fixed) / 16 + 1;  // place_pipe(cap = lo) fits, hi does not
+    while (hi - lo > 1) {
+      const uint64_t mid = (lo + hi) / 2;
+      if (place_pipe(nullptr, pipe, 1, sg.T, (uint32_t)mid, sg.tiles) <= kStaticScratchBytes) lo = mid;
+      else hi = mid;
+    }
+    cap = (uint32_t)std::min<uint64_t>(lo, 0xFFFFFF00ull);
+  }
+  if (cap < kStaticMinPairs)
+    return fail(SWR_ERROR_UNSUPPORTED,
+                "swr_draw_noalloc: %u primitives and %u tiles leave room for %u (tile, primitive) pairs in the "
+                "%zu-byte scratch (at least %u needed)",
+                sg.T, sg.tiles, cap, kStaticScratchBytes, kStaticMinPairs);
+  const StaticDev &sd = g_static[dev];
+  place_pipe(sd.base, pipe, 1, sg.T, cap, sg.tiles);
+  hipLaunchKernelGGL(store_params_kernel, dim3(1), dim3(256), 0, s, L.P[0], pipe.params.as<DrawParams>(),
+                     pipe.aux.as<uint32_t>());
+  SWR_HIP_CHECK(hipGetLastError());
+  enqueue_front(pipe, L, sg, s);
+  SWR_HIP_CHECK(hipGetLastError());
+  DeviceCtx dc;  // enqueue_back reads only the CU count from it
+  dc.device = dev;
+  dc.cus = sd.cus;
+  return enqueue_back(&dc, pipe, L, sg, cap, tuning, pipe.counters.as<uint32_t>() + kOverflowSlot, true, nullptr, s);
+}
+
 // Validates a gradient descriptor against its draw and derives GradParams.
 int grad_params(const swr_draw_desc *desc, const DrawParams &P, const swr_grad_desc *g, GradParams &G) {
   G = GradParams{};
@@ -1396,7 +1510,9 @@ int swr_init(int device) {
   if (device < 0 || device >= count) return fail(SWR_ERROR_INVALID_ARGUMENT, "device %d of %d", device, count);
   SWR_HIP_CHECK(hipSetDevice(device));
   DeviceCtx *c = nullptr;
-  return current_ctx(&c);
+  int r = current_ctx(&c);
+  if (r != SWR_OK) return r;
+  return static_setup(device);
 }
 
 int swr_release(void) {
@@ -1414,6 +1530,11 @@ int swr_draw(const swr_draw_desc *desc, void *stream) {
   if (r != SWR_OK) return r;
   if (!desc) return fail(SWR_ERROR_INVALID_ARGUMENT, "descriptor is NULL");
   return draw_eager(c, desc, 1, reinterpret_cast<hipStream_t>(stream));
+}
+
+int swr_draw_noalloc(const swr_draw_desc *desc, void *stream) {
+  if (!desc) return fail(SWR_ERROR_INVALID_ARGUMENT, "descriptor is NULL");
+  return draw_noalloc(desc, reinterpret_cast<hipStream_t>(stream));
 }
 
 int swr_draw_batch(const swr_draw_desc *descs, uint32_t count, void *stream) {
