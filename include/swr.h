@@ -46,7 +46,7 @@
 extern "C" {
 #endif
 
-#define SWR_ABI_VERSION 10
+#define SWR_ABI_VERSION 11
 
 /* ---- status codes -------------------------------------------------------- */
 #define SWR_OK 0
@@ -314,7 +314,9 @@ typedef struct swr_draw_desc {
    * Renderer.cpp:696-716): UINT16 (0), UINT32 (1) or UINT8_EXT
    * (1000265000).  Primitive restart is the caller's: the reference splits
    * the index buffer into separate draws before Renderer::draw
-   * (Context.cpp:48-90, 249-285). */
+   * (Context.cpp:48-90, 249-285).  A zero-initialised descriptor reads
+   * 16-bit indices: callers written for ABI 5 or earlier, whose index
+   * buffers were always 32-bit, must set SWR_INDEX_TYPE_UINT32. */
   uint32_t index_type;
 
   /* Depth clamp and depth clip (Context.cpp:647-648, 686-693;
@@ -490,10 +492,12 @@ int swr_clear(const swr_clear_desc *desc, void *stream);
  * layer (src/Vulkan/VkImage.cpp:1381-1399) to fill the decompressed image
  * the sampler reads.  `src` holds ceil(width / 4) x ceil(height / 4) blocks
  * in row-major order (8 bytes for BC1 and BC4, 16 otherwise); pixels
- * outside the image are not written.  BC1-BC3 write whole B8G8R8A8 pixels
- * (dst and its pitch 4-byte aligned, dst_bytes_per_pixel 4); BC4 writes
- * byte 0 and BC5 bytes 0 and 1 of each pixel, leaving the others as they
- * are.  n is 1..5; BC6H and BC7 return SWR_ERROR_UNSUPPORTED. */
+ * outside the image are not written.  BC1-BC3 and BC7 write whole B8G8R8A8
+ * pixels (dst and its pitch 4-byte aligned, dst_bytes_per_pixel 4); BC4
+ * writes byte 0 and BC5 bytes 0 and 1 of each pixel, leaving the others as
+ * they are; BC6H writes four half floats R, G, B, 1.0 (dst_bytes_per_pixel
+ * 8, 2-byte aligned), UF16 when no_alpha_or_unsigned is set, else SF16.
+ * n is 1..7. */
 typedef struct swr_bc_desc {
   const void *src;                /* device pointer, 8-byte aligned */
   void *dst;                      /* device pointer to pixel (0, 0) */
@@ -501,10 +505,40 @@ typedef struct swr_bc_desc {
   uint32_t dst_row_pitch_bytes;
   uint32_t dst_bytes_per_pixel;
   uint32_t n;                     /* the n of BCn */
-  uint32_t no_alpha_or_unsigned;  /* BC1: RGB without alpha; BC4 / BC5: UNORM (else SNORM) */
+  uint32_t no_alpha_or_unsigned;  /* BC1: RGB without alpha; BC4 / BC5: UNORM (else SNORM); BC6H: UF16 (else SF16) */
 } swr_bc_desc;
 
 int swr_decode_bc(const swr_bc_desc *desc, void *stream);
+
+/* ETC2 / EAC decoding: ETC_Decoder::Decode (src/Device/ETC_Decoder.hpp:42),
+ * which vk::Image::decodeETC2 calls per mip level and layer
+ * (src/Vulkan/VkImage.cpp:1340-1376).  `src` holds ceil(width / 4) x
+ * ceil(height / 4) blocks in row-major order (16 bytes for RG11 and RGBA8,
+ * 8 otherwise); input_type is ETC_Decoder::InputType.  RGB8, RGB8 with
+ * punch-through alpha and RGBA8 write B8G8R8A8 pixels (dst_bytes_per_pixel
+ * 4, 4-byte aligned; a punch-through texel of index 2 is transparent
+ * black); R11 writes one and RG11 two 16-bit values per pixel, the 11-bit
+ * value clamped and shifted left by 5 (dst_bytes_per_pixel 2 and 4, 2-byte
+ * aligned).  Pixels outside the image are not written. */
+enum {
+  SWR_ETC_R_SIGNED = 0,
+  SWR_ETC_R_UNSIGNED = 1,
+  SWR_ETC_RG_SIGNED = 2,
+  SWR_ETC_RG_UNSIGNED = 3,
+  SWR_ETC_RGB = 4,
+  SWR_ETC_RGB_PUNCHTHROUGH_ALPHA = 5,
+  SWR_ETC_RGBA = 6
+};
+typedef struct swr_etc_desc {
+  const void *src;                /* device pointer, 8-byte aligned */
+  void *dst;                      /* device pointer to pixel (0, 0) */
+  uint32_t width, height;
+  uint32_t dst_row_pitch_bytes;
+  uint32_t dst_bytes_per_pixel;
+  uint32_t input_type;            /* SWR_ETC_* */
+} swr_etc_desc;
+
+int swr_decode_etc(const swr_etc_desc *desc, void *stream);
 
 /* Prepares `count` draws (as swr_draw_batch would render them) for
  * repeated runs; pair_headroom_percent (e.g. 10) over-provisions the pair

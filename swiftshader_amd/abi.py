@@ -7,7 +7,7 @@ DrawData in the reference (src/Device/Renderer.hpp:58-113, 211-213).
 """
 import ctypes as C
 
-ABI_VERSION = 10
+ABI_VERSION = 11
 
 OK = 0
 ERROR_INVALID_ARGUMENT = -1
@@ -143,6 +143,16 @@ class BCDesc(C.Structure):
                 ("no_alpha_or_unsigned", C.c_uint32)]
 
 
+class ETCDesc(C.Structure):
+    _fields_ = [("src", C.c_void_p), ("dst", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("dst_row_pitch_bytes", C.c_uint32), ("dst_bytes_per_pixel", C.c_uint32),
+                ("input_type", C.c_uint32)]
+
+
+ETC_R_SIGNED, ETC_R_UNSIGNED, ETC_RG_SIGNED, ETC_RG_UNSIGNED = 0, 1, 2, 3
+ETC_RGB, ETC_RGB_PUNCHTHROUGH_ALPHA, ETC_RGBA = 4, 5, 6
+
+
 class ResolveDesc(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("format", C.c_uint32),
                 ("src", C.c_void_p), ("src_row_pitch_bytes", C.c_uint32), ("src_sample_pitch_bytes", C.c_uint64),
@@ -185,7 +195,7 @@ class Profile(C.Structure):
 # Symbols declared by include/swr.h, in declaration order.
 EXPORTED_SYMBOLS = (
     "swr_abi_version", "swr_last_error", "swr_init", "swr_release", "swr_draw", "swr_draw_noalloc",
-    "swr_draw_batch", "swr_resolve", "swr_clear", "swr_decode_bc", "swr_batch_plan_create", "swr_batch_plan_run", "swr_batch_plan_check",
+    "swr_draw_batch", "swr_resolve", "swr_clear", "swr_decode_bc", "swr_decode_etc", "swr_batch_plan_create", "swr_batch_plan_run", "swr_batch_plan_check",
     "swr_batch_plan_autotune", "swr_batch_plan_set_grads", "swr_batch_plan_backward", "swr_batch_plan_destroy", "swr_draw_backward", "swr_generate_mips", "swr_mip_backward",
     "swr_profile_enable", "swr_profile_read", "swr_set_tuning", "swr_get_tuning",
 )
@@ -210,6 +220,8 @@ def bind(lib):
     lib.swr_clear.restype = C.c_int
     lib.swr_decode_bc.argtypes = [C.POINTER(BCDesc), C.c_void_p]
     lib.swr_decode_bc.restype = C.c_int
+    lib.swr_decode_etc.argtypes = [C.POINTER(ETCDesc), C.c_void_p]
+    lib.swr_decode_etc.restype = C.c_int
     lib.swr_batch_plan_create.argtypes = [C.POINTER(DrawDesc), C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p),
                                           C.c_void_p]
     lib.swr_batch_plan_create.restype = C.c_int

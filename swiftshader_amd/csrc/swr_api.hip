@@ -24,6 +24,7 @@
 #include "swr_kernels.hip"
 #include "swr_batch.hip"
 #include "swr_grad.hip"
+#include "swr_decode.hip"
 
 using namespace swr;
 
@@ -1690,16 +1691,19 @@ int swr_clear(const swr_clear_desc *d, void *stream) {
 
 int swr_decode_bc(const swr_bc_desc *d, void *stream) {
   if (!d || !d->src || !d->dst) return fail(SWR_ERROR_INVALID_ARGUMENT, "BC descriptor incomplete");
-  if (d->n == 6 || d->n == 7) return fail(SWR_ERROR_UNSUPPORTED, "BC%u decoding is not built", d->n);
   if (d->n < 1 || d->n > 7) return fail(SWR_ERROR_INVALID_ARGUMENT, "BC%u is not a BC format", d->n);
   if (d->width == 0 || d->height == 0) return SWR_OK;
   if (d->width > 16384 || d->height > 16384) return fail(SWR_ERROR_INVALID_ARGUMENT, "image larger than 16384");
   if (reinterpret_cast<uintptr_t>(d->src) & 7u) return fail(SWR_ERROR_INVALID_ARGUMENT, "src is not 8-byte aligned");
   const uint32_t bpp = d->dst_bytes_per_pixel;
-  if (d->n <= 3) {
+  if (d->n <= 3 || d->n == 7) {
     if (bpp != 4) return fail(SWR_ERROR_INVALID_ARGUMENT, "BC%u writes 4-byte pixels", d->n);
     if ((reinterpret_cast<uintptr_t>(d->dst) | d->dst_row_pitch_bytes) & 3u)
       return fail(SWR_ERROR_INVALID_ARGUMENT, "BC%u needs a 4-byte aligned destination and pitch", d->n);
+  } else if (d->n == 6) {
+    if (bpp != 8) return fail(SWR_ERROR_INVALID_ARGUMENT, "BC6H writes 8-byte pixels (four half floats)");
+    if ((reinterpret_cast<uintptr_t>(d->dst) | d->dst_row_pitch_bytes) & 1u)
+      return fail(SWR_ERROR_INVALID_ARGUMENT, "BC6H needs a 2-byte aligned destination and pitch");
   } else if (bpp < d->n - 3u) {
     return fail(SWR_ERROR_INVALID_ARGUMENT, "BC%u needs %u bytes per pixel", d->n, d->n - 3u);
   }
@@ -1707,8 +1711,37 @@ int swr_decode_bc(const swr_bc_desc *d, void *stream) {
     return fail(SWR_ERROR_INVALID_ARGUMENT, "destination pitch too small");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   dim3 block(256), grid((d->width + 255) / 256, d->height);
-  hipLaunchKernelGGL(decode_bc_kernel, grid, block, 0, s, (const uint8_t *)d->src, (uint8_t *)d->dst, (int)d->width,
-                     (int)d->height, (int)d->dst_row_pitch_bytes, (int)bpp, (int)d->n, (int)d->no_alpha_or_unsigned);
+  if (d->n >= 6)
+    hipLaunchKernelGGL(decode_bptc_kernel, grid, block, 0, s, (const uint8_t *)d->src, (uint8_t *)d->dst,
+                       (int)d->width, (int)d->height, (int)d->dst_row_pitch_bytes, (int)bpp, (int)d->n,
+                       (int)(d->n == 6 && !d->no_alpha_or_unsigned));
+  else
+    hipLaunchKernelGGL(decode_bc_kernel, grid, block, 0, s, (const uint8_t *)d->src, (uint8_t *)d->dst, (int)d->width,
+                       (int)d->height, (int)d->dst_row_pitch_bytes, (int)bpp, (int)d->n, (int)d->no_alpha_or_unsigned);
+  SWR_HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}
+
+int swr_decode_etc(const swr_etc_desc *d, void *stream) {
+  if (!d || !d->src || !d->dst) return fail(SWR_ERROR_INVALID_ARGUMENT, "ETC descriptor incomplete");
+  if (d->input_type > SWR_ETC_RGBA) return fail(SWR_ERROR_INVALID_ARGUMENT, "ETC input type %u", d->input_type);
+  if (d->width == 0 || d->height == 0) return SWR_OK;
+  if (d->width > 16384 || d->height > 16384) return fail(SWR_ERROR_INVALID_ARGUMENT, "image larger than 16384");
+  if (reinterpret_cast<uintptr_t>(d->src) & 7u) return fail(SWR_ERROR_INVALID_ARGUMENT, "src is not 8-byte aligned");
+  const uint32_t t = d->input_type;
+  const bool eac = t <= SWR_ETC_RG_UNSIGNED;
+  const uint32_t bpp = eac ? (t <= SWR_ETC_R_UNSIGNED ? 2u : 4u) : 4u;
+  if (d->dst_bytes_per_pixel != bpp)
+    return fail(SWR_ERROR_INVALID_ARGUMENT, "ETC input type %u writes %u-byte pixels", t, bpp);
+  const uintptr_t align = eac ? 1u : 3u;
+  if ((reinterpret_cast<uintptr_t>(d->dst) | d->dst_row_pitch_bytes) & align)
+    return fail(SWR_ERROR_INVALID_ARGUMENT, "destination or pitch not %u-byte aligned", (unsigned)align + 1);
+  if ((uint64_t)d->dst_row_pitch_bytes < (uint64_t)d->width * bpp)
+    return fail(SWR_ERROR_INVALID_ARGUMENT, "destination pitch too small");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  dim3 block(256), grid((d->width + 255) / 256, d->height);
+  hipLaunchKernelGGL(decode_etc_kernel, grid, block, 0, s, (const uint8_t *)d->src, (uint8_t *)d->dst, (int)d->width,
+                     (int)d->height, (int)d->dst_row_pitch_bytes, (int)bpp, (int)t);
   SWR_HIP_CHECK(hipGetLastError());
   return SWR_OK;
 }

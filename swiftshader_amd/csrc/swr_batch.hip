@@ -165,31 +165,32 @@ __global__ __launch_bounds__(256) void emit_index_kernel(const uint32_t *__restr
 // the block.  Clipped polygons, lines and points (n != 3) and degenerate
 // triangles keep every block.
 __device__ __forceinline__ uint32_t tile_block_mask(const TriRecord &R, int tx, int ty, int tileW, int tileH) {
-  const int bw = tileW >> 3, nb = bw * (tileH >> 3);
-  const uint32_t all = (1u << nb) - 1u;
+  const int bw = tileW >> 3, bh = tileH >> 3;
+  const uint32_t all = (1u << (bw * bh)) - 1u;
   if ((R.nflags & 0xFF) != 3) return all;
   const long long X0 = R.X[0], X1 = R.X[1], X2 = R.X[2], Y0 = R.Y[0], Y1 = R.Y[1], Y2 = R.Y[2];
   const long long area = (X1 - X0) * (Y2 - Y0) - (Y1 - Y0) * (X2 - X0);
   if (area == 0) return all;
   const long long s = area > 0 ? 1 : -1;
-  uint32_t mask = 0;
-  for (int b = 0; b < nb; b++) {
-    const long long bx0 = 16ll * (tx * tileW + 8 * (b % bw)) - 8, by0 = 16ll * (ty * tileH + 8 * (b / bw)) - 8;
-    const long long bx1 = bx0 + 16 * 8, by1 = by0 + 16 * 8;
-    bool out = false;
+  // Block (i, j) of the tile, grown: x in [bx + 128 i, bx + 128 i + 128]
+  // (28.4), likewise y.
+  const long long bx = 16ll * tx * tileW - 8, by = 16ll * ty * tileH - 8;
+  uint32_t out = 0;
 #pragma unroll
-    for (int e = 0; e < 3; e++) {
-      const long long xa = e == 0 ? X0 : (e == 1 ? X1 : X2), ya = e == 0 ? Y0 : (e == 1 ? Y1 : Y2);
-      const long long xb = e == 0 ? X1 : (e == 1 ? X2 : X0), yb = e == 0 ? Y1 : (e == 1 ? Y2 : Y0);
-      // s * f(x, y), f = (xb - xa)(y - ya) - (yb - ya)(x - xa), is positive
-      // inside; its largest value over the block is at the corner below.
-      const long long cx = -s * (yb - ya), cy = s * (xb - xa);
-      const long long x = cx > 0 ? bx1 : bx0, y = cy > 0 ? by1 : by0;
-      out |= cx * (x - xa) + cy * (y - ya) < 0;
-    }
-    if (!out) mask |= 1u << b;
+  for (int e = 0; e < 3; e++) {
+    const long long xa = e == 0 ? X0 : (e == 1 ? X1 : X2), ya = e == 0 ? Y0 : (e == 1 ? Y1 : Y2);
+    const long long xb = e == 0 ? X1 : (e == 1 ? X2 : X0), yb = e == 0 ? Y1 : (e == 1 ? Y2 : Y0);
+    // s * f(x, y), f = (xb - xa)(y - ya) - (yb - ya)(x - xa), is positive
+    // inside; its largest value over a block is at the corner picked by the
+    // signs of its slopes, and moves by 128 slopes from block to block.
+    const long long cx = -s * (yb - ya), cy = s * (xb - xa);
+    const long long v = cx * ((cx > 0 ? bx + 128 : bx) - xa) + cy * ((cy > 0 ? by + 128 : by) - ya);
+    const long long stepX = 128 * cx, stepY = 128 * cy;
+    for (int j = 0; j < bh; j++)
+      for (int i = 0; i < bw; i++)
+        if (v + i * stepX + j * stepY < 0) out |= 1u << (j * bw + i);
   }
-  return mask;
+  return all & ~out;
 }
 
 // Pairs of the slots from *split on are not emitted (their keys, like those

@@ -11,8 +11,8 @@
 //                    4k..4k+3); the block's depth / stencil / colour live in
 //                    registers while the wave walks the tile's primitive
 //                    list in order: coverage, stencil, depth, shading, blend.
-//   clear_rect_kernel, decode_bc_kernel, resolve_*_kernel  clears, BC
-//                    decoding and the 4x -> 1x resolve.
+//   clear_rect_kernel, resolve_*_kernel  clears and the 4x -> 1x resolve
+//                    (compressed texture decoding is in swr_decode.hip).
 //
 // Reference semantics: src/Pipeline/VertexRoutine.cpp:116-154,601-611,
 // src/Device/Clipper.cpp, src/Pipeline/SetupRoutine.cpp,
@@ -1263,79 +1263,6 @@ __global__ void clear_rect_kernel(char *base, int rowPitch, long long slice, int
   else if (BYTES == 4) *reinterpret_cast<uint32_t *>(p) = value.x;
   else if (BYTES == 2) *reinterpret_cast<uint16_t *>(p) = (uint16_t)value.x;
   else *reinterpret_cast<uint8_t *>(p) = (uint8_t)value.x;
-}
-
-// BC1-BC5 decoding (BC_Decoder.cpp:31-217, 1662-1777), one thread per
-// pixel: the thread reads its 4x4 block (neighbours read the same bytes
-// through the cache) and evaluates only the palette entry its 2- or 3-bit
-// index selects.  Colour endpoints widen 565 by bit replication, blue in
-// byte 0; 4-colour mode (c0 > c1, or a separate alpha block) takes thirds,
-// 3-colour mode halves and black, transparent when the format has alpha.
-// Channel blocks take sevenths (c0 > c1) or fifths plus the format's
-// extremes; divisions truncate toward zero, as C's do.
-__device__ __forceinline__ int bc_channel_value(uint64_t blk, int t, bool isSigned) {
-  const int c0 = isSigned ? (int)(int8_t)(blk & 0xFF) : (int)(blk & 0xFF);
-  const int c1 = isSigned ? (int)(int8_t)((blk >> 8) & 0xFF) : (int)((blk >> 8) & 0xFF);
-  const int i = (int)((blk >> (16 + 3 * t)) & 7);
-  if (i == 0) return c0;
-  if (i == 1) return c1;
-  if (c0 > c1) return ((8 - i) * c0 + (i - 1) * c1) / 7;
-  if (i < 6) return ((6 - i) * c0 + (i - 1) * c1) / 5;
-  return i == 6 ? (isSigned ? -128 : 0) : (isSigned ? 127 : 255);
-}
-
-__device__ __forceinline__ uint32_t bc_color_value(uint64_t blk, int t, bool hasAlpha, bool separateAlpha) {
-  const uint32_t c0 = (uint32_t)(blk & 0xFFFF), c1 = (uint32_t)((blk >> 16) & 0xFFFF);
-  const int i = (int)((blk >> (32 + 2 * t)) & 3);
-  const bool four = separateAlpha || c0 > c1;
-  uint32_t out = 0;
-#pragma unroll
-  for (int ch = 0; ch < 3; ch++) {
-    int e0, e1;
-    if (ch == 0) {
-      e0 = (int)(((c0 & 0x1Fu) << 3) | ((c0 & 0x1Cu) >> 2));
-      e1 = (int)(((c1 & 0x1Fu) << 3) | ((c1 & 0x1Cu) >> 2));
-    } else if (ch == 1) {
-      e0 = (int)(((c0 & 0x7E0u) >> 3) | ((c0 & 0x600u) >> 9));
-      e1 = (int)(((c1 & 0x7E0u) >> 3) | ((c1 & 0x600u) >> 9));
-    } else {
-      e0 = (int)(((c0 & 0xF800u) >> 8) | ((c0 & 0xE000u) >> 13));
-      e1 = (int)(((c1 & 0xF800u) >> 8) | ((c1 & 0xE000u) >> 13));
-    }
-    int v;
-    if (i == 0) v = e0;
-    else if (i == 1) v = e1;
-    else if (i == 2) v = four ? (2 * e0 + e1) / 3 : (e0 + e1) >> 1;
-    else v = four ? (2 * e1 + e0) / 3 : 0;
-    out |= (uint32_t)v << (8 * ch);
-  }
-  const bool transparent = i == 3 && !four && hasAlpha;
-  return out | (transparent ? 0u : 0xFF000000u);
-}
-
-__global__ void decode_bc_kernel(const uint8_t *__restrict__ src, uint8_t *__restrict__ dst, int w, int h, int pitch,
-                                 int bpp, int n, int noAlphaU) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-  if (x >= w || y >= h) return;
-  const int bw = (w + 3) >> 2;
-  const int t = (y & 3) * 4 + (x & 3);
-  const long long b = (long long)(y >> 2) * bw + (x >> 2);
-  uint8_t *px = dst + (long long)y * pitch + (long long)x * bpp;
-  if (n == 1 || n == 4) {
-    const uint64_t blk = reinterpret_cast<const uint64_t *>(src)[b];
-    if (n == 1) *reinterpret_cast<uint32_t *>(px) = bc_color_value(blk, t, !noAlphaU, false);
-    else px[0] = (uint8_t)bc_channel_value(blk, t, !noAlphaU);
-    return;
-  }
-  const uint64_t lo = reinterpret_cast<const uint64_t *>(src)[2 * b], hi = reinterpret_cast<const uint64_t *>(src)[2 * b + 1];
-  if (n == 5) {
-    px[0] = (uint8_t)bc_channel_value(lo, t, !noAlphaU);
-    px[1] = (uint8_t)bc_channel_value(hi, t, !noAlphaU);
-    return;
-  }
-  uint32_t v = bc_color_value(hi, t, false, true);
-  const uint32_t a = n == 2 ? (uint32_t)((lo >> (4 * t)) & 0xF) * 0x11u : (uint32_t)(uint8_t)bc_channel_value(lo, t, false);
-  *reinterpret_cast<uint32_t *>(px) = (v & 0x00FFFFFFu) | (a << 24);
 }
 
 // 4x resolve of an sRGB attachment: the reference's blit routine (Blitter.cpp

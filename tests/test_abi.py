@@ -54,7 +54,7 @@ def test_struct_layout_against_the_header(tmp_path):
         pytest.skip("gcc not available")
     structs = [("swr_draw_desc", abi.DrawDesc), ("swr_grad_desc", abi.GradDesc), ("swr_texture", abi.Texture),
                ("swr_resolve_desc", abi.ResolveDesc), ("swr_profile", abi.Profile), ("swr_clear_desc", abi.ClearDesc),
-               ("swr_bc_desc", abi.BCDesc)]
+               ("swr_bc_desc", abi.BCDesc), ("swr_etc_desc", abi.ETCDesc)]
     src = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
     want = []
     for cname, py in structs:

@@ -1,6 +1,8 @@
 """swr_decode_bc on the GPU against the oracle, bit for bit: BC1-BC5 in
-every alpha / sign mode, ragged and large images, pixel strides that leave
-bytes untouched; and the refused cases (BC6H / BC7, misaligned BC1-BC3)."""
+every alpha / sign mode, BC7 in every mode (and the invalid one) and BC6H in
+every mode (and the reserved ones), signed and unsigned; ragged and large
+images, pixel strides that leave bytes untouched; and the refused cases
+(misaligned or wrongly sized destinations)."""
 import ctypes
 
 import numpy as np
@@ -40,12 +42,62 @@ def test_decode_matches_the_oracle(n, flag):
             assert rc == 0 and np.array_equal(got, want), (w, h, bpp)
 
 
+BC6_MODES = [0, 1, 2, 6, 10, 14, 18, 22, 26, 30, 3, 7, 11, 15, 19, 23, 27, 31]  # the last four are reserved
+
+
+def _bptc_blocks(rng, nb, n, mode):
+    """Random BC7 blocks of mode `mode` (8: the invalid all-zero mode byte)
+    or BC6H blocks whose mode bits read `mode`."""
+    b = rng.integers(0, 256, (nb, 16), dtype=np.uint8)
+    if n == 7:
+        b[:, 0] = 0 if mode == 8 else (b[:, 0] & np.uint8((0xFF << (mode + 1)) & 0xFF)) | np.uint8(1 << mode)
+    elif mode in (0, 1):
+        b[:, 0] = (b[:, 0] & np.uint8(0xFC)) | np.uint8(mode)
+    else:
+        b[:, 0] = (b[:, 0] & np.uint8(0xE0)) | np.uint8(mode)
+    return b.ravel()
+
+
+@pytest.mark.parametrize("mode", range(9))
+def test_bc7_matches_the_oracle(mode):
+    rng = np.random.default_rng(900 + mode)
+    for w, h in SHAPES:
+        print(f"[gpu bc] BC7 mode {mode} {w}x{h}", flush=True)
+        nb = ((w + 3) // 4) * ((h + 3) // 4)
+        src = _bptc_blocks(rng, nb, 7, mode)
+        pitch = w * 4 + 8
+        want = np.full(pitch * h, 0x5A, np.uint8)
+        assert oracle_decode_bc(src, want, w, h, pitch, 4, 7, 0) == 1
+        rc, got = _gpu_decode(src, w, h, 4, 7, 0, pitch, 0x5A)
+        assert rc == 0 and np.array_equal(got, want), (mode, w, h)
+
+
+@pytest.mark.parametrize("mode", BC6_MODES)
+@pytest.mark.parametrize("signed", [0, 1])
+def test_bc6h_matches_the_oracle(mode, signed):
+    rng = np.random.default_rng(1000 + mode + 100 * signed)
+    flag = 0 if signed else 1
+    for w, h in SHAPES:
+        print(f"[gpu bc] BC6H mode {mode} signed {signed} {w}x{h}", flush=True)
+        nb = ((w + 3) // 4) * ((h + 3) // 4)
+        src = _bptc_blocks(rng, nb, 6, mode)
+        pitch = w * 8 + 6
+        want = np.full(pitch * h, 0x5A, np.uint8)
+        assert oracle_decode_bc(src, want, w, h, pitch, 8, 6, flag) == 1
+        rc, got = _gpu_decode(src, w, h, 8, 6, flag, pitch, 0x5A)
+        assert rc == 0 and np.array_equal(got, want), (mode, signed, w, h)
+
+
 def test_refused_cases():
     src = np.zeros(64, np.uint8)
-    rc, _ = _gpu_decode(src, 4, 4, 4, 7, 0, 16, 0)
-    assert rc == abi.ERROR_UNSUPPORTED
-    rc, _ = _gpu_decode(src, 4, 4, 4, 6, 0, 16, 0)
-    assert rc == abi.ERROR_UNSUPPORTED
+    rc, _ = _gpu_decode(src, 4, 4, 4, 6, 0, 32, 0)  # BC6H writes 8-byte pixels
+    assert rc == abi.ERROR_INVALID_ARGUMENT
+    rc, _ = _gpu_decode(src, 4, 4, 8, 7, 0, 32, 0)  # BC7 writes 4-byte pixels
+    assert rc == abi.ERROR_INVALID_ARGUMENT
+    rc, _ = _gpu_decode(src, 4, 4, 8, 6, 0, 33, 0)  # odd BC6H pitch
+    assert rc == abi.ERROR_INVALID_ARGUMENT
+    rc, _ = _gpu_decode(src, 4, 4, 4, 8, 0, 16, 0)  # no BC8
+    assert rc == abi.ERROR_INVALID_ARGUMENT
     rc, _ = _gpu_decode(src, 4, 4, 4, 1, 0, 18, 0)  # pitch not a multiple of 4
     assert rc == abi.ERROR_INVALID_ARGUMENT
     rc, _ = _gpu_decode(src, 4, 4, 2, 3, 0, 16, 0)  # BC3 writes 4-byte pixels
