@@ -1187,6 +1187,22 @@ struct swr_batch_plan {
   Scratch gparams, gbase;
   std::vector<uint32_t> bwBase;
   bool hasGrads = false;
+  // Binned texel scatter, per segment (see BinArgs): whether it is used,
+  // its bucket count, record capacity and offsets into the plan's tables.
+  struct SegBins {
+    bool on = false;
+    uint32_t nBuckets = 0, cap = 0, groupOff = 0, bucketOff = 0, drawOff = 0;
+  };
+  std::vector<SegBins> bins;
+  Scratch binGroups, binBuckets, binDrawGroup;                  // tables of every segment
+  Scratch binRec, binKeyA, binKeyB, binValA, binValB, binHist;  // record slots and their sort (largest segment)
+  Scratch binStart, binEnd, binChunks, binChunkBase, binSums;   // per bucket (largest segment)
+  Scratch binCounters;                                          // slots reserved, scatter work counter
+  void releaseBins() {
+    for (Scratch *b : {&binGroups, &binBuckets, &binDrawGroup, &binRec, &binKeyA, &binKeyB, &binValA, &binValB,
+                       &binHist, &binStart, &binEnd, &binChunks, &binChunkBase, &binSums, &binCounters})
+      b->release();
+  }
 };
 
 namespace {
@@ -1348,6 +1364,108 @@ int swr_batch_plan_autotune(swr_batch_plan *plan, swr_tuning *best, void *stream
 namespace {
 
 // Uploads the gradient targets of a plan in its layout order.
+// Draws whose texel gradients scatter alike: the same gradient buffers and
+// the texture parameters the taps depend on (level sizes and pitches,
+// filter, mip mode, address modes).
+bool same_tap_target(const DrawParams &A, const GradParams &Ga, const DrawParams &B, const GradParams &Gb) {
+  const TexParams &a = A.tex, &b = B.tex;
+  if (a.filter != b.filter || a.mipmap != b.mipmap || a.addrU != b.addrU || a.addrV != b.addrV) return false;
+  for (int l = 0; l < kMipLevels; l++)
+    if (Ga.dTex[l] != Gb.dTex[l] || a.width[l] != b.width[l] || a.height[l] != b.height[l] ||
+        a.pitchP[l] != b.pitchP[l])
+      return false;
+  return true;
+}
+
+// Per segment: the texture groups of its draws, their bucket tables and the
+// record slots: one per sample of every draw with texture gradients (a
+// bound on its fragments) plus the last reservation of every wave the
+// gather can launch (a segment with more groups than kMaxBinGroups, or more
+// slots than 32-bit indices hold, keeps the texel tables).
+int plan_build_bins(swr_batch_plan *pl, const std::vector<GradParams> &G, int cus, hipStream_t s) {
+  pl->bins.assign(pl->L.segs.size(), swr_batch_plan::SegBins{});
+  std::vector<BinGroup> groups;
+  std::vector<BinBucket> buckets;
+  std::vector<int8_t> drawGroup;
+  uint64_t maxCap = 0, maxBuckets = 0;
+  for (size_t si = 0; si < pl->L.segs.size(); si++) {
+    const auto &sg = pl->L.segs[si];
+    auto &sb = pl->bins[si];
+    std::vector<int> rep;  // a draw (segment-relative) per group
+    std::vector<int8_t> dg(sg.count, -1);
+    uint64_t cap = 0;
+    bool ok = true;
+    for (int k = 0; k < sg.count && ok; k++) {
+      const int i = sg.first + k;
+      const DrawParams &P = pl->L.P[i];
+      if (!P.useUV || !G[i].dTex[0]) continue;
+      int gi = -1;
+      for (size_t j = 0; j < rep.size() && gi < 0; j++)
+        if (same_tap_target(pl->L.P[sg.first + rep[j]], G[sg.first + rep[j]], P, G[i])) gi = (int)j;
+      if (gi < 0) {
+        if ((int)rep.size() >= kMaxBinGroups) ok = false;
+        gi = (int)rep.size();
+        rep.push_back(k);
+      }
+      dg[k] = (int8_t)gi;
+      cap += (uint64_t)P.width * (uint64_t)P.height * (uint64_t)P.ms;
+    }
+    // Up to two gather launches (one per sample count) of cus * 2 workgroups of 4 waves.
+    cap += 2ull * (uint64_t)cus * 2u * 4u * kBinReserve;
+    if (!ok || rep.empty() || cap >= (1ull << 31)) continue;
+    sb.on = true;
+    sb.cap = (uint32_t)cap;
+    sb.groupOff = (uint32_t)groups.size();
+    sb.bucketOff = (uint32_t)buckets.size();
+    sb.drawOff = (uint32_t)drawGroup.size();
+    uint32_t nb = 0;
+    for (size_t j = 0; j < rep.size(); j++) {
+      const TexParams &T = pl->L.P[sg.first + rep[j]].tex;
+      BinGroup bg{};
+      bg.draw = rep[j];
+      for (int l = 0; l < kMipLevels; l++) {
+        const uint32_t tx = ((uint32_t)T.width[l] + kBinTile - 1) / kBinTile;
+        const uint32_t ty = ((uint32_t)T.height[l] + kBinTile - 1) / kBinTile;
+        bg.levelBase[l] = nb;
+        bg.tilesX[l] = tx;
+        for (uint32_t y = 0; y < ty; y++)
+          for (uint32_t x = 0; x < tx; x++)
+            buckets.push_back(BinBucket{(uint8_t)j, (uint8_t)l, (uint16_t)x, (uint16_t)y});
+        nb += tx * ty;
+      }
+      groups.push_back(bg);
+    }
+    sb.nBuckets = nb;
+    drawGroup.insert(drawGroup.end(), dg.begin(), dg.end());
+    maxCap = std::max<uint64_t>(maxCap, cap);
+    maxBuckets = std::max<uint64_t>(maxBuckets, nb);
+  }
+  if (groups.empty()) return SWR_OK;
+  SWR_HIP_CHECK(pl->binGroups.ensure(sizeof(BinGroup) * groups.size()));
+  SWR_HIP_CHECK(pl->binBuckets.ensure(sizeof(BinBucket) * buckets.size()));
+  SWR_HIP_CHECK(pl->binDrawGroup.ensure(drawGroup.size()));
+  SWR_HIP_CHECK(pl->binRec.ensure(sizeof(TapRecord) * maxCap));
+  for (Scratch *b : {&pl->binKeyA, &pl->binKeyB, &pl->binValA, &pl->binValB})
+    SWR_HIP_CHECK(b->ensure(sizeof(uint32_t) * maxCap));
+  const size_t nbSort = (maxCap + kSortItems - 1) / kSortItems;
+  SWR_HIP_CHECK(pl->binHist.ensure(sizeof(uint32_t) * 256 * nbSort));
+  const size_t nb1 = maxBuckets + 1;
+  SWR_HIP_CHECK(pl->binStart.ensure(sizeof(uint32_t) * nb1));
+  SWR_HIP_CHECK(pl->binEnd.ensure(sizeof(uint32_t) * nb1));
+  SWR_HIP_CHECK(pl->binChunks.ensure(sizeof(uint32_t) * nb1));
+  SWR_HIP_CHECK(pl->binChunkBase.ensure(sizeof(uint32_t) * nb1));
+  const size_t scanItems = std::max(nb1, 256 * nbSort);
+  SWR_HIP_CHECK(pl->binSums.ensure(sizeof(uint32_t) * ((scanItems + kScanItems - 1) / kScanItems + 1)));
+  SWR_HIP_CHECK(pl->binCounters.ensure(sizeof(uint32_t) * 2));
+  SWR_HIP_CHECK(hipMemcpyAsync(pl->binGroups.ptr, groups.data(), sizeof(BinGroup) * groups.size(),
+                               hipMemcpyHostToDevice, s));
+  SWR_HIP_CHECK(hipMemcpyAsync(pl->binBuckets.ptr, buckets.data(), sizeof(BinBucket) * buckets.size(),
+                               hipMemcpyHostToDevice, s));
+  SWR_HIP_CHECK(hipMemcpyAsync(pl->binDrawGroup.ptr, drawGroup.data(), drawGroup.size(), hipMemcpyHostToDevice, s));
+  SWR_HIP_CHECK(hipStreamSynchronize(s));
+  return SWR_OK;
+}
+
 int plan_upload_grads(DeviceCtx *c, swr_batch_plan *pl, hipStream_t s) {
   const size_t n = pl->L.P.size();
   std::vector<GradParams> G(n);
@@ -1370,19 +1488,32 @@ int plan_upload_grads(DeviceCtx *c, swr_batch_plan *pl, hipStream_t s) {
   SWR_HIP_CHECK(hipMemcpyAsync(pl->gparams.ptr, G.data(), sizeof(GradParams) * n, hipMemcpyHostToDevice, s));
   SWR_HIP_CHECK(hipMemcpyAsync(pl->gbase.ptr, pl->bwBase.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
   SWR_HIP_CHECK(hipStreamSynchronize(s));
-  (void)c;
-  return SWR_OK;
+  return plan_build_bins(pl, G, c->cus, s);
 }
 
 int plan_enqueue_backward(swr_batch_plan *pl, int cus, hipStream_t s) {
   Pipeline &pipe = pl->pipe;
-  for (const auto &sg : pl->L.segs) {
+  for (size_t si = 0; si < pl->L.segs.size(); si++) {
+    const auto &sg = pl->L.segs[si];
+    const auto &sb = pl->bins.size() > si ? pl->bins[si] : swr_batch_plan::SegBins{};
     const DrawParams *params = pipe.params.as<DrawParams>() + sg.first;
     const uint32_t *blockBase = pipe.aux.as<uint32_t>() + sg.first;
     hipLaunchKernelGGL(setup_batch_kernel, dim3(sg.blocks), dim3(256), 0, s, params, blockBase, sg.count,
                        pipe.rec.as<TriRecord>(), pipe.poly.as<int>(), pipe.cnt.as<uint32_t>());
     const GradParams *grads = pl->gparams.as<GradParams>() + sg.first;
     const uint32_t *bb = pl->gbase.as<uint32_t>() + sg.first;
+    BinArgs bin{};
+    uint32_t *binCtr = pl->binCounters.as<uint32_t>();
+    if (sb.on) {
+      bin.rec = pl->binRec.as<TapRecord>();
+      bin.key = pl->binKeyA.as<uint32_t>();
+      bin.val = pl->binValA.as<uint32_t>();
+      bin.reserved = binCtr;
+      bin.cap = sb.cap;
+      bin.drawGroup = pl->binDrawGroup.as<int8_t>() + sb.drawOff;
+      bin.groups = pl->binGroups.as<BinGroup>() + sb.groupOff;
+      hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, binCtr, 0u, 2u);
+    }
     // Draws of one sample count are contiguous (the layout groups variants);
     // each group gets its own work counter (counters[20 + group]).
     uint32_t *ctr = pipe.counters.as<uint32_t>() + 20;
@@ -1398,15 +1529,56 @@ int plan_enqueue_backward(swr_batch_plan *pl, int cus, hipStream_t s) {
       const uint32_t bHi = pl->bwBase[sg.first + hi - 1] + (uint32_t)(((Ph.width + 15) / 16) * ((Ph.height + 15) / 16));
       if (bHi > bLo) {
         const uint32_t wgs = std::min<uint32_t>(bHi - bLo, (uint32_t)cus * 2u);
-        if (ms == 1)
-          hipLaunchKernelGGL(backward_batch_kernel<1>, dim3(wgs), dim3(256), 0, s, params, grads, bb, lo, hi, bLo, bHi,
-                             ctr + group, (const TriRecord *)pipe.rec.as<TriRecord>());
+        const TriRecord *rec = pipe.rec.as<TriRecord>();
+        if (ms == 1 && sb.on)
+          hipLaunchKernelGGL((backward_batch_kernel<1, true>), dim3(wgs), dim3(256), 0, s, params, grads, bb, lo, hi,
+                             bLo, bHi, ctr + group, rec, bin);
+        else if (ms == 1)
+          hipLaunchKernelGGL((backward_batch_kernel<1, false>), dim3(wgs), dim3(256), 0, s, params, grads, bb, lo, hi,
+                             bLo, bHi, ctr + group, rec, bin);
+        else if (sb.on)
+          hipLaunchKernelGGL((backward_batch_kernel<4, true>), dim3(wgs), dim3(256), 0, s, params, grads, bb, lo, hi,
+                             bLo, bHi, ctr + group, rec, bin);
         else
-          hipLaunchKernelGGL(backward_batch_kernel<4>, dim3(wgs), dim3(256), 0, s, params, grads, bb, lo, hi, bLo, bHi,
-                             ctr + group, (const TriRecord *)pipe.rec.as<TriRecord>());
+          hipLaunchKernelGGL((backward_batch_kernel<4, false>), dim3(wgs), dim3(256), 0, s, params, grads, bb, lo, hi,
+                             bLo, bHi, ctr + group, rec, bin);
       }
       group++;
       lo = hi;
+    }
+    if (sb.on) {
+      // The reserved slots sorted by bucket (stable radix sort of (key,
+      // slot); unused slots carry a key above every bucket and sort last),
+      // each bucket's range of the sorted slots, the work items, the scatter.
+      const uint32_t nbSort = (sb.cap + kSortItems - 1) / kSortItems;
+      uint32_t *kIn = pl->binKeyA.as<uint32_t>(), *vIn = pl->binValA.as<uint32_t>();
+      uint32_t *kOut = pl->binKeyB.as<uint32_t>(), *vOut = pl->binValB.as<uint32_t>();
+      uint32_t *hist = pl->binHist.as<uint32_t>(), *sums = pl->binSums.as<uint32_t>();
+      const uint32_t *count = binCtr;
+      for (uint32_t p = 0; p < sort_passes(sb.nBuckets); p++) {
+        hipLaunchKernelGGL(radix_hist_kernel, dim3(nbSort), dim3(256), 0, s, (const uint32_t *)kIn, count, (int)(8 * p),
+                           hist, nbSort);
+        enqueue_scan(hist, hist, (size_t)256 * nbSort, sums, s);
+        hipLaunchKernelGGL(radix_scatter_kernel, dim3(nbSort), dim3(256), 0, s, (const uint32_t *)kIn,
+                           (const uint32_t *)vIn, count, (int)(8 * p), (const uint32_t *)hist, nbSort, kOut, vOut);
+        std::swap(kIn, kOut);
+        std::swap(vIn, vOut);
+      }
+      const uint32_t nb1 = sb.nBuckets + 1;
+      uint32_t *start = pl->binStart.as<uint32_t>(), *end = pl->binEnd.as<uint32_t>();
+      uint32_t *chunks = pl->binChunks.as<uint32_t>(), *chunkBase = pl->binChunkBase.as<uint32_t>();
+      hipLaunchKernelGGL(fill_u32_kernel, dim3((nb1 + 255) / 256), dim3(256), 0, s, start, 0u, nb1);
+      hipLaunchKernelGGL(fill_u32_kernel, dim3((nb1 + 255) / 256), dim3(256), 0, s, end, 0u, nb1);
+      hipLaunchKernelGGL(tile_range_batch_kernel, dim3((sb.cap + 255) / 256), dim3(256), 0, s, (const uint32_t *)kIn,
+                         count, sb.nBuckets, start, end);
+      hipLaunchKernelGGL(tap_chunks_kernel, dim3((nb1 + 255) / 256), dim3(256), 0, s, (const uint32_t *)start,
+                         (const uint32_t *)end, sb.nBuckets, chunks);
+      enqueue_scan(chunks, chunkBase, nb1, sums, s);
+      hipLaunchKernelGGL(tap_scatter_kernel, dim3((uint32_t)cus), dim3(kScatterThreads), 0, s,
+                         (const TapRecord *)bin.rec, (const uint32_t *)vIn, (const uint32_t *)start,
+                         (const uint32_t *)end, (const uint32_t *)chunkBase, sb.nBuckets,
+                         (const BinBucket *)pl->binBuckets.as<BinBucket>() + sb.bucketOff, bin.groups, params, grads,
+                         binCtr + 1);
     }
   }
   SWR_HIP_CHECK(hipGetLastError());
@@ -1447,6 +1619,7 @@ int swr_batch_plan_destroy(swr_batch_plan *plan) {
   plan->flags.release();
   plan->gparams.release();
   plan->gbase.release();
+  plan->releaseBins();
   delete plan;
   return SWR_OK;
 }

@@ -17,6 +17,8 @@ Two ways to run the same kernels:
             one HIP graph and replayed.
 Everything stays on the device (Adam's step count is a device tensor), so a
 captured step needs no host value; the loss is computed on request.
+save() / load() checkpoint the texture and Adam's state, so that a fit can
+stop and resume where it was (also under a captured graph).
 """
 import ctypes
 from dataclasses import dataclass
@@ -249,3 +251,30 @@ class MultiViewFit:
 
     def texture(self) -> torch.Tensor:
         return self.tex0
+
+    # ---- checkpoint / resume -------------------------------------------------
+    def state_dict(self) -> dict:
+        """The fitting state: the texture (level 0), Adam's moments and its
+        step count, and the learning rate (host copies)."""
+        return {"texture": self.tex0.detach().cpu().clone(), "adam_m": self.m.detach().cpu().clone(),
+                "adam_v": self.v.detach().cpu().clone(), "adam_t": self.t.detach().cpu().clone(), "lr": self.lr}
+
+    def load_state_dict(self, state: dict):
+        """Restores a state_dict() in place (the tensors a captured graph
+        reads keep their addresses, so the graph stays valid)."""
+        if tuple(state["texture"].shape) != tuple(self.tex0.shape):
+            raise ValueError(f"checkpoint texture {tuple(state['texture'].shape)} != {tuple(self.tex0.shape)}")
+        self.tex0.copy_(state["texture"].to(self.device))
+        self.m.copy_(state["adam_m"].to(self.device))
+        self.v.copy_(state["adam_v"].to(self.device))
+        self.t.copy_(state["adam_t"].to(self.device))
+        self.lr = float(state["lr"])
+
+    def save(self, path: str):
+        """Writes state_dict() to `path` (torch.save of tensors and a float)."""
+        torch.save(self.state_dict(), path)
+
+    def load(self, path: str):
+        """Restores a checkpoint written by save() (tensors only: loaded with
+        weights_only=True)."""
+        self.load_state_dict(torch.load(path, map_location="cpu", weights_only=True))

@@ -39,10 +39,17 @@ def params_of(d: DrawData) -> dict:
 
 
 def float_scene(seed, shader=abi.SHADER_MODULATE, samples=1, aniso=False, address=abi.ADDRESS_REPEAT, size=48,
-                triangles=40, tex=32, levels=None, uv_hi=2.0, area_hi=300.0):
-    """A small RGBA32F draw with a float texture: the differentiable form."""
+                triangles=40, tex=32, levels=None, uv_hi=2.0, area_hi=300.0, singular_uv=False):
+    """A small RGBA32F draw with a float texture: the differentiable form.
+    singular_uv gives every third triangle v = u at its vertices: on a square
+    texture its UV Jacobian is exactly singular (det = -0), so the
+    anisotropic probe count rounds to INT_MIN and the probe loop runs once."""
     rng = np.random.default_rng(seed)
     pos, uv, col = scenes.screen_triangles(rng, triangles, size, size, area_lo=20.0, area_hi=area_hi, uv_hi=uv_hi)
+    if singular_uv:
+        uv = uv.reshape(-1, 3, 2).copy()
+        uv[::3, :, 1] = uv[::3, :, 0]
+        uv = uv.reshape(-1, 2)
     chain = scenes.float_texture(rng, tex)
     if levels:
         chain = chain[:levels]

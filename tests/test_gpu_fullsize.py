@@ -9,7 +9,10 @@
 * the 16 differentiable 512^2 views (seed 4) and the 64 uneven 4x MSAA,
   8x anisotropic views (seed 5) through the captured batched fitting step:
   colour, depth and the primitive_id record of every checked view equal the
-  oracle's, and the resolved images equal the oracle's resolve.
+  oracle's, and the resolved images equal the oracle's resolve;
+* the batched backward of the 16 differentiable views equals the per-view
+  backward, and for both whole workloads every mip level's texel gradient
+  (and the vertex gradients) is within 1e-4 of the FP64 oracle.
 
 The oracle runs on several host threads (swo_draw_mt, which renders the
 same bytes as swo_draw; tests/test_oracle_mt.py) or several draws at once.
@@ -155,3 +158,117 @@ def test_msaa_aniso_64_batched_forward():
     order = sorted(range(len(draws)), key=lambda k: draws[k].width * draws[k].height)
     picks = sorted(set(list(range(0, len(draws), 6)) + order[-2:]))
     _fit_views_against_oracle(draws, picks, "msaa_aniso")
+
+
+def _grad_arrays(fit):
+    out = [l.detach().cpu().numpy().copy() for l in fit.d_levels]
+    if fit.geometry_grads:
+        for vb in fit.views:
+            out += [vb.d_pos.cpu().numpy().copy(), vb.d_col.cpu().numpy().copy(), vb.d_uv.cpu().numpy().copy()]
+    return out
+
+
+def test_batched_backward_full_size_equals_per_view():
+    """A fitting step of the 16 differentiable views with the batch plan's
+    backward -- the texel scatter binned by texture tile across all views --
+    against the same step with swr_draw_backward per view: every gradient
+    array (texture levels after the mip adjoint, vertex positions, colours,
+    texcoords) within 1e-5."""
+    from grad_cases import gradient_rel_error
+    draws = scenes.diff_views(seed=4)
+    rng = np.random.default_rng(90)
+    targets = [torch.from_numpy(rng.normal(0.5, 0.2, (d.height, d.width, 4)).astype(np.float32)) for d in draws]
+    fit = texfit.MultiViewFit(draws, torch.from_numpy(draws[0].texture.levels[0]), targets, geometry_grads=True,
+                              device=DEV)
+    print("[fullsize] backward diff_views_16: per view", flush=True)
+    fit.step_plain(update=False)
+    torch.cuda.synchronize()
+    want = _grad_arrays(fit)
+    print("[fullsize] backward diff_views_16: batched", flush=True)
+    fit.step_batched(update=False)
+    torch.cuda.synchronize()
+    errs = [gradient_rel_error(b, a) for a, b in zip(want, _grad_arrays(fit))]
+    print(f"[fullsize] backward diff_views_16: max relative error {max(errs):.2e} over {len(errs)} arrays", flush=True)
+    assert max(errs) < 1e-5
+
+
+@pytest.mark.parametrize("which", ["diff_views_16", "msaa_aniso_64"])
+def test_batched_backward_against_fp64_oracle(which):
+    """The batch plan's backward of a whole fitting workload -- the 16
+    differentiable 512^2 views with vertex gradients, or the 64 uneven 4x
+    MSAA 8x-anisotropic views -- against the FP64 oracle (oracle/grad_fp64.py)
+    summed over the views, evaluated on the oracle's own visibility and
+    sampling state (the GPU's primitive ids are checked equal to the
+    oracle's): the texel gradient of every mip level, and each view's vertex
+    gradients, within 1e-4.  The per-view path's errors on the same upstream
+    gradients are printed beside them."""
+    import ctypes
+    from concurrent.futures import ThreadPoolExecutor
+    import threading
+    from grad_cases import G, gradient_rel_error, params_of, view_of
+    from oracle_runner import oracle_render_state
+    from swiftshader_amd import diff, runtime
+    geo = which == "diff_views_16"
+    draws = scenes.diff_views(seed=4) if geo else scenes.msaa_aniso(seed=5)
+    fit = texfit.MultiViewFit(draws, torch.from_numpy(draws[0].texture.levels[0]), geometry_grads=geo, device=DEV)
+    fit.make_plan()
+    rng = np.random.default_rng(91)
+    d_colors = [rng.normal(size=(d.samples, d.height, d.width, 4)).astype(np.float32) for d in draws]
+    diff.generate_mips(fit.levels)
+    fit._clear()
+    fit.plan.run()
+    for vb, dc in zip(fit.views, d_colors):
+        vb.d_color.copy_(torch.from_numpy(dc))
+    fit._zero_grads()
+    print(f"[fullsize] fp64 {which}: batched backward of {len(draws)} views", flush=True)
+    fit.plan.backward()
+    torch.cuda.synchronize()
+
+    def snapshot():
+        lv = [l.double().cpu().numpy() for l in fit.d_levels]
+        vx = [(vb.d_pos.cpu().numpy(), vb.d_col.cpu().numpy(), vb.d_uv.cpu().numpy()) for vb in fit.views] if geo \
+            else None
+        return lv, vx
+
+    got_lv, got_vx = snapshot()
+    ids_gpu = [vb.ids.cpu().numpy().view(np.uint32) for vb in fit.views]
+    fit._zero_grads()
+    for vb in fit.views:
+        g = fit._grad(vb)
+        runtime.check(runtime.lib().swr_draw_backward(ctypes.byref(vb.desc), ctypes.byref(g),
+                                                      runtime._stream_handle(None)), "swr_draw_backward")
+    torch.cuda.synchronize()
+    pv_lv, _ = snapshot()
+
+    want_lv = None
+    vx_errs = []
+    lock = threading.Lock()
+
+    def one(k):
+        nonlocal want_lv
+        d = draws[k]
+        _, _, ids, state, _ = oracle_render_state(d, threads=2)
+        assert np.array_equal(ids, ids_gpu[k]), f"view {k}: primitive ids differ from the oracle's"
+        frag = G.fragments(ids, state)
+        out = G.gradient(view_of(d), params_of(d), frag, G.upstream(frag, d_colors[k]))
+        with lock:
+            if want_lv is None:
+                want_lv = [np.array(l) for l in out["texture_levels"]]
+            else:
+                for acc, l in zip(want_lv, out["texture_levels"]):
+                    acc += l
+            if geo:
+                for a, key in zip(got_vx[k], ("positions", "colors", "texcoords")):
+                    vx_errs.append(gradient_rel_error(a, out[key]))
+        print(f"[fullsize] fp64 {which}: view {k} done", flush=True)
+
+    with ThreadPoolExecutor(max(1, min(8, host_threads() // 2))) as ex:
+        list(ex.map(one, range(len(draws))))
+    errs = [gradient_rel_error(g, w) for g, w in zip(got_lv, want_lv) if np.abs(w).max() > 0]
+    errs_pv = [gradient_rel_error(g, w) for g, w in zip(pv_lv, want_lv) if np.abs(w).max() > 0]
+    print(f"[fullsize] fp64 {which}: batched, per level " + " ".join(f"{e:.1e}" for e in errs), flush=True)
+    print(f"[fullsize] fp64 {which}: per view, per level " + " ".join(f"{e:.1e}" for e in errs_pv), flush=True)
+    if geo:
+        print(f"[fullsize] fp64 {which}: vertex gradients max {max(vx_errs):.2e}", flush=True)
+        assert max(vx_errs) < 1e-4
+    assert max(errs) < 1e-4

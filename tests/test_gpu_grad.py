@@ -105,7 +105,8 @@ def run_backward(d, seed):
 
 
 BWD_CASES = [dict(), dict(samples=4), dict(aniso=True), dict(aniso=True, address=abi.ADDRESS_CLAMP_TO_EDGE, samples=4),
-             dict(shader=abi.SHADER_SAMPLE, address=abi.ADDRESS_CLAMP_TO_EDGE), dict(shader=abi.SHADER_INTERPOLATE)]
+             dict(shader=abi.SHADER_SAMPLE, address=abi.ADDRESS_CLAMP_TO_EDGE), dict(shader=abi.SHADER_INTERPOLATE),
+             dict(aniso=True, singular_uv=True), dict(aniso=True, singular_uv=True, samples=4)]
 
 
 @pytest.mark.parametrize("case", range(len(BWD_CASES)))
