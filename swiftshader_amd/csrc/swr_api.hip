@@ -360,6 +360,13 @@ int build_params(const swr_draw_desc *d, const swr_tuning &tuning, DrawParams &P
   P.depthBoundsMin = d->min_depth_bounds;
   P.depthBoundsMax = d->max_depth_bounds;
   P.depthCompare = (int)d->depth_compare_op;
+  {
+    // VkCompareOp as (predicate, inversion); NaN depths fail <=, < and ==.
+    static const int kPred[8] = {3, 0, 2, 1, 1, 2, 0, 3}, kInv[8] = {1, 1, 0, 1, 0, 1, 0, 0};
+    const uint32_t op = d->depth_compare_op < 8 ? d->depth_compare_op : 7u;
+    P.zPred = kPred[op];
+    P.zInv = kInv[op];
+  }
   // Depth bias state (Context.cpp:640-661: a NaN clamp is no clamp;
   // SetupProcessor.cpp:75-77 and PixelProcessor.cpp:119: each term only
   // when its factor is non-zero).
@@ -422,6 +429,11 @@ int build_params(const swr_draw_desc *d, const swr_tuning &tuning, DrawParams &P
     P.srcFA = bf(b.alpha_blend_op, b.src_alpha_blend_factor);
     P.dstFA = bf(b.alpha_blend_op, b.dst_alpha_blend_factor);
     P.alphaOp = aop;
+    P.blendKind = (P.srcF == SWR_BLEND_FACTOR_SRC_ALPHA && P.dstF == SWR_BLEND_FACTOR_ONE_MINUS_SRC_ALPHA &&
+                   P.colorOp == 0 && P.srcFA == SWR_BLEND_FACTOR_SRC_ALPHA &&
+                   P.dstFA == SWR_BLEND_FACTOR_ONE_MINUS_SRC_ALPHA && P.alphaOp == 0)
+                      ? 1
+                      : 0;
   }
   for (int k = 0; k < 4; k++) {
     float c = d->blend_constants[k];

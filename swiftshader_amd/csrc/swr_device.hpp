@@ -523,6 +523,10 @@ __device__ __noinline__ float4 advanced_blend(int op, float4 src, float4 dst) {
 template <bool XF>
 __device__ __forceinline__ float4 alpha_blend(const DrawParams &P, float4 s, float4 d) {
   if (XF && is_advanced_op(P.colorOp)) return advanced_blend(P.colorOp, s, d);
+  if (P.blendKind == 1) {  // the factors and operation below for SRC_ALPHA, ONE_MINUS_SRC_ALPHA, ADD
+    const float sa = s.w, k = 1.0f - sa;
+    return make_float4(s.x * sa + d.x * k, s.y * sa + d.y * k, s.z * sa + d.z * k, s.w * sa + d.w * k);
+  }
   float sfr = blend_factor_rgb(P, P.srcF, s.x, s.w, d.x, d.w, 0);
   float sfg = blend_factor_rgb(P, P.srcF, s.y, s.w, d.y, d.w, 1);
   float sfb = blend_factor_rgb(P, P.srcF, s.z, s.w, d.z, d.w, 2);

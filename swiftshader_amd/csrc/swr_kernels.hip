@@ -731,20 +731,20 @@ __device__ __forceinline__ void blend_write(const DrawParams &P, PixelState<MS, 
       o.y = linear_to_srgb(o.y);
       o.z = linear_to_srgb(o.z);
     }
+    // Masked channels keep their value (selects, not branches).
     if (FCOL) {
       float4 &c = S.cf[FCOL ? q : 0];
-      if (P.colorMask & 1) c.x = o.x;
-      if (P.colorMask & 2) c.y = o.y;
-      if (P.colorMask & 4) c.z = o.z;
-      if (P.colorMask & 8) c.w = o.w;
+      c.x = (P.colorMask & 1) ? o.x : c.x;
+      c.y = (P.colorMask & 2) ? o.y : c.y;
+      c.z = (P.colorMask & 4) ? o.z : c.z;
+      c.w = (P.colorMask & 8) ? o.w : c.w;
       S.id[FCOL ? q : 0] = tri - (uint32_t)P.triBase + 1u;
     } else {
-      uint32_t p = S.cb[FCOL ? 0 : q];
-      if (P.colorMask & 1) p = (p & 0xFFFFFF00u) | float_to_unorm8(o.x);
-      if (P.colorMask & 2) p = (p & 0xFFFF00FFu) | (float_to_unorm8(o.y) << 8);
-      if (P.colorMask & 4) p = (p & 0xFF00FFFFu) | (float_to_unorm8(o.z) << 16);
-      if (P.colorMask & 8) p = (p & 0x00FFFFFFu) | (float_to_unorm8(o.w) << 24);
-      S.cb[FCOL ? 0 : q] = p;
+      const uint32_t wm = ((P.colorMask & 1) ? 0x000000FFu : 0u) | ((P.colorMask & 2) ? 0x0000FF00u : 0u) |
+                          ((P.colorMask & 4) ? 0x00FF0000u : 0u) | ((P.colorMask & 8) ? 0xFF000000u : 0u);
+      const uint32_t o8 = float_to_unorm8(o.x) | (float_to_unorm8(o.y) << 8) | (float_to_unorm8(o.z) << 16) |
+                          (float_to_unorm8(o.w) << 24);
+      S.cb[FCOL ? 0 : q] = (S.cb[FCOL ? 0 : q] & ~wm) | (o8 & wm);
     }
   }
 }
@@ -979,18 +979,11 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
           if ((XF && P.depth16)) z = d16_value(z);
           zq[q] = z;
           const float zb = S.zb[q];
-          bool pass;
-          switch (P.depthCompare) {
-            case 0: pass = false; break;
-            case 1: pass = !(zb <= z); break;
-            case 2: pass = (zb == z); break;
-            case 3: pass = !(zb < z); break;
-            case 4: pass = (zb < z); break;
-            case 5: pass = !(zb == z); break;
-            case 6: pass = (zb <= z); break;
-            default: pass = true; break;
-          }
-          if (pass) zMask |= 1 << q;
+          // The compare op without branches: one of zb <= z, zb < z, zb == z
+          // or true, inverted for LESS, LESS_OR_EQUAL, NOT_EQUAL and NEVER.
+          const bool le = zb <= z, lt = zb < z, eq = zb == z;
+          const bool pr = P.zPred == 0 ? le : (P.zPred == 1 ? lt : (P.zPred == 2 ? eq : true));
+          if (pr != (P.zInv != 0)) zMask |= 1 << q;
         }
         zMask &= cMask;
         if (P.stencilActive) zMask &= sMask;

@@ -73,6 +73,9 @@ struct DrawParams {
   int ms, msMask;
   // Depth / stencil.
   int depthTest, depthWrite, depthCompare;
+  // The depth compare as a predicate of the stored (zb) and fragment (z)
+  // depths, inverted when zInv: 0 zb <= z, 1 zb < z, 2 zb == z, 3 true.
+  int zPred, zInv;
   int depth16;  // D16_UNORM depth attachment (else D32_SFLOAT)
   int depthClip;                    // clip against the near and far planes
   int depthBounds;                  // depth bounds test on the stored depth
@@ -93,6 +96,7 @@ struct DrawParams {
   int blendEnable;
   int advancedBlend;  // colorOp (= alphaOp) is a VK_EXT_blend_operation_advanced mode
   int srcF, dstF, colorOp, srcFA, dstFA, alphaOp;
+  int blendKind;  // 1: SRC_ALPHA, ONE_MINUS_SRC_ALPHA, ADD for colour and alpha
   float blendConst[4], invBlendConst[4];
   int shader, useColor, useUV;
   TexParams tex;
