@@ -581,10 +581,13 @@ __device__ bool fragment_backward(GradLds<BIN> &S, const DrawParams &P, const Gr
   const float4 gf = make_float4((float)g.x, (float)g.y, (float)g.z, (float)g.w);
   float4 dcol = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   float dU = 0.0f, dV = 0.0f;
+  const bool attrGrads = G.dPos || G.dCol || G.dUV;
   if (P.useUV) {
     const float4 gTex = P.useColor ? make_float4(gf.x * col.x, gf.y * col.y, gf.z * col.z, gf.w * col.w) : gf;
     float4 tex = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    texture_backward<BIN>(S, P.tex, G, L, u, v, gTex, tex, dU, dV);
+    // With the texel scatter binned, the sampled value and its u, v
+    // derivatives serve only the vertex-attribute gradients.
+    if (!BIN || attrGrads) texture_backward<BIN>(S, P.tex, G, L, u, v, gTex, tex, dU, dV);
     if constexpr (BIN) {
       if (G.dTex[0]) {
         rec.u = u;
@@ -598,6 +601,7 @@ __device__ bool fragment_backward(GradLds<BIN> &S, const DrawParams &P, const Gr
   } else {
     dcol = gf;
   }
+  if (!attrGrads) return has;
 
   // Attribute gradients and dL/db.
   float val[kTriVals];
@@ -643,7 +647,6 @@ __device__ bool fragment_backward(GradLds<BIN> &S, const DrawParams &P, const Gr
       val[3 * k + 2] = (float)(w[k] != 0.0 ? -dr * rhw[k] * rhw[k] : 0.0);
     }
   }
-  if (!G.dPos && !G.dCol && !G.dUV) return has;
   float *slot = tri_slot(S, t);
   if (!slot) {
     tri_scatter_global(P, G, t, val);
