@@ -59,12 +59,15 @@ class BatchPlan:
 
 def grad_desc(d_color, d_positions=None, d_colors=None, d_texcoords=None, d_texture: Optional[Sequence] = None,
               sample_state=None) -> abi.GradDesc:
-    """A swr_grad_desc over torch tensors: d_color (S, H, W, 4) float32."""
+    """A swr_grad_desc over torch tensors: d_color (S, H, W, 4) float32, rows
+    and pixels contiguous (the sample planes may lie apart)."""
     S, H, W, _ = d_color.shape
+    if d_color.stride()[1:] != (W * 4, 4, 1):
+        raise ValueError("d_color rows must be contiguous (S, H, W, 4) planes")
     g = abi.GradDesc()
     g.d_color = d_color.data_ptr()
     g.d_color_row_pitch_bytes = W * 16
-    g.d_color_sample_pitch_bytes = H * W * 16
+    g.d_color_sample_pitch_bytes = d_color.stride(0) * 4 if S > 1 else H * W * 16
     g.d_positions = d_positions.data_ptr() if d_positions is not None else None
     g.d_colors = d_colors.data_ptr() if d_colors is not None else None
     g.d_texcoords = d_texcoords.data_ptr() if d_texcoords is not None else None

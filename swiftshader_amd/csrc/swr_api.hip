@@ -617,13 +617,25 @@ uint64_t image_bytes(uint32_t w, uint32_t h, uint32_t samples, uint64_t rowPitch
   return (uint64_t)(samples - 1) * slice + (uint64_t)(h - 1) * rowPitch + (uint64_t)w * bpp;
 }
 
+// An attachment's sample planes: one range each when they are apart (sample
+// planes of several images may interleave), else one range for them all.
+void add_image(std::vector<std::pair<uintptr_t, uintptr_t>> &v, const void *p, uint32_t w, uint32_t h,
+               uint32_t samples, uint64_t rowPitch, uint64_t slice, uint32_t bpp) {
+  const uint64_t plane = image_bytes(w, h, 1, rowPitch, 0, bpp);
+  if (samples > 1 && slice > plane) {
+    for (uint32_t q = 0; q < samples; q++) add_range(v, (const char *)p + q * slice, plane);
+  } else {
+    add_range(v, p, image_bytes(w, h, samples, rowPitch, slice, bpp));
+  }
+}
+
 Extents extents_of(const swr_draw_desc *d, const DrawParams &P) {
   Extents e;
   const uint32_t W = d->framebuffer_width, H = d->framebuffer_height, S = d->samples;
-  if (P.color) add_range(e.writes, P.color, image_bytes(W, H, S, P.colorPitch, P.colorSlice, P.colorFloat ? 16 : 4));
-  if (P.depth) add_range(e.writes, P.depth, image_bytes(W, H, S, P.depthPitch, P.depthSlice, P.depth16 ? 2 : 4));
-  if (P.stencil) add_range(e.writes, P.stencil, image_bytes(W, H, S, P.stencilPitch, P.stencilSlice, 1));
-  if (P.primId) add_range(e.writes, P.primId, image_bytes(W, H, S, P.primIdPitch, P.primIdSlice, 4));
+  if (P.color) add_image(e.writes, P.color, W, H, S, P.colorPitch, P.colorSlice, P.colorFloat ? 16 : 4);
+  if (P.depth) add_image(e.writes, P.depth, W, H, S, P.depthPitch, P.depthSlice, P.depth16 ? 2 : 4);
+  if (P.stencil) add_image(e.writes, P.stencil, W, H, S, P.stencilPitch, P.stencilSlice, 1);
+  if (P.primId) add_image(e.writes, P.primId, W, H, S, P.primIdPitch, P.primIdSlice, 4);
   const uint64_t V = (uint64_t)d->vertex_count;
   add_range(e.reads, P.pos, V * 16);
   add_range(e.reads, P.col, V * 16);

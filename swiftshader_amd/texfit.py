@@ -43,6 +43,22 @@ def _flat(shapes, dtype, device):
     return flat, out
 
 
+def _planes(samples: int, shapes, channels: int, dtype, device, align: int = 4096):
+    """A zeroed (samples, N) tensor and, per (H, W) shape, a (samples, H, W[,
+    channels]) view of it: every view's sample plane q lies in row q, so an
+    elementwise pass over sample q of all views is one launch.  N is the
+    total pixel count rounded up to `align` pixels, times channels."""
+    pix = [H * W for H, W in shapes]
+    total = -(-sum(pix) // align) * align
+    flat = torch.zeros((samples, total * channels), dtype=dtype, device=device)
+    out, off = [], 0
+    for (H, W), n in zip(shapes, pix):
+        v = flat[:, off * channels:(off + n) * channels]
+        out.append(v.view(samples, H, W, channels) if channels > 1 else v.view(samples, H, W))
+        off += n
+    return flat, out, total
+
+
 @dataclass
 class _ViewBuffers:
     draw: DrawData
@@ -85,12 +101,26 @@ class MultiViewFit:
         dev = self.device
         samples = [(d.samples, d.height, d.width) for d in draws]
         self.single_sample = all(d.samples == 1 for d in draws)
-        self._color_flat, colors = _flat([(S, H, W, 4) for S, H, W in samples], torch.float32, dev)
-        self._depth_flat, depths = _flat(samples, torch.float32, dev)
-        self._ids_flat, ids = _flat(samples, torch.int32, dev)
-        self._dcolor_flat, dcolors = _flat([(S, H, W, 4) for S, H, W in samples], torch.float32, dev)
-        self._resolved_flat, resolved = _flat([(1, H, W, 4) for _, H, W in samples], torch.float32, dev)
-        self._target_flat, tgts = _flat([(1, H, W, 4) for _, H, W in samples], torch.float32, dev)
+        # Views that all have 4 samples keep their sample planes sample-major
+        # across views, so that the resolve, the loss gradient and its spread
+        # over the samples each take one launch for all views.
+        self.planar = all(d.samples == 4 for d in draws)
+        if self.planar:
+            hw = [(H, W) for _, H, W in samples]
+            self._color_flat, colors, self._plane_pixels = _planes(4, hw, 4, torch.float32, dev)
+            self._depth_flat, depths, _ = _planes(4, hw, 1, torch.float32, dev)
+            self._ids_flat, ids, _ = _planes(4, hw, 1, torch.int32, dev)
+            self._dcolor_flat, dcolors, _ = _planes(4, hw, 4, torch.float32, dev)
+            self._resolved_flat, resolved, _ = _planes(1, hw, 4, torch.float32, dev)
+            self._target_flat, tgts, _ = _planes(1, hw, 4, torch.float32, dev)
+            self._g_flat = torch.zeros_like(self._resolved_flat)
+        else:
+            self._color_flat, colors = _flat([(S, H, W, 4) for S, H, W in samples], torch.float32, dev)
+            self._depth_flat, depths = _flat(samples, torch.float32, dev)
+            self._ids_flat, ids = _flat(samples, torch.int32, dev)
+            self._dcolor_flat, dcolors = _flat([(S, H, W, 4) for S, H, W in samples], torch.float32, dev)
+            self._resolved_flat, resolved = _flat([(1, H, W, 4) for _, H, W in samples], torch.float32, dev)
+            self._target_flat, tgts = _flat([(1, H, W, 4) for _, H, W in samples], torch.float32, dev)
         geo = [None] * len(draws), [None] * len(draws), [None] * len(draws)
         self._geo_flat = []
         if geometry_grads:
@@ -123,10 +153,10 @@ class MultiViewFit:
         S, H, W = d.samples, d.height, d.width
         buf = Buffers(positions=vb.pos.data_ptr(), colors=vb.col.data_ptr(), texcoords=vb.uv.data_ptr())
         buf.texture_levels = [(l.data_ptr(), l.shape[1], l.shape[0], l.shape[1] * 16) for l in self.levels]
-        buf.color, buf.color_pitch, buf.color_sample_pitch = vb.color.data_ptr(), W * 16, H * W * 16
-        buf.depth, buf.depth_pitch, buf.depth_sample_pitch = vb.depth.data_ptr(), W * 4, H * W * 4
+        buf.color, buf.color_pitch, buf.color_sample_pitch = vb.color.data_ptr(), W * 16, vb.color.stride(0) * 4
+        buf.depth, buf.depth_pitch, buf.depth_sample_pitch = vb.depth.data_ptr(), W * 4, vb.depth.stride(0) * 4
         desc = build_desc(d, buf)
-        desc.primitive_id = abi.Attachment(vb.ids.data_ptr(), abi.FORMAT_R32_UINT, W * 4, H * W * 4)
+        desc.primitive_id = abi.Attachment(vb.ids.data_ptr(), abi.FORMAT_R32_UINT, W * 4, vb.ids.stride(0) * 4)
         return desc
 
     def _grad(self, vb: _ViewBuffers) -> abi.GradDesc:
@@ -156,6 +186,16 @@ class MultiViewFit:
             # Images are the colour buffers themselves, laid out like the targets.
             torch.sub(self._color_flat, self._target_flat, out=self._dcolor_flat)
             self._dcolor_flat.mul_(2.0 * inv)
+            return
+        if self.planar:
+            # All views at once: the sample planes as one image 4096 pixels wide.
+            rows = self._plane_pixels // 4096
+            r = abi.ResolveDesc(4096, rows, abi.FORMAT_R32G32B32A32_SFLOAT, self._color_flat.data_ptr(), 4096 * 16,
+                                self._color_flat.stride(0) * 4, 4, self._resolved_flat.data_ptr(), 4096 * 16)
+            runtime.check(runtime.lib().swr_resolve(ctypes.byref(r), runtime._stream_handle(None)), "swr_resolve")
+            torch.sub(self._resolved_flat, self._target_flat, out=self._g_flat)
+            self._g_flat.mul_(2.0 * inv)
+            torch.mul(self._g_flat.expand(4, -1), 0.25, out=self._dcolor_flat)
             return
         for vb in self.views:
             if vb.draw.samples > 1:
