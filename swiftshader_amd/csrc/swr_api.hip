@@ -1598,7 +1598,7 @@ int plan_enqueue_backward(swr_batch_plan *pl, int cus, hipStream_t s) {
       hipLaunchKernelGGL(tap_chunks_kernel, dim3((nb1 + 255) / 256), dim3(256), 0, s, (const uint32_t *)start,
                          (const uint32_t *)end, sb.nBuckets, chunks);
       enqueue_scan(chunks, chunkBase, nb1, sums, s);
-      hipLaunchKernelGGL(tap_scatter_kernel, dim3((uint32_t)cus), dim3(kScatterThreads), 0, s,
+      hipLaunchKernelGGL(tap_scatter_kernel, dim3((uint32_t)cus * 2u), dim3(kScatterThreads), 0, s,
                          (const TapRecord *)bin.rec, (const uint32_t *)vIn, (const uint32_t *)start,
                          (const uint32_t *)end, (const uint32_t *)chunkBase, sb.nBuckets,
                          (const BinBucket *)pl->binBuckets.as<BinBucket>() + sb.bucketOff, bin.groups, params, grads,

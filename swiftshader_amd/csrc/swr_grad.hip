@@ -57,13 +57,13 @@ struct GradParams {
 // both mip levels, which reach global memory once per group: all views that
 // hit one tile share one set of atomics.
 constexpr int kBinTile = 32;                          // tile side at the finer level (texels)
-constexpr int kBinHalo0 = 10;                         // halo at the finer level
-constexpr int kBinHalo1 = 6;                          // halo at the coarser level
+constexpr int kBinHalo0 = 6;                          // halo at the finer level (an 8-probe footprint)
+constexpr int kBinHalo1 = 4;                          // halo at the coarser level
 constexpr int kBinR0 = kBinTile + 2 * kBinHalo0;      // LDS region side, finer level
 constexpr int kBinR1 = kBinTile / 2 + 2 * kBinHalo1;  // LDS region side, coarser level
 constexpr int kBinCells = kBinR0 * kBinR0 + kBinR1 * kBinR1;  // LDS cells of both regions
 constexpr uint32_t kBinChunk = 8192;                  // records per scatter work item
-constexpr int kScatterThreads = 512;                   // tap_scatter_kernel workgroup (one per CU: its LDS)
+constexpr int kScatterThreads = 512;                   // tap_scatter_kernel workgroup (two per CU fit in LDS)
 constexpr int kMaxBinGroups = 8;
 constexpr uint32_t kBinReserve = 512;                 // record slots a wave reserves at a time
 constexpr uint32_t kNoBucket = 0xFFFFFFFFu;           // key of a reserved slot left unused
