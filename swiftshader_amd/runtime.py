@@ -164,11 +164,19 @@ def draw_batch(dds: List[DeviceDraw], stream=None):
 
 
 def resolve(src: torch.Tensor, dst: torch.Tensor, stream=None):
-    """4x -> 1x colour resolve (sw::Blitter::resolve); src (4,H,W,4), dst (1,H,W,4)."""
+    """4x -> 1x colour resolve (sw::Blitter::resolve); src (4,H,W,4), dst (1,H,W,4).
+
+    Pixels and channels must be contiguous; rows and sample planes may lie
+    apart (the pitches come from the strides)."""
     S, H, W, _ = src.shape
     fmt = abi.FORMAT_R32G32B32A32_SFLOAT if src.dtype == torch.float32 else abi.FORMAT_R8G8B8A8_UNORM
     bpp = 16 if src.dtype == torch.float32 else 4
-    r = abi.ResolveDesc(W, H, fmt, src.data_ptr(), W * bpp, H * W * bpp, S, dst.data_ptr(), W * bpp)
+    es = src.element_size()
+    for t in (src, dst):
+        if t.stride(2) * t.element_size() != bpp or t.stride(3) != 1:
+            raise ValueError("resolve needs contiguous pixels (H, W, 4) within each row")
+    r = abi.ResolveDesc(W, H, fmt, src.data_ptr(), src.stride(1) * es, src.stride(0) * es, S, dst.data_ptr(),
+                        dst.stride(1) * dst.element_size())
     check(lib().swr_resolve(ctypes.byref(r), _stream_handle(stream)), "swr_resolve")
 
 
@@ -179,8 +187,11 @@ def clear_desc(target: torch.Tensor, fmt: int, value, area=None) -> abi.ClearDes
     stencil); area: (x, y, width, height) or None for the whole image."""
     S, H, W = target.shape[:3]
     bpp = FORMAT_BYTES[fmt]
+    es = target.element_size()
+    if target.stride(2) * es != bpp:
+        raise ValueError("clear needs contiguous pixels within each row")
     d = abi.ClearDesc()
-    d.target = abi.Attachment(target.data_ptr(), fmt, W * bpp, H * W * bpp)
+    d.target = abi.Attachment(target.data_ptr(), fmt, target.stride(1) * es, target.stride(0) * es)
     d.width, d.height, d.samples = W, H, S
     if area is not None:
         d.use_area = 1
