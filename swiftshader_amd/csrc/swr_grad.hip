@@ -509,10 +509,12 @@ __device__ __forceinline__ void texture_backward(GradLds<BIN> &S, const TexParam
 
 // Gradient of one fragment: primitive t at pixel (px, py), upstream g.
 // With BIN the texel gradient is not scattered: the fragment's footprint
-// record goes to `rec` and the function returns true.
+// record goes to `rec` and the function returns true.  The primitive's 27
+// attribute gradients go to `val` (all zero when none are wanted or the
+// fragment has none); the caller adds them to the primitive table.
 template <bool BIN>
 __device__ bool fragment_backward(GradLds<BIN> &S, const DrawParams &P, const GradParams &G, uint32_t t, int px,
-                                  int py, const Acc4 &g, const QuadLod &L, TapRecord &rec) {
+                                  int py, const Acc4 &g, const QuadLod &L, TapRecord &rec, float (&val)[kTriVals]) {
   bool has = false;
   uint32_t vi[3];
   tri_vertices(P, t, vi);
@@ -604,9 +606,6 @@ __device__ bool fragment_backward(GradLds<BIN> &S, const DrawParams &P, const Gr
   if (!attrGrads) return has;
 
   // Attribute gradients and dL/db.
-  float val[kTriVals];
-#pragma unroll
-  for (int k = 0; k < kTriVals; k++) val[k] = 0.0f;
   double Gk[3];
 #pragma unroll
   for (int k = 0; k < 3; k++) {
@@ -647,15 +646,46 @@ __device__ bool fragment_backward(GradLds<BIN> &S, const DrawParams &P, const Gr
       val[3 * k + 2] = (float)(w[k] != 0.0 ? -dr * rhw[k] * rhw[k] : 0.0);
     }
   }
+  return has;
+}
+
+// Adds each lane's primitive gradients val (primitive t; t = kEmptyKey: none)
+// to the primitive table.  Lanes of one primitive usually sit next to each
+// other in the batch: runs of equal t are summed first (a segmented scan
+// across the wave), and the last lane of each run adds the run's sums, so a
+// run of L lanes costs one set of float LDS atomics instead of L.
+template <bool BIN>
+__device__ __forceinline__ void tri_add_wave(GradLds<BIN> &S, const DrawParams &P, const GradParams &G, uint32_t t,
+                                             float (&val)[kTriVals]) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t any = __ballot(t != kEmptyKey);
+  if (!any) return;
+  const uint32_t prev = (uint32_t)__shfl_up((int)t, 1, 64);
+  const uint64_t starts = __ballot(lane == 0 || prev != t);
+  bool add = t != kEmptyKey;
+  if (starts != ~0ull) {
+    const uint64_t upTo = lane == 63 ? ~0ull : (2ull << lane) - 1ull;
+    const int runStart = 63 - __builtin_clzll(starts & upTo);
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const bool take = lane - d >= runStart;
+#pragma unroll
+      for (int k = 0; k < kTriVals; k++) {
+        const float up = __shfl_up(val[k], d, 64);
+        if (take) val[k] += up;
+      }
+    }
+    add = add && (lane == 63 || ((starts >> (lane + 1)) & 1ull));
+  }
+  if (!add) return;
   float *slot = tri_slot(S, t);
   if (!slot) {
     tri_scatter_global(P, G, t, val);
-    return has;
+    return;
   }
 #pragma unroll
   for (int k = 0; k < kTriVals; k++)
     if (val[k] != 0.0f) atomicAdd(slot + k, val[k]);
-  return has;
 }
 
 // Per-wave ring of fragments: one entry per (pixel, distinct owning
@@ -796,6 +826,10 @@ __device__ __forceinline__ void drain_fragments(GradLds<BIN> &S, FragQueue &Q, R
     bool has = false;
     TapRecord r;
     uint32_t key = 0;
+    float val[kTriVals];
+#pragma unroll
+    for (int k = 0; k < kTriVals; k++) val[k] = 0.0f;
+    uint32_t tv = kEmptyKey;  // primitive whose gradients val holds
     if ((uint32_t)lane < n) {
       const uint32_t xy = Q.xy[wave][e], t = Q.tri[wave][e], mask = Q.mask[wave][e];
       const int x = (int)(xy & 0xFFFFu), y = (int)(xy >> 16);
@@ -819,9 +853,11 @@ __device__ __forceinline__ void drain_fragments(GradLds<BIN> &S, FragQueue &Q, R
             *reinterpret_cast<float4 *>(G.state + q * G.stateSlice + (long long)y * G.statePitch + 16ll * x) =
                 make_float4(L.lod, L.aniso, L.uDelta, L.vDelta);
       }
-      has = fragment_backward<BIN>(S, P, G, t, x, y, g, L, r);
+      has = fragment_backward<BIN>(S, P, G, t, x, y, g, L, r, val);
       if (BIN && has) key = tap_bucket(P.tex, bin.groups[grp], r.u, r.v, r.L.lod);
+      tv = t;
     }
+    if (G.dPos || G.dCol || G.dUV) tri_add_wave(S, P, G, tv, val);
     if constexpr (BIN) emit_records(bin, sr, has, r, key);
     rp.head = (rp.head + n) & (kRing - 1);
     rp.count -= n;
