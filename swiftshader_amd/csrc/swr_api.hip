@@ -89,7 +89,7 @@ struct Pipeline {
   Scratch keysA, keysB, valsA, valsB, hist; // per pair
   Scratch first;                            // per block of emitted pairs
   Scratch tStart, tEnd;                     // per tile
-  Scratch counters;  // [0, 16) raster work counters per variant, [16] overflow flag, [20, 22) backward groups
+  Scratch counters;  // work counters and flags (kOverflowSlot and below)
   void release() {
     for (Scratch *s : {&params, &aux, &rec, &poly, &cnt, &off, &sums, &keysA, &keysB, &valsA, &valsB, &hist, &first,
                        &tStart, &tEnd, &counters})
@@ -576,7 +576,11 @@ int variant_of(const DrawParams &P) {
   const bool ftex = P.useUV && P.tex.isFloat;
   const bool xf = P.depth16 || P.colorBGRA || P.colorSRGB || P.advancedBlend || P.depthBounds || P.alphaToCoverage ||
                   (P.useUV && (P.tex.bgra || P.tex.srgb || P.tex.addrU > 1 || P.tex.addrV > 1));
-  return (xf ? 8 : 0) | (P.ms > 1 ? 4 : 0) | (P.colorFloat ? 2 : 0) | (ftex ? 1 : 0);
+  // The common output state (no stencil or depth bias; depth test and write;
+  // all colour channels) has kernels of its own with those checks folded.
+  const bool simple = !xf && !P.stencilActive && !P.stencilWrite && !P.depthBias && P.depthTest && P.depthWrite &&
+                      P.colorMask == 15;
+  return (simple ? 16 : 0) | (xf ? 8 : 0) | (P.ms > 1 ? 4 : 0) | (P.colorFloat ? 2 : 0) | (ftex ? 1 : 0);
 }
 
 // Host plan of a batch: per-draw state in enqueue order, split into
@@ -751,13 +755,15 @@ int upload_layout(DeviceCtx *c, Pipeline &pipe, const BatchLayout &L, hipStream_
 }
 
 // Device words of Pipeline::counters: the persistent raster workgroups'
-// tile counters (one per raster variant and pass, 16 variants x 2 passes),
-// the eager draws' overflow word, the segment's split slot and the number
-// of pairs the segment emitted.
-constexpr int kOverflowSlot = 32;
-constexpr int kSplitSlot = 33;
-constexpr int kEmittedSlot = 34;
-constexpr int kCounterWords = 40;
+// tile counters in [0, 64) (one per raster variant and pass: at most 24
+// variants x 2 passes), the eager draws' overflow word, the segment's split
+// slot, the number of pairs the segment emitted and the backward's two work
+// counters.
+constexpr int kOverflowSlot = 64;
+constexpr int kSplitSlot = 65;
+constexpr int kEmittedSlot = 66;
+constexpr int kBackwardSlot = 68;
+constexpr int kCounterWords = 72;
 
 // 8-bit radix passes that order every tile key below the key 0xFFFFFFFF of
 // unused capacity and dropped pairs: 2^(8 passes) > tiles, so the sorted
@@ -825,24 +831,24 @@ struct RasterArgs {
   uint32_t segT;
 };
 
-template <int MS, bool FCOL, bool FTEX, bool XF>
+template <int MS, bool FCOL, bool FTEX, bool XF, bool SIMPLE>
 void launch_raster_batch(dim3 grid, dim3 block, hipStream_t s, const RasterArgs &a) {
-  hipLaunchKernelGGL((raster_batch_kernel<MS, FCOL, FTEX, XF>), grid, block, 0, s, a.params, a.tileBase, a.lo, a.hi,
-                     a.tLo, a.tHi, a.counter, a.rec, a.poly, a.bin, a.ts, a.te, a.split, a.segT);
+  hipLaunchKernelGGL((raster_batch_kernel<MS, FCOL, FTEX, XF, SIMPLE>), grid, block, 0, s, a.params, a.tileBase, a.lo,
+                     a.hi, a.tLo, a.tHi, a.counter, a.rec, a.poly, a.bin, a.ts, a.te, a.split, a.segT);
 }
 
 // The raster kernel of variant v (variant_of).
-template <bool XF>
+template <bool XF, bool SIMPLE>
 void launch_raster_variant(int v, dim3 grid, dim3 block, hipStream_t s, const RasterArgs &a) {
   switch (v & 7) {
-    case 0: launch_raster_batch<1, false, false, XF>(grid, block, s, a); break;
-    case 1: launch_raster_batch<1, false, true, XF>(grid, block, s, a); break;
-    case 2: launch_raster_batch<1, true, false, XF>(grid, block, s, a); break;
-    case 3: launch_raster_batch<1, true, true, XF>(grid, block, s, a); break;
-    case 4: launch_raster_batch<4, false, false, XF>(grid, block, s, a); break;
-    case 5: launch_raster_batch<4, false, true, XF>(grid, block, s, a); break;
-    case 6: launch_raster_batch<4, true, false, XF>(grid, block, s, a); break;
-    default: launch_raster_batch<4, true, true, XF>(grid, block, s, a); break;
+    case 0: launch_raster_batch<1, false, false, XF, SIMPLE>(grid, block, s, a); break;
+    case 1: launch_raster_batch<1, false, true, XF, SIMPLE>(grid, block, s, a); break;
+    case 2: launch_raster_batch<1, true, false, XF, SIMPLE>(grid, block, s, a); break;
+    case 3: launch_raster_batch<1, true, true, XF, SIMPLE>(grid, block, s, a); break;
+    case 4: launch_raster_batch<4, false, false, XF, SIMPLE>(grid, block, s, a); break;
+    case 5: launch_raster_batch<4, false, true, XF, SIMPLE>(grid, block, s, a); break;
+    case 6: launch_raster_batch<4, true, false, XF, SIMPLE>(grid, block, s, a); break;
+    default: launch_raster_batch<4, true, true, XF, SIMPLE>(grid, block, s, a); break;
   }
 }
 
@@ -935,9 +941,11 @@ int enqueue_back(DeviceCtx *c, Pipeline &pipe, const BatchLayout &L, const Batch
         a.segT = sg.T;
         const int var = variant_of(L.P[sg.first + lo]);
         if (var & 8)
-          launch_raster_variant<true>(var, dim3(wgs), block, s, a);
+          launch_raster_variant<true, false>(var, dim3(wgs), block, s, a);
+        else if (var & 16)
+          launch_raster_variant<false, true>(var, dim3(wgs), block, s, a);
         else
-          launch_raster_variant<false>(var, dim3(wgs), block, s, a);
+          launch_raster_variant<false, false>(var, dim3(wgs), block, s, a);
       }
     }
   } else if (es) {
@@ -1539,8 +1547,8 @@ int plan_enqueue_backward(swr_batch_plan *pl, int cus, hipStream_t s) {
       hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, binCtr, 0u, 2u);
     }
     // Draws of one sample count are contiguous (the layout groups variants);
-    // each group gets its own work counter (counters[20 + group]).
-    uint32_t *ctr = pipe.counters.as<uint32_t>() + 20;
+    // each group gets its own work counter (counters[kBackwardSlot + group]).
+    uint32_t *ctr = pipe.counters.as<uint32_t>() + kBackwardSlot;
     hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, ctr, 0u, 2u);
     int group = 0;
     int lo = 0;

@@ -754,10 +754,16 @@ constexpr int kQueue = 64;  // deferred-shading queue entries per wave
 // One screen tile of draw P: `bin` is the tile's index in the draw's tile
 // grid, [start, end) its range of the sorted (tile, primitive) list.  The
 // workgroup has one wave per 8x8 block of the tile.
-template <int MS, bool FCOL, bool FTEX, bool XF>
+// SIMPLE: the draw has no stencil and no depth bias, tests and writes depth
+// and writes all four colour channels (the checks below fold away).
+template <int MS, bool FCOL, bool FTEX, bool XF, bool SIMPLE>
 __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32_t start, uint32_t end0,
                                             const TriRecord *__restrict__ rec, const int *__restrict__ poly,
                                             const uint32_t *__restrict__ binTri) {
+  const bool stencilActive = !SIMPLE && P.stencilActive, stencilWrite = !SIMPLE && P.stencilWrite;
+  const bool depthBias = !SIMPLE && P.depthBias, depthTest = SIMPLE || P.depthTest;
+  const bool depthWrite = SIMPLE || P.depthWrite;
+  const int colorMask = SIMPLE ? 15 : P.colorMask;
   // Per-wave queue of fragments that passed the tests and await shading,
   // in API order: primitive, owner lane | sample mask << 8, shaded colour.
   __shared__ uint32_t qTri[4][kQueue];
@@ -802,13 +808,13 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
     if (FCOL && P.primId)
       S.id[FCOL ? q : 0] =
           *reinterpret_cast<const uint32_t *>(P.primId + q * P.primIdSlice + (long long)py * P.primIdPitch + 4ll * px);
-    if (P.depthTest || (XF && P.depthBounds))  // D16 holds the float of its integer (readDepth16, PixelRoutine.cpp:466-481)
+    if (depthTest || (XF && P.depthBounds))  // D16 holds the float of its integer (readDepth16, PixelRoutine.cpp:466-481)
       S.zb[q] = (XF && P.depth16) ? (float)*reinterpret_cast<const uint16_t *>(P.depth + q * P.depthSlice +
                                                                         (long long)py * P.depthPitch + 2ll * px)
                           : *reinterpret_cast<const float *>(P.depth + q * P.depthSlice +
                                                              (long long)py * P.depthPitch + 4ll * px);
-    if (P.stencilActive) S.sb[q] = (uint8_t)P.stencil[q * P.stencilSlice + (long long)py * P.stencilPitch + px];
-    if (P.colorMask) {
+    if (stencilActive) S.sb[q] = (uint8_t)P.stencil[q * P.stencilSlice + (long long)py * P.stencilPitch + px];
+    if (colorMask) {
       if (FCOL)
         S.cf[FCOL ? q : 0] =
             *reinterpret_cast<const float4 *>(P.color + q * P.colorSlice + (long long)py * P.colorPitch + 16ll * px);
@@ -823,7 +829,7 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
   // by zlo cannot pass LESS (LESS_OR_EQUAL) anywhere in a block whose
   // largest stored depth is <= zlo (< zlo).  Only when a failing fragment
   // has no side effect (no stencil writes).
-  const bool hiz = P.depthTest && !P.stencilWrite && (P.depthCompare == 1 || P.depthCompare == 3);
+  const bool hiz = depthTest && !stencilWrite && (P.depthCompare == 1 || P.depthCompare == 3);
 
   unsigned long long occl = 0;
   uint32_t nTests = 0, nCovers = 0, nShaded = 0;  // statistics (profiling only)
@@ -938,7 +944,7 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
       const StencilParams &SP = (fl & 2) ? P.front : P.back;
 
       int sMask = cMask, zMask = cMask;
-      if (P.stencilActive && cMask) {
+      if (stencilActive && cMask) {
 #pragma unroll
         for (int q = 0; q < MS; q++)
           if (((cMask >> q) & 1) && !stencil_compare(SP, S.sb[q])) sMask &= ~(1 << q);
@@ -962,11 +968,11 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
         zMask &= cMask;
       }
       float zq[MS];
-      if (P.depthTest) {
+      if (depthTest) {
         const float xf = (float)px - rdl(x0, j);  // xFragment (PixelRoutine.cpp:125)
         const float yf = (float)py - rdl(y0, j);  // yFragment (QuadRasterizer.cpp:126)
         const float a = rdl(zA, j), b = rdl(zB, j), c = rdl(zC, j);
-        const float zbias = P.depthBias ? rdl(zBias, j) : 0.0f;
+        const float zbias = depthBias ? rdl(zBias, j) : 0.0f;
         zMask = 0;
 #pragma unroll
         for (int q = 0; q < MS; q++) {
@@ -974,7 +980,7 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
           const float xx = (MS > 1) ? xf - kSampleX[q] : xf;
           const float Dz = c + yy * b;
           float z = xx * a + Dz;
-          if (P.depthBias) z = z + zbias;  // PixelRoutine.cpp:141-144, before the clamp
+          if (depthBias) z = z + zbias;  // PixelRoutine.cpp:141-144, before the clamp
           z = rr_min(rr_max(z, P.depthClampLo), P.depthClampHi);  // clampDepth
           if ((XF && P.depth16)) z = d16_value(z);
           zq[q] = z;
@@ -986,7 +992,7 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
           if (pr != (P.zInv != 0)) zMask |= 1 << q;
         }
         zMask &= cMask;
-        if (P.stencilActive) zMask &= sMask;
+        if (stencilActive) zMask &= sMask;
       }
       if (XF && P.depthBounds && cMask) {
         // depthBoundsTest (PixelRoutine.cpp:576-641) on the stored depth, D16
@@ -998,16 +1004,16 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
           const float zs = P.depth16 ? S.zb[q] * (1.0f / 65535.0f) : S.zb[q];
           if (P.depthBoundsMin <= zs && zs <= P.depthBoundsMax) bMask |= 1 << q;
         }
-        if (P.depthTest) zMask &= bMask;
+        if (depthTest) zMask &= bMask;
         else cMask &= bMask;
       }
-      if (P.stencilWrite && cMask) {
+      if (stencilWrite && cMask) {
 #pragma unroll
         for (int q = 0; q < MS; q++) {
           if (!((cMask >> q) & 1)) continue;
           int op;
           if (!((sMask >> q) & 1)) op = SP.failOp;
-          else if (P.depthTest && !((zMask >> q) & 1)) op = SP.depthFailOp;
+          else if (depthTest && !((zMask >> q) & 1)) op = SP.depthFailOp;
           else op = SP.passOp;
           const uint32_t bv = S.sb[q];
           uint32_t nv = stencil_apply_op(bv, op, SP.reference);
@@ -1015,15 +1021,15 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
           S.sb[q] = nv & 0xFFu;
         }
       }
-      if (P.depthWrite) {
+      if (depthWrite) {
 #pragma unroll
         for (int q = 0; q < MS; q++)
           if ((zMask >> q) & 1) S.zb[q] = zq[q];
       }
       occl += __builtin_popcount((unsigned)(zMask & sMask));
-      if (!P.colorMask) continue;
-      int xMask = P.depthTest ? zMask : cMask;
-      if (P.stencilActive) xMask &= sMask;
+      if (!colorMask) continue;
+      int xMask = depthTest ? zMask : cMask;
+      if (stencilActive) xMask &= sMask;
       const uint64_t need = __ballot(xMask != 0);
       if (!need) continue;
       // Queue the fragments that write colour; shading is deferred so that
@@ -1080,7 +1086,7 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
         const float4 h3 = reinterpret_cast<const float4 *>(rp)[3];  // x0, y0, zA, zB
         const float4 h4 = reinterpret_cast<const float4 *>(rp)[4];  // zC, zBias, ...
         const float x0 = h3.x, y0 = h3.y, zA = h3.z, zB = h3.w, zC = h4.x;
-        const float zBias = P.depthBias ? h4.y : 0.0f;
+        const float zBias = depthBias ? h4.y : 0.0f;
         // Lower bound of the evaluated plane over this block (sample offsets
         // included), tightened with the primitive's bound over its box.
         const float xa = (float)ox - 0.5f - x0, xb = (float)(ox + 7) + 0.5f - x0;
@@ -1193,7 +1199,7 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
 // Occupancy: 4 waves per SIMD (128 VGPRs) for the single-sample kernels
 // (the float-colour ones spill 12 bytes there and still run faster than at
 // 3 waves), 3 for the multisample ones.
-template <int MS, bool FCOL, bool FTEX, bool XF>
+template <int MS, bool FCOL, bool FTEX, bool XF, bool SIMPLE>
 __global__ __launch_bounds__(256, MS == 1 ? 4 : 3) void raster_batch_kernel(
                                                            const DrawParams *__restrict__ params,
                                                            const uint32_t *__restrict__ tileBase, int drawLo,
@@ -1236,7 +1242,7 @@ __global__ __launch_bounds__(256, MS == 1 ? 4 : 3) void raster_batch_kernel(
       s = from > b ? from : b;
       e = b + (uint32_t)params[lo].triCount;
     }
-    if (s < e) raster_tile<MS, FCOL, FTEX, XF>(params[lo], (int)(t - tileBase[lo]), s, e, rec, poly, binTri);
+    if (s < e) raster_tile<MS, FCOL, FTEX, XF, SIMPLE>(params[lo], (int)(t - tileBase[lo]), s, e, rec, poly, binTri);
   }
 }
 
@@ -1327,26 +1333,34 @@ __global__ void resolve_rgba32f_kernel(const char *src, int srcPitch, long long 
 }
 
 
-#define SWR_RASTER_INSTANCE(MS, FCOL, FTEX, XF)                                                                  \
-  template __global__ void raster_batch_kernel<MS, FCOL, FTEX, XF>(                                             \
+#define SWR_RASTER_INSTANCE(MS, FCOL, FTEX, XF, SIMPLE)                                                          \
+  template __global__ void raster_batch_kernel<MS, FCOL, FTEX, XF, SIMPLE>(                                     \
       const DrawParams *, const uint32_t *, int, int, uint32_t, uint32_t, uint32_t *, const TriRecord *, const int *, \
       const uint32_t *, const uint32_t *, const uint32_t *, const uint32_t *, uint32_t);
-SWR_RASTER_INSTANCE(1, false, false, false)
-SWR_RASTER_INSTANCE(1, false, true, false)
-SWR_RASTER_INSTANCE(1, true, false, false)
-SWR_RASTER_INSTANCE(1, true, true, false)
-SWR_RASTER_INSTANCE(4, false, false, false)
-SWR_RASTER_INSTANCE(4, false, true, false)
-SWR_RASTER_INSTANCE(4, true, false, false)
-SWR_RASTER_INSTANCE(4, true, true, false)
-SWR_RASTER_INSTANCE(1, false, false, true)
-SWR_RASTER_INSTANCE(1, false, true, true)
-SWR_RASTER_INSTANCE(1, true, false, true)
-SWR_RASTER_INSTANCE(1, true, true, true)
-SWR_RASTER_INSTANCE(4, false, false, true)
-SWR_RASTER_INSTANCE(4, false, true, true)
-SWR_RASTER_INSTANCE(4, true, false, true)
-SWR_RASTER_INSTANCE(4, true, true, true)
+SWR_RASTER_INSTANCE(1, false, false, false, false)
+SWR_RASTER_INSTANCE(1, false, true, false, false)
+SWR_RASTER_INSTANCE(1, true, false, false, false)
+SWR_RASTER_INSTANCE(1, true, true, false, false)
+SWR_RASTER_INSTANCE(4, false, false, false, false)
+SWR_RASTER_INSTANCE(4, false, true, false, false)
+SWR_RASTER_INSTANCE(4, true, false, false, false)
+SWR_RASTER_INSTANCE(4, true, true, false, false)
+SWR_RASTER_INSTANCE(1, false, false, true, false)
+SWR_RASTER_INSTANCE(1, false, true, true, false)
+SWR_RASTER_INSTANCE(1, true, false, true, false)
+SWR_RASTER_INSTANCE(1, true, true, true, false)
+SWR_RASTER_INSTANCE(4, false, false, true, false)
+SWR_RASTER_INSTANCE(4, false, true, true, false)
+SWR_RASTER_INSTANCE(4, true, false, true, false)
+SWR_RASTER_INSTANCE(4, true, true, true, false)
+SWR_RASTER_INSTANCE(1, false, false, false, true)
+SWR_RASTER_INSTANCE(1, false, true, false, true)
+SWR_RASTER_INSTANCE(1, true, false, false, true)
+SWR_RASTER_INSTANCE(1, true, true, false, true)
+SWR_RASTER_INSTANCE(4, false, false, false, true)
+SWR_RASTER_INSTANCE(4, false, true, false, true)
+SWR_RASTER_INSTANCE(4, true, false, false, true)
+SWR_RASTER_INSTANCE(4, true, true, false, true)
 #undef SWR_RASTER_INSTANCE
 
 }  // namespace swr
