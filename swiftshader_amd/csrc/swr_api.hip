@@ -577,10 +577,18 @@ int variant_of(const DrawParams &P) {
   const bool xf = P.depth16 || P.colorBGRA || P.colorSRGB || P.advancedBlend || P.depthBounds || P.alphaToCoverage ||
                   (P.useUV && (P.tex.bgra || P.tex.srgb || P.tex.addrU > 1 || P.tex.addrV > 1));
   // The common output state (no stencil or depth bias; depth test and write;
-  // all colour channels) has kernels of its own with those checks folded.
+  // all colour channels) has kernels of its own with those checks folded
+  // (level 1); so does that state with the common shading state: MODULATE,
+  // linear or anisotropic filtering, linear mips, REPEAT, blending off or
+  // SRC_ALPHA / ONE_MINUS_SRC_ALPHA / ADD (level 2).
   const bool simple = !xf && !P.stencilActive && !P.stencilWrite && !P.depthBias && P.depthTest && P.depthWrite &&
                       P.colorMask == 15;
-  return (simple ? 16 : 0) | (xf ? 8 : 0) | (P.ms > 1 ? 4 : 0) | (P.colorFloat ? 2 : 0) | (ftex ? 1 : 0);
+  const bool fast = simple && P.shader == SHADER_MODULATE && P.useColor && P.useUV &&
+                    (!P.blendEnable || P.blendKind == 1) && !P.tex.skipLod && P.tex.mipmap == MIP_LINEAR &&
+                    P.tex.addrU == 0 && P.tex.addrV == 0 &&
+                    (P.tex.filter == FILT_LINEAR || P.tex.filter == FILT_ANISOTROPIC);
+  return (fast ? 32 : simple ? 16 : 0) | (xf ? 8 : 0) | (P.ms > 1 ? 4 : 0) | (P.colorFloat ? 2 : 0) |
+         (ftex ? 1 : 0);
 }
 
 // Host plan of a batch: per-draw state in enqueue order, split into
@@ -755,7 +763,7 @@ int upload_layout(DeviceCtx *c, Pipeline &pipe, const BatchLayout &L, hipStream_
 }
 
 // Device words of Pipeline::counters: the persistent raster workgroups'
-// tile counters in [0, 64) (one per raster variant and pass: at most 24
+// tile counters in [0, 64) (one per raster variant and pass: at most 32
 // variants x 2 passes), the eager draws' overflow word, the segment's split
 // slot, the number of pairs the segment emitted and the backward's two work
 // counters.
@@ -831,24 +839,24 @@ struct RasterArgs {
   uint32_t segT;
 };
 
-template <int MS, bool FCOL, bool FTEX, bool XF, bool SIMPLE>
+template <int MS, bool FCOL, bool FTEX, bool XF, int LEVEL>
 void launch_raster_batch(dim3 grid, dim3 block, hipStream_t s, const RasterArgs &a) {
-  hipLaunchKernelGGL((raster_batch_kernel<MS, FCOL, FTEX, XF, SIMPLE>), grid, block, 0, s, a.params, a.tileBase, a.lo,
+  hipLaunchKernelGGL((raster_batch_kernel<MS, FCOL, FTEX, XF, LEVEL>), grid, block, 0, s, a.params, a.tileBase, a.lo,
                      a.hi, a.tLo, a.tHi, a.counter, a.rec, a.poly, a.bin, a.ts, a.te, a.split, a.segT);
 }
 
 // The raster kernel of variant v (variant_of).
-template <bool XF, bool SIMPLE>
+template <bool XF, int LEVEL>
 void launch_raster_variant(int v, dim3 grid, dim3 block, hipStream_t s, const RasterArgs &a) {
   switch (v & 7) {
-    case 0: launch_raster_batch<1, false, false, XF, SIMPLE>(grid, block, s, a); break;
-    case 1: launch_raster_batch<1, false, true, XF, SIMPLE>(grid, block, s, a); break;
-    case 2: launch_raster_batch<1, true, false, XF, SIMPLE>(grid, block, s, a); break;
-    case 3: launch_raster_batch<1, true, true, XF, SIMPLE>(grid, block, s, a); break;
-    case 4: launch_raster_batch<4, false, false, XF, SIMPLE>(grid, block, s, a); break;
-    case 5: launch_raster_batch<4, false, true, XF, SIMPLE>(grid, block, s, a); break;
-    case 6: launch_raster_batch<4, true, false, XF, SIMPLE>(grid, block, s, a); break;
-    default: launch_raster_batch<4, true, true, XF, SIMPLE>(grid, block, s, a); break;
+    case 0: launch_raster_batch<1, false, false, XF, LEVEL>(grid, block, s, a); break;
+    case 1: launch_raster_batch<1, false, true, XF, LEVEL>(grid, block, s, a); break;
+    case 2: launch_raster_batch<1, true, false, XF, LEVEL>(grid, block, s, a); break;
+    case 3: launch_raster_batch<1, true, true, XF, LEVEL>(grid, block, s, a); break;
+    case 4: launch_raster_batch<4, false, false, XF, LEVEL>(grid, block, s, a); break;
+    case 5: launch_raster_batch<4, false, true, XF, LEVEL>(grid, block, s, a); break;
+    case 6: launch_raster_batch<4, true, false, XF, LEVEL>(grid, block, s, a); break;
+    default: launch_raster_batch<4, true, true, XF, LEVEL>(grid, block, s, a); break;
   }
 }
 
@@ -941,11 +949,13 @@ int enqueue_back(DeviceCtx *c, Pipeline &pipe, const BatchLayout &L, const Batch
         a.segT = sg.T;
         const int var = variant_of(L.P[sg.first + lo]);
         if (var & 8)
-          launch_raster_variant<true, false>(var, dim3(wgs), block, s, a);
+          launch_raster_variant<true, 0>(var, dim3(wgs), block, s, a);
+        else if (var & 32)
+          launch_raster_variant<false, 2>(var, dim3(wgs), block, s, a);
         else if (var & 16)
-          launch_raster_variant<false, true>(var, dim3(wgs), block, s, a);
+          launch_raster_variant<false, 1>(var, dim3(wgs), block, s, a);
         else
-          launch_raster_variant<false, false>(var, dim3(wgs), block, s, a);
+          launch_raster_variant<false, 0>(var, dim3(wgs), block, s, a);
       }
     }
   } else if (es) {

@@ -652,6 +652,7 @@ __device__ __forceinline__ float quad_rhw(const TriRecord &R, float xf, float yf
 // differences of u, v over the quad's pixels (0,0), (1,0), (0,1), helper
 // lanes included, evaluated from the plane equations (computeLod2D,
 // SamplerCore.cpp:1376-1422).
+template <bool FAST = false>
 __device__ __forceinline__ QuadLod quad_state(const DrawParams &P, const TriRecord &R, int px, int py) {
   const int qx = px & ~1, qy = py & ~1;
   float uq[4], vq[4];
@@ -664,30 +665,33 @@ __device__ __forceinline__ QuadLod quad_state(const DrawParams &P, const TriReco
   }
   uq[3] = 0.0f;
   vq[3] = 0.0f;
-  return quad_lod(P.tex, uq, vq);
+  return quad_lod<FAST>(P.tex, uq, vq);
 }
 
 // Shades one queued fragment: the fixed shader family at pixel (px, py)
 // of primitive R, with the texture LOD of the 2x2 quad the pixel belongs
 // to (helper lanes evaluated from the plane equations).  Output clamped
 // for UNORM targets (PixelProgram::clampColor).
-template <bool FCOL, bool FTEX, bool XF>
+// FAST: the draw's shader is MODULATE and its sampler state the common one
+// (tex_filter and the other readers of the sampler state).
+template <bool FCOL, bool FTEX, bool XF, bool FAST = false>
 __device__ __forceinline__ float4 shade(const DrawParams &P, const TriRecord &R, int px, int py,
                                         bool clampUnorm = true) {
+  const int shader = FAST ? (int)SHADER_MODULATE : P.shader;
   const float xf = (float)px - R.x0, yf = (float)py - R.y0;
   const float rhw = quad_rhw(R, xf, yf);
   float4 col = make_float4(0, 0, 0, 0);
-  if (P.shader != SHADER_SAMPLE) {
+  if (shader != SHADER_SAMPLE) {
     col.x = interp(R, 0, xf, yf, rhw);
     col.y = interp(R, 1, xf, yf, rhw);
     col.z = interp(R, 2, xf, yf, rhw);
     col.w = interp(R, 3, xf, yf, rhw);
   }
-  if (P.shader != SHADER_INTERPOLATE) {
+  if (shader != SHADER_INTERPOLATE) {
     const float u = interp(R, 4, xf, yf, rhw), v = interp(R, 5, xf, yf, rhw);
-    const QuadLod Lq = quad_state(P, R, px, py);
-    float4 c = sample_lane<FTEX, XF>(P.tex, u, v, Lq);
-    if (P.shader == SHADER_MODULATE) {
+    const QuadLod Lq = quad_state<FAST>(P, R, px, py);
+    float4 c = sample_lane<FTEX, XF, FAST>(P.tex, u, v, Lq);
+    if (shader == SHADER_MODULATE) {
       c.x = c.x * col.x;
       c.y = c.y * col.y;
       c.z = c.z * col.z;
@@ -706,7 +710,8 @@ __device__ __forceinline__ float4 shade(const DrawParams &P, const TriRecord &R,
 
 // Blend a shaded colour into the samples of this lane's pixel selected by
 // xMask, then write (PixelRoutine.cpp:1617-2936).
-template <int MS, bool FCOL, bool XF>
+// FAST: blending, when on, is SRC_ALPHA, ONE_MINUS_SRC_ALPHA, ADD.
+template <int MS, bool FCOL, bool XF, bool FAST = false>
 __device__ __forceinline__ void blend_write(const DrawParams &P, PixelState<MS, FCOL> &S, float4 s, int xMask,
                                             uint32_t tri) {
 #pragma unroll
@@ -725,7 +730,7 @@ __device__ __forceinline__ void blend_write(const DrawParams &P, PixelState<MS, 
         dst.z = srgb_to_linear(dst.z);
       }
     }
-    float4 o = P.blendEnable ? alpha_blend<XF>(P, s, dst) : s;
+    float4 o = P.blendEnable ? (FAST ? blend_src_alpha(s, dst) : alpha_blend<XF>(P, s, dst)) : s;
     if (!FCOL && (XF && P.colorSRGB)) {  // writeColor encodes R, G, B first (PixelRoutine.cpp:1963-1969)
       o.x = linear_to_srgb(o.x);
       o.y = linear_to_srgb(o.y);
@@ -754,12 +759,14 @@ constexpr int kQueue = 64;  // deferred-shading queue entries per wave
 // One screen tile of draw P: `bin` is the tile's index in the draw's tile
 // grid, [start, end) its range of the sorted (tile, primitive) list.  The
 // workgroup has one wave per 8x8 block of the tile.
-// SIMPLE: the draw has no stencil and no depth bias, tests and writes depth
-// and writes all four colour channels (the checks below fold away).
-template <int MS, bool FCOL, bool FTEX, bool XF, bool SIMPLE>
+// LEVEL 1: the draw has no stencil and no depth bias, tests and writes
+// depth and writes all four colour channels (the checks below fold away).
+// LEVEL 2: also the common shading state (shade's and blend_write's FAST).
+template <int MS, bool FCOL, bool FTEX, bool XF, int LEVEL>
 __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32_t start, uint32_t end0,
                                             const TriRecord *__restrict__ rec, const int *__restrict__ poly,
                                             const uint32_t *__restrict__ binTri) {
+  constexpr bool SIMPLE = LEVEL >= 1, FAST = LEVEL >= 2;
   const bool stencilActive = !SIMPLE && P.stencilActive, stencilWrite = !SIMPLE && P.stencilWrite;
   const bool depthBias = !SIMPLE && P.depthBias, depthTest = SIMPLE || P.depthTest;
   const bool depthWrite = SIMPLE || P.depthWrite;
@@ -842,7 +849,7 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
       const uint32_t t = qTri[wave][lane];
       const uint32_t pix = qPix[wave][lane];
       const int l = (int)(pix & 63u);
-      qCol[wave][lane] = shade<FCOL, FTEX, XF>(P, rec[t], ox + lane_cx(l), oy + lane_cy(l));
+      qCol[wave][lane] = shade<FCOL, FTEX, XF, FAST>(P, rec[t], ox + lane_cx(l), oy + lane_cy(l));
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
@@ -851,7 +858,7 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
     while (m) {
       const int k = __builtin_ctzll(m);
       m &= m - 1;
-      blend_write<MS, FCOL, XF>(P, S, qCol[wave][k], (int)(qPix[wave][k] >> 8), qTri[wave][k]);
+      blend_write<MS, FCOL, XF, FAST>(P, S, qCol[wave][k], (int)(qPix[wave][k] >> 8), qTri[wave][k]);
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
@@ -957,7 +964,7 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
         // fragment is shaded again, identically, if it reaches the queue.
         if (cMask) {
           const uint32_t tj = __builtin_amdgcn_readlane(tid, j);
-          const float a = shade<FCOL, FTEX, XF>(P, rec[tj], px, py, P.color != nullptr).w;
+          const float a = shade<FCOL, FTEX, XF, FAST>(P, rec[tj], px, py, P.color != nullptr).w;
           int aMask = 0;
 #pragma unroll
           for (int q = 0; q < MS; q++)
@@ -1199,7 +1206,7 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
 // Occupancy: 4 waves per SIMD (128 VGPRs) for the single-sample kernels
 // (the float-colour ones spill 12 bytes there and still run faster than at
 // 3 waves), 3 for the multisample ones.
-template <int MS, bool FCOL, bool FTEX, bool XF, bool SIMPLE>
+template <int MS, bool FCOL, bool FTEX, bool XF, int LEVEL>
 __global__ __launch_bounds__(256, MS == 1 ? 4 : 3) void raster_batch_kernel(
                                                            const DrawParams *__restrict__ params,
                                                            const uint32_t *__restrict__ tileBase, int drawLo,
@@ -1242,7 +1249,7 @@ __global__ __launch_bounds__(256, MS == 1 ? 4 : 3) void raster_batch_kernel(
       s = from > b ? from : b;
       e = b + (uint32_t)params[lo].triCount;
     }
-    if (s < e) raster_tile<MS, FCOL, FTEX, XF, SIMPLE>(params[lo], (int)(t - tileBase[lo]), s, e, rec, poly, binTri);
+    if (s < e) raster_tile<MS, FCOL, FTEX, XF, LEVEL>(params[lo], (int)(t - tileBase[lo]), s, e, rec, poly, binTri);
   }
 }
 
@@ -1333,34 +1340,42 @@ __global__ void resolve_rgba32f_kernel(const char *src, int srcPitch, long long 
 }
 
 
-#define SWR_RASTER_INSTANCE(MS, FCOL, FTEX, XF, SIMPLE)                                                          \
-  template __global__ void raster_batch_kernel<MS, FCOL, FTEX, XF, SIMPLE>(                                     \
+#define SWR_RASTER_INSTANCE(MS, FCOL, FTEX, XF, LEVEL)                                                           \
+  template __global__ void raster_batch_kernel<MS, FCOL, FTEX, XF, LEVEL>(                                      \
       const DrawParams *, const uint32_t *, int, int, uint32_t, uint32_t, uint32_t *, const TriRecord *, const int *, \
       const uint32_t *, const uint32_t *, const uint32_t *, const uint32_t *, uint32_t);
-SWR_RASTER_INSTANCE(1, false, false, false, false)
-SWR_RASTER_INSTANCE(1, false, true, false, false)
-SWR_RASTER_INSTANCE(1, true, false, false, false)
-SWR_RASTER_INSTANCE(1, true, true, false, false)
-SWR_RASTER_INSTANCE(4, false, false, false, false)
-SWR_RASTER_INSTANCE(4, false, true, false, false)
-SWR_RASTER_INSTANCE(4, true, false, false, false)
-SWR_RASTER_INSTANCE(4, true, true, false, false)
-SWR_RASTER_INSTANCE(1, false, false, true, false)
-SWR_RASTER_INSTANCE(1, false, true, true, false)
-SWR_RASTER_INSTANCE(1, true, false, true, false)
-SWR_RASTER_INSTANCE(1, true, true, true, false)
-SWR_RASTER_INSTANCE(4, false, false, true, false)
-SWR_RASTER_INSTANCE(4, false, true, true, false)
-SWR_RASTER_INSTANCE(4, true, false, true, false)
-SWR_RASTER_INSTANCE(4, true, true, true, false)
-SWR_RASTER_INSTANCE(1, false, false, false, true)
-SWR_RASTER_INSTANCE(1, false, true, false, true)
-SWR_RASTER_INSTANCE(1, true, false, false, true)
-SWR_RASTER_INSTANCE(1, true, true, false, true)
-SWR_RASTER_INSTANCE(4, false, false, false, true)
-SWR_RASTER_INSTANCE(4, false, true, false, true)
-SWR_RASTER_INSTANCE(4, true, false, false, true)
-SWR_RASTER_INSTANCE(4, true, true, false, true)
+SWR_RASTER_INSTANCE(1, false, false, false, 0)
+SWR_RASTER_INSTANCE(1, false, true, false, 0)
+SWR_RASTER_INSTANCE(1, true, false, false, 0)
+SWR_RASTER_INSTANCE(1, true, true, false, 0)
+SWR_RASTER_INSTANCE(4, false, false, false, 0)
+SWR_RASTER_INSTANCE(4, false, true, false, 0)
+SWR_RASTER_INSTANCE(4, true, false, false, 0)
+SWR_RASTER_INSTANCE(4, true, true, false, 0)
+SWR_RASTER_INSTANCE(1, false, false, true, 0)
+SWR_RASTER_INSTANCE(1, false, true, true, 0)
+SWR_RASTER_INSTANCE(1, true, false, true, 0)
+SWR_RASTER_INSTANCE(1, true, true, true, 0)
+SWR_RASTER_INSTANCE(4, false, false, true, 0)
+SWR_RASTER_INSTANCE(4, false, true, true, 0)
+SWR_RASTER_INSTANCE(4, true, false, true, 0)
+SWR_RASTER_INSTANCE(4, true, true, true, 0)
+SWR_RASTER_INSTANCE(1, false, false, false, 1)
+SWR_RASTER_INSTANCE(1, false, true, false, 1)
+SWR_RASTER_INSTANCE(1, true, false, false, 1)
+SWR_RASTER_INSTANCE(1, true, true, false, 1)
+SWR_RASTER_INSTANCE(4, false, false, false, 1)
+SWR_RASTER_INSTANCE(4, false, true, false, 1)
+SWR_RASTER_INSTANCE(4, true, false, false, 1)
+SWR_RASTER_INSTANCE(4, true, true, false, 1)
+SWR_RASTER_INSTANCE(1, false, false, false, 2)
+SWR_RASTER_INSTANCE(1, false, true, false, 2)
+SWR_RASTER_INSTANCE(1, true, false, false, 2)
+SWR_RASTER_INSTANCE(1, true, true, false, 2)
+SWR_RASTER_INSTANCE(4, false, false, false, 2)
+SWR_RASTER_INSTANCE(4, false, true, false, 2)
+SWR_RASTER_INSTANCE(4, true, false, false, 2)
+SWR_RASTER_INSTANCE(4, true, true, false, 2)
 #undef SWR_RASTER_INSTANCE
 
 }  // namespace swr
