@@ -126,8 +126,9 @@ def _ms(fn, reps):
 def leg_batched(dev, reps):
     """512 independent draws of uneven size (targets 64..1024 per side, 2k..50k
     triangles, RGBA8 noise textures 256..2048 with full mips, trilinear,
-    CLAMP_TO_EDGE or REPEAT): one autotuned batch plan, clears + plan run
-    captured in one HIP graph, against one swr_draw per draw (no graph)."""
+    CLAMP_TO_EDGE or REPEAT): one swr_clear_batch of every target and one
+    autotuned batch plan run, captured in one HIP graph, against one swr_clear
+    per target and one swr_draw per draw (no graph)."""
     import torch
     from swiftshader_amd import runtime, workloads
     from swiftshader_amd.batch import BatchPlan
@@ -135,8 +136,7 @@ def leg_batched(dev, reps):
     tris = sum(d.triangle_count for d in draws)
 
     def clear_all():
-        for dd in dds:
-            runtime.clear_targets(dd.draw, dd.targets, reset_occlusion=False)
+        runtime.clear_targets_batch([(dd.draw, dd.targets) for dd in dds])
 
     def plain():
         for dd in dds:

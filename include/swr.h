@@ -46,7 +46,7 @@
 extern "C" {
 #endif
 
-#define SWR_ABI_VERSION 11
+#define SWR_ABI_VERSION 12
 
 /* ---- status codes -------------------------------------------------------- */
 #define SWR_OK 0
@@ -486,6 +486,19 @@ int swr_resolve(const swr_resolve_desc *desc, void *stream);
  * blit routine's Round(65535 * clamp(d)) (a NaN becomes 0xFFFF,
  * Blitter.cpp:1427-1484); S8 the stencil's low byte. */
 int swr_clear(const swr_clear_desc *desc, void *stream);
+
+/* count clears with the result of count swr_clear calls, in as few kernel
+ * launches as their element sizes allow (up to 24 clears per launch, the
+ * areas passed as kernel arguments: nothing is uploaded, so the call can be
+ * captured into a graph).  No counterpart in the reference: a render pass
+ * clears its attachments one vk::Image::clear at a time; this is for
+ * callers that clear many small targets at once (one per draw of a batch).
+ * Every descriptor is validated before anything is launched.  The clears
+ * run concurrently, so their byte spans -- per sample plane, from the first
+ * to the last byte of the area -- must not overlap
+ * (SWR_ERROR_INVALID_ARGUMENT otherwise): side-by-side areas of the same
+ * rows count as overlapping. */
+int swr_clear_batch(const swr_clear_desc *descs, uint32_t count, void *stream);
 
 /* Compressed-texture decoding: BC_Decoder::Decode (src/Device/
  * BC_Decoder.hpp:29), which vk::Image::decodeBC calls per mip level and

@@ -7,7 +7,7 @@ DrawData in the reference (src/Device/Renderer.hpp:58-113, 211-213).
 """
 import ctypes as C
 
-ABI_VERSION = 11
+ABI_VERSION = 12
 
 OK = 0
 ERROR_INVALID_ARGUMENT = -1
@@ -195,7 +195,7 @@ class Profile(C.Structure):
 # Symbols declared by include/swr.h, in declaration order.
 EXPORTED_SYMBOLS = (
     "swr_abi_version", "swr_last_error", "swr_init", "swr_release", "swr_draw", "swr_draw_noalloc",
-    "swr_draw_batch", "swr_resolve", "swr_clear", "swr_decode_bc", "swr_decode_etc", "swr_batch_plan_create", "swr_batch_plan_run", "swr_batch_plan_check",
+    "swr_draw_batch", "swr_resolve", "swr_clear", "swr_clear_batch", "swr_decode_bc", "swr_decode_etc", "swr_batch_plan_create", "swr_batch_plan_run", "swr_batch_plan_check",
     "swr_batch_plan_autotune", "swr_batch_plan_set_grads", "swr_batch_plan_backward", "swr_batch_plan_destroy", "swr_draw_backward", "swr_generate_mips", "swr_mip_backward",
     "swr_profile_enable", "swr_profile_read", "swr_set_tuning", "swr_get_tuning",
 )
@@ -218,6 +218,8 @@ def bind(lib):
     lib.swr_resolve.restype = C.c_int
     lib.swr_clear.argtypes = [C.POINTER(ClearDesc), C.c_void_p]
     lib.swr_clear.restype = C.c_int
+    lib.swr_clear_batch.argtypes = [C.POINTER(ClearDesc), C.c_uint32, C.c_void_p]
+    lib.swr_clear_batch.restype = C.c_int
     lib.swr_decode_bc.argtypes = [C.POINTER(BCDesc), C.c_void_p]
     lib.swr_decode_bc.restype = C.c_int
     lib.swr_decode_etc.argtypes = [C.POINTER(ETCDesc), C.c_void_p]

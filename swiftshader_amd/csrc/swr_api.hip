@@ -1831,9 +1831,11 @@ uint32_t srgb8_clear(float c, int k) {
   v = v > 0.0f ? v : 0.0f;
   return (uint32_t)std::nearbyint(v);
 }
-}  // namespace
 
-int swr_clear(const swr_clear_desc *d, void *stream) {
+// A validated clear: the packed value, element size and the area (w <= 0:
+// nothing to clear).
+int clear_prepare(const swr_clear_desc *d, ClearRect &r) {
+  r = ClearRect{};
   if (!d || !d->target.data) return fail(SWR_ERROR_INVALID_ARGUMENT, "clear descriptor incomplete");
   if (d->samples != 1 && d->samples != 4) return fail(SWR_ERROR_UNSUPPORTED, "clear supports 1 or 4 samples");
   uint32_t bpp = 0;
@@ -1886,20 +1888,85 @@ int swr_clear(const swr_clear_desc *d, void *stream) {
     y1 = std::min<int64_t>(y1, (int64_t)d->area.y + d->area.height);
   }
   if (x1 <= x0 || y1 <= y0) return SWR_OK;
-  const int w = (int)(x1 - x0), h = (int)(y1 - y0);
+  r.base = (char *)d->target.data;
+  r.pitch = (int)d->target.row_pitch_bytes;
+  r.slice = (long long)d->target.sample_pitch_bytes;
+  r.x0 = (int)x0;
+  r.y0 = (int)y0;
+  r.w = (int)(x1 - x0);
+  r.h = (int)(y1 - y0);
+  r.samples = (int)d->samples;
+  r.bytes = (int)bpp;
+  r.value = v;
+  return SWR_OK;
+}
+}  // namespace
+
+int swr_clear(const swr_clear_desc *d, void *stream) {
+  ClearRect r;
+  const int rc = clear_prepare(d, r);
+  if (rc != SWR_OK) return rc;
+  if (r.w <= 0) return SWR_OK;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  dim3 block(256), grid((w + 255) / 256, h, d->samples);
-  char *base = (char *)d->target.data;
-  const int pitch = (int)d->target.row_pitch_bytes;
-  const long long slice = (long long)d->target.sample_pitch_bytes;
-  if (bpp == 16)
-    hipLaunchKernelGGL(clear_rect_kernel<16>, grid, block, 0, s, base, pitch, slice, (int)x0, (int)y0, w, h, v);
-  else if (bpp == 4)
-    hipLaunchKernelGGL(clear_rect_kernel<4>, grid, block, 0, s, base, pitch, slice, (int)x0, (int)y0, w, h, v);
-  else if (bpp == 2)
-    hipLaunchKernelGGL(clear_rect_kernel<2>, grid, block, 0, s, base, pitch, slice, (int)x0, (int)y0, w, h, v);
+  dim3 block(256), grid((r.w + 255) / 256, r.h, r.samples);
+  if (r.bytes == 16)
+    hipLaunchKernelGGL(clear_rect_kernel<16>, grid, block, 0, s, r.base, r.pitch, r.slice, r.x0, r.y0, r.w, r.h, r.value);
+  else if (r.bytes == 4)
+    hipLaunchKernelGGL(clear_rect_kernel<4>, grid, block, 0, s, r.base, r.pitch, r.slice, r.x0, r.y0, r.w, r.h, r.value);
+  else if (r.bytes == 2)
+    hipLaunchKernelGGL(clear_rect_kernel<2>, grid, block, 0, s, r.base, r.pitch, r.slice, r.x0, r.y0, r.w, r.h, r.value);
   else
-    hipLaunchKernelGGL(clear_rect_kernel<1>, grid, block, 0, s, base, pitch, slice, (int)x0, (int)y0, w, h, v);
+    hipLaunchKernelGGL(clear_rect_kernel<1>, grid, block, 0, s, r.base, r.pitch, r.slice, r.x0, r.y0, r.w, r.h, r.value);
+  SWR_HIP_CHECK(hipGetLastError());
+  return SWR_OK;
+}
+
+int swr_clear_batch(const swr_clear_desc *descs, uint32_t count, void *stream) {
+  if (count && !descs) return fail(SWR_ERROR_INVALID_ARGUMENT, "clear batch without descriptors");
+  std::vector<ClearRect> rects;
+  rects.reserve(count);
+  std::vector<std::pair<uintptr_t, uintptr_t>> ranges;
+  for (uint32_t i = 0; i < count; i++) {
+    ClearRect r;
+    const int rc = clear_prepare(&descs[i], r);
+    if (rc != SWR_OK) return fail(rc, "clear %u: %s", i, g_error.c_str());
+    if (r.w <= 0) continue;
+    // The span of bytes the clear writes, plane by plane (from the area's
+    // first byte to its last: side-by-side areas of the same rows count as
+    // overlapping).
+    for (int q = 0; q < r.samples; q++) {
+      const char *lo = r.base + (long long)q * r.slice + (long long)r.y0 * r.pitch + (long long)r.x0 * r.bytes;
+      ranges.push_back({(uintptr_t)lo, (uintptr_t)(lo + (long long)(r.h - 1) * r.pitch + (long long)r.w * r.bytes)});
+    }
+    rects.push_back(r);
+  }
+  // The clears run concurrently: their byte ranges must be disjoint.
+  std::sort(ranges.begin(), ranges.end());
+  for (size_t k = 1; k < ranges.size(); k++)
+    if (ranges[k].first < ranges[k - 1].second)
+      return fail(SWR_ERROR_INVALID_ARGUMENT, "clear batch targets overlap");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  for (int bytes : {16, 4, 2, 1}) {
+    ClearBatch cb{};
+    auto launch = [&]() {
+      if (!cb.n) return;
+      dim3 block(256), grid(cb.start[cb.n]);
+      if (bytes == 16) hipLaunchKernelGGL(clear_batch_kernel<16>, grid, block, 0, s, cb);
+      else if (bytes == 4) hipLaunchKernelGGL(clear_batch_kernel<4>, grid, block, 0, s, cb);
+      else if (bytes == 2) hipLaunchKernelGGL(clear_batch_kernel<2>, grid, block, 0, s, cb);
+      else hipLaunchKernelGGL(clear_batch_kernel<1>, grid, block, 0, s, cb);
+      cb = ClearBatch{};
+    };
+    for (const ClearRect &r : rects) {
+      if (r.bytes != bytes) continue;
+      const uint64_t blocks = (uint64_t)((r.w + 255) / 256) * (uint64_t)r.h * (uint64_t)r.samples;
+      if (cb.n == kClearBatch || (uint64_t)cb.start[cb.n] + blocks > 0x7FFFFFFFull) launch();
+      cb.r[cb.n] = r;
+      cb.start[cb.n + 1] = cb.start[cb.n] + (uint32_t)blocks;
+      cb.n++;
+    }
+    launch();
+  }
   SWR_HIP_CHECK(hipGetLastError());
   return SWR_OK;
 }

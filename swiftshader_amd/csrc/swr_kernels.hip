@@ -1271,6 +1271,40 @@ __global__ void clear_rect_kernel(char *base, int rowPitch, long long slice, int
   else *reinterpret_cast<uint8_t *>(p) = (uint8_t)value.x;
 }
 
+// Up to kClearBatch clears of one element size in one launch (a kernel
+// argument, so nothing is uploaded): rect k owns blocks [start[k],
+// start[k + 1]), each block 256 pixels of one row of one sample plane.
+constexpr int kClearBatch = 24;
+struct ClearRect {
+  char *base;
+  long long slice;
+  int pitch, x0, y0, w, h, samples, bytes;
+  uint4 value;
+};
+struct ClearBatch {
+  ClearRect r[kClearBatch];
+  uint32_t start[kClearBatch + 1];
+  int n;
+};
+
+template <int BYTES>
+__global__ void clear_batch_kernel(ClearBatch cb) {
+  const uint32_t b = blockIdx.x;
+  int k = 0;
+  while (k + 1 < cb.n && cb.start[k + 1] <= b) k++;
+  const ClearRect &r = cb.r[k];
+  const uint32_t local = b - cb.start[k];
+  const uint32_t bx = (uint32_t)(r.w + 255) / 256u;
+  const uint32_t row = local / bx;  // row + h * sample
+  const int x = (int)(local % bx) * 256 + (int)threadIdx.x, y = (int)(row % (uint32_t)r.h), q = (int)(row / (uint32_t)r.h);
+  if (x >= r.w) return;
+  char *p = r.base + (long long)q * r.slice + (long long)(r.y0 + y) * r.pitch + (long long)BYTES * (r.x0 + x);
+  if (BYTES == 16) *reinterpret_cast<uint4 *>(p) = r.value;
+  else if (BYTES == 4) *reinterpret_cast<uint32_t *>(p) = r.value.x;
+  else if (BYTES == 2) *reinterpret_cast<uint16_t *>(p) = (uint16_t)r.value.x;
+  else *reinterpret_cast<uint8_t *>(p) = (uint8_t)r.value.x;
+}
+
 // 4x resolve of an sRGB attachment: the reference's blit routine (Blitter.cpp
 // 1505-1545, 1427-1484, 676-690) decodes each sample to linear, averages in
 // linear space and encodes the average: per R, G, B channel

@@ -180,6 +180,30 @@ def resolve(src: torch.Tensor, dst: torch.Tensor, stream=None):
     check(lib().swr_resolve(ctypes.byref(r), _stream_handle(stream)), "swr_resolve")
 
 
+def clear_batch(items, stream=None):
+    """Clears several targets with one swr_clear_batch call: items is a list
+    of (target, fmt, value) or (target, fmt, value, area), as clear() takes
+    them; the targets must not overlap."""
+    descs = (abi.ClearDesc * max(1, len(items)))()
+    for i, it in enumerate(items):
+        descs[i] = clear_desc(*it)
+    check(lib().swr_clear_batch(descs, len(items), _stream_handle(stream)), "swr_clear_batch")
+
+
+def clear_targets_batch(pairs, stream=None):
+    """clear_targets for many (draw, targets) pairs in one swr_clear_batch call
+    (occlusion counters untouched)."""
+    items = []
+    for draw, t in pairs:
+        if t.color is not None:
+            items.append((t.color, draw.color_format, draw.clear_color))
+        if t.depth is not None:
+            items.append((t.depth, draw.depth_format, draw.clear_depth))
+        if t.stencil is not None:
+            items.append((t.stencil, abi.FORMAT_S8_UINT, draw.clear_stencil))
+    clear_batch(items, stream)
+
+
 def clear_desc(target: torch.Tensor, fmt: int, value, area=None) -> abi.ClearDesc:
     """A swr_clear_desc for a (S, H, W[, C]) target tensor of format fmt.
 
