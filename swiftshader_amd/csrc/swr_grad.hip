@@ -1013,19 +1013,17 @@ __device__ __forceinline__ long long fixed_round(float x) {
 }
 
 __device__ __forceinline__ void lds_add_fixed(long long *c, float x) {
-  if (x != 0.0f) atomicAdd((unsigned long long *)c, (unsigned long long)fixed_round(x));
+  atomicAdd((unsigned long long *)c, (unsigned long long)fixed_round(x));
 }
 
-// Adds the scaled products v (g * w * S) to texel (x, y) of a level: into its
-// region when the texel has a cell there, else (unscaled, v * invS) straight
-// to the gradient buffer.
-__device__ __forceinline__ void bin_add(long long *regions, int base, int side, int ox, int oy, int w, int h,
-                                        bool wrapU, bool wrapV, float *g, int pitch, float invS, int x, int y,
+// Adds the scaled products v (g * w * S) to texel (x, y) of a level: into
+// region cell `cell` when the texel has one (cell >= 0), else (unscaled,
+// v * invS) straight to the gradient buffer.
+__device__ __forceinline__ void bin_add(long long *regions, int cell, float *g, int pitch, float invS, int x, int y,
                                         const float4 &v) {
   if (v.x == 0.0f && v.y == 0.0f && v.z == 0.0f && v.w == 0.0f) return;
-  const int rx = bin_cell_axis(x, ox, w, wrapU), ry = bin_cell_axis(y, oy, h, wrapV);
-  if ((unsigned)rx < (unsigned)side && (unsigned)ry < (unsigned)side) {
-    long long *c = regions + base + ry * side + rx;
+  if (cell >= 0) {
+    long long *c = regions + cell;
     lds_add_fixed(c, v.x);
     lds_add_fixed(c + kBinCells, v.y);
     lds_add_fixed(c + 2 * kBinCells, v.z);
@@ -1195,13 +1193,19 @@ __global__ __launch_bounds__(kScatterThreads) void tap_scatter_kernel(const TapR
           // (exact: a power of two).
           const float w00 = (float)(wgt * ((1.0 - fu) * (1.0 - fv))) * S, w10 = (float)(wgt * (fu * (1.0 - fv))) * S;
           const float w01 = (float)(wgt * ((1.0 - fu) * fv)) * S, w11 = (float)(wgt * (fu * fv)) * S;
-          bin_add(R, base, side, ox, oy, W, H, addrU == 0, addrV == 0, g, pitch, invS, x0, y0,
+          // The footprint's cells: rows and columns placed once for its four taps.
+          const int rx0 = bin_cell_axis(x0, ox, W, addrU == 0), rx1 = bin_cell_axis(x1, ox, W, addrU == 0);
+          const int ry0 = bin_cell_axis(y0, oy, H, addrV == 0), ry1 = bin_cell_axis(y1, oy, H, addrV == 0);
+          const bool cx0 = (unsigned)rx0 < (unsigned)side, cx1 = (unsigned)rx1 < (unsigned)side;
+          const bool cy0 = (unsigned)ry0 < (unsigned)side, cy1 = (unsigned)ry1 < (unsigned)side;
+          const int row0 = base + ry0 * side, row1 = base + ry1 * side;
+          bin_add(R, cx0 && cy0 ? row0 + rx0 : -1, g, pitch, invS, x0, y0,
                   make_float4(r.g.x * w00, r.g.y * w00, r.g.z * w00, r.g.w * w00));
-          bin_add(R, base, side, ox, oy, W, H, addrU == 0, addrV == 0, g, pitch, invS, x1, y0,
+          bin_add(R, cx1 && cy0 ? row0 + rx1 : -1, g, pitch, invS, x1, y0,
                   make_float4(r.g.x * w10, r.g.y * w10, r.g.z * w10, r.g.w * w10));
-          bin_add(R, base, side, ox, oy, W, H, addrU == 0, addrV == 0, g, pitch, invS, x0, y1,
+          bin_add(R, cx0 && cy1 ? row1 + rx0 : -1, g, pitch, invS, x0, y1,
                   make_float4(r.g.x * w01, r.g.y * w01, r.g.z * w01, r.g.w * w01));
-          bin_add(R, base, side, ox, oy, W, H, addrU == 0, addrV == 0, g, pitch, invS, x1, y1,
+          bin_add(R, cx1 && cy1 ? row1 + rx1 : -1, g, pitch, invS, x1, y1,
                   make_float4(r.g.x * w11, r.g.y * w11, r.g.z * w11, r.g.w * w11));
         }
       }
