@@ -110,6 +110,12 @@ struct DeviceCtx {
   bool profiling = false;
   std::vector<EventSet> pending, pool;
   swr_profile acc{};
+  // Single draws that do not wait on the host: their pair capacity (0:
+  // unknown yet), and the read-back of a draw's pair count into
+  // hostTotal[2], complete when pairsEvent is.
+  uint32_t pairCap = 0;
+  hipEvent_t pairsEvent = nullptr;
+  bool pairsPending = false;
 };
 
 hipError_t take_events(DeviceCtx *c, EventSet &es) {
@@ -973,8 +979,44 @@ int check_tuning(const swr_tuning &t) {
   return SWR_OK;
 }
 
+__global__ void store_params_kernel(DrawParams P, DrawParams *dst, uint32_t *aux);
+
+// A single draw with a known pair capacity: parameters written by a kernel
+// (no staging buffer to wait for), pairs beyond the capacity rendered by
+// the unbinned raster pass (no count to wait for); the draw's count is read
+// back without a wait and grows the capacity for the draws after it.
+int draw_nowait(DeviceCtx *c, const BatchLayout &L, const swr_tuning &tuning, hipStream_t s) {
+  const BatchLayout::Seg &sg = L.segs[0];
+  Pipeline &pipe = c->pipe;
+  SWR_HIP_CHECK(pipe.params.ensure(sizeof(DrawParams)));
+  SWR_HIP_CHECK(pipe.aux.ensure(sizeof(uint32_t) * 2));
+  int r = ensure_pipe(pipe, sg.T, c->pairCap, sg.tiles);
+  if (r != SWR_OK) return r;
+  hipLaunchKernelGGL(store_params_kernel, dim3(1), dim3(256), 0, s, L.P[0], pipe.params.as<DrawParams>(),
+                     pipe.aux.as<uint32_t>());
+  enqueue_front(pipe, L, sg, s);
+  SWR_HIP_CHECK(hipGetLastError());
+  r = enqueue_back(c, pipe, L, sg, c->pairCap, tuning, pipe.counters.as<uint32_t>() + kOverflowSlot, true, nullptr,
+                   s);
+  if (r != SWR_OK) return r;
+  if (!c->pairsPending) {
+    if (!c->pairsEvent) SWR_HIP_CHECK(hipEventCreateWithFlags(&c->pairsEvent, hipEventDisableTiming));
+    SWR_HIP_CHECK(hipMemcpyAsync(c->hostTotal + 2, pipe.off.as<uint32_t>() + sg.T, sizeof(uint32_t),
+                                 hipMemcpyDeviceToHost, s));
+    SWR_HIP_CHECK(hipEventRecord(c->pairsEvent, s));
+    c->pairsPending = true;
+  }
+  return SWR_OK;
+}
+
+// The pair capacity for no-wait single draws after a draw needing `pairs`.
+void grow_pair_cap(DeviceCtx *c, uint64_t pairs) {
+  c->pairCap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(c->pairCap, pairs + pairs / 8 + 1024), 0xFFFFFF00ull);
+}
+
 // Eager draws: each segment reads its pair count back before sizing the
-// pair buffers (one host wait per segment).
+// pair buffers (one host wait per segment) -- except a single draw once a
+// pair capacity is known (draw_nowait).
 int draw_eager(DeviceCtx *c, const swr_draw_desc *descs, uint32_t n, hipStream_t s) {
   BatchLayout L;
   swr_tuning tuning = c->tuning;
@@ -982,6 +1024,16 @@ int draw_eager(DeviceCtx *c, const swr_draw_desc *descs, uint32_t n, hipStream_t
   if (r != SWR_OK) return r;
   if (L.segs.empty()) return SWR_OK;
   const bool prof = c->profiling;
+  if (c->pairsPending) {
+    const hipError_t q = hipEventQuery(c->pairsEvent);
+    if (q == hipSuccess) {
+      grow_pair_cap(c, c->hostTotal[2]);
+      c->pairsPending = false;
+    } else {
+      (void)hipGetLastError();  // not ready yet: no error to keep
+    }
+  }
+  if (n == 1 && !prof && L.segs.size() == 1 && c->pairCap) return draw_nowait(c, L, tuning, s);
   if (prof) {
     SWR_HIP_CHECK(c->stats.ensure(4 * sizeof(unsigned long long)));
     for (DrawParams &P : L.P) P.stats = c->stats.as<unsigned long long>();
@@ -1004,6 +1056,7 @@ int draw_eager(DeviceCtx *c, const swr_draw_desc *descs, uint32_t n, hipStream_t
                                  hipMemcpyDeviceToHost, s));
     SWR_HIP_CHECK(hipStreamSynchronize(s));
     const uint32_t pairs = *c->hostTotal;
+    if (n == 1) grow_pair_cap(c, pairs);
     r = ensure_pipe(pipe, sg.T, pairs, sg.tiles);
     if (r != SWR_OK) return r;
     r = enqueue_back(c, pipe, L, sg, pairs, tuning, pipe.counters.as<uint32_t>() + kOverflowSlot, false,
@@ -1737,6 +1790,8 @@ int swr_release(void) {
   if (r != SWR_OK) return r;
   SWR_HIP_CHECK(hipDeviceSynchronize());
   c->pipe.release();
+  c->pairCap = 0;
+  c->pairsPending = false;
   return SWR_OK;
 }
 
