@@ -74,3 +74,27 @@ def test_resolve_matches_oracle():
         torch.cuda.synchronize()
         got = d.cpu().numpy()
         assert np.array_equal(ref.view(np.uint8), got.view(np.uint8))
+
+
+def test_draw_beyond_the_known_pair_capacity():
+    """After the first single draw, swr_draw stops waiting on the host: it
+    sizes the (tile, primitive) pair buffers from the counts of earlier draws
+    and renders the pairs beyond that capacity through the unbinned pass.  A
+    large draw right after a small one (far beyond the small one's capacity),
+    then the same draw again (capacity grown from its count), both match the
+    oracle bit for bit."""
+    import torch
+    from swiftshader_amd import runtime
+    # The smallest case leaves a capacity of at most a few thousand pairs
+    # (its count plus an eighth plus 1024); the case with the most triangles
+    # needs at least one pair per triangle, well beyond it.
+    by_size = sorted(CASES, key=lambda c: (c[1].triangle_count, c[1].width * c[1].height))
+    small, big = by_size[0], by_size[-1]
+    assert small[1].triangle_count <= 8 and big[1].triangle_count >= 3000
+    runtime.check(runtime.lib().swr_release(), "swr_release")  # forget the learnt capacity
+    print(f"[gpu parity] capacity from {small[0]}, then {big[0]} twice", flush=True)
+    gpu_render(small[1])
+    ref = oracle_render(big[1])
+    _compare(big[0] + " (beyond capacity)", ref, gpu_render(big[1]))
+    torch.cuda.synchronize()
+    _compare(big[0] + " (capacity grown)", ref, gpu_render(big[1]))
