@@ -443,11 +443,18 @@ const char *swr_last_error(void);
  * per-device state.  Optional: the first call on a device does it. */
 int swr_init(int device);
 
-/* Releases library scratch on the current device. */
+/* Releases library scratch on the current device (and the (tile,
+ * primitive) pair capacity swr_draw has learnt). */
 int swr_release(void);
 
 /* Renders one draw (sw::Renderer::draw).  Reads and writes only the memory
- * the descriptor names plus library scratch.  Asynchronous on `stream`. */
+ * the descriptor names plus library scratch.  Asynchronous on `stream`: the
+ * first draw on a device (and the first after swr_release) waits once for
+ * its pair count to size the library scratch; later draws size it from the
+ * counts of earlier ones, read back without waiting, and render any pairs
+ * beyond it through the unbinned pass (as a batch plan does), so they
+ * return without waiting on the device.  Scratch grows by allocation when a
+ * count shows it too small. */
 int swr_draw(const swr_draw_desc *desc, void *stream);
 
 /* Renders one draw as swr_draw does, without allocating.  No counterpart in
