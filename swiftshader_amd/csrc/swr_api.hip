@@ -585,16 +585,17 @@ int variant_of(const DrawParams &P) {
   // The common output state (no stencil or depth bias; depth test and write;
   // all colour channels) has kernels of its own with those checks folded
   // (level 1); so does that state with the common shading state: MODULATE,
-  // linear or anisotropic filtering, linear mips, REPEAT, blending off or
-  // SRC_ALPHA / ONE_MINUS_SRC_ALPHA / ADD (level 2).
+  // linear or anisotropic filtering, linear mips, blending off or SRC_ALPHA /
+  // ONE_MINUS_SRC_ALPHA / ADD, and REPEAT (level 2) or CLAMP_TO_EDGE (level
+  // 3) on both axes.
   const bool simple = !xf && !P.stencilActive && !P.stencilWrite && !P.depthBias && P.depthTest && P.depthWrite &&
                       P.colorMask == 15;
   const bool fast = simple && P.shader == SHADER_MODULATE && P.useColor && P.useUV &&
                     (!P.blendEnable || P.blendKind == 1) && !P.tex.skipLod && P.tex.mipmap == MIP_LINEAR &&
-                    P.tex.addrU == 0 && P.tex.addrV == 0 &&
+                    P.tex.addrU == P.tex.addrV && P.tex.addrU <= 1 &&
                     (P.tex.filter == FILT_LINEAR || P.tex.filter == FILT_ANISOTROPIC);
-  return (fast ? 32 : simple ? 16 : 0) | (xf ? 8 : 0) | (P.ms > 1 ? 4 : 0) | (P.colorFloat ? 2 : 0) |
-         (ftex ? 1 : 0);
+  const int level = fast ? 2 + P.tex.addrU : simple ? 1 : 0;
+  return (level << 4) | (xf ? 8 : 0) | (P.ms > 1 ? 4 : 0) | (P.colorFloat ? 2 : 0) | (ftex ? 1 : 0);
 }
 
 // Host plan of a batch: per-draw state in enqueue order, split into
@@ -769,15 +770,15 @@ int upload_layout(DeviceCtx *c, Pipeline &pipe, const BatchLayout &L, hipStream_
 }
 
 // Device words of Pipeline::counters: the persistent raster workgroups'
-// tile counters in [0, 64) (one per raster variant and pass: at most 32
+// tile counters in [0, 80) (one per raster variant and pass: at most 40
 // variants x 2 passes), the eager draws' overflow word, the segment's split
 // slot, the number of pairs the segment emitted and the backward's two work
 // counters.
-constexpr int kOverflowSlot = 64;
-constexpr int kSplitSlot = 65;
-constexpr int kEmittedSlot = 66;
-constexpr int kBackwardSlot = 68;
-constexpr int kCounterWords = 72;
+constexpr int kOverflowSlot = 80;
+constexpr int kSplitSlot = 81;
+constexpr int kEmittedSlot = 82;
+constexpr int kBackwardSlot = 84;
+constexpr int kCounterWords = 88;
 
 // 8-bit radix passes that order every tile key below the key 0xFFFFFFFF of
 // unused capacity and dropped pairs: 2^(8 passes) > tiles, so the sorted
@@ -954,14 +955,13 @@ int enqueue_back(DeviceCtx *c, Pipeline &pipe, const BatchLayout &L, const Batch
         a.split = split;
         a.segT = sg.T;
         const int var = variant_of(L.P[sg.first + lo]);
-        if (var & 8)
-          launch_raster_variant<true, 0>(var, dim3(wgs), block, s, a);
-        else if (var & 32)
-          launch_raster_variant<false, 2>(var, dim3(wgs), block, s, a);
-        else if (var & 16)
-          launch_raster_variant<false, 1>(var, dim3(wgs), block, s, a);
-        else
-          launch_raster_variant<false, 0>(var, dim3(wgs), block, s, a);
+        switch (var & 8 ? -1 : var >> 4) {
+          case -1: launch_raster_variant<true, 0>(var, dim3(wgs), block, s, a); break;
+          case 3: launch_raster_variant<false, 3>(var, dim3(wgs), block, s, a); break;
+          case 2: launch_raster_variant<false, 2>(var, dim3(wgs), block, s, a); break;
+          case 1: launch_raster_variant<false, 1>(var, dim3(wgs), block, s, a); break;
+          default: launch_raster_variant<false, 0>(var, dim3(wgs), block, s, a); break;
+        }
       }
     }
   } else if (es) {
@@ -2014,7 +2014,7 @@ int swr_clear_batch(const swr_clear_desc *descs, uint32_t count, void *stream) {
     };
     for (const ClearRect &r : rects) {
       if (r.bytes != bytes) continue;
-      const uint64_t blocks = (uint64_t)((r.w + 255) / 256) * (uint64_t)r.h * (uint64_t)r.samples;
+      const uint64_t blocks = ((uint64_t)r.w * (uint64_t)r.h * (uint64_t)r.samples + kClearPixels - 1) / kClearPixels;
       if (cb.n == kClearBatch || (uint64_t)cb.start[cb.n] + blocks > 0x7FFFFFFFull) launch();
       cb.r[cb.n] = r;
       cb.start[cb.n + 1] = cb.start[cb.n] + (uint32_t)blocks;

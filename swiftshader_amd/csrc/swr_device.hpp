@@ -138,44 +138,45 @@ __device__ __forceinline__ void fetch16(const TexLevel &L, uint32_t u, uint32_t 
   }
 }
 
-// Sampler state as the sampling code reads it.  With FAST the draw is known
-// to use the common state -- linear or anisotropic filtering, linear mips,
-// REPEAT on both axes, a computed LOD -- and those reads are constants.
-template <bool FAST>
+// Sampler state as the sampling code reads it.  FA < 0: as the parameters
+// say.  FA >= 0: the draw is known to use the common state -- linear or
+// anisotropic filtering, linear mips, a computed LOD, and address mode FA
+// (0 REPEAT, 1 CLAMP_TO_EDGE) on both axes -- and those reads are constants.
+template <int FA>
 __device__ __forceinline__ int tex_filter(const TexParams &T) {
-  if constexpr (FAST) return T.filter == FILT_ANISOTROPIC ? FILT_ANISOTROPIC : FILT_LINEAR;
+  if constexpr (FA >= 0) return T.filter == FILT_ANISOTROPIC ? FILT_ANISOTROPIC : FILT_LINEAR;
   return T.filter;
 }
-template <bool FAST> __device__ __forceinline__ int tex_mipmap(const TexParams &T) { return FAST ? MIP_LINEAR : T.mipmap; }
-template <bool FAST> __device__ __forceinline__ int tex_addr_u(const TexParams &T) { return FAST ? 0 : T.addrU; }
-template <bool FAST> __device__ __forceinline__ int tex_addr_v(const TexParams &T) { return FAST ? 0 : T.addrV; }
-template <bool FAST> __device__ __forceinline__ bool tex_skip_lod(const TexParams &T) { return !FAST && T.skipLod; }
+template <int FA> __device__ __forceinline__ int tex_mipmap(const TexParams &T) { return FA >= 0 ? MIP_LINEAR : T.mipmap; }
+template <int FA> __device__ __forceinline__ int tex_addr_u(const TexParams &T) { return FA >= 0 ? FA : T.addrU; }
+template <int FA> __device__ __forceinline__ int tex_addr_v(const TexParams &T) { return FA >= 0 ? FA : T.addrV; }
+template <int FA> __device__ __forceinline__ bool tex_skip_lod(const TexParams &T) { return FA < 0 && T.skipLod; }
 
-template <bool FAST = false>
+template <int FA = -1>
 __device__ __forceinline__ int select_mip(const TexParams &T, float lod, int second) {
-  if (tex_mipmap<FAST>(T) == MIP_NONE) return 0;
-  int ilod = (tex_mipmap<FAST>(T) == MIP_POINT) ? rr_round_int(lod) : rr_trunc_int(lod);
+  if (tex_mipmap<FA>(T) == MIP_NONE) return 0;
+  int ilod = (tex_mipmap<FA>(T) == MIP_POINT) ? rr_round_int(lod) : rr_trunc_int(lod);
   return ilod + second;
 }
 
 // sampleQuad2D + bilinearInterpolate, RGBA8 16-bit path (:668-909, :516-666).
-template <bool XF, bool FAST = false>
+template <bool XF, int FA = -1>
 __device__ __forceinline__ void sample_quad16(const TexParams &T, const TexLevel &L, float u, float v, float lod,
                                               uint32_t out[4]) {
-  uint32_t uu = address16<XF>(u, tex_addr_u<FAST>(T));
-  uint32_t vv = address16<XF>(v, tex_addr_v<FAST>(T));  // offsets wrap only in wrap mode (:720-723)
-  if (tex_filter<FAST>(T) == FILT_POINT) {
+  uint32_t uu = address16<XF>(u, tex_addr_u<FA>(T));
+  uint32_t vv = address16<XF>(v, tex_addr_v<FA>(T));  // offsets wrap only in wrap mode (:720-723)
+  if (tex_filter<FA>(T) == FILT_POINT) {
     fetch16<XF>(L, uu, vv, out);
     return;
   }
   uint32_t uh = L.uHalf, vh = L.vHalf;
-  if (tex_filter<FAST>(T) == FILT_MIN_LINEAR_MAG_POINT) {
+  if (tex_filter<FA>(T) == FILT_MIN_LINEAR_MAG_POINT) {
     if (lod <= 0.0f) { uh = 0; vh = 0; }        // offset &= CmpNLE(lod, 0)
-  } else if (tex_filter<FAST>(T) == FILT_MIN_POINT_MAG_LINEAR) {
+  } else if (tex_filter<FA>(T) == FILT_MIN_POINT_MAG_LINEAR) {
     if (!(lod <= 0.0f)) { uh = 0; vh = 0; }     // offset &= CmpLE(lod, 0)
   }
-  uint32_t u0 = offset16(uu, uh, tex_addr_u<FAST>(T) == 0, -1), v0 = offset16(vv, vh, tex_addr_v<FAST>(T) == 0, -1);
-  uint32_t u1 = offset16(uu, uh, tex_addr_u<FAST>(T) == 0, +1), v1 = offset16(vv, vh, tex_addr_v<FAST>(T) == 0, +1);
+  uint32_t u0 = offset16(uu, uh, tex_addr_u<FA>(T) == 0, -1), v0 = offset16(vv, vh, tex_addr_v<FA>(T) == 0, -1);
+  uint32_t u1 = offset16(uu, uh, tex_addr_u<FA>(T) == 0, +1), v1 = offset16(vv, vh, tex_addr_v<FA>(T) == 0, +1);
   uint32_t c00[4], c10[4], c01[4], c11[4];
   fetch16<XF>(L, u0, v0, c00);
   fetch16<XF>(L, u1, v0, c10);
@@ -194,12 +195,12 @@ __device__ __forceinline__ void sample_quad16(const TexParams &T, const TexLevel
   }
 }
 
-template <bool XF, bool FAST = false>
+template <bool XF, int FA = -1>
 __device__ __forceinline__ void sample_aniso16(const TexParams &T, float u, float v, float lod, float aniso,
                                                float uDelta, float vDelta, int second, uint32_t out[4]) {
-  const TexLevel L = tex_level(T, select_mip<FAST>(T, lod, second));
-  if (tex_filter<FAST>(T) != FILT_ANISOTROPIC) {
-    sample_quad16<XF, FAST>(T, L, u, v, lod, out);
+  const TexLevel L = tex_level(T, select_mip<FA>(T, lod, second));
+  if (tex_filter<FA>(T) != FILT_ANISOTROPIC) {
+    sample_quad16<XF, FA>(T, L, u, v, lod, out);
     return;
   }
   int N = rr_round_int(aniso);
@@ -215,7 +216,7 @@ __device__ __forceinline__ void sample_aniso16(const TexParams &T, float u, floa
   int i = 0;
   do {
     uint32_t c[4];
-    sample_quad16<XF, FAST>(T, L, u0, v0, lod, c);
+    sample_quad16<XF, FA>(T, L, u0, v0, lod, c);
     u0 = u0 + du;
     v0 = v0 + dv;
 #pragma unroll
@@ -228,13 +229,13 @@ __device__ __forceinline__ void sample_aniso16(const TexParams &T, float u, floa
 
 // Float path: address (:2463-2638), sampleFloat2D (:1155-1248).
 // computeIndices (SamplerCore.cpp:2480-2630) for one axis; mode as address16.
-template <bool XF, bool FAST = false>
+template <bool XF, int FA = -1>
 __device__ __forceinline__ void address_float(const TexParams &T, float uvw, int dim, int mode, float lod, int &x0,
                                               int &x1, float &f) {
   int filterMask;
-  if (tex_filter<FAST>(T) == FILT_POINT) filterMask = 0;
-  else if (tex_filter<FAST>(T) == FILT_MIN_LINEAR_MAG_POINT) filterMask = !(lod <= 0.0f) ? -1 : 0;
-  else if (tex_filter<FAST>(T) == FILT_MIN_POINT_MAG_LINEAR) filterMask = (lod <= 0.0f) ? -1 : 0;
+  if (tex_filter<FA>(T) == FILT_POINT) filterMask = 0;
+  else if (tex_filter<FA>(T) == FILT_MIN_LINEAR_MAG_POINT) filterMask = !(lod <= 0.0f) ? -1 : 0;
+  else if (tex_filter<FA>(T) == FILT_MIN_POINT_MAG_LINEAR) filterMask = (lod <= 0.0f) ? -1 : 0;
   else filterMask = -1;
   const float one = __uint_as_float(0x3F7FFFFFu);  // just under 1
   float coord;
@@ -249,11 +250,11 @@ __device__ __forceinline__ void address_float(const TexParams &T, float uvw, int
     coord = rr_frac(uvw);
   }
   coord = coord * (float)dim;
-  if (tex_filter<FAST>(T) == FILT_POINT) {
+  if (tex_filter<FA>(T) == FILT_POINT) {
     x0 = rr_trunc_int(coord);
     f = 0.0f;
   } else {
-    if (tex_filter<FAST>(T) == FILT_MIN_POINT_MAG_LINEAR || tex_filter<FAST>(T) == FILT_MIN_LINEAR_MAG_POINT)
+    if (tex_filter<FA>(T) == FILT_MIN_POINT_MAG_LINEAR || tex_filter<FA>(T) == FILT_MIN_LINEAR_MAG_POINT)
       coord = coord - __uint_as_float(0x3F000000u & (uint32_t)filterMask);
     else
       coord = coord - 0.5f;
@@ -262,7 +263,7 @@ __device__ __forceinline__ void address_float(const TexParams &T, float uvw, int
     f = coord - fl;
   }
   x1 = x0 - filterMask;
-  if (tex_filter<FAST>(T) != FILT_POINT) {
+  if (tex_filter<FA>(T) != FILT_POINT) {
     if (mode != 0) {
       x0 = x0 > 0 ? x0 : 0;
       x1 = x1 < dim - 1 ? x1 : dim - 1;
@@ -277,15 +278,15 @@ __device__ __forceinline__ float4 fetchf(const TexLevel &L, int x, int yoff) {
   return reinterpret_cast<const float4 *>(L.data)[(uint32_t)(x + yoff)];
 }
 
-template <bool XF, bool FAST = false>
+template <bool XF, int FA = -1>
 __device__ __forceinline__ float4 sample_float2d(const TexParams &T, const TexLevel &L, float u, float v, float lod) {
   int x0, x1, y0, y1;
   float fu, fv;
-  address_float<XF, FAST>(T, u, L.width, tex_addr_u<FAST>(T), lod, x0, x1, fu);
-  address_float<XF, FAST>(T, v, L.height, tex_addr_v<FAST>(T), lod, y0, y1, fv);
+  address_float<XF, FA>(T, u, L.width, tex_addr_u<FA>(T), lod, x0, x1, fu);
+  address_float<XF, FA>(T, v, L.height, tex_addr_v<FA>(T), lod, y0, y1, fv);
   int pitch = L.pitch;
   y0 = (int)((uint32_t)y0 * (uint32_t)pitch);
-  if (tex_filter<FAST>(T) == FILT_POINT) return fetchf(L, x0, y0);
+  if (tex_filter<FA>(T) == FILT_POINT) return fetchf(L, x0, y0);
   y1 = (int)((uint32_t)y1 * (uint32_t)pitch);
   float4 c00 = fetchf(L, x0, y0), c10 = fetchf(L, x1, y0);
   float4 c01 = fetchf(L, x0, y1), c11 = fetchf(L, x1, y1);
@@ -298,11 +299,11 @@ __device__ __forceinline__ float4 sample_float2d(const TexParams &T, const TexLe
   return r;
 }
 
-template <bool XF, bool FAST = false>
+template <bool XF, int FA = -1>
 __device__ __forceinline__ float4 sample_float_aniso(const TexParams &T, float u, float v, float lod, float aniso,
                                                      float uDelta, float vDelta, int second) {
-  const TexLevel L = tex_level(T, select_mip<FAST>(T, lod, second));
-  if (tex_filter<FAST>(T) != FILT_ANISOTROPIC) return sample_float2d<XF, FAST>(T, L, u, v, lod);
+  const TexLevel L = tex_level(T, select_mip<FA>(T, lod, second));
+  if (tex_filter<FA>(T) != FILT_ANISOTROPIC) return sample_float2d<XF, FA>(T, L, u, v, lod);
   int N = rr_round_int(aniso);
   const int row = aniso_row(N);
   float A = uv_weight(row), B = uv_start(row);
@@ -314,7 +315,7 @@ __device__ __forceinline__ float4 sample_float_aniso(const TexParams &T, float u
   dv = dv * A;
   int i = 0;
   do {
-    float4 c = sample_float2d<XF, FAST>(T, L, u0, v0, lod);
+    float4 c = sample_float2d<XF, FA>(T, L, u0, v0, lod);
     u0 = u0 + du;
     v0 = v0 + dv;
     sum.x = sum.x + c.x * A;
@@ -332,13 +333,13 @@ struct QuadLod {
   float lod, aniso, uDelta, vDelta;
 };
 
-template <bool FAST = false>
+template <int FA = -1>
 __device__ __forceinline__ QuadLod quad_lod(const TexParams &T, const float u[4], const float v[4]) {
   QuadLod L;
   L.aniso = 0.0f;
   L.uDelta = 0.0f;
   L.vDelta = 0.0f;
-  if (tex_skip_lod<FAST>(T)) {
+  if (tex_skip_lod<FA>(T)) {
     L.lod = T.minLod;
     return L;
   }
@@ -347,7 +348,7 @@ __device__ __forceinline__ QuadLod quad_lod(const TexParams &T, const float u[4]
   float sx = dUdx * dUdx + dVdx * dVdx;
   float sy = dUdy * dUdy + dVdy * dVdy;
   float lod = rr_max(sx, sy);
-  if (tex_filter<FAST>(T) == FILT_ANISOTROPIC) {
+  if (tex_filter<FA>(T) == FILT_ANISOTROPIC) {
     float det = dUdx * dVdy - dUdy * dVdx;
     det = det > 0.0f ? det : -det;
     bool major = !(sx < sy);
@@ -368,12 +369,12 @@ __device__ __forceinline__ QuadLod quad_lod(const TexParams &T, const float u[4]
 
 // One lane of sampleTexture128 after the LOD is known (FTEX: RGBA32F
 // texture, 32-bit float path; otherwise RGBA8, 16-bit fixed-point path).
-template <bool FTEX, bool XF, bool FAST = false>
+template <bool FTEX, bool XF, int FA = -1>
 __device__ __forceinline__ float4 sample_lane(const TexParams &T, float u, float v, const QuadLod &L) {
   if (FTEX) {
-    float4 c = sample_float_aniso<XF, FAST>(T, u, v, L.lod, L.aniso, L.uDelta, L.vDelta, 0);
-    if (tex_mipmap<FAST>(T) == MIP_LINEAR) {
-      float4 cc = sample_float_aniso<XF, FAST>(T, u, v, L.lod, L.aniso, L.uDelta, L.vDelta, 1);
+    float4 c = sample_float_aniso<XF, FA>(T, u, v, L.lod, L.aniso, L.uDelta, L.vDelta, 0);
+    if (tex_mipmap<FA>(T) == MIP_LINEAR) {
+      float4 cc = sample_float_aniso<XF, FA>(T, u, v, L.lod, L.aniso, L.uDelta, L.vDelta, 1);
       float l4 = rr_frac(L.lod);
       c.x = (cc.x - c.x) * l4 + c.x;
       c.y = (cc.y - c.y) * l4 + c.y;
@@ -383,10 +384,10 @@ __device__ __forceinline__ float4 sample_lane(const TexParams &T, float u, float
     return c;
   }
   uint32_t c[4];
-  sample_aniso16<XF, FAST>(T, u, v, L.lod, L.aniso, L.uDelta, L.vDelta, 0, c);
-  if (tex_mipmap<FAST>(T) == MIP_LINEAR) {
+  sample_aniso16<XF, FA>(T, u, v, L.lod, L.aniso, L.uDelta, L.vDelta, 0, c);
+  if (tex_mipmap<FA>(T) == MIP_LINEAR) {
     uint32_t cc[4];
-    sample_aniso16<XF, FAST>(T, u, v, L.lod, L.aniso, L.uDelta, L.vDelta, 1, cc);
+    sample_aniso16<XF, FA>(T, u, v, L.lod, L.aniso, L.uDelta, L.vDelta, 1, cc);
     uint32_t utri = (uint32_t)rr_trunc_int(L.lod * 65536.0f) & 0xFFFFu;
     uint32_t inv = utri ^ 0xFFFFu;
 #pragma unroll

@@ -652,7 +652,7 @@ __device__ __forceinline__ float quad_rhw(const TriRecord &R, float xf, float yf
 // differences of u, v over the quad's pixels (0,0), (1,0), (0,1), helper
 // lanes included, evaluated from the plane equations (computeLod2D,
 // SamplerCore.cpp:1376-1422).
-template <bool FAST = false>
+template <int FA = -1>
 __device__ __forceinline__ QuadLod quad_state(const DrawParams &P, const TriRecord &R, int px, int py) {
   const int qx = px & ~1, qy = py & ~1;
   float uq[4], vq[4];
@@ -665,19 +665,20 @@ __device__ __forceinline__ QuadLod quad_state(const DrawParams &P, const TriReco
   }
   uq[3] = 0.0f;
   vq[3] = 0.0f;
-  return quad_lod<FAST>(P.tex, uq, vq);
+  return quad_lod<FA>(P.tex, uq, vq);
 }
 
 // Shades one queued fragment: the fixed shader family at pixel (px, py)
 // of primitive R, with the texture LOD of the 2x2 quad the pixel belongs
 // to (helper lanes evaluated from the plane equations).  Output clamped
 // for UNORM targets (PixelProgram::clampColor).
-// FAST: the draw's shader is MODULATE and its sampler state the common one
-// (tex_filter and the other readers of the sampler state).
-template <bool FCOL, bool FTEX, bool XF, bool FAST = false>
+// FA >= 0: the draw's shader is MODULATE and its sampler state the common
+// one with address mode FA (tex_filter and the other readers of the sampler
+// state).
+template <bool FCOL, bool FTEX, bool XF, int FA = -1>
 __device__ __forceinline__ float4 shade(const DrawParams &P, const TriRecord &R, int px, int py,
                                         bool clampUnorm = true) {
-  const int shader = FAST ? (int)SHADER_MODULATE : P.shader;
+  const int shader = FA >= 0 ? (int)SHADER_MODULATE : P.shader;
   const float xf = (float)px - R.x0, yf = (float)py - R.y0;
   const float rhw = quad_rhw(R, xf, yf);
   float4 col = make_float4(0, 0, 0, 0);
@@ -689,8 +690,8 @@ __device__ __forceinline__ float4 shade(const DrawParams &P, const TriRecord &R,
   }
   if (shader != SHADER_INTERPOLATE) {
     const float u = interp(R, 4, xf, yf, rhw), v = interp(R, 5, xf, yf, rhw);
-    const QuadLod Lq = quad_state<FAST>(P, R, px, py);
-    float4 c = sample_lane<FTEX, XF, FAST>(P.tex, u, v, Lq);
+    const QuadLod Lq = quad_state<FA>(P, R, px, py);
+    float4 c = sample_lane<FTEX, XF, FA>(P.tex, u, v, Lq);
     if (shader == SHADER_MODULATE) {
       c.x = c.x * col.x;
       c.y = c.y * col.y;
@@ -761,12 +762,14 @@ constexpr int kQueue = 64;  // deferred-shading queue entries per wave
 // workgroup has one wave per 8x8 block of the tile.
 // LEVEL 1: the draw has no stencil and no depth bias, tests and writes
 // depth and writes all four colour channels (the checks below fold away).
-// LEVEL 2: also the common shading state (shade's and blend_write's FAST).
+// LEVEL 2 (3): also the common shading state with REPEAT (CLAMP_TO_EDGE)
+// addressing (shade's FA, blend_write's FAST).
 template <int MS, bool FCOL, bool FTEX, bool XF, int LEVEL>
 __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32_t start, uint32_t end0,
                                             const TriRecord *__restrict__ rec, const int *__restrict__ poly,
                                             const uint32_t *__restrict__ binTri) {
   constexpr bool SIMPLE = LEVEL >= 1, FAST = LEVEL >= 2;
+  constexpr int FA = LEVEL >= 2 ? LEVEL - 2 : -1;
   const bool stencilActive = !SIMPLE && P.stencilActive, stencilWrite = !SIMPLE && P.stencilWrite;
   const bool depthBias = !SIMPLE && P.depthBias, depthTest = SIMPLE || P.depthTest;
   const bool depthWrite = SIMPLE || P.depthWrite;
@@ -849,7 +852,7 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
       const uint32_t t = qTri[wave][lane];
       const uint32_t pix = qPix[wave][lane];
       const int l = (int)(pix & 63u);
-      qCol[wave][lane] = shade<FCOL, FTEX, XF, FAST>(P, rec[t], ox + lane_cx(l), oy + lane_cy(l));
+      qCol[wave][lane] = shade<FCOL, FTEX, XF, FA>(P, rec[t], ox + lane_cx(l), oy + lane_cy(l));
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
@@ -964,7 +967,7 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
         // fragment is shaded again, identically, if it reaches the queue.
         if (cMask) {
           const uint32_t tj = __builtin_amdgcn_readlane(tid, j);
-          const float a = shade<FCOL, FTEX, XF, FAST>(P, rec[tj], px, py, P.color != nullptr).w;
+          const float a = shade<FCOL, FTEX, XF, FA>(P, rec[tj], px, py, P.color != nullptr).w;
           int aMask = 0;
 #pragma unroll
           for (int q = 0; q < MS; q++)
@@ -1273,8 +1276,10 @@ __global__ void clear_rect_kernel(char *base, int rowPitch, long long slice, int
 
 // Up to kClearBatch clears of one element size in one launch (a kernel
 // argument, so nothing is uploaded): rect k owns blocks [start[k],
-// start[k + 1]), each block 256 pixels of one row of one sample plane.
+// start[k + 1]), each block kClearPixels consecutive pixels of the rect's
+// pixels in (sample, row, column) order, kClearPixels / 256 per thread.
 constexpr int kClearBatch = 24;
+constexpr int kClearPixels = 1024;
 struct ClearRect {
   char *base;
   long long slice;
@@ -1293,16 +1298,20 @@ __global__ void clear_batch_kernel(ClearBatch cb) {
   int k = 0;
   while (k + 1 < cb.n && cb.start[k + 1] <= b) k++;
   const ClearRect &r = cb.r[k];
-  const uint32_t local = b - cb.start[k];
-  const uint32_t bx = (uint32_t)(r.w + 255) / 256u;
-  const uint32_t row = local / bx;  // row + h * sample
-  const int x = (int)(local % bx) * 256 + (int)threadIdx.x, y = (int)(row % (uint32_t)r.h), q = (int)(row / (uint32_t)r.h);
-  if (x >= r.w) return;
-  char *p = r.base + (long long)q * r.slice + (long long)(r.y0 + y) * r.pitch + (long long)BYTES * (r.x0 + x);
-  if (BYTES == 16) *reinterpret_cast<uint4 *>(p) = r.value;
-  else if (BYTES == 4) *reinterpret_cast<uint32_t *>(p) = r.value.x;
-  else if (BYTES == 2) *reinterpret_cast<uint16_t *>(p) = (uint16_t)r.value.x;
-  else *reinterpret_cast<uint8_t *>(p) = (uint8_t)r.value.x;
+  const uint64_t plane = (uint64_t)r.w * (uint64_t)r.h, total = plane * (uint64_t)r.samples;
+  const uint64_t first = (uint64_t)(b - cb.start[k]) * kClearPixels;
+#pragma unroll
+  for (int j = 0; j < kClearPixels / 256; j++) {
+    const uint64_t i = first + (uint64_t)threadIdx.x + 256u * (uint64_t)j;
+    if (i >= total) break;
+    const uint64_t q = i / plane, rem = i - q * plane;
+    const uint32_t y = (uint32_t)(rem / (uint64_t)r.w), x = (uint32_t)(rem - (uint64_t)y * (uint64_t)r.w);
+    char *p = r.base + (long long)q * r.slice + (long long)(r.y0 + (int)y) * r.pitch + (long long)BYTES * (r.x0 + (int)x);
+    if (BYTES == 16) *reinterpret_cast<uint4 *>(p) = r.value;
+    else if (BYTES == 4) *reinterpret_cast<uint32_t *>(p) = r.value.x;
+    else if (BYTES == 2) *reinterpret_cast<uint16_t *>(p) = (uint16_t)r.value.x;
+    else *reinterpret_cast<uint8_t *>(p) = (uint8_t)r.value.x;
+  }
 }
 
 // 4x resolve of an sRGB attachment: the reference's blit routine (Blitter.cpp
@@ -1410,6 +1419,14 @@ SWR_RASTER_INSTANCE(4, false, false, false, 2)
 SWR_RASTER_INSTANCE(4, false, true, false, 2)
 SWR_RASTER_INSTANCE(4, true, false, false, 2)
 SWR_RASTER_INSTANCE(4, true, true, false, 2)
+SWR_RASTER_INSTANCE(1, false, false, false, 3)
+SWR_RASTER_INSTANCE(1, false, true, false, 3)
+SWR_RASTER_INSTANCE(1, true, false, false, 3)
+SWR_RASTER_INSTANCE(1, true, true, false, 3)
+SWR_RASTER_INSTANCE(4, false, false, false, 3)
+SWR_RASTER_INSTANCE(4, false, true, false, 3)
+SWR_RASTER_INSTANCE(4, true, false, false, 3)
+SWR_RASTER_INSTANCE(4, true, true, false, 3)
 #undef SWR_RASTER_INSTANCE
 
 }  // namespace swr
