@@ -823,12 +823,45 @@ int ensure_pipe(Pipeline &pipe, uint32_t T, uint32_t cap, uint32_t tiles) {
   return SWR_OK;
 }
 
-// Exclusive prefix sum of n u32 (in may equal out).
-void enqueue_scan(const uint32_t *in, uint32_t *out, size_t n, uint32_t *sums, hipStream_t s) {
+// Exclusive prefix sum of n u32 (in may equal out): two launches up to
+// kFusedScanBlocks blocks, three above.  With `first`, the emission index of
+// pair capacity `capacity` is built as well (emit_index_kernel's work; in
+// must then differ from out and n be the slot count + 1).
+void enqueue_scan(const uint32_t *in, uint32_t *out, size_t n, uint32_t *sums, hipStream_t s,
+                  uint32_t capacity = 0, uint32_t *first = nullptr, uint32_t *split = nullptr) {
   const uint32_t blocks = (uint32_t)((n + kScanItems - 1) / kScanItems);
   hipLaunchKernelGGL(scan_partial_kernel, dim3(blocks), dim3(256), 0, s, in, out, (uint32_t)n, sums);
-  hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(1024), 0, s, sums, blocks);
-  hipLaunchKernelGGL(scan_add_kernel, dim3(blocks), dim3(256), 0, s, out, (uint32_t)n, (const uint32_t *)sums);
+  if (blocks <= kFusedScanBlocks) {
+    hipLaunchKernelGGL(scan_finish_kernel, dim3(blocks), dim3(256), 0, s, out, (uint32_t)n, (const uint32_t *)sums,
+                       in, capacity, first, split);
+  } else {
+    hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(1024), 0, s, sums, blocks);
+    hipLaunchKernelGGL(scan_add_kernel, dim3(blocks), dim3(256), 0, s, out, (uint32_t)n, (const uint32_t *)sums);
+    if (first) {
+      const uint32_t T = (uint32_t)n - 1;  // >= kScanItems here
+      hipLaunchKernelGGL(emit_index_kernel, dim3((T + 255) / 256), dim3(256), 0, s, (const uint32_t *)out, T,
+                         capacity, first, split);
+    }
+  }
+}
+
+// One pass of the stable LSD radix sort of *count (key, value) pairs in
+// nb blocks of kSortItems, on bits [shift, shift + 8) of the keys.
+void enqueue_radix_pass(const uint32_t *kIn, const uint32_t *vIn, const uint32_t *count, int shift, uint32_t nb,
+                        uint32_t *hist, uint32_t *sums, uint32_t *kOut, uint32_t *vOut, hipStream_t s) {
+  hipLaunchKernelGGL(radix_hist_kernel, dim3(nb), dim3(256), 0, s, kIn, count, shift, hist, nb);
+  const size_t n = (size_t)256 * nb;
+  const uint32_t blocks = (uint32_t)((n + kScanItems - 1) / kScanItems);
+  if (blocks <= kFusedScanBlocks) {
+    hipLaunchKernelGGL(scan_partial_kernel, dim3(blocks), dim3(256), 0, s, (const uint32_t *)hist, hist, (uint32_t)n,
+                       sums);
+    hipLaunchKernelGGL(radix_scatter_kernel, dim3(nb), dim3(256), 0, s, kIn, vIn, count, shift,
+                       (const uint32_t *)hist, nb, (const uint32_t *)sums, blocks, kOut, vOut);
+  } else {
+    enqueue_scan(hist, hist, n, sums, s);
+    hipLaunchKernelGGL(radix_scatter_kernel, dim3(nb), dim3(256), 0, s, kIn, vIn, count, shift,
+                       (const uint32_t *)hist, nb, (const uint32_t *)nullptr, 0u, kOut, vOut);
+  }
 }
 
 // Arguments of one raster launch over tiles [tLo, tHi) of draws [lo, hi).
@@ -868,13 +901,18 @@ void launch_raster_variant(int v, dim3 grid, dim3 block, hipStream_t s, const Ra
 }
 
 // Setup and tile-count prefix sum of one segment (off[T] = pair count).
-void enqueue_front(Pipeline &pipe, const BatchLayout &L, const BatchLayout::Seg &sg, hipStream_t s) {
+// With a pair capacity `cap` > 0 the scan also builds the emission index for
+// it (enqueue_back is then told so); `zeroed`: cnt[T] is already zero.
+void enqueue_front(Pipeline &pipe, const BatchLayout &L, const BatchLayout::Seg &sg, hipStream_t s,
+                   uint32_t cap = 0, bool zeroed = false) {
   const DrawParams *params = pipe.params.as<DrawParams>() + sg.first;
   const uint32_t *blockBase = pipe.aux.as<uint32_t>() + sg.first;
   hipLaunchKernelGGL(setup_batch_kernel, dim3(sg.blocks), dim3(256), 0, s, params, blockBase, sg.count,
                      pipe.rec.as<TriRecord>(), pipe.poly.as<int>(), pipe.cnt.as<uint32_t>());
-  hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(1), 0, s, pipe.cnt.as<uint32_t>() + sg.T, 0u, 1u);
-  enqueue_scan(pipe.cnt.as<uint32_t>(), pipe.off.as<uint32_t>(), (size_t)sg.T + 1, pipe.sums.as<uint32_t>(), s);
+  if (!zeroed) hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(1), 0, s, pipe.cnt.as<uint32_t>() + sg.T, 0u, 1u);
+  enqueue_scan(pipe.cnt.as<uint32_t>(), pipe.off.as<uint32_t>(), (size_t)sg.T + 1, pipe.sums.as<uint32_t>(), s, cap,
+               cap ? pipe.first.as<uint32_t>() : nullptr,
+               cap ? pipe.counters.as<uint32_t>() + kSplitSlot : nullptr);
 }
 
 // Pair emission, sort, ranges and raster of one segment with pair capacity
@@ -882,14 +920,19 @@ void enqueue_front(Pipeline &pipe, const BatchLayout &L, const BatchLayout::Seg 
 // since it was built), the total needed goes to *overflowFlag and, with
 // `unbinned`, the primitives whose pairs did not fit are rendered by an
 // unbinned raster pass after the binned one, so that none is dropped.
+// `indexed`: enqueue_front built the emission index for `cap`; `zeroed`:
+// the tile ranges and the raster's work counters are already zero.
 int enqueue_back(DeviceCtx *c, Pipeline &pipe, const BatchLayout &L, const BatchLayout::Seg &sg, uint32_t cap,
-                 const swr_tuning &tuning, uint32_t *overflowFlag, bool unbinned, EventSet *es, hipStream_t s) {
+                 const swr_tuning &tuning, uint32_t *overflowFlag, bool unbinned, EventSet *es, hipStream_t s,
+                 bool indexed = false, bool zeroed = false) {
   const DrawParams *params = pipe.params.as<DrawParams>() + sg.first;
   const uint32_t *tileBase = pipe.aux.as<uint32_t>() + L.P.size() + sg.first;
   uint32_t *split = pipe.counters.as<uint32_t>() + kSplitSlot;
   uint32_t *emitted = pipe.counters.as<uint32_t>() + kEmittedSlot;
   if (cap > 0) {
-    if (sg.T > 0)
+    if (indexed)
+      ;
+    else if (sg.T > 0)
       hipLaunchKernelGGL(emit_index_kernel, dim3((sg.T + 255) / 256), dim3(256), 0, s,
                          (const uint32_t *)pipe.off.as<uint32_t>(), sg.T, cap, pipe.first.as<uint32_t>(), split);
     else
@@ -903,18 +946,16 @@ int enqueue_back(DeviceCtx *c, Pipeline &pipe, const BatchLayout &L, const Batch
     uint32_t *kIn = pipe.keysA.as<uint32_t>(), *vIn = pipe.valsA.as<uint32_t>();
     uint32_t *kOut = pipe.keysB.as<uint32_t>(), *vOut = pipe.valsB.as<uint32_t>();
     for (uint32_t p = 0; p < passes; p++) {
-      hipLaunchKernelGGL(radix_hist_kernel, dim3(nb), dim3(256), 0, s, (const uint32_t *)kIn,
-                         (const uint32_t *)emitted, (int)(8 * p), pipe.hist.as<uint32_t>(), nb);
-      enqueue_scan(pipe.hist.as<uint32_t>(), pipe.hist.as<uint32_t>(), (size_t)256 * nb, pipe.sums.as<uint32_t>(), s);
-      hipLaunchKernelGGL(radix_scatter_kernel, dim3(nb), dim3(256), 0, s, (const uint32_t *)kIn, (const uint32_t *)vIn,
-                         (const uint32_t *)emitted, (int)(8 * p), (const uint32_t *)pipe.hist.as<uint32_t>(), nb,
-                         kOut, vOut);
+      enqueue_radix_pass(kIn, vIn, emitted, (int)(8 * p), nb, pipe.hist.as<uint32_t>(), pipe.sums.as<uint32_t>(), kOut,
+                         vOut, s);
       std::swap(kIn, kOut);
       std::swap(vIn, vOut);
     }
     const uint32_t tb = (sg.tiles + 255) / 256;
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(tb), dim3(256), 0, s, pipe.tStart.as<uint32_t>(), 0u, sg.tiles);
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(tb), dim3(256), 0, s, pipe.tEnd.as<uint32_t>(), 0u, sg.tiles);
+    if (!zeroed) {
+      hipLaunchKernelGGL(fill_u32_kernel, dim3(tb), dim3(256), 0, s, pipe.tStart.as<uint32_t>(), 0u, sg.tiles);
+      hipLaunchKernelGGL(fill_u32_kernel, dim3(tb), dim3(256), 0, s, pipe.tEnd.as<uint32_t>(), 0u, sg.tiles);
+    }
     hipLaunchKernelGGL(tile_range_batch_kernel, dim3((cap + 255) / 256), dim3(256), 0, s, (const uint32_t *)kIn,
                        (const uint32_t *)emitted, sg.tiles, pipe.tStart.as<uint32_t>(), pipe.tEnd.as<uint32_t>());
     SWR_HIP_CHECK(hipGetLastError());
@@ -925,7 +966,7 @@ int enqueue_back(DeviceCtx *c, Pipeline &pipe, const BatchLayout &L, const Batch
     uint32_t *counters = pipe.counters.as<uint32_t>();
     const bool persistent = tuning.raster_workgroups_per_cu != ~0u;
     const uint32_t nv = (uint32_t)sg.variants.size();
-    if (persistent)
+    if (persistent && !zeroed)
       hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, counters, 0u, unbinned ? 2 * nv : nv);
     // Binned pass, then (plans) the unbinned pass of the slots left out.
     for (int pass = 0; pass < (unbinned ? 2 : 1); pass++) {
@@ -979,7 +1020,19 @@ int check_tuning(const swr_tuning &t) {
   return SWR_OK;
 }
 
-__global__ void store_params_kernel(DrawParams P, DrawParams *dst, uint32_t *aux);
+__global__ void store_params_kernel(DrawParams P, DrawParams *dst, uint32_t *aux, uint32_t *cntEnd, uint32_t *counters,
+                                    uint32_t nCounters, uint32_t *ts, uint32_t *te, uint32_t tiles);
+
+// store_params_kernel's launch for segment sg of pipe, which also clears
+// what the front and back ends would otherwise clear with fills: cnt[T],
+// the tile ranges and the raster's work counters.
+void enqueue_store_params(const Pipeline &pipe, const DrawParams &P, const BatchLayout::Seg &sg, uint32_t nCounters,
+                          hipStream_t s) {
+  const uint32_t blocks = std::max<uint32_t>((sg.tiles + 255) / 256, 1u);
+  hipLaunchKernelGGL(store_params_kernel, dim3(blocks), dim3(256), 0, s, P, pipe.params.as<DrawParams>(),
+                     pipe.aux.as<uint32_t>(), pipe.cnt.as<uint32_t>() + sg.T, pipe.counters.as<uint32_t>(), nCounters,
+                     pipe.tStart.as<uint32_t>(), pipe.tEnd.as<uint32_t>(), sg.tiles);
+}
 
 // A single draw with a known pair capacity: parameters written by a kernel
 // (no staging buffer to wait for), pairs beyond the capacity rendered by
@@ -992,12 +1045,11 @@ int draw_nowait(DeviceCtx *c, const BatchLayout &L, const swr_tuning &tuning, hi
   SWR_HIP_CHECK(pipe.aux.ensure(sizeof(uint32_t) * 2));
   int r = ensure_pipe(pipe, sg.T, c->pairCap, sg.tiles);
   if (r != SWR_OK) return r;
-  hipLaunchKernelGGL(store_params_kernel, dim3(1), dim3(256), 0, s, L.P[0], pipe.params.as<DrawParams>(),
-                     pipe.aux.as<uint32_t>());
-  enqueue_front(pipe, L, sg, s);
+  enqueue_store_params(pipe, L.P[0], sg, 2, s);
+  enqueue_front(pipe, L, sg, s, c->pairCap, true);
   SWR_HIP_CHECK(hipGetLastError());
-  r = enqueue_back(c, pipe, L, sg, c->pairCap, tuning, pipe.counters.as<uint32_t>() + kOverflowSlot, true, nullptr,
-                   s);
+  r = enqueue_back(c, pipe, L, sg, c->pairCap, tuning, pipe.counters.as<uint32_t>() + kOverflowSlot, true, nullptr, s,
+                   true, true);
   if (r != SWR_OK) return r;
   if (!c->pairsPending) {
     if (!c->pairsEvent) SWR_HIP_CHECK(hipEventCreateWithFlags(&c->pairsEvent, hipEventDisableTiming));
@@ -1128,13 +1180,22 @@ size_t place_pipe(char *base, Pipeline &pipe, uint32_t n, uint32_t T, uint32_t c
 // Writes a one-draw segment's parameters and bases into the scratch (a
 // kernel, so that nothing goes through host memory).
 static_assert(sizeof(DrawParams) % 4 == 0 && sizeof(DrawParams) + 32 <= 4096, "DrawParams as a kernel argument");
-__global__ void store_params_kernel(DrawParams P, DrawParams *dst, uint32_t *aux) {
+__global__ void store_params_kernel(DrawParams P, DrawParams *dst, uint32_t *aux, uint32_t *cntEnd, uint32_t *counters,
+                                    uint32_t nCounters, uint32_t *ts, uint32_t *te, uint32_t tiles) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < tiles) {
+    ts[t] = 0u;
+    te[t] = 0u;
+  }
+  if (blockIdx.x != 0) return;
   const uint32_t *src = reinterpret_cast<const uint32_t *>(&P);
   uint32_t *d = reinterpret_cast<uint32_t *>(dst);
   for (uint32_t i = threadIdx.x; i < sizeof(DrawParams) / 4; i += blockDim.x) d[i] = src[i];
+  if (threadIdx.x < nCounters) counters[threadIdx.x] = 0u;
   if (threadIdx.x == 0) {
     aux[0] = 0u;  // blockBase
     aux[1] = 0u;  // tileBase
+    *cntEnd = 0u;
   }
 }
 
@@ -1175,15 +1236,15 @@ int draw_noalloc(const swr_draw_desc *desc, hipStream_t s) {
                 sg.T, sg.tiles, cap, kStaticScratchBytes, kStaticMinPairs);
   const StaticDev &sd = g_static[dev];
   place_pipe(sd.base, pipe, 1, sg.T, cap, sg.tiles);
-  hipLaunchKernelGGL(store_params_kernel, dim3(1), dim3(256), 0, s, L.P[0], pipe.params.as<DrawParams>(),
-                     pipe.aux.as<uint32_t>());
+  enqueue_store_params(pipe, L.P[0], sg, 2, s);
   SWR_HIP_CHECK(hipGetLastError());
-  enqueue_front(pipe, L, sg, s);
+  enqueue_front(pipe, L, sg, s, cap, true);
   SWR_HIP_CHECK(hipGetLastError());
   DeviceCtx dc;  // enqueue_back reads only the CU count from it
   dc.device = dev;
   dc.cus = sd.cus;
-  return enqueue_back(&dc, pipe, L, sg, cap, tuning, pipe.counters.as<uint32_t>() + kOverflowSlot, true, nullptr, s);
+  return enqueue_back(&dc, pipe, L, sg, cap, tuning, pipe.counters.as<uint32_t>() + kOverflowSlot, true, nullptr, s,
+                      true, true);
 }
 
 // Validates a gradient descriptor against its draw and derives GradParams.
@@ -1337,8 +1398,9 @@ int plan_build(DeviceCtx *c, swr_batch_plan *pl, hipStream_t s) {
 int plan_enqueue(DeviceCtx *c, swr_batch_plan *pl, hipStream_t s) {
   for (size_t i = 0; i < pl->L.segs.size(); i++) {
     const auto &sg = pl->L.segs[i];
-    enqueue_front(pl->pipe, pl->L, sg, s);
-    int r = enqueue_back(c, pl->pipe, pl->L, sg, sg.pairCap, pl->tuning, pl->flags.as<uint32_t>() + i, true, nullptr, s);
+    enqueue_front(pl->pipe, pl->L, sg, s, sg.pairCap);
+    int r = enqueue_back(c, pl->pipe, pl->L, sg, sg.pairCap, pl->tuning, pl->flags.as<uint32_t>() + i, true, nullptr, s,
+                         sg.pairCap > 0);
     if (r != SWR_OK) return r;
   }
   SWR_HIP_CHECK(hipGetLastError());
@@ -1651,11 +1713,7 @@ int plan_enqueue_backward(swr_batch_plan *pl, int cus, hipStream_t s) {
       uint32_t *hist = pl->binHist.as<uint32_t>(), *sums = pl->binSums.as<uint32_t>();
       const uint32_t *count = binCtr;
       for (uint32_t p = 0; p < sort_passes(sb.nBuckets); p++) {
-        hipLaunchKernelGGL(radix_hist_kernel, dim3(nbSort), dim3(256), 0, s, (const uint32_t *)kIn, count, (int)(8 * p),
-                           hist, nbSort);
-        enqueue_scan(hist, hist, (size_t)256 * nbSort, sums, s);
-        hipLaunchKernelGGL(radix_scatter_kernel, dim3(nbSort), dim3(256), 0, s, (const uint32_t *)kIn,
-                           (const uint32_t *)vIn, count, (int)(8 * p), (const uint32_t *)hist, nbSort, kOut, vOut);
+        enqueue_radix_pass(kIn, vIn, count, (int)(8 * p), nbSort, hist, sums, kOut, vOut, s);
         std::swap(kIn, kOut);
         std::swap(vIn, vOut);
       }

@@ -7,8 +7,11 @@
 //
 //   setup_batch_kernel    setup of every primitive of every draw
 //   scan_*                exclusive prefix sum of the per-primitive tile
-//                         counts (three passes, no memset, fixed sizes)
+//                         counts (block scans, then each block adds the
+//                         totals before it; no memset, fixed sizes)
 //   emit_index_kernel     the first primitive slot of every 256-pair block
+//                         (done by the scan's second launch when the pair
+//                         capacity is known beforehand)
 //   bin_emit_batch_kernel one thread per (tile, primitive) pair of the
 //                         primitives' bounding boxes up to a fixed capacity,
 //                         with the 8x8 blocks of the tile the primitive's
@@ -16,8 +19,9 @@
 //                         entries past the real count, get the key
 //                         0xFFFFFFFF so that they sort last
 //   radix_*               stable LSD radix sort of the pairs by tile, 8 bits
-//                         per pass: per-block digit histograms, a scan, and
-//                         an in-order scatter (waves rank their keys with
+//                         per pass: per-block digit histograms, their block
+//                         scans, and an in-order scatter that adds the block
+//                         totals itself (waves rank their keys with
 //                         ballots, so equal keys keep list order, which
 //                         keeps every tile's primitives in API order; the
 //                         block is staged in LDS sorted, then written in
@@ -127,6 +131,53 @@ __global__ __launch_bounds__(256) void scan_add_kernel(uint32_t *__restrict__ ou
     if (base + k < n) out[base + k] += add;
 }
 
+// Scans of at most this many blocks finish in one launch after
+// scan_partial_kernel: each block sums the totals of the blocks before it
+// itself (scan_finish_kernel), and the radix scatter scans the block totals
+// of its histogram in LDS.
+constexpr uint32_t kFusedScanBlocks = 1024;
+
+// Emission index of the pairs of slot g, whose pairs are [lo, hi0) (see
+// emit_index_kernel).
+__device__ __forceinline__ void emit_slot(uint32_t g, uint32_t lo, uint32_t hi0, uint32_t capacity,
+                                          uint32_t *__restrict__ first, uint32_t *__restrict__ split) {
+  const uint32_t hi = hi0 < capacity ? hi0 : capacity;
+  for (uint32_t b = (lo + kEmitBlock - 1) / kEmitBlock; b * kEmitBlock < hi; b++) first[b] = g;
+  if (lo <= capacity && capacity < hi0) *split = g;
+}
+
+// Second launch of a scan of n = T + 1 counts `in` (in != out): block b adds
+// the totals of blocks [0, b) to its items of out.  With `first`, it also
+// does emit_index_kernel's work for slots [0, T) (slot g's pairs end at
+// out[g] + in[g]); the last item (the total) writes *split = T when
+// everything fits below `capacity`.
+__global__ __launch_bounds__(256) void scan_finish_kernel(uint32_t *__restrict__ out, uint32_t n,
+                                                          const uint32_t *__restrict__ sums,
+                                                          const uint32_t *__restrict__ in, uint32_t capacity,
+                                                          uint32_t *__restrict__ first, uint32_t *__restrict__ split) {
+  __shared__ uint32_t wsum[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t p = 0;
+  for (uint32_t i = threadIdx.x; i < blockIdx.x; i += 256) p += sums[i];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) p += __shfl_xor(p, o, 64);
+  if (lane == 0) wsum[wave] = p;
+  __syncthreads();
+  const uint32_t add = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+  const uint32_t base = blockIdx.x * kScanItems + threadIdx.x * 4;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const uint32_t g = base + k;
+    if (g >= n) break;
+    const uint32_t v = out[g] + add;
+    out[g] = v;
+    if (first) {
+      if (g + 1 < n) emit_slot(g, v, v + in[g], capacity, first, split);
+      else if (v <= capacity) *split = g;
+    }
+  }
+}
+
 // ---- pair emission ---------------------------------------------------------
 // off: exclusive prefix of the tile counts over the T record slots (off[T]
 // = total pairs).  Pair k belongs to the slot g with off[g] <= k <
@@ -146,10 +197,8 @@ __global__ __launch_bounds__(256) void emit_index_kernel(const uint32_t *__restr
                                                          uint32_t *__restrict__ split) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= T) return;
-  const uint32_t lo = off[g], hi0 = off[g + 1];
-  const uint32_t hi = hi0 < capacity ? hi0 : capacity;
-  for (uint32_t b = (lo + kEmitBlock - 1) / kEmitBlock; b * kEmitBlock < hi; b++) first[b] = g;
-  if (lo <= capacity && capacity < hi0) *split = g;
+  const uint32_t hi0 = off[g + 1];
+  emit_slot(g, off[g], hi0, capacity, first, split);
   if (g == T - 1 && hi0 <= capacity) *split = T;
 }
 
@@ -297,10 +346,16 @@ __global__ __launch_bounds__(256) void radix_hist_kernel(const uint32_t *__restr
 // block sorted by digit, the block is staged in LDS in that order, and
 // written out with consecutive threads on consecutive positions of each
 // digit's run.
+//
+// histScan is the exclusive scan of the digit-major histogram; with
+// chunkSums it is only scanned within blocks of kScanItems entries
+// (scan_partial_kernel), chunkSums holding the m block totals, and each
+// workgroup scans those totals itself (m <= kFusedScanBlocks).
 __global__ __launch_bounds__(256) void radix_scatter_kernel(const uint32_t *__restrict__ keysIn,
                                                             const uint32_t *__restrict__ valsIn,
                                                             const uint32_t *__restrict__ count, int shift,
                                                             const uint32_t *__restrict__ histScan, uint32_t nb,
+                                                            const uint32_t *__restrict__ chunkSums, uint32_t m,
                                                             uint32_t *__restrict__ keysOut,
                                                             uint32_t *__restrict__ valsOut) {
   constexpr int kSteps = kSortItems / 256;
@@ -313,7 +368,29 @@ __global__ __launch_bounds__(256) void radix_scatter_kernel(const uint32_t *__re
   __shared__ uint32_t skey[kSortItems], sval[kSortItems];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int i = threadIdx.x; i < 4 * 256; i += 256) (&cnt[0][0])[i] = 0u;
-  gstart[threadIdx.x] = histScan[threadIdx.x * nb + blockIdx.x];
+  {
+    const uint32_t e = threadIdx.x * nb + blockIdx.x;
+    uint32_t g = histScan[e];
+    if (chunkSums) {
+      // Exclusive scan of the block totals into skey (free until the keys
+      // are staged), 256 at a time.
+      uint32_t carry = 0;
+      for (uint32_t c0 = 0; c0 < m; c0 += 256) {
+        const uint32_t i = c0 + threadIdx.x;
+        const uint32_t v = i < m ? chunkSums[i] : 0u;
+        const uint32_t incl = wave_incl_scan(v, lane);
+        if (lane == 63) wsum[wave] = incl;
+        __syncthreads();
+        uint32_t pre = carry;
+        for (int w = 0; w < wave; w++) pre += wsum[w];
+        if (i < m) skey[i] = pre + incl - v;
+        carry += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        __syncthreads();
+      }
+      g += skey[e / kScanItems];
+    }
+    gstart[threadIdx.x] = g;
+  }
   __syncthreads();
 
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
