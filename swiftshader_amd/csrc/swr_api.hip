@@ -770,8 +770,8 @@ int upload_layout(DeviceCtx *c, Pipeline &pipe, const BatchLayout &L, hipStream_
 }
 
 // Device words of Pipeline::counters: the persistent raster workgroups'
-// tile counters in [0, 80) (one per raster variant and pass: at most 40
-// variants x 2 passes), the eager draws' overflow word, the segment's split
+// tile counters in [0, 80) (one per raster variant: at most 40 variants),
+// the eager draws' overflow word, the segment's split
 // slot, the number of pairs the segment emitted and the backward's two work
 // counters.
 constexpr int kOverflowSlot = 80;
@@ -877,12 +877,14 @@ struct RasterArgs {
   const uint32_t *ts, *te;
   const uint32_t *split;
   uint32_t segT;
+  int tail;  // with bin: each tile also walks the slots from *split on
 };
 
 template <int MS, bool FCOL, bool FTEX, bool XF, int LEVEL>
 void launch_raster_batch(dim3 grid, dim3 block, hipStream_t s, const RasterArgs &a) {
   hipLaunchKernelGGL((raster_batch_kernel<MS, FCOL, FTEX, XF, LEVEL>), grid, block, 0, s, a.params, a.tileBase, a.lo,
-                     a.hi, a.tLo, a.tHi, a.counter, a.rec, a.poly, a.bin, a.ts, a.te, a.split, a.segT);
+                     a.hi, a.tLo, a.tHi, a.counter, a.rec, a.poly, a.bin, a.ts, a.te, a.split, a.segT,
+                     a.tail);
 }
 
 // The raster kernel of variant v (variant_of).
@@ -966,43 +968,42 @@ int enqueue_back(DeviceCtx *c, Pipeline &pipe, const BatchLayout &L, const Batch
     uint32_t *counters = pipe.counters.as<uint32_t>();
     const bool persistent = tuning.raster_workgroups_per_cu != ~0u;
     const uint32_t nv = (uint32_t)sg.variants.size();
-    if (persistent && !zeroed)
-      hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, counters, 0u, unbinned ? 2 * nv : nv);
-    // Binned pass, then (plans) the unbinned pass of the slots left out.
-    for (int pass = 0; pass < (unbinned ? 2 : 1); pass++) {
-      for (uint32_t v = 0; v < nv; v++) {
-        const int lo = sg.variants[v].first, hi = sg.variants[v].second;
-        const uint32_t tLo = L.tileBase[sg.first + lo];
-        const uint32_t tHi = hi < sg.count ? L.tileBase[sg.first + hi] : sg.tiles;
-        if (tHi <= tLo) continue;
-        uint32_t wgs = tHi - tLo;
-        if (persistent) {
-          const uint32_t per = tuning.raster_workgroups_per_cu ? tuning.raster_workgroups_per_cu : 4u;
-          wgs = std::min<uint32_t>(wgs, (uint32_t)c->cus * per);
-        }
-        RasterArgs a;
-        a.params = params;
-        a.tileBase = tileBase;
-        a.lo = lo;
-        a.hi = hi;
-        a.tLo = tLo;
-        a.tHi = tHi;
-        a.counter = persistent ? counters + pass * nv + v : nullptr;
-        a.rec = pipe.rec.as<TriRecord>();
-        a.poly = pipe.poly.as<int>();
-        a.bin = pass == 0 ? vIn : nullptr;
-        a.ts = pipe.tStart.as<uint32_t>();
-        a.te = pipe.tEnd.as<uint32_t>();
-        a.split = split;
-        a.segT = sg.T;
-        const int var = variant_of(L.P[sg.first + lo]);
-        switch (var & 8 ? -1 : var >> 4) {
-          case -1: launch_raster_variant<true, 0>(var, dim3(wgs), block, s, a); break;
-          case 3: launch_raster_variant<false, 3>(var, dim3(wgs), block, s, a); break;
-          case 2: launch_raster_variant<false, 2>(var, dim3(wgs), block, s, a); break;
-          case 1: launch_raster_variant<false, 1>(var, dim3(wgs), block, s, a); break;
-          default: launch_raster_variant<false, 0>(var, dim3(wgs), block, s, a); break;
-        }
+    if (persistent && !zeroed) hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, counters, 0u, nv);
+    // One launch per variant; with `unbinned`, each tile renders its binned
+    // list and then the slots left out of it.
+    for (uint32_t v = 0; v < nv; v++) {
+      const int lo = sg.variants[v].first, hi = sg.variants[v].second;
+      const uint32_t tLo = L.tileBase[sg.first + lo];
+      const uint32_t tHi = hi < sg.count ? L.tileBase[sg.first + hi] : sg.tiles;
+      if (tHi <= tLo) continue;
+      uint32_t wgs = tHi - tLo;
+      if (persistent) {
+        const uint32_t per = tuning.raster_workgroups_per_cu ? tuning.raster_workgroups_per_cu : 4u;
+        wgs = std::min<uint32_t>(wgs, (uint32_t)c->cus * per);
+      }
+      RasterArgs a;
+      a.params = params;
+      a.tileBase = tileBase;
+      a.lo = lo;
+      a.hi = hi;
+      a.tLo = tLo;
+      a.tHi = tHi;
+      a.counter = persistent ? counters + v : nullptr;
+      a.rec = pipe.rec.as<TriRecord>();
+      a.poly = pipe.poly.as<int>();
+      a.bin = vIn;
+      a.tail = unbinned ? 1 : 0;
+      a.ts = pipe.tStart.as<uint32_t>();
+      a.te = pipe.tEnd.as<uint32_t>();
+      a.split = split;
+      a.segT = sg.T;
+      const int var = variant_of(L.P[sg.first + lo]);
+      switch (var & 8 ? -1 : var >> 4) {
+        case -1: launch_raster_variant<true, 0>(var, dim3(wgs), block, s, a); break;
+        case 3: launch_raster_variant<false, 3>(var, dim3(wgs), block, s, a); break;
+        case 2: launch_raster_variant<false, 2>(var, dim3(wgs), block, s, a); break;
+        case 1: launch_raster_variant<false, 1>(var, dim3(wgs), block, s, a); break;
+        default: launch_raster_variant<false, 0>(var, dim3(wgs), block, s, a); break;
       }
     }
   } else if (es) {
@@ -1045,7 +1046,7 @@ int draw_nowait(DeviceCtx *c, const BatchLayout &L, const swr_tuning &tuning, hi
   SWR_HIP_CHECK(pipe.aux.ensure(sizeof(uint32_t) * 2));
   int r = ensure_pipe(pipe, sg.T, c->pairCap, sg.tiles);
   if (r != SWR_OK) return r;
-  enqueue_store_params(pipe, L.P[0], sg, 2, s);
+  enqueue_store_params(pipe, L.P[0], sg, 1, s);
   enqueue_front(pipe, L, sg, s, c->pairCap, true);
   SWR_HIP_CHECK(hipGetLastError());
   r = enqueue_back(c, pipe, L, sg, c->pairCap, tuning, pipe.counters.as<uint32_t>() + kOverflowSlot, true, nullptr, s,
@@ -1236,7 +1237,7 @@ int draw_noalloc(const swr_draw_desc *desc, hipStream_t s) {
                 sg.T, sg.tiles, cap, kStaticScratchBytes, kStaticMinPairs);
   const StaticDev &sd = g_static[dev];
   place_pipe(sd.base, pipe, 1, sg.T, cap, sg.tiles);
-  enqueue_store_params(pipe, L.P[0], sg, 2, s);
+  enqueue_store_params(pipe, L.P[0], sg, 1, s);
   SWR_HIP_CHECK(hipGetLastError());
   enqueue_front(pipe, L, sg, s, cap, true);
   SWR_HIP_CHECK(hipGetLastError());

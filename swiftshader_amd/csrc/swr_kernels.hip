@@ -1205,7 +1205,10 @@ __device__ __forceinline__ void raster_tile(const DrawParams &P, int bin, uint32
 // slots of its draw from *split on (the primitives the binned pass left out
 // because their pairs exceeded the planned capacity) in slot order, testing
 // each against its blocks as it would a binned list.  It does nothing when
-// *split >= segT (everything was binned).
+// *split >= segT (everything was binned).  With binTri and `tail`, each tile
+// walks its binned list and then those slots, so that one launch does both
+// passes (the tiles of one launch never share pixels, so only the order
+// within a tile matters).
 // Occupancy: 4 waves per SIMD (128 VGPRs) for the single-sample kernels
 // (the float-colour ones spill 12 bytes there and still run faster than at
 // 3 waves), 3 for the multisample ones.
@@ -1220,10 +1223,13 @@ __global__ __launch_bounds__(256, MS == 1 ? 4 : 3) void raster_batch_kernel(
                                                            const uint32_t *__restrict__ binTri,
                                                            const uint32_t *__restrict__ tileStart,
                                                            const uint32_t *__restrict__ tileEnd,
-                                                           const uint32_t *__restrict__ split, uint32_t segT) {
+                                                           const uint32_t *__restrict__ split, uint32_t segT,
+                                                           int tail) {
   __shared__ uint32_t sTile;
-  const uint32_t from = binTri ? 0u : *split;
+  const uint32_t from = (binTri && !tail) ? segT : *split;
   if (!binTri && from >= segT) return;
+  // Passes per tile: 0 = the binned list, 1 = the slots from *split on.
+  const int pass0 = binTri ? 0 : 1, pass1 = from < segT ? 2 : 1;
   for (uint32_t iter = 0;; iter++) {
     uint32_t t;
     if (counter) {
@@ -1243,16 +1249,21 @@ __global__ __launch_bounds__(256, MS == 1 ? 4 : 3) void raster_batch_kernel(
       else hi = mid;
     }
     lo = __builtin_amdgcn_readfirstlane(lo);
-    uint32_t s, e;
-    if (binTri) {
-      s = tileStart[t];
-      e = tileEnd[t];
-    } else {
-      const uint32_t b = (uint32_t)params[lo].triBase;
-      s = from > b ? from : b;
-      e = b + (uint32_t)params[lo].triCount;
+#pragma nounroll
+    for (int pass = pass0; pass < pass1; pass++) {
+      uint32_t s, e;
+      if (pass == 0) {
+        s = tileStart[t];
+        e = tileEnd[t];
+      } else {
+        const uint32_t b = (uint32_t)params[lo].triBase;
+        s = from > b ? from : b;
+        e = b + (uint32_t)params[lo].triCount;
+      }
+      if (s < e)
+        raster_tile<MS, FCOL, FTEX, XF, LEVEL>(params[lo], (int)(t - tileBase[lo]), s, e, rec, poly,
+                                               pass == 0 ? binTri : nullptr);
     }
-    if (s < e) raster_tile<MS, FCOL, FTEX, XF, LEVEL>(params[lo], (int)(t - tileBase[lo]), s, e, rec, poly, binTri);
   }
 }
 
@@ -1386,7 +1397,7 @@ __global__ void resolve_rgba32f_kernel(const char *src, int srcPitch, long long 
 #define SWR_RASTER_INSTANCE(MS, FCOL, FTEX, XF, LEVEL)                                                           \
   template __global__ void raster_batch_kernel<MS, FCOL, FTEX, XF, LEVEL>(                                      \
       const DrawParams *, const uint32_t *, int, int, uint32_t, uint32_t, uint32_t *, const TriRecord *, const int *, \
-      const uint32_t *, const uint32_t *, const uint32_t *, const uint32_t *, uint32_t);
+      const uint32_t *, const uint32_t *, const uint32_t *, const uint32_t *, uint32_t, int);
 SWR_RASTER_INSTANCE(1, false, false, false, 0)
 SWR_RASTER_INSTANCE(1, false, true, false, 0)
 SWR_RASTER_INSTANCE(1, true, false, false, 0)
