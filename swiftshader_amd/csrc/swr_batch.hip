@@ -307,10 +307,14 @@ __global__ __launch_bounds__(kEmitBlock) void bin_emit_batch_kernel(const DrawPa
   const TriRecord &R = rec[g];
   const DrawParams &P = params[R.draw];
   const uint32_t i = k - off[g];
-  const int tx0 = R.xMin / P.tileW, tx1 = (R.xMax - 1) / P.tileW;
-  const int ty0 = R.yMin / P.tileH;
+  // Tiles are 8 or 16 pixels a side and the box is not negative (the
+  // render area starts at x, y >= 0): shifts divide exactly.
+  const int sw = __builtin_ctz((uint32_t)P.tileW), sh = __builtin_ctz((uint32_t)P.tileH);
+  const int tx0 = R.xMin >> sw, tx1 = (R.xMax - 1) >> sw;
+  const int ty0 = R.yMin >> sh;
   const uint32_t w = (uint32_t)(tx1 - tx0 + 1);
-  const int tx = tx0 + (int)(i % w), ty = ty0 + (int)(i / w);
+  const uint32_t row = i / w;
+  const int tx = tx0 + (int)(i - row * w), ty = ty0 + (int)row;
   const uint32_t bm = tile_block_mask(R, tx, ty, P.tileW, P.tileH);
   keys[k] = bm ? tileBase[R.draw] + (uint32_t)(ty * P.tilesX + tx) : 0xFFFFFFFFu;
   vals[k] = (uint32_t)g | (bm << 28);

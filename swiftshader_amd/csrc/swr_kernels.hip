@@ -505,8 +505,11 @@ __device__ __forceinline__ void setup_one(const DrawParams &P, int t, size_t g, 
     for (int i = 0; i < n; i++) { pp[i] = X[i]; pp[kMaxPolygon + i] = Y[i]; }
   }
   R.nflags = n | (d << 8) | (frontFacing ? 1 << 9 : 0);
-  int tx0 = xMin / P.tileW, tx1 = (xMax - 1) / P.tileW;
-  int ty0 = yMin / P.tileH, ty1 = (yMax - 1) / P.tileH;
+  // Tiles are 8 or 16 pixels a side and xMin, yMin >= 0 (scissor-clamped to
+  // a render area at x, y >= 0): shifts divide exactly.
+  const int sw = __builtin_ctz((uint32_t)P.tileW), sh = __builtin_ctz((uint32_t)P.tileH);
+  int tx0 = xMin >> sw, tx1 = (xMax - 1) >> sw;
+  int ty0 = yMin >> sh, ty1 = (yMax - 1) >> sh;
   tileCount[g] = (uint32_t)((tx1 - tx0 + 1) * (ty1 - ty0 + 1));
 }
 
