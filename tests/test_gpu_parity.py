@@ -98,3 +98,58 @@ def test_draw_beyond_the_known_pair_capacity():
     _compare(big[0] + " (beyond capacity)", ref, gpu_render(big[1]))
     torch.cuda.synchronize()
     _compare(big[0] + " (capacity grown)", ref, gpu_render(big[1]))
+
+
+def _oracle_threads():
+    import os
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def test_draw_above_the_one_launch_scan_limit():
+    """More than 1024 x 1024 primitive slots: the tile-count scan takes its
+    three-launch form (block sums scanned by one workgroup) and the emission
+    index its own kernel, both in the first draw (which reads the pair count
+    back) and in the second (whose pair capacity is known beforehand).  Both
+    match the oracle bit for bit."""
+    import torch
+    from swiftshader_amd import runtime, scenes
+    draw = scenes.plain_gouraud(seed=11, triangles=1_100_000, size=64)
+    runtime.check(runtime.lib().swr_release(), "swr_release")  # the first draw waits for its count
+    print(f"[gpu parity] 1.1M triangles at 64x64, twice", flush=True)
+    ref = oracle_render(draw, threads=_oracle_threads())
+    _compare("1.1M triangles (count read back)", ref, gpu_render(draw))
+    torch.cuda.synchronize()
+    _compare("1.1M triangles (capacity known)", ref, gpu_render(draw))
+
+
+def test_draw_above_the_fused_radix_limit():
+    """More than 4096 sort blocks of (tile, primitive) pairs (8x8 tiles, 250k
+    triangles spanning about a quarter of a 128x128 target: about 20 M pairs):
+    the radix passes scan their histograms in three launches instead of
+    adding the block totals in the scatter.  The first draw sizes its pair
+    buffers from the count it reads back, the second from the capacity learnt
+    from it; both sort every pair and match the oracle bit for bit."""
+    import ctypes
+    import torch
+    from swiftshader_amd import abi, runtime, scenes
+    lib = runtime.lib()
+    base = abi.Tuning()
+    runtime.check(lib.swr_get_tuning(ctypes.byref(base)))
+    draw = scenes.plain_gouraud(seed=12, triangles=250_000, size=128)
+    print(f"[gpu parity] 250k triangles at 128x128, 8x8 tiles", flush=True)
+    ref = oracle_render(draw, threads=_oracle_threads())
+    try:
+        runtime.check(lib.swr_set_tuning(ctypes.byref(abi.Tuning(8, 8, 4))))
+        runtime.check(lib.swr_release(), "swr_release")  # no capacity learnt from smaller draws
+        _compare("250k triangles, 8x8 tiles (count read back)", ref, gpu_render(draw))
+        torch.cuda.synchronize()
+        _compare("250k triangles, 8x8 tiles (capacity known)", ref, gpu_render(draw))
+        torch.cuda.synchronize()
+        runtime.profile_read()  # discard what earlier draws left
+        runtime.profile_enable(True)
+        _compare("250k triangles, 8x8 tiles (profiled)", ref, gpu_render(draw))
+        prof = runtime.profile_read()
+        assert prof["draws"] == 1 and prof["pairs"] > 4096 * 4096, prof
+    finally:
+        runtime.profile_enable(False)
+        runtime.check(lib.swr_set_tuning(ctypes.byref(base)))
