@@ -920,8 +920,9 @@ void enqueue_front(Pipeline &pipe, const BatchLayout &L, const BatchLayout::Seg 
 // Pair emission, sort, ranges and raster of one segment with pair capacity
 // `cap`.  When a run needs more pairs than `cap` (a plan whose geometry grew
 // since it was built), the total needed goes to *overflowFlag and, with
-// `unbinned`, the primitives whose pairs did not fit are rendered by an
-// unbinned raster pass after the binned one, so that none is dropped.
+// `unbinned`, the primitives whose pairs did not fit are rendered by each
+// tile after its binned list (in the same raster launch), so that none is
+// dropped.
 // `indexed`: enqueue_front built the emission index for `cap`; `zeroed`:
 // the tile ranges and the raster's work counters are already zero.
 int enqueue_back(DeviceCtx *c, Pipeline &pipe, const BatchLayout &L, const BatchLayout::Seg &sg, uint32_t cap,
@@ -932,13 +933,13 @@ int enqueue_back(DeviceCtx *c, Pipeline &pipe, const BatchLayout &L, const Batch
   uint32_t *split = pipe.counters.as<uint32_t>() + kSplitSlot;
   uint32_t *emitted = pipe.counters.as<uint32_t>() + kEmittedSlot;
   if (cap > 0) {
-    if (indexed)
-      ;
-    else if (sg.T > 0)
-      hipLaunchKernelGGL(emit_index_kernel, dim3((sg.T + 255) / 256), dim3(256), 0, s,
-                         (const uint32_t *)pipe.off.as<uint32_t>(), sg.T, cap, pipe.first.as<uint32_t>(), split);
-    else
-      hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(1), 0, s, split, 0u, 1u);
+    if (!indexed) {
+      if (sg.T > 0)
+        hipLaunchKernelGGL(emit_index_kernel, dim3((sg.T + 255) / 256), dim3(256), 0, s,
+                           (const uint32_t *)pipe.off.as<uint32_t>(), sg.T, cap, pipe.first.as<uint32_t>(), split);
+      else
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(1), 0, s, split, 0u, 1u);
+    }
     hipLaunchKernelGGL(bin_emit_batch_kernel, dim3((cap + kEmitBlock - 1) / kEmitBlock), dim3(kEmitBlock), 0, s, params,
                        tileBase, (const TriRecord *)pipe.rec.as<TriRecord>(), (const uint32_t *)pipe.off.as<uint32_t>(),
                        sg.T, (const uint32_t *)pipe.first.as<uint32_t>(), cap, pipe.keysA.as<uint32_t>(),
