@@ -64,11 +64,11 @@ __device__ __forceinline__ ClipVtx clip_edge(const ClipVtx &Vi, const ClipVtx &V
 
 // Sutherland-Hodgman against near, far, left, right, top, bottom in that
 // order, stopping below 3 vertices (Clipper.cpp:271-299).  Returns the
-// number of clip stages applied (the reference's Polygon::i).
-__device__ int clip_polygon(ClipVtx (&poly)[kMaxPolygon], int &n, int flagsOr) {
+// number of clip stages applied (the reference's Polygon::i).  tmp: the
+// caller's second vertex buffer.
+__device__ int clip_polygon(ClipVtx (&poly)[kMaxPolygon], int &n, int flagsOr, ClipVtx (&tmp)[kMaxPolygon]) {
   const int bits[6] = {CLIP_NEAR, CLIP_FAR, CLIP_LEFT, CLIP_RIGHT, CLIP_TOP, CLIP_BOTTOM};
   int stages = 0;
-  ClipVtx tmp[kMaxPolygon];
   for (int k = 0; k < 6; k++) {
     if (k > 0 && n < 3) break;
     if (!(flagsOr & bits[k])) continue;
@@ -228,10 +228,23 @@ __device__ __forceinline__ void point_polygon(const DrawParams &P, const float (
 template <typename T>
 __device__ __forceinline__ T pick3(T a, T b, T c, int i) { return i == 0 ? a : (i == 1 ? b : c); }
 
+// Vertex arrays of the polygon path (clipped triangles, lines, points),
+// indexed at run time.
+struct PolyWork {
+  ClipVtx cp[kMaxPolygon], tmp[kMaxPolygon];
+  int X[kMaxPolygon], Y[kMaxPolygon];
+};
+
 // Setup of primitive t of the draw P, written to record slot g (the
-// primitive's index in the batch); `draw` is the draw's index in the batch.
-__device__ __forceinline__ void setup_one(const DrawParams &P, int t, size_t g, int draw, TriRecord *__restrict__ rec,
-                                          int *__restrict__ poly, uint32_t *__restrict__ tileCount) {
+// primitive's index in the batch); `draw` is the draw's index in the batch;
+// w holds the polygon path's arrays.  Without POLY, the primitives that need
+// the polygon path are left alone and false is returned (w is not touched),
+// so that a kernel can give that path workspace in LDS for a few lanes at a
+// time instead of scratch memory for every thread.
+template <bool POLY = true>
+__device__ __forceinline__ bool setup_one(const DrawParams &P, int t, size_t g, int draw, TriRecord *__restrict__ rec,
+                                          int *__restrict__ poly, uint32_t *__restrict__ tileCount, PolyWork &w) {
+  if (!POLY && (P.point || P.line)) return false;
 
   uint32_t vi[3];
   tri_vertices(P, (uint32_t)t, vi);
@@ -282,7 +295,7 @@ __device__ __forceinline__ void setup_one(const DrawParams &P, int t, size_t g, 
   tileCount[g] = 0;
   R.nflags = 0;
   R.draw = draw;
-  ClipVtx cp[kMaxPolygon];
+  ClipVtx (&cp)[kMaxPolygon] = w.cp;
   int n = 3;
   int stages = 0;
   int d = 1;
@@ -292,25 +305,26 @@ __device__ __forceinline__ void setup_one(const DrawParams &P, int t, size_t g, 
     // no culling, front-facing, d = 1 (SetupRoutine.cpp:115-119).
     point_polygon(P, pos[0], P.psize ? P.psize[vi[0]] : 1.0f, cp);
     n = 4;
-    stages = clip_polygon(cp, n, P.depthClip ? CLIP_FRUSTUM : CLIP_SIDES);
-    if (n < 3) return;
+    stages = clip_polygon(cp, n, P.depthClip ? CLIP_FRUSTUM : CLIP_SIDES, w.tmp);
+    if (n < 3) return true;
   } else if (P.line) {
     // setupLine: the line's polygon, clipped against every frustum plane;
     // no culling, front-facing, d = 1 (SetupRoutine.cpp:115-119).
-    if (!line_polygon(P, pos[0], pos[1], cp)) return;
+    if (!line_polygon(P, pos[0], pos[1], cp)) return true;
     n = 4;
-    stages = clip_polygon(cp, n, P.depthClip ? CLIP_FRUSTUM : CLIP_SIDES);
-    if (n < 3) return;
+    stages = clip_polygon(cp, n, P.depthClip ? CLIP_FRUSTUM : CLIP_SIDES, w.tmp);
+    if (n < 3) return true;
   } else {
     // setupSolidTriangles (Renderer.cpp:733-776).
-    if ((flags[0] & flags[1] & flags[2]) != CLIP_FINITE) return;
+    if ((flags[0] & flags[1] & flags[2]) != CLIP_FINITE) return true;
     int flagsOr = flags[0] | flags[1] | flags[2];
     if (flagsOr != CLIP_FINITE) {
+      if (!POLY) return false;
 #pragma unroll
       for (int k = 0; k < 3; k++)
         for (int c = 0; c < 4; c++) cp[k].v[c] = pos[k][c];
-      stages = clip_polygon(cp, n, flagsOr);
-      if (n < 3) return;
+      stages = clip_polygon(cp, n, flagsOr, w.tmp);
+      if (n < 3) return true;
     }
     // Culling and facing (SetupRoutine.cpp:71-120).
     float x0 = (float)PX[0], x1 = (float)PX[1], x2 = (float)PX[2];
@@ -319,8 +333,8 @@ __device__ __forceinline__ void setup_one(const DrawParams &P, int t, size_t g, 
     int w012 = (int)(__float_as_uint(pos[0][3]) ^ __float_as_uint(pos[1][3]) ^ __float_as_uint(pos[2][3]));
     if (w012 < 0) A = -A;
     frontFacing = (P.frontFace == 0) ? (A >= 0.0f) : (A <= 0.0f);
-    if ((P.cullMode & 1) && frontFacing) return;
-    if ((P.cullMode & 2) && !frontFacing) return;
+    if ((P.cullMode & 1) && frontFacing) return true;
+    if ((P.cullMode & 2) && !frontFacing) return true;
     d = (A > 0.0f) ? 1 : 0;
   }
 
@@ -328,8 +342,9 @@ __device__ __forceinline__ void setup_one(const DrawParams &P, int t, size_t g, 
   // SetupRoutine.cpp:122-143).
   // An unclipped triangle keeps its three vertices in registers; only a
   // clipped polygon, a line or a point goes through the n-vertex arrays.
-  const bool isPolygon = stages != 0 || P.line || P.point;
-  int X[kMaxPolygon], Y[kMaxPolygon];
+  const bool isPolygon = POLY && (stages != 0 || P.line || P.point);
+  int (&X)[kMaxPolygon] = w.X;
+  int (&Y)[kMaxPolygon] = w.Y;
   int yMin, yMax, xLo, xHi;
   if (isPolygon) {
     for (int i = 0; i < n; i++) {
@@ -358,12 +373,12 @@ __device__ __forceinline__ void setup_one(const DrawParams &P, int t, size_t g, 
   }
   yMin = max(yMin, P.sy0);
   yMax = min(yMax, P.sy1);
-  if (yMin >= yMax) return;
+  if (yMin >= yMax) return true;
   // Columns that can hold coverage: left >= ceil(min X / 16), right <= ceil(max X / 16).
   const int sxo = (P.ms > 1) ? 6 : 0;
   int xMin = max(wadd(wsub(xLo, sxo), 15) >> 4, P.sx0);
   int xMax = min(wadd(wadd(xHi, sxo), 15) >> 4, P.sx1);
-  if (xMin >= xMax) return;
+  if (xMin >= xMax) return true;
 
   // Sort by minimum y, then by maximum w (SetupRoutine.cpp:270-294),
   // triangles only.
@@ -511,6 +526,7 @@ __device__ __forceinline__ void setup_one(const DrawParams &P, int t, size_t g, 
   int tx0 = xMin >> sw, tx1 = (xMax - 1) >> sw;
   int ty0 = yMin >> sh, ty1 = (yMax - 1) >> sh;
   tileCount[g] = (uint32_t)((tx1 - tx0 + 1) * (ty1 - ty0 + 1));
+  return true;
 }
 
 // One draw: one thread per primitive.
@@ -518,7 +534,8 @@ __global__ __launch_bounds__(256) void setup_kernel(DrawParams P, TriRecord *__r
                                                     int *__restrict__ poly, uint32_t *__restrict__ tileCount) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= P.triCount) return;
-  setup_one(P, t, (size_t)t, 0, rec, poly, tileCount);
+  PolyWork w;
+  setup_one(P, t, (size_t)t, 0, rec, poly, tileCount, w);
 }
 
 __global__ void fill_u32_kernel(uint32_t *p, uint32_t v, uint32_t n) {
