@@ -53,13 +53,8 @@ __device__ __forceinline__ int find_segment(const uint32_t *__restrict__ base, i
   return lo;
 }
 
-// Polygon-path workspaces per wave of setup_batch_kernel (640 B each).
-constexpr int kPolySlots = 16;
-
-// blockBase[d]: first 256-thread block of draw d.  Every thread first runs
-// the setup without the polygon path; the lanes whose primitive needs it
-// (clipped triangles, lines, points) then run it kPolySlots at a time per
-// wave, each with a workspace in LDS, so that no thread uses scratch memory.
+// blockBase[d]: first 256-thread block of draw d (the polygon path as in
+// setup_lanes).
 __global__ __launch_bounds__(256) void setup_batch_kernel(const DrawParams *__restrict__ params,
                                                           const uint32_t *__restrict__ blockBase, int nDraws,
                                                           TriRecord *__restrict__ rec, int *__restrict__ poly,
@@ -68,19 +63,8 @@ __global__ __launch_bounds__(256) void setup_batch_kernel(const DrawParams *__re
   const int d = find_segment(blockBase, nDraws, blockIdx.x);
   const DrawParams &P = params[d];
   const int t = (int)(blockIdx.x - blockBase[d]) * 256 + (int)threadIdx.x;
-  const size_t g = (size_t)P.triBase + (size_t)t;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  bool later = t < P.triCount && !setup_one<false>(P, t, g, d, rec, poly, tileCount, work[wave][0]);
-  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-  for (uint64_t m = __ballot(later); m; m = __ballot(later)) {  // wave-uniform; each round takes up to kPolySlots lanes
-    const uint32_t rank = (uint32_t)__builtin_popcountll(m & below);
-    const bool mine = later && rank < (uint32_t)kPolySlots;
-    if (mine) setup_one<true>(P, t, g, d, rec, poly, tileCount, work[wave][rank]);
-    later = later && !mine;
-    // The next round's lanes reuse these workspaces.
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-  }
+  setup_lanes(P, t < P.triCount ? t : -1, (size_t)P.triBase + (size_t)t, d, rec, poly, tileCount,
+              work[threadIdx.x >> 6]);
 }
 
 // ---- exclusive scan (u32) --------------------------------------------------

@@ -529,13 +529,38 @@ __device__ __forceinline__ bool setup_one(const DrawParams &P, int t, size_t g, 
   return true;
 }
 
+// Polygon-path workspaces per wave of the setup kernels (640 B each).
+constexpr int kPolySlots = 16;
+
+// Setup of primitive t (slot g, draw `draw`) by the calling lane, t < 0 for
+// none: every lane first runs the setup without the polygon path; the lanes
+// whose primitive needs it (clipped triangles, lines, points) then run it
+// kPolySlots at a time per wave, each with a workspace in `work` (the
+// wave's kPolySlots entries in LDS), so that no thread uses scratch memory.
+// Called by every lane of the wave.
+__device__ __forceinline__ void setup_lanes(const DrawParams &P, int t, size_t g, int draw, TriRecord *__restrict__ rec,
+                                            int *__restrict__ poly, uint32_t *__restrict__ tileCount,
+                                            PolyWork *work) {
+  const int lane = threadIdx.x & 63;
+  bool later = t >= 0 && !setup_one<false>(P, t, g, draw, rec, poly, tileCount, work[0]);
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  for (uint64_t m = __ballot(later); m; m = __ballot(later)) {  // wave-uniform
+    const uint32_t rank = (uint32_t)__builtin_popcountll(m & below);
+    const bool mine = later && rank < (uint32_t)kPolySlots;
+    if (mine) setup_one<true>(P, t, g, draw, rec, poly, tileCount, work[rank]);
+    later = later && !mine;
+    // The next round's lanes reuse these workspaces.
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  }
+}
+
 // One draw: one thread per primitive.
 __global__ __launch_bounds__(256) void setup_kernel(DrawParams P, TriRecord *__restrict__ rec,
                                                     int *__restrict__ poly, uint32_t *__restrict__ tileCount) {
+  __shared__ PolyWork work[4][kPolySlots];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= P.triCount) return;
-  PolyWork w;
-  setup_one(P, t, (size_t)t, 0, rec, poly, tileCount, w);
+  setup_lanes(P, t < P.triCount ? t : -1, (size_t)t, 0, rec, poly, tileCount, work[threadIdx.x >> 6]);
 }
 
 __global__ void fill_u32_kernel(uint32_t *p, uint32_t v, uint32_t n) {
